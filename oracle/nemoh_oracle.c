@@ -1,0 +1,1872 @@
+/*
+ * CPU restatement of the NEMOH Solver hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This file is the oracle that tests/ and bench.py's CPU-baseline leg use to
+ * check and time the HIP product.  Nothing in nemoh_amd/ may link or call it.
+ * It restates the reference algorithm step by step (same branch order, same
+ * thresholds, same literal constants) so that FP64 results agree with the
+ * FP64 build of the reference to rounding; every function cites the
+ * reference file:line it follows.  Indices written as 1-based macros below
+ * mirror the reference's Fortran indexing to keep the restatement auditable.
+ *
+ * Undefined reference behaviour (AKZ > -1.5e-6 in the submerged branch
+ * leaves the wave terms unset, COMPUTE_GREEN_INFD.f90:284-289,
+ * COMPUTE_GREEN_FD.f90:307-312) is reported through the return value and
+ * the wave terms are set to zero.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).
+ */
+#include "nemoh_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NPINTE 251
+
+/* 1: complex / real in EXPORS is done as the compiler runtime divides two
+ * complex numbers (logb/scalbn-scaled), which is what the FP64 reference
+ * build executes for CLOG(COM)/H (COMPUTE_GREEN_FD.f90:1109); 0: plain
+ * (re/h, im/h).  The Prony amplitudes are ill-conditioned at small kh,
+ * so the two differ visibly there. */
+int ora_div_mode = 1;
+
+static inline double sq(double v) { return v * v; }
+static inline double cube(double v) { return v * v * v; }
+static inline double fsign(double a, double b) { return copysign(fabs(a), b); }
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+static inline int imin(int a, int b) { return a < b ? a : b; }
+
+/* ------------------------------------------------------------------------ */
+/* GG: complex exponential-integral approximation, ELEMENTARY_FNS.f90:31-81 */
+double complex ora_gg(double complex z, double complex cex) {
+  double complex gg, y;
+  double t;
+  if (creal(z) + 16.0 <= 0.0) { /* :35-38 */
+    y = 1.0 / z;
+    return y * (1.0 + y * (-1.0 + y * (2.0 + y * (-6.0 + y * (24.0 + y * (-120.0))))));
+  }
+  t = cimag(z);
+  if (fabs(t) - 10.0 > 0.0) { /* :40-42 */
+    gg = 0.711093 / (z + 0.415775) + 0.278518 / (z + 2.29428) + 0.010389 / (z + 6.2900);
+  } else if (creal(z) + 0.5 > 0.0) { /* :44-50 */
+    gg = -(clog(z) * (.1E+01 + z * (0.23721365E+00 + z * (0.206543E-01 + z * (0.763297E-03 +
+                                                                              z * 0.97087007E-05)))) +
+           0.5772156649E+00 * (0.99999207E+00 +
+                               z * (-0.149545886E+01 +
+                                    z * (0.41806426E-01 +
+                                         z * (-0.3000591E-01 +
+                                              z * (0.19387339E-02 + z * (-0.51801555E-03))))))) /
+         (0.1E+01 +
+          z * (-0.76273617E+00 +
+               z * (0.28388363E+00 +
+                    z * (-0.66786033E-01 +
+                         z * (0.12982719E-01 + z * (-0.8700861E-03 + z * 0.2989204E-03))))));
+  } else if (t < 0.0) { /* :60-69 */
+    gg = ((((((((1.000000 + 1.3935496E-06 * I) * z + (15.82958 - 20.14222 * I)) * z +
+                (-70.52863 - 227.9511 * I)) * z + (-985.4221 - 226.6272 * I)) * z +
+              (-1202.318 + 1580.907 * I)) * z + (953.2441 + 1342.447 * I)) * z +
+            (417.3716 - 196.6665 * I)) * z + (-9.881266 - 24.24952 * I)) /
+         (((((((((1.000000 + 0.0 * I) * z + (16.83184 - 20.14481 * I)) * z +
+                (-55.66969 - 248.1167 * I)) * z + (-1068.640 - 434.4707 * I)) * z +
+              (-2082.250 + 1522.471 * I)) * z + (383.3455 + 2730.378 * I)) * z +
+            (1216.791 + 351.7189 * I)) * z + (115.3926 - 161.2647 * I)) * z +
+          (-3.777369 - 4.510900 * I)) -
+         (3.14159265 * I) * cex;
+    return gg;
+  } else { /* :71-79 */
+    gg = ((((((((1.000000 - 1.3935496E-06 * I) * z + (15.82958 + 20.14222 * I)) * z +
+                (-70.52863 + 227.9511 * I)) * z + (-985.4221 + 226.6272 * I)) * z +
+              (-1202.318 - 1580.907 * I)) * z + (953.2441 - 1342.447 * I)) * z +
+            (417.3716 + 196.6665 * I)) * z + (-9.881266 + 24.24952 * I)) /
+         (((((((((1.000000 + 0.0 * I) * z + (16.83184 + 20.14481 * I)) * z +
+                (-55.66969 + 248.1167 * I)) * z + (-1068.640 + 434.4707 * I)) * z +
+              (-2082.250 - 1522.471 * I)) * z + (383.3455 - 2730.378 * I)) * z +
+            (1216.791 - 351.7189 * I)) * z + (115.3926 + 161.2647 * I)) * z +
+          (-3.777369 + 4.510900 * I)) +
+         (3.14159265 * I) * cex;
+    return gg;
+  }
+  /* :51-55: +-i*pi*exp(z) branch term */
+  if (t < 0.0) return gg - (3.14159265 * I) * cex;
+  return gg + (3.14159265 * I) * cex;
+}
+
+/* ------------------------------------------------------------------------ */
+/* COFINT (NEMOH.f90:254-274): 251-point Simpson rule on [-pi/2, pi/2]      */
+static void ora_cofint(double *cqt, double *qqt) {
+  const double pi = 4.0 * atan(1.0);
+  for (int j = 1; j <= NPINTE; ++j) {
+    qqt[j - 1] = -pi / 2.0 + (j - 1.0) / (NPINTE - 1.0) * pi;
+    if (j == 1 || j == NPINTE) cqt[j - 1] = pi / (3.0 * (NPINTE - 1.0));
+    else if (j % 2 != 0) cqt[j - 1] = 2.0 / (3.0 * (NPINTE - 1.0)) * pi;
+    else cqt[j - 1] = 4.0 / (3.0 * (NPINTE - 1.0)) * pi;
+  }
+}
+
+/* CREK + VNS (NEMOH.f90:179-251): tabulated wave-Green-function integrals.
+ * Tables are (i, j) -> i + 328 * j, 0-based, i over XR and j over XZ. */
+void ora_crek(double *xr, double *xz, double *apd1x, double *apd1z, double *apd2x,
+              double *apd2z) {
+  const int jz = ORA_NZ, ir = ORA_NR;
+  double qqt[NPINTE], cqt[NPINTE];
+  for (int j = 1; j <= jz; ++j) /* :186-189 */
+    xz[j - 1] = -dmin(dmin(pow(10.0, j / 5.0 - 6.0), pow(10.0, j / 8.0 - 4.5)), 16.0);
+  xr[0] = 0.0; /* :190-198 */
+  for (int i = 2; i <= ir; ++i) {
+    double akr;
+    if (i < 40) akr = dmin(pow(10.0, (i - 1.0) / 5 - 6), 4.0 / 3.0 + fabs((i - 32.0) / 3.0));
+    else akr = 4.0 / 3.0 + fabs((i - 32.0) / 3.0);
+    xr[i - 1] = akr;
+  }
+  ora_cofint(cqt, qqt);
+  for (int j = 1; j <= jz; ++j) {
+    for (int i = 1; i <= ir; ++i) { /* VNS :211-251 */
+      const double akz = xz[j - 1], akr = xr[i - 1];
+      double fd1jx = 0.0, fd1jz = 0.0, fd2jx = 0.0, fd2jz = 0.0;
+      for (int it = 0; it < NPINTE; ++it) {
+        const double teta = qqt[it], cqit = cqt[it];
+        const double ct = cos(teta);
+        const double complex zik = akz + I * (akr * ct);
+        const double complex cex = (creal(zik) + 30.0 <= 0.0) ? 0.0 : cexp(zik);
+        const double complex gz = ora_gg(zik, cex);
+        const double complex c1 = cqit * (gz - 1.0 / zik);
+        const double complex c2 = cqit * cex;
+        fd1jx = fd1jx + ct * cimag(c1);
+        fd1jz = fd1jz + creal(c1);
+        fd2jx = fd2jx + ct * cimag(c2);
+        fd2jz = fd2jz + creal(c2);
+      }
+      const int k = (i - 1) + ORA_NR * (j - 1);
+      apd1x[k] = fd1jx;
+      apd1z[k] = fd1jz;
+      apd2x[k] = fd2jx;
+      apd2z[k] = fd2jz;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CINT_INFD / CINT_FD (COMPUTE_GREEN_INFD.f90:505-638, COMPUTE_GREEN_FD.f90:
+ * 871-1004): theta rule selected by AKK.  Gauss half-rules are the
+ * reference's 7-digit DATA literals. */
+static const double Q8[4] = {.4801449, .3983332, .2627662, .09171732};
+static const double CQ8[4] = {.05061427, .1111905, .1568533, .1813419};
+static const double Q12[6] = {.4907803, .4520586, .3849513, .2936589, .1839157, .06261670};
+static const double CQ12[6] = {.2358766E-1, .5346966E-1, .8003916E-1, .1015837, .1167462, .1245735};
+static const double Q16[8] = {.4947004, .4722875, .4328156, .3777022,
+                              .3089381, .2290084, .1408017, .04750625};
+static const double CQ16[8] = {.01357622, .03112676, .04757925, .06231448,
+                               .07479799, .08457826, .09130170, .09472530};
+static const double Q24[12] = {.4975936, .4873642, .469137,  .4432077, .4100009, .3700621,
+                               .3240468, .2727107, .2168967, .1575213, .09555943, .032028446};
+static const double CQ24[12] = {.6170615E-2, .1426569E-1, .2213872E-1, .2964929E-1,
+                                .366732E-1,  .4309508E-1, .4880932E-1, .5372213E-1,
+                                .5775283E-1, .6083523E-1, .6291873E-1, .6396909E-1};
+static const double Q32[16] = {.4986319, .4928057, .4823811, .4674530, .4481605, .4246838,
+                               .3972418, .3660910, .3315221, .2938578, .2534499, .2106756,
+                               .1659343, .1196436, .07223598, .02415383};
+static const double CQ32[16] = {.350930E-2,  .8137197E-2, .1269603E-1, .1713693E-1,
+                                .2141794E-1, .2549903E-1, .2934204E-1, .3291111E-1,
+                                .3617289E-1, .3909694E-1, .4165596E-1, .4382604E-1,
+                                .4558693E-1, .4692219E-1, .4781936E-1, .4827004E-1};
+
+void ora_cint(double akk, int *n_out, double *qq, double *cq) {
+  const double pi = 4.0 * atan(1.0);
+  const double *q = NULL, *c = NULL;
+  int n;
+  if (akk - .4 <= 0) { n = 8; q = Q8; c = CQ8; }
+  else if (akk - 2.5 <= 0) { n = 12; q = Q12; c = CQ12; }
+  else if (akk - 4. <= 0) { n = 16; q = Q16; c = CQ16; }
+  else if (akk - 8. <= 0) { n = 24; q = Q24; c = CQ24; }
+  else if (akk - 25. <= 0) { n = 32; q = Q32; c = CQ32; }
+  else {
+    n = (akk > 40.) ? 101 : 51;
+    for (int j = 1; j <= n; ++j) {
+      qq[j - 1] = -pi / 2. + (j - 1.) / (n - 1.) * pi;
+      if (j == 1 || j == n) cq[j - 1] = pi / (3. * (n - 1.));
+      else if (j % 2 != 0) cq[j - 1] = 2. / (3. * (n - 1.)) * pi;
+      else cq[j - 1] = 4. / (3. * (n - 1.)) * pi;
+    }
+    *n_out = n;
+    return;
+  }
+  const int h = n / 2;
+  for (int i = 1; i <= h; ++i) {
+    qq[i - 1] = q[i - 1] * pi;
+    qq[n - i] = -q[i - 1] * pi;
+    cq[i - 1] = c[i - 1] * pi;
+    cq[n - i] = c[i - 1] * pi;
+  }
+  *n_out = n;
+}
+
+/* X0 (COMPUTE_GREEN_FD.f90:1635-1680): bisection root of x*tanh(x) = AK */
+double ora_x0(double ak) {
+  const double eps = 5.E-6;
+  int itour = 0, iiter = 0;
+  double xi = 0., xs = xi, xm, val1, val2;
+  const double pas = dmax(ak, sqrt(ak));
+  for (;;) {
+    xs = xs + pas;
+    itour = itour + 1;
+    if (itour <= 1000) {
+      val1 = ak - xs * tanh(xs);
+      val2 = ak - xi * tanh(xi);
+      if (val1 * val2 > 0) continue;
+    }
+    break;
+  }
+  for (;;) {
+    xm = (xi + xs) * 0.5;
+    iiter = iiter + 1;
+    if (iiter > 1000 || itour > 1000) return NAN;
+    if (fabs((xs - xi) / xm) > eps) {
+      val1 = ak - xm * tanh(xm);
+      val2 = ak - xi * tanh(xi);
+      if (fabs(val1 * val2) <= 1.E-12) val1 = 0.;
+      if (val1 * val2 < 0) xs = xm;
+      else xi = xm;
+    } else {
+      return xm;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Prony fit of the finite-depth kernel: LISC and its helpers
+ * (COMPUTE_GREEN_FD.f90:1007-1632).  Arrays use the reference's fixed
+ * dimensions S(120,32), XT/YT(121), SC/VR/VC/POL(31). */
+#define LDS 120
+#define S_(i, j) s[((i) - 1) + LDS * ((j) - 1)]
+
+/* FF (:1612-1632) */
+static double ora_ff(double xtt, double ak, double am) {
+  const double coef = sq(am + ak) / (sq(am) - sq(ak) + ak);
+  const double h = xtt - am;
+  const double tol = dmax(0.1, 0.1 * am);
+  if (fabs(h) > tol)
+    return (xtt + ak) * exp(xtt) / (xtt * sinh(xtt) - ak * cosh(xtt)) - coef / (xtt - am) - 2.;
+  const double a = am - tol, b = am, c = am + tol;
+  const double d = (a + ak) * exp(a) / (a * sinh(a) - ak * cosh(a)) - coef / (a - am) - 2;
+  const double e = coef / (am + ak) * (am + ak + 1) - sq(coef / (am + ak)) * am - 2;
+  const double f = (c + ak) * exp(c) / (c * sinh(c) - ak * cosh(c)) - coef / (c - am) - 2;
+  return (xtt - b) * (xtt - c) * d / ((a - b) * (a - c)) +
+         (xtt - c) * (xtt - a) * e / ((b - c) * (b - a)) +
+         (xtt - a) * (xtt - b) * f / ((c - a) * (c - b));
+}
+
+/* HOUSRS (:1454-1511): Householder least squares on A(NL, NCC+NS), ld NMAX */
+static int ora_housrs(double *s, int nl, int ncc, int ns, double eps) {
+  const int ntc = ncc + ns;
+  if (ncc > nl) return 1;
+  for (int k = 1; k <= ncc; ++k) {
+    double e = 0;
+    for (int i = k; i <= nl; ++i) e = e + sq(S_(i, k));
+    const double e0 = sqrt(e);
+    if (e0 < eps) return 2;
+    double ar;
+    if (S_(k, k) == 0) ar = -e0;
+    else ar = -fsign(e0, S_(k, k));
+    const double eta = ar * (ar - S_(k, k));
+    const int kp1 = k + 1;
+    for (int j = kp1; j <= ntc; ++j) {
+      double ba = (S_(k, k) - ar) * S_(k, j);
+      for (int i = kp1; i <= nl; ++i) ba = ba + S_(i, k) * S_(i, j);
+      S_(k, j) = S_(k, j) + ba / ar;
+      for (int i = kp1; i <= nl; ++i) S_(i, j) = S_(i, j) - S_(i, k) * ba / eta;
+    }
+    S_(k, k) = ar;
+    for (int i = kp1; i <= nl; ++i) S_(i, k) = 0;
+  }
+  for (int j = 1; j <= ns; ++j) {
+    const int ncj = ncc + j;
+    S_(ncc, ncj) = S_(ncc, ncj) / S_(ncc, ncc);
+    for (int l = 2; l <= ncc; ++l) {
+      const int i1 = ncc + 1 - l;
+      const int m = i1 + 1;
+      for (int i = m; i <= ncc; ++i) S_(i1, ncj) = S_(i1, ncj) - S_(i1, i) * S_(i, ncj);
+      S_(i1, ncj) = S_(i1, ncj) / S_(i1, i1);
+    }
+  }
+  return 0;
+}
+
+/* SPQFB (:1320-1451): Bairstow iteration for one quadratic factor.
+ * c[] and q[] are 1-based views (c[1..ic], q[1..4]). */
+static void ora_spqfb(double *c, int ic, double *q, int lim, int *ier) {
+  double h = 0., hh = 0., a, a1 = 0, aa, b, bb, b1 = 0, c1 = 0, ca, cb, cc, cd, dq1, dq2,
+         eps, eps1, q1, q2, qq1, qq2, qqq1, qqq2;
+  int i, j, l, ll;
+  *ier = 0;
+  j = ic + 1;
+L1:
+  j = j - 1;
+  if (j - 1 <= 0) goto L40;
+  if (c[j] == 0) goto L1;
+  a = c[j];
+  if (a - 1. != 0) {
+    for (i = 1; i <= j; ++i) {
+      c[i] = c[i] / a;
+      if (fabs(c[i]) - 1.E27 >= 0) goto L40;
+    }
+  }
+  if (j - 3 < 0) goto L41;
+  if (j - 3 == 0) goto L38;
+L7:
+  eps = 1.E-14;
+  eps1 = 1.E-6;
+  l = 0;
+  ll = 0;
+  q1 = q[1];
+  q2 = q[2];
+  qq1 = 0.;
+  qq2 = 0.;
+  aa = c[1];
+  bb = c[2];
+  cb = fabs(aa);
+  ca = fabs(bb);
+  if (cb - ca < 0) {
+    cc = cb + cb;
+    cb = cb / ca;
+    ca = 1.;
+  } else if (cb - ca == 0) {
+    cc = ca + ca;
+    ca = 1.;
+    cb = 1.;
+  } else {
+    cc = ca + ca;
+    ca = ca / cb;
+    cb = 1.;
+  }
+  cd = cc * .1;
+L12:
+  a = 0.;
+  b = a;
+  a1 = a;
+  b1 = a;
+  i = j;
+  qqq1 = q1;
+  qqq2 = q2;
+  dq1 = hh;
+  dq2 = h;
+L13:
+  h = -q1 * b - q2 * a + c[i];
+  if (fabs(h) - 1.E27 >= 0) goto L42;
+  b = a;
+  a = h;
+  i = i - 1;
+  if (i - 1 < 0) goto L18;
+  if (i - 1 == 0) h = 0.;
+  h = -q1 * b1 - q2 * a1 + h;
+  if (fabs(h) - 1.E27 >= 0) goto L42;
+  c1 = b1;
+  b1 = a1;
+  a1 = h;
+  goto L13;
+L18:
+  h = ca * fabs(a) + cb * fabs(b);
+  if (ll > 0) goto L39;
+  l = l + 1;
+  if (fabs(a) - eps * fabs(c[1]) <= 0) {
+    if (fabs(b) - eps * fabs(c[2]) <= 0) goto L39;
+  }
+  if (h - cc <= 0) {
+    aa = a;
+    bb = b;
+    cc = h;
+    qq1 = q1;
+    qq2 = q2;
+  }
+  if (l - lim <= 0) goto L28;
+  if (h - cd <= 0) goto L43;
+  if (q[1] == 0 && q[2] == 0) goto L42;
+  q[1] = 0.;
+  q[2] = 0.;
+  goto L7;
+L28:
+  hh = dmax(dmax(fabs(a1), fabs(b1)), fabs(c1));
+  if (hh <= 0) goto L42;
+  a1 = a1 / hh;
+  b1 = b1 / hh;
+  c1 = c1 / hh;
+  h = a1 * c1 - b1 * b1;
+  if (h == 0) goto L42;
+  a = a / hh;
+  b = b / hh;
+  hh = (b * a1 - a * b1) / h;
+  h = (a * c1 - b * b1) / h;
+  q1 = q1 + hh;
+  q2 = q2 + h;
+  if (fabs(hh) - eps * fabs(q1) <= 0) {
+    if (fabs(h) - eps * fabs(q2) <= 0) {
+      ll = 1;
+      goto L12;
+    }
+  }
+  if (l - 1 <= 0) goto L12;
+  if (fabs(hh) - eps1 * fabs(q1) > 0) goto L12;
+  if (fabs(h) - eps1 * fabs(q2) > 0) goto L12;
+  if (fabs(qqq1 * hh) - fabs(q1 * dq1) >= 0) goto L44;
+  if (fabs(qqq2 * h) - fabs(q2 * dq2) < 0) goto L12;
+  goto L44;
+L38:
+  q[1] = c[1];
+  q[2] = c[2];
+  q[3] = 0.;
+  q[4] = 0.;
+  return;
+L39:
+  q[1] = q1;
+  q[2] = q2;
+  q[3] = a;
+  q[4] = b;
+  return;
+L40:
+  *ier = -1;
+  return;
+L41:
+  *ier = -2;
+  return;
+L42:
+  *ier = -3;
+  goto L44;
+L43:
+  *ier = 1;
+L44:
+  q[1] = qq1;
+  q[2] = qq2;
+  q[3] = aa;
+  q[4] = bb;
+}
+
+/* SPRBM (:1182-1317): real and complex roots of the polynomial c[1..ic]
+ * (1-based views).  Returns IER (2 is a fatal error in the reference). */
+static int ora_sprbm(double *c, int ic, double *rr, double *rc, double *pol, int *ir_out) {
+  const double eps = 1.E-6;
+  const int lim = 100;
+  int i, j, l, n, ir, ist = 0, ier = 0;
+  double a, b, h, q1, q2, q[5];
+  ir = ic + 1;
+L1:
+  ir = ir - 1;
+  if (ir - 1 <= 0) goto L42;
+  if (c[ir] == 0) goto L1;
+  ier = 0;
+  j = ir;
+  l = 0;
+  a = c[ir];
+  for (i = 1; i <= ir; ++i) {
+    if (l <= 0) {
+      if (c[i] == 0) {
+        rr[i] = 0.;
+        rc[i] = 0.;
+        pol[j] = 0.;
+        j = j - 1;
+        continue;
+      }
+      l = 1;
+      ist = i;
+      j = 0;
+    }
+    j = j + 1;
+    c[i] = c[i] / a;
+    pol[j] = c[i];
+    if (fabs(pol[j]) - 1.E27 >= 0) goto L42;
+  }
+  q1 = 0.;
+  q2 = 0.;
+L9:
+  if (j - 2 < 0) goto L33;
+  if (j - 2 == 0) {
+    a = pol[1];
+    rr[ist] = -a;
+    rc[ist] = 0.;
+    ir = ir - 1;
+    q2 = 0.;
+    if (ir - 1 > 0) {
+      for (i = 2; i <= ir; ++i) {
+        q1 = q2;
+        q2 = pol[i + 1];
+        pol[i] = a * q2 + q1;
+      }
+    }
+    pol[ir + 1] = a + q2;
+    goto L34;
+  }
+  for (l = 1; l <= 10; ++l) {
+    n = 1;
+  L15:
+    q[1] = q1;
+    q[2] = q2;
+    ora_spqfb(pol, j, q, lim, &i);
+    if (i == 0) goto L24;
+    if (i > 0) goto L23;
+    if (q1 != 0 || q2 != 0) {
+      switch (n) {
+        case 1:
+        case 3:
+          q1 = -q1;
+          n = n + 1;
+          goto L15;
+        case 2:
+          q2 = -q2;
+          n = n + 1;
+          goto L15;
+        default:
+          break;
+      }
+    }
+    q1 = 1. + q1;
+    q2 = 1. - q2;
+  }
+  ier = 3;
+  ir = ir - j;
+  goto L45;
+L23:
+  ier = 1;
+L24:
+  q1 = q[1];
+  q2 = q[2];
+  b = 0.;
+  a = 0.;
+  i = j;
+  do {
+    h = -q1 * b - q2 * a + pol[i];
+    pol[i] = b;
+    b = a;
+    a = h;
+    i = i - 1;
+  } while (i - 2 > 0);
+  pol[2] = b;
+  pol[1] = a;
+  l = ir - 1;
+  if (j - l <= 0) {
+    for (i = j; i <= l; ++i) pol[i - 1] = pol[i - 1] + pol[i] * q2 + pol[i + 1] * q1;
+  }
+  pol[l] = pol[l] + pol[l + 1] * q2 + q1;
+  pol[ir] = pol[ir] + q2;
+  h = -.5 * q2;
+  a = h * h - q1;
+  b = sqrt(fabs(a));
+  if (a <= 0) {
+    rr[ist] = h;
+    rc[ist] = b;
+    ist = ist + 1;
+    rr[ist] = h;
+    rc[ist] = -b;
+  } else {
+    b = h + fsign(b, h);
+    rr[ist] = q1 / b;
+    rc[ist] = 0.;
+    ist = ist + 1;
+    rr[ist] = b;
+    rc[ist] = 0.;
+  }
+  ist = ist + 1;
+  j = j - 2;
+  goto L9;
+L33:
+  ir = ir - 1;
+L34:
+  a = 0.;
+  for (i = 1; i <= ir; ++i) {
+    q1 = c[i];
+    q2 = pol[i + 1];
+    pol[i] = q2;
+    if (q1 != 0) q2 = (q1 - q2) / q1;
+    q2 = fabs(q2);
+    if (q2 - a > 0) a = q2;
+  }
+  i = ir + 1;
+  pol[i] = 1.;
+  rr[i] = a;
+  rc[i] = 0.;
+  if (ier <= 0 && a - eps > 0) ier = -1;
+  goto L45;
+L42:
+  ier = 2;
+  ir = 0;
+L45:
+  *ir_out = ir;
+  return ier;
+}
+
+/* MCAS (:1146-1179): amplitudes of the fitted exponentials by least squares */
+static int ora_mcas(int nexp, const double *texp, const double *xt, const double *yt, int npp,
+                    double *ar, double *s) {
+  const double eps = 1.E-20;
+  for (int i = 1; i <= nexp; ++i)
+    for (int j = 1; j <= nexp; ++j) {
+      double acc = 0;
+      for (int l = 1; l <= npp; ++l) {
+        const double tt = (texp[i] + texp[j]) * xt[l];
+        if (tt + 30 >= 0) acc = acc + exp(tt);
+      }
+      S_(i, j) = acc;
+    }
+  for (int i = 1; i <= nexp; ++i) {
+    double acc = 0;
+    for (int l = 1; l <= npp; ++l) {
+      const double ttt = texp[i] * xt[l];
+      if (ttt + 30 >= 0) acc = acc + exp(ttt) * yt[l];
+    }
+    S_(i, nexp + 1) = acc;
+  }
+  if (ora_housrs(s, nexp, nexp, 1, eps)) return 1;
+  for (int i = 1; i <= nexp; ++i) ar[i] = S_(i, nexp + 1);
+  return 0;
+}
+
+/* EXPORS (:1080-1143): Prony step.  All arrays are 1-based views. */
+static int ora_expors(const double *xt, const double *yt, int nj, int *nm, double *vcom,
+                      double *s, double *sc, double *vr, double *vc, double *pol,
+                      double *ar) {
+  const int npp = nj + 1;
+  const double h = (xt[npp] - xt[1]) / nj;
+  const int k = npp - *nm;
+  const int m = *nm;
+  for (int i = 1; i <= k; ++i) {
+    for (int j = 1; j <= m; ++j) S_(i, j) = yt[m - j + i];
+    S_(i, m + 1) = -yt[m + i];
+  }
+  if (ora_housrs(s, k, m, 1, 1.E-20)) return 1;
+  for (int i = 1; i <= m; ++i) sc[m - i + 1] = S_(i, m + 1);
+  sc[m + 1] = 1.;
+  int is;
+  if (ora_sprbm(sc, m + 1, vr, vc, pol, &is) == 2) return 2;
+  for (int i = 1; i <= m; ++i) {
+    double complex com = vr[i] + I * vc[i];
+    com = clog(com);
+    if (ora_div_mode == 0) {
+      com = com / h;
+    } else {
+      /* complex / complex(h, 0) by the scaled algorithm of the compiler
+       * runtime's complex division */
+      const double a = creal(com), b = cimag(com);
+      double c = h, d = 0.;
+      int ilogbw = 0;
+      const double logbw = logb(fmax(fabs(c), fabs(d)));
+      if (isfinite(logbw)) {
+        ilogbw = (int)logbw;
+        c = scalbn(c, -ilogbw);
+        d = scalbn(d, -ilogbw);
+      }
+      const double denom = c * c + d * d;
+      com = scalbn((a * c + b * d) / denom, -ilogbw) + I * scalbn((b * c - a * d) / denom, -ilogbw);
+    }
+    vr[i] = creal(com);
+    vc[i] = cimag(com);
+  }
+  int i = 1, j = 0;
+  do {
+    if (vc[i] == 0) {
+      j = j + 1;
+      vcom[j] = vr[i];
+      i = i + 1;
+    } else if (fabs(vr[i] - vr[i + 1]) - 1.E-5 <= 0) {
+      j = j + 1;
+      vcom[j] = vr[i];
+      i = i + 2;
+    } else {
+      j = j + 1;
+      vcom[j] = vr[i];
+      i = i + 1;
+    }
+  } while (i - m <= 0);
+  int nexp = j;
+  j = 0;
+  for (i = 1; i <= nexp; ++i) {
+    j = j + 1;
+    if (vcom[i] >= 0. || vcom[i] + 20. <= 0) j = j - 1;
+    else vcom[j] = vcom[i];
+  }
+  nexp = j;
+  *nm = nexp;
+  return ora_mcas(nexp, vcom, xt, yt, npp, ar, s) ? 3 : 0;
+}
+
+/* LISC (:1007-1077).  ambda/ar are 0-based outputs of length 31. */
+int ora_lisc(double ak0, double am0, int *nexp_out, double *ambda, double *ar) {
+  double *s = calloc((size_t)LDS * 32, sizeof(double));
+  double xt[122], yt[122], sc[33], vr[33], vc[33], pol[33], amb1[33], ar1[33];
+  memset(sc, 0, sizeof sc);
+  memset(vr, 0, sizeof vr);
+  memset(vc, 0, sizeof vc);
+  memset(pol, 0, sizeof pol);
+  memset(ar1, 0, sizeof ar1);
+  for (int k = 0; k < 31; ++k) amb1[k + 1] = ambda[k];
+  const double nexr = 31;
+  const double preci = 1.E-02;
+  const double a = -0.1, b = 20.;
+  int nk = 4, nm = 0, err = 0;
+  for (;;) {
+    nm = nk;
+    const int nj = 4 * nm;
+    const int npp = nj + 1;
+    const double h = (b - a) / nj;
+    for (int i = 1; i <= npp; ++i) {
+      xt[i] = a + (i - 1) * h;
+      yt[i] = ora_ff(xt[i], ak0, am0);
+    }
+    int isor = 0;
+    err = ora_expors(xt, yt, nj, &nm, amb1, s, sc, vr, vc, pol, ar1);
+    if (err) break;
+    const int npi = 2;
+    const int nmo = npi * npp - npi + 1;
+    for (int i = 1; i <= nmo; ++i) {
+      const double xx = (i - 1) * b / (nmo - 1);
+      const double yy = ora_ff(xx, ak0, am0);
+      double tt = 0.;
+      for (int j = 1; j <= nm; ++j) {
+        const double rt = amb1[j] * xx;
+        if (rt > -20.) tt = tt + ar1[j] * exp(rt);
+      }
+      const double dif = yy - tt;
+      if (fabs(dif) > preci) isor = 1;
+    }
+    if (isor == 0) break;
+    nk = nk + 2;
+    if (nk - (nexr - 1) > 0) break;
+  }
+  free(s);
+  if (!err)
+    for (int j = 1; j <= nm; ++j)
+      if (amb1[j] > 0.) err = 4;
+  for (int k = 0; k < 31; ++k) {
+    ambda[k] = amb1[k + 1];
+    ar[k] = ar1[k + 1];
+  }
+  *nexp_out = nm;
+  return err;
+}
+#undef S_
+
+/* ------------------------------------------------------------------------ */
+/* PRE_PROC_MESH + GOMG (PREPARE_MESH.f90:30-180).  Connectivity 0-based.
+ * The unused DIST array (:161-168) is not computed. */
+int ora_prepare_mesh(int np, int nfa, int nsym, const double *x, const double *y,
+                     const double *z, const int *m1, const int *m2, const int *m3,
+                     const int *m4, double *xg, double *yg, double *zg, double *xn,
+                     double *yn, double *zn, double *aire, double *tdis, double *zer) {
+  double gl = 0., bl = 0.;
+  for (int i = 0; i < np; ++i)
+    for (int j = i + 1; j < np; ++j) {
+      gl = dmax(fabs(x[j] - x[i]), gl);
+      if (nsym == 0) bl = dmax(fabs(y[j] - y[i]), bl);
+      if (nsym == 1) bl = dmax(dmax(fabs(y[j]), fabs(y[i])), bl);
+    }
+  if (nsym == 1) bl = 2. * bl;
+  gl = dmax(gl, bl);
+  *zer = -1.E-5 * gl;
+  const double eppm = 1.E-25;
+  for (int i = 0; i < nfa; ++i) {
+    const int k = m1[i], l = m2[i], m = m3[i], n = m4[i];
+    const double t1x = x[m] - x[k], t1y = y[m] - y[k], t1z = z[m] - z[k];
+    const double t2x = x[n] - x[l], t2y = y[n] - y[l], t2z = z[n] - z[l];
+    const double xnx = t1y * t2z - t2y * t1z;
+    const double xny = t1z * t2x - t2z * t1x;
+    const double xnz = t1x * t2y - t2x * t1y;
+    const double xnq = sqrt(sq(xnx) + sq(xny) + sq(xnz));
+    if (xnq <= eppm) return i + 1;
+    xn[i] = xnx / xnq;
+    yn[i] = xny / xnq;
+    zn[i] = xnz / xnq;
+    const double xaver = (x[k] + x[m] + x[l] + x[n]) * 0.25;
+    const double yaver = (y[k] + y[m] + y[n] + y[l]) * 0.25;
+    const double zaver = (z[k] + z[m] + z[n] + z[l]) * 0.25;
+    const double t1 = sqrt(sq(t1x) + sq(t1y) + sq(t1z));
+    const double t1unx = t1x / t1, t1uny = t1y / t1, t1unz = t1z / t1;
+    const double t2unx = yn[i] * t1unz - zn[i] * t1uny;
+    const double t2uny = zn[i] * t1unx - xn[i] * t1unz;
+    const double t2unz = xn[i] * t1uny - yn[i] * t1unx;
+    const double xi1 = t1unx * (x[k] - xaver) + t1uny * (y[k] - yaver) + t1unz * (z[k] - zaver);
+    const double xi2 = t1unx * (x[l] - xaver) + t1uny * (y[l] - yaver) + t1unz * (z[l] - zaver);
+    const double xi3 = t1unx * (x[m] - xaver) + t1uny * (y[m] - yaver) + t1unz * (z[m] - zaver);
+    const double xi4 = t1unx * (x[n] - xaver) + t1uny * (y[n] - yaver) + t1unz * (z[n] - zaver);
+    const double eta1 = t2unx * (x[k] - xaver) + t2uny * (y[k] - yaver) + t2unz * (z[k] - zaver);
+    const double eta2 = t2unx * (x[l] - xaver) + t2uny * (y[l] - yaver) + t2unz * (z[l] - zaver);
+    const double eta3 = t2unx * (x[m] - xaver) + t2uny * (y[m] - yaver) + t2unz * (z[m] - zaver);
+    const double eta4 = t2unx * (x[n] - xaver) + t2uny * (y[n] - yaver) + t2unz * (z[n] - zaver);
+    const double xio = (xi4 * (eta1 - eta2) + xi2 * (eta4 - eta1)) / (3. * (eta2 - eta4));
+    const double etao = -eta1 / 3.;
+    const double etan1 = eta1 - etao, etan2 = eta2 - etao, etan3 = eta3 - etao,
+                 etan4 = eta4 - etao;
+    const double xin1 = xi1 - xio, xin2 = xi2 - xio, xin3 = xi3 - xio, xin4 = xi4 - xio;
+    xg[i] = xaver + t1unx * xio + t2unx * etao;
+    yg[i] = yaver + t1uny * xio + t2uny * etao;
+    zg[i] = zaver + t1unz * xio + t2unz * etao;
+    aire[i] = (xin3 - xin1) * (etan4 - etan2) * 0.5;
+    const double tt1 = sqrt(xin1 * xin1 + etan1 * etan1);
+    const double tt2 = sqrt(xin2 * xin2 + etan2 * etan2);
+    const double tt3 = sqrt(xin3 * xin3 + etan3 * etan3);
+    const double tt4 = sqrt(xin4 * xin4 + etan4 * etan4);
+    tdis[i] = dmax(dmax(tt1, tt2), dmax(tt3, tt4));
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Rankine part: VAVINFD (COMPUTE_GREEN_INFD.f90:33-220) and VAVFD
+ * (COMPUTE_GREEN_FD.f90:32-219) are the same routine up to DH (0 and
+ * 2*Depth); kkk = 1 (infinite depth) or 2 (finite depth, MK = -1). */
+typedef struct {
+  double fsp, fsm, vsxp, vsyp, vszp, vsxm, vsym, vszm;
+} ora_rk;
+
+static void ora_vav(const ora_ctx *c, int kkk, double dh, double xgi, double ygi, double zgi,
+                    int i, int j, ora_rk *o) {
+  const double pi4 = atan(1.), pi = 4. * pi4, dpi = 2. * pi, qpi = 4. * pi;
+  const int njj = 2 * (c->nsym + 1);
+  const int mk = (kkk == 1) ? 1 : -1;
+  const double xoi = xgi, yoi = ygi;
+  double zoi;
+  if (kkk == 1) zoi = (zgi > c->zer) ? c->zer : zgi;
+  else zoi = (zgi > c->zer) ? 20 * c->zer : zgi;
+  double aijs[4] = {0}, vxs[4] = {0}, vys[4] = {0}, vzs[4] = {0};
+  for (int jj = 1; jj <= njj; ++jj) {
+    const int mjj = (jj % 2) ? 1 : -1;
+    const int my = ((jj / 3) % 2) ? -1 : 1;
+    const int mz = ((jj / 2) % 2) ? -1 : 1;
+    const int mh = (1 - mz) / 2;
+    const double xoj = c->xg[j], yoj = c->yg[j] * my, zoj = c->zg[j] * mz - dh * mh;
+    const double a3j = c->xn[j], a6j = c->yn[j] * my, a9j = c->zn[j] * mz;
+    const double ro = sqrt(sq(xoi - xoj) + sq(yoi - yoj) + sq(zoi - zoj));
+    const int q = jj - 1;
+    if (ro > 7. * c->tdis[j]) { /* far field :92-97 */
+      aijs[q] = c->aire[j] / ro;
+      const double asro = aijs[q] / sq(ro);
+      vxs[q] = -(xoi - xoj) * asro;
+      vys[q] = -(yoi - yoj) * asro;
+      vzs[q] = -(zoi - zoj) * asro;
+      continue;
+    }
+    aijs[q] = vxs[q] = vys[q] = vzs[q] = 0.;
+    const int kk[5] = {c->m1[j], c->m2[j], c->m3[j], c->m4[j], c->m1[j]};
+    double txn[5], tyn[5], tzn[5], rr[5], drx[5], dry[5], drz[5];
+    for (int l = 0; l < 4; ++l) {
+      txn[l] = c->x[kk[l]];
+      tyn[l] = c->y[kk[l]] * my;
+      tzn[l] = c->z[kk[l]] * mz - dh * mh;
+    }
+    txn[4] = txn[0];
+    tyn[4] = tyn[0];
+    tzn[4] = tzn[0];
+    for (int l = 0; l < 4; ++l) {
+      rr[l] = sqrt(sq(xoi - txn[l]) + sq(yoi - tyn[l]) + sq(zoi - tzn[l]));
+      drx[l] = (xoi - txn[l]) / rr[l];
+      dry[l] = (yoi - tyn[l]) / rr[l];
+      drz[l] = (zoi - tzn[l]) / rr[l];
+    }
+    rr[4] = rr[0];
+    drx[4] = drx[0];
+    dry[4] = dry[0];
+    drz[4] = drz[0];
+    const double gz = (xoi - xoj) * a3j + (yoi - yoj) * a6j + (zoi - zoj) * a9j;
+    for (int l = 0; l < 4; ++l) { /* edge loop :127-174 */
+      const double dk = sqrt(sq(txn[l + 1] - txn[l]) + sq(tyn[l + 1] - tyn[l]) +
+                             sq(tzn[l + 1] - tzn[l]));
+      if (!(dk >= 1.E-3 * c->tdis[j])) continue;
+      const double pj = (txn[l + 1] - txn[l]) / dk;
+      const double qj = (tyn[l + 1] - tyn[l]) / dk;
+      const double rj = (tzn[l + 1] - tzn[l]) / dk;
+      const double gyx = a6j * rj - a9j * qj;
+      const double gyy = a9j * pj - a3j * rj;
+      const double gyz = a3j * qj - a6j * pj;
+      const double gy = (xoi - txn[l]) * gyx + (yoi - tyn[l]) * gyy + (zoi - tzn[l]) * gyz;
+      const double sgn = fsign(1., gz);
+      const double ddk = 2. * dk;
+      const double ant = gy * ddk;
+      const double dnt = sq(rr[l + 1] + rr[l]) - dk * dk + 2. * fabs(gz) * (rr[l + 1] + rr[l]);
+      const double arg = ant / dnt;
+      const double anl = rr[l + 1] + rr[l] + dk;
+      const double dnl = rr[l + 1] + rr[l] - dk;
+      const double den = anl / dnl;
+      const double alden = log(den);
+      const double at = (fabs(gz) >= 1.E-4 * c->tdis[j]) ? atan(arg) : 0.;
+      aijs[q] = aijs[q] + gy * alden - 2. * fabs(gz) * at;
+      const double dat = 2. * at * sgn;
+      const double antx = gyx * ddk, anty = gyy * ddk, antz = gyz * ddk;
+      const double anlx = drx[l + 1] + drx[l], anly = dry[l + 1] + dry[l],
+                   anlz = drz[l + 1] + drz[l];
+      const double dr = 2. * (rr[l + 1] + rr[l] + fabs(gz));
+      const double ds = 2. * (rr[l + 1] + rr[l]) * sgn;
+      const double dntx = dr * anlx + a3j * ds;
+      const double dnty = dr * anly + a6j * ds;
+      const double dntz = dr * anlz + a9j * ds;
+      const double denl = anl * dnl;
+      const double dent = ant * ant + dnt * dnt;
+      const double atx = (antx * dnt - dntx * ant) / dent;
+      const double aty = (anty * dnt - dnty * ant) / dent;
+      const double atz = (antz * dnt - dntz * ant) / dent;
+      const double dlogg = (dnl - anl) / denl;
+      vxs[q] = vxs[q] + gyx * alden + gy * anlx * dlogg - 2. * fabs(gz) * atx - dat * a3j;
+      vys[q] = vys[q] + gyy * alden + gy * anly * dlogg - 2. * fabs(gz) * aty - dat * a6j;
+      vzs[q] = vzs[q] + gyz * alden + gy * anlz * dlogg - 2. * fabs(gz) * atz - dat * a9j;
+    }
+    if (i == j && jj == 1) { /* self term :175-178 */
+      vxs[0] = vxs[0] - dpi * a3j;
+      vys[0] = vys[0] - dpi * a6j;
+      vzs[0] = vzs[0] - dpi * a9j;
+    } else {
+      aijs[q] = aijs[q] * mjj;
+      vxs[q] = vxs[q] * mjj;
+      vys[q] = vys[q] * mjj;
+      vzs[q] = vzs[q] * mjj;
+    }
+  }
+  double w;
+  if (c->nsym == 1) { /* :187-203 */
+    w = aijs[0] - mk * (aijs[1] + aijs[2]) + aijs[3];
+    o->fsp = -w / qpi;
+    w = aijs[0] - mk * (aijs[1] - aijs[2]) - aijs[3];
+    o->fsm = -w / qpi;
+    w = vxs[0] - mk * (vxs[1] + vxs[2]) + vxs[3];
+    o->vsxp = -w / qpi;
+    w = vys[0] - mk * (vys[1] + vys[2]) + vys[3];
+    o->vsyp = -w / qpi;
+    w = vzs[0] - mk * (vzs[1] + vzs[2]) + vzs[3];
+    o->vszp = -w / qpi;
+    w = vxs[0] - mk * (vxs[1] - vxs[2]) - vxs[3];
+    o->vsxm = -w / qpi;
+    w = vys[0] - mk * (vys[1] - vys[2]) - vys[3];
+    o->vsym = -w / qpi;
+    w = vzs[0] - mk * (vzs[1] - vzs[2]) - vzs[3];
+    o->vszm = -w / qpi;
+  } else { /* :205-216 */
+    w = aijs[0] - mk * aijs[1];
+    o->fsp = -w / qpi;
+    o->fsm = o->fsp;
+    w = vxs[0] - mk * vxs[1];
+    o->vsxp = -w / qpi;
+    o->vsxm = o->vsxp;
+    w = vys[0] - mk * vys[1];
+    o->vsyp = -w / qpi;
+    o->vsym = o->vsyp;
+    w = vzs[0] - mk * vzs[1];
+    o->vszp = -w / qpi;
+    o->vszm = o->vszp;
+  }
+}
+
+static inline double pl2(double u1, double u2, double u3, double xu) { /* PL2 :641-645 */
+  return ((xu - u1) * (xu - u2)) / ((u3 - u1) * (u3 - u2));
+}
+
+#define XR(i) c->xr[(i) - 1]
+#define XZ(j) c->xz[(j) - 1]
+#define TAB(t, i, j) c->t[((i) - 1) + ORA_NR * ((j) - 1)]
+
+/* 3x3 quadratic-Lagrange interpolation of one table, as written out at
+ * COMPUTE_GREEN_INFD.f90:312-315 */
+static double interp(const double *t, int ki, int kj, double xl1, double xl2, double xl3,
+                     double zl1, double zl2, double zl3) {
+#define T_(i, j) t[((i) - 1) + ORA_NR * ((j) - 1)]
+  const double f1 = xl1 * T_(ki - 1, kj - 1) + xl2 * T_(ki, kj - 1) + xl3 * T_(ki + 1, kj - 1);
+  const double f2 = xl1 * T_(ki - 1, kj) + xl2 * T_(ki, kj) + xl3 * T_(ki + 1, kj);
+  const double f3 = xl1 * T_(ki - 1, kj + 1) + xl2 * T_(ki, kj + 1) + xl3 * T_(ki + 1, kj + 1);
+#undef T_
+  return zl1 * f1 + zl2 * f2 + zl3 * f3;
+}
+
+/* table index selection, COMPUTE_GREEN_INFD.f90:292-305 */
+static int kj_of(double akz) {
+  int kj;
+  if (akz < -1.E-2) kj = (int)(8 * (log10(-akz) + 4.5));
+  else kj = (int)(5 * (log10(-akz) + 6));
+  return imin(imax(kj, 2), 45);
+}
+static int ki_of(double akr) {
+  int ki;
+  if (akr < 1.) ki = (int)(5 * (log10(akr + 1.E-20) + 6) + 1);
+  else ki = (int)(3 * akr + 28);
+  return imin(imax(ki, 2), 327);
+}
+
+/* VNSINFD (COMPUTE_GREEN_INFD.f90:223-503) on top of the Rankine part rk */
+static int ora_vnsinfd(const ora_ctx *c, const ora_freq *f, int i, int j, double xgi,
+                       double ygi, double zgi, const ora_rk *rk, double *out) {
+  const double pi4 = atan(1.), pi = 4. * pi4, dpi = 2. * pi, dpi2 = 2. * sq(pi);
+  const double eps = 0.0001;
+  const double complex zi = I;
+  const double wh = dpi / f->t, wr = wh;
+  const double ak0 = sq(wr) / c->g;
+  const int njj = c->nsym + 1;
+  int undefined = 0;
+  (void)i;
+  double zmiii = zgi;
+  if (zgi > c->zer) zmiii = c->zer;
+  double fs1[2], fs2[2], vsx1[2], vsy1[2], vsz1[2], vsx2[2], vsy2[2], vsz2[2];
+  /* values the reference leaves unset on its undefined branch */
+  double xl1 = 0, xl2 = 0, xl3 = 0, zl1 = 0, zl2 = 0, zl3 = 0, pd1z = 0, pd2z = 0, vz1 = 0,
+         vz2 = 0, pd1x = 0, pd2x = 0, vr1 = 0, vr2 = 0, epz = 0, sqv = 0, csk = 0, sik = 0;
+  int ki = 2, kj = 2;
+  for (int jj = 1; jj <= njj; ++jj) {
+    const int q = jj - 1;
+    const int bx = (jj % 2) ? 1 : -1;
+    if (zgi < c->zer && c->zg[j] < c->zer) { /* submerged :271-384 */
+      const double ymjjj = c->yg[j] * bx;
+      const double rrr = sqrt(sq(xgi - c->xg[j]) + sq(ygi - ymjjj));
+      const double akr = ak0 * rrr;
+      const double zzz = zmiii + c->zg[j];
+      const double akz = ak0 * zzz;
+      const double dd = sqrt(sq(rrr) + sq(zzz));
+      const double psurr = (dd > eps) ? pi / cube(ak0 * dd) : 0.;
+      if (akz > -1.5E-6) {
+        undefined = 1;
+        pd1z = pd2z = vz1 = vz2 = 0.;
+      } else if (akz > -16.) {
+        if (akr < 99.7) {
+          kj = kj_of(akz);
+          ki = ki_of(akr);
+          xl1 = pl2(XR(ki), XR(ki + 1), XR(ki - 1), akr);
+          xl2 = pl2(XR(ki + 1), XR(ki - 1), XR(ki), akr);
+          xl3 = pl2(XR(ki - 1), XR(ki), XR(ki + 1), akr);
+          zl1 = pl2(XZ(kj), XZ(kj + 1), XZ(kj - 1), akz);
+          zl2 = pl2(XZ(kj + 1), XZ(kj - 1), XZ(kj), akz);
+          zl3 = pl2(XZ(kj - 1), XZ(kj), XZ(kj + 1), akz);
+          pd1z = interp(c->apd1z, ki, kj, xl1, xl2, xl3, zl1, zl2, zl3);
+          pd2z = interp(c->apd2z, ki, kj, xl1, xl2, xl3, zl1, zl2, zl3);
+        } else {
+          epz = exp(akz);
+          const double akp4 = akr - pi4;
+          sqv = sqrt(dpi / akr);
+          csk = cos(akp4);
+          sik = sin(akp4);
+          pd1z = psurr * akz - pi * epz * sqv * sik;
+          pd2z = epz * sqv * csk;
+        }
+        vz1 = pd1z - psurr * akz;
+        vz2 = pd2z;
+      } else {
+        pd1z = psurr * akz;
+        pd2z = 0.;
+        vz1 = 0.;
+        vz2 = 0.;
+      }
+      fs1[q] = pd1z;
+      fs2[q] = pd2z;
+      if (rrr > eps) {
+        if (akz <= -1.5E-6) {
+          if (akz > -16.) {
+            if (akr < 99.7) {
+              pd1x = interp(c->apd1x, ki, kj, xl1, xl2, xl3, zl1, zl2, zl3);
+              pd2x = interp(c->apd2x, ki, kj, xl1, xl2, xl3, zl1, zl2, zl3);
+            } else {
+              const double dsk = 0.5 / akr;
+              pd1x = -psurr * akr + pi * epz * sqv * (csk - dsk * sik);
+              pd2x = epz * sqv * (sik + dsk * csk);
+            }
+            vr1 = -pd1x - psurr * akr;
+            vr2 = -pd2x;
+          } else {
+            pd1x = -psurr * akr;
+            pd2x = 0.;
+            vr1 = 0.;
+            vr2 = 0.;
+          }
+        } else {
+          vr1 = vr2 = 0.;
+        }
+        const double cvx = (xgi - c->xg[j]) / rrr;
+        const double cvy = (ygi - ymjjj) / rrr;
+        vsx1[q] = vr1 * cvx;
+        vsx2[q] = vr2 * cvx;
+        vsy1[q] = vr1 * cvy;
+        vsy2[q] = vr2 * cvy;
+        vsz1[q] = vz1;
+        vsz2[q] = vz2;
+      } else {
+        vsx1[q] = 0.;
+        vsx2[q] = 0.;
+        vsy1[q] = 0.;
+        vsy2[q] = 0.;
+        vsz1[q] = vz1;
+        vsz2[q] = vz2;
+      }
+    } else { /* near the free surface: theta quadrature :385-453 */
+      fs1[q] = fs2[q] = vsx1[q] = vsx2[q] = vsy1[q] = vsy2[q] = vsz1[q] = vsz2[q] = 0.;
+      const int kk[5] = {c->m1[j], c->m2[j], c->m3[j], c->m4[j], c->m1[j]};
+      for (int it = 0; it < f->nq; ++it) {
+        const double teta = f->qq[it];
+        const double ct = cos(teta), st = sin(teta);
+        double complex zij[5], cex[5], gz[5];
+        for (int l = 0; l < 4; ++l) {
+          zij[l] = ak0 * (zmiii + c->z[kk[l]] +
+                          zi * ((xgi - c->x[kk[l]]) * ct + (ygi - c->y[kk[l]] * bx) * st));
+          cex[l] = (creal(zij[l]) > -25.) ? cexp(zij[l]) : 0.;
+          gz[l] = ora_gg(zij[l], cex[l]);
+        }
+        zij[4] = zij[0];
+        cex[4] = cex[0];
+        gz[4] = gz[0];
+        double complex za = 0., zb = 0., zvs = 0.;
+        for (int l = 0; l < 4; ++l) {
+          const double complex c1 = zij[l + 1] - zij[l];
+          double complex asd, bsd, csd;
+          if (fabs(cimag(c1)) < eps && fabs(creal(c1)) < eps) {
+            asd = (gz[l + 1] + gz[l]) * 0.5;
+            bsd = (cex[l + 1] + cex[l]) * 0.5;
+            csd = asd - (0.5 / zij[l + 1] + 0.5 / zij[l]);
+          } else {
+            csd = (gz[l + 1] - gz[l]) / c1;
+            asd = (gz[l + 1] - gz[l] + clog(zij[l + 1] / zij[l])) / c1;
+            bsd = (cex[l + 1] - cex[l]) / c1;
+          }
+          const double complex c2 =
+              (c->yn[j] * bx - zi * c->zn[j] * st) * (c->x[kk[l + 1]] - c->x[kk[l]]) -
+              (c->xn[j] - zi * c->zn[j] * ct) * (c->y[kk[l + 1]] - c->y[kk[l]]) * bx;
+          za = za + asd * c2;
+          zb = zb + bsd * c2;
+          zvs = zvs + csd * c2;
+        }
+        fs1[q] = fs1[q] + f->cq[it] * creal(za);
+        fs2[q] = fs2[q] + f->cq[it] * creal(zb);
+        vsx1[q] = vsx1[q] - f->cq[it] * ct * cimag(zvs);
+        vsx2[q] = vsx2[q] - f->cq[it] * ct * cimag(zb);
+        vsy1[q] = vsy1[q] - f->cq[it] * st * cimag(zvs);
+        vsy2[q] = vsy2[q] - f->cq[it] * st * cimag(zb);
+        vsz1[q] = vsz1[q] + f->cq[it] * creal(zvs);
+        vsz2[q] = vsz2[q] + f->cq[it] * creal(zb);
+      }
+      const double akair = ak0 * c->aire[j];
+      fs1[q] = fs1[q] * bx / akair;
+      fs2[q] = fs2[q] * bx / akair;
+      vsx1[q] = vsx1[q] * bx / akair;
+      vsx2[q] = vsx2[q] * bx / akair;
+      vsy1[q] = vsy1[q] * bx / akair;
+      vsy2[q] = vsy2[q] * bx / akair;
+      vsz1[q] = vsz1[q] * bx / akair;
+      vsz2[q] = vsz2[q] * bx / akair;
+    }
+  }
+  const double akair = ak0 * c->aire[j];
+  const double adpi2 = akair / dpi2, adpi = akair / dpi;
+  const double akdpi2 = adpi2 * ak0, akdpi = adpi * ak0;
+  if (c->nsym == 1) { /* :456-477 */
+    out[1] = rk->fsm - (fs1[0] - fs1[1]) * adpi2;           /* SM1 */
+    out[0] = rk->fsp - (fs1[0] + fs1[1]) * adpi2;           /* SP1 */
+    out[3] = -(fs2[0] - fs2[1]) * adpi;                     /* SM2 */
+    out[2] = -(fs2[0] + fs2[1]) * adpi;                     /* SP2 */
+    out[4] = rk->vsxp - (vsx1[0] + vsx1[1]) * akdpi2;       /* VSXP1 */
+    out[7] = rk->vsxm - (vsx1[0] - vsx1[1]) * akdpi2;       /* VSXM1 */
+    out[5] = rk->vsyp - (vsy1[0] + vsy1[1]) * akdpi2;       /* VSYP1 */
+    out[8] = rk->vsym - (vsy1[0] - vsy1[1]) * akdpi2;       /* VSYM1 */
+    out[6] = rk->vszp - (vsz1[0] + vsz1[1]) * akdpi2;       /* VSZP1 */
+    out[9] = rk->vszm - (vsz1[0] - vsz1[1]) * akdpi2;       /* VSZM1 */
+    out[10] = -(vsx2[0] + vsx2[1]) * akdpi;                 /* VSXP2 */
+    out[13] = -(vsx2[0] - vsx2[1]) * akdpi;                 /* VSXM2 */
+    out[11] = -(vsy2[0] + vsy2[1]) * akdpi;                 /* VSYP2 */
+    out[14] = -(vsy2[0] - vsy2[1]) * akdpi;                 /* VSYM2 */
+    out[12] = -(vsz2[0] + vsz2[1]) * akdpi;                 /* VSZP2 */
+    out[15] = -(vsz2[0] - vsz2[1]) * akdpi;                 /* VSZM2 */
+  } else { /* :478-499 */
+    out[0] = rk->fsp - fs1[0] * adpi2;
+    out[1] = out[0];
+    out[2] = -fs2[0] * adpi;
+    out[3] = out[2];
+    out[4] = rk->vsxp - vsx1[0] * akdpi2;
+    out[7] = out[4];
+    out[5] = rk->vsyp - vsy1[0] * akdpi2;
+    out[8] = out[5];
+    out[6] = rk->vszp - vsz1[0] * akdpi2;
+    out[9] = out[6];
+    out[10] = -vsx2[0] * akdpi;
+    out[13] = out[10];
+    out[11] = -vsy2[0] * akdpi;
+    out[14] = out[11];
+    out[12] = -vsz2[0] * akdpi;
+    out[15] = out[12];
+  }
+  return undefined;
+}
+
+/* VSD (COMPUTE_GREEN_FD.f90:1514-1609): Rankine integral of panel (xft,
+ * yft, zft) seen from a field point shifted to depth zig */
+static void ora_vsd(const double *xft, const double *yft, const double *zft, int jj, double xnf,
+                    double ynf, double znf, double a, double td, double xgj, double ygj,
+                    double zgj, double xgi, double ygi, double zig, double *fs, double *vx,
+                    double *vy, double *vz) {
+  const double qpi = 4. * 4. * atan(1.);
+  const int mjj = (jj % 2) ? 1 : -1;
+  const double qmp = mjj / (2 * qpi);
+  const double ro = sqrt(sq(xgi - xgj) + sq(ygi - ygj * mjj) + sq(zig - zgj));
+  const double gz = (xgi - xgj) * xnf + (ygi - ygj * mjj) * ynf * mjj + (zig - zgj) * znf;
+  if (ro > 7 * td) {
+    *fs = -a / (ro * 2 * qpi);
+    const double asro = *fs / sq(ro);
+    *vx = -(xgi - xgj) * asro;
+    *vy = -(ygi - ygj * mjj) * asro;
+    *vz = -(zig - zgj) * asro;
+    return;
+  }
+  double rr[5], drx[5], dry[5], drz[5];
+  for (int l = 0; l < 4; ++l) {
+    rr[l] = sqrt(sq(xgi - xft[l]) + sq(ygi - yft[l] * mjj) + sq(zig - zft[l]));
+    drx[l] = (xgi - xft[l]) / rr[l];
+    dry[l] = (ygi - yft[l] * mjj) / rr[l];
+    drz[l] = (zig - zft[l]) / rr[l];
+  }
+  rr[4] = rr[0];
+  drx[4] = drx[0];
+  dry[4] = dry[0];
+  drz[4] = drz[0];
+  double aijs = 0., vxs = 0., vys = 0., vzs = 0.;
+  for (int l = 0; l < 4; ++l) {
+    const double dk = sqrt(sq(xft[l + 1] - xft[l]) + sq(yft[l + 1] - yft[l]) +
+                           sq(zft[l + 1] - zft[l]));
+    if (!(dk >= 1.E-3 * td)) continue;
+    const double pj = (xft[l + 1] - xft[l]) / dk;
+    const double qj = (yft[l + 1] - yft[l]) * mjj / dk;
+    const double rj = (zft[l + 1] - zft[l]) / dk;
+    const double gyx = ynf * mjj * rj - znf * qj;
+    const double gyy = znf * pj - xnf * rj;
+    const double gyz = xnf * qj - ynf * mjj * pj;
+    const double gy = (xgi - xft[l]) * gyx + (ygi - yft[l] * mjj) * gyy + (zig - zft[l]) * gyz;
+    const double sgn = fsign(1., gz);
+    const double ddk = 2. * dk;
+    const double ant = gy * ddk;
+    const double dnt = sq(rr[l + 1] + rr[l]) - dk * dk + 2. * fabs(gz) * (rr[l + 1] + rr[l]);
+    const double anl = rr[l + 1] + rr[l] + dk;
+    const double dnl = rr[l + 1] + rr[l] - dk;
+    const double den = anl / dnl;
+    const double alden = log(den);
+    double at;
+    if (fabs(gz) > 1.E-4 * td) {
+      const double arg = ant / dnt;
+      at = atan(arg);
+    } else {
+      at = 0.;
+    }
+    aijs = aijs + gy * alden - 2. * fabs(gz) * at;
+    const double dat = 2. * at * sgn;
+    const double antx = gyx * ddk, anty = gyy * ddk, antz = gyz * ddk;
+    const double anlx = drx[l + 1] + drx[l], anly = dry[l + 1] + dry[l],
+                 anlz = drz[l + 1] + drz[l];
+    const double dr = 2. * (rr[l + 1] + rr[l] + fabs(gz));
+    const double ds = 2. * (rr[l + 1] + rr[l]) * sgn;
+    const double dntx = dr * anlx + xnf * ds;
+    const double dnty = dr * anly + ynf * mjj * ds;
+    const double dntz = dr * anlz + znf * ds;
+    const double denl = anl * dnl;
+    const double dent = ant * ant + dnt * dnt;
+    const double atx = (antx * dnt - dntx * ant) / dent;
+    const double aty = (anty * dnt - dnty * ant) / dent;
+    const double atz = (antz * dnt - dntz * ant) / dent;
+    const double dlg = (dnl - anl) / denl;
+    vxs = vxs + gyx * alden + gy * anlx * dlg - 2. * fabs(gz) * atx - dat * xnf;
+    vys = vys + gyy * alden + gy * anly * dlg - 2. * fabs(gz) * aty - dat * ynf * mjj;
+    vzs = vzs + gyz * alden + gy * anlz * dlg - 2. * fabs(gz) * atz - dat * znf;
+  }
+  *fs = -aijs * qmp;
+  *vx = -vxs * qmp;
+  *vy = -vys * qmp;
+  *vz = -vzs * qmp;
+}
+
+/* one z-image of the finite-depth submerged branch, COMPUTE_GREEN_FD.f90:
+ * 295-518 (the four images are written out in the reference; the shared
+ * quantities XL*, KI, PSK, SCK, SCDS, SSDS come from image 1) */
+typedef struct {
+  double akz, psr, psurr, pd1z, pd2z, vz1, vz2, epz, zl1, zl2, zl3;
+  int kj, undefined;
+} fd_img;
+
+typedef struct {
+  double xl1, xl2, xl3, psk, sck, sq, csk, sik, scds, ssds;
+  int ki;
+} fd_shared;
+
+static void fd_image_z(const ora_ctx *c, double am0, double rrr, double akr, double zzz,
+                       int first, fd_shared *sh, fd_img *m) {
+  const double pi4 = atan(1.), pi = 4. * atan(1.), dpi = 2. * pi;
+  const double eps = 0.0001;
+  m->undefined = 0;
+  m->akz = am0 * zzz;
+  const double dd = sqrt(sq(rrr) + sq(zzz));
+  if (dd > eps) {
+    const double rrv = am0 * dd;
+    m->psr = pi / rrv;
+    m->psurr = pi / cube(rrv);
+  } else {
+    m->psr = 0.;
+    m->psurr = 0.;
+  }
+  if (m->akz > -1.5E-6) {
+    m->undefined = 1;
+    m->pd1z = m->pd2z = m->vz1 = m->vz2 = 0.;
+    return;
+  }
+  if (m->akz > -16.) {
+    if (akr < 99.7) {
+      m->kj = kj_of(m->akz);
+      if (first) {
+        sh->ki = ki_of(akr);
+        const int ki = sh->ki;
+        sh->xl1 = pl2(XR(ki), XR(ki + 1), XR(ki - 1), akr);
+        sh->xl2 = pl2(XR(ki + 1), XR(ki - 1), XR(ki), akr);
+        sh->xl3 = pl2(XR(ki - 1), XR(ki), XR(ki + 1), akr);
+      }
+      const int kj = m->kj;
+      m->zl1 = pl2(XZ(kj), XZ(kj + 1), XZ(kj - 1), m->akz);
+      m->zl2 = pl2(XZ(kj + 1), XZ(kj - 1), XZ(kj), m->akz);
+      m->zl3 = pl2(XZ(kj - 1), XZ(kj), XZ(kj + 1), m->akz);
+      m->pd1z = interp(c->apd1z, sh->ki, kj, sh->xl1, sh->xl2, sh->xl3, m->zl1, m->zl2, m->zl3);
+      m->pd2z = interp(c->apd2z, sh->ki, kj, sh->xl1, sh->xl2, sh->xl3, m->zl1, m->zl2, m->zl3);
+    } else {
+      m->epz = exp(m->akz);
+      if (first) {
+        const double akp4 = akr - pi4;
+        sh->sq = sqrt(dpi / akr);
+        sh->csk = cos(akp4);
+        sh->sik = sin(akp4);
+        sh->psk = pi * sh->sq * sh->sik;
+        sh->sck = sh->sq * sh->csk;
+      }
+      m->pd1z = m->psurr * m->akz - sh->psk * m->epz;
+      m->pd2z = m->epz * sh->sck;
+    }
+    m->vz1 = m->pd1z - m->psurr * m->akz;
+    m->vz2 = m->pd2z;
+  } else {
+    m->pd1z = m->psurr * m->akz;
+    m->pd2z = 0.;
+    m->vz1 = 0.;
+    m->vz2 = 0.;
+  }
+}
+
+/* r-derivative of one z-image, COMPUTE_GREEN_FD.f90:526-616 */
+static void fd_image_r(const ora_ctx *c, double akr, int first, fd_shared *sh,
+                       const fd_img *m, double *pd1x, double *vr2) {
+  const double pi = 4. * atan(1.);
+  if (!(m->akz <= -1.5E-6)) {
+    *pd1x = 0.;
+    *vr2 = 0.;
+    return;
+  }
+  if (m->akz > -16.) {
+    double pd2x;
+    if (akr < 99.7) {
+      *pd1x = interp(c->apd1x, sh->ki, m->kj, sh->xl1, sh->xl2, sh->xl3, m->zl1, m->zl2, m->zl3);
+      pd2x = interp(c->apd2x, sh->ki, m->kj, sh->xl1, sh->xl2, sh->xl3, m->zl1, m->zl2, m->zl3);
+    } else {
+      if (first) {
+        const double dsk = 0.5 / akr;
+        sh->scds = pi * sh->sq * (sh->csk - dsk * sh->sik);
+        sh->ssds = sh->sq * (sh->sik + dsk * sh->csk);
+      }
+      *pd1x = -m->psurr * akr - m->epz * sh->scds;
+      pd2x = m->epz * sh->ssds;
+    }
+    *vr2 = -pd2x;
+  } else {
+    *pd1x = -m->psurr * akr;
+    *vr2 = 0.;
+  }
+}
+
+/* VNSFD (COMPUTE_GREEN_FD.f90:222-868) on top of the Rankine part rk */
+static int ora_vnsfd(const ora_ctx *c, const ora_freq *f, int i, int j, double xgi,
+                     double ygi, double zgi, const ora_rk *rk, double *out) {
+  const double pi = 4. * atan(1.), qpi = 4. * pi;
+  const double h = c->depth;
+  const double complex zi = I;
+  const double amh = f->amh;
+  const double am0 = amh / h;
+  const double akh = amh * tanh(amh);
+  const double a = sq(amh + akh) / (h * (sq(amh) - sq(akh) + akh));
+  const int nexp1 = f->nexp + 1;
+  double ambda[32], ar[32];
+  for (int k = 0; k < f->nexp; ++k) {
+    ambda[k] = f->ambda[k];
+    ar[k] = f->ar[k];
+  }
+  ambda[nexp1 - 1] = 0.; /* :262-263 */
+  ar[nexp1 - 1] = 2.;
+  const double adpi2 = -a / (8. * sq(pi));
+  const double adpi = -a / (8 * pi);
+  const double coe1 = adpi2 / am0, coe2 = adpi / am0, coe3 = adpi2, coe4 = adpi;
+  const double eps = 0.0001;
+  const int njj = c->nsym + 1;
+  int undefined = 0;
+  (void)i;
+  double zmiii = zgi;
+  if (zgi > c->zer) zmiii = 20 * c->zer;
+  double fs1[2], fs2[2], vsx1[2], vsy1[2], vsz1[2], vsx2[2], vsy2[2], vsz2[2];
+  for (int jj = 1; jj <= njj; ++jj) {
+    const int q = jj - 1;
+    const int bx = (jj % 2) ? 1 : -1;
+    if (zgi < c->zer && c->zg[j] < c->zer) { /* submerged :286-706 */
+      const double ymjjj = bx * c->yg[j];
+      const double cof1 = coe3 * c->aire[j], cof2 = coe4 * c->aire[j];
+      const double cof3 = am0 * cof1, cof4 = am0 * cof2;
+      const double rrr = sqrt(sq(xgi - c->xg[j]) + sq(ygi - ymjjj));
+      const double akr = am0 * rrr;
+      fd_shared sh;
+      memset(&sh, 0, sizeof sh);
+      fd_img m[4];
+      fd_image_z(c, am0, rrr, akr, zmiii + c->zg[j], 1, &sh, &m[0]);
+      fd_image_z(c, am0, rrr, akr, c->zg[j] - zmiii - 2 * h, 0, &sh, &m[1]);
+      fd_image_z(c, am0, rrr, akr, zmiii - c->zg[j] - 2 * h, 0, &sh, &m[2]);
+      fd_image_z(c, am0, rrr, akr, -c->zg[j] - zmiii - 4 * h, 0, &sh, &m[3]);
+      for (int k = 0; k < 4; ++k) undefined |= m[k].undefined;
+      const double qtqq = m[0].pd1z + m[1].pd1z + m[2].pd1z + m[3].pd1z;
+      fs1[q] = cof1 * (qtqq - m[0].psr - m[1].psr - m[2].psr - m[3].psr);
+      const double stss = m[0].pd2z + m[1].pd2z + m[2].pd2z + m[3].pd2z;
+      fs2[q] = cof2 * stss;
+      if (rrr > eps) {
+        double pd1x[4], vr2[4];
+        for (int k = 0; k < 4; ++k) fd_image_r(c, akr, k == 0, &sh, &m[k], &pd1x[k], &vr2[k]);
+        const double c1v3 = -cof3 * (pd1x[0] + pd1x[1] + pd1x[2] + pd1x[3]);
+        const double c2v3 = cof4 * (vr2[0] + vr2[1] + vr2[2] + vr2[3]);
+        const double cvx = (xgi - c->xg[j]) / rrr;
+        const double cvy = (ygi - ymjjj) / rrr;
+        vsx1[q] = c1v3 * cvx;
+        vsx2[q] = c2v3 * cvx;
+        vsy1[q] = c1v3 * cvy;
+        vsy2[q] = c2v3 * cvy;
+      } else {
+        vsx1[q] = vsx2[q] = vsy1[q] = vsy2[q] = 0.;
+      }
+      vsz1[q] = cof3 * (m[0].pd1z - m[1].pd1z + m[2].pd1z - m[3].pd1z);
+      vsz2[q] = cof4 * (m[0].vz2 - m[1].vz2 + m[2].vz2 - m[3].vz2);
+      /* Prony images as one-point Rankine sources :635-706 */
+      const double xpg = xgi - c->xg[j], ypg = ygi - ymjjj;
+      const double act = -0.5 * c->aire[j] / qpi;
+      for (int ke = 0; ke < nexp1; ++ke) {
+        const double aqt = act * ar[ke];
+        const double zpg[4] = {zmiii - 2. * h + h * ambda[ke] - c->zg[j],
+                               -zmiii - h * ambda[ke] - c->zg[j],
+                               -zmiii - 4. * h + h * ambda[ke] - c->zg[j],
+                               zmiii + 2. * h - h * ambda[ke] - c->zg[j]};
+        double fts[4], vxs[4], vys[4], vzs[4];
+        for (int k = 0; k < 4; ++k) {
+          const double rr = sq(rrr) + sq(zpg[k]);
+          const double ro = sqrt(rr);
+          double asro;
+          if (ro > eps) {
+            fts[k] = aqt / ro;
+            asro = fts[k] / rr;
+          } else {
+            fts[k] = 0.;
+            asro = 0.;
+          }
+          vxs[k] = -xpg * asro;
+          vys[k] = -ypg * asro;
+          vzs[k] = -zpg[k] * asro;
+        }
+        fs1[q] = fs1[q] + fts[0] + fts[1] + fts[2] + fts[3];
+        vsx1[q] = vsx1[q] + (vxs[0] + vxs[1] + vxs[2] + vxs[3]);
+        vsy1[q] = vsy1[q] + (vys[0] + vys[1] + vys[2] + vys[3]);
+        vsz1[q] = vsz1[q] + (vzs[0] - vzs[1] - vzs[2] + vzs[3]);
+      }
+    } else { /* near the free surface :707-827 */
+      fs1[q] = fs2[q] = vsx1[q] = vsx2[q] = vsy1[q] = vsy2[q] = vsz1[q] = vsz2[q] = 0.;
+      const int kk[5] = {c->m1[j], c->m2[j], c->m3[j], c->m4[j], c->m1[j]};
+      for (int it = 0; it < f->nq; ++it) {
+        const double teta = f->qq[it];
+        const double ct = cos(teta), st = sin(teta);
+        double complex zij[4][5], cex[4][5], gz[4][5], cl[4][5];
+        for (int l = 0; l < 4; ++l) {
+          const double om = (xgi - c->x[kk[l]]) * ct + (ygi - bx * c->y[kk[l]]) * st;
+          zij[0][l] = am0 * (c->z[kk[l]] + zmiii + zi * om);
+          zij[1][l] = am0 * (c->z[kk[l]] - zmiii - 2 * h + zi * om);
+          zij[2][l] = am0 * (zmiii - c->z[kk[l]] - 2 * h + zi * om);
+          zij[3][l] = am0 * (-c->z[kk[l]] - zmiii - 4 * h + zi * om);
+          for (int kl = 0; kl < 4; ++kl) {
+            cex[kl][l] = (creal(zij[kl][l]) > -25.) ? cexp(zij[kl][l]) : 0.;
+            gz[kl][l] = ora_gg(zij[kl][l], cex[kl][l]);
+            cl[kl][l] = clog(-zij[kl][l]);
+          }
+        }
+        for (int kl = 0; kl < 4; ++kl) {
+          zij[kl][4] = zij[kl][0];
+          cex[kl][4] = cex[kl][0];
+          gz[kl][4] = gz[kl][0];
+          cl[kl][4] = cl[kl][0];
+        }
+        double complex s1 = 0., s2 = 0., zv1 = 0., zv2 = 0., zv3 = 0., zv4 = 0.;
+        const double complex zirs = zi * c->zn[j] * st, zirc = zi * c->zn[j] * ct;
+        for (int l = 0; l < 4; ++l) {
+          const double dxl = (c->x[kk[l + 1]] - c->x[kk[l]]);
+          const double dyl = (c->y[kk[l + 1]] - c->y[kk[l]]) * bx;
+          for (int kl = 0; kl < 4; ++kl) {
+            const int bkl = (kl % 2) ? -1 : 1;
+            double complex aux;
+            if (kl < 2) aux = dxl * (c->yn[j] * bx - zirs) - dyl * (c->xn[j] - zirc);
+            else aux = dxl * (-c->yn[j] * bx - zirs) - dyl * (-c->xn[j] - zirc);
+            const double complex z1 = zij[kl][l + 1], z0 = zij[kl][l];
+            const double complex cl1 = cl[kl][l + 1], cl0 = cl[kl][l];
+            const double complex g1 = gz[kl][l + 1], g0 = gz[kl][l];
+            const double complex cex1 = cex[kl][l + 1], cex0 = cex[kl][l];
+            const double complex zam = z1 - z0;
+            if (fabs(cimag(zam)) < eps && fabs(creal(zam)) < eps) {
+              s1 = s1 + aux * (g1 + g0 + cl1 + cl0) * 0.5;
+              zv1 = zv1 + aux * bkl * (g1 + g0) * 0.5;
+              zv3 = zv3 + aux * (g1 + g0) * 0.5;
+              s2 = s2 + aux * (cex1 + cex0) * 0.5;
+              zv2 = zv2 + aux * bkl * (cex1 + cex0) * 0.5;
+              zv4 = zv4 + aux * (cex1 + cex0) * 0.5;
+            } else {
+              s1 = s1 + aux * (g1 - g0 + cl1 - cl0 + z1 * cl1 - z0 * cl0 - zam) / zam;
+              zv1 = zv1 + aux * bkl * (g1 - g0 + cl1 - cl0) / zam;
+              zv3 = zv3 + aux * (g1 - g0 + cl1 - cl0) / zam;
+              s2 = s2 + aux * (cex1 - cex0) / zam;
+              zv2 = zv2 + aux * bkl * (cex1 - cex0) / zam;
+              zv4 = zv4 + aux * (cex1 - cex0) / zam;
+            }
+          }
+        }
+        fs1[q] = fs1[q] + f->cq[it] * creal(s1) * coe1 * bx;
+        fs2[q] = fs2[q] + f->cq[it] * creal(s2) * coe2 * bx;
+        vsx1[q] = vsx1[q] - f->cq[it] * ct * cimag(zv3) * coe3 * bx;
+        vsx2[q] = vsx2[q] - f->cq[it] * ct * cimag(zv4) * coe4 * bx;
+        vsy1[q] = vsy1[q] - f->cq[it] * st * cimag(zv3) * coe3 * bx;
+        vsy2[q] = vsy2[q] - f->cq[it] * st * cimag(zv4) * coe4 * bx;
+        vsz1[q] = vsz1[q] + f->cq[it] * creal(zv1) * coe3 * bx;
+        vsz2[q] = vsz2[q] + f->cq[it] * creal(zv2) * coe4 * bx;
+      }
+      const double zp[4] = {-zmiii, zmiii + 2 * h, zmiii - 2. * h, -zmiii - 4 * h};
+      double xft[5], yft[5], zft[5];
+      for (int l = 0; l < 5; ++l) {
+        xft[l] = c->x[kk[l]];
+        yft[l] = c->y[kk[l]];
+        zft[l] = c->z[kk[l]];
+      }
+      for (int ke = 0; ke < nexp1; ++ke) {
+        double ffs[4], vx[4], vy[4], vz[4];
+        const double zig[4] = {zp[0] - h * ambda[ke], zp[1] - h * ambda[ke],
+                               zp[2] + h * ambda[ke], zp[3] + h * ambda[ke]};
+        for (int k = 0; k < 4; ++k)
+          ora_vsd(xft, yft, zft, jj, c->xn[j], c->yn[j], c->zn[j], c->aire[j], c->tdis[j],
+                  c->xg[j], c->yg[j], c->zg[j], xgi, ygi, zig[k], &ffs[k], &vx[k], &vy[k],
+                  &vz[k]);
+        fs1[q] = fs1[q] + (ffs[0] + ffs[1] + ffs[2] + ffs[3]) * ar[ke];
+        vsx1[q] = vsx1[q] + (vx[0] + vx[1] + vx[2] + vx[3]) * ar[ke];
+        vsy1[q] = vsy1[q] + (vy[0] + vy[1] + vy[2] + vy[3]) * ar[ke];
+        vsz1[q] = vsz1[q] + (-vz[0] + vz[1] + vz[2] - vz[3]) * ar[ke];
+      }
+    }
+  }
+  if (c->nsym == 1) { /* :830-847 */
+    out[1] = rk->fsm + fs1[0] - fs1[1];
+    out[0] = rk->fsp + fs1[0] + fs1[1];
+    out[3] = fs2[0] - fs2[1];
+    out[2] = fs2[0] + fs2[1];
+    out[4] = rk->vsxp + vsx1[0] + vsx1[1];
+    out[7] = rk->vsxm + vsx1[0] - vsx1[1];
+    out[5] = rk->vsyp + vsy1[0] + vsy1[1];
+    out[8] = rk->vsym + vsy1[0] - vsy1[1];
+    out[6] = rk->vszp + vsz1[0] + vsz1[1];
+    out[9] = rk->vszm + vsz1[0] - vsz1[1];
+    out[10] = vsx2[0] + vsx2[1];
+    out[13] = vsx2[0] - vsx2[1];
+    out[11] = vsy2[0] + vsy2[1];
+    out[14] = vsy2[0] - vsy2[1];
+    out[12] = vsz2[0] + vsz2[1];
+    out[15] = vsz2[0] - vsz2[1];
+  } else { /* :848-864 */
+    out[0] = rk->fsp + fs1[0];
+    out[1] = out[0];
+    out[2] = fs2[0];
+    out[3] = out[2];
+    out[4] = rk->vsxp + vsx1[0];
+    out[7] = out[4];
+    out[5] = rk->vsyp + vsy1[0];
+    out[8] = out[5];
+    out[6] = rk->vszp + vsz1[0];
+    out[9] = out[6];
+    out[10] = vsx2[0];
+    out[13] = out[10];
+    out[11] = vsy2[0];
+    out[14] = out[11];
+    out[12] = vsz2[0];
+    out[15] = out[12];
+  }
+  return undefined;
+}
+#undef XR
+#undef XZ
+#undef TAB
+
+int ora_pair_infd(const ora_ctx *c, const ora_freq *f, int isp, int ifp, double *out) {
+  ora_rk rk;
+  ora_vav(c, 1, 0., c->xg[isp], c->yg[isp], c->zg[isp], isp, ifp, &rk);
+  return ora_vnsinfd(c, f, isp, ifp, c->xg[isp], c->yg[isp], c->zg[isp], &rk, out);
+}
+
+int ora_pair_fd(const ora_ctx *c, const ora_freq *f, int isp, int ifp, double *out) {
+  ora_rk rk;
+  ora_vav(c, 2, 2 * c->depth, c->xg[isp], c->yg[isp], c->zg[isp], isp, ifp, &rk);
+  return ora_vnsfd(c, f, isp, ifp, c->xg[isp], c->yg[isp], c->zg[isp], &rk, out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* GAUSSZ (M_SOLVER.f90:31-73): Gauss-Jordan without row interchanges (the
+ * row search at :38-41 only ever swaps a row with itself). */
+int ora_gaussz(double complex *a, int lda, int n, int m) {
+#define A_(i, j) a[((i) - 1) + (size_t)lda * ((j) - 1)]
+  const double eps = 1.E-20;
+  for (int j = 1; j <= n - 1; ++j) {
+    if (cabs(A_(j, j)) - eps <= 0.) {
+      /* the reference only checks this when the row search moved (K>J);
+       * a zero pivot is fatal either way */
+      int moved = 0;
+      for (int i = j + 1; i <= n; ++i)
+        if (cabs(A_(j, j)) - cabs(A_(i, j)) <= 0.) moved = 1;
+      if (moved) return 1;
+    }
+    for (int k = j + 1; k <= m; ++k) {
+      const double complex p = A_(j, k) / A_(j, j);
+      for (int i = j + 1; i <= n; ++i) A_(i, k) = A_(i, k) - A_(i, j) * p;
+    }
+  }
+  if (cabs(A_(n, n)) - eps <= 0.) return 1;
+  for (int il = n + 1; il <= m; ++il)
+    for (int j = n; j >= 1; --j) {
+      A_(j, il) = A_(j, il) / A_(j, j);
+      for (int i = 1; i <= j - 1; ++i) A_(i, il) = A_(i, il) - A_(i, j) * A_(j, il);
+    }
+  return 0;
+#undef A_
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solver session: SOLVE_BVP (SOLVE_BEM.f90:28-96) and the two direct
+ * drivers (SOLVE_BEM_INFD_DIRECT.f90:34-165, SOLVE_BEM_FD_DIRECT.f90:33-185)
+ * with their module state: w_previous (COM_VAR.f90:83, initialised to -1 at
+ * INITIALIZATION.f90:63), the AInv cache (:82) and NQ/QQ/CQ (:94-95). */
+struct ora_session {
+  const ora_ctx *c;
+  double w_previous;
+  int nq;               /* COM_VAR NQ: module scalar, starts at 0 */
+  double qq[251], cq[251];
+  double complex *ainv; /* [nfa * nfa * nb] */
+  int undefined;
+};
+
+ora_session *ora_session_new(const ora_ctx *c) {
+  ora_session *s = calloc(1, sizeof *s);
+  s->c = c;
+  s->w_previous = -1.;
+  s->nq = 0;
+  const int nb = 1 << c->nsym;
+  s->ainv = calloc((size_t)c->nfa * c->nfa * nb, sizeof(double complex));
+  return s;
+}
+
+void ora_session_free(ora_session *s) {
+  if (!s) return;
+  free(s->ainv);
+  free(s);
+}
+
+int ora_session_undefined_pairs(const ora_session *s) { return s->undefined; }
+
+static double ora_gm(const ora_ctx *c) { /* SOLVE_BEM_INFD_DIRECT.f90:66-76 */
+  double gm = 0.;
+  const int njj = c->nsym + 1;
+  for (int i = 0; i < c->np - 1; ++i)
+    for (int jj = 1; jj <= njj; ++jj) {
+      const double bj = (jj % 2) ? 1. : -1.;
+      for (int j = i + 1; j < c->np; ++j) {
+        const double dij = sqrt(sq(c->x[j] - c->x[i]) + sq(c->y[i] - c->y[j] * bj));
+        gm = dmax(dij, gm);
+      }
+    }
+  return gm;
+}
+
+/* Per-frequency set-up of SOLVE_BVP and the two direct drivers
+ * (SOLVE_BEM.f90:60-89, SOLVE_BEM_INFD_DIRECT.f90:47-80,
+ * SOLVE_BEM_FD_DIRECT.f90:48-100).  *nq_state is the module NQ (updated
+ * here the way the reference updates it); qq/cq receive the module QQ/CQ. */
+int ora_freq_setup(const ora_ctx *c, double period, int *nq_state, ora_freq *f, int *is_fd) {
+  const double pi = 4. * atan(1.);
+  const double w = 2. * pi / period;
+  const double kwave = w * w / c->g;
+  memset(f, 0, sizeof *f);
+  f->t = period;
+  if (c->depth == 0. || kwave * c->depth >= 20) {
+    *is_fd = 0;
+    if (fabs(w) < 1.E-4) return 10;
+    /* QQ/CQ are zeroed, CINT_INFD fills the first N entries, and `N=NQ`
+     * (SOLVE_BEM_INFD_DIRECT.f90:79) leaves the module NQ untouched. */
+    const double am0 = sq(w) / c->g;
+    int n;
+    ora_cint(am0 * ora_gm(c), &n, f->qq, f->cq);
+  } else {
+    *is_fd = 1;
+    const double akh0 = w * w * c->depth / c->g;
+    f->amh = ora_x0(akh0);
+    if (isnan(f->amh)) return 11;
+    if (fabs(w) < 1.E-4) return 10;
+    const double akh = f->amh * tanh(f->amh);
+    const double am0 = f->amh / c->depth;
+    /* QQ/CQ/AMBDA/AR are zeroed; CINT_FD writes local arrays only
+     * (COMPUTE_GREEN_FD.f90:874), NQ=N; LISC fills AMBDA/AR. */
+    double qloc[251], cloc[251];
+    int n;
+    ora_cint(am0 * ora_gm(c), &n, qloc, cloc);
+    *nq_state = n;
+    const int le = ora_lisc(akh, f->amh, &f->nexp, f->ambda, f->ar);
+    if (le) return 20 + le;
+  }
+  f->nq = *nq_state;
+  return 0;
+}
+
+int ora_pairs_all(const ora_ctx *c, const ora_freq *f, int fd, double *out) {
+  int undefined = 0;
+  for (int i = 0; i < c->nfa; ++i)
+    for (int j = 0; j < c->nfa; ++j) {
+      double *o = out + ((size_t)i * c->nfa + j) * ORA_PAIR_N;
+      undefined += fd ? ora_pair_fd(c, f, i, j, o) : ora_pair_infd(c, f, i, j, o);
+    }
+  return undefined;
+}
+
+int ora_solve_bvp(ora_session *s, double period, const double complex *nvel,
+                  double complex *pressure, double complex *zigb_o, double complex *zigs_o,
+                  double complex *zpb_o, double complex *zps_o) {
+  const ora_ctx *c = s->c;
+  const int nfa = c->nfa, nb = 1 << c->nsym, njj = c->nsym + 1;
+  const double pi = 4. * atan(1.);
+  const double w = 2. * pi / period;
+  int fd;
+  ora_freq f;
+  const int se = ora_freq_setup(c, period, &s->nq, &f, &fd);
+  if (se) return se;
+
+  double *pair = malloc(sizeof(double) * ORA_PAIR_N);
+  if (w != s->w_previous) { /* :82-120 */
+    s->w_previous = w;
+    double complex *zij = calloc((size_t)nfa * 2 * nfa, sizeof(double complex));
+    for (int isym = 1; isym <= njj; ++isym) {
+      for (int isp = 0; isp < nfa; ++isp)
+        for (int ifp = 0; ifp < nfa; ++ifp) {
+          s->undefined += fd ? ora_pair_fd(c, &f, isp, ifp, pair) : ora_pair_infd(c, &f, isp, ifp, pair);
+          double pcos, psin;
+          if (isym == 1) {
+            pcos = pair[4] * c->xn[isp] + pair[5] * c->yn[isp] + pair[6] * c->zn[isp];
+            psin = pair[10] * c->xn[isp] + pair[11] * c->yn[isp] + pair[12] * c->zn[isp];
+          } else {
+            pcos = pair[7] * c->xn[isp] + pair[8] * c->yn[isp] + pair[9] * c->zn[isp];
+            psin = pair[13] * c->xn[isp] + pair[14] * c->yn[isp] + pair[15] * c->zn[isp];
+          }
+          zij[isp + (size_t)nfa * ifp] = pcos + I * psin;
+        }
+      for (int i = 0; i < nfa; ++i) {
+        for (int j = 0; j < nfa; ++j) zij[i + (size_t)nfa * (j + nfa)] = 0.;
+        zij[i + (size_t)nfa * (i + nfa)] = 1.;
+      }
+      if (ora_gaussz(zij, nfa, nfa, 2 * nfa)) {
+        free(zij);
+        free(pair);
+        s->w_previous = -1.;
+        return 30;
+      }
+      for (int i = 0; i < nfa; ++i)
+        for (int j = 0; j < nfa; ++j)
+          s->ainv[i + (size_t)nfa * j + (size_t)nfa * nfa * (isym - 1)] =
+              zij[i + (size_t)nfa * (nfa + j)];
+    }
+    free(zij);
+  }
+  /* :123-151 */
+  double complex *zol = calloc((size_t)nfa * 2, sizeof(double complex));
+  double complex *b = calloc((size_t)nfa, sizeof(double complex));
+  double complex *zigb = calloc((size_t)nfa, sizeof(double complex));
+  double complex *zigs = calloc((size_t)nfa, sizeof(double complex));
+  double complex *zpb = calloc((size_t)nfa, sizeof(double complex));
+  double complex *zps = calloc((size_t)nfa, sizeof(double complex));
+  for (int isym = 1; isym <= njj; ++isym) {
+    const int bx = (isym % 2) ? 1 : -1;
+    for (int i = 0; i < nfa; ++i) {
+      if (c->nsym == 1) b[i] = (nvel[i] + bx * nvel[i + nfa]) * 0.5;
+      else b[i] = nvel[i];
+    }
+    for (int i = 0; i < nfa; ++i) {
+      double complex acc = 0.;
+      for (int k = 0; k < nfa; ++k)
+        acc = acc + s->ainv[i + (size_t)nfa * k + (size_t)nfa * nfa * (isym - 1)] * b[k];
+      zol[i + (size_t)nfa * (isym - 1)] = acc;
+    }
+  }
+  for (int i = 0; i < nfa; ++i) {
+    if (c->nsym == 0) {
+      zigb[i] = zol[i];
+      zigs[i] = 0.;
+    } else {
+      zigb[i] = zol[i] + zol[i + nfa];
+      zigs[i] = zol[i] - zol[i + nfa];
+    }
+  }
+  /* phi = S sigma by re-evaluating every pair :154-163 */
+  for (int i = 0; i < nfa; ++i)
+    for (int j = 0; j < nfa; ++j) {
+      if (fd) ora_pair_fd(c, &f, i, j, pair);
+      else ora_pair_infd(c, &f, i, j, pair);
+      const double sp1 = pair[0], sm1 = pair[1], sp2 = pair[2], sm2 = pair[3];
+      zpb[i] = zpb[i] + 0.5 * (zigb[j] * ((sp1 + sm1) + I * (sp2 + sm2)) +
+                               zigs[j] * ((sp1 - sm1) + I * (sp2 - sm2)));
+      zps[i] = zps[i] + 0.5 * (zigs[j] * ((sp1 + sm1) + I * (sp2 + sm2)) +
+                               zigb[j] * ((sp1 - sm1) + I * (sp2 - sm2)));
+    }
+  /* pressure SOLVE_BEM.f90:91-96 */
+  const double complex ii = I;
+  for (int i = 0; i < nfa; ++i) {
+    pressure[i] = c->rho * ii * w * zpb[i];
+    if (c->nsym == 1) pressure[i + nfa] = c->rho * ii * w * zps[i];
+  }
+  (void)nb;
+  if (zigb_o) memcpy(zigb_o, zigb, sizeof(double complex) * nfa);
+  if (zigs_o) memcpy(zigs_o, zigs, sizeof(double complex) * nfa);
+  if (zpb_o) memcpy(zpb_o, zpb, sizeof(double complex) * nfa);
+  if (zps_o) memcpy(zps_o, zps, sizeof(double complex) * nfa);
+  free(zol);
+  free(b);
+  free(zigb);
+  free(zigs);
+  free(zpb);
+  free(zps);
+  free(pair);
+  return 0;
+}
+
+/* Force integration, NEMOH.f90:137-141: F(i) = F(i) - PRESSURE(c)*NDS(i,c)
+ * summed over c in order; nds is (nint, ntot) row-major here. */
+void ora_forces(int nint, int ntot, const double *nds, const double complex *pressure,
+                double complex *force) {
+  for (int i = 0; i < nint; ++i) {
+    double complex acc = 0.;
+    for (int c = 0; c < ntot; ++c) acc = acc - pressure[c] * nds[(size_t)i * ntot + c];
+    force[i] = acc;
+  }
+}
+
+/* vectorised GG for the tests: out[k] = GG(z[k], cex[k]) */
+void ora_gg_vec(int n, const double complex *z, const double complex *cex, double complex *out) {
+  for (int k = 0; k < n; ++k) out[k] = ora_gg(z[k], cex[k]);
+}
+
+/* test entry points for the Prony-fit pieces */
+double ora_ff_pub(double x, double ak, double am) { return ora_ff(x, ak, am); }
+int ora_housrs_pub(double *s, int nl, int ncc) { return ora_housrs(s, nl, ncc, 1, 1.E-20); }
+int ora_sprbm_pub(double *c1, int ic, double *rr1, double *rc1) {
+  double pol[33] = {0};
+  int ir;
+  return ora_sprbm(c1 - 1, ic, rr1 - 1, rc1 - 1, pol, &ir);
+}
+
+/* test entry point: the Rankine part alone for pair (isp, ifp), field point
+ * at panel isp's collocation point; out = FSP FSM VSXP VSYP VSZP VSXM VSYM VSZM */
+void ora_rankine_pub(const ora_ctx *c, int kkk, double dh, int isp, int ifp, double *out) {
+  ora_rk rk;
+  ora_vav(c, kkk, dh, c->xg[isp], c->yg[isp], c->zg[isp], isp, ifp, &rk);
+  out[0] = rk.fsp; out[1] = rk.fsm; out[2] = rk.vsxp; out[3] = rk.vsyp;
+  out[4] = rk.vszp; out[5] = rk.vsxm; out[6] = rk.vsym; out[7] = rk.vszm;
+}

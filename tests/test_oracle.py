@@ -1,0 +1,289 @@
+"""The oracle (C restatement of the reference hot path) pinned against the
+FP64 reference build (golden fixtures) and against closed forms.
+
+Fixtures come from tests/golden/make_golden.py; no reference code runs here.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+from conftest import case_geometry, load_golden
+from nemoh_amd import io as nio
+from oracle import pyoracle as po
+
+
+# -- GG (ELEMENTARY_FNS.f90:31-81) ------------------------------------------
+def test_gg_matches_reference_samples():
+    d = load_golden("fn_gg.npz")
+    got = po.gg(d["z"])
+    assert np.max(np.abs(got - d["gg"]) / np.abs(d["gg"])) < 1e-14
+
+
+def test_gg_is_exp_e1_approximation():
+    """GG(z) ~ e^z E1(z) + sgn(Im z) i pi e^z; the reference's rational and
+    series fits are good to ~3e-4 relative (worst in the rational regime)."""
+    d = load_golden("fn_gg.npz")
+    z = d["z"]
+    exact = np.exp(z) * special.exp1(z) + np.where(z.imag >= 0, 1, -1) * 1j * np.pi * np.exp(z)
+    rel = np.abs(po.gg(z) - exact) / np.abs(exact)
+    assert rel.max() < 3e-4
+    assert np.median(rel) < 5e-6
+
+
+# -- CREK tables (NEMOH.f90:179-274) ----------------------------------------
+def test_crek_matches_reference(ref_tables):
+    xr, xz, tabs = po.crek()
+    rxr, rxz, rtabs = ref_tables
+    assert np.array_equal(xr, rxr) and np.array_equal(xz, rxz)
+    for a, b in zip(tabs, rtabs):
+        assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b))
+
+
+def test_crek_exponential_tables_are_bessel_integrals():
+    """APD2Z = int cos(akr cos t) e^akz dt = pi e^akz J0(akr) and
+    APD2X = pi e^akz J1(akr): pins the Simpson rule, grids and (i, j) order."""
+    xr, xz, (a1x, a1z, a2x, a2z) = po.crek()
+    R, Z = np.meshgrid(xr, xz, indexing="ij")
+    m = R < 10
+    assert np.allclose(a2z[m], (np.pi * np.exp(Z) * special.j0(R))[m], rtol=0, atol=1e-9)
+    assert np.allclose(a2x[m], (np.pi * np.exp(Z) * special.j1(R))[m], rtol=0, atol=1e-9)
+
+
+def test_crek_gg_table_against_exact_integrand():
+    """APD1Z(i,j) = S[Re(GG(z) - 1/z)] and APD1X = S[cos t Im(GG(z) - 1/z)],
+    z = akz + i akr cos t, S = composite Simpson on 250 intervals of
+    [-pi/2, pi/2]; evaluated here with the exact e^z E1(z) in place of GG, so
+    it agrees to GG's approximation error."""
+    xr, xz, (a1x, a1z, a2x, a2z) = po.crek()
+    t = np.linspace(-np.pi / 2, np.pi / 2, 251)
+    w = np.full(251, 2.0)
+    w[1::2] = 4.0
+    w[0] = w[-1] = 1.0
+    w *= (np.pi / 250) / 3
+    for i, j in [(60, 20), (40, 30), (100, 10), (35, 40), (80, 45), (5, 5), (200, 44)]:
+        z = xz[j] + 1j * xr[i] * np.cos(t)
+        g = np.exp(z) * special.exp1(z) + np.where(z.imag >= 0, 1, -1) * 1j * np.pi * np.exp(z)
+        c = g - 1.0 / z
+        ez, ex = (w * c.real).sum(), (w * np.cos(t) * c.imag).sum()
+        assert abs(a1z[i, j] - ez) < 1e-3 * max(1.0, abs(ez)), (i, j)
+        assert abs(a1x[i, j] - ex) < 1e-3 * max(1.0, abs(ex)), (i, j)
+
+
+# -- CINT (COMPUTE_GREEN_INFD.f90:505-638) ----------------------------------
+def test_cint_matches_reference_rules():
+    d = load_golden("fn_cint.npz")
+    for k, akk in enumerate(d["akk"]):
+        n, qq, cq = po.cint(akk)
+        assert n == d["nq"][k]
+        assert np.array_equal(qq, d["qq"][k, :n]) and np.array_equal(cq, d["cq"][k, :n])
+    # the module NQ is never written by CINT_INFD (SOLVE_BEM_INFD_DIRECT.f90:79)
+    assert np.all(d["nq_module"] == 0)
+
+
+@pytest.mark.parametrize("akk,n", [(0.1, 8), (1.0, 12), (3.0, 16), (6.0, 24), (20.0, 32)])
+def test_cint_gauss_rules_are_gauss_legendre(akk, n):
+    nq, qq, cq = po.cint(akk)
+    assert nq == n
+    x, w = np.polynomial.legendre.leggauss(n)
+    order = np.argsort(qq)
+    assert np.allclose(qq[order] / np.pi, x / 2, atol=1e-7)
+    assert np.allclose(cq[order] / np.pi, w / 2, atol=1e-7)
+
+
+def test_cint_simpson_rules():
+    for akk, n in ((30.0, 51), (41.0, 101)):
+        nq, qq, cq = po.cint(akk)
+        assert nq == n
+        assert np.isclose(cq.sum(), np.pi, rtol=1e-14)
+        f = np.cos(qq) ** 2
+        assert abs((cq * f).sum() - np.pi / 2) < 1e-6
+
+
+# -- X0 (COMPUTE_GREEN_FD.f90:1635-1680) ------------------------------------
+def test_x0_matches_reference_and_root():
+    d = load_golden("fn_x0.npz")
+    for ak, ref in zip(d["ak"], d["x0"]):
+        x = po.x0(ak)
+        assert x == ref
+        exact = __import__("scipy.optimize", fromlist=["brentq"]).brentq(
+            lambda v: v * np.tanh(v) - ak, 1e-12, 50.0, xtol=1e-15)
+        # the bisection treats |val1*val2| <= 1e-12 as a sign change
+        # (COMPUTE_GREEN_FD.f90:1666), which biases the root: ~2e-3 relative
+        # at AK = 1e-3, ~1e-5 near AK = 1; above AK ~ 11, where tanh(x) is 1
+        # to 1e-9, the clamp can flip the bracket and the reference root is
+        # off by up to ~6% (kept bit-for-bit above, not checked here)
+        if ak <= 10:
+            assert abs(x - exact) <= (2e-3 if ak < 0.1 else 2e-4) * exact
+
+
+# -- LISC Prony fit (COMPUTE_GREEN_FD.f90:1007-1632) ------------------------
+def test_lisc_matches_reference_bitwise():
+    d = load_golden("fn_lisc.npz")
+    for k in range(d["akh"].size):
+        err, n, lam, ar = po.lisc(d["akh"][k], d["amh"][k])
+        assert err == 0 and n == d["nexp"][k]
+        assert np.array_equal(lam, d["ambda"][k, :n])
+        assert np.array_equal(ar, d["ar"][k, :n])
+
+
+def test_lisc_fit_reaches_reference_precision():
+    """The fitted exponential sum matches the kernel FF on [0, 20] to the
+    reference's stopping criterion (1e-2) whenever fewer than 30 terms
+    sufficed."""
+    L = po.lib()
+    L.ora_ff_pub.restype = C.c_double
+    L.ora_ff_pub.argtypes = [C.c_double] * 3
+    for h, w in ((20.0, 0.5), (50.0, 0.8), (10.0, 1.2)):
+        amh = po.x0(w * w * h / 9.81)
+        akh = amh * np.tanh(amh)
+        err, n, lam, ar = po.lisc(akh, amh)
+        xs = np.linspace(0, 20, 241)
+        fit = np.array([np.sum(ar * np.exp(lam * x)) for x in xs])
+        ff = np.array([L.ora_ff_pub(x, akh, amh) for x in xs])
+        assert np.max(np.abs(fit - ff)) < 2e-2  # 1e-2 at the check points, 241 points here
+        assert np.all(lam < 0)
+
+
+# -- GOMG geometry (PREPARE_MESH.f90:80-180) --------------------------------
+@pytest.mark.parametrize("name", ["pairs_lidcyl.npz", "pairs_barge.npz"])
+def test_prepare_mesh_matches_reference(name):
+    d = load_golden(name)
+    geo = po.prepare_mesh(d["vertices"], d["panels"], int(d["isym"]))
+    ref = case_geometry(d)
+    for k in ref:
+        assert np.array_equal(np.asarray(geo[k]), np.asarray(ref[k])), k
+
+
+def test_prepare_mesh_rectangle_closed_form():
+    v = np.array([[0, 0, -1], [2, 0, -1], [2, 1, -1], [0, 1, -1]], float)
+    geo = po.prepare_mesh(v, np.array([[0, 1, 2, 3]], np.int32), 0)
+    assert np.isclose(geo["aire"][0], 2.0, rtol=1e-14)
+    assert np.allclose([geo["xn"][0], geo["yn"][0], geo["zn"][0]], [0, 0, 1])
+    assert np.allclose([geo["xg"][0], geo["yg"][0], geo["zg"][0]], [1, 0.5, -1])
+    assert np.isclose(geo["tdis"][0], np.sqrt(1.25))
+    assert np.isclose(geo["zer"], -2e-5)
+
+
+# -- Green function pairs (VAV*/VNS*) ---------------------------------------
+def _pairs_case(d, ref_tables, depth):
+    return po.Case(d["vertices"], d["panels"], int(d["isym"]), depth=depth, rho=1025.0, g=9.81,
+                   tables=ref_tables, geometry=case_geometry(d))
+
+
+def _close(out, ref, tol):
+    scale = np.max(np.abs(ref), axis=(0, 1))
+    return np.max(np.abs(out - ref) / np.where(scale > 0, scale, 1.0))
+
+
+@pytest.mark.parametrize("name", ["pairs_lidcyl.npz", "pairs_barge.npz"])
+def test_pairs_match_reference(name, ref_tables):
+    d = load_golden(name)
+    fd_case = _pairs_case(d, ref_tables, float(d["depth_fd"]))
+    out, und = fd_case.pairs(float(d["period_fd"]))
+    assert und == 0
+    assert _close(out, d["fd"], 0) < 1e-12
+    f, is_fd, nq = fd_case.freq(float(d["period_fd"]))
+    assert is_fd and nq == int(d["fd_nq"]) and f.nexp == d["fd_ambda"].size
+    inf_case = _pairs_case(d, ref_tables, 0.0)
+    for key, pk in (("infd1", "period_i1"), ("infd2", "period_i2")):
+        if key in d:
+            out, und = inf_case.pairs(float(d[pk]), nq_state=nq)
+            assert und == 0
+            assert _close(out, d[key], 0) < 1e-12, key
+
+
+def _single_panel_ctx(field):
+    """Panel 0: unit square at z=-3 (normal +z); panel 1: a tiny panel whose
+    collocation point is `field`."""
+    s = 0.01
+    fx, fy, fz = field
+    v = np.array([[0, 0, -3], [1, 0, -3], [1, 1, -3], [0, 1, -3],
+                  [fx - s, fy - s, fz], [fx + s, fy - s, fz], [fx + s, fy + s, fz], [fx - s, fy + s, fz]], float)
+    p = np.array([[0, 1, 2, 3], [4, 5, 6, 7]], np.int32)
+    return po.Case(v, p, 0, depth=0.0)
+
+
+@pytest.mark.parametrize("field", [(0.3, 0.4, -2.7), (0.5, 0.5, -2.5), (1.4, 0.2, -3.2),
+                                   (6.0, 5.0, -1.0), (0.2, 0.9, -0.4)])
+def test_rankine_against_quadrature(field):
+    """FSP = -(I - I')/4pi with I the integral of 1/r over the panel and I'
+    over its z-mirror (infinite depth, MK = +1); VS = the field-point
+    gradient.  Each image is integrated exactly (adaptive quadrature) when
+    within 7 TDIS of the field point and by the one-point rule AIRE/RO
+    beyond, which is the reference's discretisation (VAVINFD.f90:92-97)."""
+    c = _single_panel_ctx(field)
+    L = po.lib()
+    L.ora_rankine_pub.argtypes = [C.POINTER(po._Ctx), C.c_int, C.c_double, C.c_int, C.c_int, po._dp]
+    out = np.zeros(8)
+    L.ora_rankine_pub(C.byref(c.ctx), 1, 0.0, 1, 0, out)
+    P = np.array([c.geo["xg"][1], c.geo["yg"][1], c.geo["zg"][1]])
+    tdis, aire = c.geo["tdis"][0], c.geo["aire"][0]
+
+    def image(zsign):
+        cen = np.array([0.5, 0.5, zsign * -3.0])
+        ro = np.linalg.norm(P - cen)
+        if ro > 7 * tdis:
+            return aire / ro, -(P - cen) * aire / ro ** 3
+
+        def integ(fun):
+            return integrate.dblquad(lambda y, x: fun(P - np.array([x, y, zsign * -3.0])), 0, 1, 0, 1,
+                                     epsabs=1e-13, epsrel=1e-12)[0]
+        pot = integ(lambda r: 1 / np.linalg.norm(r))
+        grad = np.array([integ(lambda r, k=k: -r[k] / np.linalg.norm(r) ** 3) for k in range(3)])
+        return pot, grad
+    p1, g1 = image(1)
+    p2, g2 = image(-1)
+    fsp, vs = -(p1 - p2) / (4 * np.pi), -(g1 - g2) / (4 * np.pi)
+    assert abs(out[0] - fsp) < 1e-9 * abs(fsp)
+    assert np.max(np.abs(out[2:5] - vs)) < 1e-9 * np.max(np.abs(vs))
+
+
+def test_rankine_self_term_is_half():
+    """n . grad of -(1/4pi) int 1/r at the panel's own collocation point is
+    +1/2 (the -2 pi n jump of VAVINFD.f90:175-178), images aside."""
+    v = np.array([[0, 0, -50], [1, 0, -50], [1, 1, -50], [0, 1, -50]], float)
+    c = po.Case(v, np.array([[0, 1, 2, 3]], np.int32), 0, depth=0.0)
+    L = po.lib()
+    out = np.zeros(8)
+    L.ora_rankine_pub(C.byref(c.ctx), 1, 0.0, 0, 0, out)
+    n = np.array([c.geo["xn"][0], c.geo["yn"][0], c.geo["zn"][0]])
+    assert abs(out[2:5] @ n - 0.5) < 1e-4  # mirror image at 100 m is ~1e-5
+
+
+# -- GAUSSZ (M_SOLVER.f90:31-73) --------------------------------------------
+def test_gaussz_inverse():
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((40, 40)) + 1j * rng.standard_normal((40, 40)) + 8 * np.eye(40)
+    inv = po.gaussz_inverse(a)
+    assert np.max(np.abs(inv - np.linalg.inv(a))) < 1e-13
+
+
+def test_gaussz_zero_pivot_stops():
+    a = np.array([[0.0, 1.0], [1.0, 0.0]], complex)
+    with pytest.raises(ZeroDivisionError):
+        po.gaussz_inverse(a)
+
+
+# -- end to end: Solver Forces.dat ------------------------------------------
+E2E = [("Cylinder", None), ("barge_fd", None), ("Postprocessing", None),
+       ("NonSymmetrical", 14), ("cyl512", 7), ("2Bodies", 13)]
+
+
+@pytest.mark.parametrize("name,nprob", E2E)
+def test_session_reproduces_reference_forces(name, nprob, ref_tables):
+    d = load_golden(f"case_{name}.npz")
+    c = po.Case(d["vertices"], d["panels"], int(d["isym"]), depth=float(d["depth"]),
+                rho=float(d["rho"]), g=float(d["g"]), tables=ref_tables)
+    s = c.session()
+    om, kind = d["omega"], d["kind"]
+    n = om.size if nprob is None else nprob
+    F = np.zeros((d["nds"].shape[0], n), complex)
+    for j in range(n):
+        p, *_ = s.solve_bvp(2 * np.pi / om[j], d["nvel"][:, j])
+        F[:, j] = po.forces(d["nds"], p)
+    ref = nio.table_to_forces(d["forces_table"][:, :2 * n], om[:n], kind[:n])
+    err = np.abs(F - ref).max(axis=0) / np.abs(ref).max(axis=0)
+    assert s.undefined_pairs == 0
+    assert err.max() < 1e-12
