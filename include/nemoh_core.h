@@ -1,0 +1,144 @@
+/*
+ * nemoh_core -- MI355X solver core for the NEMOH Solver stage (C ABI).
+ *
+ * Replaces influence-matrix assembly and the dense complex solve of the
+ * reference Solver (Solver/Core/SOLVE_BEM_INFD_DIRECT.f90,
+ * Solver/Core/SOLVE_BEM_FD_DIRECT.f90 and the Green-function / solver
+ * modules they call).  The Fortran host binds these with ISO_C_BINDING:
+ * every argument is a plain pointer or scalar, arrays are owned by the
+ * caller (column-major where 2-D), complex values are interleaved (re, im)
+ * doubles as in COMPLEX(8).  The core copies inputs to the device and
+ * writes outputs into the caller's buffers before returning; device copies
+ * of geometry, tables, influence matrices and factors live in the handle.
+ *
+ * Every call returns NEMOH_OK (0) or an error code; nemoh_core_last_error()
+ * gives the message.  One handle must not be used from two threads at once.
+ */
+#ifndef NEMOH_CORE_H
+#define NEMOH_CORE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nemoh_core nemoh_core;
+
+enum {
+  NEMOH_OK = 0,
+  NEMOH_ERR_ARG = 1,         /* invalid argument                               */
+  NEMOH_ERR_HIP = 2,         /* HIP runtime failure (no device, out of memory) */
+  NEMOH_ERR_PIVOT = 3,       /* pivot below 1e-20 (M_SOLVER.f90:49-52,61-64)   */
+  NEMOH_ERR_OMEGA = 4,       /* |w| < 1e-4 (COMPUTE_GREEN_INFD.f90:252-255)    */
+  NEMOH_ERR_FIT = 5,         /* Prony fit failure (COMPUTE_GREEN_FD.f90:1073, 1311-1314) */
+  NEMOH_ERR_PANEL = 6,       /* degenerate panel (PREPARE_MESH.f90:112-120)    */
+  NEMOH_ERR_CONVERGENCE = 7, /* mixed precision refinement did not converge    */
+  NEMOH_ERR_WAVENUMBER = 8   /* X0 root search failed (COMPUTE_GREEN_FD.f90:1656-1660) */
+};
+
+enum { NEMOH_FP64 = 0, NEMOH_MIXED = 1 };
+
+/* Creates a handle holding the mesh, the panel geometry and the wave
+ * tables on the device.  Replaces the COM_VAR module state set up by
+ * INITIALIZE (Solver/Core/INITIALIZATION.f90:28-69, COM_VAR.f90:29-95):
+ * x,y,z = X,Y,Z(NP); m1..m4 = M1..M4(NFA), 1-based as in Fortran;
+ * xg..tdis = XG,YG,ZG,XN,YN,ZN,AIRE,TDIS(NFA) and zer = ZER from GOMG
+ * (PREPARE_MESH.f90:30-180); xr(328), xz(46), apd*(328,46) from CREK
+ * (Solver/NEMOH.f90:179-208); nsym = NSYMY. */
+int nemoh_core_create(nemoh_core **ctx, int np, int nfa, int nsym, const double *x, const double *y,
+                      const double *z, const int *m1, const int *m2, const int *m3, const int *m4,
+                      const double *xg, const double *yg, const double *zg, const double *xn,
+                      const double *yn, const double *zn, const double *aire, const double *tdis,
+                      double zer, double g, double rho, double depth, const double *xr, const double *xz,
+                      const double *apd1x, const double *apd1z, const double *apd2x,
+                      const double *apd2z);
+
+/* One boundary-value problem: the exact drop-in for the calls
+ * SOLVE_POTENTIAL_INFD_DIRECT(NVEL) (Solver/Core/SOLVE_BEM.f90:71) and
+ * SOLVE_POTENTIAL_FD_DIRECT(NVEL,AMH,NEXP) (SOLVE_BEM.f90:83).
+ * period = T; the regime (infinite vs finite depth) is chosen as at
+ * SOLVE_BEM.f90:66.  Finite depth reads amh = AMH (X0 of w^2 h/g), nexp,
+ * ambda(nexp), ar(nexp) = NEXP, AMBDA, AR from LISC; infinite depth reads
+ * nq, qq(nq), cq(nq) = the module NQ, QQ, CQ at call time.  nvel =
+ * NVEL(NFA*2**NSYMY).  Outputs ZIGB, ZIGS, ZPB, ZPS (NFA each).  The
+ * factorisation is kept on the device while the period is unchanged,
+ * mirroring w_previous (SOLVE_BEM_INFD_DIRECT.f90:82). */
+int nemoh_core_solve(nemoh_core *ctx, double period, double amh, int nexp, const double *ambda,
+                     const double *ar, int nq, const double *qq, const double *cq, const double *nvel,
+                     double *zigb, double *zigs, double *zpb, double *zps);
+
+/* Every problem of one frequency at once: nvel(NFA*2**NSYMY, nrhs) ->
+ * pressure(NFA*2**NSYMY, nrhs) = rho i w phi (SOLVE_BEM.f90:91-96), the
+ * argument SOLVE_BVP returns (SOLVE_BEM.f90:28).  When nint > 0 and nds
+ * (nint, NFA*2**NSYMY) is given, forces(nint, nrhs) = -NDS . pressure
+ * (Solver/NEMOH.f90:137-141) is returned as well (either output may be
+ * NULL).  Same frequency parameters as nemoh_core_solve. */
+int nemoh_core_solve_multi(nemoh_core *ctx, double period, double amh, int nexp, const double *ambda,
+                           const double *ar, int nq, const double *qq, const double *cq, int nrhs,
+                           const double *nvel, double *pressure, int nint, const double *nds,
+                           double *forces);
+
+/* Standalone mode: the frequency set-up of SOLVE_BVP and the direct
+ * drivers done by the core itself (X0, theta rule, LISC), with the
+ * reference's module NQ carried across calls in the handle (infinite-depth
+ * calls reuse the NQ left by the last finite-depth call,
+ * SOLVE_BEM_INFD_DIRECT.f90:79).  Then as nemoh_core_solve_multi. */
+int nemoh_core_solve_frequency(nemoh_core *ctx, double period, int nrhs, const double *nvel,
+                               double *pressure, int nint, const double *nds, double *forces);
+
+/* Frequency parameters as the standalone mode computes them. */
+typedef struct {
+  int finite_depth;
+  double amh;
+  int nexp;
+  double ambda[31], ar[31];
+  int nq;
+  double qq[101], cq[101];
+} nemoh_freq;
+int nemoh_core_freq_setup(nemoh_core *ctx, double period, nemoh_freq *out);
+
+/* Precision of the solve: NEMOH_FP64 (complex128 LU) or NEMOH_MIXED
+ * (complex64 LU on MFMA refined in FP64 until max|r| <= tol * max|b|). */
+int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter);
+
+/* Timings (ms) of the last solve's phases and refinement statistics. */
+typedef struct {
+  double ms_assembly, ms_factor, ms_solve, ms_total;
+  int refine_iterations;
+  double refine_residual;
+  int cache_hit;
+} nemoh_stats;
+int nemoh_core_get_stats(const nemoh_core *ctx, nemoh_stats *out);
+int nemoh_core_enable_timing(nemoh_core *ctx, int on);
+
+const char *nemoh_core_last_error(const nemoh_core *ctx);
+void nemoh_core_destroy(nemoh_core *ctx);
+
+/* Host-side set-up routines of the reference, for callers without the
+ * Fortran host.  prepare_mesh = PRE_PROC_MESH/GOMG (PREPARE_MESH.f90:30-180,
+ * 1-based m1..m4); build_tables = CREK (Solver/NEMOH.f90:179-274) on the
+ * GPU; wavenumber = X0 (COMPUTE_GREEN_FD.f90:1635); theta_rule = CINT_INFD
+ * (COMPUTE_GREEN_INFD.f90:505), returns NQ; prony = LISC
+ * (COMPUTE_GREEN_FD.f90:1007). */
+int nemoh_core_prepare_mesh(int np, int nfa, int nsym, const double *x, const double *y, const double *z,
+                            const int *m1, const int *m2, const int *m3, const int *m4, double *xg,
+                            double *yg, double *zg, double *xn, double *yn, double *zn, double *aire,
+                            double *tdis, double *zer);
+int nemoh_core_build_tables(double *xr, double *xz, double *apd1x, double *apd1z, double *apd2x,
+                            double *apd2z);
+double nemoh_core_wavenumber(double ak);
+int nemoh_core_theta_rule(double akk, double *qq, double *cq);
+int nemoh_core_prony(double akh, double amh, int *nexp, double *ambda, double *ar);
+
+/* Diagnostics used by the tests: the 16 Green-function values of every
+ * (ISP, IFP) pair (the VAV*+VNS* outputs SP1 SM1 SP2 SM2 VSXP1 VSYP1 VSZP1
+ * VSXM1 VSYM1 VSZM1 VSXP2 VSYP2 VSZP2 VSXM2 VSYM2 VSZM2, COM_VAR.f90:85-90)
+ * into out[(isp*nfa + ifp)*16 + k]; and a check of the MFMA lane layout
+ * (out: 16x16 product of fixed 16x4 and 4x16 operands, row-major). */
+int nemoh_core_pairs(nemoh_core *ctx, double period, double amh, int nexp, const double *ambda,
+                     const double *ar, int nq, const double *qq, const double *cq, double *out);
+int nemoh_core_mfma_probe(int fp32, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,730 @@
+// C-ABI of the solver core: handle lifetime, per-frequency pipeline and the
+// mapping of the reference's Solver module state onto device-resident data.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nemoh_core.h"
+#include "host_math.h"
+#include "kernels.h"
+
+using namespace nemoh;
+
+struct nemoh_core {
+  int np = 0, nfa = 0, nsym = 0, nb = 1;
+  double zer = 0, g = 9.81, rho = 1025, depth = 0;
+  double gm = 0;  // max horizontal vertex distance (theta-rule argument)
+  std::vector<double> hx, hy;
+  hipStream_t st = nullptr;
+  // resident geometry and tables
+  double* d_geo = nullptr;  // xg yg zg xn yn zn aire tdis, n each
+  double* d_vtx = nullptr;  // vx vy vz, 4n each
+  double* d_xr = nullptr;
+  double* d_xz = nullptr;
+  double* d_tab = nullptr;
+  Geometry G{};
+  Tables T{};
+  FreqParams* d_par = nullptr;
+  FreqParams h_par{};
+  // influence matrices of the current frequency (per symmetry block)
+  double2* D[2] = {nullptr, nullptr};
+  double2* S[2] = {nullptr, nullptr};
+  float2* D32[2] = {nullptr, nullptr};
+  double w_prev = -1.0;
+  int factored_prec = -1;
+  int nq_state = 0;  // module NQ of the standalone session
+  // precision
+  int prec = NEMOH_FP64;
+  double tol = 1e-10;
+  int max_iter = 12;
+  // right-hand-side workspace (capacity in right-hand sides)
+  int cap = 0;
+  double2 *V = nullptr, *B = nullptr, *W = nullptr, *Z[2] = {nullptr, nullptr}, *F[2] = {nullptr, nullptr};
+  double2 *R = nullptr, *sig = nullptr, *pot = nullptr, *pres = nullptr;
+  float2 *R32 = nullptr, *W32 = nullptr;
+  double* d_nds = nullptr;
+  size_t nds_cap = 0;
+  double2* d_force = nullptr;
+  int force_cap = 0;
+  double* d_red = nullptr;  // reduction scratch
+  int* d_info = nullptr;
+  // statistics
+  bool timing = false;
+  nemoh_stats stats{};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::string err;
+};
+
+namespace {
+
+int fail(nemoh_core* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail((c), NEMOH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(nemoh_core* c, T** p, size_t count) {
+  if (*p) return NEMOH_OK;
+  HIPCHK(c, hipMalloc((void**)p, count * sizeof(T)));
+  return NEMOH_OK;
+}
+
+int ensure_matrices(nemoh_core* c) {
+  const size_t n2 = (size_t)c->nfa * c->nfa;
+  for (int b = 0; b < c->nb; ++b) {
+    if (int r = dalloc(c, &c->D[b], n2)) return r;
+    if (int r = dalloc(c, &c->S[b], n2)) return r;
+    if (c->prec == NEMOH_MIXED)
+      if (int r = dalloc(c, &c->D32[b], n2)) return r;
+  }
+  return NEMOH_OK;
+}
+
+int ensure_rhs(nemoh_core* c, int nrhs) {
+  if (nrhs <= c->cap) return NEMOH_OK;
+  auto release = [](auto*& p) {
+    if (p) hipFree(p);
+    p = nullptr;
+  };
+  release(c->V);
+  release(c->B);
+  release(c->W);
+  release(c->R);
+  release(c->sig);
+  release(c->pot);
+  release(c->pres);
+  release(c->R32);
+  release(c->W32);
+  for (int b = 0; b < 2; ++b) {
+    release(c->Z[b]);
+    release(c->F[b]);
+  }
+  c->cap = 0;
+  const size_t n = c->nfa, nt = (size_t)c->nfa * c->nb;
+  const size_t k = nrhs;
+  int r = 0;
+  if ((r = dalloc(c, &c->V, nt * k)) || (r = dalloc(c, &c->B, n * k)) || (r = dalloc(c, &c->W, n * k)) ||
+      (r = dalloc(c, &c->R, n * k)) || (r = dalloc(c, &c->sig, 2 * n * k)) || (r = dalloc(c, &c->pot, 2 * n * k)) ||
+      (r = dalloc(c, &c->pres, 2 * n * k)) || (r = dalloc(c, &c->R32, n * k)) || (r = dalloc(c, &c->W32, n * k)))
+    return r;
+  for (int b = 0; b < c->nb; ++b)
+    if ((r = dalloc(c, &c->Z[b], n * k)) || (r = dalloc(c, &c->F[b], n * k))) return r;
+  c->cap = nrhs;
+  return NEMOH_OK;
+}
+
+// FreqParams for one call: the per-frequency scalars of VNSINFD / VNSFD.
+int build_params(nemoh_core* c, double period, double amh, int nexp, const double* ambda, const double* ar, int nq,
+                 const double* qq, const double* cq, bool* fd_out, double* w_out) {
+  const double pi = 4.0 * std::atan(1.0);
+  const double w = 2.0 * pi / period;
+  if (!(std::fabs(w) >= 1.E-4)) return fail(c, NEMOH_ERR_OMEGA, "ABS(WR) < 1.E-4");
+  const double kwave = w * w / c->g;
+  const bool fd = !(c->depth == 0.0 || kwave * c->depth >= 20);  // SOLVE_BEM.f90:66
+  FreqParams& P = c->h_par;
+  std::memset(&P, 0, sizeof P);
+  P.fd = fd ? 1 : 0;
+  P.nsym = c->nsym;
+  P.zer = c->zer;
+  P.depth = c->depth;
+  if (!fd) {
+    P.ak0 = w * w / c->g;
+    if (nq < 0 || nq > 101 || (nq > 0 && (!qq || !cq))) return fail(c, NEMOH_ERR_ARG, "nq out of range");
+    P.nq = nq;
+    for (int k = 0; k < nq; ++k) {
+      P.qq[k] = qq[k];
+      P.cq[k] = cq[k];
+    }
+  } else {
+    if (nexp < 0 || nexp > 30 || (nexp > 0 && (!ambda || !ar))) return fail(c, NEMOH_ERR_ARG, "nexp out of range");
+    const double h = c->depth;
+    const double akh = amh * std::tanh(amh);
+    const double a = (amh + akh) * (amh + akh) / (h * (amh * amh - akh * akh + akh));
+    const double am0 = amh / h;
+    const double adpi2 = -a / (8. * pi * pi), adpi = -a / (8 * pi);
+    P.ak0 = am0;
+    P.coe1 = adpi2 / am0;
+    P.coe2 = adpi / am0;
+    P.coe3 = adpi2;
+    P.coe4 = adpi;
+    for (int k = 0; k < nexp; ++k) {
+      P.ambda[k] = ambda[k];
+      P.ar[k] = ar[k];
+    }
+    P.ambda[nexp] = 0.0;  // VNSFD.f90:262-263
+    P.ar[nexp] = 2.0;
+    P.nexp1 = nexp + 1;
+  }
+  *fd_out = fd;
+  *w_out = w;
+  HIPCHK(c, hipMemcpyAsync(c->d_par, &P, sizeof P, hipMemcpyHostToDevice, c->st));
+  return NEMOH_OK;
+}
+
+int check_info(nemoh_core* c) {
+  int info = 0;
+  HIPCHK(c, hipMemcpyAsync(&info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (info) {
+    c->w_prev = -1.0;
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "PIVOT INFERIEUR A 1e-20 (row %d)", info);
+    return fail(c, NEMOH_ERR_PIVOT, buf);
+  }
+  return NEMOH_OK;
+}
+
+// Assembly + factorisation for frequency w (skipped while w is unchanged).
+int prepare_frequency(nemoh_core* c, double w) {
+  if (w == c->w_prev && c->factored_prec == c->prec) {
+    c->stats.cache_hit = 1;
+    return NEMOH_OK;
+  }
+  c->stats.cache_hit = 0;
+  if (int r = ensure_matrices(c)) return r;
+  MatOut out{c->nfa, c->D[0], c->S[0], c->D[1], c->S[1], nullptr, nullptr};
+  if (c->prec == NEMOH_MIXED) {
+    out.dp32 = c->D32[0];
+    out.dm32 = c->D32[1];
+  }
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->st));
+  launch_assemble(c->G, c->T, c->d_par, c->nsym, out, c->st);
+  HIPCHK(c, hipGetLastError());
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
+  HIPCHK(c, hipMemsetAsync(c->d_info, 0, sizeof(int), c->st));
+  for (int b = 0; b < c->nb; ++b) {
+    if (c->prec == NEMOH_MIXED) launch_lu<float>(reinterpret_cast<C2<float>*>(c->D32[b]), c->nfa, c->nfa, c->d_info, c->st);
+    else launch_lu<double>(reinterpret_cast<C2<double>*>(c->D[b]), c->nfa, c->nfa, c->d_info, c->st);
+  }
+  HIPCHK(c, hipGetLastError());
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
+  if (int r = check_info(c)) return r;
+  if (c->timing) {
+    float a = 0, f = 0;
+    HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&f, c->ev[1], c->ev[2]));
+    c->stats.ms_assembly = a;
+    c->stats.ms_factor = f;
+  }
+  c->w_prev = w;
+  c->factored_prec = c->prec;
+  return NEMOH_OK;
+}
+
+double device_absmax(nemoh_core* c, const double2* a, size_t count, int* err) {
+  const int nblk = 64;
+  launch_absmax(a, count, c->d_red, nblk, c->st);
+  double h[64];
+  if (hipMemcpyAsync(h, c->d_red, sizeof h, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess) {
+    *err = NEMOH_ERR_HIP;
+    return 0;
+  }
+  double m = 0;
+  for (int k = 0; k < nblk; ++k) m = std::fmax(m, h[k]);
+  return m;
+}
+
+// Solve block b for the right-hand sides in c->B (n x nrhs, row-major):
+// solution into c->Z[b].
+int solve_block(nemoh_core* c, int b, int nrhs) {
+  const int n = c->nfa;
+  const size_t cnt = (size_t)n * nrhs;
+  if (c->prec == NEMOH_FP64) {
+    HIPCHK(c, hipMemcpyAsync(c->Z[b], c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    launch_lu_solve<double>(reinterpret_cast<const C2<double>*>(c->D[b]), n, n,
+                            reinterpret_cast<C2<double>*>(c->Z[b]), reinterpret_cast<C2<double>*>(c->W), nrhs, c->st);
+    HIPCHK(c, hipGetLastError());
+    return NEMOH_OK;
+  }
+  // mixed: complex64 LU solve, FP64 residual r = b - D x, refine
+  HIPCHK(c, hipMemcpyAsync(c->R, c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+  int e = 0;
+  const double bmax = device_absmax(c, c->B, cnt, &e);
+  if (e) return fail(c, e, "reduction failed");
+  double rmax = bmax;
+  int it = 0;
+  for (; it < c->max_iter; ++it) {
+    if (rmax == 0.0) break;
+    launch_f64_to_f32(c->R, c->R32, cnt, 1.0 / rmax, c->st);
+    launch_lu_solve<float>(reinterpret_cast<const C2<float>*>(c->D32[b]), n, n, reinterpret_cast<C2<float>*>(c->R32),
+                           reinterpret_cast<C2<float>*>(c->W32), nrhs, c->st);
+    launch_f32_to_f64(c->R32, c->Z[b], cnt, rmax, it > 0, c->st);
+    HIPCHK(c, hipMemcpyAsync(c->R, c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    launch_matvec<double>(reinterpret_cast<const C2<double>*>(c->D[b]), n, n, c->Z[b], nrhs, c->R, -1.0, 1.0,
+                          c->st);
+    HIPCHK(c, hipGetLastError());
+    rmax = device_absmax(c, c->R, cnt, &e);
+    if (e) return fail(c, e, "reduction failed");
+    if (rmax <= c->tol * bmax) {
+      ++it;
+      break;
+    }
+  }
+  c->stats.refine_iterations = std::max(c->stats.refine_iterations, it);
+  c->stats.refine_residual = std::fmax(c->stats.refine_residual, bmax > 0 ? rmax / bmax : 0.0);
+  if (rmax > c->tol * bmax && bmax > 0) {
+    char buf[128];
+    std::snprintf(buf, sizeof buf, "refinement stopped at max|r|/max|b| = %.3e after %d iterations", rmax / bmax, it);
+    return fail(c, NEMOH_ERR_CONVERGENCE, buf);
+  }
+  return NEMOH_OK;
+}
+
+// Everything after the frequency set-up: c->V holds (ntot x nrhs) row-major
+// normal velocities on the device.
+int run_pipeline(nemoh_core* c, double w, int nrhs) {
+  if (int r = prepare_frequency(c, w)) return r;
+  const int n = c->nfa;
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
+  c->stats.refine_iterations = 0;
+  c->stats.refine_residual = 0;
+  for (int b = 0; b < c->nb; ++b) {
+    launch_rhs(c->V, n, c->nsym, nrhs, b == 0 ? 1.0 : -1.0, c->B, c->st);
+    if (int r = solve_block(c, b, nrhs)) return r;
+    launch_matvec<double>(reinterpret_cast<const C2<double>*>(c->S[b]), n, n, c->Z[b], nrhs, c->F[b], 1.0, 0.0, c->st);
+  }
+  launch_combine(c->Z[0], c->Z[1], c->F[0], c->F[1], n, c->nsym, nrhs, c->rho * w, c->sig, c->pot, c->pres, c->st);
+  HIPCHK(c, hipGetLastError());
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], c->st));
+  return NEMOH_OK;
+}
+
+int upload_nvel_colmajor(nemoh_core* c, const double* nvel, int nrhs) {
+  // caller: (ntot, nrhs) column-major complex; device: (ntot x nrhs) row-major
+  const size_t nt = (size_t)c->nfa * c->nb;
+  std::vector<double> tmp(2 * nt * nrhs);
+  for (int q = 0; q < nrhs; ++q)
+    for (size_t i = 0; i < nt; ++i) {
+      tmp[2 * (i * nrhs + q)] = nvel[2 * (q * nt + i)];
+      tmp[2 * (i * nrhs + q) + 1] = nvel[2 * (q * nt + i) + 1];
+    }
+  HIPCHK(c, hipMemcpyAsync(c->V, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return NEMOH_OK;
+}
+
+}  // namespace
+
+// nds staging is kept outside the anonymous namespace helpers above
+static int stage_nds(nemoh_core* c, int nint, const double* nds) {
+  const size_t nt = (size_t)c->nfa * c->nb;
+  const size_t need = (size_t)nint * nt;
+  if (need > c->nds_cap) {
+    if (c->d_nds) hipFree(c->d_nds);
+    c->d_nds = nullptr;
+    c->nds_cap = 0;
+    HIPCHK(c, hipMalloc((void**)&c->d_nds, need * sizeof(double)));
+    c->nds_cap = need;
+  }
+  // caller: NDS(nint, ntot) column-major -> device (nint x ntot) row-major
+  std::vector<double> tmp(need);
+  for (int k = 0; k < nint; ++k)
+    for (size_t i = 0; i < nt; ++i) tmp[k * nt + i] = nds[k + (size_t)nint * i];
+  HIPCHK(c, hipMemcpyAsync(c->d_nds, tmp.data(), need * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return NEMOH_OK;
+}
+
+static int finish_outputs(nemoh_core* c, int nrhs, double* pressure, int nint, const double* nds, double* forces) {
+  const size_t nt = (size_t)c->nfa * c->nb;
+  if (forces && nint > 0) {
+    if (!nds) return fail(c, NEMOH_ERR_ARG, "nds is NULL");
+    if (int r = stage_nds(c, nint, nds)) return r;
+    if (nint * nrhs > c->force_cap) {
+      if (c->d_force) hipFree(c->d_force);
+      c->d_force = nullptr;
+      HIPCHK(c, hipMalloc((void**)&c->d_force, (size_t)nint * nrhs * sizeof(double2)));
+      c->force_cap = nint * nrhs;
+    }
+    launch_force(c->pres, c->d_nds, nint, (int)nt, nrhs, c->d_force, c->st);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> f(2 * (size_t)nint * nrhs);
+    HIPCHK(c, hipMemcpyAsync(f.data(), c->d_force, f.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int k = 0; k < nint; ++k)  // device (nint x nrhs) row-major -> caller column-major
+      for (int q = 0; q < nrhs; ++q) {
+        forces[2 * (k + (size_t)nint * q)] = f[2 * (k * nrhs + q)];
+        forces[2 * (k + (size_t)nint * q) + 1] = f[2 * (k * nrhs + q) + 1];
+      }
+  }
+  if (pressure) {
+    std::vector<double> p(2 * nt * nrhs);
+    HIPCHK(c, hipMemcpyAsync(p.data(), c->pres, p.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < nt; ++i)
+      for (int q = 0; q < nrhs; ++q) {
+        pressure[2 * (i + nt * q)] = p[2 * (i * nrhs + q)];
+        pressure[2 * (i + nt * q) + 1] = p[2 * (i * nrhs + q) + 1];
+      }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (c->timing) {
+    float s = 0, t = 0;
+    HIPCHK(c, hipEventElapsedTime(&s, c->ev[2], c->ev[3]));
+    c->stats.ms_solve = s;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev[c->stats.cache_hit ? 2 : 0], c->ev[3]));
+    c->stats.ms_total = t;
+  }
+  return NEMOH_OK;
+}
+
+extern "C" {
+
+int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double* x, const double* y, const double* z,
+                      const int* m1, const int* m2, const int* m3, const int* m4, const double* xg, const double* yg,
+                      const double* zg, const double* xn, const double* yn, const double* zn, const double* aire,
+                      const double* tdis, double zer, double g, double rho, double depth, const double* xr,
+                      const double* xz, const double* apd1x, const double* apd1z, const double* apd2x,
+                      const double* apd2z) {
+  if (!out) return NEMOH_ERR_ARG;
+  *out = nullptr;
+  if (np <= 0 || nfa <= 0 || (nsym != 0 && nsym != 1) || !x || !y || !z || !m1 || !m2 || !m3 || !m4 || !xg || !yg ||
+      !zg || !xn || !yn || !zn || !aire || !tdis || !xr || !xz || !apd1x || !apd1z || !apd2x || !apd2z || depth < 0)
+    return NEMOH_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NEMOH_ERR_HIP;
+  nemoh_core* c = new nemoh_core();
+  c->np = np;
+  c->nfa = nfa;
+  c->nsym = nsym;
+  c->nb = 1 << nsym;
+  c->zer = zer;
+  c->g = g;
+  c->rho = rho;
+  c->depth = depth;
+  c->hx.assign(x, x + np);
+  c->hy.assign(y, y + np);
+  c->gm = host::max_horizontal_extent(np, nsym, x, y);
+  auto bail = [&](int code) {
+    nemoh_core_destroy(c);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return bail(NEMOH_ERR_HIP);
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return bail(NEMOH_ERR_HIP);
+  std::vector<double> geo(8 * (size_t)nfa), vtx(12 * (size_t)nfa);
+  const double* src[8] = {xg, yg, zg, xn, yn, zn, aire, tdis};
+  for (int k = 0; k < 8; ++k) std::memcpy(&geo[(size_t)k * nfa], src[k], nfa * sizeof(double));
+  for (int j = 0; j < nfa; ++j) {
+    const int m[4] = {m1[j] - 1, m2[j] - 1, m3[j] - 1, m4[j] - 1};
+    for (int l = 0; l < 4; ++l) {
+      if (m[l] < 0 || m[l] >= np) return bail(NEMOH_ERR_ARG);
+      vtx[4 * (size_t)j + l] = x[m[l]];
+      vtx[4 * (size_t)nfa + 4 * (size_t)j + l] = y[m[l]];
+      vtx[8 * (size_t)nfa + 4 * (size_t)j + l] = z[m[l]];
+    }
+  }
+  std::vector<double> tab(4 * (size_t)kNR * kNZ);
+  for (int j = 0; j < kNZ; ++j)
+    for (int i = 0; i < kNR; ++i) {
+      const size_t f = i + (size_t)kNR * j;  // Fortran (i, j)
+      tab[4 * f + 0] = apd1x[f];
+      tab[4 * f + 1] = apd1z[f];
+      tab[4 * f + 2] = apd2x[f];
+      tab[4 * f + 3] = apd2z[f];
+    }
+  if (hipMalloc((void**)&c->d_geo, geo.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->d_vtx, vtx.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->d_tab, tab.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->d_xr, kNR * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->d_xz, kNZ * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->d_par, sizeof(FreqParams)) != hipSuccess ||
+      hipMalloc((void**)&c->d_red, 64 * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->d_info, sizeof(int)) != hipSuccess)
+    return bail(NEMOH_ERR_HIP);
+  if (hipMemcpy(c->d_geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_vtx, vtx.data(), vtx.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_xr, xr, kNR * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_xz, xz, kNZ * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(NEMOH_ERR_HIP);
+  const double* d = c->d_geo;
+  c->G = Geometry{nfa, d, d + nfa, d + 2 * (size_t)nfa, d + 3 * (size_t)nfa, d + 4 * (size_t)nfa,
+                  d + 5 * (size_t)nfa, d + 6 * (size_t)nfa, d + 7 * (size_t)nfa,
+                  c->d_vtx, c->d_vtx + 4 * (size_t)nfa, c->d_vtx + 8 * (size_t)nfa};
+  c->T = Tables{c->d_xr, c->d_xz, c->d_tab};
+  *out = c;
+  return NEMOH_OK;
+}
+
+void nemoh_core_destroy(nemoh_core* c) {
+  if (!c) return;
+  if (c->st) hipStreamSynchronize(c->st);
+  auto release = [](auto*& p) {
+    if (p) hipFree(p);
+    p = nullptr;
+  };
+  release(c->d_geo);
+  release(c->d_vtx);
+  release(c->d_xr);
+  release(c->d_xz);
+  release(c->d_tab);
+  release(c->d_par);
+  release(c->d_red);
+  release(c->d_info);
+  release(c->d_nds);
+  release(c->d_force);
+  for (int b = 0; b < 2; ++b) {
+    release(c->D[b]);
+    release(c->S[b]);
+    release(c->D32[b]);
+    release(c->Z[b]);
+    release(c->F[b]);
+  }
+  release(c->V);
+  release(c->B);
+  release(c->W);
+  release(c->R);
+  release(c->sig);
+  release(c->pot);
+  release(c->pres);
+  release(c->R32);
+  release(c->W32);
+  for (auto& e : c->ev)
+    if (e) hipEventDestroy(e);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char* nemoh_core_last_error(const nemoh_core* c) { return c ? c->err.c_str() : "null handle"; }
+
+int nemoh_core_set_precision(nemoh_core* c, int mode, double tol, int max_iter) {
+  if (!c || (mode != NEMOH_FP64 && mode != NEMOH_MIXED)) return NEMOH_ERR_ARG;
+  c->prec = mode;
+  if (tol > 0) c->tol = tol;
+  if (max_iter > 0) c->max_iter = max_iter;
+  return NEMOH_OK;
+}
+
+int nemoh_core_enable_timing(nemoh_core* c, int on) {
+  if (!c) return NEMOH_ERR_ARG;
+  c->timing = on != 0;
+  return NEMOH_OK;
+}
+
+int nemoh_core_get_stats(const nemoh_core* c, nemoh_stats* out) {
+  if (!c || !out) return NEMOH_ERR_ARG;
+  *out = c->stats;
+  return NEMOH_OK;
+}
+
+int nemoh_core_solve_multi(nemoh_core* c, double period, double amh, int nexp, const double* ambda, const double* ar,
+                           int nq, const double* qq, const double* cq, int nrhs, const double* nvel, double* pressure,
+                           int nint, const double* nds, double* forces) {
+  if (!c || nrhs <= 0 || nrhs > kMaxRhs || !nvel) return fail(c, NEMOH_ERR_ARG, "bad arguments");
+  bool fd;
+  double w;
+  if (int r = build_params(c, period, amh, nexp, ambda, ar, nq, qq, cq, &fd, &w)) return r;
+  if (int r = ensure_rhs(c, nrhs)) return r;
+  if (int r = upload_nvel_colmajor(c, nvel, nrhs)) return r;
+  if (int r = run_pipeline(c, w, nrhs)) return r;
+  return finish_outputs(c, nrhs, pressure, nint, nds, forces);
+}
+
+int nemoh_core_solve(nemoh_core* c, double period, double amh, int nexp, const double* ambda, const double* ar, int nq,
+                     const double* qq, const double* cq, const double* nvel, double* zigb, double* zigs, double* zpb,
+                     double* zps) {
+  if (!c || !nvel) return fail(c, NEMOH_ERR_ARG, "bad arguments");
+  bool fd;
+  double w;
+  if (int r = build_params(c, period, amh, nexp, ambda, ar, nq, qq, cq, &fd, &w)) return r;
+  if (int r = ensure_rhs(c, 1)) return r;
+  if (int r = upload_nvel_colmajor(c, nvel, 1)) return r;
+  if (int r = run_pipeline(c, w, 1)) return r;
+  const size_t n = c->nfa;
+  std::vector<double> s(4 * n), p(4 * n);
+  HIPCHK(c, hipMemcpyAsync(s.data(), c->sig, s.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(p.data(), c->pot, p.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (zigb) std::memcpy(zigb, s.data(), 2 * n * sizeof(double));
+  if (zigs) std::memcpy(zigs, s.data() + 2 * n, 2 * n * sizeof(double));
+  if (zpb) std::memcpy(zpb, p.data(), 2 * n * sizeof(double));
+  if (zps) std::memcpy(zps, p.data() + 2 * n, 2 * n * sizeof(double));
+  return NEMOH_OK;
+}
+
+int nemoh_core_freq_setup(nemoh_core* c, double period, nemoh_freq* f) {
+  if (!c || !f) return NEMOH_ERR_ARG;
+  std::memset(f, 0, sizeof *f);
+  const double pi = 4.0 * std::atan(1.0);
+  const double w = 2.0 * pi / period;
+  if (!(std::fabs(w) >= 1.E-4)) return fail(c, NEMOH_ERR_OMEGA, "ABS(WR) < 1.E-4");
+  const double kwave = w * w / c->g;
+  if (c->depth == 0.0 || kwave * c->depth >= 20) {
+    // CINT_INFD fills QQ/CQ; the module NQ stays as it was
+    f->finite_depth = 0;
+    const double ak0 = w * w / c->g;
+    double qq[101], cq[101];
+    const int n = host::theta_rule(ak0 * c->gm, qq, cq);
+    f->nq = c->nq_state;
+    for (int k = 0; k < 101; ++k) {
+      f->qq[k] = (k < n) ? qq[k] : 0.0;
+      f->cq[k] = (k < n) ? cq[k] : 0.0;
+    }
+  } else {
+    f->finite_depth = 1;
+    const double akh0 = w * w * c->depth / c->g;
+    f->amh = host::x0(akh0);
+    if (std::isnan(f->amh)) return fail(c, NEMOH_ERR_WAVENUMBER, "ERREUR DANS LA RECHERCHE DE LA RACINE");
+    const double akh = f->amh * std::tanh(f->amh);
+    double qq[101], cq[101];
+    c->nq_state = host::theta_rule(f->amh / c->depth * c->gm, qq, cq);  // NQ=N; rule stays local
+    f->nq = c->nq_state;
+    const int e = host::prony_fit(akh, f->amh, &f->nexp, f->ambda, f->ar);
+    if (e) return fail(c, NEMOH_ERR_FIT, "exponential fit of the finite-depth kernel failed");
+  }
+  return NEMOH_OK;
+}
+
+int nemoh_core_solve_frequency(nemoh_core* c, double period, int nrhs, const double* nvel, double* pressure, int nint,
+                               const double* nds, double* forces) {
+  nemoh_freq f;
+  if (int r = nemoh_core_freq_setup(c, period, &f)) return r;
+  return nemoh_core_solve_multi(c, period, f.amh, f.nexp, f.ambda, f.ar, f.nq, f.qq, f.cq, nrhs, nvel, pressure, nint,
+                                nds, forces);
+}
+
+int nemoh_core_pairs(nemoh_core* c, double period, double amh, int nexp, const double* ambda, const double* ar, int nq,
+                     const double* qq, const double* cq, double* out) {
+  if (!c || !out) return NEMOH_ERR_ARG;
+  bool fd;
+  double w;
+  if (int r = build_params(c, period, amh, nexp, ambda, ar, nq, qq, cq, &fd, &w)) return r;
+  const size_t cnt = (size_t)c->nfa * c->nfa * 16;
+  double* d = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d, cnt * sizeof(double)));
+  launch_pairs(c->G, c->T, c->d_par, d, c->st);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, cnt * sizeof(double), hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  hipFree(d);
+  if (e != hipSuccess) return fail(c, NEMOH_ERR_HIP, hipGetErrorString(e));
+  return NEMOH_OK;
+}
+
+int nemoh_core_prepare_mesh(int np, int nfa, int nsym, const double* x, const double* y, const double* z, const int* m1,
+                            const int* m2, const int* m3, const int* m4, double* xg, double* yg, double* zg, double* xn,
+                            double* yn, double* zn, double* aire, double* tdis, double* zer) {
+  if (np <= 0 || nfa <= 0 || !x || !y || !z || !m1 || !m2 || !m3 || !m4 || !zer) return NEMOH_ERR_ARG;
+  std::vector<int> a(nfa), b(nfa), cc(nfa), d(nfa);
+  for (int i = 0; i < nfa; ++i) {
+    a[i] = m1[i] - 1;
+    b[i] = m2[i] - 1;
+    cc[i] = m3[i] - 1;
+    d[i] = m4[i] - 1;
+    if (a[i] < 0 || b[i] < 0 || cc[i] < 0 || d[i] < 0 || a[i] >= np || b[i] >= np || cc[i] >= np || d[i] >= np)
+      return NEMOH_ERR_ARG;
+  }
+  host::PanelGeometry g{xg, yg, zg, xn, yn, zn, aire, tdis, 0.0};
+  const int r = host::prepare_mesh(np, nfa, nsym, x, y, z, a.data(), b.data(), cc.data(), d.data(), &g);
+  *zer = g.zer;
+  return r ? NEMOH_ERR_PANEL : NEMOH_OK;
+}
+
+int nemoh_core_build_tables(double* xr, double* xz, double* apd1x, double* apd1z, double* apd2x, double* apd2z) {
+  if (!xr || !xz || !apd1x || !apd1z || !apd2x || !apd2z) return NEMOH_ERR_ARG;
+  // grids (NEMOH.f90:184-198)
+  for (int j = 1; j <= kNZ; ++j)
+    xz[j - 1] = -std::fmin(std::fmin(std::pow(10.0, j / 5.0 - 6.0), std::pow(10.0, j / 8.0 - 4.5)), 16.0);
+  xr[0] = 0.0;
+  for (int i = 2; i <= kNR; ++i)
+    xr[i - 1] = (i < 40) ? std::fmin(std::pow(10.0, (i - 1.0) / 5 - 6), 4.0 / 3.0 + std::fabs((i - 32.0) / 3.0))
+                         : 4.0 / 3.0 + std::fabs((i - 32.0) / 3.0);
+  double *dxr = nullptr, *dxz = nullptr, *dtab = nullptr;
+  std::vector<double> tab(4 * (size_t)kNR * kNZ);
+  hipError_t e = hipMalloc((void**)&dxr, kNR * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&dxz, kNZ * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&dtab, tab.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(dxr, xr, kNR * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dxz, xz, kNZ * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch_crek(dxr, dxz, dtab, nullptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(tab.data(), dtab, tab.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (dxr) hipFree(dxr);
+  if (dxz) hipFree(dxz);
+  if (dtab) hipFree(dtab);
+  if (e != hipSuccess) return NEMOH_ERR_HIP;
+  for (size_t f = 0; f < (size_t)kNR * kNZ; ++f) {  // node f = i + 328 j (Fortran order)
+    apd1x[f] = tab[4 * f + 0];
+    apd1z[f] = tab[4 * f + 1];
+    apd2x[f] = tab[4 * f + 2];
+    apd2z[f] = tab[4 * f + 3];
+  }
+  return NEMOH_OK;
+}
+
+double nemoh_core_wavenumber(double ak) { return host::x0(ak); }
+
+int nemoh_core_theta_rule(double akk, double* qq, double* cq) {
+  if (!qq || !cq) return -1;
+  return host::theta_rule(akk, qq, cq);
+}
+
+int nemoh_core_prony(double akh, double amh, int* nexp, double* ambda, double* ar) {
+  if (!nexp || !ambda || !ar) return NEMOH_ERR_ARG;
+  return host::prony_fit(akh, amh, nexp, ambda, ar) ? NEMOH_ERR_FIT : NEMOH_OK;
+}
+
+int nemoh_core_mfma_probe(int fp32, double* out) {
+  if (!out) return NEMOH_ERR_ARG;
+  double ha[64], hb[64];
+  for (int i = 0; i < 16; ++i)
+    for (int k = 0; k < 4; ++k) ha[i * 4 + k] = (i + 1) * 0.5 + k * 3.0;
+  for (int k = 0; k < 4; ++k)
+    for (int j = 0; j < 16; ++j) hb[k * 16 + j] = (k + 1) * 1.25 - j * 0.75;
+  hipError_t e;
+  if (fp32) {
+    float fa[64], fb[64], fd[256];
+    for (int k = 0; k < 64; ++k) {
+      fa[k] = (float)ha[k];
+      fb[k] = (float)hb[k];
+    }
+    float *da = nullptr, *db = nullptr, *dd = nullptr;
+    e = hipMalloc((void**)&da, sizeof fa);
+    if (e == hipSuccess) e = hipMalloc((void**)&db, sizeof fb);
+    if (e == hipSuccess) e = hipMalloc((void**)&dd, sizeof fd);
+    if (e == hipSuccess) e = hipMemcpy(da, fa, sizeof fa, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, fb, sizeof fb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      launch_mfma_probe_f32(da, db, dd, nullptr);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(fd, dd, sizeof fd, hipMemcpyDeviceToHost);
+    if (da) hipFree(da);
+    if (db) hipFree(db);
+    if (dd) hipFree(dd);
+    for (int k = 0; k < 256; ++k) out[k] = fd[k];
+  } else {
+    double *da = nullptr, *db = nullptr, *dd = nullptr;
+    e = hipMalloc((void**)&da, sizeof ha);
+    if (e == hipSuccess) e = hipMalloc((void**)&db, sizeof hb);
+    if (e == hipSuccess) e = hipMalloc((void**)&dd, 256 * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(da, ha, sizeof ha, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, hb, sizeof hb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      launch_mfma_probe(da, db, dd, nullptr);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dd, 256 * sizeof(double), hipMemcpyDeviceToHost);
+    if (da) hipFree(da);
+    if (db) hipFree(db);
+    if (dd) hipFree(dd);
+  }
+  return e == hipSuccess ? NEMOH_OK : NEMOH_ERR_HIP;
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// Launch interface of the solver-core kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+
+#include "green.h"
+#include "linalg.h"
+
+namespace nemoh {
+
+constexpr int kMaxRhs = 16;  // right-hand sides per triangular-solve pass
+
+// Influence-matrix outputs of one assembly launch (row-major, leading
+// dimension ld).  dm/sm are the antisymmetric block (nsym = 1 only);
+// dp32/dm32 are optional single-precision copies of D for the mixed path.
+struct MatOut {
+  int ld;
+  double2* dp;
+  double2* sp;
+  double2* dm;
+  double2* sm;
+  float2* dp32;
+  float2* dm32;
+};
+
+void launch_crek(const double* xr, const double* xz, double* tab, hipStream_t st);
+void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, int nsym, const MatOut& out,
+                     hipStream_t st);
+void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, double* out, hipStream_t st);
+template <typename T>
+void launch_lu(C2<T>* A, int ld, int n, int* info, hipStream_t st);
+// Solves (LU) X = B in place: on return B holds the solution, X is scratch.
+template <typename T>
+void launch_lu_solve(const C2<T>* A, int ld, int n, C2<T>* B, C2<T>* X, int nrhs, hipStream_t st);
+template <typename TM>
+void launch_matvec(const C2<TM>* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
+                   double beta, hipStream_t st);
+void launch_rhs(const double2* V, int n, int nsym, int nrhs, double bx, double2* B, hipStream_t st);
+void launch_combine(const double2* Z1, const double2* Z2, const double2* F1, const double2* F2, int n, int nsym,
+                    int nrhs, double rho_w, double2* sig, double2* pot, double2* pres, hipStream_t st);
+void launch_force(const double2* pres, const double* nds, int nint, int ntot, int nrhs, double2* F, hipStream_t st);
+void launch_f64_to_f32(const double2* a, float2* b, size_t count, double scale, hipStream_t st);
+void launch_f32_to_f64(const float2* a, double2* b, size_t count, double scale, int accumulate, hipStream_t st);
+void launch_absmax(const double2* a, size_t count, double* out, int nblocks, hipStream_t st);
+void launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t st);
+void launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t st);
+
+}  // namespace nemoh

@@ -1,0 +1,62 @@
+// Dense complex linear algebra for the influence-matrix solve on CDNA4.
+//
+// Matrices are row-major, element = {re, im} (double2 / float2), leading
+// dimension ld.  The factorisation is a blocked right-looking LU without row
+// interchanges, like the reference's Gauss-Jordan elimination (GAUSSZ,
+// Solver/Core/M_SOLVER.f90:31-73, whose row search never swaps) -- the BEM
+// matrix has a dominant 1/2 diagonal from the -2 pi n self term.  Each block
+// step is two launches:
+//   panel_kernel  every workgroup re-factors the 64x64 diagonal block in LDS
+//                 (identical arithmetic in each) and then solves one 64-row
+//                 block of L21 = A21 U11^-1 or one 64-column block of
+//                 U12 = L11^-1 A12; workgroup 0 also stores the factored block
+//   update_kernel A22 -= L21 U12 on MFMA (16x16x4 tiles, complex product as
+//                 four real MFMAs), 64x64 output tile per workgroup.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace nemoh {
+
+constexpr int kNB = 64;  // LU / solve block size
+
+template <typename T>
+struct C2 {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ C2<T> csub_mul(C2<T> acc, C2<T> a, C2<T> b) {  // acc - a*b
+  return {acc.re - (a.re * b.re - a.im * b.im), acc.im - (a.re * b.im + a.im * b.re)};
+}
+template <typename T>
+__device__ __forceinline__ C2<T> cdiv2(C2<T> a, C2<T> b) {
+  const T d = b.re * b.re + b.im * b.im;
+  return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+
+template <typename T>
+struct MfmaTraits;
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <>
+struct MfmaTraits<double> {
+  typedef v4d acc_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MfmaTraits<float> {
+  typedef v4f acc_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+}  // namespace nemoh

@@ -1870,3 +1870,16 @@ void ora_rankine_pub(const ora_ctx *c, int kkk, double dh, int isp, int ifp, dou
   out[0] = rk.fsp; out[1] = rk.fsm; out[2] = rk.vsxp; out[3] = rk.vsyp;
   out[4] = rk.vszp; out[5] = rk.vsxm; out[6] = rk.vsym; out[7] = rk.vszm;
 }
+
+/* CPU-baseline sample: every pair of rows [r0, r0+nr) (all columns) as the
+ * reference evaluates them, returning a checksum of the outputs. */
+double ora_pairs_rows(const ora_ctx *c, const ora_freq *f, int fd, int r0, int nr) {
+  double acc = 0.0, o[ORA_PAIR_N];
+  for (int i = r0; i < r0 + nr && i < c->nfa; ++i)
+    for (int j = 0; j < c->nfa; ++j) {
+      if (fd) ora_pair_fd(c, f, i, j, o);
+      else ora_pair_infd(c, f, i, j, o);
+      for (int k = 0; k < ORA_PAIR_N; ++k) acc += o[k];
+    }
+  return acc;
+}

@@ -67,6 +67,8 @@ def lib():
         L.ora_gaussz.argtypes = [_cp, C.c_int, C.c_int, C.c_int]
         L.ora_forces.argtypes = [C.c_int, C.c_int, _dp, _cp, _cp]
         L.ora_forces.restype = None
+        L.ora_pairs_rows.argtypes = [C.POINTER(_Ctx), C.POINTER(Freq), C.c_int, C.c_int, C.c_int]
+        L.ora_pairs_rows.restype = C.c_double
         L.ora_gg_vec.argtypes = [C.c_int, _cp, _cp, _cp]
         L.ora_gg_vec.restype = None
         _lib = L
@@ -167,6 +169,10 @@ class Case:
         out = np.zeros(self.nfa * self.nfa * PAIR_N)
         und = lib().ora_pairs_all(C.byref(self.ctx), C.byref(f), int(fd), out)
         return out.reshape(self.nfa, self.nfa, PAIR_N), und
+
+    def pairs_rows(self, period: float, r0: int, nr: int, nq_state: int = 0) -> float:
+        f, fd, _ = self.freq(period, nq_state)
+        return lib().ora_pairs_rows(C.byref(self.ctx), C.byref(f), int(fd), int(r0), int(nr))
 
     def session(self):
         return Session(self)

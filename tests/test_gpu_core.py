@@ -1,0 +1,159 @@
+"""Parity of the HIP solver core with the reference (golden fixtures from the
+FP64 reference build) and with the oracle, through the C ABI.
+
+Tolerances: FP64 path <= 1e-10 relative (normwise per problem column: the
+structurally-zero symmetry terms make elementwise ratios meaningless); mixed
+path (complex64 LU + FP64 refinement) <= 1e-6 on the same measure.
+"""
+import numpy as np
+import pytest
+
+from conftest import case_geometry, load_golden
+from nemoh_amd import bodycond as bcm
+from nemoh_amd import core
+from nemoh_amd import io as nio
+from nemoh_amd import mesh as nmesh
+from nemoh_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+MIXED_TOL = 1e-6
+CASES = ["Cylinder", "NonSymmetrical", "2Bodies", "Postprocessing", "cyl512", "barge_fd"]
+
+
+def _tables():
+    t = load_golden("fn_crek.npz")
+    return t["xr"], t["xz"], (t["apd1x"], t["apd1z"], t["apd2x"], t["apd2z"])
+
+
+def _normwise(a, ref):
+    return float((np.abs(a - ref).max(axis=0) / np.abs(ref).max(axis=0)).max())
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_mfma_lane_layout(fp32):
+    a = np.array([[(i + 1) * 0.5 + k * 3.0 for k in range(4)] for i in range(16)])
+    b = np.array([[(k + 1) * 1.25 - j * 0.75 for j in range(16)] for k in range(4)])
+    got = core.mfma_probe(fp32)
+    assert np.allclose(got, a @ b, rtol=1e-6 if fp32 else 1e-14, atol=1e-5 if fp32 else 1e-12)
+
+
+def test_tables_match_reference():
+    xr, xz, tabs = core.build_tables()
+    rxr, rxz, rtabs = _tables()
+    assert np.array_equal(xr, rxr) and np.array_equal(xz, rxz)
+    for a, b in zip(tabs, rtabs):
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("name", ["pairs_lidcyl.npz", "pairs_barge.npz"])
+def test_pairs_match_reference(name):
+    d = load_golden(name)
+    c = core.Core(d["vertices"], d["panels"], int(d["isym"]), float(d["depth_fd"]), 1025.0, 9.81,
+                  geometry=case_geometry(d), tables=_tables())
+    f = c.freq_setup(float(d["period_fd"]))
+    assert f.finite_depth == 1 and f.nq == int(d["fd_nq"]) and f.nexp == d["fd_ambda"].size
+    out = c.pairs(float(d["period_fd"]), f)
+    scale = np.max(np.abs(d["fd"]), axis=(0, 1))
+    assert np.max(np.abs(out - d["fd"]) / np.where(scale > 0, scale, 1)) < 1e-12
+    c.close()
+    # infinite depth on the same mesh; NQ carried over from the finite-depth call
+    c0 = core.Core(d["vertices"], d["panels"], int(d["isym"]), 0.0, 1025.0, 9.81, geometry=case_geometry(d),
+                   tables=_tables())
+    for key, pk in (("infd1", "period_i1"), ("infd2", "period_i2")):
+        if key not in d:
+            continue
+        fi = c0.freq_setup(float(d[pk]))
+        fi.nq = int(d[key + "_nq"])
+        out = c0.pairs(float(d[pk]), fi)
+        scale = np.max(np.abs(d[key]), axis=(0, 1))
+        assert np.max(np.abs(out - d[key]) / np.where(scale > 0, scale, 1)) < 1e-12, key
+    c0.close()
+
+
+def _case(name):
+    d = load_golden(f"case_{name}.npz")
+    m = nmesh.Mesh(d["vertices"], d["panels"], int(d["isym"]))
+    ref = nio.table_to_forces(d["forces_table"], d["omega"], d["kind"])
+    return d, m, ref
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_forces_match_reference_fp64(name):
+    d, m, ref = _case(name)
+    s = Solver(m, float(d["depth"]), float(d["rho"]), float(d["g"]), precision=core.FP64, tables=_tables())
+    f = s.solve_problems(d["omega"], d["nvel"], d["nds"])
+    s.close()
+    err = _normwise(f, ref)
+    print(f"{name}: FP64 max normwise rel err {err:.3e}")
+    assert err < FP64_TOL
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_forces_match_reference_mixed(name):
+    d, m, ref = _case(name)
+    s = Solver(m, float(d["depth"]), float(d["rho"]), float(d["g"]), precision=core.MIXED, tables=_tables())
+    f = s.solve_problems(d["omega"], d["nvel"], d["nds"])
+    st = s.core.stats()
+    s.close()
+    err = _normwise(f, ref)
+    print(f"{name}: mixed max normwise rel err {err:.3e} (last refinement: {st.refine_iterations} it)")
+    assert err < MIXED_TOL
+
+
+def test_dropin_solve_matches_oracle():
+    """nemoh_core_solve, one problem at a time as SOLVE_POTENTIAL_*_DIRECT is
+    called, against the oracle's ZIGB/ZIGS/ZPB/ZPS; the second problem of a
+    frequency reuses the resident factorisation."""
+    from oracle import pyoracle as po
+    d, m, _ = _case("Cylinder")
+    c = core.Core(d["vertices"], d["panels"], 1, 0.0, float(d["rho"]), float(d["g"]), tables=_tables())
+    c.enable_timing(True)
+    oc = po.Case(d["vertices"], d["panels"], 1, depth=0.0, rho=float(d["rho"]), g=float(d["g"]), tables=_tables())
+    s = oc.session()
+    for j in range(7):
+        period = 2 * np.pi / d["omega"][j]
+        f = c.freq_setup(period)
+        got = c.solve(period, f, d["nvel"][:, j])
+        assert c.stats().cache_hit == (1 if j > 0 else 0)
+        _, zigb, zigs, zpb, zps = s.solve_bvp(period, d["nvel"][:, j])
+        for a, b in zip(got, (zigb, zigs, zpb, zps)):
+            assert np.abs(a - b).max() <= FP64_TOL * max(np.abs(zigb).max(), np.abs(zpb).max())
+    c.close()
+
+
+def test_omega_below_threshold_is_an_error():
+    d, m, _ = _case("barge_fd")
+    c = core.Core(d["vertices"], d["panels"], 0, 0.0, tables=_tables())
+    with pytest.raises(core.NemohError) as e:
+        c.solve_frequency(2 * np.pi / 1e-5, d["nvel"][:, :1])
+    assert e.value.code == core.ERR_OMEGA
+    with pytest.raises(core.NemohError) as e:
+        c.solve_frequency(1.0, np.zeros((d["nvel"].shape[0], 17), complex))
+    assert e.value.code == core.ERR_ARG
+    c.close()
+
+
+def test_full_size_barge_mixed_vs_fp64():
+    """The 4,000-panel barge at h = 50 m: one finite-depth and the
+    infinite-depth (k h >= 20) frequency; mixed within 1e-6 of FP64 on every
+    force, and radiation damping B_ii >= 0."""
+    m = nmesh.box(100.0, 20.0, 10.0, 90, 20, 10)
+    assert m.npanels == 4000
+    modes = bcm.six_dof()
+    om, kind, nvel = bcm.problems(m, [0.9, 2.0], [0.0, 45.0, 90.0], modes, 50.0, 9.81, 1025.0)
+    nds = bcm.integration(m, modes)
+    out = {}
+    for prec in (core.FP64, core.MIXED):
+        s = Solver(m, 50.0, 1025.0, 9.81, precision=prec)
+        out[prec] = s.solve_problems(om, nvel, nds)
+        s.close()
+    err = _normwise(out[core.MIXED], out[core.FP64])
+    print(f"barge 4000: mixed vs FP64 max normwise rel err {err:.3e}")
+    assert err < MIXED_TOL
+    rad = kind == -1.0
+    F = out[core.FP64][:, rad].reshape(6, 2, 6)  # (force dof, omega, mode)
+    for w in range(2):
+        damping = -F[:, w, :].real
+        assert np.all(np.diag(damping) >= -1e-6 * np.abs(damping).max())
