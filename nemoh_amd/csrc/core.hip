@@ -34,6 +34,8 @@ struct nemoh_core {
   double2* D[2] = {nullptr, nullptr};
   double2* S[2] = {nullptr, nullptr};
   float2* D32[2] = {nullptr, nullptr};
+  double2* Dinv[2] = {nullptr, nullptr};  // inverses of the factors' diagonal blocks
+  float2* Dinv32[2] = {nullptr, nullptr};
   double w_prev = -1.0;
   int factored_prec = -1;
   int nq_state = 0;  // module NQ of the standalone session
@@ -85,8 +87,13 @@ int ensure_matrices(nemoh_core* c) {
   for (int b = 0; b < c->nb; ++b) {
     if (int r = dalloc(c, &c->D[b], n2)) return r;
     if (int r = dalloc(c, &c->S[b], n2)) return r;
-    if (c->prec == NEMOH_MIXED)
+    const size_t ninv = 2 * (size_t)((c->nfa + kNB - 1) / kNB) * kNB * kNB;
+    if (c->prec == NEMOH_MIXED) {
       if (int r = dalloc(c, &c->D32[b], n2)) return r;
+      if (int r = dalloc(c, &c->Dinv32[b], ninv)) return r;
+    } else {
+      if (int r = dalloc(c, &c->Dinv[b], ninv)) return r;
+    }
   }
   return NEMOH_OK;
 }
@@ -204,19 +211,20 @@ int prepare_frequency(nemoh_core* c, double w) {
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
   HIPCHK(c, hipMemsetAsync(c->d_info, 0, sizeof(int), c->st));
   for (int b = 0; b < c->nb; ++b) {
-    if (c->prec == NEMOH_MIXED) launch_lu<float>(reinterpret_cast<C2<float>*>(c->D32[b]), c->nfa, c->nfa, c->d_info, c->st);
-    else launch_lu<double>(reinterpret_cast<C2<double>*>(c->D[b]), c->nfa, c->nfa, c->d_info, c->st);
+    if (c->prec == NEMOH_MIXED) {
+      auto* a = reinterpret_cast<C2<float>*>(c->D32[b]);
+      launch_lu<float>(a, c->nfa, c->nfa, c->d_info, c->st);
+      launch_diag_inverse<float>(a, c->nfa, c->nfa, reinterpret_cast<C2<float>*>(c->Dinv32[b]), c->st);
+    } else {
+      auto* a = reinterpret_cast<C2<double>*>(c->D[b]);
+      launch_lu<double>(a, c->nfa, c->nfa, c->d_info, c->st);
+      launch_diag_inverse<double>(a, c->nfa, c->nfa, reinterpret_cast<C2<double>*>(c->Dinv[b]), c->st);
+    }
   }
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
-  if (int r = check_info(c)) return r;
-  if (c->timing) {
-    float a = 0, f = 0;
-    HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-    HIPCHK(c, hipEventElapsedTime(&f, c->ev[1], c->ev[2]));
-    c->stats.ms_assembly = a;
-    c->stats.ms_factor = f;
-  }
+  // the pivot flag is read back with the outputs (check_info), so the host
+  // never waits between factorisation and solves
   c->w_prev = w;
   c->factored_prec = c->prec;
   return NEMOH_OK;
@@ -244,7 +252,8 @@ int solve_block(nemoh_core* c, int b, int nrhs) {
   if (c->prec == NEMOH_FP64) {
     HIPCHK(c, hipMemcpyAsync(c->Z[b], c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
     launch_lu_solve<double>(reinterpret_cast<const C2<double>*>(c->D[b]), n, n,
-                            reinterpret_cast<C2<double>*>(c->Z[b]), reinterpret_cast<C2<double>*>(c->W), nrhs, c->st);
+                            reinterpret_cast<const C2<double>*>(c->Dinv[b]), reinterpret_cast<C2<double>*>(c->Z[b]),
+                            reinterpret_cast<C2<double>*>(c->W), nrhs, c->st);
     HIPCHK(c, hipGetLastError());
     return NEMOH_OK;
   }
@@ -258,7 +267,8 @@ int solve_block(nemoh_core* c, int b, int nrhs) {
   for (; it < c->max_iter; ++it) {
     if (rmax == 0.0) break;
     launch_f64_to_f32(c->R, c->R32, cnt, 1.0 / rmax, c->st);
-    launch_lu_solve<float>(reinterpret_cast<const C2<float>*>(c->D32[b]), n, n, reinterpret_cast<C2<float>*>(c->R32),
+    launch_lu_solve<float>(reinterpret_cast<const C2<float>*>(c->D32[b]), n, n,
+                           reinterpret_cast<const C2<float>*>(c->Dinv32[b]), reinterpret_cast<C2<float>*>(c->R32),
                            reinterpret_cast<C2<float>*>(c->W32), nrhs, c->st);
     launch_f32_to_f64(c->R32, c->Z[b], cnt, rmax, it > 0, c->st);
     HIPCHK(c, hipMemcpyAsync(c->R, c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
@@ -370,7 +380,15 @@ static int finish_outputs(nemoh_core* c, int nrhs, double* pressure, int nint, c
       }
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
+  if (int r = check_info(c)) return r;
   if (c->timing) {
+    if (!c->stats.cache_hit) {
+      float a = 0, f = 0;
+      HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+      HIPCHK(c, hipEventElapsedTime(&f, c->ev[1], c->ev[2]));
+      c->stats.ms_assembly = a;
+      c->stats.ms_factor = f;
+    }
     float s = 0, t = 0;
     HIPCHK(c, hipEventElapsedTime(&s, c->ev[2], c->ev[3]));
     c->stats.ms_solve = s;
@@ -480,6 +498,8 @@ void nemoh_core_destroy(nemoh_core* c) {
     release(c->D[b]);
     release(c->S[b]);
     release(c->D32[b]);
+    release(c->Dinv[b]);
+    release(c->Dinv32[b]);
     release(c->Z[b]);
     release(c->F[b]);
   }
@@ -548,6 +568,7 @@ int nemoh_core_solve(nemoh_core* c, double period, double amh, int nexp, const d
   HIPCHK(c, hipMemcpyAsync(s.data(), c->sig, s.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipMemcpyAsync(p.data(), c->pot, p.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  if (int r = check_info(c)) return r;
   if (zigb) std::memcpy(zigb, s.data(), 2 * n * sizeof(double));
   if (zigs) std::memcpy(zigs, s.data() + 2 * n, 2 * n * sizeof(double));
   if (zpb) std::memcpy(zpb, p.data(), 2 * n * sizeof(double));

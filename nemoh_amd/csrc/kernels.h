@@ -31,9 +31,14 @@ void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, in
 void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, double* out, hipStream_t st);
 template <typename T>
 void launch_lu(C2<T>* A, int ld, int n, int* info, hipStream_t st);
+// Inverses of the factors' 64x64 diagonal blocks: inv[kb][0] = L_kk^-1,
+// inv[kb][1] = U_kk^-1 (2 * ceil(n/64) * 64 * 64 elements).
+template <typename T>
+void launch_diag_inverse(const C2<T>* A, int ld, int n, C2<T>* inv, hipStream_t st);
 // Solves (LU) X = B in place: on return B holds the solution, X is scratch.
 template <typename T>
-void launch_lu_solve(const C2<T>* A, int ld, int n, C2<T>* B, C2<T>* X, int nrhs, hipStream_t st);
+void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* inv, C2<T>* B, C2<T>* X, int nrhs,
+                     hipStream_t st);
 template <typename TM>
 void launch_matvec(const C2<TM>* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
                    double beta, hipStream_t st);

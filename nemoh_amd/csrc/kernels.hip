@@ -98,171 +98,388 @@ __global__ void pairs_kernel(Geometry G, Tables T, const FreqParams* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// LU panel step.  LDS holds the 64x64 diagonal block (factored in place by
-// every workgroup) and one 64x64 panel block.
+// LU panel step.  A 256-thread workgroup holds a 64x64 complex tile in
+// registers, thread (ty, tx) = (tid / 16, tid % 16) owning the 4x4 strided
+// elements (ty + 16 i, tx + 16 j).  A sequential sweep over k publishes the
+// pivot row / column of step k through double-buffered LDS vectors, so each
+// step costs one barrier and 16 complex FMAs per thread.
 template <typename T>
-__device__ __forceinline__ void lds_factor(C2<T> (*a)[kNB + 1], int nb) {
-  const int tid = threadIdx.x;
-  for (int k = 0; k < nb; ++k) {
-    __syncthreads();
-    const C2<T> piv = a[k][k];
-    for (int r = k + 1 + tid; r < nb; r += blockDim.x) a[r][k] = cdiv2(a[r][k], piv);
-    __syncthreads();
-    const int w = nb - k - 1;
-    for (int e = tid; e < w * w; e += blockDim.x) {
-      const int r = k + 1 + e / w, c = k + 1 + e % w;
-      a[r][c] = csub_mul(a[r][c], a[r][k], a[k][c]);
+struct PanelLds {
+  C2<T> d[kNB][kNB + 1];  // factored diagonal block
+  C2<T> col[2][kNB];
+  C2<T> row[2][kNB];
+};
+
+template <typename T>
+__device__ __forceinline__ C2<T> crecip(C2<T> p) {
+  const T d = p.re * p.re + p.im * p.im;
+  return {p.re / d, -p.im / d};
+}
+
+template <typename T>
+__device__ __forceinline__ void load_tile(C2<T> a[4][4], const C2<T>* __restrict__ A, int ld, int r0, int c0, int nr,
+                                          int nc, int ty, int tx) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = ty + 16 * i, c = tx + 16 * j;
+      a[i][j] = (r < nr && c < nc) ? A[(size_t)(r0 + r) * ld + c0 + c] : C2<T>{0, 0};
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_tile(const C2<T> a[4][4], C2<T>* __restrict__ A, int ld, int r0, int c0, int nr,
+                                           int nc, int ty, int tx) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = ty + 16 * i, c = tx + 16 * j;
+      if (r < nr && c < nc) A[(size_t)(r0 + r) * ld + c0 + c] = a[i][j];
+    }
+}
+
+// In-register LU without pivoting of the nb x nb leading part of the tile.
+// The sweep index k = 16 kk + kl runs with kk unrolled so that every
+// register index is a compile-time constant.
+template <typename T>
+__device__ __forceinline__ void reg_factor(C2<T> a[4][4], int nb, PanelLds<T>& L, int ty, int tx) {
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    for (int kl = 0; kl < 16; ++kl) {
+      const int k = 16 * kk + kl;
+      if (k >= nb) break;
+      const int buf = k & 1;
+      if (tx == kl) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = ty + 16 * i;
+          if (r > k && r < nb) L.col[buf][r] = a[i][kk];
+        }
+      }
+      if (ty == kl) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int c = tx + 16 * j;
+          if (c >= k && c < nb) L.row[buf][c] = a[kk][j];
+        }
+      }
+      __syncthreads();
+      const C2<T> inv = crecip(L.row[buf][k]);
+      C2<T> lr[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = ty + 16 * i;
+        lr[i] = (r > k && r < nb) ? cmul(L.col[buf][r], inv) : C2<T>{0, 0};
+        if (tx == kl && r > k && r < nb) a[i][kk] = lr[i];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = tx + 16 * j;
+        if (c > k && c < nb) {
+          const C2<T> u = L.row[buf][c];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) a[i][j] = csub_mul(a[i][j], lr[i], u);
+        }
+      }
     }
   }
-  __syncthreads();
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void lu_panel_kernel(C2<T>* __restrict__ A, int ld, int n, int k0,
                                                        int* __restrict__ info) {
-  __shared__ C2<T> d[kNB][kNB + 1];
-  __shared__ C2<T> b[kNB][kNB + 1];
+  __shared__ PanelLds<T> L;
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
   const int nb = min(kNB, n - k0);
-  const int tid = threadIdx.x;
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int r = e / nb, c = e % nb;
-    d[r][c] = A[(size_t)(k0 + r) * ld + k0 + c];
-  }
-  lds_factor(d, nb);
+  C2<T> a[4][4];
+  load_tile(a, A, ld, k0, k0, nb, nb, ty, tx);
+  reg_factor(a, nb, L, ty, tx);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) L.d[ty + 16 * i][tx + 16 * j] = a[i][j];
   const int rest = n - k0 - nb;
   const int nblk = (rest + kNB - 1) / kNB;
   const int bid = blockIdx.x;
   if (bid == 0) {
-    for (int e = tid; e < nb * nb; e += blockDim.x) {
-      const int r = e / nb, c = e % nb;
-      A[(size_t)(k0 + r) * ld + k0 + c] = d[r][c];
-    }
-    if (tid == 0) {
-      for (int k = 0; k < nb; ++k) {
-        const C2<T> p = d[k][k];
-        if (!(sqrt((double)p.re * p.re + (double)p.im * p.im) > 1e-20)) atomicMax(info, k0 + k + 1);
+    store_tile(a, A, ld, k0, k0, nb, nb, ty, tx);
+    if (tx == ty) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = ty + 16 * i;
+        const C2<T> p = a[i][i];
+        if (r < nb && !(sqrt((double)p.re * p.re + (double)p.im * p.im) > 1e-20)) atomicMax(info, k0 + r + 1);
       }
     }
   }
+  __syncthreads();
   if (bid >= 2 * nblk) return;
+  C2<T> b[4][4];
   if (bid < nblk) {
-    // rows r0..r0+m of L21 = A21 U11^-1 : column sweep x_k = a_k / u_kk,
-    // a_c -= x_k u_kc
+    // 64 rows of L21 = A21 U11^-1: column sweep x_k = b_k / u_kk, b_c -= x_k u_kc
     const int r0 = k0 + nb + bid * kNB;
     const int m = min(kNB, n - r0);
-    for (int e = tid; e < m * nb; e += blockDim.x) {
-      const int r = e / nb, c = e % nb;
-      b[r][c] = A[(size_t)(r0 + r) * ld + k0 + c];
-    }
-    for (int k = 0; k < nb; ++k) {
-      __syncthreads();
-      for (int r = tid; r < m; r += blockDim.x) b[r][k] = cdiv2(b[r][k], d[k][k]);
-      __syncthreads();
-      const int w = nb - k - 1;
-      for (int e = tid; e < m * w; e += blockDim.x) {
-        const int r = e / w, c = k + 1 + e % w;
-        b[r][c] = csub_mul(b[r][c], b[r][k], d[k][c]);
+    load_tile(b, A, ld, r0, k0, m, nb, ty, tx);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      for (int kl = 0; kl < 16; ++kl) {
+        const int k = 16 * kk + kl;
+        if (k >= nb) break;
+        const int buf = k & 1;
+        if (tx == kl) {
+          const C2<T> inv = crecip(L.d[k][k]);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            b[i][kk] = cmul(b[i][kk], inv);
+            L.col[buf][ty + 16 * i] = b[i][kk];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int c = tx + 16 * j;
+          if (c > k && c < nb) {
+            const C2<T> u = L.d[k][c];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) b[i][j] = csub_mul(b[i][j], L.col[buf][ty + 16 * i], u);
+          }
+        }
       }
     }
-    __syncthreads();
-    for (int e = tid; e < m * nb; e += blockDim.x) {
-      const int r = e / nb, c = e % nb;
-      A[(size_t)(r0 + r) * ld + k0 + c] = b[r][c];
-    }
+    store_tile(b, A, ld, r0, k0, m, nb, ty, tx);
   } else {
-    // columns c0..c0+m of U12 = L11^-1 A12 (unit lower)
+    // 64 columns of U12 = L11^-1 A12 (unit lower): b_r -= l_rk b_k
     const int c0 = k0 + nb + (bid - nblk) * kNB;
     const int m = min(kNB, n - c0);
-    for (int e = tid; e < nb * m; e += blockDim.x) {
-      const int r = e / m, c = e % m;
-      b[r][c] = A[(size_t)(k0 + r) * ld + c0 + c];
-    }
-    for (int k = 0; k < nb; ++k) {
-      __syncthreads();
-      const int w = nb - k - 1;
-      for (int e = tid; e < w * m; e += blockDim.x) {
-        const int r = k + 1 + e / m, c = e % m;
-        b[r][c] = csub_mul(b[r][c], d[r][k], b[k][c]);
+    load_tile(b, A, ld, k0, c0, nb, m, ty, tx);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      for (int kl = 0; kl < 16; ++kl) {
+        const int k = 16 * kk + kl;
+        if (k >= nb) break;
+        const int buf = k & 1;
+        if (ty == kl) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) L.row[buf][tx + 16 * j] = b[kk][j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = ty + 16 * i;
+          if (r > k && r < nb) {
+            const C2<T> l = L.d[r][k];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[i][j] = csub_mul(b[i][j], l, L.row[buf][tx + 16 * j]);
+          }
+        }
       }
     }
-    __syncthreads();
-    for (int e = tid; e < nb * m; e += blockDim.x) {
-      const int r = e / m, c = e % m;
-      A[(size_t)(k0 + r) * ld + c0 + c] = b[r][c];
-    }
+    store_tile(b, A, ld, k0, c0, nb, m, ty, tx);
   }
 }
 
+// Inverses of the diagonal blocks of the factors, one workgroup per block:
+// inv[kb][0] = L_kk^-1 (unit lower), inv[kb][1] = U_kk^-1, each 64x64
+// row-major.  They turn each block step of the triangular solves into a
+// small matrix product.
+template <typename T>
+__global__ __launch_bounds__(256) void diag_inverse_kernel(const C2<T>* __restrict__ A, int ld, int n,
+                                                           C2<T>* __restrict__ inv) {
+  __shared__ PanelLds<T> L;
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  const int kb = blockIdx.x, k0 = kb * kNB;
+  const int nb = min(kNB, n - k0);
+  {
+    C2<T> a[4][4];
+    load_tile(a, A, ld, k0, k0, nb, nb, ty, tx);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) L.d[ty + 16 * i][tx + 16 * j] = a[i][j];
+  }
+  __syncthreads();
+  C2<T> x[4][4];
+  // L X = I, forward sweep over rows
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = ty + 16 * i, c = tx + 16 * j;
+      x[i][j] = C2<T>{(r == c && r < nb) ? T(1) : T(0), T(0)};
+    }
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    for (int kl = 0; kl < 16; ++kl) {
+      const int k = 16 * kk + kl;
+      if (k >= nb) break;
+      const int buf = k & 1;
+      if (ty == kl) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) L.row[buf][tx + 16 * j] = x[kk][j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = ty + 16 * i;
+        if (r > k && r < nb) {
+          const C2<T> l = L.d[r][k];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) x[i][j] = csub_mul(x[i][j], l, L.row[buf][tx + 16 * j]);
+        }
+      }
+    }
+  }
+  C2<T>* o = inv + (size_t)kb * 2 * kNB * kNB;
+  store_tile(x, o, kNB, 0, 0, kNB, kNB, ty, tx);
+  __syncthreads();  // the backward sweep reuses the row buffers
+  // U X = I, backward sweep
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = ty + 16 * i, c = tx + 16 * j;
+      x[i][j] = C2<T>{(r == c && r < nb) ? T(1) : T(0), T(0)};
+    }
+#pragma unroll
+  for (int kk = 3; kk >= 0; --kk) {
+    for (int kl = 15; kl >= 0; --kl) {
+      const int k = 16 * kk + kl;
+      if (k >= nb) continue;
+      const int buf = k & 1;
+      if (ty == kl) {
+        const C2<T> d = crecip(L.d[k][k]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          x[kk][j] = cmul(x[kk][j], d);
+          L.row[buf][tx + 16 * j] = x[kk][j];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = ty + 16 * i;
+        if (r < k) {
+          const C2<T> u = L.d[r][k];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) x[i][j] = csub_mul(x[i][j], u, L.row[buf][tx + 16 * j]);
+        }
+      }
+    }
+  }
+  store_tile(x, o + kNB * kNB, kNB, 0, 0, kNB, kNB, ty, tx);
+}
+
 // Trailing update A22 -= L21 U12 (K = 64), 64x64 output tile per workgroup,
-// 4 waves each owning a 32x32 quadrant = 2x2 MFMA 16x16 tiles, complex
-// product as Cr += Ar Br + (-Ai) Bi, Ci += Ar Bi + Ai Br.
-// LDS keeps the operands k-major and split into real / imaginary planes.
+// 4 waves each owning a 32x32 quadrant = 2x2 MFMA 16x16 tiles; the complex
+// product is four real MFMAs, Cr += (-Ar) Br + Ai Bi, Ci += (-Ar) Bi +
+// (-Ai) Br, with the accumulators initialised from C (so C is read once, at
+// the start, and written once).  K streams through LDS in chunks of 16 with
+// double buffering: the global loads of chunk c+1 are in flight while chunk
+// c feeds the MFMAs.  LDS keeps the operands k-major, split into real and
+// imaginary planes, A already negated.
+template <typename T>
+struct UpdLds {
+  T ar[2][16][64], ai[2][16][64], br[2][16][64], bi[2][16][64];
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void lu_update_kernel(C2<T>* __restrict__ A, int ld, int n, int k0) {
   typedef typename MfmaTraits<T>::acc_t acc_t;
-  __shared__ T ar[kNB][kNB], ai[kNB][kNB], br[kNB][kNB], bi[kNB][kNB];
-  const int s = k0 + kNB;  // first row / column of the trailing matrix
-  const int row0 = s + blockIdx.y * 64, col0 = s + blockIdx.x * 64;
+  __shared__ UpdLds<T> S;
+  const int s0 = k0 + kNB;  // first row / column of the trailing matrix
+  const int row0 = s0 + blockIdx.y * 64, col0 = s0 + blockIdx.x * 64;
   const int tid = threadIdx.x;
-  // stage A rows row0.. (columns k0..k0+63) and B rows k0.. (columns col0..)
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int r = e >> 6, c = e & 63;
-    C2<T> va = {0, 0}, vb = {0, 0};
-    if (row0 + r < n) va = A[(size_t)(row0 + r) * ld + k0 + c];
-    if (col0 + c < n) vb = A[(size_t)(k0 + r) * ld + col0 + c];
-    ar[c][r] = va.re;
-    ai[c][r] = va.im;
-    br[r][c] = vb.re;
-    bi[r][c] = vb.im;
-  }
-  __syncthreads();
   const int wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
+  // accumulators <- C
   acc_t cr[2][2], ci[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      cr[a][b] = acc_t{0, 0, 0, 0};
-      ci[a][b] = acc_t{0, 0, 0, 0};
-    }
-#pragma unroll 4
-  for (int k = 0; k < kNB; k += 4) {
-    T xr[2], xi[2], yr[2], yi[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      xr[a] = ar[k + lk][wr + a * 16 + li];
-      xi[a] = ai[k + lk][wr + a * 16 + li];
-      yr[a] = br[k + lk][wc + a * 16 + li];
-      yi[a] = bi[k + lk][wc + a * 16 + li];
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        cr[a][b] = MfmaTraits<T>::mma(xr[a], yr[b], cr[a][b]);
-        cr[a][b] = MfmaTraits<T>::mma(-xi[a], yi[b], cr[a][b]);
-        ci[a][b] = MfmaTraits<T>::mma(xr[a], yi[b], ci[a][b]);
-        ci[a][b] = MfmaTraits<T>::mma(xi[a], yr[b], ci[a][b]);
-      }
-  }
-  // D layout of the 16x16 tile: lane l, register q -> row 4*(l/16)+q, col l%16
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int r = row0 + wr + a * 16 + 4 * lk + q;
+        const int r = row0 + wr + a * 16 + MfmaTraits<T>::row(lk, q);
         const int c = col0 + wc + b * 16 + li;
-        if (r < n && c < n) {
-          C2<T>* p = A + (size_t)r * ld + c;
-          C2<T> v = *p;
-          v.re -= cr[a][b][q];
-          v.im -= ci[a][b][q];
-          *p = v;
+        C2<T> v = {0, 0};
+        if (r < n && c < n) v = A[(size_t)r * ld + c];
+        cr[a][b][q] = v.re;
+        ci[a][b][q] = v.im;
+      }
+  // staging map: A chunk = 64 rows x 16 k, B chunk = 16 k x 64 cols; each
+  // thread moves 4 elements of each
+  C2<T> pa[4], pb[4];
+  auto fetch = [&](int chunk) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      const int ar_ = e >> 4, ak = e & 15;  // A: row, k (16 consecutive k per row)
+      const int bk = e >> 6, bc = e & 63;   // B: k, col
+      pa[u] = (row0 + ar_ < n) ? A[(size_t)(row0 + ar_) * ld + k0 + chunk * 16 + ak] : C2<T>{0, 0};
+      pb[u] = (col0 + bc < n) ? A[(size_t)(k0 + chunk * 16 + bk) * ld + col0 + bc] : C2<T>{0, 0};
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      const int ar_ = e >> 4, ak = e & 15;
+      const int bk = e >> 6, bc = e & 63;
+      S.ar[buf][ak][ar_] = -pa[u].re;
+      S.ai[buf][ak][ar_] = -pa[u].im;
+      S.br[buf][bk][bc] = pb[u].re;
+      S.bi[buf][bk][bc] = pb[u].im;
+    }
+  };
+  fetch(0);
+  stash(0);
+  __syncthreads();
+#pragma unroll
+  for (int chunk = 0; chunk < 4; ++chunk) {
+    const int buf = chunk & 1;
+    if (chunk < 3) fetch(chunk + 1);
+#pragma unroll
+    for (int k = 0; k < 16; k += 4) {
+      T xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        xr[a] = S.ar[buf][k + lk][wr + a * 16 + li];
+        xi[a] = S.ai[buf][k + lk][wr + a * 16 + li];
+        yr[a] = S.br[buf][k + lk][wc + a * 16 + li];
+        yi[a] = S.bi[buf][k + lk][wc + a * 16 + li];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          cr[a][b] = MfmaTraits<T>::mma(xr[a], yr[b], cr[a][b]);
+          ci[a][b] = MfmaTraits<T>::mma(xr[a], yi[b], ci[a][b]);
         }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          cr[a][b] = MfmaTraits<T>::mma(-xi[a], yi[b], cr[a][b]);
+          ci[a][b] = MfmaTraits<T>::mma(xi[a], yr[b], ci[a][b]);
+        }
+    }
+    if (chunk < 3) {
+      stash(buf ^ 1);
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = row0 + wr + a * 16 + MfmaTraits<T>::row(lk, q);
+        const int c = col0 + wc + b * 16 + li;
+        if (r < n && c < n) A[(size_t)r * ld + c] = C2<T>{cr[a][b][q], ci[a][b][q]};
       }
 }
 
@@ -275,76 +492,62 @@ __global__ void mfma_probe_kernel(const T* __restrict__ a, const T* __restrict__
   const int li = lane & 15, lk = lane >> 4;
   acc_t c = acc_t{0, 0, 0, 0};
   c = MfmaTraits<T>::mma(a[li * 4 + lk], b[lk * 16 + li], c);
-  for (int q = 0; q < 4; ++q) d[(4 * lk + q) * 16 + li] = c[q];
+  for (int q = 0; q < 4; ++q) d[MfmaTraits<T>::row(lk, q) * 16 + li] = c[q];
 }
 
 // ---------------------------------------------------------------------------
-// Multi-RHS triangular solves, one 64-row block per launch.  B (n x nrhs,
-// row-major) holds the right-hand sides being updated; X receives the solved
-// block.  Every workgroup solves the diagonal block redundantly from B (never
-// written in the same launch), workgroup 0 stores it, the others update their
-// 64-row block of B.
+// Multi-RHS triangular solves, one 64-row block step per launch.  B (n x
+// nrhs, row-major) holds the right-hand sides being updated, X receives the
+// solved block.  Every workgroup forms x_k = inv_kk b_k (inverse diagonal
+// block from diag_inverse_kernel) from B, which this launch never writes;
+// workgroup 0 stores x_k, the others update their 64-row block of B.
 template <typename T, bool kLower>
 __global__ __launch_bounds__(256) void trsv_kernel(const C2<T>* __restrict__ A, int ld, int n, int kb,
-                                                   C2<T>* __restrict__ B, C2<T>* __restrict__ X, int nrhs) {
-  __shared__ C2<T> d[kNB][kNB + 1];
-  __shared__ C2<T> x[kNB][kMaxRhs];
+                                                   const C2<T>* __restrict__ inv, C2<T>* __restrict__ B,
+                                                   C2<T>* __restrict__ X, int nrhs) {
+  __shared__ C2<T> t[kNB][kNB + 1];
+  __shared__ C2<T> sb[kNB][kMaxRhs];
+  __shared__ C2<T> sx[kNB][kMaxRhs];
   const int tid = threadIdx.x;
   const int k0 = kb * kNB;
   const int nb = min(kNB, n - k0);
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int r = e / nb, c = e % nb;
-    d[r][c] = A[(size_t)(k0 + r) * ld + k0 + c];
-  }
-  for (int e = tid; e < nb * nrhs; e += blockDim.x) {
-    const int r = e / nrhs, q = e % nrhs;
-    x[r][q] = B[(size_t)(k0 + r) * nrhs + q];
+  const C2<T>* iv = inv + ((size_t)kb * 2 + (kLower ? 0 : 1)) * kNB * kNB;
+  for (int e = tid; e < kNB * kNB; e += 256) t[e >> 6][e & 63] = iv[e];
+  for (int e = tid; e < kNB * kMaxRhs; e += 256) {
+    const int r = e / kMaxRhs, q = e % kMaxRhs;
+    sb[r][q] = (r < nb && q < nrhs) ? B[(size_t)(k0 + r) * nrhs + q] : C2<T>{0, 0};
   }
   __syncthreads();
-  if (kLower) {  // unit lower: forward
-    for (int k = 0; k < nb; ++k) {
-      for (int e = tid; e < (nb - k - 1) * nrhs; e += blockDim.x) {
-        const int r = k + 1 + e / nrhs, q = e % nrhs;
-        x[r][q] = csub_mul(x[r][q], d[r][k], x[k][q]);
-      }
-      __syncthreads();
+  for (int e = tid; e < kNB * kMaxRhs; e += 256) {
+    const int r = e / kMaxRhs, q = e % kMaxRhs;
+    C2<T> acc = {0, 0};
+    for (int m = 0; m < nb; ++m) {
+      const C2<T> p = cmul(t[r][m], sb[m][q]);
+      acc.re += p.re;
+      acc.im += p.im;
     }
-  } else {  // upper: backward
-    for (int k = nb - 1; k >= 0; --k) {
-      for (int q = tid; q < nrhs; q += blockDim.x) x[k][q] = cdiv2(x[k][q], d[k][k]);
-      __syncthreads();
-      for (int e = tid; e < k * nrhs; e += blockDim.x) {
-        const int r = e / nrhs, q = e % nrhs;
-        x[r][q] = csub_mul(x[r][q], d[r][k], x[k][q]);
-      }
-      __syncthreads();
-    }
+    sx[r][q] = acc;
   }
+  __syncthreads();
   if (blockIdx.x == 0) {
-    for (int e = tid; e < nb * nrhs; e += blockDim.x) {
+    for (int e = tid; e < nb * nrhs; e += 256) {
       const int r = e / nrhs, q = e % nrhs;
-      X[(size_t)(k0 + r) * nrhs + q] = x[r][q];
+      X[(size_t)(k0 + r) * nrhs + q] = sx[r][q];
     }
     return;
   }
-  // rows of the other blocks: below (lower) or above (upper) the diagonal
-  int r0;
-  if (kLower) r0 = k0 + nb + (blockIdx.x - 1) * kNB;
-  else r0 = (blockIdx.x - 1) * kNB;
-  const int rend = kLower ? n : k0;
-  const int m = min(kNB, rend - r0);
+  const int r0 = kLower ? k0 + nb + (blockIdx.x - 1) * kNB : (blockIdx.x - 1) * kNB;
+  const int m = min(kNB, (kLower ? n : k0) - r0);
   if (m <= 0) return;
-  __syncthreads();
-  // reuse d for the off-diagonal tile A[r0.., k0..]
-  for (int e = tid; e < m * nb; e += blockDim.x) {
-    const int r = e / nb, c = e % nb;
-    d[r][c] = A[(size_t)(r0 + r) * ld + k0 + c];
+  for (int e = tid; e < kNB * kNB; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    t[r][c] = (r < m && c < nb) ? A[(size_t)(r0 + r) * ld + k0 + c] : C2<T>{0, 0};
   }
   __syncthreads();
-  for (int e = tid; e < m * nrhs; e += blockDim.x) {
+  for (int e = tid; e < m * nrhs; e += 256) {
     const int r = e / nrhs, q = e % nrhs;
     C2<T> acc = B[(size_t)(r0 + r) * nrhs + q];
-    for (int k = 0; k < nb; ++k) acc = csub_mul(acc, d[r][k], x[k][q]);
+    for (int k = 0; k < nb; ++k) acc = csub_mul(acc, t[r][k], sx[k][q]);
     B[(size_t)(r0 + r) * nrhs + q] = acc;
   }
 }
@@ -540,19 +743,28 @@ template void launch_lu<double>(C2<double>*, int, int, int*, hipStream_t);
 template void launch_lu<float>(C2<float>*, int, int, int*, hipStream_t);
 
 template <typename T>
-void launch_lu_solve(const C2<T>* A, int ld, int n, C2<T>* B, C2<T>* X, int nrhs, hipStream_t st) {
+void launch_diag_inverse(const C2<T>* A, int ld, int n, C2<T>* inv, hipStream_t st) {
+  diag_inverse_kernel<T><<<(n + kNB - 1) / kNB, 256, 0, st>>>(A, ld, n, inv);
+}
+template void launch_diag_inverse<double>(const C2<double>*, int, int, C2<double>*, hipStream_t);
+template void launch_diag_inverse<float>(const C2<float>*, int, int, C2<float>*, hipStream_t);
+
+template <typename T>
+void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* inv, C2<T>* B, C2<T>* X, int nrhs, hipStream_t st) {
   const int nblk = (n + kNB - 1) / kNB;
   for (int kb = 0; kb < nblk; ++kb) {
     const int below = nblk - kb - 1;
-    trsv_kernel<T, true><<<1 + below, 256, 0, st>>>(A, ld, n, kb, B, X, nrhs);
+    trsv_kernel<T, true><<<1 + below, 256, 0, st>>>(A, ld, n, kb, inv, B, X, nrhs);
   }
   // X now holds y = L^-1 b; backward sweep reads it as the right-hand side
   for (int kb = nblk - 1; kb >= 0; --kb) {
-    trsv_kernel<T, false><<<1 + kb, 256, 0, st>>>(A, ld, n, kb, X, B, nrhs);
+    trsv_kernel<T, false><<<1 + kb, 256, 0, st>>>(A, ld, n, kb, inv, X, B, nrhs);
   }
 }
-template void launch_lu_solve<double>(const C2<double>*, int, int, C2<double>*, C2<double>*, int, hipStream_t);
-template void launch_lu_solve<float>(const C2<float>*, int, int, C2<float>*, C2<float>*, int, hipStream_t);
+template void launch_lu_solve<double>(const C2<double>*, int, int, const C2<double>*, C2<double>*, C2<double>*, int,
+                                      hipStream_t);
+template void launch_lu_solve<float>(const C2<float>*, int, int, const C2<float>*, C2<float>*, C2<float>*, int,
+                                     hipStream_t);
 
 template <typename TM>
 void launch_matvec(const C2<TM>* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,

@@ -44,12 +44,16 @@ struct MfmaTraits;
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// 16x16x4 MFMA operands: lane l holds A[l%16][l/16] and B[l/16][l%16];
+// the 16x16 result is spread as 4 registers per lane, column l%16, with the
+// row given by row(l/16, q) -- the FP64 and FP32 instructions differ here.
 template <>
 struct MfmaTraits<double> {
   typedef v4d acc_t;
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int q) { return lk + 4 * q; }
 };
 template <>
 struct MfmaTraits<float> {
@@ -57,6 +61,7 @@ struct MfmaTraits<float> {
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int q) { return 4 * lk + q; }
 };
 
 }  // namespace nemoh
