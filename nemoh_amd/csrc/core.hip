@@ -53,7 +53,10 @@ struct nemoh_core {
   double2* d_force = nullptr;
   int force_cap = 0;
   double* d_red = nullptr;  // reduction scratch
-  int* d_info = nullptr;
+  int* d_info = nullptr;     // [0] first bad pivot row, [1] solve wait timeout
+  int* d_flags = nullptr;    // block flags of the single-launch triangular solves
+  int epoch = 0;
+  int cu_count = 0;
   // statistics
   bool timing = false;
   nemoh_stats stats{};
@@ -180,9 +183,14 @@ int build_params(nemoh_core* c, double period, double amh, int nexp, const doubl
 }
 
 int check_info(nemoh_core* c) {
-  int info = 0;
-  HIPCHK(c, hipMemcpyAsync(&info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  int h[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(h, c->d_info, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  if (h[1]) {
+    c->w_prev = -1.0;
+    return fail(c, NEMOH_ERR_HIP, "triangular solve: a bounded wait for a solved block timed out");
+  }
+  const int info = h[0];
   if (info) {
     c->w_prev = -1.0;
     char buf[96];
@@ -209,7 +217,7 @@ int prepare_frequency(nemoh_core* c, double w) {
   launch_assemble(c->G, c->T, c->d_par, c->nsym, out, c->st);
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
-  HIPCHK(c, hipMemsetAsync(c->d_info, 0, sizeof(int), c->st));
+  HIPCHK(c, hipMemsetAsync(c->d_info, 0, 2 * sizeof(int), c->st));
   for (int b = 0; b < c->nb; ++b) {
     if (c->prec == NEMOH_MIXED) {
       auto* a = reinterpret_cast<C2<float>*>(c->D32[b]);
@@ -244,6 +252,21 @@ double device_absmax(nemoh_core* c, const double2* a, size_t count, int* err) {
   return m;
 }
 
+// (LU) x = b in place in B (X scratch): one launch per triangle when every
+// row block can have its own CU, else one launch per block step.
+template <typename T>
+void lu_solve(nemoh_core* c, const C2<T>* A, const C2<T>* inv, C2<T>* B, C2<T>* X, int nrhs) {
+  const int n = c->nfa;
+  const int nblk = (n + kNB - 1) / kNB;
+  if (nblk <= c->cu_count) {
+    ++c->epoch;
+    // about a second of polling before a wait gives up
+    launch_lu_solve_persistent<T>(A, n, n, inv, B, X, nrhs, c->d_flags, c->epoch, c->d_info + 1, 1 << 22, c->st);
+  } else {
+    launch_lu_solve<T>(A, n, n, inv, B, X, nrhs, c->st);
+  }
+}
+
 // Solve block b for the right-hand sides in c->B (n x nrhs, row-major):
 // solution into c->Z[b].
 int solve_block(nemoh_core* c, int b, int nrhs) {
@@ -251,9 +274,8 @@ int solve_block(nemoh_core* c, int b, int nrhs) {
   const size_t cnt = (size_t)n * nrhs;
   if (c->prec == NEMOH_FP64) {
     HIPCHK(c, hipMemcpyAsync(c->Z[b], c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
-    launch_lu_solve<double>(reinterpret_cast<const C2<double>*>(c->D[b]), n, n,
-                            reinterpret_cast<const C2<double>*>(c->Dinv[b]), reinterpret_cast<C2<double>*>(c->Z[b]),
-                            reinterpret_cast<C2<double>*>(c->W), nrhs, c->st);
+    lu_solve(c, reinterpret_cast<const C2<double>*>(c->D[b]), reinterpret_cast<const C2<double>*>(c->Dinv[b]),
+             reinterpret_cast<C2<double>*>(c->Z[b]), reinterpret_cast<C2<double>*>(c->W), nrhs);
     HIPCHK(c, hipGetLastError());
     return NEMOH_OK;
   }
@@ -267,9 +289,8 @@ int solve_block(nemoh_core* c, int b, int nrhs) {
   for (; it < c->max_iter; ++it) {
     if (rmax == 0.0) break;
     launch_f64_to_f32(c->R, c->R32, cnt, 1.0 / rmax, c->st);
-    launch_lu_solve<float>(reinterpret_cast<const C2<float>*>(c->D32[b]), n, n,
-                           reinterpret_cast<const C2<float>*>(c->Dinv32[b]), reinterpret_cast<C2<float>*>(c->R32),
-                           reinterpret_cast<C2<float>*>(c->W32), nrhs, c->st);
+    lu_solve(c, reinterpret_cast<const C2<float>*>(c->D32[b]), reinterpret_cast<const C2<float>*>(c->Dinv32[b]),
+             reinterpret_cast<C2<float>*>(c->R32), reinterpret_cast<C2<float>*>(c->W32), nrhs);
     launch_f32_to_f64(c->R32, c->Z[b], cnt, rmax, it > 0, c->st);
     HIPCHK(c, hipMemcpyAsync(c->R, c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
     launch_matvec<double>(reinterpret_cast<const C2<double>*>(c->D[b]), n, n, c->Z[b], nrhs, c->R, -1.0, 1.0,
@@ -460,8 +481,18 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
       hipMalloc((void**)&c->d_xz, kNZ * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_par, sizeof(FreqParams)) != hipSuccess ||
       hipMalloc((void**)&c->d_red, 64 * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&c->d_info, sizeof(int)) != hipSuccess)
+      hipMalloc((void**)&c->d_info, 2 * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&c->d_flags, 2 * sizeof(int) * ((nfa + kNB - 1) / kNB)) != hipSuccess)
     return bail(NEMOH_ERR_HIP);
+  if (hipMemset(c->d_info, 0, 2 * sizeof(int)) != hipSuccess ||
+      hipMemset(c->d_flags, 0, 2 * sizeof(int) * ((nfa + kNB - 1) / kNB)) != hipSuccess)
+    return bail(NEMOH_ERR_HIP);
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return bail(NEMOH_ERR_HIP);
+    c->cu_count = prop.multiProcessorCount;
+  }
   if (hipMemcpy(c->d_geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_vtx, vtx.data(), vtx.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -492,6 +523,7 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->d_par);
   release(c->d_red);
   release(c->d_info);
+  release(c->d_flags);
   release(c->d_nds);
   release(c->d_force);
   for (int b = 0; b < 2; ++b) {

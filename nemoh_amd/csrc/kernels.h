@@ -39,6 +39,13 @@ void launch_diag_inverse(const C2<T>* A, int ld, int n, C2<T>* inv, hipStream_t 
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* inv, C2<T>* B, C2<T>* X, int nrhs,
                      hipStream_t st);
+// Same solve in two launches (one per triangle) of workgroups that pass
+// solved blocks through flags; requires ceil(n/64) <= number of CUs.
+// flags: 2*ceil(n/64) ints, never reset (epoch increases per call);
+// status: set nonzero if a bounded wait timed out.
+template <typename T>
+void launch_lu_solve_persistent(const C2<T>* A, int ld, int n, const C2<T>* inv, C2<T>* B, C2<T>* X, int nrhs,
+                                int* flags, int epoch, int* status, int spin_limit, hipStream_t st);
 template <typename TM>
 void launch_matvec(const C2<TM>* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
                    double beta, hipStream_t st);

@@ -103,17 +103,22 @@ __global__ void pairs_kernel(Geometry G, Tables T, const FreqParams* __restrict_
 // elements (ty + 16 i, tx + 16 j).  A sequential sweep over k publishes the
 // pivot row / column of step k through double-buffered LDS vectors, so each
 // step costs one barrier and 16 complex FMAs per thread.
+__device__ __forceinline__ bool tid_is_zero(int ty, int tx) { return (ty | tx) == 0; }
+
 template <typename T>
 struct PanelLds {
   C2<T> d[kNB][kNB + 1];  // factored diagonal block
-  C2<T> col[2][kNB];
-  C2<T> row[2][kNB];
+  C2<T> col[2][kNB];      // pivot column of the current step (double buffered)
+  C2<T> row[2][kNB];      // pivot row of the current step
+  C2<T> lp[kNB][17];      // 16 finished columns of a sub-panel (batched update)
+  C2<T> up[16][kNB + 1];  // 16 finished rows of a sub-panel
+  C2<T> ipiv[kNB];        // reciprocal pivots
 };
 
 template <typename T>
 __device__ __forceinline__ C2<T> crecip(C2<T> p) {
-  const T d = p.re * p.re + p.im * p.im;
-  return {p.re / d, -p.im / d};
+  const T s = T(1) / (p.re * p.re + p.im * p.im);
+  return {p.re * s, -p.im * s};
 }
 
 template <typename T>
@@ -140,49 +145,81 @@ __device__ __forceinline__ void store_tile(const C2<T> a[4][4], C2<T>* __restric
     }
 }
 
-// In-register LU without pivoting of the nb x nb leading part of the tile.
-// The sweep index k = 16 kk + kl runs with kk unrolled so that every
-// register index is a compile-time constant.
+// In-register LU without pivoting of the nb x nb leading part of the tile,
+// blocked by 16-column sub-panels: a step k touches only its sub-panel
+// (column block S = k / 16) and the rows of that sub-panel to the right;
+// the rest of the trailing tile gets one rank-16 update per sub-panel.
+// Register indices are compile-time constants (S unrolled).
 template <typename T>
 __device__ __forceinline__ void reg_factor(C2<T> a[4][4], int nb, PanelLds<T>& L, int ty, int tx) {
 #pragma unroll
-  for (int kk = 0; kk < 4; ++kk) {
+  for (int S = 0; S < 4; ++S) {
+    if (16 * S >= nb) break;
     for (int kl = 0; kl < 16; ++kl) {
-      const int k = 16 * kk + kl;
+      const int k = 16 * S + kl;
       if (k >= nb) break;
       const int buf = k & 1;
-      if (tx == kl) {
+      if (tx == kl) {  // column k: rows below the pivot
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int r = ty + 16 * i;
-          if (r > k && r < nb) L.col[buf][r] = a[i][kk];
+          if (r > k && r < nb) L.col[buf][r] = a[i][S];
         }
       }
-      if (ty == kl) {
+      if (ty == kl) {  // row k from the pivot on
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int c = tx + 16 * j;
-          if (c >= k && c < nb) L.row[buf][c] = a[kk][j];
+          if (c >= k && c < nb) L.row[buf][c] = a[S][j];
         }
       }
       __syncthreads();
       const C2<T> inv = crecip(L.row[buf][k]);
+      if (tid_is_zero(ty, tx)) L.ipiv[k] = inv;
       C2<T> lr[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = ty + 16 * i;
         lr[i] = (r > k && r < nb) ? cmul(L.col[buf][r], inv) : C2<T>{0, 0};
-        if (tx == kl && r > k && r < nb) a[i][kk] = lr[i];
       }
+      if (tx == kl) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int c = tx + 16 * j;
-        if (c > k && c < nb) {
-          const C2<T> u = L.row[buf][c];
+        for (int i = 0; i < 4; ++i)
+          if (ty + 16 * i > k && ty + 16 * i < nb) a[i][S] = lr[i];
+      }
+      // inside the sub-panel: column block S, columns after k
+      if (tx > kl && 16 * S + tx < nb) {
+        const C2<T> u = L.row[buf][16 * S + tx];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) a[i][j] = csub_mul(a[i][j], lr[i], u);
+        for (int i = 0; i < 4; ++i) a[i][S] = csub_mul(a[i][S], lr[i], u);
+      }
+      // rows of the sub-panel below k, column blocks to the right
+      if (ty > kl) {
+#pragma unroll
+        for (int j = S + 1; j < 4; ++j) {
+          const int c = tx + 16 * j;
+          if (c < nb) a[S][j] = csub_mul(a[S][j], lr[S], L.row[buf][c]);
         }
       }
+    }
+    if (S < 3 && 16 * (S + 1) < nb) {
+      // rank-16 update of rows and columns beyond the sub-panel
+      if (tx < 16) {
+#pragma unroll
+        for (int i = S + 1; i < 4; ++i) L.lp[ty + 16 * i][tx] = a[i][S];
+      }
+#pragma unroll
+      for (int j = S + 1; j < 4; ++j) L.up[ty][tx + 16 * j] = a[S][j];
+      __syncthreads();
+#pragma unroll
+      for (int i = S + 1; i < 4; ++i)
+#pragma unroll
+        for (int j = S + 1; j < 4; ++j) {
+          C2<T> acc = a[i][j];
+          for (int q = 0; q < 16; ++q) acc = csub_mul(acc, L.lp[ty + 16 * i][q], L.up[q][tx + 16 * j]);
+          a[i][j] = acc;
+        }
+      __syncthreads();
     }
   }
 }
@@ -218,62 +255,92 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(C2<T>* __restrict__ A, in
   if (bid >= 2 * nblk) return;
   C2<T> b[4][4];
   if (bid < nblk) {
-    // 64 rows of L21 = A21 U11^-1: column sweep x_k = b_k / u_kk, b_c -= x_k u_kc
+    // 64 rows of L21 = A21 U11^-1: column sweep x_k = b_k / u_kk,
+    // b_c -= x_k u_kc, blocked by 16 columns like reg_factor
     const int r0 = k0 + nb + bid * kNB;
     const int m = min(kNB, n - r0);
     load_tile(b, A, ld, r0, k0, m, nb, ty, tx);
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
+    for (int S = 0; S < 4; ++S) {
+      if (16 * S >= nb) break;
       for (int kl = 0; kl < 16; ++kl) {
-        const int k = 16 * kk + kl;
+        const int k = 16 * S + kl;
         if (k >= nb) break;
         const int buf = k & 1;
         if (tx == kl) {
-          const C2<T> inv = crecip(L.d[k][k]);
+          const C2<T> inv = L.ipiv[k];
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            b[i][kk] = cmul(b[i][kk], inv);
-            L.col[buf][ty + 16 * i] = b[i][kk];
+            b[i][S] = cmul(b[i][S], inv);
+            L.col[buf][ty + 16 * i] = b[i][S];
           }
         }
         __syncthreads();
+        if (tx > kl && 16 * S + tx < nb) {
+          const C2<T> u = L.d[k][16 * S + tx];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int c = tx + 16 * j;
-          if (c > k && c < nb) {
-            const C2<T> u = L.d[k][c];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) b[i][j] = csub_mul(b[i][j], L.col[buf][ty + 16 * i], u);
-          }
+          for (int i = 0; i < 4; ++i) b[i][S] = csub_mul(b[i][S], L.col[buf][ty + 16 * i], u);
         }
+      }
+      if (S < 3 && 16 * (S + 1) < nb) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) L.lp[ty + 16 * i][tx] = b[i][S];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = S + 1; j < 4; ++j) {
+            const int c = tx + 16 * j;
+            if (c < nb) {
+              C2<T> acc = b[i][j];
+              for (int q = 0; q < 16; ++q) acc = csub_mul(acc, L.lp[ty + 16 * i][q], L.d[16 * S + q][c]);
+              b[i][j] = acc;
+            }
+          }
+        __syncthreads();
       }
     }
     store_tile(b, A, ld, r0, k0, m, nb, ty, tx);
   } else {
-    // 64 columns of U12 = L11^-1 A12 (unit lower): b_r -= l_rk b_k
+    // 64 columns of U12 = L11^-1 A12 (unit lower): b_r -= l_rk b_k, blocked
     const int c0 = k0 + nb + (bid - nblk) * kNB;
     const int m = min(kNB, n - c0);
     load_tile(b, A, ld, k0, c0, nb, m, ty, tx);
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
+    for (int S = 0; S < 4; ++S) {
+      if (16 * S >= nb) break;
       for (int kl = 0; kl < 16; ++kl) {
-        const int k = 16 * kk + kl;
+        const int k = 16 * S + kl;
         if (k >= nb) break;
         const int buf = k & 1;
         if (ty == kl) {
 #pragma unroll
-          for (int j = 0; j < 4; ++j) L.row[buf][tx + 16 * j] = b[kk][j];
+          for (int j = 0; j < 4; ++j) L.row[buf][tx + 16 * j] = b[S][j];
         }
         __syncthreads();
+        if (ty > kl && 16 * S + ty < nb) {
+          const C2<T> l = L.d[16 * S + ty][k];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+          for (int j = 0; j < 4; ++j) b[S][j] = csub_mul(b[S][j], l, L.row[buf][tx + 16 * j]);
+        }
+      }
+      if (S < 3 && 16 * (S + 1) < nb) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) L.up[ty][tx + 16 * j] = b[S][j];
+        __syncthreads();
+#pragma unroll
+        for (int i = S + 1; i < 4; ++i) {
           const int r = ty + 16 * i;
-          if (r > k && r < nb) {
-            const C2<T> l = L.d[r][k];
+          if (r < nb) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[i][j] = csub_mul(b[i][j], l, L.row[buf][tx + 16 * j]);
+            for (int j = 0; j < 4; ++j) {
+              C2<T> acc = b[i][j];
+              for (int q = 0; q < 16; ++q) acc = csub_mul(acc, L.d[r][16 * S + q], L.up[q][tx + 16 * j]);
+              b[i][j] = acc;
+            }
           }
         }
+        __syncthreads();
       }
     }
     store_tile(b, A, ld, k0, c0, nb, m, ty, tx);
@@ -552,6 +619,94 @@ __global__ __launch_bounds__(256) void trsv_kernel(const C2<T>* __restrict__ A, 
   }
 }
 
+// Single-launch triangular solve.  Workgroup w owns row block b (lower:
+// b = w, upper: b = nblk-1-w, so blocks without dependencies are dispatched
+// first) and accumulates r_b -= A_bj x_j for every solved block j in
+// dependency order, waiting for block j's flag (set with release semantics
+// after x_j is stored), then forms x_b = inv_bb r_b and publishes it.
+// The grid never exceeds the device's CU count, so all workgroups are
+// resident.  Every wait is bounded: after spin_limit polls the workgroup
+// records a timeout in *status and goes on, so the kernel always ends (the
+// host then reports an error instead of the result).
+template <typename T, bool kLower>
+__global__ __launch_bounds__(256) void trsv_persistent_kernel(const C2<T>* __restrict__ A, int ld, int n,
+                                                              const C2<T>* __restrict__ inv,
+                                                              const C2<T>* __restrict__ R, C2<T>* __restrict__ X,
+                                                              int nrhs, int* __restrict__ flags, int epoch,
+                                                              int* __restrict__ status, int spin_limit) {
+  __shared__ C2<T> t[kNB][kNB + 1];
+  __shared__ C2<T> sx[kNB][kMaxRhs];
+  __shared__ C2<T> acc[kNB][kMaxRhs];
+  __shared__ int timed_out;
+  const int tid = threadIdx.x;
+  const int nblk = (n + kNB - 1) / kNB;
+  const int b = kLower ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+  const int k0 = b * kNB;
+  const int nb = min(kNB, n - k0);
+  for (int e = tid; e < kNB * kMaxRhs; e += 256) {
+    const int r = e / kMaxRhs, q = e % kMaxRhs;
+    acc[r][q] = (r < nb && q < nrhs) ? R[(size_t)(k0 + r) * nrhs + q] : C2<T>{0, 0};
+  }
+  if (tid == 0) timed_out = 0;
+  const int ndep = kLower ? b : nblk - 1 - b;
+  for (int d = 0; d < ndep; ++d) {
+    const int j = kLower ? d : nblk - 1 - d;
+    const int j0 = j * kNB, nj = min(kNB, n - j0);
+    __syncthreads();
+    // the coupling tile does not depend on x_j: load it before waiting
+    for (int e = tid; e < kNB * kNB; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      t[r][c] = (r < nb && c < nj) ? A[(size_t)(k0 + r) * ld + j0 + c] : C2<T>{0, 0};
+    }
+    if (tid == 0) {
+      int spins = 0;
+      while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++spins > spin_limit) {
+          timed_out = 1;
+          atomicMax(status, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int e = tid; e < kNB * kMaxRhs; e += 256) {
+      const int r = e / kMaxRhs, q = e % kMaxRhs;
+      sx[r][q] = (r < nj && q < nrhs) ? X[(size_t)(j0 + r) * nrhs + q] : C2<T>{0, 0};
+    }
+    __syncthreads();
+    for (int e = tid; e < kNB * kMaxRhs; e += 256) {
+      const int r = e / kMaxRhs, q = e % kMaxRhs;
+      if (q < nrhs) {
+        C2<T> a = acc[r][q];
+#pragma unroll 8
+        for (int k = 0; k < kNB; ++k) a = csub_mul(a, t[r][k], sx[k][q]);
+        acc[r][q] = a;
+      }
+    }
+  }
+  __syncthreads();
+  // x_b = inv_bb r_b
+  const C2<T>* iv = inv + ((size_t)b * 2 + (kLower ? 0 : 1)) * kNB * kNB;
+  for (int e = tid; e < kNB * kNB; e += 256) t[e >> 6][e & 63] = iv[e];
+  __syncthreads();
+  for (int e = tid; e < nb * nrhs; e += 256) {
+    const int r = e / nrhs, q = e % nrhs;
+    C2<T> x = {0, 0};
+#pragma unroll 8
+    for (int m = 0; m < kNB; ++m) {
+      const C2<T> p = cmul(t[r][m], acc[m][q]);
+      x.re += p.re;
+      x.im += p.im;
+    }
+    X[(size_t)(k0 + r) * nrhs + q] = x;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------------------
 // Y(n x nrhs) = beta*Y + alpha*M(n x n) X(n x nrhs), FP64 accumulation.  One
 // wavefront per matrix row: lanes stride the columns (coalesced row reads of
@@ -763,6 +918,21 @@ void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* inv, C2<T>* B, 
 }
 template void launch_lu_solve<double>(const C2<double>*, int, int, const C2<double>*, C2<double>*, C2<double>*, int,
                                       hipStream_t);
+
+template <typename T>
+void launch_lu_solve_persistent(const C2<T>* A, int ld, int n, const C2<T>* inv, C2<T>* B, C2<T>* X, int nrhs,
+                                int* flags, int epoch, int* status, int spin_limit, hipStream_t st) {
+  const int nblk = (n + kNB - 1) / kNB;
+  // forward: B -> X (= L^-1 b); backward: X -> B (= U^-1 y); epochs 2e, 2e+1
+  trsv_persistent_kernel<T, true><<<nblk, 256, 0, st>>>(A, ld, n, inv, B, X, nrhs, flags, 2 * epoch, status,
+                                                         spin_limit);
+  trsv_persistent_kernel<T, false><<<nblk, 256, 0, st>>>(A, ld, n, inv, X, B, nrhs, flags + nblk, 2 * epoch + 1,
+                                                          status, spin_limit);
+}
+template void launch_lu_solve_persistent<double>(const C2<double>*, int, int, const C2<double>*, C2<double>*,
+                                                 C2<double>*, int, int*, int, int*, int, hipStream_t);
+template void launch_lu_solve_persistent<float>(const C2<float>*, int, int, const C2<float>*, C2<float>*, C2<float>*,
+                                                int, int*, int, int*, int, hipStream_t);
 template void launch_lu_solve<float>(const C2<float>*, int, int, const C2<float>*, C2<float>*, C2<float>*, int,
                                      hipStream_t);
 
