@@ -100,6 +100,14 @@ int nemoh_core_freq_setup(nemoh_core *ctx, double period, nemoh_freq *out);
  * (complex64 LU on MFMA refined in FP64 until max|r| <= tol * max|b|). */
 int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter);
 
+/* Execution options (no counterpart in the reference).
+ * NEMOH_OPT_STEPWISE_SOLVE: 1 = triangular solves as one launch per 64-row
+ * block step (the path taken when the row blocks outnumber the compute
+ * units), 0 = automatic (single flag-chained launch per triangle when the
+ * row blocks fit). */
+enum { NEMOH_OPT_STEPWISE_SOLVE = 1 };
+int nemoh_core_set_option(nemoh_core *ctx, int option, int value);
+
 /* Timings (ms) of the last solve's phases and refinement statistics. */
 typedef struct {
   double ms_assembly, ms_factor, ms_solve, ms_total;

@@ -14,10 +14,11 @@ LIB_PATH = Path(__file__).resolve().parent / "libnemoh_core.so"
 
 OK, ERR_ARG, ERR_HIP, ERR_PIVOT, ERR_OMEGA, ERR_FIT, ERR_PANEL, ERR_CONVERGENCE, ERR_WAVENUMBER = range(9)
 FP64, MIXED = 0, 1
+OPT_STEPWISE_SOLVE = 1
 
 EXPORTS = (
     "nemoh_core_create", "nemoh_core_solve", "nemoh_core_solve_multi", "nemoh_core_solve_frequency",
-    "nemoh_core_freq_setup", "nemoh_core_set_precision", "nemoh_core_get_stats", "nemoh_core_enable_timing",
+    "nemoh_core_freq_setup", "nemoh_core_set_precision", "nemoh_core_set_option", "nemoh_core_get_stats", "nemoh_core_enable_timing",
     "nemoh_core_last_error", "nemoh_core_destroy", "nemoh_core_prepare_mesh", "nemoh_core_build_tables",
     "nemoh_core_wavenumber", "nemoh_core_theta_rule", "nemoh_core_prony", "nemoh_core_pairs",
     "nemoh_core_mfma_probe",
@@ -64,6 +65,7 @@ def load():
     L.nemoh_core_solve_frequency.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, C.c_int, _dp, _dp]
     L.nemoh_core_freq_setup.argtypes = [C.c_void_p, C.c_double, C.POINTER(FreqSetup)]
     L.nemoh_core_set_precision.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+    L.nemoh_core_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.nemoh_core_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
     L.nemoh_core_enable_timing.argtypes = [C.c_void_p, C.c_int]
     L.nemoh_core_last_error.argtypes = [C.c_void_p]
@@ -195,6 +197,9 @@ class Core:
 
     def set_precision(self, mode: int, tol: float = 0.0, max_iter: int = 0):
         self._check(load().nemoh_core_set_precision(self.h, int(mode), float(tol), int(max_iter)))
+
+    def set_option(self, option: int, value: int):
+        self._check(load().nemoh_core_set_option(self.h, int(option), int(value)))
 
     def enable_timing(self, on=True):
         self._check(load().nemoh_core_enable_timing(self.h, int(on)))

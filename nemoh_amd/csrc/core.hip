@@ -34,8 +34,8 @@ struct nemoh_core {
   double2* D[2] = {nullptr, nullptr};
   double2* S[2] = {nullptr, nullptr};
   float2* D32[2] = {nullptr, nullptr};
-  double2* Dinv[2] = {nullptr, nullptr};  // inverses of the factors' diagonal blocks
-  float2* Dinv32[2] = {nullptr, nullptr};
+  double2* Pinv[2] = {nullptr, nullptr};  // inverses of the pivot blocks (64x64 each)
+  float2* Pinv32[2] = {nullptr, nullptr};
   double w_prev = -1.0;
   int factored_prec = -1;
   int nq_state = 0;  // module NQ of the standalone session
@@ -57,6 +57,7 @@ struct nemoh_core {
   int* d_flags = nullptr;    // block flags of the single-launch triangular solves
   int epoch = 0;
   int cu_count = 0;
+  bool stepwise_solve = false;
   // statistics
   bool timing = false;
   nemoh_stats stats{};
@@ -90,12 +91,12 @@ int ensure_matrices(nemoh_core* c) {
   for (int b = 0; b < c->nb; ++b) {
     if (int r = dalloc(c, &c->D[b], n2)) return r;
     if (int r = dalloc(c, &c->S[b], n2)) return r;
-    const size_t ninv = 2 * (size_t)((c->nfa + kNB - 1) / kNB) * kNB * kNB;
+    const size_t ninv = (size_t)((c->nfa + kNB - 1) / kNB) * kNB * kNB;
     if (c->prec == NEMOH_MIXED) {
       if (int r = dalloc(c, &c->D32[b], n2)) return r;
-      if (int r = dalloc(c, &c->Dinv32[b], ninv)) return r;
+      if (int r = dalloc(c, &c->Pinv32[b], ninv)) return r;
     } else {
-      if (int r = dalloc(c, &c->Dinv[b], ninv)) return r;
+      if (int r = dalloc(c, &c->Pinv[b], ninv)) return r;
     }
   }
   return NEMOH_OK;
@@ -219,15 +220,12 @@ int prepare_frequency(nemoh_core* c, double w) {
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
   HIPCHK(c, hipMemsetAsync(c->d_info, 0, 2 * sizeof(int), c->st));
   for (int b = 0; b < c->nb; ++b) {
-    if (c->prec == NEMOH_MIXED) {
-      auto* a = reinterpret_cast<C2<float>*>(c->D32[b]);
-      launch_lu<float>(a, c->nfa, c->nfa, c->d_info, c->st);
-      launch_diag_inverse<float>(a, c->nfa, c->nfa, reinterpret_cast<C2<float>*>(c->Dinv32[b]), c->st);
-    } else {
-      auto* a = reinterpret_cast<C2<double>*>(c->D[b]);
-      launch_lu<double>(a, c->nfa, c->nfa, c->d_info, c->st);
-      launch_diag_inverse<double>(a, c->nfa, c->nfa, reinterpret_cast<C2<double>*>(c->Dinv[b]), c->st);
-    }
+    if (c->prec == NEMOH_MIXED)
+      launch_lu<float>(reinterpret_cast<C2<float>*>(c->D32[b]), c->nfa, c->nfa,
+                       reinterpret_cast<C2<float>*>(c->Pinv32[b]), c->d_info, c->st);
+    else
+      launch_lu<double>(reinterpret_cast<C2<double>*>(c->D[b]), c->nfa, c->nfa,
+                        reinterpret_cast<C2<double>*>(c->Pinv[b]), c->d_info, c->st);
   }
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
@@ -252,19 +250,16 @@ double device_absmax(nemoh_core* c, const double2* a, size_t count, int* err) {
   return m;
 }
 
-// (LU) x = b in place in B (X scratch): one launch per triangle when every
-// row block can have its own CU, else one launch per block step.
+// A x = b in place in B (W scratch): one launch per triangle when every row
+// block can have its own CU, else one launch per block step.
 template <typename T>
-void lu_solve(nemoh_core* c, const C2<T>* A, const C2<T>* inv, C2<T>* B, C2<T>* X, int nrhs) {
+void lu_solve(nemoh_core* c, const C2<T>* A, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs) {
   const int n = c->nfa;
   const int nblk = (n + kNB - 1) / kNB;
-  if (nblk <= c->cu_count) {
-    ++c->epoch;
-    // about a second of polling before a wait gives up
-    launch_lu_solve_persistent<T>(A, n, n, inv, B, X, nrhs, c->d_flags, c->epoch, c->d_info + 1, 1 << 22, c->st);
-  } else {
-    launch_lu_solve<T>(A, n, n, inv, B, X, nrhs, c->st);
-  }
+  const bool chained = nblk <= c->cu_count && !c->stepwise_solve;
+  if (chained) ++c->epoch;
+  // about a second of polling before a wait gives up
+  launch_lu_solve<T>(A, n, n, P, B, W, nrhs, chained, c->d_flags, c->epoch, c->d_info + 1, 1 << 22, c->st);
 }
 
 // Solve block b for the right-hand sides in c->B (n x nrhs, row-major):
@@ -274,7 +269,7 @@ int solve_block(nemoh_core* c, int b, int nrhs) {
   const size_t cnt = (size_t)n * nrhs;
   if (c->prec == NEMOH_FP64) {
     HIPCHK(c, hipMemcpyAsync(c->Z[b], c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
-    lu_solve(c, reinterpret_cast<const C2<double>*>(c->D[b]), reinterpret_cast<const C2<double>*>(c->Dinv[b]),
+    lu_solve(c, reinterpret_cast<const C2<double>*>(c->D[b]), reinterpret_cast<const C2<double>*>(c->Pinv[b]),
              reinterpret_cast<C2<double>*>(c->Z[b]), reinterpret_cast<C2<double>*>(c->W), nrhs);
     HIPCHK(c, hipGetLastError());
     return NEMOH_OK;
@@ -289,7 +284,7 @@ int solve_block(nemoh_core* c, int b, int nrhs) {
   for (; it < c->max_iter; ++it) {
     if (rmax == 0.0) break;
     launch_f64_to_f32(c->R, c->R32, cnt, 1.0 / rmax, c->st);
-    lu_solve(c, reinterpret_cast<const C2<float>*>(c->D32[b]), reinterpret_cast<const C2<float>*>(c->Dinv32[b]),
+    lu_solve(c, reinterpret_cast<const C2<float>*>(c->D32[b]), reinterpret_cast<const C2<float>*>(c->Pinv32[b]),
              reinterpret_cast<C2<float>*>(c->R32), reinterpret_cast<C2<float>*>(c->W32), nrhs);
     launch_f32_to_f64(c->R32, c->Z[b], cnt, rmax, it > 0, c->st);
     HIPCHK(c, hipMemcpyAsync(c->R, c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
@@ -530,8 +525,8 @@ void nemoh_core_destroy(nemoh_core* c) {
     release(c->D[b]);
     release(c->S[b]);
     release(c->D32[b]);
-    release(c->Dinv[b]);
-    release(c->Dinv32[b]);
+    release(c->Pinv[b]);
+    release(c->Pinv32[b]);
     release(c->Z[b]);
     release(c->F[b]);
   }
@@ -558,6 +553,15 @@ int nemoh_core_set_precision(nemoh_core* c, int mode, double tol, int max_iter) 
   if (tol > 0) c->tol = tol;
   if (max_iter > 0) c->max_iter = max_iter;
   return NEMOH_OK;
+}
+
+int nemoh_core_set_option(nemoh_core* c, int option, int value) {
+  if (!c) return NEMOH_ERR_ARG;
+  if (option == NEMOH_OPT_STEPWISE_SOLVE) {
+    c->stepwise_solve = value != 0;
+    return NEMOH_OK;
+  }
+  return fail(c, NEMOH_ERR_ARG, "unknown option");
 }
 
 int nemoh_core_enable_timing(nemoh_core* c, int on) {

@@ -29,23 +29,23 @@ void launch_crek(const double* xr, const double* xz, double* tab, hipStream_t st
 void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, int nsym, const MatOut& out,
                      hipStream_t st);
 void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, double* out, hipStream_t st);
+// Block LU without pivoting in place in A (n x n, leading dimension ld):
+// afterwards the blocks below the diagonal hold L (unit block lower), the
+// blocks above hold U (unit block upper) and P (ceil(n/64) blocks of 64x64,
+// row-major) the inverses of the pivot blocks.  *info receives the first row
+// whose pivot has modulus <= 1e-20 (1-based), if any.
 template <typename T>
-void launch_lu(C2<T>* A, int ld, int n, int* info, hipStream_t st);
-// Inverses of the factors' 64x64 diagonal blocks: inv[kb][0] = L_kk^-1,
-// inv[kb][1] = U_kk^-1 (2 * ceil(n/64) * 64 * 64 elements).
+void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st);
+// Solves A x = b for nrhs <= kMaxRhs right-hand sides (B: n x nrhs
+// row-major) with the factors of launch_lu; on return B holds the solution,
+// W is scratch of the same size.  chained: one launch per triangle whose
+// workgroups pass solved blocks through flags (requires every row block to
+// be resident at once; flags: 2*ceil(n/64) ints, never reset, epoch must
+// increase per call; status set nonzero if a bounded wait timed out); else
+// one launch per block step.
 template <typename T>
-void launch_diag_inverse(const C2<T>* A, int ld, int n, C2<T>* inv, hipStream_t st);
-// Solves (LU) X = B in place: on return B holds the solution, X is scratch.
-template <typename T>
-void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* inv, C2<T>* B, C2<T>* X, int nrhs,
-                     hipStream_t st);
-// Same solve in two launches (one per triangle) of workgroups that pass
-// solved blocks through flags; requires ceil(n/64) <= number of CUs.
-// flags: 2*ceil(n/64) ints, never reset (epoch increases per call);
-// status: set nonzero if a bounded wait timed out.
-template <typename T>
-void launch_lu_solve_persistent(const C2<T>* A, int ld, int n, const C2<T>* inv, C2<T>* B, C2<T>* X, int nrhs,
-                                int* flags, int epoch, int* status, int spin_limit, hipStream_t st);
+void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
+                     int* flags, int epoch, int* status, int spin_limit, hipStream_t st);
 template <typename TM>
 void launch_matvec(const C2<TM>* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
                    double beta, hipStream_t st);

@@ -1,0 +1,633 @@
+// Dense complex block LU without pivoting and the multi-RHS triangular solves
+// of the influence-matrix system, on CDNA4 matrix cores.
+//
+// The reference inverts the matrix by Gauss-Jordan elimination without row
+// interchanges (GAUSSZ, Solver/Core/M_SOLVER.f90:31-73; its row search never
+// swaps).  Here the same unpivoted elimination is organised in 64x64 blocks
+// (nblk = ceil(n/64), block step k):
+//
+//   P_k   = (A_kk)^-1            Gauss-Jordan on the updated diagonal block,
+//                                one workgroup
+//   L_ik  = A_ik P_k    (i > k)  stored over A_ik          (panel GEMM)
+//   A_ij -= L_ik A_kj   (i,j > k)                          (update GEMM)
+//   U_kj  = P_k A_kj    (j > k)  stored over A_kj after update k has read it
+//
+// so A = L D U with L, U unit block triangular and D = diag(A_kk), and
+//   forward   y_b = b_b - sum_{j<b} L_bj y_j
+//   backward  x_b = P_b y_b - sum_{j>b} U_bj x_j
+// are both plain block recurrences without a diagonal solve on the critical
+// path.  The Gauss-Jordan pivots are the pivots of the pointwise elimination,
+// so the reference's "pivot < 1e-20" check keeps its meaning.
+//
+// Launch sequence of one factorisation: block_inverse(0), then for each k:
+// panel(k) [L_ik for i > k and U_{k-1,j} for j >= k], update(k) [A_ij for
+// i,j > k; workgroup 0 updates the next diagonal block first and inverts it
+// while the other workgroups finish the trailing matrix (look-ahead)].
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "linalg.h"
+
+namespace nemoh {
+
+constexpr int kTile = kNB * kNB;  // elements of one 64x64 block
+
+// ---------------------------------------------------------------------------
+// 64x64x64 complex tile product on MFMA.  Four waves each own a 32x32
+// quadrant = 2x2 MFMA 16x16 tiles; the complex product is four real MFMAs.
+// K streams through LDS in chunks of 16, double buffered (the global loads of
+// chunk c+1 are in flight while chunk c feeds the MFMAs); LDS keeps the
+// operands k-major in separate real / imaginary planes.
+
+template <typename T>
+struct GemmLds {
+  T ar[2][16][64], ai[2][16][64], br[2][16][64], bi[2][16][64];
+};
+
+template <typename T>
+struct Acc64 {
+  typename MfmaTraits<T>::acc_t re[2][2], im[2][2];
+};
+
+// 64x64 operand window: element (r, c) = p[r * ld + c] for r < nr, c < nc,
+// zero outside.
+template <typename T>
+struct View {
+  const C2<T>* p;
+  int ld, nr, nc;
+};
+
+template <typename T>
+__device__ __forceinline__ void acc_zero(Acc64<T>& acc) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc.re[a][b][q] = acc.im[a][b][q] = T(0);
+}
+
+// element (row, col) of the tile held by this lane in accumulator [a][b][q]
+__device__ __forceinline__ void acc_coord(int a, int b, int row_in16, int& r, int& c) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  r = (wave >> 1) * 32 + a * 16 + row_in16;
+  c = (wave & 1) * 32 + b * 16 + (lane & 15);
+}
+
+template <typename T>
+__device__ __forceinline__ void acc_load(Acc64<T>& acc, const C2<T>* __restrict__ C, int ld, int nr, int nc) {
+  const int lk = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int r, c;
+        acc_coord(a, b, MfmaTraits<T>::row(lk, q), r, c);
+        C2<T> v = {0, 0};
+        if (r < nr && c < nc) v = C[(size_t)r * ld + c];
+        acc.re[a][b][q] = v.re;
+        acc.im[a][b][q] = v.im;
+      }
+}
+
+template <typename T>
+__device__ __forceinline__ void acc_store(const Acc64<T>& acc, C2<T>* __restrict__ C, int ld, int nr, int nc) {
+  const int lk = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int r, c;
+        acc_coord(a, b, MfmaTraits<T>::row(lk, q), r, c);
+        if (r < nr && c < nc) C[(size_t)r * ld + c] = C2<T>{acc.re[a][b][q], acc.im[a][b][q]};
+      }
+}
+
+// acc += (kNegA ? -A : A) B.  All global reads of both operands complete
+// before the last barrier, so the caller may store over an operand.
+template <typename T, bool kNegA>
+__device__ __forceinline__ void gemm64(Acc64<T>& acc, View<T> A, View<T> B, GemmLds<T>& S) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  // staging map: A chunk = 64 rows x 16 k, B chunk = 16 k x 64 cols; each
+  // thread moves 4 elements of each
+  C2<T> pa[4], pb[4];
+  auto fetch = [&](int chunk) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      const int ar_ = e >> 4, ak = chunk * 16 + (e & 15);
+      const int bk = chunk * 16 + (e >> 6), bc = e & 63;
+      pa[u] = (ar_ < A.nr && ak < A.nc) ? A.p[(size_t)ar_ * A.ld + ak] : C2<T>{0, 0};
+      pb[u] = (bk < B.nr && bc < B.nc) ? B.p[(size_t)bk * B.ld + bc] : C2<T>{0, 0};
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      const int ar_ = e >> 4, ak = e & 15;
+      const int bk = e >> 6, bc = e & 63;
+      S.ar[buf][ak][ar_] = kNegA ? -pa[u].re : pa[u].re;
+      S.ai[buf][ak][ar_] = kNegA ? -pa[u].im : pa[u].im;
+      S.br[buf][bk][bc] = pb[u].re;
+      S.bi[buf][bk][bc] = pb[u].im;
+    }
+  };
+  fetch(0);
+  stash(0);
+  __syncthreads();
+#pragma unroll
+  for (int chunk = 0; chunk < 4; ++chunk) {
+    const int buf = chunk & 1;
+    if (chunk < 3) fetch(chunk + 1);
+#pragma unroll
+    for (int k = 0; k < 16; k += 4) {
+      T xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        xr[a] = S.ar[buf][k + lk][wr + a * 16 + li];
+        xi[a] = S.ai[buf][k + lk][wr + a * 16 + li];
+        yr[a] = S.br[buf][k + lk][wc + a * 16 + li];
+        yi[a] = S.bi[buf][k + lk][wc + a * 16 + li];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc.re[a][b] = MfmaTraits<T>::mma(xr[a], yr[b], acc.re[a][b]);
+          acc.im[a][b] = MfmaTraits<T>::mma(xr[a], yi[b], acc.im[a][b]);
+        }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc.re[a][b] = MfmaTraits<T>::mma(-xi[a], yi[b], acc.re[a][b]);
+          acc.im[a][b] = MfmaTraits<T>::mma(xi[a], yr[b], acc.im[a][b]);
+        }
+    }
+    if (chunk < 3) {
+      stash(buf ^ 1);
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// In-place Gauss-Jordan inversion without pivoting of a 64x64 tile held in
+// registers: thread (ty, tx) = (tid / 16, tid % 16) owns the strided elements
+// (ty + 16 i, tx + 16 j).  Step k publishes the pivot row and column through
+// double-buffered LDS vectors (one barrier per step), then every thread
+// applies  a_kj <- a_kj / p,  a_ik <- -a_ik / p,  a_ij <- a_ij - a_ik a_kj / p,
+// a_kk <- 1 / p  to its 16 elements.
+template <typename T>
+struct GjLds {
+  C2<T> col[2][kNB];
+  C2<T> row[2][kNB];
+};
+
+template <typename T>
+__device__ __forceinline__ C2<T> crecip(C2<T> p) {
+  const T s = T(1) / (p.re * p.re + p.im * p.im);
+  return {p.re * s, -p.im * s};
+}
+
+// nb: order of the real block (rows / columns >= nb hold the identity);
+// pivots of modulus <= 1e-20 are reported as row kbase + k + 1 in *info.
+template <typename T>
+__device__ __forceinline__ void gj_invert(C2<T> a[4][4], GjLds<T>& L, int nb, int kbase, int* __restrict__ info) {
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+#pragma unroll
+  for (int S = 0; S < 4; ++S) {
+    for (int kl = 0; kl < 16; ++kl) {
+      const int k = 16 * S + kl, buf = k & 1;
+      if (tx == kl) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) L.col[buf][ty + 16 * i] = a[i][S];
+      }
+      if (ty == kl) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) L.row[buf][tx + 16 * j] = a[S][j];
+      }
+      __syncthreads();
+      const C2<T> p = L.row[buf][k];
+      if (tid == 0 && k < nb && !(sqrt((double)p.re * p.re + (double)p.im * p.im) > 1e-20))
+        atomicMax(info, kbase + k + 1);
+      const C2<T> pinv = crecip(p);
+      C2<T> u[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = tx + 16 * j;
+        u[j] = (c == k) ? pinv : cmul(L.row[buf][c], pinv);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const C2<T> l = L.col[buf][ty + 16 * i];
+        const bool rowk = (i == S) && (ty == kl);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int c = tx + 16 * j;
+          const C2<T> base = (c == k) ? C2<T>{0, 0} : a[i][j];
+          a[i][j] = rowk ? u[j] : csub_mul(base, l, u[j]);
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_inverse(const C2<T> a[4][4], C2<T>* __restrict__ P, int nb) {
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = ty + 16 * i, c = tx + 16 * j;
+      P[r * kNB + c] = (r < nb && c < nb) ? a[i][j] : C2<T>{0, 0};
+    }
+}
+
+// P_kb for one diagonal block read from A (the first block of a factorisation).
+template <typename T>
+__global__ __launch_bounds__(256) void block_inverse_kernel(const C2<T>* __restrict__ A, int ld, int n, int kb,
+                                                            C2<T>* __restrict__ P, int* __restrict__ info) {
+  __shared__ GjLds<T> L;
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  const int k0 = kb * kNB, nb = min(kNB, n - k0);
+  C2<T> a[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = ty + 16 * i, c = tx + 16 * j;
+      a[i][j] = (r < nb && c < nb) ? A[(size_t)(k0 + r) * ld + k0 + c] : C2<T>{T(r == c ? 1 : 0), 0};
+    }
+  gj_invert(a, L, nb, k0, info);
+  store_inverse(a, P + (size_t)kb * kTile, nb);
+}
+
+// Panel step kb: workgroups [0, ncol) form L_ib = A_ib P_kb for the row
+// blocks i = kb+1+w below the diagonal, the rest form U_{kb-1,j} = P_{kb-1}
+// A_{kb-1,j} for j = kb + (w - ncol), in place.
+template <typename T>
+__global__ __launch_bounds__(256) void lu_panel_kernel(C2<T>* __restrict__ A, int ld, int n, int kb,
+                                                       const C2<T>* __restrict__ P, int ncol) {
+  __shared__ GemmLds<T> S;
+  Acc64<T> acc;
+  acc_zero(acc);
+  const int w = blockIdx.x;
+  if (w < ncol) {
+    const int r0 = (kb + 1 + w) * kNB, c0 = kb * kNB;
+    const int nr = min(kNB, n - r0), nk = min(kNB, n - c0);
+    C2<T>* blk = A + (size_t)r0 * ld + c0;
+    gemm64<T, false>(acc, View<T>{blk, ld, nr, nk}, View<T>{P + (size_t)kb * kTile, kNB, kNB, kNB}, S);
+    acc_store(acc, blk, ld, nr, nk);
+  } else {
+    const int r0 = (kb - 1) * kNB, c0 = (kb + w - ncol) * kNB;
+    const int nk = min(kNB, n - r0), nc = min(kNB, n - c0);
+    C2<T>* blk = A + (size_t)r0 * ld + c0;
+    gemm64<T, false>(acc, View<T>{P + (size_t)(kb - 1) * kTile, kNB, kNB, kNB}, View<T>{blk, ld, nk, nc}, S);
+    acc_store(acc, blk, ld, nk, nc);
+  }
+}
+
+template <typename T>
+struct InvLds {
+  C2<T> t[kNB][kNB + 1];
+  GjLds<T> gj;
+};
+template <typename T>
+union UpdLds {
+  GemmLds<T> g;
+  InvLds<T> inv;
+};
+
+// Trailing update of step kb: A_ij -= L_i,kb A_kb,j for i, j > kb, one 64x64
+// tile per workgroup.  Workgroup 0 owns tile (kb+1, kb+1), the next diagonal
+// block, and inverts it after its update (P_{kb+1}).
+template <typename T>
+__global__ __launch_bounds__(256) void lu_update_kernel(C2<T>* __restrict__ A, int ld, int n, int kb,
+                                                        C2<T>* __restrict__ P, int* __restrict__ info) {
+  __shared__ UpdLds<T> S;
+  const int s0 = (kb + 1) * kNB, k0 = kb * kNB;
+  const int m = (n - s0 + kNB - 1) / kNB;
+  const int ti = blockIdx.x / m, tj = blockIdx.x % m;
+  const int r0 = s0 + ti * kNB, c0 = s0 + tj * kNB;
+  const int nr = min(kNB, n - r0), nc = min(kNB, n - c0);
+  C2<T>* blk = A + (size_t)r0 * ld + c0;
+  Acc64<T> acc;
+  acc_load(acc, blk, ld, nr, nc);
+  gemm64<T, true>(acc, View<T>{A + (size_t)r0 * ld + k0, ld, nr, kNB}, View<T>{A + (size_t)k0 * ld + c0, ld, kNB, nc},
+                  S.g);
+  acc_store(acc, blk, ld, nr, nc);
+  if (blockIdx.x != 0) return;
+  // look-ahead: invert the freshly updated diagonal block
+  __syncthreads();  // every wave is done with the GEMM staging area
+  {
+    const int lk = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int r, c;
+          acc_coord(a, b, MfmaTraits<T>::row(lk, q), r, c);
+          S.inv.t[r][c] = C2<T>{acc.re[a][b][q], acc.im[a][b][q]};
+        }
+  }
+  __syncthreads();
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  C2<T> a[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = ty + 16 * i, c = tx + 16 * j;
+      a[i][j] = (r < nr && c < nr) ? S.inv.t[r][c] : C2<T>{T(r == c ? 1 : 0), 0};
+    }
+  gj_invert(a, S.inv.gj, nr, r0, info);
+  store_inverse(a, P + (size_t)(kb + 1) * kTile, nr);
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves.  One block step is r_b -= M_bj x_j with M_bj a 64x64
+// block of L or U and x_j (64 x nrhs, nrhs <= 16): each wave computes 16 rows
+// with 16 MFMA k-steps (N = 16 right-hand-side columns).  The wave's operand
+// registers for M_bj are loaded before x_j is needed (HopTile), with the sign
+// folded in: pr = s Re M, pi = s Im M, npi = -s Im M.
+
+template <typename T>
+struct HopTile {
+  T pr[16], pi[16], npi[16];
+};
+
+template <typename T>
+__device__ __forceinline__ void hop_load_tile(HopTile<T>& h, const C2<T>* __restrict__ M, int ld, int nr, int nc,
+                                              T sign) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = 16 * wave + (lane & 15), lk = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int c = 4 * s + lk;
+    const C2<T> v = (r < nr && c < nc) ? M[(size_t)r * ld + c] : C2<T>{0, 0};
+    h.pr[s] = sign * v.re;
+    h.pi[s] = sign * v.im;
+    h.npi[s] = -sign * v.im;
+  }
+}
+
+// Loads and stores of solved blocks that other workgroups of the same launch
+// read: system-scope relaxed atomics (coherent across the XCDs' L2 caches
+// without a cache-wide write-back or invalidation).
+__device__ __forceinline__ C2<double> ld_coherent(const C2<double>* p) {
+  const double* q = reinterpret_cast<const double*>(p);
+  return {__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+          __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)};
+}
+__device__ __forceinline__ C2<float> ld_coherent(const C2<float>* p) {
+  const unsigned long long u =
+      __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  C2<float> v;
+  __builtin_memcpy(&v, &u, sizeof v);
+  return v;
+}
+__device__ __forceinline__ void st_coherent(C2<double>* p, C2<double> v) {
+  double* q = reinterpret_cast<double*>(p);
+  __hip_atomic_store(q, v.re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(q + 1, v.im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_coherent(C2<float>* p, C2<float> v) {
+  unsigned long long u;
+  __builtin_memcpy(&u, &v, sizeof u);
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// x operand of one block step: lane (li, lk) holds x[4 s + lk][li], s < 16.
+template <typename T, bool kCoherent>
+__device__ __forceinline__ void hop_load_x(T xr[16], T xi[16], const C2<T>* __restrict__ X, int nj, int nrhs) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int r = 4 * s + lk;
+    C2<T> v = {0, 0};
+    if (r < nj && li < nrhs) v = kCoherent ? ld_coherent(X + (size_t)r * nrhs + li) : X[(size_t)r * nrhs + li];
+    xr[s] = v.re;
+    xi[s] = v.im;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void hop_mma(typename MfmaTraits<T>::acc_t& cr, typename MfmaTraits<T>::acc_t& ci,
+                                        const HopTile<T>& h, const T xr[16], const T xi[16]) {
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    cr = MfmaTraits<T>::mma(h.pr[s], xr[s], cr);
+    ci = MfmaTraits<T>::mma(h.pr[s], xi[s], ci);
+    cr = MfmaTraits<T>::mma(h.npi[s], xi[s], cr);
+    ci = MfmaTraits<T>::mma(h.pi[s], xr[s], ci);
+  }
+}
+
+// rows of the wave's 16-row slab held in accumulator slot q
+template <typename T>
+__device__ __forceinline__ int hop_row(int q) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  return 16 * wave + MfmaTraits<T>::row(lane >> 4, q);
+}
+
+template <typename T>
+__device__ __forceinline__ void hop_acc_load(typename MfmaTraits<T>::acc_t& cr, typename MfmaTraits<T>::acc_t& ci,
+                                             const C2<T>* __restrict__ R, int nb, int nrhs) {
+  const int li = threadIdx.x & 15;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = hop_row<T>(q);
+    C2<T> v = {0, 0};
+    if (r < nb && li < nrhs) v = R[(size_t)r * nrhs + li];
+    cr[q] = v.re;
+    ci[q] = v.im;
+  }
+}
+
+template <typename T, bool kCoherent>
+__device__ __forceinline__ void hop_acc_store(const typename MfmaTraits<T>::acc_t& cr,
+                                              const typename MfmaTraits<T>::acc_t& ci, C2<T>* __restrict__ X, int nb,
+                                              int nrhs) {
+  const int li = threadIdx.x & 15;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = hop_row<T>(q);
+    if (r < nb && li < nrhs) {
+      const C2<T> v = {cr[q], ci[q]};
+      if (kCoherent)
+        st_coherent(X + (size_t)r * nrhs + li, v);
+      else
+        X[(size_t)r * nrhs + li] = v;
+    }
+  }
+}
+
+// Single-launch triangular solve.  Workgroup w owns row block b (forward:
+// b = w, backward: b = nblk-1-w, so blocks without dependencies come first)
+// and subtracts M_bj x_j for every solved block j in dependency order,
+// waiting for block j's flag, then publishes x_b.  The host launches it only
+// when every row block can be resident at once.  Every wait is bounded: after
+// spin_limit polls the workgroup records a timeout in *status and stops
+// waiting, so the kernel always ends (the host then reports an error).
+// R: right-hand sides (forward) or the forward solution y (backward); X gets
+// the solution; flags: nblk ints of this triangle, set to epoch when solved.
+template <typename T, bool kLower>
+__global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict__ A, int ld, int n,
+                                                         const C2<T>* __restrict__ P, const C2<T>* __restrict__ R,
+                                                         C2<T>* __restrict__ X, int nrhs, int* __restrict__ flags,
+                                                         int epoch, int* __restrict__ status, int spin_limit) {
+  typedef typename MfmaTraits<T>::acc_t acc_t;
+  __shared__ int timed_out;
+  const int tid = threadIdx.x;
+  const int nblk = (n + kNB - 1) / kNB;
+  const int b = kLower ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+  const int k0 = b * kNB, nb = min(kNB, n - k0);
+  if (tid == 0) timed_out = 0;
+  acc_t cr, ci;
+  HopTile<T> h;
+  T xr[16], xi[16];
+  if (kLower) {
+    hop_acc_load<T>(cr, ci, R + (size_t)k0 * nrhs, nb, nrhs);
+  } else {  // P_b y_b
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cr[q] = ci[q] = T(0);
+    hop_load_tile(h, P + (size_t)b * kTile, kNB, kNB, kNB, T(1));
+    hop_load_x<T, false>(xr, xi, R + (size_t)k0 * nrhs, nb, nrhs);
+    hop_mma<T>(cr, ci, h, xr, xi);
+  }
+  const int ndep = kLower ? b : nblk - 1 - b;
+  for (int d = 0; d < ndep; ++d) {
+    const int j = kLower ? d : nblk - 1 - d;
+    const int j0 = j * kNB, nj = min(kNB, n - j0);
+    // the coupling block does not depend on x_j: load it before waiting
+    hop_load_tile(h, A + (size_t)k0 * ld + j0, ld, nb, nj, T(-1));
+    if (tid == 0 && !timed_out) {
+      int spins = 0;
+      while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != epoch) {
+        if (++spins > spin_limit) {
+          timed_out = 1;
+          atomicMax(status, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+    hop_load_x<T, true>(xr, xi, X + (size_t)j0 * nrhs, nj, nrhs);
+    hop_mma<T>(cr, ci, h, xr, xi);
+  }
+  hop_acc_store<T, true>(cr, ci, X + (size_t)k0 * nrhs, nb, nrhs);
+  // every store of x_b is performed before the flag is raised
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Stepwise fallback (one launch per block step s, for matrices with more row
+// blocks than can be resident): every remaining row block i does
+// R_i -= M_is R_s in place (R_s is final).
+template <typename T, bool kLower>
+__global__ __launch_bounds__(256) void trsv_step_kernel(const C2<T>* __restrict__ A, int ld, int n,
+                                                        C2<T>* __restrict__ R, int nrhs, int s) {
+  typedef typename MfmaTraits<T>::acc_t acc_t;
+  const int i = kLower ? s + 1 + (int)blockIdx.x : (int)blockIdx.x;
+  const int i0 = i * kNB, ni = min(kNB, n - i0);
+  const int s0 = s * kNB, ns = min(kNB, n - s0);
+  acc_t cr, ci;
+  HopTile<T> h;
+  T xr[16], xi[16];
+  hop_acc_load<T>(cr, ci, R + (size_t)i0 * nrhs, ni, nrhs);
+  hop_load_tile(h, A + (size_t)i0 * ld + s0, ld, ni, ns, T(-1));
+  hop_load_x<T, false>(xr, xi, R + (size_t)s0 * nrhs, ns, nrhs);
+  hop_mma<T>(cr, ci, h, xr, xi);
+  hop_acc_store<T, false>(cr, ci, R + (size_t)i0 * nrhs, ni, nrhs);
+}
+
+// X_b = P_b Y_b for every block (start of the stepwise backward solve).
+template <typename T>
+__global__ __launch_bounds__(256) void block_apply_kernel(const C2<T>* __restrict__ P, int n,
+                                                          const C2<T>* __restrict__ Y, C2<T>* __restrict__ X,
+                                                          int nrhs) {
+  typedef typename MfmaTraits<T>::acc_t acc_t;
+  const int b = blockIdx.x, k0 = b * kNB, nb = min(kNB, n - k0);
+  acc_t cr, ci;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) cr[q] = ci[q] = T(0);
+  HopTile<T> h;
+  T xr[16], xi[16];
+  hop_load_tile(h, P + (size_t)b * kTile, kNB, kNB, kNB, T(1));
+  hop_load_x<T, false>(xr, xi, Y + (size_t)k0 * nrhs, nb, nrhs);
+  hop_mma<T>(cr, ci, h, xr, xi);
+  hop_acc_store<T, false>(cr, ci, X + (size_t)k0 * nrhs, nb, nrhs);
+}
+
+// MFMA layout check (tests): D = A(16x4) B(4x16) with the lane mapping the
+// kernels above assume; in-bounds for any layout.
+template <typename T>
+__global__ void mfma_probe_kernel(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ d) {
+  typedef typename MfmaTraits<T>::acc_t acc_t;
+  const int lane = threadIdx.x;
+  const int li = lane & 15, lk = lane >> 4;
+  acc_t c = acc_t{0, 0, 0, 0};
+  c = MfmaTraits<T>::mma(a[li * 4 + lk], b[lk * 16 + li], c);
+  for (int q = 0; q < 4; ++q) d[MfmaTraits<T>::row(lk, q) * 16 + li] = c[q];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+template <typename T>
+void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st) {
+  const int nblk = (n + kNB - 1) / kNB;
+  block_inverse_kernel<T><<<1, 256, 0, st>>>(A, ld, n, 0, P, info);
+  for (int kb = 0; kb < nblk; ++kb) {
+    const int ncol = nblk - kb - 1;
+    const int nrow = kb >= 1 ? nblk - kb : 0;
+    if (ncol + nrow > 0) lu_panel_kernel<T><<<ncol + nrow, 256, 0, st>>>(A, ld, n, kb, P, ncol);
+    if (ncol > 0) lu_update_kernel<T><<<ncol * ncol, 256, 0, st>>>(A, ld, n, kb, P, info);
+  }
+}
+template void launch_lu<double>(C2<double>*, int, int, C2<double>*, int*, hipStream_t);
+template void launch_lu<float>(C2<float>*, int, int, C2<float>*, int*, hipStream_t);
+
+template <typename T>
+void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
+                     int* flags, int epoch, int* status, int spin_limit, hipStream_t st) {
+  const int nblk = (n + kNB - 1) / kNB;
+  if (chained) {
+    trsv_chain_kernel<T, true><<<nblk, 256, 0, st>>>(A, ld, n, P, B, W, nrhs, flags, epoch, status, spin_limit);
+    trsv_chain_kernel<T, false>
+        <<<nblk, 256, 0, st>>>(A, ld, n, P, W, B, nrhs, flags + nblk, epoch, status, spin_limit);
+    return;
+  }
+  for (int s = 0; s + 1 < nblk; ++s) trsv_step_kernel<T, true><<<nblk - 1 - s, 256, 0, st>>>(A, ld, n, B, nrhs, s);
+  block_apply_kernel<T><<<nblk, 256, 0, st>>>(P, n, B, W, nrhs);
+  for (int s = nblk - 1; s >= 1; --s) trsv_step_kernel<T, false><<<s, 256, 0, st>>>(A, ld, n, W, nrhs, s);
+  (void)hipMemcpyAsync(B, W, (size_t)n * nrhs * sizeof(C2<T>), hipMemcpyDeviceToDevice, st);
+}
+template void launch_lu_solve<double>(const C2<double>*, int, int, const C2<double>*, C2<double>*, C2<double>*, int,
+                                      bool, int*, int, int*, int, hipStream_t);
+template void launch_lu_solve<float>(const C2<float>*, int, int, const C2<float>*, C2<float>*, C2<float>*, int, bool,
+                                     int*, int, int*, int, hipStream_t);
+
+void launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t st) {
+  mfma_probe_kernel<double><<<1, 64, 0, st>>>(a, b, d);
+}
+void launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t st) {
+  mfma_probe_kernel<float><<<1, 64, 0, st>>>(a, b, d);
+}
+
+}  // namespace nemoh

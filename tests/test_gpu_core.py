@@ -102,6 +102,21 @@ def test_forces_match_reference_mixed(name):
     assert err < MIXED_TOL
 
 
+@pytest.mark.parametrize("name", ["Cylinder", "cyl512", "barge_fd"])
+@pytest.mark.parametrize("prec", [core.FP64, core.MIXED])
+def test_stepwise_solve_matches_reference(name, prec):
+    """The one-launch-per-block-step triangular solves (taken when the row
+    blocks outnumber the compute units) against the reference forces."""
+    d, m, ref = _case(name)
+    s = Solver(m, float(d["depth"]), float(d["rho"]), float(d["g"]), precision=prec, tables=_tables())
+    s.core.set_option(core.OPT_STEPWISE_SOLVE, 1)
+    f = s.solve_problems(d["omega"], d["nvel"], d["nds"])
+    s.close()
+    err = _normwise(f, ref)
+    print(f"{name}: stepwise solve, precision {prec}: max normwise rel err {err:.3e}")
+    assert err < (FP64_TOL if prec == core.FP64 else MIXED_TOL)
+
+
 def test_dropin_solve_matches_oracle():
     """nemoh_core_solve, one problem at a time as SOLVE_POTENTIAL_*_DIRECT is
     called, against the oracle's ZIGB/ZIGS/ZPB/ZPS; the second problem of a
