@@ -195,6 +195,9 @@ bool householder_ls(LsMatrix& A, int nl, int ncc, double eps) {
     A(k, k) = ar;
     for (int i = k + 1; i <= nl; ++i) A(i, k) = 0;
   }
+  // no unknowns (LISC kept no decaying exponent): nothing to back-substitute.
+  // HOUSRS (COMPUTE_GREEN_FD.f90:1454) indexes A(0, .) here, out of bounds.
+  if (ncc < 1) return true;
   A(ncc, ntc) = A(ncc, ntc) / A(ncc, ncc);
   for (int i1 = ncc - 1; i1 >= 1; --i1) {
     for (int i = i1 + 1; i <= ncc; ++i) A(i1, ntc) = A(i1, ntc) - A(i1, i) * A(i, ntc);

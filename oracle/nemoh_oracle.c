@@ -281,6 +281,9 @@ static int ora_housrs(double *s, int nl, int ncc, int ns, double eps) {
     S_(k, k) = ar;
     for (int i = kp1; i <= nl; ++i) S_(i, k) = 0;
   }
+  /* NCC = 0 (LISC kept no exponent): the reference's back-substitution
+   * then indexes A(0, .) out of bounds; there is nothing to solve. */
+  if (ncc < 1) return 0;
   for (int j = 1; j <= ns; ++j) {
     const int ncj = ncc + j;
     S_(ncc, ncj) = S_(ncc, ncj) / S_(ncc, ncc);

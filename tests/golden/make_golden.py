@@ -179,6 +179,13 @@ def fn_fixtures() -> None:
             amh = float(harness(f"x0\n1\n{akh0:.17e}\nend\n")[0].split()[1])
             pairs.append((amh * np.tanh(amh), amh))
     pairs.append((0.5, 0.8))
+    # the solver's own call sequence (AKH = w^2 h / g, AMH = X0(AKH)) over a
+    # 40-frequency sweep 0.2-2.0 rad/s at h = 50 m, which includes fits where
+    # an intermediate step keeps no exponent
+    for w in np.linspace(0.2, 2.0, 40):
+        akh0 = w * w * 50.0 / 9.81
+        if akh0 < 20:
+            pairs.append((akh0, float(harness(f"x0\n1\n{akh0:.17e}\nend\n")[0].split()[1])))
     lines = harness("lisc\n%d\n" % len(pairs) + "".join(f"{p[0]:.17e} {p[1]:.17e}\n" for p in pairs) + "end\n")
     i, nexp, lam, ar, akh, amh = 0, [], np.zeros((len(pairs), 31)), np.zeros((len(pairs), 31)), [], []
     for k in range(len(pairs)):
