@@ -67,9 +67,55 @@ def run_precision(core, prec, omegas, nvel, nds, steps, warmup, nrhs=9):
         phases["factor"].append(st.ms_factor)
         phases["solve"].append(st.ms_solve)
         phases["it"].append(st.refine_iterations)
+        phases["update"].append(st.ms_lu_update)
+        phases["update_flops"].append(st.lu_update_flops)
+        phases.setdefault("update_launches", []).append(st.lu_update_launches)
     dt = time.perf_counter() - t0
     del ncore
     return dt, forces, phases
+
+
+REF_HARNESS = ROOT / "oracle" / "_ref" / "harness"
+
+
+def cpu_baseline_reference(m, omegas, nrhs=9, procs=8, stride=4, m_gauss=800):
+    """The reference Solver's own routines (FP64 build of the unmodified
+    sources, oracle/_ref) on the host cores: `procs` single-threaded processes,
+    one per sampled frequency (the reference is single-threaded and
+    frequencies are independent).  Each times VAVFD+VNSFD (or the INFD pair)
+    for every `stride`-th field row against all panels, and GAUSSZ on an
+    m_gauss x 2 m_gauss system; one frequency is then
+    N^2 (1 + NRHS) pairs (assembly + phi per problem, SOLVE_BEM_FD_DIRECT.f90:
+    103-183) + GAUSSZ scaled by (N / m_gauss)^3."""
+    import subprocess
+    n = m.npanels
+    head = f"mesh\n{m.npoints} {m.npanels} {m.isym} {50.0:.17e}\n"
+    head += "".join(f"{x:.17e} {y:.17e} {z:.17e}\n" for x, y, z in m.vertices)
+    head += "".join(f"{a} {b} {c} {d}\n" for a, b, c, d in m.panels + 1)
+    rows = (n + stride - 1) // stride
+    idx = np.linspace(0, omegas.size - 1, procs).round().astype(int)
+    t0 = time.perf_counter()
+    ps = [subprocess.Popen([str(REF_HARNESS)], stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
+          for _ in idx]
+    for p, k in zip(ps, idx):
+        p.stdin.write(head + f"bench\n{2 * math.pi / omegas[k]:.17e} {rows} {stride} {m_gauss}\nend\n")
+        p.stdin.close()
+    t_freq = []
+    for p in ps:
+        last = p.stdout.read().strip().split("\n")[-1].split()
+        if p.wait() != 0:
+            raise RuntimeError("reference harness failed")
+        t_pairs, t_gauss = float(last[2]), float(last[4])
+        t_freq.append(t_pairs / (rows * n) * n * n * (1 + nrhs) + t_gauss * (n / m_gauss) ** 3)
+    wall = time.perf_counter() - t0
+    mean_t = float(np.mean(t_freq))
+    return {"value": procs / mean_t, "unit": "freq-solves/s", "cores": procs, "kind": "reference",
+            "sample": f"reference Solver routines (flang -O2 FP64 build of the unmodified sources), {procs} "
+                      f"single-threaded processes at w = {', '.join(f'{omegas[k]:.3f}' for k in idx)} rad/s: "
+                      f"each times {rows} field rows x {n} panels of VAVFD+VNSFD pairs and GAUSSZ on "
+                      f"{m_gauss}x{2 * m_gauss}, scaled to one frequency as N^2(1+{nrhs}) pairs + "
+                      f"(N/{m_gauss})^3 GAUSSZ; mean {mean_t:.1f} s per frequency per core "
+                      f"(sample wall {wall:.1f} s)"}
 
 
 def cpu_baseline(m, omegas, budget_s=20.0):
@@ -128,11 +174,18 @@ def main():
     rate_fp64 = fp64_steps / t64
     common = sorted(set(f64) & set(fmx))
     err = max(float((np.abs(fmx[k] - f64[k]).max(0) / np.abs(f64[k]).max(0)).max()) for k in common)
-    # roofline of the LU factorisation (trailing updates on MFMA): complex
-    # LU = 8/3 N^3 real flops per factorisation
+    # roofline of the dominant MFMA kernel, the LU trailing update
+    # (lu_update_kernel<float> on the mixed path): algorithmic flops of its
+    # launches (8 * 64 * m^2 each, m = trailing order) over their summed
+    # durations, timed with HIP events on the solver stream
+    upd_ms = float(np.sum(phmx["update"]))
+    upd_flops = float(np.sum(phmx["update_flops"]))
+    upd_launches = int(np.sum(phmx["update_launches"]))
+    achieved = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    # LU + solve flops over the factor + solve phases (complex64 LU 8/3 N^3,
+    # 2 triangular solves 8 N^2 NRHS per refinement sweep)
     lu_flops = 8.0 / 3.0 * n ** 3
     fac_ms = float(np.median(phmx["factor"]))
-    achieved = lu_flops / (fac_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC, "value": round(rate_mixed, 3), "unit": "freq-solves/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1e3 * tmx / a.steps, 3),
@@ -148,12 +201,17 @@ def main():
             "fp64": {k: round(float(np.median(v)), 3) for k, v in ph64.items() if k in ("assembly", "factor", "solve")},
             "mixed": {k: round(float(np.median(v)), 3) for k, v in phmx.items() if k in ("assembly", "factor", "solve")},
             "mixed_refine_iterations_max": int(max(phmx["it"]) if phmx["it"] else 0)},
-        "roofline": {"bound": "mfma", "kernel": "LU factorisation (complex64, lu_panel + lu_update)",
+        "roofline": {"bound": "mfma", "kernel": "lu_update_kernel<float> (LU trailing update, complex64 MFMA)",
                      "achieved": round(achieved, 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK["fp32"], 4), "traffic": None},
+                     "frac": round(achieved / PEAK["fp32"], 4), "traffic": None,
+                     "launches": upd_launches, "avg_launch_us": round(1e3 * upd_ms / max(upd_launches, 1), 3),
+                     "flops_per_launch_avg": upd_flops / max(upd_launches, 1)},
+        "lu_factor_tflops": round(lu_flops / (fac_ms * 1e-3) / 1e12, 3),
+        "assembly_gpairs_per_s": round(n * n / (float(np.median(phmx["assembly"])) * 1e-3) / 1e9, 3),
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(m, omegas)
+        out["cpu_baseline"] = (cpu_baseline_reference(m, omegas) if REF_HARNESS.exists()
+                               else cpu_baseline(m, omegas))
     print(json.dumps(out))
 
 

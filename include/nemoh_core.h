@@ -97,7 +97,9 @@ typedef struct {
 int nemoh_core_freq_setup(nemoh_core *ctx, double period, nemoh_freq *out);
 
 /* Precision of the solve: NEMOH_FP64 (complex128 LU) or NEMOH_MIXED
- * (complex64 LU on MFMA refined in FP64 until max|r| <= tol * max|b|). */
+ * (complex64 LU on MFMA refined in FP64 until max|r| <= tol * max|b|, which
+ * bounds the normwise backward error by tol; tol <= 0 keeps the current
+ * value, default 1e-8; max_iter <= 0 keeps the current limit, default 12). */
 int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter);
 
 /* Execution options (no counterpart in the reference).
@@ -114,6 +116,13 @@ typedef struct {
   int refine_iterations;
   double refine_residual;
   int cache_hit;
+  /* trailing-update kernel of the last factorisation (timing on): summed
+   * launch durations, launch count and algorithmic real flops
+   * (8 * 64 * m^2 per launch, m = order of the trailing matrix) */
+  double ms_lu_update;
+  int lu_update_launches;
+  double lu_update_flops;
+  int lu_precision; /* NEMOH_FP64 or NEMOH_MIXED (complex64 factors) */
 } nemoh_stats;
 int nemoh_core_get_stats(const nemoh_core *ctx, nemoh_stats *out);
 int nemoh_core_enable_timing(nemoh_core *ctx, int on);

@@ -40,7 +40,8 @@ class FreqSetup(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("ms_assembly", C.c_double), ("ms_factor", C.c_double), ("ms_solve", C.c_double),
                 ("ms_total", C.c_double), ("refine_iterations", C.c_int), ("refine_residual", C.c_double),
-                ("cache_hit", C.c_int)]
+                ("cache_hit", C.c_int), ("ms_lu_update", C.c_double), ("lu_update_launches", C.c_int),
+                ("lu_update_flops", C.c_double), ("lu_precision", C.c_int)]
 
 
 _dp = C.POINTER(C.c_double)

@@ -41,7 +41,7 @@ struct nemoh_core {
   int nq_state = 0;  // module NQ of the standalone session
   // precision
   int prec = NEMOH_FP64;
-  double tol = 1e-10;
+  double tol = 1e-8;  // mixed refinement: max|r| <= tol max|b|
   int max_iter = 12;
   // right-hand-side workspace (capacity in right-hand sides)
   int cap = 0;
@@ -62,6 +62,8 @@ struct nemoh_core {
   bool timing = false;
   nemoh_stats stats{};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<hipEvent_t> upd_ev;  // around each trailing-update launch (timing on)
+  int upd_blocks = 0;              // symmetry blocks whose updates upd_ev covers
   std::string err;
 };
 
@@ -219,13 +221,25 @@ int prepare_frequency(nemoh_core* c, double w) {
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
   HIPCHK(c, hipMemsetAsync(c->d_info, 0, 2 * sizeof(int), c->st));
+  const int nblk = (c->nfa + kNB - 1) / kNB;
+  c->upd_blocks = 0;
+  if (c->timing) {
+    const size_t need = 2 * (size_t)nblk * c->nb;
+    while (c->upd_ev.size() < need) {
+      hipEvent_t e;
+      HIPCHK(c, hipEventCreate(&e));
+      c->upd_ev.push_back(e);
+    }
+    c->upd_blocks = c->nb;
+  }
   for (int b = 0; b < c->nb; ++b) {
+    hipEvent_t* ue = c->timing ? c->upd_ev.data() + 2 * (size_t)nblk * b : nullptr;
     if (c->prec == NEMOH_MIXED)
       launch_lu<float>(reinterpret_cast<C2<float>*>(c->D32[b]), c->nfa, c->nfa,
-                       reinterpret_cast<C2<float>*>(c->Pinv32[b]), c->d_info, c->st);
+                       reinterpret_cast<C2<float>*>(c->Pinv32[b]), c->d_info, c->st, ue);
     else
       launch_lu<double>(reinterpret_cast<C2<double>*>(c->D[b]), c->nfa, c->nfa,
-                        reinterpret_cast<C2<double>*>(c->Pinv[b]), c->d_info, c->st);
+                        reinterpret_cast<C2<double>*>(c->Pinv[b]), c->d_info, c->st, ue);
   }
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
@@ -288,7 +302,7 @@ int solve_block(nemoh_core* c, int b, int nrhs) {
              reinterpret_cast<C2<float>*>(c->R32), reinterpret_cast<C2<float>*>(c->W32), nrhs);
     launch_f32_to_f64(c->R32, c->Z[b], cnt, rmax, it > 0, c->st);
     HIPCHK(c, hipMemcpyAsync(c->R, c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
-    launch_matvec<double>(reinterpret_cast<const C2<double>*>(c->D[b]), n, n, c->Z[b], nrhs, c->R, -1.0, 1.0,
+    launch_matvec(c->D[b], n, n, c->Z[b], nrhs, c->R, -1.0, 1.0,
                           c->st);
     HIPCHK(c, hipGetLastError());
     rmax = device_absmax(c, c->R, cnt, &e);
@@ -319,7 +333,7 @@ int run_pipeline(nemoh_core* c, double w, int nrhs) {
   for (int b = 0; b < c->nb; ++b) {
     launch_rhs(c->V, n, c->nsym, nrhs, b == 0 ? 1.0 : -1.0, c->B, c->st);
     if (int r = solve_block(c, b, nrhs)) return r;
-    launch_matvec<double>(reinterpret_cast<const C2<double>*>(c->S[b]), n, n, c->Z[b], nrhs, c->F[b], 1.0, 0.0, c->st);
+    launch_matvec(c->S[b], n, n, c->Z[b], nrhs, c->F[b], 1.0, 0.0, c->st);
   }
   launch_combine(c->Z[0], c->Z[1], c->F[0], c->F[1], n, c->nsym, nrhs, c->rho * w, c->sig, c->pot, c->pres, c->st);
   HIPCHK(c, hipGetLastError());
@@ -404,6 +418,24 @@ static int finish_outputs(nemoh_core* c, int nrhs, double* pressure, int nint, c
       HIPCHK(c, hipEventElapsedTime(&f, c->ev[1], c->ev[2]));
       c->stats.ms_assembly = a;
       c->stats.ms_factor = f;
+      // trailing-update kernels: summed durations and algorithmic flops
+      const int n = c->nfa, nblk = (n + kNB - 1) / kNB;
+      double ms = 0, flops = 0;
+      int cnt = 0;
+      for (int b = 0; b < c->upd_blocks; ++b)
+        for (int kb = 0; kb + 1 < nblk; ++kb) {
+          float e = 0;
+          const size_t base = 2 * ((size_t)nblk * b + kb);
+          HIPCHK(c, hipEventElapsedTime(&e, c->upd_ev[base], c->upd_ev[base + 1]));
+          const double m = n - (double)(kb + 1) * kNB;
+          ms += e;
+          flops += 8.0 * kNB * m * m;
+          ++cnt;
+        }
+      c->stats.ms_lu_update = ms;
+      c->stats.lu_update_launches = cnt;
+      c->stats.lu_update_flops = flops;
+      c->stats.lu_precision = c->factored_prec;
     }
     float s = 0, t = 0;
     HIPCHK(c, hipEventElapsedTime(&s, c->ev[2], c->ev[3]));
@@ -541,6 +573,7 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->W32);
   for (auto& e : c->ev)
     if (e) hipEventDestroy(e);
+  for (hipEvent_t e : c->upd_ev) (void)hipEventDestroy(e);
   if (c->st) hipStreamDestroy(c->st);
   delete c;
 }

@@ -33,9 +33,12 @@ void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, doubl
 // afterwards the blocks below the diagonal hold L (unit block lower), the
 // blocks above hold U (unit block upper) and P (ceil(n/64) blocks of 64x64,
 // row-major) the inverses of the pivot blocks.  *info receives the first row
-// whose pivot has modulus <= 1e-20 (1-based), if any.
+// whose pivot has modulus <= 1e-20 (1-based), if any.  upd_ev (optional,
+// 2*ceil(n/64) events): the trailing-update launch of block step kb is made
+// with hipExtLaunchKernel, which stamps [2 kb] at its start and [2 kb + 1]
+// at its end.
 template <typename T>
-void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st);
+void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, hipEvent_t* upd_ev = nullptr);
 // Solves A x = b for nrhs <= kMaxRhs right-hand sides (B: n x nrhs
 // row-major) with the factors of launch_lu; on return B holds the solution,
 // W is scratch of the same size.  chained: one launch per triangle whose
@@ -46,8 +49,8 @@ void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st);
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
                      int* flags, int epoch, int* status, int spin_limit, hipStream_t st);
-template <typename TM>
-void launch_matvec(const C2<TM>* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
+// Y (n x nrhs) = beta Y + alpha M X, FP64, nrhs <= kMaxRhs, all row-major.
+void launch_matvec(const double2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
                    double beta, hipStream_t st);
 void launch_rhs(const double2* V, int n, int nsym, int nrhs, double bx, double2* B, hipStream_t st);
 void launch_combine(const double2* Z1, const double2* Z2, const double2* F1, const double2* F2, int n, int nsym,

@@ -3,8 +3,8 @@
 //   crek_kernel        wave-Green-function tables (NEMOH.f90:179-274)
 //   assemble_kernel    influence matrices D+-, S+- for one frequency
 //                      (SOLVE_BEM_INFD_DIRECT.f90:84-104 / FD :105-125)
-//   matvec_kernel      Y = alpha*(M X) + beta*Y, streaming M once (phi = S
-//                      sigma, and the FP64 refinement residual)
+//   gemv_kernel        Y = alpha*(M X) + beta*Y on FP64 MFMA, streaming M once
+//                      (phi = S sigma, and the FP64 refinement residual)
 //   rhs/combine/force  right-hand sides, sigma/phi combination of the
 //                      symmetry blocks, pressure and force integration
 //                      (SOLVE_BEM_*_DIRECT.f90:123-151, SOLVE_BEM.f90:91-96,
@@ -95,43 +95,56 @@ __global__ void pairs_kernel(Geometry G, Tables T, const FreqParams* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Y(n x nrhs) = beta*Y + alpha*M(n x n) X(n x nrhs), FP64 accumulation.  One
-// wavefront per matrix row: lanes stride the columns (coalesced row reads of
-// M), each lane accumulates all right-hand sides, then a wave reduction.
-template <typename TM>
-__global__ __launch_bounds__(256) void matvec_kernel(const C2<TM>* __restrict__ M, int ld, int n,
-                                                     const double2* __restrict__ X, int nrhs,
-                                                     double2* __restrict__ Y, double alpha, double beta) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r = blockIdx.x * 4 + wave;
-  if (r >= n) return;
-  double accr[kMaxRhs], acci[kMaxRhs];
+// Y(n x nrhs) = beta*Y + alpha*M(n x n) X(n x nrhs) in FP64 (phi = S sigma and
+// the refinement residual b - D x).  Bound by streaming M once from HBM.
+// One workgroup per 16 rows; its four waves split the columns, each running
+// FP64 16x16x4 MFMAs with A = 16 rows x 4 columns of M and B = the matching
+// 4 rows x 16 right-hand sides of X (nrhs <= 16), 8 k-steps per batch of
+// loads; the four partial blocks are summed through LDS.
+__global__ __launch_bounds__(256) void gemv_kernel(const double2* __restrict__ M, int ld, int n,
+                                                   const double2* __restrict__ X, int nrhs,
+                                                   double2* __restrict__ Y, double alpha, double beta) {
+  __shared__ double red[4][2][16][17];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  const int r0 = blockIdx.x * 16;
+  const int row = r0 + li;
+  const int nk = (n + 3) / 4;  // k-steps of 4 columns
+  const int per = (nk + 3) / 4;
+  const int s0 = wave * per, s1 = min(nk, s0 + per);
+  const double2* mrow = M + (size_t)min(row, n - 1) * ld;
+  v4d cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  for (int s = s0; s < s1; s += 8) {
+    double2 a[8], x[8];
 #pragma unroll
-  for (int q = 0; q < kMaxRhs; ++q) accr[q] = acci[q] = 0.0;
-  const C2<TM>* row = M + (size_t)r * ld;
-  for (int c = lane; c < n; c += 64) {
-    const C2<TM> m = row[c];
-    const double mr = m.re, mi = m.im;
-    const double2* xc = X + (size_t)c * nrhs;
+    for (int u = 0; u < 8; ++u) {
+      const int k = 4 * (s + u) + lk;
+      const bool kin = (s + u < s1) && k < n;
+      a[u] = (kin && row < n) ? mrow[k] : make_double2(0.0, 0.0);
+      x[u] = (kin && li < nrhs) ? X[(size_t)k * nrhs + li] : make_double2(0.0, 0.0);
+    }
 #pragma unroll
-    for (int q = 0; q < kMaxRhs; ++q) {
-      if (q < nrhs) {
-        const double2 v = xc[q];
-        accr[q] += mr * v.x - mi * v.y;
-        acci[q] += mr * v.y + mi * v.x;
-      }
+    for (int u = 0; u < 8; ++u) {
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x, x[u].x, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x, x[u].y, ci, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[u].y, x[u].y, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].y, x[u].x, ci, 0, 0, 0);
     }
   }
+  // FP64 16x16x4 output: lane (li, lk) holds rows lk + 4 q, column li
 #pragma unroll
-  for (int q = 0; q < kMaxRhs; ++q) {
-    if (q < nrhs) {
-      double vr = accr[q], vi = acci[q];
-      for (int off = 32; off > 0; off >>= 1) {
-        vr += __shfl_xor(vr, off, 64);
-        vi += __shfl_xor(vi, off, 64);
-      }
-      if (lane == 0) {
-        double2* y = Y + (size_t)r * nrhs + q;
+  for (int q = 0; q < 4; ++q) {
+    red[wave][0][lk + 4 * q][li] = cr[q];
+    red[wave][1][lk + 4 * q][li] = ci[q];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = lk + 4 * q;
+      if (r0 + r < n && li < nrhs) {
+        const double vr = red[0][0][r][li] + red[1][0][r][li] + red[2][0][r][li] + red[3][0][r][li];
+        const double vi = red[0][1][r][li] + red[1][1][r][li] + red[2][1][r][li] + red[3][1][r][li];
+        double2* y = Y + (size_t)(r0 + r) * nrhs + li;
         const double2 old = (beta == 0.0) ? make_double2(0.0, 0.0) : *y;
         *y = make_double2(beta * old.x + alpha * vr, beta * old.y + alpha * vi);
       }
@@ -268,15 +281,10 @@ void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, doubl
   pairs_kernel<<<grid, 64, 0, st>>>(G, T, P, out);
 }
 
-template <typename TM>
-void launch_matvec(const C2<TM>* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
+void launch_matvec(const double2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
                    double beta, hipStream_t st) {
-  matvec_kernel<TM><<<(n + 3) / 4, 256, 0, st>>>(M, ld, n, X, nrhs, Y, alpha, beta);
+  gemv_kernel<<<(n + 15) / 16, 256, 0, st>>>(M, ld, n, X, nrhs, Y, alpha, beta);
 }
-template void launch_matvec<double>(const C2<double>*, int, int, const double2*, int, double2*, double, double,
-                                    hipStream_t);
-template void launch_matvec<float>(const C2<float>*, int, int, const double2*, int, double2*, double, double,
-                                   hipStream_t);
 
 void launch_rhs(const double2* V, int n, int nsym, int nrhs, double bx, double2* B, hipStream_t st) {
   rhs_kernel<<<(n * nrhs + 255) / 256, 256, 0, st>>>(V, n, nsym, nrhs, bx, B);

@@ -23,6 +23,7 @@
 // panel(k) [L_ik for i > k and U_{k-1,j} for j >= k], update(k) [A_ij for
 // i,j > k; workgroup 0 updates the next diagonal block first and inverts it
 // while the other workgroups finish the trailing matrix (look-ahead)].
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -179,16 +180,21 @@ __device__ __forceinline__ void gemm64(Acc64<T>& acc, View<T> A, View<T> B, Gemm
 }
 
 // ---------------------------------------------------------------------------
-// In-place Gauss-Jordan inversion without pivoting of a 64x64 tile held in
-// registers: thread (ty, tx) = (tid / 16, tid % 16) owns the strided elements
-// (ty + 16 i, tx + 16 j).  Step k publishes the pivot row and column through
-// double-buffered LDS vectors (one barrier per step), then every thread
-// applies  a_kj <- a_kj / p,  a_ik <- -a_ik / p,  a_ij <- a_ij - a_ik a_kj / p,
-// a_kk <- 1 / p  to its 16 elements.
+// In-place inversion without pivoting of a 64x64 block held in LDS, by
+// Gauss-Jordan elimination grouped in 16x16 blocks (level S = 0..3, K the
+// rows / columns of block S):
+//   D = (A_KK)^-1                          one wave, in registers
+//   A_KJ <- D A_KJ             (J != K)    MFMA
+//   A_IJ <- A_IJ - A_IK A_KJ   (I, J != K) MFMA, with the new A_KJ
+//   A_IK <- -A_IK D            (I != K)    MFMA, with the old A_IK
+//   A_KK <- D
+// This is the scalar elimination of GAUSSZ (M_SOLVER.f90:37-60) taken 16
+// pivots at a time: the pivots met inside the 16x16 inversions are the
+// pointwise pivots, in order.
+
 template <typename T>
-struct GjLds {
-  C2<T> col[2][kNB];
-  C2<T> row[2][kNB];
+struct InvLds {
+  C2<T> m[kNB][kNB + 1];
 };
 
 template <typename T>
@@ -197,78 +203,178 @@ __device__ __forceinline__ C2<T> crecip(C2<T> p) {
   return {p.re * s, -p.im * s};
 }
 
-// nb: order of the real block (rows / columns >= nb hold the identity);
-// pivots of modulus <= 1e-20 are reported as row kbase + k + 1 in *info.
 template <typename T>
-__device__ __forceinline__ void gj_invert(C2<T> a[4][4], GjLds<T>& L, int nb, int kbase, int* __restrict__ info) {
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+__device__ __forceinline__ C2<T> shfl_c(C2<T> v, int src) {
+  return {__shfl(v.re, src, 64), __shfl(v.im, src, 64)};
+}
+
+// Reciprocal of a positive real to working precision: the hardware estimate
+// (v_rcp) refined by Newton steps y <- y + y (1 - d y) (two in FP64; the
+// FP32 estimate is already within 1 ulp).
+__device__ __forceinline__ float rcp_pos(float d) { return __builtin_amdgcn_rcpf(d); }
+__device__ __forceinline__ double rcp_pos(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-d, y, 1.0);
+  return fma(y, e, y);
+}
+
+// One wave inverts a 16x16 block held 4 elements per lane (lane l: row
+// l >> 2, columns 4 (l & 3) + e).  Step k broadcasts the pivot row and column
+// with cross-lane shuffles, so the 16 steps need no barrier; every element
+// then takes a <- base - l u with u = row k / p (u_k = 1/p), l = column k,
+// base = a except 0 in column k, and for the pivot row itself l = -1 and
+// base = 0 (so a <- u).  Pivots k < nvalid with |p| <= 1e-20 are reported as
+// row row0 + k + 1 in *info (k >= nvalid is identity padding).
+template <typename T>
+__device__ __forceinline__ void gj16_wave(C2<T> a[4], int nvalid, int row0, int* __restrict__ info) {
+  const int lane = threadIdx.x & 63, r = lane >> 2, cg = 4 * (lane & 3);
+  int first_bad = 16;
 #pragma unroll
-  for (int S = 0; S < 4; ++S) {
-    for (int kl = 0; kl < 16; ++kl) {
-      const int k = 16 * S + kl, buf = k & 1;
-      if (tx == kl) {
+  for (int k = 0; k < 16; ++k) {
+    C2<T> rowv[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) L.col[buf][ty + 16 * i] = a[i][S];
-      }
-      if (ty == kl) {
+    for (int e = 0; e < 4; ++e) rowv[e] = shfl_c(a[e], 4 * k + (lane & 3));
+    const C2<T> sel = a[k & 3];
+    C2<T> colv = shfl_c(sel, 4 * r + (k >> 2));
+    const C2<T> p = shfl_c(sel, 4 * k + (k >> 2));
+    const double pm = (double)p.re * (double)p.re + (double)p.im * (double)p.im;
+    if (k < nvalid && !(pm > 1e-40) && first_bad == 16) first_bad = k;
+    const T s = rcp_pos(p.re * p.re + p.im * p.im);
+    const C2<T> pinv = {p.re * s, -p.im * s};
+    const bool pivot_row = (r == k);
+    if (pivot_row) colv = C2<T>{T(-1), T(0)};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) L.row[buf][tx + 16 * j] = a[S][j];
-      }
-      __syncthreads();
-      const C2<T> p = L.row[buf][k];
-      if (tid == 0 && k < nb && !(sqrt((double)p.re * p.re + (double)p.im * p.im) > 1e-20))
-        atomicMax(info, kbase + k + 1);
-      const C2<T> pinv = crecip(p);
-      C2<T> u[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int c = tx + 16 * j;
-        u[j] = (c == k) ? pinv : cmul(L.row[buf][c], pinv);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const C2<T> l = L.col[buf][ty + 16 * i];
-        const bool rowk = (i == S) && (ty == kl);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int c = tx + 16 * j;
-          const C2<T> base = (c == k) ? C2<T>{0, 0} : a[i][j];
-          a[i][j] = rowk ? u[j] : csub_mul(base, l, u[j]);
-        }
-      }
+    for (int e = 0; e < 4; ++e) {
+      const bool pivot_col = (cg + e == k);
+      const C2<T> u = pivot_col ? pinv : cmul(rowv[e], pinv);
+      const C2<T> base = (pivot_col || pivot_row) ? C2<T>{0, 0} : a[e];
+      a[e] = csub_mul(base, colv, u);
     }
+  }
+  if (lane == 0 && first_bad < 16) atomicMax(info, row0 + first_bad + 1);
+}
+
+// acc += (kNeg ? -1 : 1) A B for the 16x16 blocks A = m[ra.., ka..] and
+// B = m[ka.., cb..] of the LDS matrix (one wave).
+template <typename T, bool kNeg>
+__device__ __forceinline__ void blk16_mma(typename MfmaTraits<T>::acc_t& cr, typename MfmaTraits<T>::acc_t& ci,
+                                          const InvLds<T>& L, int ra, int ka, int cb) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int k = ka + 4 * s + lk;
+    const C2<T> x = L.m[ra + li][k];
+    const C2<T> y = L.m[k][cb + li];
+    const T xr = kNeg ? -x.re : x.re, xi = kNeg ? -x.im : x.im;
+    cr = MfmaTraits<T>::mma(xr, y.re, cr);
+    ci = MfmaTraits<T>::mma(xr, y.im, ci);
+    cr = MfmaTraits<T>::mma(-xi, y.im, cr);
+    ci = MfmaTraits<T>::mma(xi, y.re, ci);
   }
 }
 
 template <typename T>
-__device__ __forceinline__ void store_inverse(const C2<T> a[4][4], C2<T>* __restrict__ P, int nb) {
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+__device__ __forceinline__ void blk16_load(typename MfmaTraits<T>::acc_t& cr, typename MfmaTraits<T>::acc_t& ci,
+                                           const InvLds<T>& L, int r0, int c0) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int q = 0; q < 4; ++q) {
+    const C2<T> v = L.m[r0 + MfmaTraits<T>::row(lk, q)][c0 + li];
+    cr[q] = v.re;
+    ci[q] = v.im;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void blk16_store(const typename MfmaTraits<T>::acc_t& cr,
+                                            const typename MfmaTraits<T>::acc_t& ci, InvLds<T>& L, int r0, int c0) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = ty + 16 * i, c = tx + 16 * j;
-      P[r * kNB + c] = (r < nb && c < nb) ? a[i][j] : C2<T>{0, 0};
+  for (int q = 0; q < 4; ++q) L.m[r0 + MfmaTraits<T>::row(lk, q)][c0 + li] = C2<T>{cr[q], ci[q]};
+}
+
+// t-th block index other than S (t = 0, 1, 2)
+__device__ __forceinline__ int other_block(int t, int S) { return t < S ? t : t + 1; }
+
+// Inverts L.m in place (256 threads; the caller synchronises before and
+// after).  nb: order of the real block (rows / columns >= nb hold the
+// identity); row0: global row of m[0] for the pivot report.
+template <typename T>
+__device__ __forceinline__ void lds_invert(InvLds<T>& L, int nb, int row0, int* __restrict__ info) {
+  typedef typename MfmaTraits<T>::acc_t acc_t;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int S = 0; S < 4; ++S) {
+    const int K = 16 * S;
+    if (wave == 0) {
+      const int r = lane >> 2, cg = 4 * (lane & 3);
+      C2<T> a[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) a[e] = L.m[K + r][K + cg + e];
+      gj16_wave(a, nb - K, row0 + K, info);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) L.m[K + r][K + cg + e] = a[e];
     }
+    __syncthreads();
+    // row blocks A_KJ <- D A_KJ, waves 0..2
+    acc_t cr = {}, ci = {};
+    if (wave < 3) blk16_mma<T, false>(cr, ci, L, K, K, 16 * other_block(wave, S));
+    __syncthreads();
+    if (wave < 3) blk16_store<T>(cr, ci, L, K, 16 * other_block(wave, S));
+    __syncthreads();
+    // 9 trailing blocks and 3 column blocks, 3 per wave
+    acc_t jr[3], ji[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const int job = 3 * wave + t;
+      if (job < 9) {
+        const int ib = other_block(job / 3, S), jb = other_block(job % 3, S);
+        blk16_load<T>(jr[t], ji[t], L, 16 * ib, 16 * jb);
+        blk16_mma<T, true>(jr[t], ji[t], L, 16 * ib, K, 16 * jb);
+      } else {
+        const int ib = other_block(job - 9, S);
+        jr[t] = acc_t{};
+        ji[t] = acc_t{};
+        blk16_mma<T, true>(jr[t], ji[t], L, 16 * ib, K, K);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const int job = 3 * wave + t;
+      if (job < 9)
+        blk16_store<T>(jr[t], ji[t], L, 16 * other_block(job / 3, S), 16 * other_block(job % 3, S));
+      else
+        blk16_store<T>(jr[t], ji[t], L, 16 * other_block(job - 9, S), K);
+    }
+    __syncthreads();
+  }
+}
+
+// P (64x64 row-major) <- L.m, zero outside the nb x nb block.
+template <typename T>
+__device__ __forceinline__ void store_inverse(const InvLds<T>& L, C2<T>* __restrict__ P, int nb) {
+  for (int e = threadIdx.x; e < kTile; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    P[e] = (r < nb && c < nb) ? L.m[r][c] : C2<T>{0, 0};
+  }
 }
 
 // P_kb for one diagonal block read from A (the first block of a factorisation).
 template <typename T>
 __global__ __launch_bounds__(256) void block_inverse_kernel(const C2<T>* __restrict__ A, int ld, int n, int kb,
                                                             C2<T>* __restrict__ P, int* __restrict__ info) {
-  __shared__ GjLds<T> L;
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  __shared__ InvLds<T> L;
   const int k0 = kb * kNB, nb = min(kNB, n - k0);
-  C2<T> a[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = ty + 16 * i, c = tx + 16 * j;
-      a[i][j] = (r < nb && c < nb) ? A[(size_t)(k0 + r) * ld + k0 + c] : C2<T>{T(r == c ? 1 : 0), 0};
-    }
-  gj_invert(a, L, nb, k0, info);
-  store_inverse(a, P + (size_t)kb * kTile, nb);
+  for (int e = threadIdx.x; e < kTile; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    L.m[r][c] = (r < nb && c < nb) ? A[(size_t)(k0 + r) * ld + k0 + c] : C2<T>{T(r == c ? 1 : 0), 0};
+  }
+  __syncthreads();
+  lds_invert(L, nb, k0, info);
+  store_inverse(L, P + (size_t)kb * kTile, nb);
 }
 
 // Panel step kb: workgroups [0, ncol) form L_ib = A_ib P_kb for the row
@@ -296,11 +402,6 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(C2<T>* __restrict__ A, in
   }
 }
 
-template <typename T>
-struct InvLds {
-  C2<T> t[kNB][kNB + 1];
-  GjLds<T> gj;
-};
 template <typename T>
 union UpdLds {
   GemmLds<T> g;
@@ -338,21 +439,12 @@ __global__ __launch_bounds__(256) void lu_update_kernel(C2<T>* __restrict__ A, i
         for (int q = 0; q < 4; ++q) {
           int r, c;
           acc_coord(a, b, MfmaTraits<T>::row(lk, q), r, c);
-          S.inv.t[r][c] = C2<T>{acc.re[a][b][q], acc.im[a][b][q]};
+          S.inv.m[r][c] = (r < nr && c < nr) ? C2<T>{acc.re[a][b][q], acc.im[a][b][q]} : C2<T>{T(r == c ? 1 : 0), 0};
         }
   }
   __syncthreads();
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-  C2<T> a[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = ty + 16 * i, c = tx + 16 * j;
-      a[i][j] = (r < nr && c < nr) ? S.inv.t[r][c] : C2<T>{T(r == c ? 1 : 0), 0};
-    }
-  gj_invert(a, S.inv.gj, nr, r0, info);
-  store_inverse(a, P + (size_t)(kb + 1) * kTile, nr);
+  lds_invert(S.inv, nr, r0, info);
+  store_inverse(S.inv, P + (size_t)(kb + 1) * kTile, nr);
 }
 
 // ---------------------------------------------------------------------------
@@ -590,18 +682,24 @@ __global__ void mfma_probe_kernel(const T* __restrict__ a, const T* __restrict__
 // launchers
 
 template <typename T>
-void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st) {
+void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, hipEvent_t* upd_ev) {
   const int nblk = (n + kNB - 1) / kNB;
   block_inverse_kernel<T><<<1, 256, 0, st>>>(A, ld, n, 0, P, info);
   for (int kb = 0; kb < nblk; ++kb) {
     const int ncol = nblk - kb - 1;
     const int nrow = kb >= 1 ? nblk - kb : 0;
     if (ncol + nrow > 0) lu_panel_kernel<T><<<ncol + nrow, 256, 0, st>>>(A, ld, n, kb, P, ncol);
-    if (ncol > 0) lu_update_kernel<T><<<ncol * ncol, 256, 0, st>>>(A, ld, n, kb, P, info);
+    if (ncol > 0) {
+      if (upd_ev)  // events stamped by the runtime at the kernel's start and end
+        hipExtLaunchKernelGGL(lu_update_kernel<T>, dim3(ncol * ncol), dim3(256), 0, st, upd_ev[2 * kb],
+                              upd_ev[2 * kb + 1], 0, A, ld, n, kb, P, info);
+      else
+        lu_update_kernel<T><<<ncol * ncol, 256, 0, st>>>(A, ld, n, kb, P, info);
+    }
   }
 }
-template void launch_lu<double>(C2<double>*, int, int, C2<double>*, int*, hipStream_t);
-template void launch_lu<float>(C2<float>*, int, int, C2<float>*, int*, hipStream_t);
+template void launch_lu<double>(C2<double>*, int, int, C2<double>*, int*, hipStream_t, hipEvent_t*);
+template void launch_lu<float>(C2<float>*, int, int, C2<float>*, int*, hipStream_t, hipEvent_t*);
 
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,

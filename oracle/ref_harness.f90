@@ -16,6 +16,10 @@
 !                                          -> sets COM_VAR, runs PRE_PROC_MESH, CREK
 !   depth  h                              -> sets Depth only (geometry kept)
 !   pairs  period  (after mesh)           -> every (ISP,IFP) pair of VAV*+VNS*
+!   bench  period rows stride m (after mesh) -> wall seconds of rows x NFA pair
+!                                            evaluations (field rows 1, 1+stride, ...;
+!                                            VAV*+VNS*, same set-up as pairs) and of
+!                                            GAUSSZ on an m x 2m system (M_SOLVER.f90:31)
 !   end
 PROGRAM ref_harness
   USE COM_VAR
@@ -24,6 +28,7 @@ PROGRAM ref_harness
   USE COMPUTE_GREEN_FD, ONLY: VAVFD, VNSFD, LISC, X0, CINT_FD, FF, HOUSRS, SPRBM
   USE PREPARE_MESH
   USE MMesh
+  USE M_SOLVER, ONLY: GAUSSZ
   IMPLICIT NONE
   CHARACTER(LEN=16) :: cmd
   INTEGER :: n, i, j, k, nq_out, nexp, isp, ifp, ios, iperiod
@@ -34,6 +39,11 @@ PROGRAM ref_harness
   REAL :: depth_in
   INTEGER :: nl, ncc, ic
   REAL :: hs(120,32), hc(31), hr(31), hi(31), hp(31)
+  INTEGER :: nrows, mg, stride, r
+  INTEGER(8) :: c0, c1, crate
+  REAL :: tpairs, tgauss, acc
+  COMPLEX, ALLOCATABLE :: zmat(:,:)
+  LOGICAL :: fd
 
   DO
     READ(*,*,IOSTAT=ios) cmd
@@ -151,6 +161,71 @@ PROGRAM ref_harness
           END DO
         END DO
       END IF
+    CASE ('bench')
+      READ(*,*) period, nrows, stride, mg
+      T = period
+      w = 2.*4.*ATAN(1.)/T
+      VSXP = 0.; VSXM = 0.; VSYP = 0.; VSYM = 0.; VSZP = 0.; VSZM = 0.
+      CQ = 0.; QQ = 0.; AMBDA = 0.; AR = 0.
+      gm = 0.
+      DO i = 1, NP - 1
+        DO k = 1, NSYMY + 1
+          bj = (-1.)**(k+1)
+          DO j = i + 1, NP
+            dij = SQRT((X(j)-X(i))**2 + (Y(i)-Y(j)*bj)**2)
+            gm = MAX(dij, gm)
+          END DO
+        END DO
+      END DO
+      fd = .NOT. ((Depth == 0.) .OR. (w*w/G*Depth >= 20))
+      acc = 0.
+      CALL SYSTEM_CLOCK(c0, crate)
+      IF (.NOT. fd) THEN
+        am0 = w**2/G
+        CALL CINT_INFD(am0*gm, n)
+        DO r = 0, nrows - 1
+          isp = 1 + MOD(r * stride, NFA)
+          DO ifp = 1, NFA
+            CALL VAVINFD(1, XG(isp), YG(isp), ZG(isp), isp, ifp)
+            CALL VNSINFD(1, isp, ifp, XG(isp), YG(isp), ZG(isp))
+            acc = acc + VSXP1 + SP2
+          END DO
+        END DO
+      ELSE
+        akh = w*w*Depth/G
+        amh = X0(akh)
+        akh = amh*TANH(amh)
+        am0 = amh/Depth
+        CALL CINT_FD(am0*gm, n)
+        NQ = n
+        CALL LISC(akh, amh, nexp)
+        DO r = 0, nrows - 1
+          isp = 1 + MOD(r * stride, NFA)
+          DO ifp = 1, NFA
+            CALL VAVFD(2, XG(isp), YG(isp), ZG(isp), isp, ifp)
+            CALL VNSFD(am0, amh, nexp, isp, ifp, XG(isp), YG(isp), ZG(isp))
+            acc = acc + VSXP1 + SP2
+          END DO
+        END DO
+      END IF
+      CALL SYSTEM_CLOCK(c1)
+      tpairs = REAL(c1 - c0) / REAL(crate)
+      ALLOCATE(zmat(mg, 2*mg))
+      DO j = 1, mg
+        DO i = 1, mg
+          zmat(i, j) = CMPLX(SIN(1.3*i + 0.7*j), COS(0.9*i - 1.1*j)) / mg
+          zmat(i, j + mg) = (0., 0.)
+        END DO
+        zmat(j, j) = zmat(j, j) + 2.
+        zmat(j, j + mg) = (1., 0.)
+      END DO
+      CALL SYSTEM_CLOCK(c0)
+      CALL GAUSSZ(zmat, mg, mg, 2*mg)
+      CALL SYSTEM_CLOCK(c1)
+      tgauss = REAL(c1 - c0) / REAL(crate)
+      acc = acc + REAL(zmat(1, mg + 1))
+      DEALLOCATE(zmat)
+      WRITE(*,'(L2,I8,ES26.17E3,I8,2ES26.17E3)') fd, nrows, tpairs, mg, tgauss, acc
     CASE ('ff')
       ! FF (COMPUTE_GREEN_FD.f90:1612) at n points: "x ak am"
       READ(*,*) n
