@@ -1,17 +1,9 @@
 // Dense complex linear algebra for the influence-matrix solve on CDNA4.
 //
 // Matrices are row-major, element = {re, im} (double2 / float2), leading
-// dimension ld.  The factorisation is a blocked right-looking LU without row
-// interchanges, like the reference's Gauss-Jordan elimination (GAUSSZ,
-// Solver/Core/M_SOLVER.f90:31-73, whose row search never swaps) -- the BEM
-// matrix has a dominant 1/2 diagonal from the -2 pi n self term.  Each block
-// step is two launches:
-//   panel_kernel  every workgroup re-factors the 64x64 diagonal block in LDS
-//                 (identical arithmetic in each) and then solves one 64-row
-//                 block of L21 = A21 U11^-1 or one 64-column block of
-//                 U12 = L11^-1 A12; workgroup 0 also stores the factored block
-//   update_kernel A22 -= L21 U12 on MFMA (16x16x4 tiles, complex product as
-//                 four real MFMAs), 64x64 output tile per workgroup.
+// dimension ld.  Shared definitions of the block size, the complex element
+// type and the CDNA4 MFMA operand / result layouts; the factorisation and the
+// solves built on them are in lu.hip.
 #pragma once
 #include <hip/hip_runtime.h>
 

@@ -108,6 +108,42 @@ def lisc(akh: float, amh: float):
     return err, n.value, ambda[:n.value].copy(), ar[:n.value].copy()
 
 
+def ff(x: float, ak: float, am: float) -> float:
+    """FF (COMPUTE_GREEN_FD.f90:1612): the regular part of the FD kernel."""
+    L = lib()
+    L.ora_ff_pub.argtypes = [C.c_double] * 3
+    L.ora_ff_pub.restype = C.c_double
+    return L.ora_ff_pub(float(x), float(ak), float(am))
+
+
+def housrs(a, b):
+    """HOUSRS (COMPUTE_GREEN_FD.f90:1454): least-squares solution of
+    a (nl x ncc) x = b by Householder reflections."""
+    a = np.asarray(a, np.float64)
+    nl, ncc = a.shape
+    s = np.zeros((32, 120))  # column-major (120, 32)
+    s[:ncc, :nl] = a.T
+    s[ncc, :nl] = b
+    flat = np.ascontiguousarray(s.ravel())
+    L = lib()
+    L.ora_housrs_pub.argtypes = [_dp, C.c_int, C.c_int]
+    if L.ora_housrs_pub(flat, nl, ncc):
+        raise ValueError("HOUSRS failed")
+    return flat.reshape(32, 120)[ncc, :ncc].copy()
+
+
+def sprbm(c):
+    """SPRBM (COMPUTE_GREEN_FD.f90:1182): roots of sum_i c[i] x^i (c[0] the
+    constant term) by Bairstow factors; returns (ier, roots re, roots im)."""
+    c = np.ascontiguousarray(c, np.float64).copy()
+    ic = c.size
+    rr, rc = np.zeros(ic), np.zeros(ic)
+    L = lib()
+    L.ora_sprbm_pub.argtypes = [_dp, C.c_int, _dp, _dp]
+    ier = L.ora_sprbm_pub(c, ic, rr, rc)
+    return ier, rr, rc
+
+
 def prepare_mesh(vertices, panels, isym):
     v = np.ascontiguousarray(vertices, dtype=np.float64)
     p = np.ascontiguousarray(panels, dtype=np.int32)

@@ -200,6 +200,45 @@ def fn_fixtures() -> None:
         lam[k, :n], ar[k, :n] = r[:, 0], r[:, 1]
     np.savez_compressed(OUT / "fn_lisc.npz", akh=np.array(akh), amh=np.array(amh),
                         nexp=np.array(nexp), ambda=lam, ar=ar)
+
+    # FF on the fit interval for three (AKH, AMH) of the solver's sweep
+    pts = [(x, pairs[k][0], pairs[k][1]) for k in (-40, -20, -3) for x in np.linspace(-0.1, 20.0, 24)]
+    lines = harness("ff\n%d\n" % len(pts) + "".join(f"{x:.17e} {a_:.17e} {m_:.17e}\n" for x, a_, m_ in pts) + "end\n")
+    np.savez_compressed(OUT / "fn_ff.npz", pts=np.array(pts), ff=np.array([float(ln) for ln in lines]))
+
+    # HOUSRS on random least-squares systems up to LISC's largest (91 x 30)
+    shapes, mats, sols = [], [], []
+    for nl, ncc in ((6, 2), (20, 7), (57, 16), (91, 30)):
+        a_ = rng.standard_normal((nl, ncc + 1))
+        lines = harness(f"housrs\n{nl} {ncc}\n" + "".join(" ".join(f"{v:.17e}" for v in row) + "\n" for row in a_)
+                        + "end\n")
+        shapes.append((nl, ncc))
+        mats.append(np.pad(a_, ((0, 91 - nl), (0, 31 - ncc - 1))))
+        sols.append(np.pad(np.array([float(ln) for ln in lines]), (0, 30 - ncc)))
+    np.savez_compressed(OUT / "fn_housrs.npz", shapes=np.array(shapes), a=np.array(mats), x=np.array(sols))
+
+    # SPRBM on polynomials with known real roots and complex pairs, c[0] the
+    # constant term (as LISC builds them, leading coefficient last)
+    polys = []
+    for deg in (4, 7, 12, 20):
+        nc = deg // 3
+        roots = list(rng.uniform(0.05, 0.95, deg - 2 * nc))
+        for _ in range(nc):
+            rho, th = rng.uniform(0.2, 0.9), rng.uniform(0.3, 2.8)
+            roots += [rho * np.exp(1j * th), rho * np.exp(-1j * th)]
+        polys.append(np.real(np.poly(roots))[::-1])
+    out = {"ier": [], "ir": [], "c": [], "rr": [], "ri": []}
+    for c in polys:
+        lines = harness(f"sprbm\n{c.size}\n" + " ".join(f"{v:.17e}" for v in c) + "\nend\n")
+        ir, ier = map(int, lines[0].split())
+        r = np.array([[float(v) for v in ln.split()] for ln in lines[1:1 + c.size]])
+        out["ier"].append(ier)
+        out["ir"].append(ir)
+        out["c"].append(np.pad(c, (0, 21 - c.size)))
+        out["rr"].append(np.pad(r[:, 0], (0, 21 - c.size)))
+        out["ri"].append(np.pad(r[:, 1], (0, 21 - c.size)))
+    np.savez_compressed(OUT / "fn_sprbm.npz", deg=np.array([p_.size - 1 for p_ in polys]),
+                        **{k: np.array(v) for k, v in out.items()})
     print("function fixtures written")
 
 

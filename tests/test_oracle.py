@@ -146,6 +146,49 @@ def test_lisc_fit_reaches_reference_precision():
         assert np.all(lam < 0)
 
 
+# -- FF / HOUSRS / SPRBM, the pieces of LISC (COMPUTE_GREEN_FD.f90) ---------
+def test_ff_matches_reference():
+    d = load_golden("fn_ff.npz")
+    got = np.array([po.ff(*p) for p in d["pts"]])
+    assert np.max(np.abs(got - d["ff"]) / np.abs(d["ff"])) <= 1e-15
+
+
+def test_housrs_matches_reference_and_least_squares():
+    """HOUSRS (:1454) against the reference on systems up to LISC's largest
+    (91 x 30), and against numpy's least-squares solution."""
+    d = load_golden("fn_housrs.npz")
+    for (nl, ncc), a, x in zip(d["shapes"], d["a"], d["x"]):
+        A, b = a[:nl, :ncc], a[:nl, ncc]
+        got = po.housrs(A, b)
+        assert np.max(np.abs(got - x[:ncc])) <= 1e-15 * np.max(np.abs(x[:ncc]))
+        ls = np.linalg.lstsq(A, b, rcond=None)[0]
+        assert np.max(np.abs(got - ls)) < 1e-12 * np.max(np.abs(ls))
+
+
+def _match_roots(a, b):
+    """max distance of a greedy one-to-one pairing of two root sets"""
+    b = list(b)
+    worst = 0.0
+    for r in a:
+        k = int(np.argmin([abs(r - s) for s in b]))
+        worst = max(worst, abs(r - b.pop(k)))
+    return worst
+
+
+def test_sprbm_matches_reference_and_numpy_roots():
+    """SPRBM (:1182) on polynomials of degree 4-20 built from known real
+    roots and complex pairs: the reference's roots and error code, and the
+    roots themselves (numpy.roots of the same coefficients)."""
+    d = load_golden("fn_sprbm.npz")
+    for deg, c, rr, ri, ier in zip(d["deg"], d["c"], d["rr"], d["ri"], d["ier"]):
+        ic = deg + 1
+        e, r, i = po.sprbm(c[:ic])
+        assert e == ier == 0
+        got = r[:deg] + 1j * i[:deg]
+        assert _match_roots(got, rr[:deg] + 1j * ri[:deg]) <= 1e-15
+        assert _match_roots(got, np.roots(c[:ic][::-1])) < 1e-9
+
+
 # -- GOMG geometry (PREPARE_MESH.f90:80-180) --------------------------------
 @pytest.mark.parametrize("name", ["pairs_lidcyl.npz", "pairs_barge.npz"])
 def test_prepare_mesh_matches_reference(name):
