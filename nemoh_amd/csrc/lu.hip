@@ -298,57 +298,83 @@ __device__ __forceinline__ void blk16_store(const typename MfmaTraits<T>::acc_t&
 // t-th block index other than S (t = 0, 1, 2)
 __device__ __forceinline__ int other_block(int t, int S) { return t < S ? t : t + 1; }
 
+// Wave 0: inverts the 16x16 diagonal block at (K, K) of L.m in place.
+template <typename T>
+__device__ __forceinline__ void gj16_block(InvLds<T>& L, int K, int nvalid, int row0, int* __restrict__ info) {
+  const int lane = threadIdx.x & 63, r = lane >> 2, cg = 4 * (lane & 3);
+  C2<T> a[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) a[e] = L.m[K + r][K + cg + e];
+  gj16_wave(a, nvalid, row0, info);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) L.m[K + r][K + cg + e] = a[e];
+}
+
 // Inverts L.m in place (256 threads; the caller synchronises before and
 // after).  nb: order of the real block (rows / columns >= nb hold the
-// identity); row0: global row of m[0] for the pivot report.
+// identity); row0: global row of m[0] for the pivot report.  Look-ahead
+// inside: in the trailing phase of level S, wave 0 first updates the next
+// diagonal block (S+1, S+1) -- which no other job of that phase reads -- and
+// inverts it while waves 1-3 do the other 11 block products.
 template <typename T>
 __device__ __forceinline__ void lds_invert(InvLds<T>& L, int nb, int row0, int* __restrict__ info) {
   typedef typename MfmaTraits<T>::acc_t acc_t;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (wave == 0) gj16_block(L, 0, nb, row0, info);
+  __syncthreads();
 #pragma unroll
   for (int S = 0; S < 4; ++S) {
     const int K = 16 * S;
-    if (wave == 0) {
-      const int r = lane >> 2, cg = 4 * (lane & 3);
-      C2<T> a[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) a[e] = L.m[K + r][K + cg + e];
-      gj16_wave(a, nb - K, row0 + K, info);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) L.m[K + r][K + cg + e] = a[e];
-    }
-    __syncthreads();
     // row blocks A_KJ <- D A_KJ, waves 0..2
     acc_t cr = {}, ci = {};
     if (wave < 3) blk16_mma<T, false>(cr, ci, L, K, K, 16 * other_block(wave, S));
     __syncthreads();
     if (wave < 3) blk16_store<T>(cr, ci, L, K, 16 * other_block(wave, S));
     __syncthreads();
-    // 9 trailing blocks and 3 column blocks, 3 per wave
-    acc_t jr[3], ji[3];
+    // trailing blocks A_IJ -= A_IK A_KJ and column blocks A_IK <- -A_IK D
+    acc_t jr[4], ji[4];
+    int jib[4], jjb[4];  // target block of each job (-1: none)
+    if (S < 3 && wave == 0) {
+      const int K1 = K + 16;
+      acc_t nr_, ni_;
+      blk16_load<T>(nr_, ni_, L, K1, K1);
+      blk16_mma<T, true>(nr_, ni_, L, K1, K, K1);
+      blk16_store<T>(nr_, ni_, L, K1, K1);
+      // the same wave reads the block back in the 16x16-inversion layout
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      gj16_block(L, K1, nb - K1, row0 + K1, info);
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      const int job = 3 * wave + t;
-      if (job < 9) {
-        const int ib = other_block(job / 3, S), jb = other_block(job % 3, S);
-        blk16_load<T>(jr[t], ji[t], L, 16 * ib, 16 * jb);
-        blk16_mma<T, true>(jr[t], ji[t], L, 16 * ib, K, 16 * jb);
-      } else {
-        const int ib = other_block(job - 9, S);
-        jr[t] = acc_t{};
-        ji[t] = acc_t{};
-        blk16_mma<T, true>(jr[t], ji[t], L, 16 * ib, K, K);
+      for (int t = 0; t < 4; ++t) jib[t] = -1;
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        // S < 3: waves 1..3 share the 11 jobs other than (S+1, S+1);
+        // S == 3: the 12 jobs over the four waves
+        const int jj = (S < 3) ? (wave - 1) + 3 * t : 3 * wave + t;
+        const int njobs = (S < 3) ? 11 : 12;
+        const int ntrail = (S < 3) ? 8 : 9;
+        jib[t] = -1;
+        if (t >= ((S < 3) ? 4 : 3) || jj >= njobs) continue;
+        if (jj < ntrail) {
+          const int t9 = (S < 3 && jj >= 4 * S) ? jj + 1 : jj;
+          jib[t] = other_block(t9 / 3, S);
+          jjb[t] = other_block(t9 % 3, S);
+          blk16_load<T>(jr[t], ji[t], L, 16 * jib[t], 16 * jjb[t]);
+          blk16_mma<T, true>(jr[t], ji[t], L, 16 * jib[t], K, 16 * jjb[t]);
+        } else {
+          jib[t] = other_block(jj - ntrail, S);
+          jjb[t] = S;
+          jr[t] = acc_t{};
+          ji[t] = acc_t{};
+          blk16_mma<T, true>(jr[t], ji[t], L, 16 * jib[t], K, K);
+        }
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      const int job = 3 * wave + t;
-      if (job < 9)
-        blk16_store<T>(jr[t], ji[t], L, 16 * other_block(job / 3, S), 16 * other_block(job % 3, S));
-      else
-        blk16_store<T>(jr[t], ji[t], L, 16 * other_block(job - 9, S), K);
-    }
+    for (int t = 0; t < 4; ++t)
+      if (jib[t] >= 0) blk16_store<T>(jr[t], ji[t], L, 16 * jib[t], 16 * jjb[t]);
     __syncthreads();
   }
 }
