@@ -26,6 +26,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "kernels.h"
 #include "linalg.h"
 
@@ -220,40 +222,74 @@ __device__ __forceinline__ double rcp_pos(double d) {
   return fma(y, e, y);
 }
 
-// One wave inverts a 16x16 block held 4 elements per lane (lane l: row
-// l >> 2, columns 4 (l & 3) + e).  Step k broadcasts the pivot row and column
-// with cross-lane shuffles, so the 16 steps need no barrier; every element
-// then takes a <- base - l u with u = row k / p (u_k = 1/p), l = column k,
-// base = a except 0 in column k, and for the pivot row itself l = -1 and
-// base = 0 (so a <- u).  Pivots k < nvalid with |p| <= 1e-20 are reported as
-// row row0 + k + 1 in *info (k >= nvalid is identity padding).
+// Lane movement inside a wave.  bcast16<K>: every lane receives the value of
+// lane K of its own row of 16 lanes (DPP row_newbcast, no LDS traffic);
+// lane_value: the value of one lane, uniform (readlane).
+template <int K>
+__device__ __forceinline__ float bcast16(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + K, 0xF, 0xF, false));
+}
+template <int K>
+__device__ __forceinline__ double bcast16(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x150 + K, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + K, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float lane_value(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// One wave inverts a 16x16 block held 4 elements per lane: lane l owns
+// column l & 15, rows (l >> 4) + 4 e.  Pivot step K: the pivot column value
+// of each of the lane's rows sits in lane K of the lane's own row of 16 lanes
+// (DPP broadcast), the pivot-row value of the lane's column in lane
+// 16 (K & 3) + column (one shuffle), the pivot itself in lane
+// 16 (K & 3) + K (readlane).  Every element then takes a <- base - l u with
+// u = row K / p (u_K = 1/p), l = column K, base = a except 0 in column K,
+// and for the pivot row l = -1 and base = 0 (so a <- u).
+template <typename T, int K>
+__device__ __forceinline__ void gj16_step(C2<T> a[4], int nvalid, int& first_bad) {
+  const int lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+  const C2<T> sel = a[K >> 2];
+  const C2<T> rowv = shfl_c(sel, 16 * (K & 3) + c);
+  C2<T> colv[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) colv[e] = C2<T>{bcast16<K>(a[e].re), bcast16<K>(a[e].im)};
+  const C2<T> p = {lane_value(sel.re, 16 * (K & 3) + K), lane_value(sel.im, 16 * (K & 3) + K)};
+  const double pm = (double)p.re * (double)p.re + (double)p.im * (double)p.im;
+  if (K < nvalid && !(pm > 1e-40) && first_bad == 16) first_bad = K;
+  const T s = rcp_pos(p.re * p.re + p.im * p.im);
+  const C2<T> pinv = {p.re * s, -p.im * s};
+  const bool pivot_col = (c == K);
+  const C2<T> u = pivot_col ? pinv : cmul(rowv, pinv);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const bool pivot_row = (e == (K >> 2)) && (g == (K & 3));
+    const C2<T> l = pivot_row ? C2<T>{T(-1), T(0)} : colv[e];
+    const C2<T> base = (pivot_col || pivot_row) ? C2<T>{0, 0} : a[e];
+    a[e] = csub_mul(base, l, u);
+  }
+}
+
+template <typename T, int... Ks>
+__device__ __forceinline__ void gj16_steps(C2<T> a[4], int nvalid, int& first_bad, std::integer_sequence<int, Ks...>) {
+  (gj16_step<T, Ks>(a, nvalid, first_bad), ...);
+}
+
+// Pivots K < nvalid with |p| <= 1e-20 are reported as row row0 + K + 1 in
+// *info (K >= nvalid is identity padding).
 template <typename T>
 __device__ __forceinline__ void gj16_wave(C2<T> a[4], int nvalid, int row0, int* __restrict__ info) {
-  const int lane = threadIdx.x & 63, r = lane >> 2, cg = 4 * (lane & 3);
   int first_bad = 16;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    C2<T> rowv[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) rowv[e] = shfl_c(a[e], 4 * k + (lane & 3));
-    const C2<T> sel = a[k & 3];
-    C2<T> colv = shfl_c(sel, 4 * r + (k >> 2));
-    const C2<T> p = shfl_c(sel, 4 * k + (k >> 2));
-    const double pm = (double)p.re * (double)p.re + (double)p.im * (double)p.im;
-    if (k < nvalid && !(pm > 1e-40) && first_bad == 16) first_bad = k;
-    const T s = rcp_pos(p.re * p.re + p.im * p.im);
-    const C2<T> pinv = {p.re * s, -p.im * s};
-    const bool pivot_row = (r == k);
-    if (pivot_row) colv = C2<T>{T(-1), T(0)};
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const bool pivot_col = (cg + e == k);
-      const C2<T> u = pivot_col ? pinv : cmul(rowv[e], pinv);
-      const C2<T> base = (pivot_col || pivot_row) ? C2<T>{0, 0} : a[e];
-      a[e] = csub_mul(base, colv, u);
-    }
-  }
-  if (lane == 0 && first_bad < 16) atomicMax(info, row0 + first_bad + 1);
+  gj16_steps<T>(a, nvalid, first_bad, std::make_integer_sequence<int, 16>{});
+  if ((threadIdx.x & 63) == 0 && first_bad < 16) atomicMax(info, row0 + first_bad + 1);
 }
 
 // acc += (kNeg ? -1 : 1) A B for the 16x16 blocks A = m[ra.., ka..] and
@@ -301,13 +337,13 @@ __device__ __forceinline__ int other_block(int t, int S) { return t < S ? t : t 
 // Wave 0: inverts the 16x16 diagonal block at (K, K) of L.m in place.
 template <typename T>
 __device__ __forceinline__ void gj16_block(InvLds<T>& L, int K, int nvalid, int row0, int* __restrict__ info) {
-  const int lane = threadIdx.x & 63, r = lane >> 2, cg = 4 * (lane & 3);
+  const int lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
   C2<T> a[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) a[e] = L.m[K + r][K + cg + e];
+  for (int e = 0; e < 4; ++e) a[e] = L.m[K + g + 4 * e][K + c];
   gj16_wave(a, nvalid, row0, info);
 #pragma unroll
-  for (int e = 0; e < 4; ++e) L.m[K + r][K + cg + e] = a[e];
+  for (int e = 0; e < 4; ++e) L.m[K + g + 4 * e][K + c] = a[e];
 }
 
 // Inverts L.m in place (256 threads; the caller synchronises before and
