@@ -217,7 +217,7 @@ int prepare_frequency(nemoh_core* c, double w) {
     out.dm32 = c->D32[1];
   }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->st));
-  launch_assemble(c->G, c->T, c->d_par, c->nsym, out, c->st);
+  launch_assemble(c->G, c->T, c->d_par, c->h_par.fd, c->nsym, out, c->st);
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
   HIPCHK(c, hipMemsetAsync(c->d_info, 0, 2 * sizeof(int), c->st));
@@ -695,7 +695,7 @@ int nemoh_core_pairs(nemoh_core* c, double period, double amh, int nexp, const d
   const size_t cnt = (size_t)c->nfa * c->nfa * 16;
   double* d = nullptr;
   HIPCHK(c, hipMalloc((void**)&d, cnt * sizeof(double)));
-  launch_pairs(c->G, c->T, c->d_par, d, c->st);
+  launch_pairs(c->G, c->T, c->d_par, c->h_par.fd, d, c->st);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, d, cnt * sizeof(double), hipMemcpyDeviceToHost, c->st);
   if (e == hipSuccess) e = hipStreamSynchronize(c->st);

@@ -87,6 +87,18 @@ __device__ __forceinline__ cplx cexp_(cplx z) {
 }
 __device__ __forceinline__ cplx clog_(cplx z) { return {log(hypot(z.re, z.im)), atan2(z.im, z.re)}; }
 
+// 1/sqrt(x) for x > 0 to working precision: the hardware estimate
+// (v_rsq_f64) refined by two Newton steps y <- y + y (1 - x y^2) / 2.  One
+// multiply-add chain instead of a correctly rounded square root followed by
+// divisions; the result differs from 1/sqrt(x) by about an ulp.
+__device__ __forceinline__ double rsqrt_pos(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 // Complex exponential-integral approximation GG(z) ~ e^z E1(z) +- i pi e^z,
 // ELEMENTARY_FNS.f90:31-81 (same regimes and coefficients).
 __device__ cplx gg(cplx z, cplx cex) {
@@ -166,12 +178,13 @@ __device__ __forceinline__ RankineImage rankine_image(const Geometry& G, int j, 
   const double xoj = G.xg[j], yoj = G.yg[j] * my, zoj = G.zg[j] * mz - dh * mh;
   const double a3j = G.xn[j], a6j = G.yn[j] * my, a9j = G.zn[j] * mz;
   const double dx = xo - xoj, dy = yo - yoj, dz = zo - zoj;
-  const double ro = sqrt(dx * dx + dy * dy + dz * dz);
+  const double ro2 = dx * dx + dy * dy + dz * dz;
   const double tdis = G.tdis[j];
   RankineImage r;
-  if (ro > 7.0 * tdis) {
-    r.a = G.aire[j] / ro;
-    const double asro = r.a / (ro * ro);
+  if (ro2 > 49.0 * tdis * tdis) {  // RO > 7 TDIS
+    const double ri = rsqrt_pos(ro2);
+    r.a = G.aire[j] * ri;
+    const double asro = r.a * ri * ri;
     r.vx = -dx * asro;
     r.vy = -dy * asro;
     r.vz = -dz * asro;
@@ -358,11 +371,11 @@ __device__ WavePart wave_infd_image(const Geometry& G, const Tables& T, const Fr
     const double akr = ak0 * rrr;
     const double zzz = zmiii + G.zg[j];
     const double akz = ak0 * zzz;
-    const double dd = sqrt(rrr * rrr + zzz * zzz);
+    const double dd2 = rrr * rrr + zzz * zzz;
     double psurr = 0.0;
-    if (dd > eps) {
-      const double t = ak0 * dd;
-      psurr = M_PI / (t * t * t);
+    if (dd2 > eps * eps) {  // DD > EPS; psurr = pi / (k dd)^3
+      const double ri = rsqrt_pos(dd2);
+      psurr = M_PI / (ak0 * ak0 * ak0) * (ri * ri * ri);
     }
     double pd1z = 0, pd2z = 0, vz1 = 0, vz2 = 0, vr1 = 0, vr2 = 0;
     if (akz <= -1.5E-6) {  // (the reference leaves values unset above)
@@ -485,18 +498,18 @@ struct FdImage {
   double pd1z, pd2z, vz2, pd1x, vr2, psr;
 };
 
-__device__ __forceinline__ FdImage fd_image(const Tables& T, const RWeights& rw, double am0, double rrr,
-                                            double akr, double zzz, double epz_sq_s, double sq_c,
-                                            double scds, double ssds) {
+__device__ __forceinline__ FdImage fd_image(const Tables& T, const RWeights& rw, double am0, double pi_k,
+                                            double pi_k3, double rrr, double akr, double zzz, double epz_sq_s,
+                                            double sq_c, double scds, double ssds) {
   FdImage m;
   const double akz = am0 * zzz;
-  const double dd = sqrt(rrr * rrr + zzz * zzz);
+  const double dd2 = rrr * rrr + zzz * zzz;
   double psurr = 0.0;
   m.psr = 0.0;
-  if (dd > 0.0001) {
-    const double rv = am0 * dd;
-    m.psr = M_PI / rv;
-    psurr = M_PI / (rv * rv * rv);
+  if (dd2 > 1e-8) {  // DD > 1e-4; psr = pi / (k dd), psurr = pi / (k dd)^3
+    const double ri = rsqrt_pos(dd2);
+    m.psr = pi_k * ri;
+    psurr = pi_k3 * ri * ri * ri;
   }
   if (akz > -16.0) {
     double pd2x;
@@ -535,11 +548,12 @@ __device__ __forceinline__ void vsd(const Geometry& G, int j, double mjj, double
   const double xnf = G.xn[j], ynf = G.yn[j], znf = G.zn[j];
   const double td = G.tdis[j];
   const double dx = xgi - G.xg[j], dy = ygi - G.yg[j] * mjj, dz = zig - G.zg[j];
-  const double ro = sqrt(dx * dx + dy * dy + dz * dz);
+  const double ro2 = dx * dx + dy * dy + dz * dz;
   const double gz = dx * xnf + dy * ynf * mjj + dz * znf;
-  if (ro > 7.0 * td) {
-    fs = -G.aire[j] / (ro * 2.0 * qpi);
-    const double asro = fs / (ro * ro);
+  if (ro2 > 49.0 * td * td) {  // RO > 7 TDIS
+    const double ri = rsqrt_pos(ro2);
+    fs = -G.aire[j] * ri / (2.0 * qpi);
+    const double asro = fs * ri * ri;
     vx = -dx * asro;
     vy = -dy * asro;
     vz = -dz * asro;
@@ -646,10 +660,11 @@ __device__ WavePart wave_fd_image(const Geometry& G, const Tables& T, const Freq
         ssds = sq * (sik + dsk * csk);
       }
     }
-    const FdImage m1 = fd_image(T, rw, am0, rrr, akr, zz1, psk, sck, scds, ssds);
-    const FdImage m2 = fd_image(T, rw, am0, rrr, akr, zgj - zmiii - 2.0 * h, psk, sck, scds, ssds);
-    const FdImage m3 = fd_image(T, rw, am0, rrr, akr, zmiii - zgj - 2.0 * h, psk, sck, scds, ssds);
-    const FdImage m4 = fd_image(T, rw, am0, rrr, akr, -zgj - zmiii - 4.0 * h, psk, sck, scds, ssds);
+    const double pi_k = M_PI / am0, pi_k3 = pi_k / (am0 * am0);
+    const FdImage m1 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zz1, psk, sck, scds, ssds);
+    const FdImage m2 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zgj - zmiii - 2.0 * h, psk, sck, scds, ssds);
+    const FdImage m3 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zmiii - zgj - 2.0 * h, psk, sck, scds, ssds);
+    const FdImage m4 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, -zgj - zmiii - 4.0 * h, psk, sck, scds, ssds);
     w.fs1 = cof1 * ((m1.pd1z + m2.pd1z + m3.pd1z + m4.pd1z) - m1.psr - m2.psr - m3.psr - m4.psr);
     w.fs2 = cof2 * (m1.pd2z + m2.pd2z + m3.pd2z + m4.pd2z);
     if (rrr > eps) {
@@ -678,9 +693,10 @@ __device__ WavePart wave_fd_image(const Geometry& G, const Tables& T, const Freq
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double rr = r2 + zp[k] * zp[k];
-        const double ro = sqrt(rr);
-        f[k] = (ro > eps) ? aqt / ro : 0.0;
-        a[k] = (ro > eps) ? f[k] / rr : 0.0;
+        const bool in = rr > eps * eps;  // RO > 1e-4
+        const double ri = rsqrt_pos(in ? rr : 1.0);
+        f[k] = in ? aqt * ri : 0.0;
+        a[k] = in ? f[k] * ri * ri : 0.0;
       }
       w.fs1 += f[0] + f[1] + f[2] + f[3];
       w.vx1 += -xpg * a[0] + -xpg * a[1] + -xpg * a[2] + -xpg * a[3];
@@ -709,16 +725,19 @@ __device__ WavePart wave_fd_image(const Geometry& G, const Tables& T, const Freq
 
 // All 16 outputs of one pair (order SP1 SM1 SP2 SM2, VS{X,Y,Z}P1,
 // VS{X,Y,Z}M1, VS{X,Y,Z}P2, VS{X,Y,Z}M2).
+// kFd: finite depth (P.fd == 1) or infinite depth, a compile-time choice so
+// that each assembly kernel carries only its own regime's code.
+template <bool kFd>
 __device__ void pair16(const Geometry& G, const Tables& T, const FreqParams& P, int i, int j, double o[16]) {
   double rk[8];
-  if (P.fd) rankine(G, i, j, P.nsym, P.zer, 2, 2.0 * P.depth, rk);
+  if (kFd) rankine(G, i, j, P.nsym, P.zer, 2, 2.0 * P.depth, rk);
   else rankine(G, i, j, P.nsym, P.zer, 1, 0.0, rk);
-  const WavePart w1 = P.fd ? wave_fd_image(G, T, P, i, j, 1.0) : wave_infd_image(G, T, P, i, j, 1.0);
+  const WavePart w1 = kFd ? wave_fd_image(G, T, P, i, j, 1.0) : wave_infd_image(G, T, P, i, j, 1.0);
   WavePart w2 = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (P.nsym == 1) w2 = P.fd ? wave_fd_image(G, T, P, i, j, -1.0) : wave_infd_image(G, T, P, i, j, -1.0);
+  if (P.nsym == 1) w2 = kFd ? wave_fd_image(G, T, P, i, j, -1.0) : wave_infd_image(G, T, P, i, j, -1.0);
   double s1, s2, s3;  // scaling of (potential 1, potential 2 / gradient 1, gradient 2)
   double g1, g2;
-  if (P.fd) {
+  if (kFd) {
     s1 = 1.0;
     s2 = 1.0;
     g1 = 1.0;
@@ -740,7 +759,7 @@ __device__ void pair16(const Geometry& G, const Tables& T, const FreqParams& P, 
     o[3] = -(w1.fs2 - w2.fs2) * s2;
   }
   (void)s3;
-  const double sg = P.fd ? 1.0 : -1.0;  // FD adds, INFD subtracts
+  const double sg = kFd ? 1.0 : -1.0;  // FD adds, INFD subtracts
   o[4] = rk[2] + sg * (w1.vx1 + w2.vx1) * g1;
   o[5] = rk[3] + sg * (w1.vy1 + w2.vy1) * g1;
   o[6] = rk[4] + sg * (w1.vz1 + w2.vz1) * g1;

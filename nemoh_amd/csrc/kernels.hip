@@ -57,15 +57,17 @@ __global__ void crek_kernel(const double* __restrict__ xr, const double* __restr
 // Assembly.  blockIdx.y = field point (row) i, 256 consecutive source panels
 // j per workgroup: one wavefront covers 64 panels of one row, so the field
 // point is uniform across the wave and panel data loads are coalesced.
-template <bool kSym, bool kF32>
-__global__ __launch_bounds__(256) void assemble_kernel(Geometry G, Tables T, const FreqParams* __restrict__ Pp,
+// Finite depth fits two waves per SIMD (a few spilled values); the infinite-
+// depth path, with its near-free-surface quadrature, keeps one.
+template <bool kFd, bool kSym, bool kF32>
+__global__ __launch_bounds__(256, kFd ? 2 : 1) void assemble_kernel(Geometry G, Tables T, const FreqParams* __restrict__ Pp,
                                                        MatOut out) {
   const int i = blockIdx.y;
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= G.n) return;
   const FreqParams& P = *Pp;
   double o[16];
-  pair16(G, T, P, i, j, o);
+  pair16<kFd>(G, T, P, i, j, o);
   const double xn = G.xn[i], yn = G.yn[i], zn = G.zn[i];
   const size_t e = (size_t)i * out.ld + j;
   const double dpr = o[4] * xn + o[5] * yn + o[6] * zn;
@@ -83,12 +85,13 @@ __global__ __launch_bounds__(256) void assemble_kernel(Geometry G, Tables T, con
 }
 
 // Raw 16-value dump of every pair (tests only; small meshes).
+template <bool kFd>
 __global__ void pairs_kernel(Geometry G, Tables T, const FreqParams* __restrict__ Pp, double* __restrict__ out) {
   const int i = blockIdx.y;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= G.n) return;
   double o[16];
-  pair16(G, T, *Pp, i, j, o);
+  pair16<kFd>(G, T, *Pp, i, j, o);
   double* d = out + ((size_t)i * G.n + j) * 16;
 #pragma unroll
   for (int k = 0; k < 16; ++k) d[k] = o[k];
@@ -263,22 +266,30 @@ void launch_crek(const double* xr, const double* xz, double* tab, hipStream_t st
   crek_kernel<<<(kNR * kNZ + 127) / 128, 128, 0, st>>>(xr, xz, tab);
 }
 
-void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, int nsym, const MatOut& out,
-                     hipStream_t st) {
+template <bool kFd>
+static void launch_assemble_regime(const Geometry& G, const Tables& T, const FreqParams* P, int nsym,
+                                   const MatOut& out, hipStream_t st) {
   dim3 grid((G.n + 255) / 256, G.n);
   const bool f32 = out.dp32 != nullptr;
   if (nsym) {
-    if (f32) assemble_kernel<true, true><<<grid, 256, 0, st>>>(G, T, P, out);
-    else assemble_kernel<true, false><<<grid, 256, 0, st>>>(G, T, P, out);
+    if (f32) assemble_kernel<kFd, true, true><<<grid, 256, 0, st>>>(G, T, P, out);
+    else assemble_kernel<kFd, true, false><<<grid, 256, 0, st>>>(G, T, P, out);
   } else {
-    if (f32) assemble_kernel<false, true><<<grid, 256, 0, st>>>(G, T, P, out);
-    else assemble_kernel<false, false><<<grid, 256, 0, st>>>(G, T, P, out);
+    if (f32) assemble_kernel<kFd, false, true><<<grid, 256, 0, st>>>(G, T, P, out);
+    else assemble_kernel<kFd, false, false><<<grid, 256, 0, st>>>(G, T, P, out);
   }
 }
 
-void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, double* out, hipStream_t st) {
+void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, int fd, int nsym, const MatOut& out,
+                     hipStream_t st) {
+  if (fd) launch_assemble_regime<true>(G, T, P, nsym, out, st);
+  else launch_assemble_regime<false>(G, T, P, nsym, out, st);
+}
+
+void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int fd, double* out, hipStream_t st) {
   dim3 grid((G.n + 63) / 64, G.n);
-  pairs_kernel<<<grid, 64, 0, st>>>(G, T, P, out);
+  if (fd) pairs_kernel<true><<<grid, 64, 0, st>>>(G, T, P, out);
+  else pairs_kernel<false><<<grid, 64, 0, st>>>(G, T, P, out);
 }
 
 void launch_matvec(const double2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
