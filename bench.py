@@ -43,6 +43,44 @@ def workload():
     return m, omegas, om, kind, nvel, nds
 
 
+def workload_c3():
+    """Semi-submersible: 4 columns r = 6 m at 50 m spacing, pontoons 8 x 6 m,
+    draft 20 m, 12,048 panels, infinite depth, 6 DOF + 8 headings (0-315 deg)
+    = 14 RHS, 60 frequencies 0.1-2.5 rad/s."""
+    from nemoh_amd import bodycond as bcm
+    from nemoh_amd import mesh as nmesh
+    m = nmesh.semisub()
+    omegas = np.linspace(0.1, 2.5, 60)
+    modes = bcm.six_dof()
+    om, kind, nvel = bcm.problems(m, omegas, list(np.arange(8) * 45.0), modes, 0.0, 9.81, 1025.0)
+    nds = bcm.integration(m, modes)
+    return m, omegas, nvel, nds
+
+
+def run_c3(steps):
+    """C3 on `steps` frequencies spread over the band, FP64 then mixed."""
+    from nemoh_amd import core as ncore
+    m, omegas, nvel, nds = workload_c3()
+    nrhs = 14
+    idx = np.linspace(0, omegas.size - 1, steps).round().astype(int)
+    cols = np.concatenate([np.arange(k * nrhs, (k + 1) * nrhs) for k in idx])
+    c = ncore.Core(m.vertices, m.panels, 0, 0.0, 1025.0, 9.81)
+    out = {}
+    for name, prec in (("fp64", ncore.FP64), ("mixed", ncore.MIXED)):
+        dt, forces, ph = run_precision(c, prec, omegas[idx], nvel[:, cols], nds, steps, 1, nrhs=nrhs)
+        out[name] = (steps / dt, forces, ph)
+    c.close()
+    err = max(float((np.abs(out["mixed"][1][k] - out["fp64"][1][k]).max(0) /
+                     np.abs(out["fp64"][1][k]).max(0)).max()) for k in out["fp64"][1])
+    return {"workload": f"semi-submersible, {m.npanels} panels, infinite depth, 14 RHS (6 DOF + 8 headings), "
+                        f"{steps} of 60 frequencies 0.1-2.5 rad/s: " + ", ".join(f"{w:.3f}" for w in omegas[idx]),
+            "unit": "freq-solves/s", "mixed": round(out["mixed"][0], 3), "fp64": round(out["fp64"][0], 3),
+            "speedup_mixed_vs_fp64": round(out["mixed"][0] / out["fp64"][0], 3), "max_rel_err_mixed_vs_fp64": err,
+            "refinement": "max|r| <= 1e-8 max|b| (normwise backward error <= 1e-8)",
+            "phase_ms_median": {name: {k: round(float(np.median(v)), 2) for k, v in out[name][2].items()
+                                       if k in ("assembly", "factor", "solve")} for name in out}}
+
+
 def run_precision(core, prec, omegas, nvel, nds, steps, warmup, nrhs=9):
     from nemoh_amd import core as ncore
     core.set_precision(prec)
@@ -156,6 +194,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--c3-steps", type=int, default=3, help="frequencies of the 12k-panel case (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: each frequency solve is independent, so N GPUs "
@@ -209,6 +248,8 @@ def main():
         "lu_factor_tflops": round(lu_flops / (fac_ms * 1e-3) / 1e12, 3),
         "assembly_gpairs_per_s": round(n * n / (float(np.median(phmx["assembly"])) * 1e-3) / 1e9, 3),
     }
+    if a.c3_steps > 0:
+        out["c3"] = run_c3(a.c3_steps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = (cpu_baseline_reference(m, omegas) if REF_HARNESS.exists()
                                else cpu_baseline(m, omegas))

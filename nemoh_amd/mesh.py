@@ -245,31 +245,61 @@ def translate(m: Mesh, dx=0.0, dy=0.0, dz=0.0) -> Mesh:
     return Mesh(m.vertices + np.array([dx, dy, dz]), m.panels.copy(), m.isym)
 
 
-def semisub(n_target: int = 12000, col_radius: float = 6.0, spacing: float = 50.0,
-            pontoon_w: float = 8.0, pontoon_h: float = 6.0, draft: float = 20.0) -> Mesh:
-    """Four-column semi-submersible: columns at (+-s/2, +-s/2) and box
-    pontoons joining neighbouring columns at the keel.  Pontoon and column
-    meshes are separate closed bodies overlapping at the joints (a standard
-    simplification for synthetic load); panel counts are scaled towards
-    ``n_target``."""
+def closed_box(length: float, width: float, height: float, nx: int, ny: int, nz: int,
+               center=(0.0, 0.0, 0.0)) -> Mesh:
+    """Fully submerged closed box (all six faces), axes along x/y/z, outward
+    normals; ``nx x ny`` panels on top and bottom, ``nz`` panels high."""
+    cx, cy, cz = center
+    pool = _VertexPool(max(length, width, height))
+    xs = np.linspace(cx - length / 2, cx + length / 2, nx + 1)
+    ys = np.linspace(cy - width / 2, cy + width / 2, ny + 1)
+    zs = np.linspace(cz - height / 2, cz + height / 2, nz + 1)
+    quads = []
+    for zv in (zs[0], zs[-1]):
+        for i in range(nx):
+            for j in range(ny):
+                quads.append([pool.add((xs[i], ys[j], zv)), pool.add((xs[i + 1], ys[j], zv)),
+                              pool.add((xs[i + 1], ys[j + 1], zv)), pool.add((xs[i], ys[j + 1], zv))])
+    for k in range(nz):
+        for i in range(nx):
+            for yv in (ys[0], ys[-1]):
+                quads.append([pool.add((xs[i], yv, zs[k])), pool.add((xs[i + 1], yv, zs[k])),
+                              pool.add((xs[i + 1], yv, zs[k + 1])), pool.add((xs[i], yv, zs[k + 1]))])
+        for j in range(ny):
+            for xv in (xs[0], xs[-1]):
+                quads.append([pool.add((xv, ys[j], zs[k])), pool.add((xv, ys[j + 1], zs[k])),
+                              pool.add((xv, ys[j + 1], zs[k + 1])), pool.add((xv, ys[j], zs[k + 1]))])
+    pts = pool.array()
+    tol = 1e-9 * max(length, width, height)
+
+    def outward(c):
+        if abs(abs(c[2] - cz) - height / 2) < tol:
+            return (0.0, 0.0, math.copysign(1.0, c[2] - cz))
+        if abs(abs(c[0] - cx) - length / 2) < tol:
+            return (math.copysign(1.0, c[0] - cx), 0.0, 0.0)
+        return (0.0, math.copysign(1.0, c[1] - cy), 0.0)
+
+    return Mesh(pts, _orient(pts, quads, outward), 0)
+
+
+def semisub(col_radius: float = 6.0, spacing: float = 50.0, pontoon_w: float = 8.0, pontoon_h: float = 6.0,
+            draft: float = 20.0, n_theta: int = 48, n_z: int = 30, n_r: int = 8, nx: int = 39, ny: int = 8,
+            nz: int = 6) -> Mesh:
+    """Four-column semi-submersible (no symmetry): vertical columns of radius
+    ``col_radius`` at (+-s/2, +-s/2), s = ``spacing``, and closed box
+    pontoons ``pontoon_w`` wide x ``pontoon_h`` high at the keel joining
+    neighbouring columns between their walls.  The default is 4 x 1824
+    column panels + 4 x 1188 pontoon panels = 12,048 panels.  The pontoon
+    ends touch the column walls without a shared seam (a common
+    simplification for synthetic meshes)."""
     h = spacing / 2
-    # column: side n_t x n_z, bottom n_r rings
-    scale = math.sqrt(n_target / 12000.0)
-    n_t, n_z, n_r = max(8, int(48 * scale)), max(4, int(30 * scale)), max(2, int(8 * scale))
-    cols = [cylinder(col_radius, draft, n_t, n_z, n_r, half=False, x0=sx * h, y0=sy * h)
+    cols = [cylinder(col_radius, draft, n_theta, n_z, n_r, half=False, x0=sx * h, y0=sy * h)
             for sx in (-1, 1) for sy in (-1, 1)]
     length = spacing - 2 * col_radius
-    nx, ny, nz = max(4, int(60 * scale)), max(2, int(12 * scale)), max(2, int(9 * scale))
+    zc = -draft + pontoon_h / 2
     pons = []
-    for k in range(4):
-        p = box(length, pontoon_w, pontoon_h, nx, ny, nz)
-        p = Mesh(p.vertices + np.array([0.0, 0.0, -(draft - pontoon_h)]), p.panels, 0)
-        if k < 2:
-            p = translate(p, 0.0, (-1 if k == 0 else 1) * h, 0.0)
-        else:
-            v = p.vertices[:, [1, 0, 2]].copy()
-            v[:, 1] *= 1.0
-            p = Mesh(v, p.panels[:, [0, 3, 2, 1]].copy(), 0)
-            p = translate(p, (-1 if k == 2 else 1) * h, 0.0, 0.0)
-        pons.append(p)
+    for sy in (-1, 1):  # along x at y = +-s/2
+        pons.append(closed_box(length, pontoon_w, pontoon_h, nx, ny, nz, center=(0.0, sy * h, zc)))
+    for sx in (-1, 1):  # along y at x = +-s/2
+        pons.append(closed_box(pontoon_w, length, pontoon_h, ny, nx, nz, center=(sx * h, 0.0, zc)))
     return merge(cols + pons)

@@ -172,3 +172,24 @@ def test_full_size_barge_mixed_vs_fp64():
     for w in range(2):
         damping = -F[:, w, :].real
         assert np.all(np.diag(damping) >= -1e-6 * np.abs(damping).max())
+
+
+def test_semisub_12k_infinite_depth_mixed_vs_fp64():
+    """The 12,048-panel semi-submersible (infinite depth, 14 right-hand
+    sides) at one frequency: mixed within 1e-6 of FP64 on every force, and
+    radiation damping B_ii >= 0."""
+    m = nmesh.semisub()
+    modes = bcm.six_dof()
+    om, kind, nvel = bcm.problems(m, [0.8], list(np.arange(8) * 45.0), modes, 0.0, 9.81, 1025.0)
+    nds = bcm.integration(m, modes)
+    out = {}
+    for prec in (core.FP64, core.MIXED):
+        s = Solver(m, 0.0, 1025.0, 9.81, precision=prec)
+        out[prec] = s.solve_problems(om, nvel, nds)
+        s.close()
+    err = _normwise(out[core.MIXED], out[core.FP64])
+    print(f"semisub {m.npanels}: mixed vs FP64 max normwise rel err {err:.3e}")
+    assert err < MIXED_TOL
+    rad = kind == -1.0
+    damping = -out[core.FP64][:, rad].real
+    assert np.all(np.diag(damping) >= -1e-6 * np.abs(damping).max())
