@@ -187,6 +187,54 @@ def cpu_baseline(m, omegas, budget_s=20.0):
                       f"as N^2(1+9) pairs + (N/{ns})^3 GAUSSZ = {t_freq:.1f} s"}
 
 
+def run_serving(rates, nreq, workers, seed=2024):
+    """Serving: Poisson arrivals (fixed seed) of two-cylinder requests
+    (r ~ U(2,15) m, spacing U(3,10) r, N log-uniform in [200, 16000], 12 DOF
+    + 1-4 headings, 1-64 frequencies, infinite depth), admitted by the
+    native request loop under a budget of N^2 complex64 per in-flight
+    frequency.  Latency = submission to completion, stamped by the loop.
+    The first rate runs on the mixed and the plain FP64 paths, the others
+    on the mixed path only."""
+    from nemoh_amd import core as ncore
+    from nemoh_amd import serving
+    rng = np.random.default_rng(seed)
+    reqs = [serving.two_cylinder_request(rng) for _ in range(nreq)]
+    gaps = rng.exponential(1.0, nreq)
+    nfreq = sum(r.nfreq for r in reqs)
+    tables = ncore.build_tables()
+
+    def stream(rate, prec):
+        arrivals = np.cumsum(gaps) / rate
+        with serving.Server(workers=workers, tables=tables) as srv:
+            t0 = time.perf_counter()
+            ids = []
+            for r, ta in zip(reqs, arrivals):
+                dt = t0 + ta - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids.append(srv.submit(r, prec))
+            tim = [srv.wait(i)[1] for i in ids]
+            wall = time.perf_counter() - t0
+        lat = np.array([t["latency"] for t in tim])
+        svc = np.array([t["service"] for t in tim])
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
+                "mean_service_ms": round(float(svc.mean()), 1), "wall_s": round(wall, 2),
+                "req_per_s": round(nreq / wall, 3), "freq_solves_per_s": round(nfreq / wall, 2)}
+
+    out = {"workload": f"{nreq} requests, Poisson arrivals (seed {seed}): two cylinders r ~ U(2,15) m, spacing "
+                       "U(3,10) r, N log-uniform 200-16000 panels, 12 DOF + 1-4 headings, 1-64 frequencies, "
+                       "infinite depth",
+           "panels": [r.npanels for r in reqs], "frequencies": nfreq, "workers": workers,
+           "budget": "N^2 complex64 per in-flight frequency, 32 GiB", "rates": {}}
+    for k, rate in enumerate(rates):
+        res = {"mixed": stream(rate, ncore.MIXED)}
+        if k == 0:
+            res["fp64"] = stream(rate, ncore.FP64)
+        out["rates"][str(rate)] = res
+        print(f"serving at {rate} req/s: {res}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +243,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--c3-steps", type=int, default=3, help="frequencies of the 12k-panel case (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=32, help="requests of the serving run (0: skip)")
+    ap.add_argument("--serve-rates", default="5", help="comma-separated arrival rates (req/s)")
+    ap.add_argument("--serve-workers", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: each frequency solve is independent, so N GPUs "
@@ -250,6 +301,9 @@ def main():
     }
     if a.c3_steps > 0:
         out["c3"] = run_c3(a.c3_steps)
+    if a.serve_requests > 0:
+        out["serving"] = run_serving([float(x) for x in a.serve_rates.split(",")], a.serve_requests,
+                                     a.serve_workers)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = (cpu_baseline_reference(m, omegas) if REF_HARNESS.exists()
                                else cpu_baseline(m, omegas))

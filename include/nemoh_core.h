@@ -154,6 +154,44 @@ int nemoh_core_prony(double akh, double amh, int *nexp, double *ambda, double *a
 int nemoh_core_pairs(nemoh_core *ctx, double period, double amh, int nexp, const double *ambda,
                      const double *ar, int nq, const double *qq, const double *cq, double *out);
 int nemoh_core_mfma_probe(int fp32, double *out);
+/* GM of the handle's mesh: the largest horizontal vertex distance, to the
+ * y-images as well when NSYMY = 1 (SOLVE_BEM_INFD_DIRECT.f90:66-75,
+ * SOLVE_BEM_FD_DIRECT.f90:86-95), computed on the device. */
+int nemoh_core_get_extent(const nemoh_core *ctx, double *gm);
+
+/* Request loop -- no counterpart in the reference (one case per process);
+ * for callers that serve many independent solves, of different meshes and
+ * sizes, on one GPU.  A request is a mesh (as nemoh_core_create, 1-based
+ * m1..m4), depth, rho, g, precision, nfreq periods, and per frequency the
+ * normal velocities nvel(NFA*2**nsym, nrhs) (nfreq such blocks, complex);
+ * nds(nint, NFA*2**nsym) is shared by the frequencies.  The reply is
+ * forces(nint, nrhs, nfreq) = -NDS . pressure at each frequency, as
+ * nemoh_core_solve_frequency of a session that solves the request's
+ * frequencies in order.  Work is admitted one (request, frequency) at a
+ * time, first come first served, while the in-flight frequencies need at
+ * most budget_bytes (N^2 complex64 = 8 N^2 bytes each, N = NFA*2**nsym; with
+ * nothing in flight the next one is always admitted).  Each of the
+ * `workers` threads owns a solver handle (own HIP stream, resident matrices
+ * and factors), so admitted work overlaps on the device; a result does not
+ * depend on what else was in flight.  xr..apd2z are the wave tables as for
+ * nemoh_core_create.  submit copies its inputs and returns at once with an
+ * id; wait blocks until that request is done, writes the forces and
+ * timing_ms[3] = {submit to done, submit to first start, first start to
+ * done} (either may be NULL), and returns the request's status (the first
+ * error of its frequencies; others are unaffected).  Calls are thread-safe;
+ * destroy finishes the queued work first. */
+typedef struct nemoh_server nemoh_server;
+int nemoh_server_create(nemoh_server **srv, int workers, double budget_bytes, const double *xr,
+                        const double *xz, const double *apd1x, const double *apd1z,
+                        const double *apd2x, const double *apd2z);
+int nemoh_server_submit(nemoh_server *srv, int np, int nfa, int nsym, const double *x, const double *y,
+                        const double *z, const int *m1, const int *m2, const int *m3, const int *m4,
+                        double depth, double rho, double g, int precision, int nfreq,
+                        const double *periods, int nrhs, const double *nvel, int nint,
+                        const double *nds, long long *id);
+int nemoh_server_wait(nemoh_server *srv, long long id, double *forces, double *timing_ms);
+const char *nemoh_server_last_error(nemoh_server *srv);
+void nemoh_server_destroy(nemoh_server *srv);
 
 #ifdef __cplusplus
 }

@@ -120,8 +120,7 @@ def diffraction(mesh: Mesh, w: float, beta: float, depth: float, g: float, rho: 
     nv, pr = [], []
     for x, y, nn in rows:
         wbar = (x - xeff) * np.cos(beta) + (y - yeff) * np.sin(beta)
-        cih, sih = zip(*[_cih_sih(k, z, depth) for z in xm[:, 2]])
-        cih, sih = np.array(cih), np.array(sih)
+        cih, sih = _cih_sih(k, xm[:, 2], depth)
         e = np.exp(1j * k * wbar)
         vx = g / w * k * np.cos(beta) * cih * e
         vy = g / w * k * np.sin(beta) * cih * e
@@ -131,9 +130,11 @@ def diffraction(mesh: Mesh, w: float, beta: float, depth: float, g: float, rho: 
     return np.concatenate(nv), np.concatenate(pr)
 
 
-def problems(mesh: Mesh, omegas, betas_deg, modes, depth: float, g: float, rho: float = 1025.0):
+def problems(mesh: Mesh, omegas, betas_deg, modes, depth: float, g: float, rho: float = 1025.0,
+             body_of_panel=None):
     """All problems in the reference order: for each omega, one diffraction
-    problem per heading then one radiation problem per mode.  Returns
+    problem per heading then one radiation problem per mode (a mode moves
+    only the panels of its body when ``body_of_panel`` is given).  Returns
     (omega (nprob,), kind (nprob,), nvel (ntot, nprob))."""
     om, kind, cols = [], [], []
     for w in omegas:
@@ -145,9 +146,9 @@ def problems(mesh: Mesh, omegas, betas_deg, modes, depth: float, g: float, rho: 
         for m in modes:
             om.append(w)
             kind.append(-1.0)
-            cols.append(radiation(mesh, m))
+            cols.append(radiation(mesh, m, body_of_panel))
     return np.array(om), np.array(kind), np.stack(cols, axis=1)
 
 
-def integration(mesh: Mesh, modes) -> np.ndarray:
-    return np.stack([nds(mesh, m) for m in modes])
+def integration(mesh: Mesh, modes, body_of_panel=None) -> np.ndarray:
+    return np.stack([nds(mesh, m, body_of_panel) for m in modes])

@@ -14,7 +14,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libnemoh_core.so"
-SOURCES = ["core.hip", "kernels.hip", "lu.hip", "host_math.cpp"]
+SOURCES = ["core.hip", "kernels.hip", "lu.hip", "host_math.cpp", "server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NEMOH_OFFLOAD_ARCH", "gfx950")
 
@@ -32,7 +32,7 @@ def build(verbose: bool = False) -> Path:
         obj = out_dir / (src + ".o")
         cmd = [HIPCC, *COMMON, *DEVICE, "-c", str(CSRC / src), "-o", str(obj)]
         if src.endswith(".cpp"):
-            cmd = [HIPCC, *COMMON, "-x", "c++", "-c", str(CSRC / src), "-o", str(obj)]
+            cmd = [HIPCC, *COMMON, "-pthread", "-x", "c++", "-c", str(CSRC / src), "-o", str(obj)]
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
     for src, p in procs:
@@ -43,7 +43,7 @@ def build(verbose: bool = False) -> Path:
         if verbose and msg.strip():
             print(msg)
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, *DEVICE, "-shared", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, *DEVICE, "-shared", "-pthread", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

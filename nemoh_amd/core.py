@@ -21,7 +21,8 @@ EXPORTS = (
     "nemoh_core_freq_setup", "nemoh_core_set_precision", "nemoh_core_set_option", "nemoh_core_get_stats", "nemoh_core_enable_timing",
     "nemoh_core_last_error", "nemoh_core_destroy", "nemoh_core_prepare_mesh", "nemoh_core_build_tables",
     "nemoh_core_wavenumber", "nemoh_core_theta_rule", "nemoh_core_prony", "nemoh_core_pairs",
-    "nemoh_core_mfma_probe",
+    "nemoh_core_mfma_probe", "nemoh_core_get_extent", "nemoh_server_create", "nemoh_server_submit",
+    "nemoh_server_wait", "nemoh_server_last_error", "nemoh_server_destroy",
 )
 
 
@@ -81,6 +82,15 @@ def load():
     L.nemoh_core_prony.argtypes = [C.c_double, C.c_double, _ip, _dp, _dp]
     L.nemoh_core_pairs.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
     L.nemoh_core_mfma_probe.argtypes = [C.c_int, _dp]
+    L.nemoh_core_get_extent.argtypes = [C.c_void_p, _dp]
+    L.nemoh_server_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_double] + [_dp] * 6
+    L.nemoh_server_submit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [_dp] * 3 + [_ip] * 4 + \
+        [C.c_double] * 3 + [C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.POINTER(C.c_longlong)]
+    L.nemoh_server_wait.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp]
+    L.nemoh_server_last_error.argtypes = [C.c_void_p]
+    L.nemoh_server_last_error.restype = C.c_char_p
+    L.nemoh_server_destroy.argtypes = [C.c_void_p]
+    L.nemoh_server_destroy.restype = None
     _lib = L
     return L
 
@@ -198,6 +208,13 @@ class Core:
 
     def set_precision(self, mode: int, tol: float = 0.0, max_iter: int = 0):
         self._check(load().nemoh_core_set_precision(self.h, int(mode), float(tol), int(max_iter)))
+
+    def extent(self) -> float:
+        """GM: the largest horizontal vertex distance (to the y-images too
+        when the mesh is symmetric), as the handle computed it."""
+        gm = C.c_double()
+        self._check(load().nemoh_core_get_extent(self.h, C.byref(gm)))
+        return gm.value
 
     def set_option(self, option: int, value: int):
         self._check(load().nemoh_core_set_option(self.h, int(option), int(value)))

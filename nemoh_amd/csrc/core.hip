@@ -2,12 +2,15 @@
 // mapping of the reference's Solver module state onto device-resident data.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
+#include "core_internal.h"
 #include "nemoh_core.h"
 #include "host_math.h"
 #include "kernels.h"
@@ -18,8 +21,13 @@ struct nemoh_core {
   int np = 0, nfa = 0, nsym = 0, nb = 1;
   double zer = 0, g = 9.81, rho = 1025, depth = 0;
   double gm = 0;  // max horizontal vertex distance (theta-rule argument)
-  std::vector<double> hx, hy;
   hipStream_t st = nullptr;
+  // element capacity of every device buffer, keyed by the address of the
+  // handle field that holds it: buffers only grow, so re-binding the handle
+  // to a mesh no larger than before allocates (and frees) nothing
+  std::unordered_map<const void*, size_t> caps;
+  double* d_pts = nullptr;  // vertex x, y (np each) for the extent reduction
+  double* d_ext = nullptr;  // per-workgroup maxima of the extent reduction
   // resident geometry and tables
   double* d_geo = nullptr;  // xg yg zg xn yn zn aire tdis, n each
   double* d_vtx = nullptr;  // vx vy vz, 4n each
@@ -49,9 +57,7 @@ struct nemoh_core {
   double2 *R = nullptr, *sig = nullptr, *pot = nullptr, *pres = nullptr;
   float2 *R32 = nullptr, *W32 = nullptr;
   double* d_nds = nullptr;
-  size_t nds_cap = 0;
   double2* d_force = nullptr;
-  int force_cap = 0;
   double* d_red = nullptr;  // reduction scratch
   int* d_info = nullptr;     // [0] first bad pivot row, [1] solve wait timeout
   int* d_flags = nullptr;    // block flags of the single-launch triangular solves
@@ -81,10 +87,17 @@ int fail(nemoh_core* c, int code, const std::string& msg) {
       return fail((c), NEMOH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Makes *p hold at least count elements (a buffer is only ever replaced by
+// a larger one).
 template <typename T>
 int dalloc(nemoh_core* c, T** p, size_t count) {
-  if (*p) return NEMOH_OK;
-  HIPCHK(c, hipMalloc((void**)p, count * sizeof(T)));
+  size_t& have = c->caps[(const void*)p];
+  if (*p && have >= count) return NEMOH_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  have = 0;
+  HIPCHK(c, hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+  have = count;
   return NEMOH_OK;
 }
 
@@ -104,25 +117,10 @@ int ensure_matrices(nemoh_core* c) {
   return NEMOH_OK;
 }
 
+// cap = right-hand sides the buffers hold for the current mesh (reset to 0
+// when the handle is bound to another mesh)
 int ensure_rhs(nemoh_core* c, int nrhs) {
   if (nrhs <= c->cap) return NEMOH_OK;
-  auto release = [](auto*& p) {
-    if (p) hipFree(p);
-    p = nullptr;
-  };
-  release(c->V);
-  release(c->B);
-  release(c->W);
-  release(c->R);
-  release(c->sig);
-  release(c->pot);
-  release(c->pres);
-  release(c->R32);
-  release(c->W32);
-  for (int b = 0; b < 2; ++b) {
-    release(c->Z[b]);
-    release(c->F[b]);
-  }
   c->cap = 0;
   const size_t n = c->nfa, nt = (size_t)c->nfa * c->nb;
   const size_t k = nrhs;
@@ -361,13 +359,7 @@ int upload_nvel_colmajor(nemoh_core* c, const double* nvel, int nrhs) {
 static int stage_nds(nemoh_core* c, int nint, const double* nds) {
   const size_t nt = (size_t)c->nfa * c->nb;
   const size_t need = (size_t)nint * nt;
-  if (need > c->nds_cap) {
-    if (c->d_nds) hipFree(c->d_nds);
-    c->d_nds = nullptr;
-    c->nds_cap = 0;
-    HIPCHK(c, hipMalloc((void**)&c->d_nds, need * sizeof(double)));
-    c->nds_cap = need;
-  }
+  if (int r = dalloc(c, &c->d_nds, need)) return r;
   // caller: NDS(nint, ntot) column-major -> device (nint x ntot) row-major
   std::vector<double> tmp(need);
   for (int k = 0; k < nint; ++k)
@@ -382,12 +374,7 @@ static int finish_outputs(nemoh_core* c, int nrhs, double* pressure, int nint, c
   if (forces && nint > 0) {
     if (!nds) return fail(c, NEMOH_ERR_ARG, "nds is NULL");
     if (int r = stage_nds(c, nint, nds)) return r;
-    if (nint * nrhs > c->force_cap) {
-      if (c->d_force) hipFree(c->d_force);
-      c->d_force = nullptr;
-      HIPCHK(c, hipMalloc((void**)&c->d_force, (size_t)nint * nrhs * sizeof(double2)));
-      c->force_cap = nint * nrhs;
-    }
+    if (int r = dalloc(c, &c->d_force, (size_t)nint * nrhs)) return r;
     launch_force(c->pres, c->d_nds, nint, (int)nt, nrhs, c->d_force, c->st);
     HIPCHK(c, hipGetLastError());
     std::vector<double> f(2 * (size_t)nint * nrhs);
@@ -446,6 +433,113 @@ static int finish_outputs(nemoh_core* c, int nrhs, double* pressure, int nint, c
   return NEMOH_OK;
 }
 
+static bool mesh_args_ok(int np, int nfa, int nsym, const double* x, const double* y, const double* z,
+                         const int* m1, const int* m2, const int* m3, const int* m4, const double* const geo[8],
+                         double depth) {
+  if (np <= 0 || nfa <= 0 || (nsym != 0 && nsym != 1) || !x || !y || !z || !m1 || !m2 || !m3 || !m4 || depth < 0)
+    return false;
+  for (int k = 0; k < 8; ++k)
+    if (!geo[k]) return false;
+  return true;
+}
+
+namespace nemoh {
+
+int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, const double* y, const double* z,
+                   const int* m1, const int* m2, const int* m3, const int* m4, const double* const geo[8], double zer,
+                   double g, double rho, double depth) {
+  if (!c) return NEMOH_ERR_ARG;
+  if (!mesh_args_ok(np, nfa, nsym, x, y, z, m1, m2, m3, m4, geo, depth))
+    return fail(c, NEMOH_ERR_ARG, "bad mesh arguments");
+  std::vector<double> vtx(12 * (size_t)nfa);
+  for (int j = 0; j < nfa; ++j) {
+    const int m[4] = {m1[j] - 1, m2[j] - 1, m3[j] - 1, m4[j] - 1};
+    for (int l = 0; l < 4; ++l) {
+      if (m[l] < 0 || m[l] >= np) return fail(c, NEMOH_ERR_ARG, "panel vertex index out of range");
+      vtx[4 * (size_t)j + l] = x[m[l]];
+      vtx[4 * (size_t)nfa + 4 * (size_t)j + l] = y[m[l]];
+      vtx[8 * (size_t)nfa + 4 * (size_t)j + l] = z[m[l]];
+    }
+  }
+  // nothing queued on the handle may still read the buffers being replaced
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->w_prev = -1.0;
+  c->factored_prec = -1;
+  const int nblk = (nfa + kNB - 1) / kNB;
+  const int ntile = (np + 255) / 256;
+  const size_t nred = (size_t)ntile * (ntile + 1) / 2;
+  int r = 0;
+  if ((r = dalloc(c, &c->d_geo, 8 * (size_t)nfa)) || (r = dalloc(c, &c->d_vtx, 12 * (size_t)nfa)) ||
+      (r = dalloc(c, &c->d_flags, 2 * (size_t)nblk)) || (r = dalloc(c, &c->d_pts, 2 * (size_t)np)) ||
+      (r = dalloc(c, &c->d_ext, nred)))
+    return r;
+  for (int k = 0; k < 8; ++k)
+    HIPCHK(c, hipMemcpyAsync(c->d_geo + (size_t)k * nfa, geo[k], nfa * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_vtx, vtx.data(), vtx.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_pts, x, np * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_pts + np, y, np * sizeof(double), hipMemcpyHostToDevice, c->st));
+  // flags of the chained triangular solves: cleared over the whole buffer
+  // (solves compare them with an epoch that only grows)
+  HIPCHK(c, hipMemsetAsync(c->d_flags, 0, c->caps[(const void*)&c->d_flags] * sizeof(int), c->st));
+  // GM = max over vertex pairs of the horizontal distance (and to the y
+  // images when nsym = 1), the argument of the theta rule
+  launch_extent(c->d_pts, c->d_pts + np, np, nsym, c->d_ext, c->st);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> ext(nred);
+  HIPCHK(c, hipMemcpyAsync(ext.data(), c->d_ext, nred * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  double gm2 = 0.0;
+  for (double e : ext) gm2 = std::max(e, gm2);
+  // the reference keeps the max of SQRT over the pairs; a correctly rounded
+  // square root is monotonic, so one root of the max gives the same value
+  c->gm = std::sqrt(gm2);
+  c->np = np;
+  c->nfa = nfa;
+  c->nsym = nsym;
+  c->nb = 1 << nsym;
+  c->zer = zer;
+  c->g = g;
+  c->rho = rho;
+  c->depth = depth;
+  c->nq_state = 0;
+  c->cap = 0;
+  c->stats = nemoh_stats{};
+  c->upd_blocks = 0;
+  const double* d = c->d_geo;
+  c->G = Geometry{nfa, d, d + nfa, d + 2 * (size_t)nfa, d + 3 * (size_t)nfa, d + 4 * (size_t)nfa,
+                  d + 5 * (size_t)nfa, d + 6 * (size_t)nfa, d + 7 * (size_t)nfa,
+                  c->d_vtx, c->d_vtx + 4 * (size_t)nfa, c->d_vtx + 8 * (size_t)nfa};
+  return NEMOH_OK;
+}
+
+int core_nq_sequence(nemoh_core* c, int nfreq, const double* periods, int* nq_before) {
+  if (!c || nfreq < 0 || (nfreq > 0 && (!periods || !nq_before))) return NEMOH_ERR_ARG;
+  const double pi = 4.0 * std::atan(1.0);
+  int state = c->nq_state;
+  for (int f = 0; f < nfreq; ++f) {
+    nq_before[f] = state;
+    const double w = 2.0 * pi / periods[f];
+    const std::string at = "frequency " + std::to_string(f) + ": ";
+    if (!(std::fabs(w) >= 1.E-4)) return fail(c, NEMOH_ERR_OMEGA, at + "ABS(WR) < 1.E-4");
+    const double kwave = w * w / c->g;
+    if (c->depth == 0.0 || kwave * c->depth >= 20) continue;
+    const double amh = host::x0(w * w * c->depth / c->g);
+    if (std::isnan(amh)) return fail(c, NEMOH_ERR_WAVENUMBER, at + "ERREUR DANS LA RECHERCHE DE LA RACINE");
+    double qq[101], cq[101];
+    state = host::theta_rule(amh / c->depth * c->gm, qq, cq);
+  }
+  return NEMOH_OK;
+}
+
+void core_set_nq_state(nemoh_core* c, int nq) { c->nq_state = nq; }
+
+int core_device_count() {
+  int n = 0;
+  return hipGetDeviceCount(&n) == hipSuccess ? n : 0;
+}
+
+}  // namespace nemoh
+
 extern "C" {
 
 int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double* x, const double* y, const double* z,
@@ -456,23 +550,12 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
                       const double* apd2z) {
   if (!out) return NEMOH_ERR_ARG;
   *out = nullptr;
-  if (np <= 0 || nfa <= 0 || (nsym != 0 && nsym != 1) || !x || !y || !z || !m1 || !m2 || !m3 || !m4 || !xg || !yg ||
-      !zg || !xn || !yn || !zn || !aire || !tdis || !xr || !xz || !apd1x || !apd1z || !apd2x || !apd2z || depth < 0)
-    return NEMOH_ERR_ARG;
+  if (!xr || !xz || !apd1x || !apd1z || !apd2x || !apd2z) return NEMOH_ERR_ARG;
+  const double* geo[8] = {xg, yg, zg, xn, yn, zn, aire, tdis};
+  if (!mesh_args_ok(np, nfa, nsym, x, y, z, m1, m2, m3, m4, geo, depth)) return NEMOH_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NEMOH_ERR_HIP;
   nemoh_core* c = new nemoh_core();
-  c->np = np;
-  c->nfa = nfa;
-  c->nsym = nsym;
-  c->nb = 1 << nsym;
-  c->zer = zer;
-  c->g = g;
-  c->rho = rho;
-  c->depth = depth;
-  c->hx.assign(x, x + np);
-  c->hy.assign(y, y + np);
-  c->gm = host::max_horizontal_extent(np, nsym, x, y);
   auto bail = [&](int code) {
     nemoh_core_destroy(c);
     return code;
@@ -480,18 +563,6 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return bail(NEMOH_ERR_HIP);
   for (auto& e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return bail(NEMOH_ERR_HIP);
-  std::vector<double> geo(8 * (size_t)nfa), vtx(12 * (size_t)nfa);
-  const double* src[8] = {xg, yg, zg, xn, yn, zn, aire, tdis};
-  for (int k = 0; k < 8; ++k) std::memcpy(&geo[(size_t)k * nfa], src[k], nfa * sizeof(double));
-  for (int j = 0; j < nfa; ++j) {
-    const int m[4] = {m1[j] - 1, m2[j] - 1, m3[j] - 1, m4[j] - 1};
-    for (int l = 0; l < 4; ++l) {
-      if (m[l] < 0 || m[l] >= np) return bail(NEMOH_ERR_ARG);
-      vtx[4 * (size_t)j + l] = x[m[l]];
-      vtx[4 * (size_t)nfa + 4 * (size_t)j + l] = y[m[l]];
-      vtx[8 * (size_t)nfa + 4 * (size_t)j + l] = z[m[l]];
-    }
-  }
   std::vector<double> tab(4 * (size_t)kNR * kNZ);
   for (int j = 0; j < kNZ; ++j)
     for (int i = 0; i < kNR; ++i) {
@@ -501,18 +572,12 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
       tab[4 * f + 2] = apd2x[f];
       tab[4 * f + 3] = apd2z[f];
     }
-  if (hipMalloc((void**)&c->d_geo, geo.size() * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&c->d_vtx, vtx.size() * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&c->d_tab, tab.size() * sizeof(double)) != hipSuccess ||
+  if (hipMalloc((void**)&c->d_tab, tab.size() * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_xr, kNR * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_xz, kNZ * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_par, sizeof(FreqParams)) != hipSuccess ||
       hipMalloc((void**)&c->d_red, 64 * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&c->d_info, 2 * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&c->d_flags, 2 * sizeof(int) * ((nfa + kNB - 1) / kNB)) != hipSuccess)
-    return bail(NEMOH_ERR_HIP);
-  if (hipMemset(c->d_info, 0, 2 * sizeof(int)) != hipSuccess ||
-      hipMemset(c->d_flags, 0, 2 * sizeof(int) * ((nfa + kNB - 1) / kNB)) != hipSuccess)
+      hipMalloc((void**)&c->d_info, 2 * sizeof(int)) != hipSuccess)
     return bail(NEMOH_ERR_HIP);
   {
     int dev = 0;
@@ -520,26 +585,22 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return bail(NEMOH_ERR_HIP);
     c->cu_count = prop.multiProcessorCount;
   }
-  if (hipMemcpy(c->d_geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_vtx, vtx.data(), vtx.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+  if (hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_xr, xr, kNR * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_xz, xz, kNZ * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(c->d_xz, xz, kNZ * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(c->d_info, 0, 2 * sizeof(int)) != hipSuccess)
     return bail(NEMOH_ERR_HIP);
-  const double* d = c->d_geo;
-  c->G = Geometry{nfa, d, d + nfa, d + 2 * (size_t)nfa, d + 3 * (size_t)nfa, d + 4 * (size_t)nfa,
-                  d + 5 * (size_t)nfa, d + 6 * (size_t)nfa, d + 7 * (size_t)nfa,
-                  c->d_vtx, c->d_vtx + 4 * (size_t)nfa, c->d_vtx + 8 * (size_t)nfa};
   c->T = Tables{c->d_xr, c->d_xz, c->d_tab};
+  if (int r = core_bind_mesh(c, np, nfa, nsym, x, y, z, m1, m2, m3, m4, geo, zer, g, rho, depth)) return bail(r);
   *out = c;
   return NEMOH_OK;
 }
 
 void nemoh_core_destroy(nemoh_core* c) {
   if (!c) return;
-  if (c->st) hipStreamSynchronize(c->st);
+  if (c->st) (void)hipStreamSynchronize(c->st);
   auto release = [](auto*& p) {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
   };
   release(c->d_geo);
@@ -553,6 +614,8 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->d_flags);
   release(c->d_nds);
   release(c->d_force);
+  release(c->d_pts);
+  release(c->d_ext);
   for (int b = 0; b < 2; ++b) {
     release(c->D[b]);
     release(c->S[b]);
@@ -572,9 +635,9 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->R32);
   release(c->W32);
   for (auto& e : c->ev)
-    if (e) hipEventDestroy(e);
+    if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->upd_ev) (void)hipEventDestroy(e);
-  if (c->st) hipStreamDestroy(c->st);
+  if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
 
@@ -606,6 +669,12 @@ int nemoh_core_enable_timing(nemoh_core* c, int on) {
 int nemoh_core_get_stats(const nemoh_core* c, nemoh_stats* out) {
   if (!c || !out) return NEMOH_ERR_ARG;
   *out = c->stats;
+  return NEMOH_OK;
+}
+
+int nemoh_core_get_extent(const nemoh_core* c, double* gm) {
+  if (!c || !gm) return NEMOH_ERR_ARG;
+  *gm = c->gm;
   return NEMOH_OK;
 }
 
@@ -699,7 +768,7 @@ int nemoh_core_pairs(nemoh_core* c, double period, double amh, int nexp, const d
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, d, cnt * sizeof(double), hipMemcpyDeviceToHost, c->st);
   if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return fail(c, NEMOH_ERR_HIP, hipGetErrorString(e));
   return NEMOH_OK;
 }
@@ -744,9 +813,9 @@ int nemoh_core_build_tables(double* xr, double* xz, double* apd1x, double* apd1z
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(tab.data(), dtab, tab.size() * sizeof(double), hipMemcpyDeviceToHost);
-  if (dxr) hipFree(dxr);
-  if (dxz) hipFree(dxz);
-  if (dtab) hipFree(dtab);
+  if (dxr) (void)hipFree(dxr);
+  if (dxz) (void)hipFree(dxz);
+  if (dtab) (void)hipFree(dtab);
   if (e != hipSuccess) return NEMOH_ERR_HIP;
   for (size_t f = 0; f < (size_t)kNR * kNZ; ++f) {  // node f = i + 328 j (Fortran order)
     apd1x[f] = tab[4 * f + 0];
@@ -794,9 +863,9 @@ int nemoh_core_mfma_probe(int fp32, double* out) {
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(fd, dd, sizeof fd, hipMemcpyDeviceToHost);
-    if (da) hipFree(da);
-    if (db) hipFree(db);
-    if (dd) hipFree(dd);
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dd) (void)hipFree(dd);
     for (int k = 0; k < 256; ++k) out[k] = fd[k];
   } else {
     double *da = nullptr, *db = nullptr, *dd = nullptr;
@@ -810,9 +879,9 @@ int nemoh_core_mfma_probe(int fp32, double* out) {
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(out, dd, 256 * sizeof(double), hipMemcpyDeviceToHost);
-    if (da) hipFree(da);
-    if (db) hipFree(db);
-    if (dd) hipFree(dd);
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dd) (void)hipFree(dd);
   }
   return e == hipSuccess ? NEMOH_OK : NEMOH_ERR_HIP;
 }

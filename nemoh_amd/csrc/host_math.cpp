@@ -69,19 +69,6 @@ int prepare_mesh(int np, int nfa, int nsym, const double* x, const double* y, co
   return 0;
 }
 
-double max_horizontal_extent(int np, int nsym, const double* x, const double* y) {
-  double gm = 0.0;
-  for (int i = 0; i + 1 < np; ++i)
-    for (int s = 0; s <= nsym; ++s) {
-      const double bj = (s == 0) ? 1.0 : -1.0;
-      for (int j = i + 1; j < np; ++j) {
-        const double dx = x[j] - x[i], dy = y[i] - y[j] * bj;
-        gm = std::max(std::sqrt(dx * dx + dy * dy), gm);
-      }
-    }
-  return gm;
-}
-
 // Gauss half-rules on [-1/2, 1/2] as tabulated by the reference (7 digits).
 namespace {
 const double kQ8[] = {.4801449, .3983332, .2627662, .09171732};

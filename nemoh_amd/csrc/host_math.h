@@ -13,8 +13,6 @@ struct PanelGeometry {
 // Returns 0 or the 1-based index of a degenerate panel.
 int prepare_mesh(int np, int nfa, int nsym, const double* x, const double* y, const double* z, const int* m1,
                  const int* m2, const int* m3, const int* m4, PanelGeometry* out);
-// Largest horizontal vertex distance including the y-image (GM).
-double max_horizontal_extent(int np, int nsym, const double* x, const double* y);
 // Theta rule for AKK = k * GM; returns the rule length.
 int theta_rule(double akk, double* qq, double* cq);
 // Root of x tanh x = ak by the reference's bisection.

@@ -60,6 +60,9 @@ void launch_force(const double2* pres, const double* nds, int nint, int ntot, in
 void launch_f64_to_f32(const double2* a, float2* b, size_t count, double scale, hipStream_t st);
 void launch_f32_to_f64(const float2* a, double2* b, size_t count, double scale, int accumulate, hipStream_t st);
 void launch_absmax(const double2* a, size_t count, double* out, int nblocks, hipStream_t st);
+// out[ntile*(ntile+1)/2] (ntile = ceil(np/256)): per-workgroup maxima of the
+// squared horizontal vertex distances (GM^2)
+void launch_extent(const double* x, const double* y, int np, int nsym, double* out, hipStream_t st);
 void launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t st);
 void launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t st);
 

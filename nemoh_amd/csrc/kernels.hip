@@ -260,6 +260,50 @@ __global__ __launch_bounds__(256) void absmax_kernel(const double2* __restrict__
   if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
+// Largest squared horizontal vertex distance (GM^2 of
+// SOLVE_BEM_INFD_DIRECT.f90:66-75, SOLVE_BEM_FD_DIRECT.f90:86-95): over the
+// pairs i < j, and with nsym = 1 also from vertex i to the y-image of j.
+// Workgroup b takes one pair of 256-vertex tiles (ti <= tj) in triangular
+// order; thread t holds vertex ti*256+t against tile tj staged in LDS.  Max
+// is exact and order-free, and the squares are formed without contraction,
+// so the result equals the sequential loop bit for bit.
+__global__ __launch_bounds__(256) void extent_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                     int np, int nsym, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ double sx[256], sy[256], red[256];
+  const int ntile = (np + 255) / 256;
+  int b = blockIdx.x, ti = 0;
+  while (b >= ntile - ti) {
+    b -= ntile - ti;
+    ++ti;
+  }
+  const int tj = ti + b;
+  const int t = threadIdx.x, i = ti * 256 + t, j0 = tj * 256;
+  const int jn = min(256, np - j0);
+  sx[t] = (t < jn) ? x[j0 + t] : 0.0;
+  sy[t] = (t < jn) ? y[j0 + t] : 0.0;
+  __syncthreads();
+  double m = 0.0;
+  if (i < np) {
+    const double xi = x[i], yi = y[i];
+    for (int q = (tj == ti) ? t + 1 : 0; q < jn; ++q) {
+      const double dx = sx[q] - xi, dy = yi - sy[q];
+      m = fmax(dx * dx + dy * dy, m);
+      if (nsym) {
+        const double dz = yi + sy[q];
+        m = fmax(dx * dx + dz * dz, m);
+      }
+    }
+  }
+  red[t] = m;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (t < k) red[t] = fmax(red[t], red[t + k]);
+    __syncthreads();
+  }
+  if (t == 0) out[blockIdx.x] = red[0];
+}
+
 // ---------------------------------------------------------------------------
 // host-side launchers
 void launch_crek(const double* xr, const double* xz, double* tab, hipStream_t st) {
@@ -318,6 +362,11 @@ void launch_f32_to_f64(const float2* a, double2* b, size_t count, double scale, 
 }
 void launch_absmax(const double2* a, size_t count, double* out, int nblocks, hipStream_t st) {
   absmax_kernel<<<nblocks, 256, 0, st>>>(a, count, out);
+}
+
+void launch_extent(const double* x, const double* y, int np, int nsym, double* out, hipStream_t st) {
+  const int ntile = (np + 255) / 256;
+  extent_kernel<<<(unsigned)((size_t)ntile * (ntile + 1) / 2), 256, 0, st>>>(x, y, np, nsym, out);
 }
 
 }  // namespace nemoh

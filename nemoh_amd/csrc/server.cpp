@@ -1,0 +1,344 @@
+// Request loop of the solver core (no counterpart in the reference, which
+// solves one case per process).
+//
+// A request is a mesh, its frequencies and the normal velocities of every
+// problem at each frequency; the reply is the forces of every problem.  The
+// unit of work is one (request, frequency).  Work is handed out first come,
+// first served, while the device-memory budget allows: N^2 complex64 =
+// 8 N^2 bytes per in-flight frequency (with nothing in flight the head of
+// the queue is always admitted, so an oversized request still runs).  Each
+// worker thread owns one solver handle, and so one HIP stream and one set of
+// resident matrices and factors; a handle is re-bound to the mesh of
+// whichever request its worker takes next, reusing its device buffers.  So
+// the frequencies of one request, and of requests that arrive while others
+// run, overlap on the GPU as soon as a worker and budget are free.
+//
+// A frequency's result does not depend on the schedule: every kernel is
+// deterministic, handles share nothing but the read-only wave tables, and
+// the module NQ that the reference carries from one frequency to the next
+// (an infinite-depth frequency reuses the NQ of the last finite-depth one)
+// is fixed per request before any of its frequencies runs, as a sequential
+// session over the request's frequency list would see it.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "core_internal.h"
+#include "nemoh_core.h"
+
+namespace {
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  long long id = 0;
+  // inputs (copied at submission)
+  int np = 0, nfa = 0, nsym = 0, nfreq = 0, nrhs = 0, nint = 0, precision = NEMOH_MIXED;
+  std::vector<double> x, y, z, periods, nvel, nds;
+  std::vector<int> m1, m2, m3, m4;
+  double depth = 0, rho = 1025, g = 9.81;
+  double budget = 0;  // bytes charged per in-flight frequency
+  // set-up shared by the frequencies, made once by the first worker
+  std::mutex pm;
+  std::condition_variable pcv;
+  int prep = 0;  // 0 not started, 1 running, 2 done, -1 failed
+  std::vector<double> geo;  // XG YG ZG XN YN ZN AIRE TDIS, nfa each
+  double zer = 0;
+  std::vector<int> nq_before;
+  // progress (guarded by the server mutex)
+  int next = 0, finished = 0;
+  bool done = false;
+  double t_submit = 0, t_start = -1, t_done = 0;
+  // results; each frequency writes its own slice
+  std::vector<double> forces;
+  std::mutex em;  // guards status/err
+  int status = NEMOH_OK;
+  std::string err;
+
+  void set_error(int code, const std::string& msg) {
+    std::lock_guard<std::mutex> lk(em);
+    if (status == NEMOH_OK) {
+      status = code;
+      err = msg;
+    }
+  }
+  bool failed() {
+    std::lock_guard<std::mutex> lk(em);
+    return status != NEMOH_OK;
+  }
+};
+
+struct Worker {
+  nemoh_core* h = nullptr;
+  long long bound = -1;  // request the handle's mesh belongs to
+};
+
+}  // namespace
+
+struct nemoh_server {
+  std::vector<std::thread> threads;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<Request>> queue;  // requests with frequencies not yet handed out
+  std::unordered_map<long long, std::shared_ptr<Request>> table;
+  double budget = 0, in_flight = 0;
+  int running = 0;
+  bool stop = false;
+  long long next_id = 0;
+  std::vector<double> xr, xz, apd1x, apd1z, apd2x, apd2z;
+  std::string err;
+};
+
+namespace {
+
+void geo_ptrs(const Request& r, const double* out[8]) {
+  for (int k = 0; k < 8; ++k) out[k] = r.geo.data() + (size_t)k * r.nfa;
+}
+
+// Points the worker's handle at r's mesh (creating the handle on first use).
+int bind(nemoh_server* s, Worker& w, Request& r) {
+  if (w.h && w.bound == r.id) return NEMOH_OK;
+  const double* geo[8];
+  geo_ptrs(r, geo);
+  int e;
+  if (!w.h) {
+    e = nemoh_core_create(&w.h, r.np, r.nfa, r.nsym, r.x.data(), r.y.data(), r.z.data(), r.m1.data(), r.m2.data(),
+                          r.m3.data(), r.m4.data(), geo[0], geo[1], geo[2], geo[3], geo[4], geo[5], geo[6], geo[7],
+                          r.zer, r.g, r.rho, r.depth, s->xr.data(), s->xz.data(), s->apd1x.data(), s->apd1z.data(),
+                          s->apd2x.data(), s->apd2z.data());
+    if (e) {
+      w.h = nullptr;
+      r.set_error(e, "could not create a solver handle");
+      return e;
+    }
+  } else {
+    e = nemoh::core_bind_mesh(w.h, r.np, r.nfa, r.nsym, r.x.data(), r.y.data(), r.z.data(), r.m1.data(),
+                              r.m2.data(), r.m3.data(), r.m4.data(), geo, r.zer, r.g, r.rho, r.depth);
+    if (e) {
+      w.bound = -1;
+      r.set_error(e, nemoh_core_last_error(w.h));
+      return e;
+    }
+  }
+  w.bound = r.id;
+  return NEMOH_OK;
+}
+
+// Panel geometry and the per-frequency NQ, once per request.
+bool prepare(nemoh_server* s, Worker& w, Request& r) {
+  std::unique_lock<std::mutex> lk(r.pm);
+  if (r.prep == 0) {
+    r.prep = 1;
+    lk.unlock();
+    const int n = r.nfa;
+    r.geo.assign(8 * (size_t)n, 0.0);
+    double* g = r.geo.data();
+    int e = nemoh_core_prepare_mesh(r.np, n, r.nsym, r.x.data(), r.y.data(), r.z.data(), r.m1.data(), r.m2.data(),
+                                    r.m3.data(), r.m4.data(), g, g + n, g + 2 * n, g + 3 * n, g + 4 * n, g + 5 * n,
+                                    g + 6 * n, g + 7 * n, &r.zer);
+    if (e) r.set_error(e, "degenerate panel in the request mesh");
+    if (!e) e = bind(s, w, r);
+    if (!e) {
+      r.nq_before.assign(r.nfreq, 0);
+      e = nemoh::core_nq_sequence(w.h, r.nfreq, r.periods.data(), r.nq_before.data());
+      if (e) r.set_error(e, nemoh_core_last_error(w.h));
+    }
+    lk.lock();
+    r.prep = e ? -1 : 2;
+    r.pcv.notify_all();
+  } else {
+    r.pcv.wait(lk, [&] { return r.prep != 1; });
+  }
+  return r.prep == 2;
+}
+
+void run_item(nemoh_server* s, Worker& w, Request& r, int f) {
+  if (!prepare(s, w, r) || r.failed()) return;
+  if (bind(s, w, r)) return;
+  const size_t ntot = (size_t)r.nfa << r.nsym;
+  const size_t per_f = 2 * ntot * r.nrhs, per_force = 2 * (size_t)r.nint * r.nrhs;
+  int e = nemoh_core_set_precision(w.h, r.precision, 0.0, 0);
+  nemoh::core_set_nq_state(w.h, r.nq_before[f]);
+  nemoh_freq fs;
+  if (!e) e = nemoh_core_freq_setup(w.h, r.periods[f], &fs);
+  if (!e)
+    e = nemoh_core_solve_multi(w.h, r.periods[f], fs.amh, fs.nexp, fs.ambda, fs.ar, fs.nq, fs.qq, fs.cq, r.nrhs,
+                               r.nvel.data() + f * per_f, nullptr, r.nint, r.nds.data(),
+                               r.forces.data() + f * per_force);
+  if (e) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "frequency %d: ", f);
+    r.set_error(e, buf + std::string(nemoh_core_last_error(w.h)));
+  }
+}
+
+void worker_loop(nemoh_server* s) {
+  Worker w;
+  for (;;) {
+    std::shared_ptr<Request> r;
+    int f = 0;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] {
+        if (s->stop) return true;
+        if (s->queue.empty()) return false;
+        return s->in_flight + s->queue.front()->budget <= s->budget || s->running == 0;
+      });
+      if (s->queue.empty()) {
+        if (s->stop) break;
+        continue;
+      }
+      r = s->queue.front();
+      f = r->next++;
+      if (r->next == r->nfreq) s->queue.pop_front();
+      s->in_flight += r->budget;
+      ++s->running;
+      if (r->t_start < 0) r->t_start = now_ms();
+    }
+    run_item(s, w, *r, f);
+    bool last;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->in_flight -= r->budget;
+      --s->running;
+      last = ++r->finished == r->nfreq;
+      if (last) {
+        r->t_done = now_ms();
+        r->done = true;
+      }
+    }
+    if (last) s->cv_done.notify_all();
+    s->cv_work.notify_all();
+  }
+  if (w.h) nemoh_core_destroy(w.h);
+}
+
+}  // namespace
+
+extern "C" {
+
+int nemoh_server_create(nemoh_server** out, int workers, double budget_bytes, const double* xr, const double* xz,
+                        const double* apd1x, const double* apd1z, const double* apd2x, const double* apd2z) {
+  if (!out) return NEMOH_ERR_ARG;
+  *out = nullptr;
+  if (workers < 1 || workers > 64 || !(budget_bytes > 0) || !xr || !xz || !apd1x || !apd1z || !apd2x || !apd2z)
+    return NEMOH_ERR_ARG;
+  if (nemoh::core_device_count() < 1) return NEMOH_ERR_HIP;
+  nemoh_server* s = new nemoh_server();
+  s->budget = budget_bytes;
+  const size_t nr = 328, nz = 46, nt = nr * nz;
+  s->xr.assign(xr, xr + nr);
+  s->xz.assign(xz, xz + nz);
+  s->apd1x.assign(apd1x, apd1x + nt);
+  s->apd1z.assign(apd1z, apd1z + nt);
+  s->apd2x.assign(apd2x, apd2x + nt);
+  s->apd2z.assign(apd2z, apd2z + nt);
+  for (int k = 0; k < workers; ++k) s->threads.emplace_back(worker_loop, s);
+  *out = s;
+  return NEMOH_OK;
+}
+
+int nemoh_server_submit(nemoh_server* s, int np, int nfa, int nsym, const double* x, const double* y, const double* z,
+                        const int* m1, const int* m2, const int* m3, const int* m4, double depth, double rho,
+                        double g, int precision, int nfreq, const double* periods, int nrhs, const double* nvel,
+                        int nint, const double* nds, long long* id) {
+  if (!s || !id || np <= 0 || nfa <= 0 || (nsym != 0 && nsym != 1) || !x || !y || !z || !m1 || !m2 || !m3 || !m4 ||
+      nfreq <= 0 || !periods || nrhs <= 0 || nrhs > 16 || !nvel || nint <= 0 || !nds || depth < 0 ||
+      (precision != NEMOH_FP64 && precision != NEMOH_MIXED))
+    return NEMOH_ERR_ARG;
+  auto r = std::make_shared<Request>();
+  const size_t ntot = (size_t)nfa << nsym;
+  r->np = np;
+  r->nfa = nfa;
+  r->nsym = nsym;
+  r->x.assign(x, x + np);
+  r->y.assign(y, y + np);
+  r->z.assign(z, z + np);
+  r->m1.assign(m1, m1 + nfa);
+  r->m2.assign(m2, m2 + nfa);
+  r->m3.assign(m3, m3 + nfa);
+  r->m4.assign(m4, m4 + nfa);
+  r->depth = depth;
+  r->rho = rho;
+  r->g = g;
+  r->precision = precision;
+  r->nfreq = nfreq;
+  r->periods.assign(periods, periods + nfreq);
+  r->nrhs = nrhs;
+  r->nint = nint;
+  r->nvel.assign(nvel, nvel + 2 * ntot * nrhs * nfreq);
+  r->nds.assign(nds, nds + (size_t)nint * ntot);
+  r->forces.assign(2 * (size_t)nint * nrhs * nfreq, 0.0);
+  r->budget = 8.0 * (double)ntot * (double)ntot;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return NEMOH_ERR_ARG;
+    r->id = s->next_id++;
+    r->t_submit = now_ms();
+    s->table[r->id] = r;
+    s->queue.push_back(r);
+    *id = r->id;
+  }
+  s->cv_work.notify_all();
+  return NEMOH_OK;
+}
+
+int nemoh_server_wait(nemoh_server* s, long long id, double* forces, double* timing_ms) {
+  if (!s) return NEMOH_ERR_ARG;
+  std::shared_ptr<Request> r;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->table.find(id);
+    if (it == s->table.end()) {
+      s->err = "unknown request id";
+      return NEMOH_ERR_ARG;
+    }
+    r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    s->table.erase(id);
+  }
+  if (timing_ms) {
+    timing_ms[0] = r->t_done - r->t_submit;   // latency
+    timing_ms[1] = r->t_start - r->t_submit;  // wait for the first frequency to start
+    timing_ms[2] = r->t_done - r->t_start;    // first start to last finish
+  }
+  std::lock_guard<std::mutex> lk(r->em);
+  if (r->status) {
+    std::lock_guard<std::mutex> lk2(s->mu);
+    s->err = r->err;
+    return r->status;
+  }
+  if (forces) std::memcpy(forces, r->forces.data(), r->forces.size() * sizeof(double));
+  return NEMOH_OK;
+}
+
+const char* nemoh_server_last_error(nemoh_server* s) {
+  if (!s) return "null handle";
+  std::lock_guard<std::mutex> lk(s->mu);
+  return s->err.c_str();
+}
+
+void nemoh_server_destroy(nemoh_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (auto& t : s->threads) t.join();
+  delete s;
+}
+
+}  // extern "C"

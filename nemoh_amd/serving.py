@@ -1,0 +1,148 @@
+"""Request loop on one GPU: many independent solves of different sizes.
+
+``Server`` wraps the nemoh_server_* C ABI (the loop itself is native:
+worker threads, each owning a solver handle and HIP stream, take one
+(request, frequency) at a time first come first served while the
+in-flight frequencies fit the device-memory budget of N^2 complex64
+each).  ``two_cylinder_request`` draws the serving workload: two vertical
+cylinders of one random radius r ~ U(2, 15) m at a spacing U(3, 10) r,
+N_total log-uniform in [200, 16000] panels, 12 degrees of freedom (six
+per body), 1-4 headings and 1-64 frequencies, infinite depth.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import bodycond as bcm
+from . import core
+from . import mesh as nmesh
+
+
+@dataclass
+class Request:
+    """One solve request: mesh, frequencies and the problems at each."""
+    mesh: nmesh.Mesh
+    depth: float
+    omegas: np.ndarray      # (nfreq,) rad/s
+    nvel: np.ndarray        # (nfreq, ntot, nrhs) complex normal velocities
+    nds: np.ndarray         # (nint, ntot) force weights
+    rho: float = 1025.0
+    g: float = 9.81
+
+    @property
+    def npanels(self) -> int:
+        return self.mesh.npanels * 2 ** self.mesh.isym
+
+    @property
+    def nfreq(self) -> int:
+        return self.omegas.size
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Server:
+    def __init__(self, workers: int = 4, budget_bytes: float | None = None, tables=None):
+        L = core.load()
+        if tables is None:
+            tables = core.build_tables()
+        xr, xz, (a1x, a1z, a2x, a2z) = tables
+        self._tabs = [_f64(xr), _f64(xz)] + [np.asfortranarray(np.asarray(a, dtype=np.float64)).ravel(order="F")
+                                             for a in (a1x, a1z, a2x, a2z)]
+        if budget_bytes is None:
+            budget_bytes = 32.0 * 2 ** 30
+        h = C.c_void_p()
+        r = L.nemoh_server_create(C.byref(h), int(workers), float(budget_bytes), *[core._p(a) for a in self._tabs])
+        if r:
+            raise core.NemohError(r, "nemoh_server_create failed (is a HIP device present?)")
+        self.h = h
+        self._shapes = {}
+
+    def submit(self, req: Request, precision: int = core.MIXED) -> int:
+        m = req.mesh
+        v = _f64(m.vertices)
+        pc = [np.ascontiguousarray(m.panels[:, k] + 1, dtype=np.int32) for k in range(4)]
+        nfreq, ntot, nrhs = req.nvel.shape
+        if ntot != req.npanels or req.nds.shape[1] != ntot or req.omegas.size != nfreq:
+            raise ValueError("request arrays do not match the mesh")
+        periods = _f64(2.0 * np.pi / np.asarray(req.omegas, dtype=np.float64))
+        # per frequency (ntot, nrhs) column-major, interleaved complex
+        nv = np.ascontiguousarray(np.asarray(req.nvel, dtype=np.complex128).transpose(0, 2, 1))
+        nds = _f64(np.asarray(req.nds).T)  # (nint, ntot) column-major
+        nint = req.nds.shape[0]
+        rid = C.c_longlong()
+        r = core.load().nemoh_server_submit(
+            self.h, v.shape[0], m.npanels, int(m.isym), core._p(_f64(v[:, 0])), core._p(_f64(v[:, 1])),
+            core._p(_f64(v[:, 2])), *[core._ip_(a) for a in pc], float(req.depth), float(req.rho), float(req.g),
+            int(precision), nfreq, core._p(periods), nrhs, nv.ctypes.data_as(core._dp), nint, core._p(nds),
+            C.byref(rid))
+        if r:
+            raise core.NemohError(r, "nemoh_server_submit: bad request")
+        self._shapes[rid.value] = (nfreq, nrhs, nint)
+        return rid.value
+
+    def wait(self, rid: int):
+        """Forces (nfreq, nint, nrhs) complex and the request's timings (ms):
+        latency (submit to done), queue (submit to first start), service
+        (first start to done)."""
+        nfreq, nrhs, nint = self._shapes.pop(rid)
+        out = np.zeros((nfreq, nrhs, nint), np.complex128)
+        t = np.zeros(3)
+        r = core.load().nemoh_server_wait(self.h, C.c_longlong(rid), out.ctypes.data_as(core._dp), core._p(t))
+        if r:
+            raise core.NemohError(r, core.load().nemoh_server_last_error(self.h).decode())
+        return out.transpose(0, 2, 1), dict(zip(("latency", "queue", "service"), t.tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            core.load().nemoh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _cylinder_for(npan: int, radius: float, draft: float, x0: float) -> nmesh.Mesh:
+    """Full vertical cylinder of about npan panels, side panels near square:
+    npan = n_theta (n_z + n_r) with n_z ~ n_theta draft / (2 pi r) and
+    n_r ~ n_theta / (2 pi)."""
+    f = (draft / radius + 1.0) / (2.0 * math.pi)
+    n_theta = max(8, 4 * round(math.sqrt(npan / f) / 4))
+    n_r = max(1, round(n_theta / (2.0 * math.pi)))
+    n_z = max(1, round(npan / n_theta) - n_r)
+    return nmesh.cylinder(radius, draft, n_theta, n_z, n_r, half=False, x0=x0)
+
+
+def two_cylinder_request(rng: np.random.Generator, n_range=(200, 16000), max_freq: int = 64,
+                         max_headings: int = 4, g: float = 9.81, rho: float = 1025.0) -> Request:
+    r = rng.uniform(2.0, 15.0)
+    spacing = rng.uniform(3.0, 10.0) * r
+    n_total = math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))
+    nh = int(rng.integers(1, max_headings + 1))
+    nf = int(rng.integers(1, max_freq + 1))
+    draft = r
+    a = _cylinder_for(int(n_total / 2), r, draft, -spacing / 2)
+    b = _cylinder_for(int(n_total / 2), r, draft, spacing / 2)
+    m = nmesh.merge([a, b])
+    body = np.concatenate([np.ones(a.npanels, int), np.full(b.npanels, 2)])
+    modes = bcm.six_dof(axis=(-spacing / 2, 0.0, 0.0), body=1) + bcm.six_dof(axis=(spacing / 2, 0.0, 0.0), body=2)
+    omegas = np.linspace(0.3, 2.0, nf) if nf > 1 else np.array([0.8])
+    betas = [180.0 * k / nh for k in range(nh)]
+    rad = np.stack([bcm.radiation(m, md, body) for md in modes], axis=1)
+    nvel = np.empty((nf, m.npanels, nh + len(modes)), np.complex128)
+    for i, w in enumerate(omegas):
+        for j, bdeg in enumerate(betas):
+            nvel[i, :, j] = bcm.diffraction(m, w, np.deg2rad(bdeg), 0.0, g, rho)[0]
+        nvel[i, :, nh:] = rad
+    return Request(m, 0.0, omegas, nvel, bcm.integration(m, modes, body), rho, g)

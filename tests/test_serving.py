@@ -1,0 +1,133 @@
+"""The request loop (nemoh_server_*) and the serving workload.
+
+GPU tests: a request's forces are bit for bit the same alone and in a mix
+of requests of other sizes, depths and precisions running concurrently;
+they equal those of a sequential session over the same frequencies; the
+finite-depth barge request matches the reference's forces; a failing
+request leaves the others untouched.  The handle's GM (largest horizontal
+vertex distance, computed on the device) equals the exact maximum.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from nemoh_amd import bodycond as bcm
+from nemoh_amd import core
+from nemoh_amd import io as nio
+from nemoh_amd import mesh as nmesh
+from nemoh_amd import serving
+from nemoh_amd.solver import Solver
+
+
+def _tables():
+    t = load_golden("fn_crek.npz")
+    return t["xr"], t["xz"], (t["apd1x"], t["apd1z"], t["apd2x"], t["apd2z"])
+
+
+def test_two_cylinder_request_shape():
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        r = serving.two_cylinder_request(rng, n_range=(200, 3000), max_freq=4)
+        nf, ntot, nrhs = r.nvel.shape
+        assert r.mesh.isym == 0 and ntot == r.npanels and 150 <= ntot <= 3600
+        assert nf == r.omegas.size and 13 <= nrhs <= 16 and r.nds.shape == (12, ntot)
+        # each body's modes act on its own panels only
+        half = ntot // 2
+        assert np.all(r.nds[:6, half:] == 0) and np.all(r.nds[6:, :half] == 0)
+        assert np.array_equal(r.nvel[0, :, -12:], r.nvel[-1, :, -12:])
+
+
+def _barge_request():
+    d = load_golden("case_barge_fd.npz")
+    m = nmesh.Mesh(d["vertices"], d["panels"], int(d["isym"]))
+    om = np.unique(d["omega"])
+    nf = om.size
+    nvel = d["nvel"].reshape(m.npanels, nf, -1).transpose(1, 0, 2)
+    ref = nio.table_to_forces(d["forces_table"], d["omega"], d["kind"])
+    return serving.Request(m, float(d["depth"]), om, nvel, d["nds"], float(d["rho"]), float(d["g"])), ref
+
+
+def _cyl_request(seed, omegas):
+    rng = np.random.default_rng(seed)
+    r = serving.two_cylinder_request(rng, n_range=(500, 900), max_freq=1)
+    nh = r.nvel.shape[2] - 12  # headings first, then the 12 radiation columns
+    nvel = np.repeat(r.nvel[:1], len(omegas), axis=0)
+    for i, w in enumerate(omegas):
+        for j in range(nh):
+            nvel[i, :, j] = bcm.diffraction(r.mesh, w, np.deg2rad(180.0 * j / nh), 0.0, r.g, r.rho)[0]
+    return serving.Request(r.mesh, 0.0, np.asarray(omegas, float), nvel, r.nds, r.rho, r.g)
+
+
+def _flat(f):
+    """(nfreq, nint, nrhs) -> (nint, nfreq * nrhs) in problem order."""
+    return f.transpose(1, 0, 2).reshape(f.shape[1], -1)
+
+
+def _sequential(req, precision):
+    s = Solver(req.mesh, req.depth, req.rho, req.g, precision=precision, tables=_tables())
+    out = []
+    for i, w in enumerate(req.omegas):
+        _, f = s.core.solve_frequency(2 * np.pi / w, req.nvel[i], nds=req.nds, want_pressure=False)
+        out.append(f)
+    s.close()
+    return np.stack(out)
+
+
+@pytest.mark.gpu
+def test_extent_matches_exact_maximum():
+    rng = np.random.default_rng(11)
+    for np_, isym in ((5, 0), (255, 1), (256, 0), (257, 1), (700, 0), (1100, 1)):
+        v = rng.uniform(-40, 40, (np_, 3))
+        v[:, 2] = -np.abs(v[:, 2]) - 1
+        nq = max(1, np_ // 4)
+        pan = (np.arange(4 * nq) % np_).reshape(nq, 4).astype(np.int32)
+        c = core.Core(v, pan, isym, 0.0, geometry=_dummy_geometry(nq), tables=_tables())
+        x, y = v[:, 0], v[:, 1]
+        i, j = np.triu_indices(np_, 1)
+        dx = x[j] - x[i]
+        best = (dx * dx + (y[i] - y[j]) ** 2).max()
+        if isym:
+            best = max(best, (dx * dx + (y[i] + y[j]) ** 2).max())
+        assert c.extent() == np.sqrt(best), (np_, isym)
+        c.close()
+
+
+def _dummy_geometry(nq):
+    g = {k: np.ones(nq) for k in ("xg", "yg", "zg", "xn", "yn", "zn", "aire", "tdis")}
+    g["zer"] = -1.0
+    return g
+
+
+@pytest.mark.gpu
+def test_server_results_identical_alone_and_in_mix():
+    barge, ref = _barge_request()
+    a = _cyl_request(5, [0.5, 1.1, 1.7])
+    b = _cyl_request(9, [0.4, 0.9])
+    bad = _cyl_request(5, [0.7, 1e-5, 1.2])  # |w| < 1e-4 at its second frequency
+    with serving.Server(workers=3, tables=_tables()) as srv:
+        alone_a, ta = srv.wait(srv.submit(a, core.MIXED))
+        alone_barge, _ = srv.wait(srv.submit(barge, core.FP64))
+        print(f"alone: cylinders {a.npanels} panels, {ta['service']:.1f} ms")
+        ids = [srv.submit(b, core.MIXED), srv.submit(bad, core.MIXED), srv.submit(barge, core.FP64),
+               srv.submit(a, core.MIXED), srv.submit(barge, core.MIXED), srv.submit(a, core.FP64)]
+        mix_b, _ = srv.wait(ids[0])
+        with pytest.raises(core.NemohError) as e:
+            srv.wait(ids[1])
+        assert e.value.code == core.ERR_OMEGA and "frequency 1" in str(e.value)
+        mix_barge, _ = srv.wait(ids[2])
+        mix_a, tm = srv.wait(ids[3])
+        mix_barge_mixed, _ = srv.wait(ids[4])
+        mix_a_fp64, _ = srv.wait(ids[5])
+    assert np.array_equal(alone_a, mix_a)
+    assert np.array_equal(alone_barge, mix_barge)
+    # the same as a sequential session over the request's frequencies
+    assert np.array_equal(mix_a, _sequential(a, core.MIXED))
+    assert np.array_equal(mix_a_fp64, _sequential(a, core.FP64))
+    assert np.array_equal(mix_b, _sequential(b, core.MIXED))
+    # finite-depth barge (its last frequency on the infinite-depth path with
+    # the carried NQ) against the reference's forces
+    def normwise(x, r):
+        return float((np.abs(x - r).max(axis=0) / np.abs(r).max(axis=0)).max())
+    assert normwise(_flat(mix_barge), ref) < 1e-10
+    assert normwise(_flat(mix_barge_mixed), ref) < 1e-6
+    assert normwise(_flat(mix_a), _flat(mix_a_fp64)) < 1e-6
