@@ -29,6 +29,11 @@ sys.path.insert(0, str(ROOT))
 METRIC = "freq-solves/s at 4k panels, mixed vs FP64 (<=1e-6 rel err); serving p99 ms"
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense TFLOP/s (FP64 / FP32 matrix)
 HBM_PEAK_GBS = 8000.0
+# HBM bytes of lu_update_kernel<float> per launch over its algorithmic bytes
+# (16 m^2 + 1024 m), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# this workload, each scaled by the counter's reading for a 256 MiB copy
+# (profiles/lu_update_barge4000_hbm_traffic.md)
+LU_UPDATE_TRAFFIC_RATIO = 1.246
 
 
 def workload():
@@ -235,6 +240,38 @@ def run_serving(rates, nreq, workers, seed=2024):
     return out
 
 
+def run_c4(nbodies, workers, seed=256):
+    """Batched bodies: nbodies independent bodies (cylinder / box / prolate
+    spheroid, dimensions U(2,50) m, draft U(0.2,0.8) of height, N
+    log-uniform in [300, 6000], seeded PCG64), 7 RHS (6 DOF + heading 0), 8
+    frequencies in 0.2-2.0 rad/s, infinite depth, all submitted at once to
+    the native request loop; freq-solves/s = bodies x frequencies / wall."""
+    from nemoh_amd import core as ncore
+    from nemoh_amd import serving
+    rng = np.random.default_rng(seed)
+    reqs = [serving.random_body_request(rng) for _ in range(nbodies)]
+    ns = np.array([r.npanels for r in reqs], dtype=np.float64)
+    nfreq = sum(r.nfreq for r in reqs)
+    tables = ncore.build_tables()
+    res, forces = {}, {}
+    for name, prec in (("mixed", ncore.MIXED), ("fp64", ncore.FP64)):
+        with serving.Server(workers=workers, tables=tables) as srv:
+            t0 = time.perf_counter()
+            ids = [srv.submit(r, prec) for r in reqs]
+            forces[name] = [srv.wait(i)[0] for i in ids]
+            wall = time.perf_counter() - t0
+        res[name] = round(nfreq / wall, 2)
+        print(f"c4 {name}: {nfreq} freq-solves in {wall:.2f} s", file=sys.stderr, flush=True)
+    err = max(float((np.abs(a - b).max(axis=1) / np.abs(b).max(axis=1)).max())
+              for a, b in zip(forces["mixed"], forces["fp64"]))
+    return {"workload": f"{nbodies} bodies (seed {seed}): cylinder/box/prolate spheroid, U(2,50) m, draft "
+                        "U(0.2,0.8) of height, N log-uniform 300-6000, 7 RHS, 8 frequencies 0.2-2.0 rad/s, "
+                        "infinite depth",
+            "unit": "freq-solves/s", "mixed": res["mixed"], "fp64": res["fp64"],
+            "speedup_mixed_vs_fp64": round(res["mixed"] / res["fp64"], 3), "max_rel_err_mixed_vs_fp64": err,
+            "E_N2": float((ns ** 2).mean()), "E_N3": float((ns ** 3).mean()), "workers": workers}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +283,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rates", default="5", help="comma-separated arrival rates (req/s)")
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--c4-bodies", type=int, default=256, help="bodies of the batched run (0: skip)")
+    ap.add_argument("--c4-workers", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: each frequency solve is independent, so N GPUs "
@@ -272,6 +311,8 @@ def main():
     upd_flops = float(np.sum(phmx["update_flops"]))
     upd_launches = int(np.sum(phmx["update_launches"]))
     achieved = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    trail = n - 64 * np.arange(1, (n + 63) // 64)  # trailing orders of the update launches
+    upd_bytes = float(np.mean(16.0 * trail ** 2 + 1024.0 * trail))
     # LU + solve flops over the factor + solve phases (complex64 LU 8/3 N^3,
     # 2 triangular solves 8 N^2 NRHS per refinement sweep)
     lu_flops = 8.0 / 3.0 * n ** 3
@@ -293,7 +334,10 @@ def main():
             "mixed_refine_iterations_max": int(max(phmx["it"]) if phmx["it"] else 0)},
         "roofline": {"bound": "mfma", "kernel": "lu_update_kernel<float> (LU trailing update, complex64 MFMA)",
                      "achieved": round(achieved, 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK["fp32"], 4), "traffic": None,
+                     "frac": round(achieved / PEAK["fp32"], 4),
+                     "traffic": round(LU_UPDATE_TRAFFIC_RATIO * upd_bytes),
+                     "algorithmic_bytes_per_launch": round(upd_bytes),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated, x algorithmic",
                      "launches": upd_launches, "avg_launch_us": round(1e3 * upd_ms / max(upd_launches, 1), 3),
                      "flops_per_launch_avg": upd_flops / max(upd_launches, 1)},
         "lu_factor_tflops": round(lu_flops / (fac_ms * 1e-3) / 1e12, 3),
@@ -301,6 +345,8 @@ def main():
     }
     if a.c3_steps > 0:
         out["c3"] = run_c3(a.c3_steps)
+    if a.c4_bodies > 0:
+        out["c4"] = run_c4(a.c4_bodies, a.c4_workers)
     if a.serve_requests > 0:
         out["serving"] = run_serving([float(x) for x in a.serve_rates.split(",")], a.serve_requests,
                                      a.serve_workers)

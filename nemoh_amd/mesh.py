@@ -192,29 +192,37 @@ def spheroid(length: float, diameter: float, draft_frac: float, n_u: int, n_v: i
              half: bool = False) -> Mesh:
     """Prolate spheroid with axis along x, cut at the free surface: the
     wetted part below z = 0, with the centre at depth so that the wetted
-    draft is ``draft_frac * diameter``."""
+    draft is ``draft_frac * diameter``.  Each cross-section x = a u is a
+    circle of radius r(u) about (0, zc); its wetted arc is where
+    zc + r cos(phi) <= 0, i.e. phi in [phi0(u), 2 pi - phi0(u)] with
+    cos(phi0) = -zc / r (the whole circle when -zc >= r).  Sections that are
+    dry (r <= zc, above-axis cuts) are left out, so the mesh closes at a
+    waterline point where a section dries out and at a tip otherwise."""
     a, b = length / 2, diameter / 2
     zc = b - draft_frac * diameter
     pool = _VertexPool(max(a, b))
-    # wetted azimuth range: b*cos(phi) + zc <= 0 about the x axis
-    cmax = -zc / b
-    phi0 = math.acos(max(-1.0, min(1.0, cmax)))
-    phis = np.linspace(phi0, math.pi, n_v + 1) if half else np.linspace(phi0, 2 * math.pi - phi0, n_v + 1)
-    us = np.linspace(-1.0, 1.0, n_u + 1)
+    u_max = math.sqrt(max(0.0, 1.0 - (zc / b) ** 2)) if zc > 0 else 1.0
+    us = np.linspace(-u_max, u_max, n_u + 1)
+    ts = np.linspace(0.0, 1.0, n_v + 1)
 
-    def pt(u, phi):
+    def pt(u, t):
         r = b * math.sqrt(max(0.0, 1.0 - u * u))
-        return (a * u, r * math.sin(phi), zc + r * math.cos(phi))
+        c = -zc / r if r > 0 else (1.0 if zc < 0 else -1.0)
+        phi0 = math.acos(max(-1.0, min(1.0, c)))
+        phi = phi0 + t * ((math.pi if half else 2 * math.pi - phi0) - phi0)
+        z = 0.0 if (r > 0 and -1.0 < c < 1.0 and t in (0.0, 1.0) and not (half and t == 1.0)) \
+            else zc + r * math.cos(phi)
+        return (a * u, r * math.sin(phi), min(z, 0.0))
 
     quads = []
     for i in range(n_u):
         for j in range(n_v):
-            q = [pool.add(pt(us[i], phis[j])), pool.add(pt(us[i + 1], phis[j])),
-                 pool.add(pt(us[i + 1], phis[j + 1])), pool.add(pt(us[i], phis[j + 1]))]
+            q = [pool.add(pt(us[i], ts[j])), pool.add(pt(us[i + 1], ts[j])),
+                 pool.add(pt(us[i + 1], ts[j + 1])), pool.add(pt(us[i], ts[j + 1]))]
             if len(set(q)) < 3:
                 continue
             if len(set(q)) == 3:  # degenerate at a tip: repeat the shared vertex
-                seen, out = [], []
+                seen = []
                 for v in q:
                     if v not in seen:
                         seen.append(v)

@@ -146,3 +146,37 @@ def two_cylinder_request(rng: np.random.Generator, n_range=(200, 16000), max_fre
             nvel[i, :, j] = bcm.diffraction(m, w, np.deg2rad(bdeg), 0.0, g, rho)[0]
         nvel[i, :, nh:] = rad
     return Request(m, 0.0, omegas, nvel, bcm.integration(m, modes, body), rho, g)
+
+
+def random_body_request(rng: np.random.Generator, n_range=(300, 6000), nfreq: int = 8, g: float = 9.81,
+                        rho: float = 1025.0) -> Request:
+    """One body of the batched workload: a cylinder, box or prolate spheroid
+    (equally likely) with dimensions U(2, 50) m and a draft of U(0.2, 0.8)
+    of its height, meshed to about N panels (N log-uniform in n_range);
+    6 DOF about the waterplane centre + heading 0 (7 right-hand sides),
+    nfreq frequencies uniform in 0.2-2.0 rad/s, infinite depth."""
+    kind = int(rng.integers(0, 3))
+    length, width, height = rng.uniform(2.0, 50.0, 3)
+    draft = rng.uniform(0.2, 0.8) * height
+    target = math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))
+    if kind == 0:  # vertical cylinder of diameter `width`
+        r = width / 2  # n_theta (n_z + n_r) panels: every bottom ring has n_theta sectors
+        h = math.sqrt(2 * math.pi * r * (draft + r) / target)
+        m = nmesh.cylinder(r, draft, max(8, 4 * round(2 * math.pi * r / h / 4)), max(1, round(draft / h)),
+                           max(1, round(r / h)), half=False)
+    elif kind == 1:  # box length x width x draft
+        h = math.sqrt((length * width + 2 * draft * (length + width)) / target)
+        m = nmesh.box(length, width, draft, max(1, round(length / h)), max(1, round(width / h)),
+                      max(1, round(draft / h)))
+    else:  # prolate spheroid, axis along x, diameter min(width, length)
+        d = min(width, length)
+        n_u = max(4, round(math.sqrt(target * length / (length + d))))
+        m = nmesh.spheroid(max(length, d * 1.01), d, draft / height, n_u, max(4, round(target / n_u)))
+    modes = bcm.six_dof()
+    omegas = np.linspace(0.2, 2.0, nfreq)
+    rad = np.stack([bcm.radiation(m, md) for md in modes], axis=1)
+    nvel = np.empty((nfreq, m.npanels, 1 + len(modes)), np.complex128)
+    for i, w in enumerate(omegas):
+        nvel[i, :, 0] = bcm.diffraction(m, w, 0.0, 0.0, g, rho)[0]
+        nvel[i, :, 1:] = rad
+    return Request(m, 0.0, omegas, nvel, bcm.integration(m, modes), rho, g)

@@ -43,3 +43,30 @@ def test_semisub_configuration():
         assert abs(v - vol) < 1e-9 * vol
     r = np.hypot(np.abs(g["xg"][cols]) - 25.0, np.abs(g["yg"][cols]) - 25.0)
     assert r.max() <= 6.0 + 1e-9
+
+
+def test_spheroid_wetted_part_and_volume():
+    """Prolate spheroid 40 x 10 m cut at z = 0 for drafts below, at and
+    above the axis: no vertex above the free surface, and sum(z n_z A)
+    within 3% below the exact submerged volume (the inscribed polyhedron
+    is slightly smaller), the integral of the circular-segment areas."""
+    import math
+    from scipy.integrate import quad
+    a, b = 20.0, 5.0
+    for df in (0.2, 0.5, 0.8):
+        m = nmesh.spheroid(2 * a, 2 * b, df, 24, 16)
+        assert m.vertices[:, 2].max() <= 0.0
+        g = _geometry(m)
+        vol = np.sum(g["zg"] * g["zn"] * g["aire"])
+        d = b - df * 2 * b  # height of the axis above the free surface
+
+        def seg(x):
+            r = b * math.sqrt(max(0.0, 1 - (x / a) ** 2))
+            if d >= r:
+                return 0.0
+            if d <= -r:
+                return math.pi * r * r
+            return r * r * math.acos(d / r) - d * math.sqrt(r * r - d * d)
+
+        exact = quad(seg, -a, a, limit=200)[0]
+        assert 0.97 * exact < vol < exact, (df, vol, exact)

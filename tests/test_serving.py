@@ -131,3 +131,29 @@ def test_server_results_identical_alone_and_in_mix():
     assert normwise(_flat(mix_barge), ref) < 1e-10
     assert normwise(_flat(mix_barge_mixed), ref) < 1e-6
     assert normwise(_flat(mix_a), _flat(mix_a_fp64)) < 1e-6
+
+
+def test_random_body_request_shape():
+    rng = np.random.default_rng(7)
+    for _ in range(12):
+        r = serving.random_body_request(rng, n_range=(300, 1200), nfreq=3)
+        nf, ntot, nrhs = r.nvel.shape
+        assert (nf, nrhs) == (3, 7) and r.nds.shape == (6, ntot) and r.mesh.isym == 0
+        assert 250 <= ntot <= 1400
+        assert r.mesh.vertices[:, 2].max() <= 1e-9  # wetted surface only
+
+
+@pytest.mark.gpu
+def test_batched_bodies_mixed_vs_fp64():
+    """A seeded batch of cylinders, boxes and spheroids through the request
+    loop: mixed within 1e-6 of FP64 (normwise per problem) on every body."""
+    rng = np.random.default_rng(21)
+    reqs = [serving.random_body_request(rng, n_range=(300, 1500), nfreq=2) for _ in range(9)]
+    out = {}
+    for prec in (core.MIXED, core.FP64):
+        with serving.Server(workers=4, tables=_tables()) as srv:
+            ids = [srv.submit(r, prec) for r in reqs]
+            out[prec] = [srv.wait(i)[0] for i in ids]
+    for r, a, b in zip(reqs, out[core.MIXED], out[core.FP64]):
+        err = float((np.abs(a - b).max(axis=1) / np.abs(b).max(axis=1)).max())
+        assert err < 1e-6, (r.npanels, err)
