@@ -284,7 +284,7 @@ def main():
     ap.add_argument("--serve-rates", default="5", help="comma-separated arrival rates (req/s)")
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--c4-bodies", type=int, default=256, help="bodies of the batched run (0: skip)")
-    ap.add_argument("--c4-workers", type=int, default=4)
+    ap.add_argument("--c4-workers", type=int, default=8)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: each frequency solve is independent, so N GPUs "
