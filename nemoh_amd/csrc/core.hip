@@ -291,6 +291,10 @@ int solve_block(nemoh_core* c, int b, int nrhs) {
   int e = 0;
   const double bmax = device_absmax(c, c->B, cnt, &e);
   if (e) return fail(c, e, "reduction failed");
+  if (bmax == 0.0) {  // zero right-hand sides (e.g. one symmetry block of a symmetric mode): x = 0
+    HIPCHK(c, hipMemsetAsync(c->Z[b], 0, cnt * sizeof(double2), c->st));
+    return NEMOH_OK;
+  }
   double rmax = bmax;
   int it = 0;
   for (; it < c->max_iter; ++it) {
