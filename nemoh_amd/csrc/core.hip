@@ -34,6 +34,7 @@ struct nemoh_core {
   double* d_xr = nullptr;
   double* d_xz = nullptr;
   double* d_tab = nullptr;
+  double* d_rden = nullptr;  // reciprocal Lagrange denominators (XR, XZ stencils)
   Geometry G{};
   Tables T{};
   FreqParams* d_par = nullptr;
@@ -576,7 +577,21 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
       tab[4 * f + 2] = apd2x[f];
       tab[4 * f + 3] = apd2z[f];
     }
+  // reciprocal Lagrange denominators of every 3-node stencil (PL2's
+  // (U3-U1)*(U3-U2) for the three weight orderings), XR then XZ
+  std::vector<double> rden(3 * (size_t)(kNR - 2 + kNZ - 2));
+  auto stencils = [&](const double* u, int n, double* out) {
+    for (int k = 0; k + 2 < n; ++k) {
+      const double a = u[k], b = u[k + 1], cc = u[k + 2];
+      out[3 * k + 0] = 1.0 / ((a - b) * (a - cc));
+      out[3 * k + 1] = 1.0 / ((b - cc) * (b - a));
+      out[3 * k + 2] = 1.0 / ((cc - a) * (cc - b));
+    }
+  };
+  stencils(xr, kNR, rden.data());
+  stencils(xz, kNZ, rden.data() + 3 * (kNR - 2));
   if (hipMalloc((void**)&c->d_tab, tab.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->d_rden, rden.size() * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_xr, kNR * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_xz, kNZ * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_par, sizeof(FreqParams)) != hipSuccess ||
@@ -590,11 +605,12 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
     c->cu_count = prop.multiProcessorCount;
   }
   if (hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_rden, rden.data(), rden.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_xr, xr, kNR * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_xz, xz, kNZ * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(c->d_info, 0, 2 * sizeof(int)) != hipSuccess)
     return bail(NEMOH_ERR_HIP);
-  c->T = Tables{c->d_xr, c->d_xz, c->d_tab};
+  c->T = Tables{c->d_xr, c->d_xz, c->d_tab, c->d_rden, c->d_rden + 3 * (kNR - 2)};
   if (int r = core_bind_mesh(c, np, nfa, nsym, x, y, z, m1, m2, m3, m4, geo, zer, g, rho, depth)) return bail(r);
   *out = c;
   return NEMOH_OK;
@@ -612,6 +628,7 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->d_xr);
   release(c->d_xz);
   release(c->d_tab);
+  release(c->d_rden);
   release(c->d_par);
   release(c->d_red);
   release(c->d_info);

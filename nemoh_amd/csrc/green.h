@@ -59,6 +59,10 @@ struct Tables {
   const double* xr;   // [328]
   const double* xz;   // [46]
   const double* tab;  // [4 * 328 * 46]
+  // reciprocals of the Lagrange denominators of the 3-node stencil starting
+  // at node k (0-based): rr[3k..3k+2] for XR (k < 326), rz likewise for XZ
+  const double* rr;
+  const double* rz;
 };
 
 struct cplx {
@@ -298,17 +302,34 @@ __device__ __forceinline__ void rankine(const Geometry& G, int i, int j, int nsy
   }
 }
 
-__device__ __forceinline__ double pl2(double u1, double u2, double u3, double xu) {
-  return ((xu - u1) * (xu - u2)) / ((u3 - u1) * (u3 - u2));
+// PL2 (COMPUTE_GREEN_INFD.f90:641-645): ((x-u1)(x-u2)) / ((u3-u1)(u3-u2)),
+// with the reciprocal of the node-only denominator taken from the tables.
+__device__ __forceinline__ double pl2r(double u1, double u2, double rden, double xu) {
+  return ((xu - u1) * (xu - u2)) * rden;
+}
+
+// INT(scale * (LOG10(x) + offset) + add) for x > 0, as the reference
+// evaluates it in double precision.  The hardware single-precision log2
+// gives the value to within 1e-5; unless that lands within 1e-4 of an
+// integer the truncation is already certain, otherwise the double-precision
+// expression decides, so the index is always the FP64 one.  Negative values
+// are only ever clamped up to 2 by the callers, where truncation and floor
+// agree.
+__device__ __forceinline__ int log10_index(double x, double scale, double offset, double add) {
+  const float lf = __builtin_amdgcn_logf((float)x) * 0.30102999566398120f;  // log10 x
+  const float yf = (float)scale * (lf + (float)offset) + (float)add;
+  const float fr = yf - floorf(yf);
+  if (fr > 1e-4f && fr < 1.0f - 1e-4f && fabsf(yf) < 1e6f) return (int)floorf(yf);
+  return (int)(scale * (log10(x) + offset) + add);
 }
 
 // Table index selection (COMPUTE_GREEN_INFD.f90:292-305), 1-based.
 __device__ __forceinline__ int kj_of(double akz) {
-  int kj = (akz < -1.E-2) ? (int)(8.0 * (log10(-akz) + 4.5)) : (int)(5.0 * (log10(-akz) + 6.0));
+  int kj = (akz < -1.E-2) ? log10_index(-akz, 8.0, 4.5, 0.0) : log10_index(-akz, 5.0, 6.0, 0.0);
   return min(max(kj, 2), 45);
 }
 __device__ __forceinline__ int ki_of(double akr) {
-  int ki = (akr < 1.0) ? (int)(5.0 * (log10(akr + 1.E-20) + 6.0) + 1.0) : (int)(3.0 * akr + 28.0);
+  int ki = (akr < 1.0) ? log10_index(akr + 1.E-20, 5.0, 6.0, 1.0) : (int)(3.0 * akr + 28.0);
   return min(max(ki, 2), 327);
 }
 
@@ -321,9 +342,10 @@ __device__ __forceinline__ RWeights r_weights(const Tables& T, double akr) {
   RWeights w;
   w.ki = ki_of(akr);
   const double a = T.xr[w.ki - 2], b = T.xr[w.ki - 1], c = T.xr[w.ki];
-  w.x1 = pl2(b, c, a, akr);
-  w.x2 = pl2(c, a, b, akr);
-  w.x3 = pl2(a, b, c, akr);
+  const double* rd = T.rr + 3 * (w.ki - 2);
+  w.x1 = pl2r(b, c, rd[0], akr);
+  w.x2 = pl2r(c, a, rd[1], akr);
+  w.x3 = pl2r(a, b, rd[2], akr);
   return w;
 }
 
@@ -332,7 +354,8 @@ __device__ __forceinline__ RWeights r_weights(const Tables& T, double akr) {
 __device__ __forceinline__ void interp4(const Tables& T, const RWeights& w, double akz, int kj,
                                         double t[4]) {
   const double a = T.xz[kj - 2], b = T.xz[kj - 1], c = T.xz[kj];
-  const double z1 = pl2(b, c, a, akz), z2 = pl2(c, a, b, akz), z3 = pl2(a, b, c, akz);
+  const double* rd = T.rz + 3 * (kj - 2);
+  const double z1 = pl2r(b, c, rd[0], akz), z2 = pl2r(c, a, rd[1], akz), z3 = pl2r(a, b, rd[2], akz);
   const double zl[3] = {z1, z2, z3};
   double f[3][4];
 #pragma unroll
