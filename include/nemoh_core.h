@@ -85,6 +85,44 @@ int nemoh_core_solve_multi(nemoh_core *ctx, double period, double amh, int nexp,
 int nemoh_core_solve_frequency(nemoh_core *ctx, double period, int nrhs, const double *nvel,
                                double *pressure, int nint, const double *nds, double *forces);
 
+/* Standalone mode with no per-frequency host input: the preProcessor's body
+ * conditions, force weights and Froude-Krylov pressures are generated on
+ * the device from the handle's mesh (preProcessor/Mesh.f90:207-233 panel
+ * geometry, BodyConditions.f90:27-137, Integration.f90:27-86, Main.f90:
+ * 199-243, Compute_Wave and CIH/SIH in Common/Environment.f90:86-123, the
+ * preProcessor's own wave number Environment.f90:45-84).
+ *
+ * set_dofs: body_of_panel(NFA) = the body of each panel (1-based; NULL = all
+ * body 1, the mesh files' cPanel); rad(nrad) the radiation cases and
+ * force(nforce) the force-integration cases of Nemoh.cal; xeff, yeff the
+ * wave reference point (Nemoh.cal line 5).  Fails with NEMOH_ERR_PANEL on a
+ * panel the preProcessor rejects (area < 1e-7, Mesh.f90:222-225).  Binding
+ * the handle to another mesh clears the set-up.
+ *
+ * body_conditions: at omega (rad/s) for nbeta headings beta (radians),
+ * nvel(NFA*2**NSYMY, nbeta + nrad) = the normal velocities of the
+ * diffraction problems then the radiation problems (the preProcessor's
+ * order), pinc(NFA*2**NSYMY, nbeta) the incident-wave pressures and
+ * nds(nforce, NFA*2**NSYMY) the force weights; any output may be NULL.
+ *
+ * solve_waves: the same problems solved at T = 2 pi / omega (the Solver's
+ * main program, NEMOH.f90:135) with the set-up of nemoh_core_solve_frequency:
+ * forces(nforce, nbeta + nrad) = -NDS . pressure and
+ * fk(nforce, nbeta) = -NDS . pinc, the Froude-Krylov forces (Main.f90:
+ * 229-234); nbeta + nrad <= 16. */
+typedef struct {
+  int body;      /* 1-based body index */
+  int kind;      /* 1 = translation along dir, 2 = rotation about the axis through `axis` along dir */
+  double dir[3];
+  double axis[3];
+} nemoh_dof;
+int nemoh_core_set_dofs(nemoh_core *ctx, const int *body_of_panel, int nrad, const nemoh_dof *rad,
+                        int nforce, const nemoh_dof *force, double xeff, double yeff);
+int nemoh_core_body_conditions(nemoh_core *ctx, double omega, int nbeta, const double *beta, double *nvel,
+                               double *pinc, double *nds);
+int nemoh_core_solve_waves(nemoh_core *ctx, double omega, int nbeta, const double *beta, double *forces,
+                           double *fk);
+
 /* Frequency parameters as the standalone mode computes them. */
 typedef struct {
   int finite_depth;
@@ -143,6 +181,9 @@ int nemoh_core_prepare_mesh(int np, int nfa, int nsym, const double *x, const do
 int nemoh_core_build_tables(double *xr, double *xz, double *apd1x, double *apd1z, double *apd2x,
                             double *apd2z);
 double nemoh_core_wavenumber(double ak);
+/* The preProcessor's wave number for frequency w (Common/Environment.f90:
+ * 45-84); NaN if its bisection fails. */
+double nemoh_core_incident_wavenumber(double w, double depth, double g);
 int nemoh_core_theta_rule(double akk, double *qq, double *cq);
 int nemoh_core_prony(double akh, double amh, int *nexp, double *ambda, double *ar);
 

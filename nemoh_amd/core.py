@@ -21,7 +21,8 @@ EXPORTS = (
     "nemoh_core_freq_setup", "nemoh_core_set_precision", "nemoh_core_set_option", "nemoh_core_get_stats", "nemoh_core_enable_timing",
     "nemoh_core_last_error", "nemoh_core_destroy", "nemoh_core_prepare_mesh", "nemoh_core_build_tables",
     "nemoh_core_wavenumber", "nemoh_core_theta_rule", "nemoh_core_prony", "nemoh_core_pairs",
-    "nemoh_core_mfma_probe", "nemoh_core_get_extent", "nemoh_server_create", "nemoh_server_submit",
+    "nemoh_core_mfma_probe", "nemoh_core_get_extent", "nemoh_core_set_dofs", "nemoh_core_body_conditions",
+    "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_server_create", "nemoh_server_submit",
     "nemoh_server_wait", "nemoh_server_last_error", "nemoh_server_destroy",
 )
 
@@ -36,6 +37,23 @@ class FreqSetup(C.Structure):
     _fields_ = [("finite_depth", C.c_int), ("amh", C.c_double), ("nexp", C.c_int),
                 ("ambda", C.c_double * 31), ("ar", C.c_double * 31), ("nq", C.c_int),
                 ("qq", C.c_double * 101), ("cq", C.c_double * 101)]
+
+
+class Dof(C.Structure):
+    """nemoh_dof: body (1-based), kind 1 translation / 2 rotation, dir, axis."""
+    _fields_ = [("body", C.c_int), ("kind", C.c_int), ("dir", C.c_double * 3), ("axis", C.c_double * 3)]
+
+
+def _dofs(modes):
+    """bodycond.Mode list -> nemoh_dof array (None when empty)."""
+    if not modes:
+        return None, 0
+    arr = (Dof * len(modes))()
+    for a, m in zip(arr, modes):
+        a.body, a.kind = int(m.body), int(m.kind)
+        a.dir[:] = [float(v) for v in m.direction]
+        a.axis[:] = [float(v) for v in m.axis]
+    return arr, len(modes)
 
 
 class Stats(C.Structure):
@@ -83,6 +101,12 @@ def load():
     L.nemoh_core_pairs.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
     L.nemoh_core_mfma_probe.argtypes = [C.c_int, _dp]
     L.nemoh_core_get_extent.argtypes = [C.c_void_p, _dp]
+    L.nemoh_core_set_dofs.argtypes = [C.c_void_p, _ip, C.c_int, C.POINTER(Dof), C.c_int, C.POINTER(Dof), C.c_double,
+                                      C.c_double]
+    L.nemoh_core_body_conditions.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
+    L.nemoh_core_solve_waves.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp]
+    L.nemoh_core_incident_wavenumber.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.nemoh_core_incident_wavenumber.restype = C.c_double
     L.nemoh_server_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_double] + [_dp] * 6
     L.nemoh_server_submit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [_dp] * 3 + [_ip] * 4 + \
         [C.c_double] * 3 + [C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.POINTER(C.c_longlong)]
@@ -111,6 +135,11 @@ def _f64(a):
 def wavenumber(ak: float) -> float:
     """X0: root of x tanh x = ak (COMPUTE_GREEN_FD.f90:1635)."""
     return load().nemoh_core_wavenumber(float(ak))
+
+
+def incident_wavenumber(w: float, depth: float, g: float) -> float:
+    """The preProcessor's wave number (Common/Environment.f90:45-84)."""
+    return load().nemoh_core_incident_wavenumber(float(w), float(depth), float(g))
 
 
 def theta_rule(akk: float):
@@ -208,6 +237,43 @@ class Core:
 
     def set_precision(self, mode: int, tol: float = 0.0, max_iter: int = 0):
         self._check(load().nemoh_core_set_precision(self.h, int(mode), float(tol), int(max_iter)))
+
+    def set_dofs(self, radiation, forces=None, body_of_panel=None, xeff: float = 0.0, yeff: float = 0.0):
+        """Degrees of freedom (bodycond.Mode lists) of the radiation problems
+        and of the force integration (default: the same), the body of each
+        panel (1-based, default all 1) and the wave reference point."""
+        self._nrad = len(radiation)
+        forces = radiation if forces is None else forces
+        self._nforce = len(forces)
+        r, nr = _dofs(radiation)
+        f, nf = _dofs(forces)
+        body = None if body_of_panel is None else np.ascontiguousarray(body_of_panel, dtype=np.int32)
+        self._check(load().nemoh_core_set_dofs(self.h, None if body is None else _ip_(body), nr, r, nf, f,
+                                               float(xeff), float(yeff)))
+
+    def body_conditions(self, omega: float, betas):
+        """(nvel (ntot, nbeta + nrad), pinc (ntot, nbeta), nds (nforce, ntot))
+        generated on the device; betas in radians."""
+        b = _f64(np.atleast_1d(betas)) if len(betas) else np.zeros(0)
+        nb = b.size
+        nvel = np.zeros((self.ntot, nb + self._nrad), np.complex128, order="F")
+        pinc = np.zeros((self.ntot, nb), np.complex128, order="F")
+        nds = np.zeros((self._nforce, self.ntot), order="F")
+        self._check(load().nemoh_core_body_conditions(self.h, float(omega), nb, _p(b) if nb else None,
+                                                      nvel.ctypes.data_as(_dp), pinc.ctypes.data_as(_dp),
+                                                      nds.ctypes.data_as(_dp)))
+        return nvel, pinc, nds
+
+    def solve_waves(self, omega: float, betas):
+        """Forces (nforce, nbeta + nrad) and Froude-Krylov forces (nforce,
+        nbeta) with the problems generated on the device."""
+        b = _f64(np.atleast_1d(betas)) if len(betas) else np.zeros(0)
+        nb = b.size
+        forces = np.zeros((self._nforce, nb + self._nrad), np.complex128, order="F")
+        fk = np.zeros((self._nforce, max(nb, 1)), np.complex128, order="F")
+        self._check(load().nemoh_core_solve_waves(self.h, float(omega), nb, _p(b) if nb else None,
+                                                  forces.ctypes.data_as(_dp), fk.ctypes.data_as(_dp)))
+        return forces, fk[:, :nb]
 
     def extent(self) -> float:
         """GM: the largest horizontal vertex distance (to the y-images too

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -26,6 +27,17 @@ struct nemoh_core {
   // handle field that holds it: buffers only grow, so re-binding the handle
   // to a mesh no larger than before allocates (and frees) nothing
   std::unordered_map<const void*, size_t> caps;
+  // preProcessor side (nemoh_core_set_dofs): bodies, degrees of freedom,
+  // its own panel geometry, force weights, incident-wave buffers
+  bool dofs_set = false;
+  int nrad = 0, nforce = 0;
+  double xeff = 0, yeff = 0;
+  double* d_pg = nullptr;     // [xm(3n) | normal(3n) | area(n)]
+  int* d_body = nullptr;      // body of each panel (1-based)
+  nemoh_dof* d_dofs = nullptr;  // nrad radiation dofs, then nforce force dofs
+  double* d_fnds = nullptr;   // force weights (nforce x ntot)
+  double* d_beta = nullptr;   // headings of the current call (<= 16)
+  double2* d_pinc = nullptr;  // incident pressure (ntot x nbeta)
   double* d_pts = nullptr;  // vertex x, y (np each) for the extent reduction
   double* d_ext = nullptr;  // per-workgroup maxima of the extent reduction
   // resident geometry and tables
@@ -374,33 +386,26 @@ static int stage_nds(nemoh_core* c, int nint, const double* nds) {
   return NEMOH_OK;
 }
 
-static int finish_outputs(nemoh_core* c, int nrhs, double* pressure, int nint, const double* nds, double* forces) {
+// F(k, q) = -sum_c P(c, q) nds(k, c) on the device (P: ntot x nrhs, nds:
+// nint x ntot, both row-major), written to the caller column-major.
+static int download_forces(nemoh_core* c, const double2* P, const double* d_nds, int nint, int nrhs, double* out) {
   const size_t nt = (size_t)c->nfa * c->nb;
-  if (forces && nint > 0) {
-    if (!nds) return fail(c, NEMOH_ERR_ARG, "nds is NULL");
-    if (int r = stage_nds(c, nint, nds)) return r;
-    if (int r = dalloc(c, &c->d_force, (size_t)nint * nrhs)) return r;
-    launch_force(c->pres, c->d_nds, nint, (int)nt, nrhs, c->d_force, c->st);
-    HIPCHK(c, hipGetLastError());
-    std::vector<double> f(2 * (size_t)nint * nrhs);
-    HIPCHK(c, hipMemcpyAsync(f.data(), c->d_force, f.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    for (int k = 0; k < nint; ++k)  // device (nint x nrhs) row-major -> caller column-major
-      for (int q = 0; q < nrhs; ++q) {
-        forces[2 * (k + (size_t)nint * q)] = f[2 * (k * nrhs + q)];
-        forces[2 * (k + (size_t)nint * q) + 1] = f[2 * (k * nrhs + q) + 1];
-      }
-  }
-  if (pressure) {
-    std::vector<double> p(2 * nt * nrhs);
-    HIPCHK(c, hipMemcpyAsync(p.data(), c->pres, p.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    for (size_t i = 0; i < nt; ++i)
-      for (int q = 0; q < nrhs; ++q) {
-        pressure[2 * (i + nt * q)] = p[2 * (i * nrhs + q)];
-        pressure[2 * (i + nt * q) + 1] = p[2 * (i * nrhs + q) + 1];
-      }
-  }
+  if (int r = dalloc(c, &c->d_force, (size_t)nint * nrhs)) return r;
+  launch_force(P, d_nds, nint, (int)nt, nrhs, c->d_force, c->st);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> f(2 * (size_t)nint * nrhs);
+  HIPCHK(c, hipMemcpyAsync(f.data(), c->d_force, f.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (int k = 0; k < nint; ++k)  // device (nint x nrhs) row-major -> caller column-major
+    for (int q = 0; q < nrhs; ++q) {
+      out[2 * (k + (size_t)nint * q)] = f[2 * (k * nrhs + q)];
+      out[2 * (k + (size_t)nint * q) + 1] = f[2 * (k * nrhs + q) + 1];
+    }
+  return NEMOH_OK;
+}
+
+// End of a solve call: wait, report a pivot or wait failure, timing stats.
+static int finish_call(nemoh_core* c) {
   HIPCHK(c, hipStreamSynchronize(c->st));
   if (int r = check_info(c)) return r;
   if (c->timing) {
@@ -435,6 +440,51 @@ static int finish_outputs(nemoh_core* c, int nrhs, double* pressure, int nint, c
     HIPCHK(c, hipEventElapsedTime(&t, c->ev[c->stats.cache_hit ? 2 : 0], c->ev[3]));
     c->stats.ms_total = t;
   }
+  return NEMOH_OK;
+}
+
+static int finish_outputs(nemoh_core* c, int nrhs, double* pressure, int nint, const double* nds, double* forces) {
+  const size_t nt = (size_t)c->nfa * c->nb;
+  if (forces && nint > 0) {
+    if (!nds) return fail(c, NEMOH_ERR_ARG, "nds is NULL");
+    if (int r = stage_nds(c, nint, nds)) return r;
+    if (int r = download_forces(c, c->pres, c->d_nds, nint, nrhs, forces)) return r;
+  }
+  if (pressure) {
+    std::vector<double> p(2 * nt * nrhs);
+    HIPCHK(c, hipMemcpyAsync(p.data(), c->pres, p.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < nt; ++i)
+      for (int q = 0; q < nrhs; ++q) {
+        pressure[2 * (i + nt * q)] = p[2 * (i * nrhs + q)];
+        pressure[2 * (i + nt * q) + 1] = p[2 * (i * nrhs + q) + 1];
+      }
+  }
+  return finish_call(c);
+}
+
+// Body conditions of one frequency into c->V (ntot x (nbeta + nrad),
+// headings first) and the incident pressures into c->d_pinc.
+static int generate_waves(nemoh_core* c, double omega, int nbeta, const double* beta) {
+  if (!c->dofs_set) return fail(c, NEMOH_ERR_ARG, "nemoh_core_set_dofs has not been called for this mesh");
+  const int nrhs = nbeta + c->nrad;
+  if (nbeta < 0 || nrhs < 1 || nrhs > kMaxRhs || (nbeta > 0 && !beta))
+    return fail(c, NEMOH_ERR_ARG, "need 1 <= headings + radiation dofs <= 16");
+  if (!(std::fabs(omega) >= 1.E-4)) return fail(c, NEMOH_ERR_OMEGA, "ABS(WR) < 1.E-4");
+  const double k = host::incident_wavenumber(omega, c->depth, c->g);
+  if (std::isnan(k)) return fail(c, NEMOH_ERR_WAVENUMBER, "Error: unable to find the wavenumber");
+  if (int r = ensure_rhs(c, nrhs)) return r;
+  const int n = c->nfa, nt = n * c->nb;
+  if (nbeta > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->d_beta, beta, nbeta * sizeof(double), hipMemcpyHostToDevice, c->st));
+    launch_diffraction(c->d_pg, k, omega, c->g, c->rho, c->depth, c->xeff, c->yeff, c->d_beta, nbeta, n, nt, c->V,
+                       nrhs, c->d_pinc, c->st);
+  }
+  launch_radiation(c->d_pg, c->d_body, reinterpret_cast<const Dof*>(c->d_dofs), c->nrad, n, nt, c->V, nrhs, nbeta,
+                   c->st);
+  HIPCHK(c, hipGetLastError());
+  // the headings are read from host memory by the copy above before return
+  HIPCHK(c, hipStreamSynchronize(c->st));
   return NEMOH_OK;
 }
 
@@ -508,6 +558,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   c->depth = depth;
   c->nq_state = 0;
   c->cap = 0;
+  c->dofs_set = false;  // body conditions belong to the previous mesh
   c->stats = nemoh_stats{};
   c->upd_blocks = 0;
   const double* d = c->d_geo;
@@ -637,6 +688,12 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->d_force);
   release(c->d_pts);
   release(c->d_ext);
+  release(c->d_pg);
+  release(c->d_body);
+  release(c->d_dofs);
+  release(c->d_fnds);
+  release(c->d_beta);
+  release(c->d_pinc);
   for (int b = 0; b < 2; ++b) {
     release(c->D[b]);
     release(c->S[b]);
@@ -691,6 +748,108 @@ int nemoh_core_get_stats(const nemoh_core* c, nemoh_stats* out) {
   if (!c || !out) return NEMOH_ERR_ARG;
   *out = c->stats;
   return NEMOH_OK;
+}
+
+int nemoh_core_set_dofs(nemoh_core* c, const int* body_of_panel, int nrad, const nemoh_dof* rad, int nforce,
+                        const nemoh_dof* force, double xeff, double yeff) {
+  static_assert(sizeof(Dof) == sizeof(nemoh_dof) && offsetof(Dof, dir) == offsetof(nemoh_dof, dir) &&
+                    offsetof(Dof, axis) == offsetof(nemoh_dof, axis),
+                "Dof mirrors nemoh_dof");
+  if (!c) return NEMOH_ERR_ARG;
+  c->dofs_set = false;
+  if (nrad < 0 || nrad > kMaxRhs || nforce < 0 || (nrad > 0 && !rad) || (nforce > 0 && !force))
+    return fail(c, NEMOH_ERR_ARG, "bad numbers of degrees of freedom");
+  std::vector<nemoh_dof> dofs;
+  if (nrad > 0) dofs.assign(rad, rad + nrad);
+  if (nforce > 0) dofs.insert(dofs.end(), force, force + nforce);
+  for (const nemoh_dof& d : dofs)
+    if ((d.kind != 1 && d.kind != 2) || d.body < 1)
+      return fail(c, NEMOH_ERR_ARG, "a degree of freedom needs kind 1 (translation) or 2 (rotation) and body >= 1");
+  const int n = c->nfa, nt = n * c->nb;
+  std::vector<int> body(n, 1);
+  if (body_of_panel) body.assign(body_of_panel, body_of_panel + n);
+  int r = 0;
+  if ((r = dalloc(c, &c->d_pg, 7 * (size_t)n)) || (r = dalloc(c, &c->d_body, (size_t)n)) ||
+      (r = dalloc(c, &c->d_dofs, std::max<size_t>(1, dofs.size()))) ||
+      (r = dalloc(c, &c->d_fnds, std::max<size_t>(1, (size_t)nforce * nt))) || (r = dalloc(c, &c->d_beta, kMaxRhs)) ||
+      (r = dalloc(c, &c->d_pinc, (size_t)nt * kMaxRhs)))
+    return r;
+  HIPCHK(c, hipMemcpyAsync(c->d_body, body.data(), n * sizeof(int), hipMemcpyHostToDevice, c->st));
+  if (!dofs.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_dofs, dofs.data(), dofs.size() * sizeof(nemoh_dof), hipMemcpyHostToDevice, c->st));
+  launch_panel_props(c->G, c->d_pg, c->st);
+  launch_nds(c->d_pg, c->d_body, reinterpret_cast<const Dof*>(c->d_dofs) + nrad, nforce, n, nt, c->d_fnds, c->st);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> area(n);
+  HIPCHK(c, hipMemcpyAsync(area.data(), c->d_pg + 6 * (size_t)n, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (int i = 0; i < n; ++i)
+    if (!(area[i] >= 1.0E-07)) {  // preProcessor/Mesh.f90:222-225
+      char buf[96];
+      std::snprintf(buf, sizeof buf, "Error: surface of panel %d is too small (%.7e)", i + 1, area[i]);
+      return fail(c, NEMOH_ERR_PANEL, buf);
+    }
+  c->nrad = nrad;
+  c->nforce = nforce;
+  c->xeff = xeff;
+  c->yeff = yeff;
+  c->dofs_set = true;
+  return NEMOH_OK;
+}
+
+int nemoh_core_body_conditions(nemoh_core* c, double omega, int nbeta, const double* beta, double* nvel,
+                               double* pinc, double* nds) {
+  if (!c) return NEMOH_ERR_ARG;
+  if (int r = generate_waves(c, omega, nbeta, beta)) return r;
+  const int nrhs = nbeta + c->nrad;
+  const size_t nt = (size_t)c->nfa * c->nb;
+  auto colmajor = [&](const double2* d, size_t rows, int cols, double* out) -> int {
+    std::vector<double> h(2 * rows * cols);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < rows; ++i)
+      for (int q = 0; q < cols; ++q) {
+        out[2 * (i + rows * q)] = h[2 * (i * cols + q)];
+        out[2 * (i + rows * q) + 1] = h[2 * (i * cols + q) + 1];
+      }
+    return NEMOH_OK;
+  };
+  if (nvel)
+    if (int r = colmajor(c->V, nt, nrhs, nvel)) return r;
+  if (pinc && nbeta > 0)
+    if (int r = colmajor(c->d_pinc, nt, nbeta, pinc)) return r;
+  if (nds && c->nforce > 0) {
+    std::vector<double> h((size_t)c->nforce * nt);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_fnds, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int k = 0; k < c->nforce; ++k)
+      for (size_t i = 0; i < nt; ++i) nds[k + (size_t)c->nforce * i] = h[k * nt + i];
+  }
+  return NEMOH_OK;
+}
+
+int nemoh_core_solve_waves(nemoh_core* c, double omega, int nbeta, const double* beta, double* forces, double* fk) {
+  if (!c) return NEMOH_ERR_ARG;
+  if (!(std::fabs(omega) >= 1.E-4)) return fail(c, NEMOH_ERR_OMEGA, "ABS(WR) < 1.E-4");
+  const double pi = 4.0 * std::atan(1.0);
+  const double period = 2.0 * pi / omega;  // T as the Solver's main program forms it
+  nemoh_freq f;
+  if (int r = nemoh_core_freq_setup(c, period, &f)) return r;
+  bool fd;
+  double w;
+  if (int r = build_params(c, period, f.amh, f.nexp, f.ambda, f.ar, f.nq, f.qq, f.cq, &fd, &w)) return r;
+  if (int r = generate_waves(c, omega, nbeta, beta)) return r;
+  const int nrhs = nbeta + c->nrad;
+  if (int r = run_pipeline(c, w, nrhs)) return r;
+  if (forces && c->nforce > 0)
+    if (int r = download_forces(c, c->pres, c->d_fnds, c->nforce, nrhs, forces)) return r;
+  if (fk && c->nforce > 0 && nbeta > 0)
+    if (int r = download_forces(c, c->d_pinc, c->d_fnds, c->nforce, nbeta, fk)) return r;
+  return finish_call(c);
+}
+
+double nemoh_core_incident_wavenumber(double w, double depth, double g) {
+  return host::incident_wavenumber(w, depth, g);
 }
 
 int nemoh_core_get_extent(const nemoh_core* c, double* gm) {

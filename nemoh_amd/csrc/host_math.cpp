@@ -114,6 +114,36 @@ int theta_rule(double akk, double* qq, double* cq) {
   return n;
 }
 
+// Wavenumber (Common/Environment.f90:45-84): deep water (w^2/g) when k h is
+// outside (0, 20], else a bisection of x tanh x = k h to a relative width of
+// 1e-6 (its iteration counter is reset before the second loop, as there).
+double incident_wavenumber(double w, double depth, double g) {
+  double k = w * w / g;
+  const double kh = k * depth;
+  if (kh <= 20.0 && kh > 0.0) {
+    double xg = 0.0, xd = kh;
+    int nite = 0;
+    while (nite < 10000 && (kh - xg * std::tanh(xg)) * (kh - xd * std::tanh(xd)) > 0.0) {
+      xg = xd;
+      xd = 2.0 * xd;
+      ++nite;
+    }
+    nite = 0;
+    double xc = 0.5 * (xd + xg);
+    while (nite < 10000 && std::fabs(xd - xg) / std::fabs(xc) >= 1.0e-06) {
+      xc = 0.5 * (xd + xg);
+      if ((kh - xg * std::tanh(xg)) * (kh - xc * std::tanh(xc)) > 0.0)
+        xg = xc;
+      else
+        xd = xc;
+      ++nite;
+    }
+    if (nite >= 10000) return std::nan("");
+    k = xc / depth;
+  }
+  return k;
+}
+
 double x0(double ak) {
   const double eps = 5.E-6;
   double xi = 0.0, xs = 0.0;

@@ -17,6 +17,8 @@ int prepare_mesh(int np, int nfa, int nsym, const double* x, const double* y, co
 int theta_rule(double akk, double* qq, double* cq);
 // Root of x tanh x = ak by the reference's bisection.
 double x0(double ak);
+// The preProcessor's wave number k(w, depth, g) (NaN if its bisection fails).
+double incident_wavenumber(double w, double depth, double g);
 // Exponential fit of the finite-depth kernel; returns 0 or an error code.
 int prony_fit(double akh, double amh, int* nexp, double* ambda, double* ar);
 

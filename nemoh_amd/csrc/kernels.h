@@ -60,6 +60,26 @@ void launch_force(const double2* pres, const double* nds, int nint, int ntot, in
 void launch_f64_to_f32(const double2* a, float2* b, size_t count, double scale, hipStream_t st);
 void launch_f32_to_f64(const float2* a, double2* b, size_t count, double scale, int accumulate, hipStream_t st);
 void launch_absmax(const double2* a, size_t count, double* out, int nblocks, hipStream_t st);
+// Degree of freedom of the preProcessor's radiation / force cases: body
+// (1-based), kind 1 = translation along dir, 2 = rotation about the axis
+// through `axis` along dir (same layout as nemoh_dof of the C ABI).
+struct Dof {
+  int body, kind;
+  double dir[3], axis[3];
+};
+// preProcessor panel geometry pg = [xm(3n) | normal(3n) | area(n)]
+void launch_panel_props(const Geometry& G, double* pg, hipStream_t st);
+// radiation right-hand sides into out[c * ld + col0 + q], q < nrad
+void launch_radiation(const double* pg, const int* body, const Dof* dofs, int nrad, int n, int ntot, double2* out,
+                      int ld, int col0, hipStream_t st);
+// force weights, row-major (nforce x ntot)
+void launch_nds(const double* pg, const int* body, const Dof* dofs, int nforce, int n, int ntot, double* nds,
+                hipStream_t st);
+// incident-wave right-hand sides into out[c * ld + b] and pressures into
+// pinc[c * nbeta + b], b < nbeta (beta in radians)
+void launch_diffraction(const double* pg, double k, double w, double g, double rho, double depth, double xeff,
+                        double yeff, const double* beta, int nbeta, int n, int ntot, double2* out, int ld,
+                        double2* pinc, hipStream_t st);
 // out[ntile*(ntile+1)/2] (ntile = ceil(np/256)): per-workgroup maxima of the
 // squared horizontal vertex distances (GM^2)
 void launch_extent(const double* x, const double* y, int np, int nsym, double* out, hipStream_t st);

@@ -11,6 +11,7 @@
 !   crek                                  -> XR, XZ, APD1X..APD2Z    NEMOH.f90:179
 !   cinti  n  then n values of AKK        -> N, module NQ, QQ, CQ    COMPUTE_GREEN_INFD.f90:505
 !   x0     n  then n values of AK         -> X0(AK)                  COMPUTE_GREEN_FD.f90:1635
+!   wavenumber n then n lines "w depth g" -> the preProcessor's k   Common/Environment.f90:45-84
 !   lisc   n  then n lines "akh amh"      -> NEXP, AMBDA, AR         COMPUTE_GREEN_FD.f90:1007
 !   mesh   np nfa nsym depth, np lines "x y z", nfa lines "m1 m2 m3 m4"
 !                                          -> sets COM_VAR, runs PRE_PROC_MESH, CREK
@@ -29,6 +30,7 @@ PROGRAM ref_harness
   USE PREPARE_MESH
   USE MMesh
   USE M_SOLVER, ONLY: GAUSSZ
+  USE MEnvironment, ONLY: TEnvironment, Wavenumber
   IMPLICIT NONE
   CHARACTER(LEN=16) :: cmd
   INTEGER :: n, i, j, k, nq_out, nexp, isp, ifp, ios, iperiod
@@ -44,6 +46,7 @@ PROGRAM ref_harness
   REAL :: tpairs, tgauss, acc
   COMPLEX, ALLOCATABLE :: zmat(:,:)
   LOGICAL :: fd
+  TYPE(TEnvironment) :: env
 
   DO
     READ(*,*,IOSTAT=ios) cmd
@@ -80,6 +83,12 @@ PROGRAM ref_harness
       DO i = 1, n
         READ(*,*) ak
         WRITE(*,'(2ES26.17E3)') ak, X0(ak)
+      END DO
+    CASE ('wavenumber')
+      READ(*,*) n
+      DO i = 1, n
+        READ(*,*) w, env%Depth, env%G
+        WRITE(*,'(4ES26.17E3)') w, env%Depth, env%G, Wavenumber(w, env)
       END DO
     CASE ('lisc')
       READ(*,*) n

@@ -29,3 +29,27 @@ def case_geometry(d):
     geo = dict(zip(keys, d["geo"].T))
     geo["zer"] = float(d["zer"])
     return geo
+
+
+def write_synthetic_case(work, name):
+    """Inputs of the synthetic golden cases (the reference preProcessor and
+    Solver read them), with the exact mesh they were made from."""
+    from nemoh_amd import io as nio
+    from nemoh_amd import mesh as nmesh
+    (work / "mesh").mkdir(parents=True)
+    (work / "results").mkdir()
+    (work / "ID.dat").write_text("1\n.\n")
+    (work / "input.txt").write_text("--- Calculation parameters ---\n0\t! Indiq_solver\n")
+    if name == "cyl512":
+        m = nmesh.cylinder(10.0, 5.0, 32, 8, 8, half=True)
+        m.write_dat(work / "cyl.dat")
+        body = nio.BodySpec("cyl.dat", m.npoints, m.npanels, nio.six_dof(), nio.six_dof())
+        nio.write_nemoh_cal(work / "Nemoh.cal", 1025.0, 9.81, 0.0, [body], 20, 0.1, 2.0, 1, 0.0, 0.0)
+        depth = 0.0
+    else:
+        m = nmesh.box(100.0, 20.0, 10.0, 18, 4, 2)
+        m.write_dat(work / "barge.dat")
+        body = nio.BodySpec("barge.dat", m.npoints, m.npanels, nio.six_dof(), nio.six_dof())
+        nio.write_nemoh_cal(work / "Nemoh.cal", 1025.0, 9.81, 50.0, [body], 3, 0.2, 2.0, 3, 0.0, 90.0)
+        depth = 50.0
+    return m, depth

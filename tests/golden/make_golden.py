@@ -329,6 +329,19 @@ def pair_fixtures() -> None:
     print("pair fixtures written")
 
 
+def wavenumber_fixture() -> None:
+    """The preProcessor's own wave number (Common/Environment.f90:45-84) over
+    frequencies and depths that cover its deep-water branch (k h > 20 or
+    h = 0) and the bisection."""
+    rows = [(w, h, 9.81) for h in (0.0, 2.0, 10.0, 50.0, 200.0, 1000.0) for w in np.linspace(0.05, 3.0, 40)]
+    rows += [(0.8, 50.0, 9.80665), (1.3, 12.5, 9.81), (2.0, 50.0, 9.81)]
+    lines = harness("wavenumber\n%d\n" % len(rows) + "".join(f"{a:.17e} {b:.17e} {c:.17e}\n" for a, b, c in rows)
+                    + "end\n")
+    a = np.array([[float(v) for v in ln.split()] for ln in lines])
+    np.savez_compressed(OUT / "fn_wavenumber.npz", w=a[:, 0], depth=a[:, 1], g=a[:, 2], k=a[:, 3])
+    print(f"wavenumber: {a.shape[0]} cases")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -339,6 +352,8 @@ def main() -> None:
         tmp = Path(td)
         if a.only in ("", "fn"):
             fn_fixtures()
+        if a.only in ("", "fn", "wavenumber"):
+            wavenumber_fixture()
         if a.only in ("", "pairs"):
             pair_fixtures()
         if a.only in ("", "cases"):

@@ -69,3 +69,11 @@ def test_create_without_device_fails_loudly():
         core.Core(d["vertices"], d["panels"], 0, 0.0, geometry=case_geometry(d),
                   tables=(t["xr"], t["xz"], (t["apd1x"], t["apd1z"], t["apd2x"], t["apd2z"])))
     assert e.value.code in (core.ERR_HIP, core.ERR_ARG)
+
+
+def test_incident_wavenumber_matches_reference_bitwise():
+    """The preProcessor's own wave number (Common/Environment.f90:45-84),
+    deep-water branch and bisection, against the reference build."""
+    d = load_golden("fn_wavenumber.npz")
+    got = [core.incident_wavenumber(w, h, g) for w, h, g in zip(d["w"], d["depth"], d["g"])]
+    assert np.array_equal(np.array(got), d["k"])

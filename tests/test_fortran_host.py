@@ -16,9 +16,8 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, load_golden
+from conftest import ROOT, load_golden, write_synthetic_case
 from nemoh_amd import io as nio
-from nemoh_amd import mesh as nmesh
 
 REF = ROOT / "oracle" / "_ref"
 FLANG = "/opt/rocm/lib/llvm/bin/flang"
@@ -38,24 +37,6 @@ def test_binding_module_compiles_and_names_exports(tmp_path):
     assert bound and all(re.search(rf"\b{n}\s*\(", header) for n in bound)
 
 
-def _case(work, name):
-    """The synthetic case's inputs as the golden fixtures were made."""
-    (work / "mesh").mkdir(parents=True)
-    (work / "results").mkdir()
-    (work / "ID.dat").write_text("1\n.\n")
-    (work / "input.txt").write_text("--- Calculation parameters ---\n0\t! Indiq_solver\n")
-    if name == "cyl512":
-        m = nmesh.cylinder(10.0, 5.0, 32, 8, 8, half=True)
-        m.write_dat(work / "cyl.dat")
-        body = nio.BodySpec("cyl.dat", m.npoints, m.npanels, nio.six_dof(), nio.six_dof())
-        nio.write_nemoh_cal(work / "Nemoh.cal", 1025.0, 9.81, 0.0, [body], 20, 0.1, 2.0, 1, 0.0, 0.0)
-    else:
-        m = nmesh.box(100.0, 20.0, 10.0, 18, 4, 2)
-        m.write_dat(work / "barge.dat")
-        body = nio.BodySpec("barge.dat", m.npoints, m.npanels, nio.six_dof(), nio.six_dof())
-        nio.write_nemoh_cal(work / "Nemoh.cal", 1025.0, 9.81, 50.0, [body], 3, 0.2, 2.0, 3, 0.0, 90.0)
-
-
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["cyl512", "barge_fd"])
 @pytest.mark.parametrize("precision", ["FP64", "MIXED"])
@@ -64,7 +45,7 @@ def test_reference_solver_through_core(tmp_path, name, precision):
     if not (exe.exists() and pre.exists()):
         pytest.skip("reference host not built (make -f oracle/Makefile.ref)")
     work = tmp_path / name
-    _case(work, name)
+    write_synthetic_case(work, name)
     r = subprocess.run([str(pre)], cwd=work, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-1000:] + r.stderr[-1000:]
     env = dict(os.environ, NEMOH_CORE_PRECISION=precision)
