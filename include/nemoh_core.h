@@ -218,7 +218,7 @@ int nemoh_core_get_extent(const nemoh_core *ctx, double *gm);
  * nemoh_core_create.  submit copies its inputs and returns at once with an
  * id; wait blocks until that request is done, writes the forces and
  * timing_ms[3] = {submit to done, submit to first start, first start to
- * done} (either may be NULL), and returns the request's status (the first
+ * done}, and returns the request's status (the first
  * error of its frequencies; others are unaffected).  Calls are thread-safe;
  * destroy finishes the queued work first. */
 typedef struct nemoh_server nemoh_server;
@@ -230,7 +230,21 @@ int nemoh_server_submit(nemoh_server *srv, int np, int nfa, int nsym, const doub
                         double depth, double rho, double g, int precision, int nfreq,
                         const double *periods, int nrhs, const double *nvel, int nint,
                         const double *nds, long long *id);
-int nemoh_server_wait(nemoh_server *srv, long long id, double *forces, double *timing_ms);
+/* A request whose problems are generated on the device, as
+ * nemoh_core_solve_waves does: nfreq frequencies omegas (rad/s), nbeta
+ * headings betas (radians), radiation and force degrees of freedom,
+ * body_of_panel(NFA) (NULL: all body 1), wave reference point xeff, yeff.
+ * Its reply: forces(nforce, nbeta + nrad, nfreq) and fk(nforce, nbeta,
+ * nfreq), the Froude-Krylov forces. */
+int nemoh_server_submit_waves(nemoh_server *srv, int np, int nfa, int nsym, const double *x,
+                              const double *y, const double *z, const int *m1, const int *m2,
+                              const int *m3, const int *m4, double depth, double rho, double g,
+                              int precision, int nfreq, const double *omegas, int nbeta,
+                              const double *betas, const int *body_of_panel, int nrad,
+                              const nemoh_dof *rad, int nforce, const nemoh_dof *force, double xeff,
+                              double yeff, long long *id);
+/* fk is written for wave requests only; forces, fk and timing_ms may be NULL. */
+int nemoh_server_wait(nemoh_server *srv, long long id, double *forces, double *fk, double *timing_ms);
 const char *nemoh_server_last_error(nemoh_server *srv);
 void nemoh_server_destroy(nemoh_server *srv);
 

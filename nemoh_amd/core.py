@@ -23,7 +23,7 @@ EXPORTS = (
     "nemoh_core_wavenumber", "nemoh_core_theta_rule", "nemoh_core_prony", "nemoh_core_pairs",
     "nemoh_core_mfma_probe", "nemoh_core_get_extent", "nemoh_core_set_dofs", "nemoh_core_body_conditions",
     "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_server_create", "nemoh_server_submit",
-    "nemoh_server_wait", "nemoh_server_last_error", "nemoh_server_destroy",
+    "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_last_error", "nemoh_server_destroy",
 )
 
 
@@ -110,7 +110,10 @@ def load():
     L.nemoh_server_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_double] + [_dp] * 6
     L.nemoh_server_submit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [_dp] * 3 + [_ip] * 4 + \
         [C.c_double] * 3 + [C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.POINTER(C.c_longlong)]
-    L.nemoh_server_wait.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp]
+    L.nemoh_server_submit_waves.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [_dp] * 3 + [_ip] * 4 + \
+        [C.c_double] * 3 + [C.c_int, C.c_int, _dp, C.c_int, _dp, _ip, C.c_int, C.POINTER(Dof), C.c_int,
+                            C.POINTER(Dof), C.c_double, C.c_double, C.POINTER(C.c_longlong)]
+    L.nemoh_server_wait.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp, _dp]
     L.nemoh_server_last_error.argtypes = [C.c_void_p]
     L.nemoh_server_last_error.restype = C.c_char_p
     L.nemoh_server_destroy.argtypes = [C.c_void_p]

@@ -21,6 +21,7 @@
 // session over the request's frequency list would see it.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstdio>
 #include <cstring>
@@ -49,6 +50,14 @@ struct Request {
   std::vector<double> x, y, z, periods, nvel, nds;
   std::vector<int> m1, m2, m3, m4;
   double depth = 0, rho = 1025, g = 9.81;
+  // wave requests: problems generated on the device (nemoh_core_solve_waves)
+  bool waves = false;
+  int nbeta = 0;
+  std::vector<double> omegas, betas;
+  std::vector<int> body;  // body of each panel (empty: all body 1)
+  std::vector<nemoh_dof> rad, force;
+  double xeff = 0, yeff = 0;
+  std::vector<double> fk;  // (nint, nbeta) per frequency
   double budget = 0;  // bytes charged per in-flight frequency
   // set-up shared by the frequencies, made once by the first worker
   std::mutex pm;
@@ -82,7 +91,8 @@ struct Request {
 
 struct Worker {
   nemoh_core* h = nullptr;
-  long long bound = -1;  // request the handle's mesh belongs to
+  long long bound = -1;     // request the handle's mesh belongs to
+  long long dofs_for = -1;  // request whose degrees of freedom the handle holds
 };
 
 }  // namespace
@@ -133,6 +143,7 @@ int bind(nemoh_server* s, Worker& w, Request& r) {
     }
   }
   w.bound = r.id;
+  w.dofs_for = -1;  // binding clears the handle's body-condition set-up
   return NEMOH_OK;
 }
 
@@ -171,12 +182,23 @@ void run_item(nemoh_server* s, Worker& w, Request& r, int f) {
   const size_t per_f = 2 * ntot * r.nrhs, per_force = 2 * (size_t)r.nint * r.nrhs;
   int e = nemoh_core_set_precision(w.h, r.precision, 0.0, 0);
   nemoh::core_set_nq_state(w.h, r.nq_before[f]);
-  nemoh_freq fs;
-  if (!e) e = nemoh_core_freq_setup(w.h, r.periods[f], &fs);
-  if (!e)
-    e = nemoh_core_solve_multi(w.h, r.periods[f], fs.amh, fs.nexp, fs.ambda, fs.ar, fs.nq, fs.qq, fs.cq, r.nrhs,
-                               r.nvel.data() + f * per_f, nullptr, r.nint, r.nds.data(),
-                               r.forces.data() + f * per_force);
+  if (r.waves) {
+    if (!e && w.dofs_for != r.id) {
+      e = nemoh_core_set_dofs(w.h, r.body.empty() ? nullptr : r.body.data(), (int)r.rad.size(), r.rad.data(),
+                              (int)r.force.size(), r.force.data(), r.xeff, r.yeff);
+      if (!e) w.dofs_for = r.id;
+    }
+    if (!e)
+      e = nemoh_core_solve_waves(w.h, r.omegas[f], r.nbeta, r.betas.data(), r.forces.data() + f * per_force,
+                                 r.nbeta > 0 ? r.fk.data() + f * 2 * (size_t)r.nint * r.nbeta : nullptr);
+  } else {
+    nemoh_freq fs;
+    if (!e) e = nemoh_core_freq_setup(w.h, r.periods[f], &fs);
+    if (!e)
+      e = nemoh_core_solve_multi(w.h, r.periods[f], fs.amh, fs.nexp, fs.ambda, fs.ar, fs.nq, fs.qq, fs.cq, r.nrhs,
+                                 r.nvel.data() + f * per_f, nullptr, r.nint, r.nds.data(),
+                                 r.forces.data() + f * per_force);
+  }
   if (e) {
     char buf[64];
     std::snprintf(buf, sizeof buf, "frequency %d: ", f);
@@ -227,6 +249,54 @@ void worker_loop(nemoh_server* s) {
 
 }  // namespace
 
+namespace {
+
+bool mesh_ok(int np, int nfa, int nsym, const double* x, const double* y, const double* z, const int* m1,
+             const int* m2, const int* m3, const int* m4, double depth, int precision) {
+  return np > 0 && nfa > 0 && (nsym == 0 || nsym == 1) && x && y && z && m1 && m2 && m3 && m4 && depth >= 0 &&
+         (precision == NEMOH_FP64 || precision == NEMOH_MIXED);
+}
+
+std::shared_ptr<Request> new_request(int np, int nfa, int nsym, const double* x, const double* y, const double* z,
+                                     const int* m1, const int* m2, const int* m3, const int* m4, double depth,
+                                     double rho, double g, int precision, int nfreq) {
+  auto r = std::make_shared<Request>();
+  r->np = np;
+  r->nfa = nfa;
+  r->nsym = nsym;
+  r->x.assign(x, x + np);
+  r->y.assign(y, y + np);
+  r->z.assign(z, z + np);
+  r->m1.assign(m1, m1 + nfa);
+  r->m2.assign(m2, m2 + nfa);
+  r->m3.assign(m3, m3 + nfa);
+  r->m4.assign(m4, m4 + nfa);
+  r->depth = depth;
+  r->rho = rho;
+  r->g = g;
+  r->precision = precision;
+  r->nfreq = nfreq;
+  const double ntot = (double)((size_t)nfa << nsym);
+  r->budget = 8.0 * ntot * ntot;
+  return r;
+}
+
+int enqueue(nemoh_server* s, const std::shared_ptr<Request>& r, long long* id) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return NEMOH_ERR_ARG;
+    r->id = s->next_id++;
+    r->t_submit = now_ms();
+    s->table[r->id] = r;
+    s->queue.push_back(r);
+    *id = r->id;
+  }
+  s->cv_work.notify_all();
+  return NEMOH_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int nemoh_server_create(nemoh_server** out, int workers, double budget_bytes, const double* xr, const double* xz,
@@ -254,48 +324,50 @@ int nemoh_server_submit(nemoh_server* s, int np, int nfa, int nsym, const double
                         const int* m1, const int* m2, const int* m3, const int* m4, double depth, double rho,
                         double g, int precision, int nfreq, const double* periods, int nrhs, const double* nvel,
                         int nint, const double* nds, long long* id) {
-  if (!s || !id || np <= 0 || nfa <= 0 || (nsym != 0 && nsym != 1) || !x || !y || !z || !m1 || !m2 || !m3 || !m4 ||
-      nfreq <= 0 || !periods || nrhs <= 0 || nrhs > 16 || !nvel || nint <= 0 || !nds || depth < 0 ||
-      (precision != NEMOH_FP64 && precision != NEMOH_MIXED))
+  if (!s || !id || !mesh_ok(np, nfa, nsym, x, y, z, m1, m2, m3, m4, depth, precision) || nfreq <= 0 || !periods ||
+      nrhs <= 0 || nrhs > 16 || !nvel || nint <= 0 || !nds)
     return NEMOH_ERR_ARG;
-  auto r = std::make_shared<Request>();
+  auto r = new_request(np, nfa, nsym, x, y, z, m1, m2, m3, m4, depth, rho, g, precision, nfreq);
   const size_t ntot = (size_t)nfa << nsym;
-  r->np = np;
-  r->nfa = nfa;
-  r->nsym = nsym;
-  r->x.assign(x, x + np);
-  r->y.assign(y, y + np);
-  r->z.assign(z, z + np);
-  r->m1.assign(m1, m1 + nfa);
-  r->m2.assign(m2, m2 + nfa);
-  r->m3.assign(m3, m3 + nfa);
-  r->m4.assign(m4, m4 + nfa);
-  r->depth = depth;
-  r->rho = rho;
-  r->g = g;
-  r->precision = precision;
-  r->nfreq = nfreq;
   r->periods.assign(periods, periods + nfreq);
   r->nrhs = nrhs;
   r->nint = nint;
   r->nvel.assign(nvel, nvel + 2 * ntot * nrhs * nfreq);
   r->nds.assign(nds, nds + (size_t)nint * ntot);
   r->forces.assign(2 * (size_t)nint * nrhs * nfreq, 0.0);
-  r->budget = 8.0 * (double)ntot * (double)ntot;
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stop) return NEMOH_ERR_ARG;
-    r->id = s->next_id++;
-    r->t_submit = now_ms();
-    s->table[r->id] = r;
-    s->queue.push_back(r);
-    *id = r->id;
-  }
-  s->cv_work.notify_all();
-  return NEMOH_OK;
+  return enqueue(s, r, id);
 }
 
-int nemoh_server_wait(nemoh_server* s, long long id, double* forces, double* timing_ms) {
+int nemoh_server_submit_waves(nemoh_server* s, int np, int nfa, int nsym, const double* x, const double* y,
+                              const double* z, const int* m1, const int* m2, const int* m3, const int* m4,
+                              double depth, double rho, double g, int precision, int nfreq, const double* omegas,
+                              int nbeta, const double* betas, const int* body_of_panel, int nrad,
+                              const nemoh_dof* rad, int nforce, const nemoh_dof* force, double xeff, double yeff,
+                              long long* id) {
+  if (!s || !id || !mesh_ok(np, nfa, nsym, x, y, z, m1, m2, m3, m4, depth, precision) || nfreq <= 0 || !omegas ||
+      nbeta < 0 || nrad < 0 || nbeta + nrad < 1 || nbeta + nrad > 16 || (nbeta > 0 && !betas) ||
+      (nrad > 0 && !rad) || nforce <= 0 || !force)
+    return NEMOH_ERR_ARG;
+  auto r = new_request(np, nfa, nsym, x, y, z, m1, m2, m3, m4, depth, rho, g, precision, nfreq);
+  r->waves = true;
+  r->omegas.assign(omegas, omegas + nfreq);
+  const double pi = 4.0 * std::atan(1.0);
+  for (int f = 0; f < nfreq; ++f) r->periods.push_back(2.0 * pi / omegas[f]);  // as nemoh_core_solve_waves
+  r->nbeta = nbeta;
+  if (nbeta > 0) r->betas.assign(betas, betas + nbeta);
+  if (body_of_panel) r->body.assign(body_of_panel, body_of_panel + nfa);
+  if (nrad > 0) r->rad.assign(rad, rad + nrad);
+  r->force.assign(force, force + nforce);
+  r->xeff = xeff;
+  r->yeff = yeff;
+  r->nrhs = nbeta + nrad;
+  r->nint = nforce;
+  r->forces.assign(2 * (size_t)nforce * r->nrhs * nfreq, 0.0);
+  r->fk.assign(2 * (size_t)nforce * nbeta * nfreq, 0.0);
+  return enqueue(s, r, id);
+}
+
+int nemoh_server_wait(nemoh_server* s, long long id, double* forces, double* fk, double* timing_ms) {
   if (!s) return NEMOH_ERR_ARG;
   std::shared_ptr<Request> r;
   {
@@ -321,6 +393,7 @@ int nemoh_server_wait(nemoh_server* s, long long id, double* forces, double* tim
     return r->status;
   }
   if (forces) std::memcpy(forces, r->forces.data(), r->forces.size() * sizeof(double));
+  if (fk && !r->fk.empty()) std::memcpy(fk, r->fk.data(), r->fk.size() * sizeof(double));
   return NEMOH_OK;
 }
 

@@ -42,6 +42,33 @@ class Request:
         return self.omegas.size
 
 
+@dataclass
+class WaveRequest:
+    """A request whose problems are generated on the device: diffraction at
+    each heading then radiation for each degree of freedom, at every
+    frequency; forces on the force degrees of freedom (default: the
+    radiation ones) and the Froude-Krylov forces."""
+    mesh: nmesh.Mesh
+    depth: float
+    omegas: np.ndarray      # (nfreq,) rad/s
+    betas: np.ndarray       # (nbeta,) headings, radians
+    radiation: list         # bodycond.Mode
+    forces: list = None     # bodycond.Mode (None: the radiation ones)
+    body_of_panel: np.ndarray = None  # (npanels,) 1-based body of each panel
+    rho: float = 1025.0
+    g: float = 9.81
+    xeff: float = 0.0
+    yeff: float = 0.0
+
+    @property
+    def npanels(self) -> int:
+        return self.mesh.npanels * 2 ** self.mesh.isym
+
+    @property
+    def nfreq(self) -> int:
+        return self.omegas.size
+
+
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
@@ -63,7 +90,9 @@ class Server:
         self.h = h
         self._shapes = {}
 
-    def submit(self, req: Request, precision: int = core.MIXED) -> int:
+    def submit(self, req, precision: int = core.MIXED) -> int:
+        if isinstance(req, WaveRequest):
+            return self._submit_waves(req, precision)
         m = req.mesh
         v = _f64(m.vertices)
         pc = [np.ascontiguousarray(m.panels[:, k] + 1, dtype=np.int32) for k in range(4)]
@@ -83,20 +112,47 @@ class Server:
             C.byref(rid))
         if r:
             raise core.NemohError(r, "nemoh_server_submit: bad request")
-        self._shapes[rid.value] = (nfreq, nrhs, nint)
+        self._shapes[rid.value] = (nfreq, nrhs, nint, 0)
         return rid.value
 
-    def wait(self, rid: int):
+    def _submit_waves(self, req: WaveRequest, precision: int) -> int:
+        m = req.mesh
+        v = _f64(m.vertices)
+        pc = [np.ascontiguousarray(m.panels[:, k] + 1, dtype=np.int32) for k in range(4)]
+        om = _f64(req.omegas)
+        be = _f64(np.atleast_1d(req.betas)) if len(req.betas) else np.zeros(1)
+        nbeta = len(req.betas)
+        rad, nrad = core._dofs(req.radiation)
+        frc, nforce = core._dofs(req.radiation if req.forces is None else req.forces)
+        body = None if req.body_of_panel is None else np.ascontiguousarray(req.body_of_panel, dtype=np.int32)
+        rid = C.c_longlong()
+        r = core.load().nemoh_server_submit_waves(
+            self.h, v.shape[0], m.npanels, int(m.isym), core._p(_f64(v[:, 0])), core._p(_f64(v[:, 1])),
+            core._p(_f64(v[:, 2])), *[core._ip_(a) for a in pc], float(req.depth), float(req.rho), float(req.g),
+            int(precision), om.size, core._p(om), nbeta, core._p(be), None if body is None else core._ip_(body),
+            nrad, rad, nforce, frc, float(req.xeff), float(req.yeff), C.byref(rid))
+        if r:
+            raise core.NemohError(r, "nemoh_server_submit_waves: bad request")
+        self._shapes[rid.value] = (om.size, nbeta + nrad, nforce, nbeta)
+        return rid.value
+
+    def wait(self, rid: int, with_fk: bool = False):
         """Forces (nfreq, nint, nrhs) complex and the request's timings (ms):
         latency (submit to done), queue (submit to first start), service
-        (first start to done)."""
-        nfreq, nrhs, nint = self._shapes.pop(rid)
+        (first start to done).  with_fk: also the Froude-Krylov forces
+        (nfreq, nint, nbeta) of a wave request, as the middle element."""
+        nfreq, nrhs, nint, nbeta = self._shapes.pop(rid)
         out = np.zeros((nfreq, nrhs, nint), np.complex128)
+        fk = np.zeros((nfreq, max(nbeta, 1), nint), np.complex128)
         t = np.zeros(3)
-        r = core.load().nemoh_server_wait(self.h, C.c_longlong(rid), out.ctypes.data_as(core._dp), core._p(t))
+        r = core.load().nemoh_server_wait(self.h, C.c_longlong(rid), out.ctypes.data_as(core._dp),
+                                          fk.ctypes.data_as(core._dp), core._p(t))
         if r:
             raise core.NemohError(r, core.load().nemoh_server_last_error(self.h).decode())
-        return out.transpose(0, 2, 1), dict(zip(("latency", "queue", "service"), t.tolist()))
+        timing = dict(zip(("latency", "queue", "service"), t.tolist()))
+        if with_fk:
+            return out.transpose(0, 2, 1), fk[:, :nbeta].transpose(0, 2, 1), timing
+        return out.transpose(0, 2, 1), timing
 
     def close(self):
         if getattr(self, "h", None):
@@ -124,8 +180,10 @@ def _cylinder_for(npan: int, radius: float, draft: float, x0: float) -> nmesh.Me
     return nmesh.cylinder(radius, draft, n_theta, n_z, n_r, half=False, x0=x0)
 
 
-def two_cylinder_request(rng: np.random.Generator, n_range=(200, 16000), max_freq: int = 64,
-                         max_headings: int = 4, g: float = 9.81, rho: float = 1025.0) -> Request:
+def two_cylinder_parts(rng: np.random.Generator, n_range=(200, 16000), max_freq: int = 64,
+                       max_headings: int = 4):
+    """The serving workload's draw: (mesh, body_of_panel, 12 modes, omegas,
+    headings in radians)."""
     r = rng.uniform(2.0, 15.0)
     spacing = rng.uniform(3.0, 10.0) * r
     n_total = math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))
@@ -138,23 +196,42 @@ def two_cylinder_request(rng: np.random.Generator, n_range=(200, 16000), max_fre
     body = np.concatenate([np.ones(a.npanels, int), np.full(b.npanels, 2)])
     modes = bcm.six_dof(axis=(-spacing / 2, 0.0, 0.0), body=1) + bcm.six_dof(axis=(spacing / 2, 0.0, 0.0), body=2)
     omegas = np.linspace(0.3, 2.0, nf) if nf > 1 else np.array([0.8])
-    betas = [180.0 * k / nh for k in range(nh)]
+    betas = np.deg2rad([180.0 * k / nh for k in range(nh)])
+    return m, body, modes, omegas, betas
+
+
+def _arrays(m, body, modes, omegas, betas, depth, g, rho) -> Request:
     rad = np.stack([bcm.radiation(m, md, body) for md in modes], axis=1)
-    nvel = np.empty((nf, m.npanels, nh + len(modes)), np.complex128)
+    nh = len(betas)
+    nvel = np.empty((omegas.size, m.npanels * 2 ** m.isym, nh + len(modes)), np.complex128)
     for i, w in enumerate(omegas):
-        for j, bdeg in enumerate(betas):
-            nvel[i, :, j] = bcm.diffraction(m, w, np.deg2rad(bdeg), 0.0, g, rho)[0]
+        for j, b in enumerate(betas):
+            nvel[i, :, j] = bcm.diffraction(m, w, b, depth, g, rho)[0]
         nvel[i, :, nh:] = rad
-    return Request(m, 0.0, omegas, nvel, bcm.integration(m, modes, body), rho, g)
+    return Request(m, depth, omegas, nvel, bcm.integration(m, modes, body), rho, g)
 
 
-def random_body_request(rng: np.random.Generator, n_range=(300, 6000), nfreq: int = 8, g: float = 9.81,
-                        rho: float = 1025.0) -> Request:
+def two_cylinder_request(rng: np.random.Generator, n_range=(200, 16000), max_freq: int = 64,
+                         max_headings: int = 4, g: float = 9.81, rho: float = 1025.0) -> Request:
+    """The serving workload with its problems as host arrays."""
+    m, body, modes, omegas, betas = two_cylinder_parts(rng, n_range, max_freq, max_headings)
+    return _arrays(m, body, modes, omegas, betas, 0.0, g, rho)
+
+
+def two_cylinder_waves(rng: np.random.Generator, n_range=(200, 16000), max_freq: int = 64,
+                       max_headings: int = 4, g: float = 9.81, rho: float = 1025.0) -> WaveRequest:
+    """The serving workload with its problems generated on the device."""
+    m, body, modes, omegas, betas = two_cylinder_parts(rng, n_range, max_freq, max_headings)
+    return WaveRequest(m, 0.0, omegas, betas, modes, None, body, rho, g)
+
+
+def random_body_parts(rng: np.random.Generator, n_range=(300, 6000), nfreq: int = 8):
     """One body of the batched workload: a cylinder, box or prolate spheroid
     (equally likely) with dimensions U(2, 50) m and a draft of U(0.2, 0.8)
     of its height, meshed to about N panels (N log-uniform in n_range);
     6 DOF about the waterplane centre + heading 0 (7 right-hand sides),
-    nfreq frequencies uniform in 0.2-2.0 rad/s, infinite depth."""
+    nfreq frequencies uniform in 0.2-2.0 rad/s, infinite depth.  Returns
+    (mesh, modes, omegas, headings in radians)."""
     kind = int(rng.integers(0, 3))
     length, width, height = rng.uniform(2.0, 50.0, 3)
     draft = rng.uniform(0.2, 0.8) * height
@@ -172,11 +249,18 @@ def random_body_request(rng: np.random.Generator, n_range=(300, 6000), nfreq: in
         d = min(width, length)
         n_u = max(4, round(math.sqrt(target * length / (length + d))))
         m = nmesh.spheroid(max(length, d * 1.01), d, draft / height, n_u, max(4, round(target / n_u)))
-    modes = bcm.six_dof()
-    omegas = np.linspace(0.2, 2.0, nfreq)
-    rad = np.stack([bcm.radiation(m, md) for md in modes], axis=1)
-    nvel = np.empty((nfreq, m.npanels, 1 + len(modes)), np.complex128)
-    for i, w in enumerate(omegas):
-        nvel[i, :, 0] = bcm.diffraction(m, w, 0.0, 0.0, g, rho)[0]
-        nvel[i, :, 1:] = rad
-    return Request(m, 0.0, omegas, nvel, bcm.integration(m, modes), rho, g)
+    return m, bcm.six_dof(), np.linspace(0.2, 2.0, nfreq), np.zeros(1)
+
+
+def random_body_request(rng: np.random.Generator, n_range=(300, 6000), nfreq: int = 8, g: float = 9.81,
+                        rho: float = 1025.0) -> Request:
+    """The batched-bodies workload with its problems as host arrays."""
+    m, modes, omegas, betas = random_body_parts(rng, n_range, nfreq)
+    return _arrays(m, None, modes, omegas, betas, 0.0, g, rho)
+
+
+def random_body_waves(rng: np.random.Generator, n_range=(300, 6000), nfreq: int = 8, g: float = 9.81,
+                      rho: float = 1025.0) -> WaveRequest:
+    """The batched-bodies workload with its problems generated on the device."""
+    m, modes, omegas, betas = random_body_parts(rng, n_range, nfreq)
+    return WaveRequest(m, 0.0, omegas, betas, modes, None, None, rho, g)

@@ -157,3 +157,35 @@ def test_batched_bodies_mixed_vs_fp64():
     for r, a, b in zip(reqs, out[core.MIXED], out[core.FP64]):
         err = float((np.abs(a - b).max(axis=1) / np.abs(b).max(axis=1)).max())
         assert err < 1e-6, (r.npanels, err)
+
+
+@pytest.mark.gpu
+def test_wave_requests_match_array_requests():
+    """A request whose problems the device generates equals the same request
+    with host-generated arrays (forces within 1e-12), its Froude-Krylov
+    forces equal -NDS . p_incident of the host restatement, and it is the
+    same alone and in a mix."""
+    rng = np.random.default_rng(17)
+    m, body, modes, omegas, betas = serving.two_cylinder_parts(rng, n_range=(500, 900), max_freq=3)
+    arr = serving._arrays(m, body, modes, omegas, betas, 0.0, 9.81, 1025.0)
+    wav = serving.WaveRequest(m, 0.0, omegas, betas, modes, None, body)
+    other = serving.random_body_waves(np.random.default_rng(4), n_range=(300, 700), nfreq=2)
+    with serving.Server(workers=3, tables=_tables()) as srv:
+        fa, _ = srv.wait(srv.submit(arr, core.FP64))
+        fw, fk, _ = srv.wait(srv.submit(wav, core.FP64), with_fk=True)
+        ids = [srv.submit(other, core.MIXED), srv.submit(wav, core.FP64), srv.submit(arr, core.MIXED)]
+        srv.wait(ids[0])
+        fw_mix, fk_mix, _ = srv.wait(ids[1], with_fk=True)
+        srv.wait(ids[2])
+    # scale per frequency over all forces: the yaw of each cylinder about its
+    # own axis moves no water, so those columns are rounding noise on both
+    # paths and a per-column ratio would compare noise with noise
+    err = max(float(np.abs(fw[i] - fa[i]).max() / np.abs(fa[i]).max()) for i in range(omegas.size))
+    print(f"wave request vs array request: {err:.2e}")
+    assert err < 1e-12
+    assert np.array_equal(fw, fw_mix) and np.array_equal(fk, fk_mix)
+    nds = bcm.integration(m, modes, body)
+    for i, w in enumerate(omegas):
+        pinc = np.stack([bcm.diffraction(m, w, b, 0.0, 9.81, 1025.0)[1] for b in betas], axis=1)
+        ref = -(nds @ pinc)
+        assert np.abs(fk[i] - ref).max() <= 1e-12 * np.abs(ref).max()
