@@ -123,6 +123,16 @@ int nemoh_core_body_conditions(nemoh_core *ctx, double omega, int nbeta, const d
 int nemoh_core_solve_waves(nemoh_core *ctx, double omega, int nbeta, const double *beta, double *forces,
                            double *fk);
 
+/* Kochin function (COMPUTE_KOCHIN.f90:25-153, called at SOLVE_BEM.f90:
+ * 98-103) of the sources of every problem of the last solve on this handle,
+ * at ntheta angles theta (radians): hkochin(ntheta, nrhs) complex, nrhs the
+ * problem count of that call; the wave number is the Solver's kwave of that
+ * frequency (w^2/g, or AMH/Depth in finite depth, SOLVE_BEM.f90:66-84) and
+ * (xeff, yeff) the wave reference point (Nemoh.cal line 5, COM_VAR XEFF,
+ * YEFF). */
+int nemoh_core_kochin(nemoh_core *ctx, double xeff, double yeff, int ntheta, const double *theta,
+                      double *hkochin);
+
 /* Frequency parameters as the standalone mode computes them. */
 typedef struct {
   int finite_depth;

@@ -22,7 +22,7 @@ EXPORTS = (
     "nemoh_core_last_error", "nemoh_core_destroy", "nemoh_core_prepare_mesh", "nemoh_core_build_tables",
     "nemoh_core_wavenumber", "nemoh_core_theta_rule", "nemoh_core_prony", "nemoh_core_pairs",
     "nemoh_core_mfma_probe", "nemoh_core_get_extent", "nemoh_core_set_dofs", "nemoh_core_body_conditions",
-    "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_server_create", "nemoh_server_submit",
+    "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_server_create", "nemoh_server_submit",
     "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_last_error", "nemoh_server_destroy",
 )
 
@@ -105,6 +105,7 @@ def load():
                                       C.c_double]
     L.nemoh_core_body_conditions.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
     L.nemoh_core_solve_waves.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp]
+    L.nemoh_core_kochin.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp]
     L.nemoh_core_incident_wavenumber.argtypes = [C.c_double, C.c_double, C.c_double]
     L.nemoh_core_incident_wavenumber.restype = C.c_double
     L.nemoh_server_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_double] + [_dp] * 6
@@ -225,6 +226,7 @@ class Core:
         if r:
             raise NemohError(r, "nemoh_core_create failed (is a HIP device present?)")
         self.h = h
+        self._last_nrhs = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -276,7 +278,17 @@ class Core:
         fk = np.zeros((self._nforce, max(nb, 1)), np.complex128, order="F")
         self._check(load().nemoh_core_solve_waves(self.h, float(omega), nb, _p(b) if nb else None,
                                                   forces.ctypes.data_as(_dp), fk.ctypes.data_as(_dp)))
+        self._last_nrhs = nb + self._nrad
         return forces, fk[:, :nb]
+
+    def kochin(self, thetas, xeff: float = 0.0, yeff: float = 0.0, nrhs: int = None) -> np.ndarray:
+        """Kochin function (ntheta, nrhs) of the last solve's problems."""
+        th = _f64(np.atleast_1d(thetas))
+        n = nrhs if nrhs is not None else self._last_nrhs
+        out = np.zeros((th.size, n), np.complex128, order="F")
+        self._check(load().nemoh_core_kochin(self.h, float(xeff), float(yeff), th.size, _p(th),
+                                             out.ctypes.data_as(_dp)))
+        return out
 
     def extent(self) -> float:
         """GM: the largest horizontal vertex distance (to the y-images too
@@ -318,6 +330,7 @@ class Core:
         outs = [np.zeros(self.nfa, np.complex128) for _ in range(4)]
         self._check(load().nemoh_core_solve(self.h, float(period), amh, nexp, _p(ambda), _p(ar), nq, _p(qq), _p(cq),
                                             nv.ctypes.data_as(_dp), *[o.ctypes.data_as(_dp) for o in outs]))
+        self._last_nrhs = 1
         return outs
 
     def solve_multi(self, period, freq, nvel, nds=None, want_pressure=True):
@@ -334,6 +347,7 @@ class Core:
             self.h, float(period), amh, nexp, _p(ambda), _p(ar), nq, _p(qq), _p(cq), nrhs, nv.ctypes.data_as(_dp),
             None if pres is None else pres.ctypes.data_as(_dp), nint, None if nd is None else nd.ctypes.data_as(_dp),
             None if forces is None else forces.ctypes.data_as(_dp)))
+        self._last_nrhs = nrhs
         return pres, forces
 
     def solve_frequency(self, period, nvel, nds=None, want_pressure=True):
@@ -348,6 +362,7 @@ class Core:
             self.h, float(period), nrhs, nv.ctypes.data_as(_dp), None if pres is None else pres.ctypes.data_as(_dp),
             nint, None if nd is None else nd.ctypes.data_as(_dp),
             None if forces is None else forces.ctypes.data_as(_dp)))
+        self._last_nrhs = nrhs
         return pres, forces
 
     def pairs(self, period, freq):

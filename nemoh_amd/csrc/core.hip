@@ -38,6 +38,9 @@ struct nemoh_core {
   double* d_fnds = nullptr;   // force weights (nforce x ntot)
   double* d_beta = nullptr;   // headings of the current call (<= 16)
   double2* d_pinc = nullptr;  // incident pressure (ntot x nbeta)
+  int last_nrhs = 0;          // problems of the last solve whose sources are in sig (0: none)
+  double* d_theta = nullptr;  // Kochin angles of the current call
+  double2* d_kochin = nullptr;
   double* d_pts = nullptr;  // vertex x, y (np each) for the extent reduction
   double* d_ext = nullptr;  // per-workgroup maxima of the extent reduction
   // resident geometry and tables
@@ -352,6 +355,7 @@ int run_pipeline(nemoh_core* c, double w, int nrhs) {
   }
   launch_combine(c->Z[0], c->Z[1], c->F[0], c->F[1], n, c->nsym, nrhs, c->rho * w, c->sig, c->pot, c->pres, c->st);
   HIPCHK(c, hipGetLastError());
+  c->last_nrhs = nrhs;
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], c->st));
   return NEMOH_OK;
 }
@@ -559,6 +563,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   c->nq_state = 0;
   c->cap = 0;
   c->dofs_set = false;  // body conditions belong to the previous mesh
+  c->last_nrhs = 0;
   c->stats = nemoh_stats{};
   c->upd_blocks = 0;
   const double* d = c->d_geo;
@@ -689,6 +694,8 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->d_pts);
   release(c->d_ext);
   release(c->d_pg);
+  release(c->d_theta);
+  release(c->d_kochin);
   release(c->d_body);
   release(c->d_dofs);
   release(c->d_fnds);
@@ -846,6 +853,30 @@ int nemoh_core_solve_waves(nemoh_core* c, double omega, int nbeta, const double*
   if (fk && c->nforce > 0 && nbeta > 0)
     if (int r = download_forces(c, c->d_pinc, c->d_fnds, c->nforce, nbeta, fk)) return r;
   return finish_call(c);
+}
+
+int nemoh_core_kochin(nemoh_core* c, double xeff, double yeff, int ntheta, const double* theta, double* hkochin) {
+  if (!c) return NEMOH_ERR_ARG;
+  if (ntheta <= 0 || !theta || !hkochin) return fail(c, NEMOH_ERR_ARG, "bad Kochin arguments");
+  if (c->last_nrhs <= 0) return fail(c, NEMOH_ERR_ARG, "no solve on this mesh yet");
+  const int nrhs = c->last_nrhs;
+  int r = 0;
+  if ((r = dalloc(c, &c->d_theta, (size_t)ntheta)) || (r = dalloc(c, &c->d_kochin, (size_t)ntheta * nrhs))) return r;
+  HIPCHK(c, hipMemcpyAsync(c->d_theta, theta, ntheta * sizeof(double), hipMemcpyHostToDevice, c->st));
+  // the Solver's kwave of that frequency (SOLVE_BEM.f90:66-84): w^2/g, or
+  // AMH/Depth in finite depth -- the ak0 of the frequency parameters
+  launch_kochin(c->G, c->sig, c->nsym, nrhs, c->h_par.ak0, c->depth, xeff, yeff, c->d_theta, ntheta, c->d_kochin,
+                c->st);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> h(2 * (size_t)ntheta * nrhs);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_kochin, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (int t = 0; t < ntheta; ++t)  // device (ntheta x nrhs) -> caller column-major
+    for (int q = 0; q < nrhs; ++q) {
+      hkochin[2 * (t + (size_t)ntheta * q)] = h[2 * ((size_t)t * nrhs + q)];
+      hkochin[2 * (t + (size_t)ntheta * q) + 1] = h[2 * ((size_t)t * nrhs + q) + 1];
+    }
+  return NEMOH_OK;
 }
 
 double nemoh_core_incident_wavenumber(double w, double depth, double g) {

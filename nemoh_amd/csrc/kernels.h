@@ -80,6 +80,10 @@ void launch_nds(const double* pg, const int* body, const Dof* dofs, int nforce, 
 void launch_diffraction(const double* pg, double k, double w, double g, double rho, double depth, double xeff,
                         double yeff, const double* beta, int nbeta, int n, int ntot, double2* out, int ld,
                         double2* pinc, hipStream_t st);
+// Kochin function of the sources sig (2n x nrhs: ZIGB rows then ZIGS rows)
+// at ntheta angles into H (ntheta x nrhs)
+void launch_kochin(const Geometry& G, const double2* sig, int nsym, int nrhs, double k, double depth, double xeff,
+                   double yeff, const double* theta, int ntheta, double2* H, hipStream_t st);
 // out[ntile*(ntile+1)/2] (ntile = ceil(np/256)): per-workgroup maxima of the
 // squared horizontal vertex distances (GM^2)
 void launch_extent(const double* x, const double* y, int np, int nsym, double* out, hipStream_t st);

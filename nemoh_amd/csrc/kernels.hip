@@ -304,6 +304,89 @@ __global__ __launch_bounds__(256) void extent_kernel(const double* __restrict__ 
   if (t == 0) out[blockIdx.x] = red[0];
 }
 
+// Kochin function (COMPUTE_KOCHIN.f90:25-153) of every problem of the last
+// solve: one workgroup per angle; thread t walks panels t, t+256, ..., forms
+// each panel's exact integral of exp(k (z - i wbar)) as a sum over its four
+// edges from the vertex exponentials CEP = exp(ZJ), CEM = exp(-ZJ - 2kh),
+// ZJ = k (z - i wbar), for the body and (NSYMY = 1) its y-image, and
+// accumulates sigma * ZS for every problem; the panel sums are reduced in a
+// fixed order.  H[theta * nrhs + q].
+struct KochinArgs {
+  double k, depth, xeff, yeff;
+  int nsym, nrhs;
+};
+__global__ __launch_bounds__(256) void kochin_kernel(Geometry G, const double2* __restrict__ sig,
+                                                     const double* __restrict__ theta, KochinArgs A,
+                                                     double2* __restrict__ H) {
+  __shared__ double2 red[256];
+  const int t = threadIdx.x, n = G.n;
+  const double th = theta[blockIdx.x];
+  const double ct = cos(th), st = sin(th);
+  const double kh = A.k * A.depth;
+  const double scale = (kh > 18.0 || kh <= 0.0) ? 1.0 / A.k : 0.5 * exp(kh) / sinh(kh) * (1.0 / A.k);
+  cplx acc[kMaxRhs];
+#pragma unroll
+  for (int q = 0; q < kMaxRhs; ++q) acc[q] = cmk(0.0, 0.0);
+  for (int i = t; i < n; i += 256) {
+    for (int img = 0; img <= A.nsym; ++img) {
+      const double sy = img ? -1.0 : 1.0;
+      cplx zj[5], cep[5], cem[5];
+      double vx[5], vy[5];
+      for (int l = 0; l < 5; ++l) {
+        const int v = 4 * i + (l & 3);
+        vx[l] = G.vx[v];
+        vy[l] = G.vy[v];
+        const double z = G.vz[v];
+        const double wbar = (vx[l] - A.xeff) * ct + (sy * vy[l] - A.yeff) * st;
+        zj[l] = cmk(A.k * z, -A.k * wbar);
+        cep[l] = (zj[l].re < -18.0) ? cmk(0.0, 0.0) : cexp_(zj[l]);
+        const cplx em = cmk(-zj[l].re - 2.0 * kh, -zj[l].im);
+        cem[l] = (em.re < -18.0 || -kh >= 0.0) ? cmk(0.0, 0.0) : cexp_(em);
+      }
+      const cplx zms = cmk(sy * G.yn[i], -G.zn[i] * st);
+      const cplx zns = cmk(G.xn[i], -G.zn[i] * ct);
+      cplx zs = cmk(0.0, 0.0);
+      for (int l = 0; l < 4; ++l) {
+        const double dx = vx[l + 1] - vx[l], dy = vy[l + 1] - vy[l];
+        const cplx zcs = img ? (dx * zms + dy * zns) : (dx * zms - dy * zns);
+        const cplx zc = zj[l + 1] - zj[l];
+        cplx zhs, zhd;
+        if (fabs(zc.im) < 1.E-04 && fabs(zc.re) < 1.E-04) {
+          zhs = 0.5 * (cep[l + 1] + cep[l]);
+          zhd = -0.5 * (cem[l + 1] + cem[l]);
+        } else {
+          zhs = cdiv(cep[l + 1] - cep[l], zc);
+          zhd = cdiv(cem[l + 1] - cem[l], zc);
+        }
+        const cplx a = zcs * zhs, b = zcs * zhd;
+        zs = zs + a + cmk(b.re, -b.im);
+      }
+      zs = (img ? -scale : scale) * zs;
+      const double2* s = sig + (size_t)(i + img * n) * A.nrhs;
+#pragma unroll
+      for (int q = 0; q < kMaxRhs; ++q)
+        if (q < A.nrhs) {
+          const double2 v = s[q];
+          acc[q] = acc[q] + cmk(v.x, v.y) * zs;
+        }
+    }
+  }
+  const double c4 = 0.25 / M_PI;
+#pragma unroll
+  for (int q = 0; q < kMaxRhs; ++q) {
+    if (q < A.nrhs) {  // uniform across the workgroup
+      red[t] = make_double2(acc[q].re, acc[q].im);
+      __syncthreads();
+      for (int k = 128; k > 0; k >>= 1) {
+        if (t < k) red[t] = make_double2(red[t].x + red[t + k].x, red[t].y + red[t + k].y);
+        __syncthreads();
+      }
+      if (t == 0) H[(size_t)blockIdx.x * A.nrhs + q] = make_double2(c4 * red[0].x, c4 * red[0].y);
+      __syncthreads();
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host-side launchers
 void launch_crek(const double* xr, const double* xz, double* tab, hipStream_t st) {
@@ -362,6 +445,11 @@ void launch_f32_to_f64(const float2* a, double2* b, size_t count, double scale, 
 }
 void launch_absmax(const double2* a, size_t count, double* out, int nblocks, hipStream_t st) {
   absmax_kernel<<<nblocks, 256, 0, st>>>(a, count, out);
+}
+
+void launch_kochin(const Geometry& G, const double2* sig, int nsym, int nrhs, double k, double depth, double xeff,
+                   double yeff, const double* theta, int ntheta, double2* H, hipStream_t st) {
+  kochin_kernel<<<ntheta, 256, 0, st>>>(G, sig, theta, KochinArgs{k, depth, xeff, yeff, nsym, nrhs}, H);
 }
 
 void launch_extent(const double* x, const double* y, int np, int nsym, double* out, hipStream_t st) {

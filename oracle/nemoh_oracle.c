@@ -1886,3 +1886,65 @@ double ora_pairs_rows(const ora_ctx *c, const ora_freq *f, int fd, int r0, int n
     }
   return acc;
 }
+
+/* ------------------------------------------------------------------------
+ * Kochin function, COMPUTE_KOCHIN.f90:25-153.  For each image (the body,
+ * and its y-mirror when NSYMY = 1) the vertex exponentials CEP = exp(ZJ)
+ * and CEM = exp(-ZJ - 2 k h) with ZJ = k (z - i wbar) (:57-70, :102-115),
+ * then the exact integral of exp over each flat panel as a sum over its
+ * four edges (:71-97, :116-142), scaled by 1/k (deep) or
+ * exp(kh) / (2 sinh kh k) (:92-96), and H = sum sigma ZS / (4 pi) (:143-151). */
+static void ora_kochin_vertex(double kwave, double depth, double x, double y, double z, double xeff,
+                              double yeff, double theta, double complex *zj, double complex *cep,
+                              double complex *cem) {
+  const double wbar = (x - xeff) * cos(theta) + (y - yeff) * sin(theta);
+  *zj = kwave * (z - I * wbar);
+  *cep = (creal(*zj) < -18.) ? 0.0 : cexp(*zj);
+  const double complex em = -*zj - 2. * kwave * depth;
+  *cem = (creal(em) < -18. || -kwave * depth >= 0.) ? 0.0 : cexp(em);
+}
+
+double complex ora_kochin(const ora_ctx *c, double kwave, double xeff, double yeff, double theta,
+                          const double complex *zigb, const double complex *zigs) {
+  const double pi = 4. * atan(1.);
+  const double kh = kwave * c->depth;
+  double complex h = 0.0;
+  for (int img = 0; img <= c->nsym; ++img) {
+    const double sy = img ? -1.0 : 1.0;
+    for (int i = 0; i < c->nfa; ++i) {
+      const int kk[5] = {c->m1[i], c->m2[i], c->m3[i], c->m4[i], c->m1[i]};
+      double complex zj[5], cep[5], cem[5];
+      for (int l = 0; l < 5; ++l)
+        ora_kochin_vertex(kwave, c->depth, c->x[kk[l]], sy * c->y[kk[l]], c->z[kk[l]], xeff, yeff, theta, &zj[l],
+                          &cep[l], &cem[l]);
+      const double complex zms = sy * c->yn[i] - I * c->zn[i] * sin(theta);
+      const double complex zns = c->xn[i] - I * c->zn[i] * cos(theta);
+      double complex zs = 0.0;
+      for (int l = 0; l < 4; ++l) {
+        const double dx = c->x[kk[l + 1]] - c->x[kk[l]], dy = c->y[kk[l + 1]] - c->y[kk[l]];
+        const double complex zcs = img ? zms * dx + zns * dy : zms * dx - zns * dy;
+        const double complex zc = zj[l + 1] - zj[l];
+        double complex zhs, zhd;
+        if (fabs(cimag(zc)) < 1.E-04 && fabs(creal(zc)) < 1.E-04) {
+          zhs = 0.5 * (cep[l + 1] + cep[l]);
+          zhd = -0.5 * (cem[l + 1] + cem[l]);
+        } else {
+          zhs = (cep[l + 1] - cep[l]) / zc;
+          zhd = (cem[l + 1] - cem[l]) / zc;
+        }
+        zs += zcs * zhs + conj(zcs * zhd);
+      }
+      if (kh > 18. || kh <= 0.)
+        zs = zs / kwave;
+      else
+        zs = 0.5 * zs * exp(kh) / sinh(kh) * 1. / kwave;
+      h += img ? zigs[i] * (-zs) : zigb[i] * zs;
+    }
+  }
+  return 0.25 / pi * h;
+}
+
+void ora_kochin_vec(const ora_ctx *c, double kwave, double xeff, double yeff, int n, const double *theta,
+                    const double complex *zigb, const double complex *zigs, double complex *out) {
+  for (int k = 0; k < n; ++k) out[k] = ora_kochin(c, kwave, xeff, yeff, theta[k], zigb, zigs);
+}

@@ -18,6 +18,9 @@
  *                     Rankine part vs adaptive quadrature of 1/r
  *   ora_session_*     golden Forces.dat of whole reference Solver runs;
  *                     GAUSSZ vs numpy.linalg on small matrices
+ *   ora_kochin        golden COMPUTE_KOCHIN values from the reference; the
+ *                     one-point panel quadrature the reference keeps beside
+ *                     it (COMPUTE_KOCHIN.f90:47-55) on a refined mesh
  */
 #ifndef NEMOH_ORACLE_H
 #define NEMOH_ORACLE_H
@@ -100,6 +103,15 @@ void ora_forces(int nint, int ntot, const double *nds, const double complex *pre
 /* GAUSSZ (M_SOLVER.f90:31) on an n x m column-major matrix with leading
  * dimension lda; returns 0 or 1 (pivot below 1e-20). */
 int ora_gaussz(double complex *a, int lda, int n, int m);
+
+/* COMPUTE_KOCHIN (COMPUTE_KOCHIN.f90:25-153): Kochin function at angle
+ * theta of the sources zigb, zigs [nfa] (zigs read only when nsym = 1) for
+ * wave number kwave, reference point (xeff, yeff). */
+double complex ora_kochin(const ora_ctx *c, double kwave, double xeff, double yeff, double theta,
+                          const double complex *zigb, const double complex *zigs);
+/* ora_kochin at n angles into out[n] (for ctypes). */
+void ora_kochin_vec(const ora_ctx *c, double kwave, double xeff, double yeff, int n, const double *theta,
+                    const double complex *zigb, const double complex *zigs, double complex *out);
 
 #ifdef __cplusplus
 }

@@ -144,6 +144,19 @@ def sprbm(c):
     return ier, rr, rc
 
 
+def kochin(case, kwave: float, xeff: float, yeff: float, thetas, zigb, zigs=None) -> np.ndarray:
+    """COMPUTE_KOCHIN at each angle for sources zigb, zigs (NFA each)."""
+    L = lib()
+    L.ora_kochin_vec.argtypes = [C.POINTER(_Ctx), C.c_double, C.c_double, C.c_double, C.c_int, _dp, _cp, _cp,
+                                 _cp]
+    th = np.ascontiguousarray(np.atleast_1d(thetas), dtype=np.float64)
+    zb = np.ascontiguousarray(zigb, dtype=np.complex128)
+    zs = np.ascontiguousarray(np.zeros_like(zb) if zigs is None else zigs, dtype=np.complex128)
+    out = np.zeros(th.size, np.complex128)
+    L.ora_kochin_vec(C.byref(case.ctx), float(kwave), float(xeff), float(yeff), th.size, th, zb, zs, out)
+    return out
+
+
 def prepare_mesh(vertices, panels, isym):
     v = np.ascontiguousarray(vertices, dtype=np.float64)
     p = np.ascontiguousarray(panels, dtype=np.int32)

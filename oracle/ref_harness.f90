@@ -12,6 +12,8 @@
 !   cinti  n  then n values of AKK        -> N, module NQ, QQ, CQ    COMPUTE_GREEN_INFD.f90:505
 !   x0     n  then n values of AK         -> X0(AK)                  COMPUTE_GREEN_FD.f90:1635
 !   wavenumber n then n lines "w depth g" -> the preProcessor's k   Common/Environment.f90:45-84
+!   kochin kwave xeff yeff ntheta, ntheta angles, NFA lines "ZIGB ZIGS" (re im re im)
+!          (after mesh)                   -> HKochin(theta)          COMPUTE_KOCHIN.f90:25-153
 !   lisc   n  then n lines "akh amh"      -> NEXP, AMBDA, AR         COMPUTE_GREEN_FD.f90:1007
 !   mesh   np nfa nsym depth, np lines "x y z", nfa lines "m1 m2 m3 m4"
 !                                          -> sets COM_VAR, runs PRE_PROC_MESH, CREK
@@ -47,6 +49,10 @@ PROGRAM ref_harness
   COMPLEX, ALLOCATABLE :: zmat(:,:)
   LOGICAL :: fd
   TYPE(TEnvironment) :: env
+  REAL :: kw, r1, i1, r2, i2
+  REAL, ALLOCATABLE :: thetas(:)
+  COMPLEX :: hk
+  EXTERNAL :: COMPUTE_KOCHIN  ! external subroutine of COMPUTE_KOCHIN.f90
 
   DO
     READ(*,*,IOSTAT=ios) cmd
@@ -90,6 +96,22 @@ PROGRAM ref_harness
         READ(*,*) w, env%Depth, env%G
         WRITE(*,'(4ES26.17E3)') w, env%Depth, env%G, Wavenumber(w, env)
       END DO
+    CASE ('kochin')
+      READ(*,*) kw, XEFF, YEFF, n
+      ALLOCATE(thetas(n))
+      DO i = 1, n
+        READ(*,*) thetas(i)
+      END DO
+      DO i = 1, NFA
+        READ(*,*) r1, i1, r2, i2
+        ZIGB(i) = CMPLX(r1, i1)
+        ZIGS(i) = CMPLX(r2, i2)
+      END DO
+      DO i = 1, n
+        CALL COMPUTE_KOCHIN(kw, thetas(i), hk)
+        WRITE(*,'(3ES26.17E3)') thetas(i), REAL(hk), AIMAG(hk)
+      END DO
+      DEALLOCATE(thetas)
     CASE ('lisc')
       READ(*,*) n
       DO i = 1, n

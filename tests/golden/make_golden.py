@@ -342,6 +342,41 @@ def wavenumber_fixture() -> None:
     print(f"wavenumber: {a.shape[0]} cases")
 
 
+def kochin_fixture() -> None:
+    """COMPUTE_KOCHIN (COMPUTE_KOCHIN.f90:25-153) of the reference build on
+    seeded random sources: a half cylinder (y-symmetry) and a barge, in
+    infinite depth and at h = 50 m on both sides of its k h = 18 switch, with
+    an off-origin wave reference point."""
+    rng = np.random.default_rng(2718)
+    cases = []
+    for name, m in (("cyl", nmesh.cylinder(10.0, 5.0, 16, 4, 4, half=True)),
+                    ("barge", nmesh.box(100.0, 20.0, 10.0, 18, 4, 2))):
+        for depth, kws in ((0.0, (0.05, 0.6)), (50.0, (0.05, 0.3, 0.5))):
+            for kw in kws:
+                thetas = np.linspace(0.0, 2.0 * np.pi, 13)
+                zb = rng.standard_normal(m.npanels) + 1j * rng.standard_normal(m.npanels)
+                zs = (rng.standard_normal(m.npanels) + 1j * rng.standard_normal(m.npanels)) * m.isym
+                xeff, yeff = 3.0, -2.0
+                cmd = f"mesh\n{m.npoints} {m.npanels} {m.isym} {depth:.17e}\n"
+                cmd += "".join(f"{x:.17e} {y:.17e} {z:.17e}\n" for x, y, z in m.vertices)
+                cmd += "".join(" ".join(str(v + 1) for v in p) + "\n" for p in m.panels)
+                cmd += f"kochin\n{kw:.17e} {xeff:.17e} {yeff:.17e} {thetas.size}\n"
+                cmd += "".join(f"{t:.17e}\n" for t in thetas)
+                cmd += "".join(f"{a.real:.17e} {a.imag:.17e} {b.real:.17e} {b.imag:.17e}\n" for a, b in zip(zb, zs))
+                lines = harness(cmd + "end\n")
+                h = np.array([[float(v) for v in ln.split()] for ln in lines[-thetas.size:]])
+                cases.append(dict(name=name, vertices=m.vertices, panels=m.panels, isym=m.isym, depth=depth,
+                                  kwave=kw, xeff=xeff, yeff=yeff, thetas=thetas, zigb=zb, zigs=zs,
+                                  h=h[:, 1] + 1j * h[:, 2]))
+    out = {}
+    for k, c in enumerate(cases):
+        for key, v in c.items():
+            out[f"c{k}_{key}"] = v
+    out["ncases"] = len(cases)
+    np.savez_compressed(OUT / "fn_kochin.npz", **out)
+    print(f"kochin: {len(cases)} cases")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -354,6 +389,8 @@ def main() -> None:
             fn_fixtures()
         if a.only in ("", "fn", "wavenumber"):
             wavenumber_fixture()
+        if a.only in ("", "fn", "kochin"):
+            kochin_fixture()
         if a.only in ("", "pairs"):
             pair_fixtures()
         if a.only in ("", "cases"):

@@ -11,6 +11,7 @@ from scipy import integrate, special
 
 from conftest import case_geometry, load_golden
 from nemoh_amd import io as nio
+from nemoh_amd import mesh as nmesh
 from oracle import pyoracle as po
 
 
@@ -330,3 +331,36 @@ def test_session_reproduces_reference_forces(name, nprob, ref_tables):
     err = np.abs(F - ref).max(axis=0) / np.abs(ref).max(axis=0)
     assert s.undefined_pairs == 0
     assert err.max() < 1e-12
+
+
+def test_kochin_matches_reference():
+    """ora_kochin against COMPUTE_KOCHIN of the reference build: half
+    cylinder (y-images) and barge, deep and h = 50 m on both sides of the
+    k h = 18 switch, off-origin reference point, random sources."""
+    d = load_golden("fn_kochin.npz")
+    for k in range(int(d["ncases"])):
+        def g(key):
+            return d[f"c{k}_{key}"]
+        case = po.Case(g("vertices"), g("panels"), int(g("isym")), depth=float(g("depth")))
+        h = po.kochin(case, float(g("kwave")), float(g("xeff")), float(g("yeff")), g("thetas"), g("zigb"), g("zigs"))
+        assert np.abs(h - g("h")).max() <= 1e-13 * np.abs(g("h")).max(), k
+
+
+def test_kochin_edge_integral_converges_to_panel_quadrature():
+    """The exact edge integral of exp(k (z - i wbar)) over each panel and the
+    one-point panel quadrature (COMPUTE_KOCHIN.f90:47-55) converge to each
+    other at second order as the mesh is refined (a wrong edge term, sign
+    or scale would not converge)."""
+    errs = []
+    for nt, nz, nr in ((16, 4, 4), (32, 8, 8), (64, 16, 16)):
+        m = nmesh.cylinder(10.0, 5.0, nt, nz, nr, half=False)
+        case = po.Case(m.vertices, m.panels, 0, depth=0.0)
+        geo = po.prepare_mesh(m.vertices, m.panels, 0)
+        kw, th = 0.3, np.array([0.0, 0.7, 2.0])
+        sig = np.exp(0.3j * geo["xg"]) + 0.5 * geo["zg"]
+        h = po.kochin(case, kw, 0.0, 0.0, th, sig)
+        h1 = np.array([0.25 / np.pi * np.sum(sig * np.exp(kw * geo["zg"]) *
+                                              np.exp(-1j * kw * (geo["xg"] * np.cos(t) + geo["yg"] * np.sin(t))) *
+                                              geo["aire"]) for t in th])
+        errs.append(np.abs(h - h1).max() / np.abs(h).max())
+    assert errs[2] < 2e-3 and 3.0 < errs[0] / errs[1] < 5.0 and 3.0 < errs[1] / errs[2] < 5.0, errs
