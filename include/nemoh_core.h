@@ -133,6 +133,16 @@ int nemoh_core_solve_waves(nemoh_core *ctx, double omega, int nbeta, const doubl
 int nemoh_core_kochin(nemoh_core *ctx, double xeff, double yeff, int ntheta, const double *theta,
                       double *hkochin);
 
+/* Potential of the sources of every problem of the last solve on this
+ * handle at npoints field points xyz[3 p + {0,1,2}] (x, y, z):
+ * phi(npoints, nrhs) complex.  Replaces COMPUTE_POTENTIAL_DOMAIN
+ * (COMPUTE_POTENTIAL_DOMAIN.f90:24-82) called per point of the free-surface
+ * mesh at SOLVE_BEM.f90:104-110 (z = 0 there; the elevation is
+ * ETA = i w / g PHI); the regime and wave number are those of the solve.
+ * NEMOH_ERR_ARG if a point has AK0 (z + ZG) > -1.5e-6 for some panel, where
+ * the reference stops. */
+int nemoh_core_potential(nemoh_core *ctx, int npoints, const double *xyz, double *phi);
+
 /* Frequency parameters as the standalone mode computes them. */
 typedef struct {
   int finite_depth;

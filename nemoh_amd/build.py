@@ -14,7 +14,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libnemoh_core.so"
-SOURCES = ["core.hip", "kernels.hip", "lu.hip", "bodycond.hip", "host_math.cpp", "server.cpp"]
+SOURCES = ["core.hip", "kernels.hip", "lu.hip", "bodycond.hip", "potential.hip", "host_math.cpp", "server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NEMOH_OFFLOAD_ARCH", "gfx950")
 

@@ -22,7 +22,7 @@ EXPORTS = (
     "nemoh_core_last_error", "nemoh_core_destroy", "nemoh_core_prepare_mesh", "nemoh_core_build_tables",
     "nemoh_core_wavenumber", "nemoh_core_theta_rule", "nemoh_core_prony", "nemoh_core_pairs",
     "nemoh_core_mfma_probe", "nemoh_core_get_extent", "nemoh_core_set_dofs", "nemoh_core_body_conditions",
-    "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_server_create", "nemoh_server_submit",
+    "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_core_potential", "nemoh_server_create", "nemoh_server_submit",
     "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_last_error", "nemoh_server_destroy",
 )
 
@@ -106,6 +106,7 @@ def load():
     L.nemoh_core_body_conditions.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
     L.nemoh_core_solve_waves.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp]
     L.nemoh_core_kochin.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp]
+    L.nemoh_core_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.nemoh_core_incident_wavenumber.argtypes = [C.c_double, C.c_double, C.c_double]
     L.nemoh_core_incident_wavenumber.restype = C.c_double
     L.nemoh_server_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_double] + [_dp] * 6
@@ -288,6 +289,16 @@ class Core:
         out = np.zeros((th.size, n), np.complex128, order="F")
         self._check(load().nemoh_core_kochin(self.h, float(xeff), float(yeff), th.size, _p(th),
                                              out.ctypes.data_as(_dp)))
+        return out
+
+    def potential(self, points, nrhs: int = None) -> np.ndarray:
+        """Potential (npoints, nrhs) of the last solve's sources at field
+        points (npoints, 3); the free-surface elevation at z = 0 is
+        1j * w / g times it."""
+        pts = _f64(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+        n = nrhs if nrhs is not None else self._last_nrhs
+        out = np.zeros((pts.shape[0], n), np.complex128, order="F")
+        self._check(load().nemoh_core_potential(self.h, pts.shape[0], _p(pts), out.ctypes.data_as(_dp)))
         return out
 
     def extent(self) -> float:

@@ -41,6 +41,10 @@ struct nemoh_core {
   int last_nrhs = 0;          // problems of the last solve whose sources are in sig (0: none)
   double* d_theta = nullptr;  // Kochin angles of the current call
   double2* d_kochin = nullptr;
+  double amh = 0;               // AMH of the last frequency set-up (finite depth)
+  double* d_field = nullptr;    // field points of the current potential call (x y z each)
+  double2* d_phi = nullptr;     // their potentials (npoints x nrhs)
+  int* d_fbad = nullptr;        // per point: 1 if it has no value
   double* d_pts = nullptr;  // vertex x, y (np each) for the extent reduction
   double* d_ext = nullptr;  // per-workgroup maxima of the extent reduction
   // resident geometry and tables
@@ -159,6 +163,8 @@ int build_params(nemoh_core* c, double period, double amh, int nexp, const doubl
   if (!(std::fabs(w) >= 1.E-4)) return fail(c, NEMOH_ERR_OMEGA, "ABS(WR) < 1.E-4");
   const double kwave = w * w / c->g;
   const bool fd = !(c->depth == 0.0 || kwave * c->depth >= 20);  // SOLVE_BEM.f90:66
+  c->last_nrhs = 0;  // the sources in sig belong to the previous parameters
+  c->amh = amh;
   FreqParams& P = c->h_par;
   std::memset(&P, 0, sizeof P);
   P.fd = fd ? 1 : 0;
@@ -696,6 +702,9 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->d_pg);
   release(c->d_theta);
   release(c->d_kochin);
+  release(c->d_field);
+  release(c->d_phi);
+  release(c->d_fbad);
   release(c->d_body);
   release(c->d_dofs);
   release(c->d_fnds);
@@ -875,6 +884,45 @@ int nemoh_core_kochin(nemoh_core* c, double xeff, double yeff, int ntheta, const
     for (int q = 0; q < nrhs; ++q) {
       hkochin[2 * (t + (size_t)ntheta * q)] = h[2 * ((size_t)t * nrhs + q)];
       hkochin[2 * (t + (size_t)ntheta * q) + 1] = h[2 * ((size_t)t * nrhs + q) + 1];
+    }
+  return NEMOH_OK;
+}
+
+int nemoh_core_potential(nemoh_core* c, int npoints, const double* xyz, double* phi) {
+  if (!c) return NEMOH_ERR_ARG;
+  if (npoints <= 0 || !xyz || !phi) return fail(c, NEMOH_ERR_ARG, "bad potential arguments");
+  if (c->last_nrhs <= 0) return fail(c, NEMOH_ERR_ARG, "no solve on this mesh yet");
+  const int nrhs = c->last_nrhs;
+  int r = 0;
+  if ((r = dalloc(c, &c->d_field, 3 * (size_t)npoints)) || (r = dalloc(c, &c->d_phi, (size_t)npoints * nrhs)) ||
+      (r = dalloc(c, &c->d_fbad, (size_t)npoints)))
+    return r;
+  HIPCHK(c, hipMemcpyAsync(c->d_field, xyz, 3 * (size_t)npoints * sizeof(double), hipMemcpyHostToDevice, c->st));
+  // VNVF's A (COMPUTE_GREEN_FREESURFACE.f90:420-442) from the AMH of the solve
+  double a = 0.0;
+  if (c->h_par.fd) {
+    const double akh = c->amh * std::tanh(c->amh);
+    a = (c->amh + akh) * (c->amh + akh) / (c->depth * (c->amh * c->amh - akh * akh + akh));
+  }
+  launch_potential(c->G, c->T, c->d_par, c->h_par.fd, 2.0 * c->depth, a, c->d_field, npoints, c->sig, nrhs, c->d_phi,
+                   c->d_fbad, c->st);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> h(2 * (size_t)npoints * nrhs);
+  std::vector<int> bad(npoints);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_phi, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(bad.data(), c->d_fbad, bad.size() * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (int p = 0; p < npoints; ++p)
+    if (bad[p]) {
+      char buf[128];
+      std::snprintf(buf, sizeof buf, "AKZ < -1.5 E-6: field point %d (z = %g) too close to the free surface", p,
+                    xyz[3 * (size_t)p + 2]);
+      return fail(c, NEMOH_ERR_ARG, buf);
+    }
+  for (int p = 0; p < npoints; ++p)  // device (npoints x nrhs) -> caller column-major
+    for (int q = 0; q < nrhs; ++q) {
+      phi[2 * (p + (size_t)npoints * q)] = h[2 * ((size_t)p * nrhs + q)];
+      phi[2 * (p + (size_t)npoints * q) + 1] = h[2 * ((size_t)p * nrhs + q) + 1];
     }
   return NEMOH_OK;
 }

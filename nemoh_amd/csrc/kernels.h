@@ -84,6 +84,13 @@ void launch_diffraction(const double* pg, double k, double w, double g, double r
 // at ntheta angles into H (ntheta x nrhs)
 void launch_kochin(const Geometry& G, const double2* sig, int nsym, int nrhs, double k, double depth, double xeff,
                    double yeff, const double* theta, int ntheta, double2* H, hipStream_t st);
+// Potential of the sources sig (ZIGB, then ZIGS when nsym = 1; nrhs
+// problems) at npts field points pts[3 p + {0,1,2}] into phi[p * nrhs + q];
+// bad[p] = 1 where the point has no value.  fd: the regime of *P; dhsl = 2
+// Depth; a: the finite-depth amplitude factor of the frequency.
+void launch_potential(const Geometry& G, const Tables& T, const FreqParams* P, int fd, double dhsl, double a,
+                      const double* pts, int npts, const double2* sig, int nrhs, double2* phi, int* bad,
+                      hipStream_t st);
 // out[ntile*(ntile+1)/2] (ntile = ceil(np/256)): per-workgroup maxima of the
 // squared horizontal vertex distances (GM^2)
 void launch_extent(const double* x, const double* y, int np, int nsym, double* out, hipStream_t st);

@@ -32,6 +32,26 @@ MODULE NEMOH_CORE_BIND
       REAL(C_DOUBLE) :: ambda(*), ar(*), qq(*), cq(*)
       COMPLEX(C_DOUBLE_COMPLEX) :: nvel(*), zigb(*), zigs(*), zpb(*), zps(*)
     END FUNCTION
+    ! nemoh_core_kochin: COMPUTE_KOCHIN per angle (SOLVE_BEM.f90:98-103),
+    ! for the sources of the last nemoh_core_solve
+    INTEGER(C_INT) FUNCTION nemoh_core_kochin(ctx, xeff, yeff, ntheta, theta, hkochin) &
+        BIND(C, NAME='nemoh_core_kochin')
+      IMPORT
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), VALUE :: xeff, yeff
+      INTEGER(C_INT), VALUE :: ntheta
+      REAL(C_DOUBLE) :: theta(*)
+      COMPLEX(C_DOUBLE_COMPLEX) :: hkochin(*)
+    END FUNCTION
+    ! nemoh_core_potential: COMPUTE_POTENTIAL_DOMAIN per field point
+    ! (SOLVE_BEM.f90:104-110), xyz(3, npoints)
+    INTEGER(C_INT) FUNCTION nemoh_core_potential(ctx, npoints, xyz, phi) BIND(C, NAME='nemoh_core_potential')
+      IMPORT
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: npoints
+      REAL(C_DOUBLE) :: xyz(*)
+      COMPLEX(C_DOUBLE_COMPLEX) :: phi(*)
+    END FUNCTION
     INTEGER(C_INT) FUNCTION nemoh_core_set_precision(ctx, mode, tol, max_iter) &
         BIND(C, NAME='nemoh_core_set_precision')
       IMPORT

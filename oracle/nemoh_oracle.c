@@ -1948,3 +1948,267 @@ void ora_kochin_vec(const ora_ctx *c, double kwave, double xeff, double yeff, in
                     const double complex *zigb, const double complex *zigs, double complex *out) {
   for (int k = 0; k < n; ++k) out[k] = ora_kochin(c, kwave, xeff, yeff, theta[k], zigb, zigs);
 }
+
+/* ------------------------------------------------------------------------
+ * Potential at a field point, COMPUTE_POTENTIAL_DOMAIN.f90:24-82, with the
+ * field-point Green routines of COMPUTE_GREEN_FREESURFACE.f90.  Only the
+ * potentials (SP1 SM1 SP2 SM2) are restated: the routine reads nothing else. */
+
+static double ipow_m1(int e) { return (e % 2 == 0) ? 1.0 : -1.0; } /* (-1)**e */
+
+/* VVV :33-196, potential part: FSP, FSM of panel j seen from (xoi, yoi, zoi).
+ * DHSL = 2*Depth for both KKK (:52), unlike VAVINFD. */
+static void ora_vvv(const ora_ctx *c, int kkk, int j, double xoi, double yoi, double zoi, double *fsp,
+                    double *fsm) {
+  const double pi4 = atan(1.), pi = 4. * pi4, qpi = 4. * pi;
+  const double dhsl = 2 * c->depth;
+  const int njj = c->nsym + 1;
+  const double mk = (kkk == 1) ? 1. : -1.;
+  double aijs[4] = {0, 0, 0, 0};
+  for (int jj = 1; jj <= 2 * njj; ++jj) {
+    const double mjj = ipow_m1(jj + 1), my = ipow_m1(jj / 3 + 2), mz = ipow_m1(jj / 2 + 2);
+    const double mh = (1 - ipow_m1(jj / 2 + 2)) / 2;
+    const double xoj = c->xg[j], yoj = c->yg[j] * my, zoj = c->zg[j] * mz - dhsl * mh;
+    const double a3j = c->xn[j], a6j = c->yn[j] * my, a9j = c->zn[j] * mz;
+    const double ro = sqrt(sq(xoi - xoj) + sq(yoi - yoj) + sq(zoi - zoj));
+    double a;
+    if (ro > 7. * c->tdis[j]) {
+      a = c->aire[j] / ro;
+    } else {
+      const int kk[5] = {c->m1[j], c->m2[j], c->m3[j], c->m4[j], c->m1[j]};
+      double txn[5], tyn[5], tzn[5], rr[5];
+      for (int l = 0; l < 5; ++l) {
+        txn[l] = c->x[kk[l]];
+        tyn[l] = c->y[kk[l]] * my;
+        tzn[l] = c->z[kk[l]] * mz - dhsl * mh;
+        rr[l] = sqrt(sq(xoi - txn[l]) + sq(yoi - tyn[l]) + sq(zoi - tzn[l]));
+      }
+      const double gz = (xoi - xoj) * a3j + (yoi - yoj) * a6j + (zoi - zoj) * a9j;
+      a = 0.;
+      for (int l = 0; l < 4; ++l) {
+        const double dk = sqrt(sq(txn[l + 1] - txn[l]) + sq(tyn[l + 1] - tyn[l]) + sq(tzn[l + 1] - tzn[l]));
+        if (!(dk >= 1.E-3 * c->tdis[j])) continue;
+        const double pj = (txn[l + 1] - txn[l]) / dk, qj = (tyn[l + 1] - tyn[l]) / dk,
+                     rj = (tzn[l + 1] - tzn[l]) / dk;
+        const double gyx = a6j * rj - a9j * qj, gyy = a9j * pj - a3j * rj, gyz = a3j * qj - a6j * pj;
+        const double gy = (xoi - txn[l]) * gyx + (yoi - tyn[l]) * gyy + (zoi - tzn[l]) * gyz;
+        const double ddk = 2. * dk, ant = gy * ddk;
+        const double dnt = sq(rr[l + 1] + rr[l]) - dk * dk + 2. * fabs(gz) * (rr[l + 1] + rr[l]);
+        const double arg = ant / dnt, anl = rr[l + 1] + rr[l] + dk, dnl = rr[l + 1] + rr[l] - dk;
+        const double alden = log(anl / dnl);
+        const double at = (fabs(gz) >= 1.E-4 * c->tdis[j]) ? atan(arg) : 0.;
+        a = a + gy * alden - 2. * fabs(gz) * at;
+      }
+      a *= mjj;
+    }
+    aijs[jj - 1] = a;
+  }
+  if (c->nsym == 1) {
+    *fsp = -(aijs[0] - mk * (aijs[1] + aijs[2]) + aijs[3]) / qpi;
+    *fsm = -(aijs[0] - mk * (aijs[1] - aijs[2]) - aijs[3]) / qpi;
+  } else {
+    *fsp = *fsm = -(aijs[0] - mk * aijs[1]) / qpi;
+  }
+}
+
+#define XR(i) c->xr[(i) - 1]
+#define XZ(j) c->xz[(j) - 1]
+
+/* submerged table / asymptotic / Rankine-only value of one z-image, as the
+ * field-point routines write it out (VNV :233-287, VNVF :466-519); returns
+ * -1 when AKZ > -1.5e-6 (the reference stops there) */
+static int field_image(const ora_ctx *c, double ak, double rrr, double akr, double zzz, double *pd1z,
+                       double *pd2z, double *psr) {
+  const double pi4 = atan(1.), pi = 4. * pi4, dpi = 2. * pi, eps = 0.0001;
+  const double akz = ak * zzz;
+  const double dd = sqrt(rrr * rrr + zzz * zzz);
+  double psurr = 0.;
+  *psr = 0.;
+  if (dd > eps) {
+    const double rr = ak * dd;
+    *psr = pi / rr;
+    psurr = pi / cube(rr);
+  }
+  if (akz > -1.5E-6) return -1;
+  if (akz > -16.) {
+    if (akr < 99.7) {
+      const int kj = kj_of(akz), ki = ki_of(akr);
+      const double xl1 = pl2(XR(ki), XR(ki + 1), XR(ki - 1), akr);
+      const double xl2 = pl2(XR(ki + 1), XR(ki - 1), XR(ki), akr);
+      const double xl3 = pl2(XR(ki - 1), XR(ki), XR(ki + 1), akr);
+      const double zl1 = pl2(XZ(kj), XZ(kj + 1), XZ(kj - 1), akz);
+      const double zl2 = pl2(XZ(kj + 1), XZ(kj - 1), XZ(kj), akz);
+      const double zl3 = pl2(XZ(kj - 1), XZ(kj), XZ(kj + 1), akz);
+      *pd1z = interp(c->apd1z, ki, kj, xl1, xl2, xl3, zl1, zl2, zl3);
+      *pd2z = interp(c->apd2z, ki, kj, xl1, xl2, xl3, zl1, zl2, zl3);
+    } else {
+      const double epz = exp(akz), akp4 = akr - pi4, sqv = sqrt(dpi / akr);
+      *pd1z = psurr * akz - pi * sqv * sin(akp4) * epz;
+      *pd2z = epz * sqv * cos(akp4);
+    }
+  } else {
+    *pd1z = psurr * akz;
+    *pd2z = 0.;
+  }
+  return 0;
+}
+
+#undef XR
+#undef XZ
+
+/* VVG :911-997, potential part: exact (near) or one-point (far) Rankine
+ * integral of the panel (y-mirrored when jj = 2) at (xpo, ypo, zpo), with
+ * the reference's 4 pi literal 12.5663706. */
+static double ora_vvg(const double *xl, const double *yl, const double *zl, int jj, double pn, double qn,
+                      double rn, double arj, double tdj, double xgg, double ygg, double zgg, double xpo,
+                      double ypo, double zpo) {
+  const double mjj = ipow_m1(jj + 1);
+  const double ro = sqrt(sq(xpo - xgg) + sq(ypo - ygg * mjj) + sq(zpo - zgg));
+  const double gz = (xpo - xgg) * pn + (ypo - ygg * mjj) * qn * mjj + (zpo - zgg) * rn;
+  if (ro - 7. * tdj > 0.) return -arj / (ro * 12.5663706);
+  double rr[5];
+  for (int l = 0; l < 5; ++l) rr[l] = sqrt(sq(xpo - xl[l]) + sq(ypo - yl[l] * mjj) + sq(zpo - zl[l]));
+  double fs = 0.;
+  for (int l = 0; l < 4; ++l) {
+    const double dk = sqrt(sq(xl[l + 1] - xl[l]) + sq(yl[l + 1] - yl[l]) + sq(zl[l + 1] - zl[l]));
+    if (dk - 1.E-3 * tdj <= 0.) continue;
+    const double pj = (xl[l + 1] - xl[l]) / dk, qj = (yl[l + 1] - yl[l]) / dk * mjj, rj = (zl[l + 1] - zl[l]) / dk;
+    const double gyx = mjj * qn * rj - rn * qj, gyy = rn * pj - pn * rj, gyz = pn * qj - qn * mjj * pj;
+    const double gy = (xpo - xl[l]) * gyx + (ypo - yl[l] * mjj) * gyy + (zpo - zl[l]) * gyz;
+    const double ddk = 2. * dk, ant = gy * ddk;
+    const double dnt = sq(rr[l + 1] + rr[l]) - dk * dk + 2. * fabs(gz) * (rr[l + 1] + rr[l]);
+    const double arg = ant / dnt, anl = rr[l + 1] + rr[l] + dk, dnl = rr[l + 1] + rr[l] - dk;
+    const double at = (fabs(gz) >= 1.E-4 * tdj) ? atan(arg) : 0.;
+    fs = fs + gy * log(anl / dnl) - 2. * fabs(gz) * at;
+  }
+  return fs * (-mjj / 12.5663706);
+}
+
+/* VNV :199-386 (infinite depth, AK0 from the period), potential part */
+static int ora_vnv(const ora_ctx *c, double t, int j, double xpo, double ypo, double zpo, double fsp, double fsm,
+                   double s[4]) {
+  const double pi = 4. * atan(1.), dpi = 2. * pi, dpi2 = 2. * sq(pi);
+  const double wr = dpi / t, ak0 = sq(wr) / c->g;
+  double fs1[2] = {0, 0}, fs2[2] = {0, 0}, psr;
+  for (int jj = 1; jj <= c->nsym + 1; ++jj) {
+    const double bx = ipow_m1(jj + 1);
+    const double rrr = sqrt(sq(xpo - c->xg[j]) + sq(ypo - c->yg[j] * bx));
+    if (field_image(c, ak0, rrr, ak0 * rrr, zpo + c->zg[j], &fs1[jj - 1], &fs2[jj - 1], &psr)) return -1;
+  }
+  const double akair = ak0 * c->aire[j], adpi2 = akair / dpi2, adpi = akair / dpi;
+  if (c->nsym == 1) { /* :338-357 */
+    s[1] = fsm - (fs1[0] - fs1[1]) * adpi2;
+    s[0] = fsp - (fs1[0] + fs1[1]) * adpi2;
+    s[3] = -(fs2[0] - fs2[1]) * adpi;
+    s[2] = -(fs2[0] + fs2[1]) * adpi;
+  } else {
+    s[0] = s[1] = fsp - fs1[0] * adpi2;
+    s[2] = s[3] = -fs2[0] * adpi;
+  }
+  return 0;
+}
+
+/* VNVF :388-908 (finite depth), potential part */
+static int ora_vnvf(const ora_ctx *c, double am0, double amh, int nexp, const double *ambda_in,
+                    const double *ar_in, int j, double xpo, double ypo, double zpo, double fsp, double fsm,
+                    double s[4]) {
+  const double pi = 4. * atan(1.), qpi = 4. * pi, eps = 0.0001, h = c->depth;
+  const double akh = amh * tanh(amh);
+  const double a = sq(amh + akh) / (h * (sq(amh) - sq(akh) + akh));
+  double ambda[32], ar[32];
+  for (int k = 0; k < nexp; ++k) {
+    ambda[k] = ambda_in[k];
+    ar[k] = ar_in[k];
+  }
+  ambda[nexp] = 0.; /* :436-437 */
+  ar[nexp] = 2.;
+  const double coe3 = -1. / (8. * sq(pi)), coe4 = -a / (8. * pi);
+  double fs1[2] = {0, 0}, fs2[2] = {0, 0};
+  for (int jj = 1; jj <= c->nsym + 1; ++jj) {
+    const double bx = ipow_m1(jj + 1);
+    const double ymjjj = bx * c->yg[j];
+    const double cof1 = a * coe3 * c->aire[j], cof2 = coe4 * c->aire[j];
+    const double rrr = sqrt(sq(xpo - c->xg[j]) + sq(ypo - ymjjj)), akr = am0 * rrr;
+    const double zzz[4] = {zpo + c->zg[j], c->zg[j] - zpo - 2 * h, zpo - c->zg[j] - 2 * h, -c->zg[j] - zpo - 4 * h};
+    double pd1z[4], pd2z[4], psr[4];
+    for (int m = 0; m < 4; ++m)
+      if (field_image(c, am0, rrr, akr, zzz[m], &pd1z[m], &pd2z[m], &psr[m])) return -1;
+    const double qtqq = pd1z[0] + pd1z[1] + pd1z[2] + pd1z[3];
+    fs1[jj - 1] = cof1 * (qtqq - psr[0] - psr[1] - psr[2] - psr[3]); /* :676 */
+    fs2[jj - 1] = cof2 * (pd2z[0] + pd2z[1] + pd2z[2] + pd2z[3]);
+    /* exponential images :787-870 */
+    const double act = -0.5 * c->aire[j] / qpi;
+    for (int ke = 0; ke <= nexp; ++ke) {
+      const double aqt = act * ar[ke];
+      const double zpg[4] = {zpo - 2. * h + h * ambda[ke] - c->zg[j], -zpo - h * ambda[ke] - c->zg[j],
+                             -zpo - 4. * h + h * ambda[ke] - c->zg[j], zpo + 2. * h - h * ambda[ke] - c->zg[j]};
+      double fts[4];
+      for (int m = 0; m < 4; ++m) {
+        const double ro = sqrt(rrr * rrr + zpg[m] * zpg[m]);
+        fts[m] = (ro > eps) ? aqt / ro : 0.;
+      }
+      if (ke == nexp) {
+        const int kk[5] = {c->m1[j], c->m2[j], c->m3[j], c->m4[j], c->m1[j]};
+        double xl[5], yl[5], zl[5];
+        for (int l = 0; l < 5; ++l) {
+          xl[l] = c->x[kk[l]];
+          yl[l] = c->y[kk[l]];
+          zl[l] = c->z[kk[l]];
+        }
+        fts[1] = ora_vvg(xl, yl, zl, jj, c->xn[j], c->yn[j], c->zn[j], c->aire[j], c->tdis[j], c->xg[j], c->yg[j],
+                         c->zg[j], xpo, ypo, -zpo);
+      }
+      fs1[jj - 1] = fs1[jj - 1] + fts[0] + fts[1] + fts[2] + fts[3];
+    }
+  }
+  if (c->nsym == 1) { /* :871-890 */
+    s[1] = fsm + fs1[0] - fs1[1];
+    s[0] = fsp + fs1[0] + fs1[1];
+    s[3] = fs2[0] - fs2[1];
+    s[2] = fs2[0] + fs2[1];
+  } else {
+    s[0] = s[1] = fsp + fs1[0];
+    s[2] = s[3] = fs2[0];
+  }
+  return 0;
+}
+
+int ora_potential(const ora_ctx *c, double t, double am0, double amh, int nexp, const double *ambda,
+                  const double *ar, double xc, double yc, double zc, const double complex *zigb,
+                  const double complex *zigs, double complex *phi) {
+  double pot1 = 0., pot2 = 0.;
+  const int infd = (c->depth == 0.) || (amh >= 20); /* :44 */
+  for (int j = 0; j < c->nfa; ++j) {
+    double fsp, fsm, s[4];
+    ora_vvv(c, infd ? 1 : 2, j, xc, yc, zc, &fsp, &fsm);
+    const int e = infd ? ora_vnv(c, t, j, xc, yc, zc, fsp, fsm, s)
+                       : ora_vnvf(c, am0, amh, nexp, ambda, ar, j, xc, yc, zc, fsp, fsm, s);
+    if (e) return e;
+    const double sp1 = s[0], sm1 = s[1], sp2 = s[2], sm2 = s[3];
+    const double s1b = creal(zigb[j]), s2b = cimag(zigb[j]);
+    if (c->nsym == 0) { /* :61-67, POT1=POT1+C-D left to right */
+      pot1 = pot1 + s1b * sp1 - s2b * sp2;
+      pot2 = pot2 + s1b * sp2 + s2b * sp1;
+    } else { /* :68-75 */
+      const double s1s = creal(zigs[j]), s2s = cimag(zigs[j]);
+      double cc = s1b * (sp1 + sm1) + s1s * (sp1 - sm1);
+      double d = s2b * (sp2 + sm2) + s2s * (sp2 - sm2);
+      pot1 = pot1 + (cc - d) * 0.5;
+      cc = s1b * (sp2 + sm2) + s1s * (sp2 - sm2);
+      d = s2b * (sp1 + sm1) + s2s * (sp1 - sm1);
+      pot2 = pot2 + (cc + d) * 0.5;
+    }
+  }
+  *phi = pot1 + I * pot2;
+  return 0;
+}
+
+int ora_potential_vec(const ora_ctx *c, double t, double am0, double amh, int nexp, const double *ambda,
+                      const double *ar, int npt, const double *pts, const double complex *zigb,
+                      const double complex *zigs, double complex *phi) {
+  for (int p = 0; p < npt; ++p)
+    if (ora_potential(c, t, am0, amh, nexp, ambda, ar, pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], zigb, zigs,
+                      &phi[p]))
+      return -1;
+  return 0;
+}

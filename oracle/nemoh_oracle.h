@@ -18,6 +18,8 @@
  *                     Rankine part vs adaptive quadrature of 1/r
  *   ora_session_*     golden Forces.dat of whole reference Solver runs;
  *                     GAUSSZ vs numpy.linalg on small matrices
+ *   ora_potential     golden COMPUTE_POTENTIAL_DOMAIN values from the
+ *                     reference; the Rankine part vs quadrature of 1/r
  *   ora_kochin        golden COMPUTE_KOCHIN values from the reference; the
  *                     one-point panel quadrature the reference keeps beside
  *                     it (COMPUTE_KOCHIN.f90:47-55) on a refined mesh
@@ -109,6 +111,17 @@ int ora_gaussz(double complex *a, int lda, int n, int m);
  * wave number kwave, reference point (xeff, yeff). */
 double complex ora_kochin(const ora_ctx *c, double kwave, double xeff, double yeff, double theta,
                           const double complex *zigb, const double complex *zigs);
+/* COMPUTE_POTENTIAL_DOMAIN (COMPUTE_POTENTIAL_DOMAIN.f90:24-82 with VVV,
+ * VNV, VNVF and VVG of COMPUTE_GREEN_FREESURFACE.f90): potential at
+ * (xc, yc, zc) of the sources zigb, zigs for period t; am0, amh, nexp,
+ * ambda, ar as SOLVE_BVP passes them (kwave, AMH; LISC's fit in finite
+ * depth).  Returns -1 where the reference stops (AKZ > -1.5e-6). */
+int ora_potential(const ora_ctx *c, double t, double am0, double amh, int nexp, const double *ambda,
+                  const double *ar, double xc, double yc, double zc, const double complex *zigb,
+                  const double complex *zigs, double complex *phi);
+int ora_potential_vec(const ora_ctx *c, double t, double am0, double amh, int nexp, const double *ambda,
+                      const double *ar, int npt, const double *pts, const double complex *zigb,
+                      const double complex *zigs, double complex *phi);
 /* ora_kochin at n angles into out[n] (for ctypes). */
 void ora_kochin_vec(const ora_ctx *c, double kwave, double xeff, double yeff, int n, const double *theta,
                     const double complex *zigb, const double complex *zigs, double complex *out);

@@ -7,6 +7,7 @@ and by bench.py's CPU-baseline leg, as the checker.  The product
 from __future__ import annotations
 
 import ctypes as C
+import math
 import subprocess
 from pathlib import Path
 
@@ -154,6 +155,41 @@ def kochin(case, kwave: float, xeff: float, yeff: float, thetas, zigb, zigs=None
     zs = np.ascontiguousarray(np.zeros_like(zb) if zigs is None else zigs, dtype=np.complex128)
     out = np.zeros(th.size, np.complex128)
     L.ora_kochin_vec(C.byref(case.ctx), float(kwave), float(xeff), float(yeff), th.size, th, zb, zs, out)
+    return out
+
+
+def potential_setup(case, period: float):
+    """(AM0, AMH, NEXP, AMBDA, AR) that the Solver hands COMPUTE_POTENTIAL_DOMAIN
+    for this period (SOLVE_BEM.f90:66-89, SOLVE_BEM_FD_DIRECT.f90:53-100):
+    AM0 = w^2/g and AMH = AM0 Depth in infinite depth (Depth = 0 or
+    w^2 Depth / g >= 20), else AMH = X0(w^2 Depth / g), AM0 = AMH / Depth and
+    LISC's expansion of AMH tanh(AMH).  LISC is ill-conditioned in its
+    first argument (one ulp of tanh can change NEXP), so the whole set-up
+    comes from the C restatement with the C library's tanh, as the
+    reference's does, never from numpy."""
+    f, fd, _ = case.freq(period)
+    if fd:
+        return f.amh / case.depth, f.amh, f.nexp, np.array(f.ambda[:]), np.array(f.ar[:])
+    w = 2 * (4 * math.atan(1.0)) / period
+    am0 = w * w / case.g
+    return am0, am0 * case.depth, 0, np.zeros(31), np.zeros(31)
+
+
+def potential(case, period: float, points, zigb, zigs=None) -> np.ndarray:
+    """COMPUTE_POTENTIAL_DOMAIN at each point (n, 3) for sources zigb, zigs,
+    with the period's set-up from potential_setup."""
+    am0, amh, nexp, lam, a = potential_setup(case, period)
+    L = lib()
+    L.ora_potential_vec.argtypes = [C.POINTER(_Ctx), C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                    C.c_int, _dp, _cp, _cp, _cp]
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    zb = np.ascontiguousarray(zigb, dtype=np.complex128)
+    zs = np.ascontiguousarray(np.zeros_like(zb) if zigs is None else zigs, dtype=np.complex128)
+    out = np.zeros(pts.shape[0], np.complex128)
+    r = L.ora_potential_vec(C.byref(case.ctx), float(period), float(am0), float(amh), int(nexp), lam, a,
+                            pts.shape[0], pts.ravel(), zb, zs, out)
+    if r:
+        raise ValueError("field point too close to the free surface (AKZ > -1.5e-6)")
     return out
 
 

@@ -14,6 +14,10 @@
 !   wavenumber n then n lines "w depth g" -> the preProcessor's k   Common/Environment.f90:45-84
 !   kochin kwave xeff yeff ntheta, ntheta angles, NFA lines "ZIGB ZIGS" (re im re im)
 !          (after mesh)                   -> HKochin(theta)          COMPUTE_KOCHIN.f90:25-153
+!   potential period npt, npt lines "x y z", NFA lines "ZIGB ZIGS" (after mesh)
+!                                         -> PHI at each point        COMPUTE_POTENTIAL_DOMAIN.f90:24-82
+!          (kwave, AMH, NEXP and LISC's AMBDA/AR set as SOLVE_BEM.f90:66-89 and
+!          SOLVE_BEM_FD_DIRECT.f90:53-100 set them)
 !   lisc   n  then n lines "akh amh"      -> NEXP, AMBDA, AR         COMPUTE_GREEN_FD.f90:1007
 !   mesh   np nfa nsym depth, np lines "x y z", nfa lines "m1 m2 m3 m4"
 !                                          -> sets COM_VAR, runs PRE_PROC_MESH, CREK
@@ -33,6 +37,7 @@ PROGRAM ref_harness
   USE MMesh
   USE M_SOLVER, ONLY: GAUSSZ
   USE MEnvironment, ONLY: TEnvironment, Wavenumber
+  USE MIDENTIFICATION, ONLY: TID
   IMPLICIT NONE
   CHARACTER(LEN=16) :: cmd
   INTEGER :: n, i, j, k, nq_out, nexp, isp, ifp, ios, iperiod
@@ -53,6 +58,10 @@ PROGRAM ref_harness
   REAL, ALLOCATABLE :: thetas(:)
   COMPLEX :: hk
   EXTERNAL :: COMPUTE_KOCHIN  ! external subroutine of COMPUTE_KOCHIN.f90
+  EXTERNAL :: COMPUTE_POTENTIAL_DOMAIN
+  TYPE(TID) :: idd
+  REAL, ALLOCATABLE :: pts(:,:)
+  COMPLEX :: phi
 
   DO
     READ(*,*,IOSTAT=ios) cmd
@@ -112,6 +121,37 @@ PROGRAM ref_harness
         WRITE(*,'(3ES26.17E3)') thetas(i), REAL(hk), AIMAG(hk)
       END DO
       DEALLOCATE(thetas)
+    CASE ('potential')
+      READ(*,*) period, n
+      ALLOCATE(pts(3, n))
+      DO i = 1, n
+        READ(*,*) pts(1,i), pts(2,i), pts(3,i)
+      END DO
+      DO i = 1, NFA
+        READ(*,*) r1, i1, r2, i2
+        ZIGB(i) = CMPLX(r1, i1)
+        ZIGS(i) = CMPLX(r2, i2)
+      END DO
+      T = period
+      w = 2.*4.*ATAN(1.)/T
+      AMBDA = 0.; AR = 0.
+      IF ((Depth == 0.) .OR. (w*w/G*Depth >= 20)) THEN
+        kw = w*w/G
+        amh = kw*Depth
+        nexp = 0
+        WRITE(*,'(A,2ES26.17E3,I6)') 'INFD', kw, amh, nexp
+      ELSE
+        akh = w*w*Depth/G
+        amh = X0(akh)
+        kw = amh/Depth
+        CALL LISC(amh*TANH(amh), amh, nexp)
+        WRITE(*,'(A,2ES26.17E3,I6)') 'FD', kw, amh, nexp
+      END IF
+      DO i = 1, n
+        CALL COMPUTE_POTENTIAL_DOMAIN(idd, phi, pts(1,i), pts(2,i), pts(3,i), kw, amh, nexp)
+        WRITE(*,'(5ES26.17E3)') pts(1,i), pts(2,i), pts(3,i), REAL(phi), AIMAG(phi)
+      END DO
+      DEALLOCATE(pts)
     CASE ('lisc')
       READ(*,*) n
       DO i = 1, n

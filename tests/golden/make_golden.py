@@ -377,6 +377,47 @@ def kochin_fixture() -> None:
     print(f"kochin: {len(cases)} cases")
 
 
+def potential_fixture() -> None:
+    """COMPUTE_POTENTIAL_DOMAIN (COMPUTE_POTENTIAL_DOMAIN.f90:24-82 with
+    VVV/VNV/VNVF/VVG of COMPUTE_GREEN_FREESURFACE.f90) of the reference
+    build on seeded random sources: a half cylinder (y-symmetry) and a
+    barge, infinite depth and h = 50 m (finite depth and one k h >= 20
+    frequency), at free-surface points (z = 0) far and near the hull and at
+    submerged points, some within a few panel sizes of a panel."""
+    rng = np.random.default_rng(31415)
+    cases = []
+    cyl = nmesh.cylinder(10.0, 5.0, 16, 4, 4, half=True)
+    barge = nmesh.box(100.0, 20.0, 10.0, 18, 4, 2)
+    pts = {"cyl": [(11.0, 2.0, 0.0), (14.0, 9.0, 0.0), (40.0, 25.0, 0.0), (-12.0, 3.0, 0.0), (0.0, 10.8, -2.0),
+                   (25.0, -30.0, -4.0), (3.0, 3.0, -6.5), (10.6, 0.5, -4.9)],
+           "barge": [(51.0, 0.0, 0.0), (0.0, 10.7, 0.0), (80.0, 40.0, 0.0), (-55.0, -12.0, 0.0), (30.0, 10.9, -3.0),
+                     (-120.0, 60.0, -20.0), (0.0, 0.0, -10.8), (49.0, -9.0, -10.5)]}
+    for name, m in (("cyl", cyl), ("barge", barge)):
+        for depth, periods in ((0.0, (4.0, 12.0)), (50.0, (2.8, 10.0, 30.0))):
+            for period in periods:
+                zb = rng.standard_normal(m.npanels) + 1j * rng.standard_normal(m.npanels)
+                zs = (rng.standard_normal(m.npanels) + 1j * rng.standard_normal(m.npanels)) * m.isym
+                p3 = np.array(pts[name], dtype=np.float64)
+                cmd = f"mesh\n{m.npoints} {m.npanels} {m.isym} {depth:.17e}\n"
+                cmd += "".join(f"{x:.17e} {y:.17e} {z:.17e}\n" for x, y, z in m.vertices)
+                cmd += "".join(" ".join(str(v + 1) for v in p) + "\n" for p in m.panels)
+                cmd += f"potential\n{period:.17e} {len(p3)}\n" + "".join(f"{a:.17e} {b:.17e} {c:.17e}\n" for a, b, c in p3)
+                cmd += "".join(f"{a.real:.17e} {a.imag:.17e} {b.real:.17e} {b.imag:.17e}\n" for a, b in zip(zb, zs))
+                lines = harness(cmd + "end\n")
+                head = lines[-len(p3) - 1].split()
+                v = np.array([[float(t) for t in ln.split()] for ln in lines[-len(p3):]])
+                cases.append(dict(name=name, vertices=m.vertices, panels=m.panels, isym=m.isym, depth=depth,
+                                  period=period, regime=head[0], kwave=float(head[1]), amh=float(head[2]),
+                                  nexp=int(head[3]), points=p3, zigb=zb, zigs=zs, phi=v[:, 3] + 1j * v[:, 4]))
+    out = {}
+    for k, c in enumerate(cases):
+        for key, val in c.items():
+            out[f"c{k}_{key}"] = val
+    out["ncases"] = len(cases)
+    np.savez_compressed(OUT / "fn_potential.npz", **out)
+    print(f"potential: {len(cases)} cases, regimes {[c['regime'] for c in cases]}")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -391,6 +432,8 @@ def main() -> None:
             wavenumber_fixture()
         if a.only in ("", "fn", "kochin"):
             kochin_fixture()
+        if a.only in ("", "fn", "potential"):
+            potential_fixture()
         if a.only in ("", "pairs"):
             pair_fixtures()
         if a.only in ("", "cases"):

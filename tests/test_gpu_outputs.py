@@ -1,10 +1,12 @@
-"""Outputs computed from the sources on the device: the Kochin function.
+"""Outputs computed from the sources on the device: the Kochin function
+and the potential at field points.
 
-The device's Kochin function (COMPUTE_KOCHIN.f90:25-153) of the sources of
-a real solve equals the oracle's on the same sources (the oracle is pinned
-to the reference build's COMPUTE_KOCHIN), for a y-symmetric case in
-infinite depth and a finite-depth case with an off-origin reference point,
-and per problem after an all-problems solve.
+The device's Kochin function (COMPUTE_KOCHIN.f90:25-153) and field
+potential (COMPUTE_POTENTIAL_DOMAIN.f90:24-82) of the sources of a real
+solve equal the oracle's on the same sources (the oracle is pinned to the
+reference build's routines), for a y-symmetric case in infinite depth and a
+barge at h = 50 m (finite depth, and an infinite-depth frequency at finite
+depth), and per problem after an all-problems solve.
 """
 import numpy as np
 import pytest
@@ -66,4 +68,80 @@ def test_kochin_needs_a_solve():
     with pytest.raises(core.NemohError) as e:
         c.kochin(np.array([0.0]), nrhs=1)
     assert e.value.code == core.ERR_ARG
+    c.close()
+
+
+def _field_points(vertices):
+    """Free-surface points (z = 0) on a ring and a line beyond the hull, as
+    the free-surface mesh gives them, and submerged points near the hull."""
+    v = np.asarray(vertices)
+    rx, ry, zmin = np.abs(v[:, 0]).max(), np.abs(v[:, 1]).max(), v[:, 2].min()
+    t = np.linspace(0.0, 2.0 * np.pi, 24, endpoint=False)
+    ring = np.stack([1.3 * rx * np.cos(t), 1.3 * max(ry, 0.3 * rx) * np.sin(t), 0.0 * t], axis=1)
+    line = np.stack([np.linspace(-4 * rx, 4 * rx, 16), np.full(16, 2.5 * max(ry, 1.0)), np.zeros(16)], axis=1)
+    sub = np.array([[1.05 * rx, 0.1 * ry, 0.5 * zmin], [0.0, 1.1 * ry + 0.5, 0.3 * zmin],
+                    [-2.0 * rx, -ry, 1.2 * zmin], [0.3 * rx, 0.2 * ry, 1.05 * zmin]])
+    return np.concatenate([ring, line, sub])
+
+
+@pytest.mark.parametrize("name", ["Cylinder", "barge_fd"])
+def test_potential_matches_oracle(name):
+    d = load_golden(f"case_{name}.npz")
+    isym, depth, g = int(d["isym"]), float(d["depth"]), float(d["g"])
+    c = core.Core(d["vertices"], d["panels"], isym, depth, float(d["rho"]), g, tables=_tables())
+    oc = po.Case(d["vertices"], d["panels"], isym, depth=depth)
+    pts = _field_points(d["vertices"])
+    for w in np.unique(d["omega"]):
+        period = 2 * np.pi / w
+        cols = np.where(d["omega"] == w)[0]
+        f = c.freq_setup(period)
+        per_problem, refs = [], []
+        for j in cols:
+            zigb, zigs, _, _ = c.solve(period, f, d["nvel"][:, j])
+            print(f"{name} w={w:.3f} problem {j}: device potential at {len(pts)} points", flush=True)
+            per_problem.append(c.potential(pts)[:, 0])
+            refs.append(po.potential(oc, period, pts, zigb, zigs if isym else None))
+        # normwise over the frequency's problems (a mode that moves no water
+        # has rounding-noise sources on both sides)
+        scale = max(np.abs(r).max() for r in refs)
+        err = max(np.abs(p - r).max() for p, r in zip(per_problem, refs)) / scale
+        print(f"{name} w={w:.3f} regime {'FD' if f.finite_depth else 'INFD'}: device vs oracle {err:.2e}")
+        assert err < 1e-12, (name, w, err)
+        c.solve_frequency(period, d["nvel"][:, cols])
+        pall = c.potential(pts)
+        assert pall.shape == (len(pts), cols.size)
+        ref = np.stack(per_problem, axis=1)
+        assert np.abs(pall - ref).max() <= 1e-11 * np.abs(ref).max()
+    c.close()
+
+
+def test_potential_at_collocation_points_is_body_potential():
+    """Infinite depth: at the collocation points the device's field
+    potential is the body potential ZPB of its own solve, to rounding."""
+    d = load_golden("case_Cylinder.npz")
+    isym = int(d["isym"])
+    c = core.Core(d["vertices"], d["panels"], isym, 0.0, float(d["rho"]), float(d["g"]), tables=_tables())
+    oc = po.Case(d["vertices"], d["panels"], isym, depth=0.0)
+    w = float(d["omega"][-1])
+    f = c.freq_setup(2 * np.pi / w)
+    _, _, zpb, _ = c.solve(2 * np.pi / w, f, d["nvel"][:, -1])
+    geo = oc.geo
+    phi = c.potential(np.stack([geo["xg"], geo["yg"], geo["zg"]], axis=1))[:, 0]
+    assert np.abs(phi - zpb).max() <= 1e-12 * np.abs(zpb).max()
+    c.close()
+
+
+def test_potential_errors():
+    d = load_golden("case_Cylinder.npz")
+    c = core.Core(d["vertices"], d["panels"], int(d["isym"]), 0.0, tables=_tables())
+    with pytest.raises(core.NemohError) as e:
+        c.potential(np.zeros((1, 3)), nrhs=1)
+    assert e.value.code == core.ERR_ARG  # no solve yet
+    w = float(d["omega"][-1])
+    f = c.freq_setup(2 * np.pi / w)
+    c.solve(2 * np.pi / w, f, d["nvel"][:, 0])
+    with pytest.raises(core.NemohError) as e:  # above every panel: AKZ > -1.5e-6
+        c.potential(np.array([[0.0, 0.0, 0.0], [20.0, 0.0, 5.0]]))
+    assert e.value.code == core.ERR_ARG and "AKZ" in str(e.value)
+    assert c.potential(np.array([[20.0, 0.0, 0.0]])).shape == (1, 1)  # the handle still works
     c.close()

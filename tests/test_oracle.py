@@ -364,3 +364,53 @@ def test_kochin_edge_integral_converges_to_panel_quadrature():
                                               geo["aire"]) for t in th])
         errs.append(np.abs(h - h1).max() / np.abs(h).max())
     assert errs[2] < 2e-3 and 3.0 < errs[0] / errs[1] < 5.0 and 3.0 < errs[1] / errs[2] < 5.0, errs
+
+
+# -- field potential (COMPUTE_POTENTIAL_DOMAIN.f90:24-82) -------------------
+def test_potential_matches_reference():
+    """ora_potential against COMPUTE_POTENTIAL_DOMAIN of the reference build:
+    half cylinder (y-images) and barge, infinite depth, h = 50 m in finite
+    depth (k h = 2.1 and 0.49) and an infinite-depth frequency at finite
+    depth, free-surface and submerged points near and far, random sources.
+    The frequency set-up (AMH, NEXP and LISC's expansion) comes from the C
+    restatement: LISC is ill-conditioned in AMH tanh(AMH), and numpy's tanh,
+    one ulp off the C library's, moves the T = 10 s results by 1e-11."""
+    d = load_golden("fn_potential.npz")
+    for k in range(int(d["ncases"])):
+        def g(key):
+            return d[f"c{k}_{key}"]
+        case = po.Case(g("vertices"), g("panels"), int(g("isym")), depth=float(g("depth")))
+        am0, amh, nexp, _, _ = po.potential_setup(case, float(g("period")))
+        assert (am0, amh, nexp) == (float(g("kwave")), float(g("amh")), int(g("nexp"))), k
+        phi = po.potential(case, float(g("period")), g("points"), g("zigb"), g("zigs"))
+        assert np.abs(phi - g("phi")).max() <= 1e-13 * np.abs(g("phi")).max(), k
+
+
+@pytest.mark.parametrize("half", [True, False])
+def test_potential_at_collocation_points_is_body_potential(half):
+    """The field-point routines (VVV + VNV / VNVF) and the collocation-point
+    routines of the solve (VAV + VNS) are separate code for the same Green
+    function: at the collocation points the potential of the solved sources
+    is the solve's body potential ZPB, to rounding in infinite depth (the
+    same table formulas) and to the accuracy of the two approximations in
+    finite depth (VNSFD and VNVF treat the last exponential image
+    differently).  A wrong image, sign, symmetry combination or source
+    part fails this."""
+    m = nmesh.cylinder(10.0, 5.0, 16, 4, 4, half=half)
+    for depth, period, tol in ((0.0, 6.0, 1e-13), (0.0, 12.0, 1e-13), (40.0, 8.0, 1e-2), (40.0, 20.0, 1e-2)):
+        case = po.Case(m.vertices, m.panels, m.isym, depth=depth)
+        geo = case.geo
+        nvel = np.concatenate([geo["zn"] + 0.3j * geo["xn"]] + [geo["zn"] - 0.3j * geo["xn"]] * m.isym)
+        _, zb, zs, zpb, _ = po.Session(case).solve_bvp(period, nvel)
+        pts = np.stack([geo["xg"], geo["yg"], geo["zg"]], axis=1)
+        phi = po.potential(case, period, pts, zb, zs if m.isym else None)
+        assert np.abs(phi - zpb).max() <= tol * np.abs(zpb).max(), (depth, period)
+
+
+def test_potential_rejects_points_at_the_free_surface_level():
+    """AKZ > -1.5e-6 for some panel: the reference stops, the oracle raises."""
+    m = nmesh.cylinder(10.0, 5.0, 16, 4, 4, half=False)
+    case = po.Case(m.vertices, m.panels, 0, depth=0.0)
+    zb = np.ones(m.npanels, np.complex128)
+    with pytest.raises(ValueError):
+        po.potential(case, 6.0, np.array([[20.0, 0.0, 5.0]]), zb)
