@@ -14,7 +14,9 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libnemoh_core.so"
-SOURCES = ["core.hip", "kernels.hip", "lu.hip", "bodycond.hip", "potential.hip", "host_math.cpp", "server.cpp"]
+SOURCES = ["core.hip", "kernels.hip", "lu.hip", "bodycond.hip", "potential.hip", "host_math.cpp", "server.cpp",
+           "fortran_io.cpp", "case_io.cpp", "run_case.cpp"]
+EXE = PKG / "nemoh_run"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NEMOH_OFFLOAD_ARCH", "gfx950")
 
@@ -48,6 +50,14 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the command-line runner over the library (rpath: beside it)
+    tmp = EXE.with_suffix(".tmp")
+    cmd = [HIPCC, *COMMON, "-x", "c++", str(CSRC / "nemoh_run.cpp"), "-x", "none", "-o", str(tmp), f"-L{PKG}",
+           "-lnemoh_core", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"nemoh_run link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, EXE)
     return LIB
 
 

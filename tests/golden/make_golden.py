@@ -418,6 +418,49 @@ def potential_fixture() -> None:
     print(f"potential: {len(cases)} cases, regimes {[c['regime'] for c in cases]}")
 
 
+FILE_CASES = {
+    # name: (input files, Nemoh.cal line edits, free-surface / pressure problems kept)
+    "Cylinder": (("Cylinder.dat",), {32: "0\t0.\t180.\t! Kochin", 33: "0\t10\t100.\t100.\t! FS"}, ()),
+    "NonSymmetrical": (("NonSymmetrical.dat",), {32: "0\t0.\t180.\t! Kochin"}, ()),
+    "2Bodies": (("Cube.dat", "Rotated_cube.dat"), {45: "4\t0.1\t2.0\t! Number of wave frequencies",
+                                                   50: "0\t0.\t180.\t! Kochin"}, ()),
+    # unmodified: Kochin function (181 angles), free surface (50 x 50) and
+    # pressure output switched on by the reference's own input
+    "Postprocessing": (("Rotated_cube.dat",), {}, (1, 7)),
+}
+
+
+def file_fixtures(ref: Path, tmp: Path) -> None:
+    """The reference pipeline's files, as text, for the file-format layer:
+    the inputs (Nemoh.cal, mesh files) and every file the preProcessor and
+    the Solver write (the free-surface and pressure files of the problems
+    listed only, to keep the fixture small).  Stored as files_<name>.npz,
+    one entry per file (its path with '/' as '|')."""
+    for name, (inputs, edits, keep) in FILE_CASES.items():
+        src = ref / "Verification" / name
+        work = tmp / ("files_" + name)
+        work.mkdir()
+        for f in inputs + ("input.txt",):
+            shutil.copy(src / f, work / f)
+        cal = (src / "Nemoh.cal").read_text(errors="replace")
+        (work / "Nemoh.cal").write_text(edit_cal(cal, edits) if edits else cal)
+        run_case(work, name)
+        out = {}
+        for f in ("Nemoh.cal",) + inputs:
+            out["in|" + f] = np.frombuffer((work / f).read_bytes(), dtype=np.uint8)
+        files = sorted(p for p in work.rglob("*.dat") if p.parent != work or p.name == "Normalvelocities.dat")
+        files += sorted(work.rglob("*.tec"))
+        for p in files:
+            rel = p.relative_to(work).as_posix()
+            m = re.match(r"results/(freesurface|pressure)\.\s*(\d+)\.dat", rel)
+            if m and int(m.group(2)) not in keep:
+                continue
+            out["out|" + rel.replace("/", "|")] = np.frombuffer(p.read_bytes(), dtype=np.uint8)
+        np.savez_compressed(OUT / f"files_{name}.npz", **out)
+        size = (OUT / f"files_{name}.npz").stat().st_size
+        print(f"files {name}: {len(out)} files, {size / 1024:.0f} KiB")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -434,6 +477,8 @@ def main() -> None:
             kochin_fixture()
         if a.only in ("", "fn", "potential"):
             potential_fixture()
+        if a.only in ("", "files"):
+            file_fixtures(Path(a.ref), tmp)
         if a.only in ("", "pairs"):
             pair_fixtures()
         if a.only in ("", "cases"):

@@ -17,6 +17,11 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in sorted(declared) if not hasattr(L, s)]
     assert not missing, missing
     assert declared == set(core.EXPORTS)
+    case_header = (ROOT / "include" / "nemoh_case.h").read_text()
+    declared = set(re.findall(r"\b(nemoh_(?:case|format)_\w+)\s*\(", case_header))
+    assert declared == {"nemoh_case_run", "nemoh_case_write_mesh_files", "nemoh_format_list", "nemoh_format_e"}
+    missing = [s for s in sorted(declared) if not hasattr(L, s)]
+    assert not missing, missing
 
 
 def test_prony_fit_matches_reference_bitwise():
