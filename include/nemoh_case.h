@@ -19,6 +19,11 @@
  *     solves every problem on the device (all problems of one frequency per
  *     call) and writes results/Forces.dat and, where the problems ask for
  *     them, Kochin.<n>.dat, freesurface.<n>.dat and pressure.<n>.dat.
+ *   stages & NEMOH_STAGE_POST:  the postProcessor (postProcessor/Main.f90:
+ *     21-101, Common/Results.f90:132-251, postProcessor/IRF.f90:66-151) on
+ *     the Solver's files: results/RadiationCoefficients.tec,
+ *     DiffractionForce.tec, ExcitationForce.tec, CA.dat, CM.dat, Fe.dat and,
+ *     when Nemoh.cal's IRF switch is 1, IRF.tec.  Host work only.
  *
  * precision: NEMOH_FP64 or NEMOH_MIXED (include/nemoh_core.h).  Returns
  * NEMOH_OK or an error code, with a message in msg (msglen bytes). */
@@ -31,7 +36,7 @@
 extern "C" {
 #endif
 
-enum { NEMOH_STAGE_PRE = 1, NEMOH_STAGE_SOLVE = 2 };
+enum { NEMOH_STAGE_PRE = 1, NEMOH_STAGE_SOLVE = 2, NEMOH_STAGE_POST = 4 };
 
 int nemoh_case_run(const char *case_dir, int stages, int precision, char *msg, int msglen);
 

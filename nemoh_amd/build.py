@@ -15,7 +15,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libnemoh_core.so"
 SOURCES = ["core.hip", "kernels.hip", "lu.hip", "bodycond.hip", "potential.hip", "host_math.cpp", "server.cpp",
-           "fortran_io.cpp", "case_io.cpp", "run_case.cpp"]
+           "fortran_io.cpp", "case_io.cpp", "postproc.cpp", "run_case.cpp"]
 EXE = PKG / "nemoh_run"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NEMOH_OFFLOAD_ARCH", "gfx950")

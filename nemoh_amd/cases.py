@@ -1,7 +1,7 @@
 """Whole-case runs over the reference's files (include/nemoh_case.h).
 
-``run_case(dir)`` does what the reference's preProcessor and Solver
-executables do for a case directory (Nemoh.cal + body meshes), with the
+``run_case(dir)`` does what the reference's preProcessor, Solver and
+postProcessor executables do for a case directory (Nemoh.cal + body meshes), with the
 per-frequency work on the GPU, and writes the same files in the same text
 formats; ``nemoh_amd/nemoh_run`` is the same as a command.
 """
@@ -12,7 +12,7 @@ from pathlib import Path
 
 from .core import NemohError, load
 
-PRE, SOLVE = 1, 2
+PRE, SOLVE, POST = 1, 2, 4
 FP64, MIXED = 0, 1
 
 
@@ -25,8 +25,9 @@ def _lib():
     return L
 
 
-def run_case(case_dir, stages: int = PRE | SOLVE, precision: int = FP64) -> None:
-    """Runs the preProcessor and/or Solver stage on a case directory."""
+def run_case(case_dir, stages: int = PRE | SOLVE | POST, precision: int = FP64) -> None:
+    """Runs the preProcessor, Solver and/or postProcessor stage on a case
+    directory (the postProcessor stage needs no GPU)."""
     msg = C.create_string_buffer(1024)
     r = _lib().nemoh_case_run(str(Path(case_dir)).encode(), int(stages), int(precision), msg, len(msg))
     if r:

@@ -121,5 +121,11 @@ std::string freesurface_text(const FreeSurfaceMesh& fs, const cplx* eta);  // OU
 std::string pressure_text(const SolverMesh& m, const cplx* pressure);     // OUTPUT.f90:65-101
 std::string problem_tag(int number);  // '(I5)' of the problem number
 
+// The postProcessor (postProcessor/Main.f90:21-101) on a case directory
+// holding the Solver's results: RadiationCoefficients.tec,
+// DiffractionForce.tec, ExcitationForce.tec, CA.dat, CM.dat, Fe.dat and,
+// when Nemoh.cal asks for it, IRF.tec.
+void postprocess(const std::string& dir);
+
 }  // namespace casefiles
 }  // namespace nemoh

@@ -104,6 +104,149 @@ std::string edit_I(long long v, int w) {
   return std::string(w - s.size(), ' ') + s;
 }
 
+namespace {
+
+// A parsed format: a flat list of descriptors with group markers.
+struct Desc {
+  char op;  // 'X' blanks, 'A', 'I', 'E', 'F', '(' group start, ')' group end
+  int w = 0, d = 0, rep = 1;
+  int match = -1;  // for '(' / ')': index of the partner
+};
+
+std::vector<Desc> parse_format(const std::string& f) {
+  std::vector<Desc> out;
+  std::vector<int> open;
+  size_t i = 0;
+  auto num = [&](int dflt) {
+    if (i >= f.size() || !isdigit((unsigned char)f[i])) return dflt;
+    int v = 0;
+    while (i < f.size() && isdigit((unsigned char)f[i])) v = 10 * v + (f[i++] - '0');
+    return v;
+  };
+  while (i < f.size()) {
+    const char c = f[i];
+    if (c == ' ' || c == ',') {
+      ++i;
+      continue;
+    }
+    if (c == ')') {
+      ++i;
+      if (open.empty()) throw FormatError("unbalanced format " + f);
+      Desc d{')'};
+      d.match = open.back();
+      out[open.back()].match = (int)out.size();
+      open.pop_back();
+      out.push_back(d);
+      continue;
+    }
+    const int rep = num(1);
+    if (i >= f.size()) throw FormatError("bad format " + f);
+    const char op = (char)toupper((unsigned char)f[i++]);
+    Desc d{op};
+    d.rep = rep;
+    if (op == '(') {
+      open.push_back((int)out.size());
+    } else if (op == 'X') {
+      d.w = rep;
+      d.rep = 1;
+    } else if (op == 'A') {
+      d.w = num(0);
+    } else if (op == 'I') {
+      d.w = num(0);
+    } else if (op == 'E' || op == 'F') {
+      d.w = num(0);
+      if (i < f.size() && f[i] == '.') {
+        ++i;
+        d.d = num(0);
+      }
+    } else {
+      throw FormatError(std::string("unsupported descriptor '") + op + "' in " + f);
+    }
+    out.push_back(d);
+  }
+  if (!open.empty()) throw FormatError("unbalanced format " + f);
+  return out;
+}
+
+}  // namespace
+
+std::string format_write(const std::string& format, const std::vector<FmtItem>& items) {
+  const std::vector<Desc> prog = parse_format(format);
+  if (prog.empty() || prog.front().op != '(' || prog.front().match != (int)prog.size() - 1)
+    throw FormatError("a format is one parenthesised list: " + format);
+  // the reversion point: the last group directly inside the outer parentheses
+  int revert = 1;
+  for (int k = 1; k < (int)prog.size() - 1; ++k)
+    if (prog[k].op == '(') {
+      revert = k;
+      k = prog[k].match;
+    }
+  std::string out, rec;
+  int pending = 0;
+  size_t next = 0;
+  auto put = [&](const std::string& s) {
+    rec.append(pending, ' ');
+    pending = 0;
+    rec += s;
+  };
+  // group repeat counters (one level of nesting)
+  int pc = 1, left = 0;
+  bool started = false;
+  for (;;) {
+    const Desc& d = prog[pc];
+    if (d.op == ')') {
+      if (pc == (int)prog.size() - 1) {  // the outer right parenthesis
+        if (next >= items.size()) break;
+        out += rec + "\n";
+        rec.clear();
+        pending = 0;
+        pc = revert;
+        started = false;
+        continue;
+      }
+      if (--left > 0) {
+        pc = d.match + 1;
+      } else {
+        ++pc;
+        started = false;
+      }
+      continue;
+    }
+    if (d.op == '(') {
+      left = d.rep;
+      started = true;
+      ++pc;
+      continue;
+    }
+    if (d.op == 'X') {
+      pending += d.w;
+      ++pc;
+      continue;
+    }
+    // a data descriptor, repeated d.rep times
+    for (int r = 0; r < d.rep; ++r) {
+      if (next >= items.size()) goto done;
+      const FmtItem& it = items[next++];
+      if (d.op == 'A') {
+        if (it.kind != FmtItem::Text) throw FormatError("A needs a character item in " + format);
+        put(d.w > 0 ? (it.s.size() >= (size_t)d.w ? it.s.substr(0, d.w) : std::string(d.w - it.s.size(), ' ') + it.s)
+                    : it.s);
+      } else if (d.op == 'I') {
+        if (it.kind != FmtItem::Int) throw FormatError("I needs an integer item in " + format);
+        put(edit_I(it.i, d.w));
+      } else {
+        if (it.kind != FmtItem::Real) throw FormatError("E/F need a real item in " + format);
+        put(d.op == 'E' ? edit_E(it.r, d.w, d.d) : edit_F(it.r, d.w, d.d));
+      }
+    }
+    ++pc;
+  }
+done:
+  (void)started;
+  out += rec + "\n";
+  return out;
+}
+
 Writer& Writer::item(const std::string& s) {
   const int len = 1 + (int)s.size();
   if (col_ > 0 && col_ + len > 79) {

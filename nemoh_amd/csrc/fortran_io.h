@@ -34,6 +34,26 @@ std::string edit_E(double x, int w, int d);      // Ew.d
 std::string edit_F(double x, int w, int d);      // Fw.d
 std::string edit_I(long long v, int w);          // Iw
 
+// One formatted WRITE: the items under a format such as
+// "(F7.4,6(X,E13.6),6(X,F7.2))" (descriptors nX, A, Iw, Ew.d, Fw.d and
+// repeated groups one level deep).  Blanks from X only appear before a
+// later character of the record; when the items outlast the format a new
+// record starts and the format reverts to its last top-level group (or to
+// its start), as the Fortran runtime does.  Returns the records, each
+// ending in a newline.
+struct FmtItem {
+  enum Kind { Real, Int, Text } kind;
+  double r = 0;
+  long long i = 0;
+  std::string s;
+  FmtItem(double v) : kind(Real), r(v) {}
+  FmtItem(int v) : kind(Int), i(v) {}
+  FmtItem(long long v) : kind(Int), i(v) {}
+  FmtItem(const char* v) : kind(Text), s(v) {}
+  FmtItem(const std::string& v) : kind(Text), s(v) {}
+};
+std::string format_write(const std::string& format, const std::vector<FmtItem>& items);
+
 // A file written record by record.
 class Writer {
  public:
@@ -74,6 +94,11 @@ class Reader {
   Reader(const std::string& name, const std::string& text);
   // READ(u,*) with no items: skips one record
   void skip();
+  // the text of the current record (a formatted '(A)' read without moving)
+  const std::string& peek_record() const {
+    if (r_ >= rec_.size()) fail("end of file");
+    return rec_[r_];
+  }
   // one READ(u,*) statement: begin, items, end
   Reader& begin();
   long long integer();

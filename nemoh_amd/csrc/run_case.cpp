@@ -282,12 +282,13 @@ extern "C" {
 
 int nemoh_case_run(const char* case_dir, int stages, int precision, char* msg, int msglen) {
   if (msg && msglen > 0) msg[0] = '\0';
-  if (!case_dir || !(stages & (NEMOH_STAGE_PRE | NEMOH_STAGE_SOLVE)) ||
+  if (!case_dir || !(stages & (NEMOH_STAGE_PRE | NEMOH_STAGE_SOLVE | NEMOH_STAGE_POST)) ||
       (precision != NEMOH_FP64 && precision != NEMOH_MIXED))
     return report(NEMOH_ERR_ARG, "bad arguments", msg, msglen);
   try {
     if (stages & NEMOH_STAGE_PRE) pre_stage(case_dir);
     if (stages & NEMOH_STAGE_SOLVE) solve_stage(case_dir, precision);
+    if (stages & NEMOH_STAGE_POST) postprocess(case_dir);
   } catch (const CaseError& e) {
     return report(e.code, e.what(), msg, msglen);
   } catch (const FormatError& e) {

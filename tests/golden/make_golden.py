@@ -432,8 +432,8 @@ FILE_CASES = {
 
 def file_fixtures(ref: Path, tmp: Path) -> None:
     """The reference pipeline's files, as text, for the file-format layer:
-    the inputs (Nemoh.cal, mesh files) and every file the preProcessor and
-    the Solver write (the free-surface and pressure files of the problems
+    the inputs (Nemoh.cal, mesh files) and every file the preProcessor, the
+    Solver and the postProcessor write (the free-surface and pressure files of the problems
     listed only, to keep the fixture small).  Stored as files_<name>.npz,
     one entry per file (its path with '/' as '|')."""
     for name, (inputs, edits, keep) in FILE_CASES.items():
@@ -445,6 +445,9 @@ def file_fixtures(ref: Path, tmp: Path) -> None:
         cal = (src / "Nemoh.cal").read_text(errors="replace")
         (work / "Nemoh.cal").write_text(edit_cal(cal, edits) if edits else cal)
         run_case(work, name)
+        r = subprocess.run([str(REF_BIN / "postproc")], cwd=work, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name}: postproc failed\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
         out = {}
         for f in ("Nemoh.cal",) + inputs:
             out["in|" + f] = np.frombuffer((work / f).read_bytes(), dtype=np.uint8)

@@ -184,3 +184,28 @@ def test_list_directed_real_rules():
     assert cases.format_e(1e100, 14, 7) == " 0.1000000+101"
     assert cases.format_e(9.99999996, 14, 7) == " 0.1000000E+02"
     assert cases.format_e(-0.0, 14, 7) == "-0.0000000E+00"
+
+
+POST_FILES = ("results/RadiationCoefficients.tec", "results/DiffractionForce.tec", "results/ExcitationForce.tec",
+              "results/CA.dat", "results/CM.dat", "results/Fe.dat", "results/IRF.tec")
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_postprocessor_matches_reference(tmp_path, name):
+    """The postProcessor stage (host only) on the reference Solver's own
+    Forces.dat, index.dat and FKForce.tec writes what the reference
+    postProcessor wrote from them: radiation coefficients, diffraction and
+    excitation forces, the Aquaplus-format CA/CM/Fe files (format reversion
+    included: 2Bodies has 12 forces for 6-item groups) and the IRFs."""
+    work, outs = stage_inputs(tmp_path, name)
+    (work / "results").mkdir()
+    for rel in ("results/Forces.dat", "results/index.dat", "results/FKForce.tec"):
+        (work / rel).write_bytes(outs[rel])
+    cases.run_case(work, stages=cases.POST)
+    checked = 0
+    for rel in POST_FILES:
+        if rel in outs:
+            assert_same_records((work / rel).read_bytes().decode("latin-1"), outs[rel].decode("latin-1"), (name, rel))
+            checked += 1
+    assert checked >= 6
+    assert ("results/IRF.tec" in outs) == (work / "results" / "IRF.tec").exists()

@@ -2,7 +2,8 @@
 the reference preProcessor and Solver wrote for the same inputs (fixtures
 files_<case>.npz of four reference Verification cases).
 
-* preProcessor + Solver stages (nemoh_case_run): the host-side mesh files
+* preProcessor + Solver + postProcessor stages (nemoh_case_run): the
+  host-side mesh files
   record for record; Normalvelocities.dat, Integration.dat and FKForce.dat
   within 1e-12 of each problem's scale; Forces.dat within 1e-10 (FP64) and
   1e-6 (mixed) normwise per problem; for the case that asks for them
@@ -146,6 +147,19 @@ def test_case_pipeline_matches_reference_files(tmp_path, name):
             print(f"{name} {rel}: elevation vs the reference's {err:.2e}")
             assert err < 2e-6, rel
             assert lg[2 + npts:] == lr[2 + npts:]
+    # the postProcessor stage ran on these files (its arithmetic is checked
+    # record for record on the reference's own files without a GPU): the
+    # radiation coefficients agree to the 7 digits they keep
+    rg = (work / "results" / "RadiationCoefficients.tec").read_text().split("\n")
+    rr = outs["results/RadiationCoefficients.tec"].decode().split("\n")
+    assert len(rg) == len(rr)
+    for a, b in zip(rg, rr):
+        if a != b:
+            va, vb = np.array(a.split(), float), np.array(b.split(), float)
+            assert va.size == vb.size and np.abs(va - vb).max() <= 2e-6 * np.abs(vb).max(), (a, b)
+    for rel in ("results/DiffractionForce.tec", "results/ExcitationForce.tec", "results/CA.dat", "results/CM.dat",
+                "results/Fe.dat"):
+        assert (work / rel).exists(), rel
     if kochin:
         scale = max(np.abs(hr).max() for _, hr in kochin.values())
         err = max(np.abs(hg - hr).max() for hg, hr in kochin.values()) / scale
