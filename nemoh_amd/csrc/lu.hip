@@ -39,12 +39,16 @@ constexpr int kTile = kNB * kNB;  // elements of one 64x64 block
 // 64x64x64 complex tile product on MFMA.  Four waves each own a 32x32
 // quadrant = 2x2 MFMA 16x16 tiles; the complex product is four real MFMAs.
 // K streams through LDS in chunks of 16, double buffered (the global loads of
-// chunk c+1 are in flight while chunk c feeds the MFMAs); LDS keeps the
-// operands k-major in separate real / imaginary planes.
+// chunk c+1 are in flight while chunk c feeds the MFMAs), in separate real /
+// imaginary planes: A row-major with a row stride of 20 elements, B k-major
+// with 72, so that the MFMA operand reads (lane (li, lk): row li, k lk of
+// A; k lk, column li of B) touch each LDS bank at most twice and the
+// staging writes at most three times.
 
 template <typename T>
 struct GemmLds {
-  T ar[2][16][64], ai[2][16][64], br[2][16][64], bi[2][16][64];
+  T ar[2][64][20], ai[2][64][20];  // [buf][row][k]
+  T br[2][16][72], bi[2][16][72];  // [buf][k][col]
 };
 
 template <typename T>
@@ -111,9 +115,13 @@ __device__ __forceinline__ void acc_store(const Acc64<T>& acc, C2<T>* __restrict
 }
 
 // acc += (kNegA ? -A : A) B.  All global reads of both operands complete
-// before the last barrier, so the caller may store over an operand.
-template <typename T, bool kNegA>
-__device__ __forceinline__ void gemm64(Acc64<T>& acc, View<T> A, View<T> B, GemmLds<T>& S) {
+// before the last barrier, so the caller may store over an operand.  With
+// C given, the tile C (nr x nc at C, leading dimension ldc) is loaded while
+// the last chunk runs and added at the end: acc = C + (+-A) B, so the
+// tile's first MFMA waits only for the first operand chunk.
+template <typename T, bool kNegA, bool kFull>
+__device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B, GemmLds<T>& S,
+                                            const C2<T>* __restrict__ C, int ldc, int cnr, int cnc) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
@@ -126,8 +134,13 @@ __device__ __forceinline__ void gemm64(Acc64<T>& acc, View<T> A, View<T> B, Gemm
       const int e = tid + 256 * u;
       const int ar_ = e >> 4, ak = chunk * 16 + (e & 15);
       const int bk = chunk * 16 + (e >> 6), bc = e & 63;
-      pa[u] = (ar_ < A.nr && ak < A.nc) ? A.p[(size_t)ar_ * A.ld + ak] : C2<T>{0, 0};
-      pb[u] = (bk < B.nr && bc < B.nc) ? B.p[(size_t)bk * B.ld + bc] : C2<T>{0, 0};
+      if (kFull) {  // unguarded loads: no select waits on their data
+        pa[u] = A.p[(size_t)ar_ * A.ld + ak];
+        pb[u] = B.p[(size_t)bk * B.ld + bc];
+      } else {
+        pa[u] = (ar_ < A.nr && ak < A.nc) ? A.p[(size_t)ar_ * A.ld + ak] : C2<T>{0, 0};
+        pb[u] = (bk < B.nr && bc < B.nc) ? B.p[(size_t)bk * B.ld + bc] : C2<T>{0, 0};
+      }
     }
   };
   auto stash = [&](int buf) {
@@ -136,8 +149,8 @@ __device__ __forceinline__ void gemm64(Acc64<T>& acc, View<T> A, View<T> B, Gemm
       const int e = tid + 256 * u;
       const int ar_ = e >> 4, ak = e & 15;
       const int bk = e >> 6, bc = e & 63;
-      S.ar[buf][ak][ar_] = kNegA ? -pa[u].re : pa[u].re;
-      S.ai[buf][ak][ar_] = kNegA ? -pa[u].im : pa[u].im;
+      S.ar[buf][ar_][ak] = pa[u].re;
+      S.ai[buf][ar_][ak] = pa[u].im;
       S.br[buf][bk][bc] = pb[u].re;
       S.bi[buf][bk][bc] = pb[u].im;
     }
@@ -145,19 +158,28 @@ __device__ __forceinline__ void gemm64(Acc64<T>& acc, View<T> A, View<T> B, Gemm
   fetch(0);
   stash(0);
   __syncthreads();
+  Acc64<T> cv;
 #pragma unroll
   for (int chunk = 0; chunk < 4; ++chunk) {
     const int buf = chunk & 1;
     if (chunk < 3) fetch(chunk + 1);
+    if (chunk == 3 && C) acc_load(cv, C, ldc, cnr, cnc);  // in flight under the last chunk's MFMAs
 #pragma unroll
     for (int k = 0; k < 16; k += 4) {
       T xr[2], xi[2], yr[2], yi[2];
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        xr[a] = S.ar[buf][k + lk][wr + a * 16 + li];
-        xi[a] = S.ai[buf][k + lk][wr + a * 16 + li];
+        xr[a] = S.ar[buf][wr + a * 16 + li][k + lk];
+        xi[a] = S.ai[buf][wr + a * 16 + li][k + lk];
         yr[a] = S.br[buf][k + lk][wc + a * 16 + li];
         yi[a] = S.bi[buf][k + lk][wc + a * 16 + li];
+        // the sign goes on the operand read from LDS, not on the staged
+        // global values: a negation there would make the wave wait for the
+        // next chunk's loads before this chunk's MFMAs
+        if (kNegA) {
+          xr[a] = -xr[a];
+          xi[a] = -xi[a];
+        }
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a)
@@ -179,6 +201,27 @@ __device__ __forceinline__ void gemm64(Acc64<T>& acc, View<T> A, View<T> B, Gemm
       __syncthreads();
     }
   }
+  if (C) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          acc.re[a][b][q] = cv.re[a][b][q] + acc.re[a][b][q];
+          acc.im[a][b][q] = cv.im[a][b][q] + acc.im[a][b][q];
+        }
+  }
+}
+
+template <typename T, bool kNegA>
+__device__ __forceinline__ void gemm64(Acc64<T>& acc, View<T> A, View<T> B, GemmLds<T>& S,
+                                       const C2<T>* __restrict__ C = nullptr, int ldc = 0, int cnr = 0,
+                                       int cnc = 0) {
+  if (A.nr == kNB && A.nc == kNB && B.nr == kNB && B.nc == kNB)  // uniform over the workgroup
+    gemm64_impl<T, kNegA, true>(acc, A, B, S, C, ldc, cnr, cnc);
+  else
+    gemm64_impl<T, kNegA, false>(acc, A, B, S, C, ldc, cnr, cnc);
 }
 
 // ---------------------------------------------------------------------------
@@ -484,9 +527,9 @@ __global__ __launch_bounds__(256) void lu_update_kernel(C2<T>* __restrict__ A, i
   const int nr = min(kNB, n - r0), nc = min(kNB, n - c0);
   C2<T>* blk = A + (size_t)r0 * ld + c0;
   Acc64<T> acc;
-  acc_load(acc, blk, ld, nr, nc);
+  acc_zero(acc);
   gemm64<T, true>(acc, View<T>{A + (size_t)r0 * ld + k0, ld, nr, kNB}, View<T>{A + (size_t)k0 * ld + c0, ld, kNB, nc},
-                  S.g);
+                  S.g, blk, ld, nr, nc);
   acc_store(acc, blk, ld, nr, nc);
   if (blockIdx.x != 0) return;
   // look-ahead: invert the freshly updated diagonal block
