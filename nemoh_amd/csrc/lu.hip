@@ -114,8 +114,9 @@ __device__ __forceinline__ void acc_store(const Acc64<T>& acc, C2<T>* __restrict
       }
 }
 
-// acc += (kNegA ? -A : A) B.  All global reads of both operands complete
-// before the last barrier, so the caller may store over an operand.  With
+// acc = (kNegA ? -A : A) B, acc zero on entry.  All global reads of both
+// operands complete before the last barrier, so the caller may store over an
+// operand.  With
 // C given, the tile C (nr x nc at C, leading dimension ldc) is loaded while
 // the last chunk runs and added at the end: acc = C + (+-A) B, so the
 // tile's first MFMA waits only for the first operand chunk.
@@ -159,6 +160,22 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
   stash(0);
   __syncthreads();
   Acc64<T> cv;
+  // Single precision forms the complex product with three real products
+  // (Gauss): K1 = (Ar + Ai) Br, K2 = Ar (Bi - Br), K3 = Ai (Br + Bi),
+  // Re = K1 - K3, Im = K1 + K2 -- a quarter fewer MFMAs.  Its rounding is a
+  // small multiple of the four-product form's, which the FP64 refinement
+  // that follows every single-precision solve absorbs; double precision
+  // keeps the four products.
+  constexpr bool kGauss = sizeof(T) == 4;
+  typename MfmaTraits<T>::acc_t k1[2][2];
+  if (kGauss) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) k1[a][b][q] = T(0);
+  }
 #pragma unroll
   for (int chunk = 0; chunk < 4; ++chunk) {
     const int buf = chunk & 1;
@@ -181,25 +198,54 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
           xi[a] = -xi[a];
         }
       }
+      if (kGauss) {  // acc.re accumulates K3, acc.im K2
+        T sx[2], dy[2], ey[2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc.re[a][b] = MfmaTraits<T>::mma(xr[a], yr[b], acc.re[a][b]);
-          acc.im[a][b] = MfmaTraits<T>::mma(xr[a], yi[b], acc.im[a][b]);
+        for (int a = 0; a < 2; ++a) {
+          sx[a] = xr[a] + xi[a];
+          dy[a] = yi[a] - yr[a];
+          ey[a] = yr[a] + yi[a];
         }
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc.re[a][b] = MfmaTraits<T>::mma(-xi[a], yi[b], acc.re[a][b]);
-          acc.im[a][b] = MfmaTraits<T>::mma(xi[a], yr[b], acc.im[a][b]);
-        }
+          for (int b = 0; b < 2; ++b) {
+            k1[a][b] = MfmaTraits<T>::mma(sx[a], yr[b], k1[a][b]);
+            acc.im[a][b] = MfmaTraits<T>::mma(xr[a], dy[b], acc.im[a][b]);
+            acc.re[a][b] = MfmaTraits<T>::mma(xi[a], ey[b], acc.re[a][b]);
+          }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            acc.re[a][b] = MfmaTraits<T>::mma(xr[a], yr[b], acc.re[a][b]);
+            acc.im[a][b] = MfmaTraits<T>::mma(xr[a], yi[b], acc.im[a][b]);
+          }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            acc.re[a][b] = MfmaTraits<T>::mma(-xi[a], yi[b], acc.re[a][b]);
+            acc.im[a][b] = MfmaTraits<T>::mma(xi[a], yr[b], acc.im[a][b]);
+          }
+      }
     }
     if (chunk < 3) {
       stash(buf ^ 1);
       __syncthreads();
     }
+  }
+  if (kGauss) {  // acc = K1 - K3, K1 + K2 (acc entered as zero)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          acc.re[a][b][q] = k1[a][b][q] - acc.re[a][b][q];
+          acc.im[a][b][q] = k1[a][b][q] + acc.im[a][b][q];
+        }
   }
   if (C) {
 #pragma unroll
