@@ -91,15 +91,14 @@ __device__ __forceinline__ cplx cexp_(cplx z) {
 }
 __device__ __forceinline__ cplx clog_(cplx z) { return {log(hypot(z.re, z.im)), atan2(z.im, z.re)}; }
 
-// 1/sqrt(x) for x > 0 to working precision: the hardware estimate
-// (v_rsq_f64) refined by two Newton steps y <- y + y (1 - x y^2) / 2.  One
-// multiply-add chain instead of a correctly rounded square root followed by
-// divisions; the result differs from 1/sqrt(x) by about an ulp.
+// 1/sqrt(x) for x > 0: the hardware estimate (v_rsq_f64, relative error
+// about 5e-8) refined by one Newton step y <- y + y (1 - x y^2) / 2, which
+// leaves 1.5 (5e-8)^2, about 4e-15 -- four instructions instead of a
+// correctly rounded square root followed by divisions.  (A second step
+// would reach about an ulp; the influence coefficients need 1e-12.)
 __device__ __forceinline__ double rsqrt_pos(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double e = fma(-x * y, y, 1.0);
-  y = fma(0.5 * y, e, y);
-  e = fma(-x * y, y, 1.0);
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
   return fma(0.5 * y, e, y);
 }
 
@@ -701,31 +700,32 @@ __device__ WavePart wave_fd_image(const Geometry& G, const Tables& T, const Freq
     }
     w.vz1 = cof3 * (m1.pd1z - m2.pd1z + m3.pd1z - m4.pd1z);
     w.vz2 = cof4 * (m1.vz2 - m2.vz2 + m3.vz2 - m4.vz2);
-    // exponential images as one-point sources (VNSFD.f90:635-706)
+    // exponential images as one-point sources (VNSFD.f90:635-706).  Every
+    // image shares the pair's horizontal offset, so its x / y gradient terms
+    // -xpg a, -ypg a are summed as one weight sum scaled once at the end.
     const double act = -0.5 * aire / (4.0 * M_PI);
     const double r2 = rrr * rrr;
+    double asum = 0.0, vz = 0.0, fs = 0.0;
     for (int ke = 0; ke < P.nexp1; ++ke) {
       const double aqt = act * P.ar[ke];
       const double hl = h * P.ambda[ke];
-      const double zp1 = zmiii - 2.0 * h + hl - zgj;
-      const double zp2 = -zmiii - hl - zgj;
-      const double zp3 = -zmiii - 4.0 * h + hl - zgj;
-      const double zp4 = zmiii + 2.0 * h - hl - zgj;
-      double f[4], a[4];
-      const double zp[4] = {zp1, zp2, zp3, zp4};
+      const double zp[4] = {zmiii - 2.0 * h + hl - zgj, -zmiii - hl - zgj, -zmiii - 4.0 * h + hl - zgj,
+                            zmiii + 2.0 * h - hl - zgj};
+      const double zs[4] = {-1.0, 1.0, 1.0, -1.0};  // signs of the z terms (VZS1-VZS2-VZS3+VZS4)
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double rr = r2 + zp[k] * zp[k];
-        const bool in = rr > eps * eps;  // RO > 1e-4
-        const double ri = rsqrt_pos(in ? rr : 1.0);
-        f[k] = in ? aqt * ri : 0.0;
-        a[k] = in ? f[k] * ri * ri : 0.0;
+        const double ri = (rr > eps * eps) ? rsqrt_pos(rr) : 0.0;  // RO > 1e-4, else no term
+        const double f = aqt * ri, a = f * (ri * ri);
+        fs += f;
+        asum += a;
+        vz = fma(zs[k] * zp[k], a, vz);
       }
-      w.fs1 += f[0] + f[1] + f[2] + f[3];
-      w.vx1 += -xpg * a[0] + -xpg * a[1] + -xpg * a[2] + -xpg * a[3];
-      w.vy1 += -ypg * a[0] + -ypg * a[1] + -ypg * a[2] + -ypg * a[3];
-      w.vz1 += -zp[0] * a[0] - -zp[1] * a[1] - -zp[2] * a[2] + -zp[3] * a[3];
     }
+    w.fs1 += fs;
+    w.vx1 += -xpg * asum;
+    w.vy1 += -ypg * asum;
+    w.vz1 += vz;
     return w;
   }
   // near the free surface: exact exponential images (VNSFD.f90:801-825)
