@@ -164,8 +164,9 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
   // (Gauss): K1 = (Ar + Ai) Br, K2 = Ar (Bi - Br), K3 = Ai (Br + Bi),
   // Re = K1 - K3, Im = K1 + K2 -- a quarter fewer MFMAs.  Its rounding is a
   // small multiple of the four-product form's, which the FP64 refinement
-  // that follows every single-precision solve absorbs; double precision
-  // keeps the four products.
+  // that follows every single-precision solve absorbs.  Double precision
+  // keeps the four products: its update is bound by moving the 16-byte
+  // elements of C, not by the MFMAs.
   constexpr bool kGauss = sizeof(T) == 4;
   typename MfmaTraits<T>::acc_t k1[2][2];
   if (kGauss) {
