@@ -50,6 +50,8 @@ struct nemoh_core {
   // resident geometry and tables
   double* d_geo = nullptr;  // xg yg zg xn yn zn aire tdis, n each
   double* d_vtx = nullptr;  // vx vy vz, 4n each
+  int* d_surf = nullptr;    // panels with zg >= ZER (free-surface branch of the wave part)
+  int nsurf = 0;
   double* d_xr = nullptr;
   double* d_xz = nullptr;
   double* d_tab = nullptr;
@@ -237,7 +239,7 @@ int prepare_frequency(nemoh_core* c, double w) {
     out.dm32 = c->D32[1];
   }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->st));
-  launch_assemble(c->G, c->T, c->d_par, c->h_par.fd, c->nsym, out, c->st);
+  launch_assemble(c->G, c->T, c->d_par, c->h_par.fd, c->nsym, out, c->d_surf, c->nsurf, c->st);
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
   HIPCHK(c, hipMemsetAsync(c->d_info, 0, 2 * sizeof(int), c->st));
@@ -533,14 +535,22 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   const int nblk = (nfa + kNB - 1) / kNB;
   const int ntile = (np + 255) / 256;
   const size_t nred = (size_t)ntile * (ntile + 1) / 2;
+  // panels whose pairs take the free-surface branch of the wave part
+  // (the reference's test ZG < ZER fails for them)
+  std::vector<int> surf;
+  for (int j = 0; j < nfa; ++j)
+    if (!(geo[2][j] < zer)) surf.push_back(j);
   int r = 0;
   if ((r = dalloc(c, &c->d_geo, 8 * (size_t)nfa)) || (r = dalloc(c, &c->d_vtx, 12 * (size_t)nfa)) ||
+      (r = dalloc(c, &c->d_surf, surf.size())) ||
       (r = dalloc(c, &c->d_flags, 2 * (size_t)nblk)) || (r = dalloc(c, &c->d_pts, 2 * (size_t)np)) ||
       (r = dalloc(c, &c->d_ext, nred)))
     return r;
   for (int k = 0; k < 8; ++k)
     HIPCHK(c, hipMemcpyAsync(c->d_geo + (size_t)k * nfa, geo[k], nfa * sizeof(double), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->d_vtx, vtx.data(), vtx.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+  if (!surf.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_surf, surf.data(), surf.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->d_pts, x, np * sizeof(double), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->d_pts + np, y, np * sizeof(double), hipMemcpyHostToDevice, c->st));
   // flags of the chained triangular solves: cleared over the whole buffer
@@ -563,6 +573,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   c->nsym = nsym;
   c->nb = 1 << nsym;
   c->zer = zer;
+  c->nsurf = (int)surf.size();
   c->g = g;
   c->rho = rho;
   c->depth = depth;
@@ -687,6 +698,7 @@ void nemoh_core_destroy(nemoh_core* c) {
   };
   release(c->d_geo);
   release(c->d_vtx);
+  release(c->d_surf);
   release(c->d_xr);
   release(c->d_xz);
   release(c->d_tab);

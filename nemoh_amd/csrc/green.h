@@ -378,7 +378,11 @@ struct WavePart {
   double fs1, fs2, vx1, vy1, vz1, vx2, vy2, vz2;
 };
 
-// Infinite depth, VNSINFD.f90:269-453.
+// Infinite depth, VNSINFD.f90:269-453.  kSurf = false leaves out the branch
+// for pairs with a point at the free surface (zero result there): the
+// assembly runs those pairs in a launch of their own, so that the bulk of the
+// matrix runs without that branch's registers.
+template <bool kSurf>
 __device__ WavePart wave_infd_image(const Geometry& G, const Tables& T, const FreqParams& P, int i, int j,
                                     double bx) {
   WavePart w = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -447,7 +451,7 @@ __device__ WavePart wave_infd_image(const Geometry& G, const Tables& T, const Fr
   }
   // near the free surface: theta quadrature over the panel edges
   // (VNSINFD.f90:385-453); with NQ = 0 the terms vanish
-  if (P.nq == 0) return w;
+  if (!kSurf || P.nq == 0) return w;
   double px[5], py[5], pz[5];
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
@@ -643,10 +647,12 @@ __device__ __forceinline__ void vsd(const Geometry& G, int j, double mjj, double
   vz = -vzs * qmp;
 }
 
-// Finite depth, VNSFD.f90:284-827 for one y-image (bx), already scaled.
+// Finite depth, VNSFD.f90:284-827 for one y-image (bx), already scaled;
+// kSurf as for wave_infd_image.
 // The near-free-surface theta quadrature is not evaluated: the reference
 // runs it with CQ = 0 (CINT_FD fills local arrays, SOLVE_BEM_FD_DIRECT.f90:
 // 73-74 zeroes the module ones), so it contributes exactly nothing.
+template <bool kSurf>
 __device__ WavePart wave_fd_image(const Geometry& G, const Tables& T, const FreqParams& P, int i, int j,
                                   double bx) {
   WavePart w = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -683,23 +689,44 @@ __device__ WavePart wave_fd_image(const Geometry& G, const Tables& T, const Freq
       }
     }
     const double pi_k = M_PI / am0, pi_k3 = pi_k / (am0 * am0);
-    const FdImage m1 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zz1, psk, sck, scds, ssds);
-    const FdImage m2 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zgj - zmiii - 2.0 * h, psk, sck, scds, ssds);
-    const FdImage m3 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zmiii - zgj - 2.0 * h, psk, sck, scds, ssds);
-    const FdImage m4 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, -zgj - zmiii - 4.0 * h, psk, sck, scds, ssds);
-    w.fs1 = cof1 * ((m1.pd1z + m2.pd1z + m3.pd1z + m4.pd1z) - m1.psr - m2.psr - m3.psr - m4.psr);
-    w.fs2 = cof2 * (m1.pd2z + m2.pd2z + m3.pd2z + m4.pd2z);
+    // the four images' terms are summed as each image is formed (left to
+    // right, as the sums are written in VNSFD.f90:548-600), so that only the
+    // running sums stay live, not four images' worth of values
+    const double zi[4] = {zz1, zgj - zmiii - 2.0 * h, zmiii - zgj - 2.0 * h, -zgj - zmiii - 4.0 * h};
+    double spd1z = 0, spd2z = 0, spd1x = 0, svr2 = 0, dpd1z = 0, dvz2 = 0, psr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const FdImage m = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zi[k], psk, sck, scds, ssds);
+      psr[k] = m.psr;
+      if (k == 0) {
+        spd1z = m.pd1z;
+        spd2z = m.pd2z;
+        spd1x = m.pd1x;
+        svr2 = m.vr2;
+        dpd1z = m.pd1z;
+        dvz2 = m.vz2;
+      } else {
+        spd1z += m.pd1z;
+        spd2z += m.pd2z;
+        spd1x += m.pd1x;
+        svr2 += m.vr2;
+        dpd1z = (k & 1) ? dpd1z - m.pd1z : dpd1z + m.pd1z;
+        dvz2 = (k & 1) ? dvz2 - m.vz2 : dvz2 + m.vz2;
+      }
+    }
+    w.fs1 = cof1 * (spd1z - psr[0] - psr[1] - psr[2] - psr[3]);
+    w.fs2 = cof2 * spd2z;
     if (rrr > eps) {
-      const double c1v3 = -cof3 * (m1.pd1x + m2.pd1x + m3.pd1x + m4.pd1x);
-      const double c2v3 = cof4 * (m1.vr2 + m2.vr2 + m3.vr2 + m4.vr2);
+      const double c1v3 = -cof3 * spd1x;
+      const double c2v3 = cof4 * svr2;
       const double cvx = xpg / rrr, cvy = ypg / rrr;
       w.vx1 = c1v3 * cvx;
       w.vx2 = c2v3 * cvx;
       w.vy1 = c1v3 * cvy;
       w.vy2 = c2v3 * cvy;
     }
-    w.vz1 = cof3 * (m1.pd1z - m2.pd1z + m3.pd1z - m4.pd1z);
-    w.vz2 = cof4 * (m1.vz2 - m2.vz2 + m3.vz2 - m4.vz2);
+    w.vz1 = cof3 * dpd1z;
+    w.vz2 = cof4 * dvz2;
     // exponential images as one-point sources (VNSFD.f90:635-706).  Every
     // image shares the pair's horizontal offset, so its x / y gradient terms
     // -xpg a, -ypg a are summed as one weight sum scaled once at the end.
@@ -728,6 +755,7 @@ __device__ WavePart wave_fd_image(const Geometry& G, const Tables& T, const Freq
     w.vz1 += vz;
     return w;
   }
+  if (!kSurf) return w;
   // near the free surface: exact exponential images (VNSFD.f90:801-825)
   const double zp[4] = {-zmiii, zmiii + 2.0 * h, zmiii - 2.0 * h, -zmiii - 4.0 * h};
   for (int ke = 0; ke < P.nexp1; ++ke) {
@@ -750,14 +778,17 @@ __device__ WavePart wave_fd_image(const Geometry& G, const Tables& T, const Freq
 // VS{X,Y,Z}M1, VS{X,Y,Z}P2, VS{X,Y,Z}M2).
 // kFd: finite depth (P.fd == 1) or infinite depth, a compile-time choice so
 // that each assembly kernel carries only its own regime's code.
-template <bool kFd>
+// kSurf = false: only pairs with both points below the free surface
+// (zg < ZER) are handled.
+template <bool kFd, bool kSurf = true>
 __device__ void pair16(const Geometry& G, const Tables& T, const FreqParams& P, int i, int j, double o[16]) {
   double rk[8];
   if (kFd) rankine(G, i, j, P.nsym, P.zer, 2, 2.0 * P.depth, rk);
   else rankine(G, i, j, P.nsym, P.zer, 1, 0.0, rk);
-  const WavePart w1 = kFd ? wave_fd_image(G, T, P, i, j, 1.0) : wave_infd_image(G, T, P, i, j, 1.0);
+  const WavePart w1 = kFd ? wave_fd_image<kSurf>(G, T, P, i, j, 1.0) : wave_infd_image<kSurf>(G, T, P, i, j, 1.0);
   WavePart w2 = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (P.nsym == 1) w2 = kFd ? wave_fd_image(G, T, P, i, j, -1.0) : wave_infd_image(G, T, P, i, j, -1.0);
+  if (P.nsym == 1)
+    w2 = kFd ? wave_fd_image<kSurf>(G, T, P, i, j, -1.0) : wave_infd_image<kSurf>(G, T, P, i, j, -1.0);
   double s1, s2, s3;  // scaling of (potential 1, potential 2 / gradient 1, gradient 2)
   double g1, g2;
   if (kFd) {

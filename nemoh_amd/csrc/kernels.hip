@@ -57,17 +57,17 @@ __global__ void crek_kernel(const double* __restrict__ xr, const double* __restr
 // Assembly.  blockIdx.y = field point (row) i, 256 consecutive source panels
 // j per workgroup: one wavefront covers 64 panels of one row, so the field
 // point is uniform across the wave and panel data loads are coalesced.
-// Finite depth fits two waves per SIMD (a few spilled values); the infinite-
-// depth path, with its near-free-surface quadrature, keeps one.
-template <bool kFd, bool kSym, bool kF32>
-__global__ __launch_bounds__(256, kFd ? 2 : 1) void assemble_kernel(Geometry G, Tables T, const FreqParams* __restrict__ Pp,
-                                                       MatOut out) {
-  const int i = blockIdx.y;
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= G.n) return;
-  const FreqParams& P = *Pp;
-  double o[16];
-  pair16<kFd>(G, T, P, i, j, o);
+//
+// The bulk launch handles the pairs with both points below the free surface
+// (zg < ZER), which is every pair of a mesh without panels at z = 0.  The
+// other branch of the wave part (the reference's exact images or theta
+// quadrature for points at the free surface) needs about 90 more registers
+// per lane; compiled into the bulk kernel it would leave two waves per SIMD
+// (one in infinite depth) instead of three.  Pairs with a point at the free
+// surface are redone by assemble_surface_kernel, over the handle's list of
+// such panels.
+template <bool kSym, bool kF32>
+__device__ __forceinline__ void store_pair(const Geometry& G, const MatOut& out, int i, int j, const double o[16]) {
   const double xn = G.xn[i], yn = G.yn[i], zn = G.zn[i];
   const size_t e = (size_t)i * out.ld + j;
   const double dpr = o[4] * xn + o[5] * yn + o[6] * zn;
@@ -82,6 +82,42 @@ __global__ __launch_bounds__(256, kFd ? 2 : 1) void assemble_kernel(Geometry G, 
     out.sm[e] = make_double2(o[1], o[3]);
     if (kF32) out.dm32[e] = make_float2((float)dmr, (float)dmi);
   }
+}
+
+template <bool kFd, bool kSym, bool kF32>
+__global__ __launch_bounds__(256, 3) void assemble_kernel(Geometry G, Tables T, const FreqParams* __restrict__ Pp,
+                                                          MatOut out) {
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= G.n) return;
+  double o[16];
+  // a pair with a point at the free surface gets the Rankine part and a zero
+  // wave part here, overwritten by assemble_surface_kernel (an early return
+  // for those pairs instead costs this kernel registers)
+  pair16<kFd, false>(G, T, *Pp, i, j, o);
+  store_pair<kSym, kF32>(G, out, i, j, o);
+}
+
+// The pairs with a point at the free surface: blockIdx.z = 0 takes the rows
+// of the listed panels (every j), blockIdx.z = 1 their columns (every i that
+// is not itself listed, so that no element is written twice).
+template <bool kFd, bool kSym, bool kF32>
+__global__ __launch_bounds__(256) void assemble_surface_kernel(Geometry G, Tables T,
+                                                               const FreqParams* __restrict__ Pp, MatOut out,
+                                                               const int* __restrict__ surf) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= G.n) return;
+  const FreqParams& P = *Pp;
+  const int s = surf[blockIdx.y];
+  int i = s, j = k;
+  if (blockIdx.z == 1) {
+    i = k;
+    j = s;
+    if (!(G.zg[i] < P.zer)) return;  // a listed row: done by blockIdx.z = 0
+  }
+  double o[16];
+  pair16<kFd, true>(G, T, P, i, j, o);
+  store_pair<kSym, kF32>(G, out, i, j, o);
 }
 
 // Raw 16-value dump of every pair (tests only; small meshes).
@@ -393,24 +429,31 @@ void launch_crek(const double* xr, const double* xz, double* tab, hipStream_t st
   crek_kernel<<<(kNR * kNZ + 127) / 128, 128, 0, st>>>(xr, xz, tab);
 }
 
+template <bool kFd, bool kSym, bool kF32>
+static void launch_assemble_variant(const Geometry& G, const Tables& T, const FreqParams* P, const MatOut& out,
+                                    const int* surf, int nsurf, hipStream_t st) {
+  const int nx = (G.n + 255) / 256;
+  assemble_kernel<kFd, kSym, kF32><<<dim3(nx, G.n), 256, 0, st>>>(G, T, P, out);
+  if (nsurf > 0) assemble_surface_kernel<kFd, kSym, kF32><<<dim3(nx, nsurf, 2), 256, 0, st>>>(G, T, P, out, surf);
+}
+
 template <bool kFd>
 static void launch_assemble_regime(const Geometry& G, const Tables& T, const FreqParams* P, int nsym,
-                                   const MatOut& out, hipStream_t st) {
-  dim3 grid((G.n + 255) / 256, G.n);
+                                   const MatOut& out, const int* surf, int nsurf, hipStream_t st) {
   const bool f32 = out.dp32 != nullptr;
   if (nsym) {
-    if (f32) assemble_kernel<kFd, true, true><<<grid, 256, 0, st>>>(G, T, P, out);
-    else assemble_kernel<kFd, true, false><<<grid, 256, 0, st>>>(G, T, P, out);
+    if (f32) launch_assemble_variant<kFd, true, true>(G, T, P, out, surf, nsurf, st);
+    else launch_assemble_variant<kFd, true, false>(G, T, P, out, surf, nsurf, st);
   } else {
-    if (f32) assemble_kernel<kFd, false, true><<<grid, 256, 0, st>>>(G, T, P, out);
-    else assemble_kernel<kFd, false, false><<<grid, 256, 0, st>>>(G, T, P, out);
+    if (f32) launch_assemble_variant<kFd, false, true>(G, T, P, out, surf, nsurf, st);
+    else launch_assemble_variant<kFd, false, false>(G, T, P, out, surf, nsurf, st);
   }
 }
 
 void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, int fd, int nsym, const MatOut& out,
-                     hipStream_t st) {
-  if (fd) launch_assemble_regime<true>(G, T, P, nsym, out, st);
-  else launch_assemble_regime<false>(G, T, P, nsym, out, st);
+                     const int* surf, int nsurf, hipStream_t st) {
+  if (fd) launch_assemble_regime<true>(G, T, P, nsym, out, surf, nsurf, st);
+  else launch_assemble_regime<false>(G, T, P, nsym, out, surf, nsurf, st);
 }
 
 void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int fd, double* out, hipStream_t st) {

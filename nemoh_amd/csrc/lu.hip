@@ -181,7 +181,9 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
   for (int chunk = 0; chunk < 4; ++chunk) {
     const int buf = chunk & 1;
     if (chunk < 3) fetch(chunk + 1);
-    if (chunk == 3 && C) acc_load(cv, C, ldc, cnr, cnc);  // in flight under the last chunk's MFMAs
+    // in flight under the last chunk's MFMAs (issued with chunk 2 instead,
+    // the 12k-panel factorisation ran 4 % slower)
+    if (chunk == 3 && C) acc_load(cv, C, ldc, cnr, cnc);
 #pragma unroll
     for (int k = 0; k < 16; k += 4) {
       T xr[2], xi[2], yr[2], yi[2];
@@ -560,6 +562,29 @@ union UpdLds {
   InvLds<T> inv;
 };
 
+// Workgroup -> trailing tile (ti, tj) of an m x m tile grid.  Workgroups are
+// dealt to the eight XCDs in turn (workgroup w to XCD w mod 8), each XCD
+// behind its own L2.  Each XCD gets one contiguous run of a tile order that
+// walks bands of kBand tile rows column by column: going down a column reuses
+// one U column panel kBand times, and the band's kBand L row panels (1 MB:
+// 32 KB each in single precision, 64 KB in double) stay in the L2 from one
+// column to the next.  An XCD's run (m^2 / 8 tiles) then reads about one
+// band's worth of L panels and a fraction m / (8 kBand) of the U panels (when
+// below one) instead of all of both, as the row-major order does on every
+// XCD.  Workgroup 0 gets tile (0, 0).  The
+// map is a bijection for every m.
+template <int kBand>
+__device__ __forceinline__ void update_tile(int bid, int m, int& ti, int& tj) {
+  constexpr int kXcds = 8;
+  const int total = m * m, per = total / kXcds, rem = total % kXcds;
+  const int x = bid % kXcds, local = bid / kXcds;
+  const int idx = (x < rem) ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
+  const int g = idx / (kBand * m), w = idx - g * kBand * m;
+  const int rows = min(kBand, m - g * kBand);
+  tj = w / rows;
+  ti = g * kBand + (w - tj * rows);
+}
+
 // Trailing update of step kb: A_ij -= L_i,kb A_kb,j for i, j > kb, one 64x64
 // tile per workgroup.  Workgroup 0 owns tile (kb+1, kb+1), the next diagonal
 // block, and inverts it after its update (P_{kb+1}).
@@ -569,7 +594,8 @@ __global__ __launch_bounds__(256) void lu_update_kernel(C2<T>* __restrict__ A, i
   __shared__ UpdLds<T> S;
   const int s0 = (kb + 1) * kNB, k0 = kb * kNB;
   const int m = (n - s0 + kNB - 1) / kNB;
-  const int ti = blockIdx.x / m, tj = blockIdx.x % m;
+  int ti, tj;
+  update_tile<sizeof(T) == 4 ? 32 : 16>(blockIdx.x, m, ti, tj);
   const int r0 = s0 + ti * kNB, c0 = s0 + tj * kNB;
   const int nr = min(kNB, n - r0), nc = min(kNB, n - c0);
   C2<T>* blk = A + (size_t)r0 * ld + c0;

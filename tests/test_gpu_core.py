@@ -143,6 +143,50 @@ def test_dropin_solve_matches_oracle(prec):
     c.close()
 
 
+def _lidcyl(isym):
+    """The half cylinder with a lid at z = 0 of the pair fixture (isym 1),
+    or the whole cylinder mirrored from it (isym 0, geometry computed)."""
+    d = load_golden("pairs_lidcyl.npz")
+    v, p = d["vertices"], d["panels"]
+    if isym == 1:
+        return v, p, case_geometry(d), float(d["depth_fd"])
+    nv = v.shape[0]
+    vm = v * np.array([1.0, -1.0, 1.0])
+    pm = p[:, ::-1] + nv  # mirrored panels keep their outward orientation
+    return np.vstack([v, vm]), np.vstack([p, pm]), None, float(d["depth_fd"])
+
+
+@pytest.mark.parametrize("isym", [1, 0])
+@pytest.mark.parametrize("prec", [core.FP64, core.MIXED])
+def test_free_surface_panels_match_oracle(isym, prec):
+    """Panels at the free surface (the lid at z = 0), whose pairs take the
+    other branch of the wave part and are assembled by a launch of their own:
+    a finite-depth frequency, then an infinite-depth one (k h >= 20 at the
+    same depth) with NQ carried over from the first, as the reference's
+    module state carries it, so that the theta quadrature runs.  One problem
+    each through the drop-in solve, against the oracle's solution."""
+    from oracle import pyoracle as po
+    v, p, geo, depth = _lidcyl(isym)
+    c = core.Core(v, p, isym, depth, 1025.0, 9.81, geometry=geo, tables=_tables())
+    c.set_precision(prec)
+    oc = po.Case(v, p, isym, depth=depth, rho=1025.0, g=9.81, tables=_tables(), geometry=geo)
+    assert np.sum(oc.geo["zg"] >= oc.geo["zer"]) >= 6
+    s = oc.session()
+    rng = np.random.default_rng(7)
+    ntot = p.shape[0] * 2 ** isym
+    tol = FP64_TOL if prec == core.FP64 else MIXED_TOL
+    for period, fd in ((6.0, 1), (1.2, 0)):
+        f = c.freq_setup(period)
+        assert f.finite_depth == fd and f.nq > 0
+        nvel = rng.standard_normal(ntot) + 1j * rng.standard_normal(ntot)
+        got = c.solve(period, f, nvel)
+        _, zigb, zigs, zpb, zps = s.solve_bvp(period, nvel)
+        scale = max(np.abs(zigb).max(), np.abs(zpb).max())
+        for a, b in zip(got, (zigb, zigs, zpb, zps)):
+            assert np.abs(a - b).max() <= tol * scale, (period, np.abs(a - b).max() / scale)
+    c.close()
+
+
 def test_omega_below_threshold_is_an_error():
     d, m, _ = _case("barge_fd")
     c = core.Core(d["vertices"], d["panels"], 0, 0.0, tables=_tables())
