@@ -136,19 +136,23 @@ __global__ void pairs_kernel(Geometry G, Tables T, const FreqParams* __restrict_
 // ---------------------------------------------------------------------------
 // Y(n x nrhs) = beta*Y + alpha*M(n x n) X(n x nrhs) in FP64 (phi = S sigma and
 // the refinement residual b - D x).  Bound by streaming M once from HBM.
-// One workgroup per 16 rows; its four waves split the columns, each running
-// FP64 16x16x4 MFMAs with A = 16 rows x 4 columns of M and B = the matching
-// 4 rows x 16 right-hand sides of X (nrhs <= 16), 8 k-steps per batch of
-// loads; the four partial blocks are summed through LDS.
-__global__ __launch_bounds__(256) void gemv_kernel(const double2* __restrict__ M, int ld, int n,
-                                                   const double2* __restrict__ X, int nrhs,
-                                                   double2* __restrict__ Y, double alpha, double beta) {
-  __shared__ double red[4][2][16][17];
+// One workgroup per 16 rows; its kGemvWaves waves split the columns, each
+// running FP64 16x16x4 MFMAs with A = 16 rows x 4 columns of M and B = the
+// matching 4 rows x 16 right-hand sides of X (nrhs <= 16), 8 k-steps per
+// batch of loads; the partial blocks are summed through LDS in wave order.
+// Sixteen waves (four per SIMD, one workgroup per CU) keep 128 KB of M in
+// flight per CU, which HBM's latency needs at full bandwidth; four waves
+// streamed at 2.9 TB/s.
+constexpr int kGemvWaves = 16;
+__global__ __launch_bounds__(64 * kGemvWaves) void gemv_kernel(const double2* __restrict__ M, int ld, int n,
+                                                               const double2* __restrict__ X, int nrhs,
+                                                               double2* __restrict__ Y, double alpha, double beta) {
+  __shared__ double red[kGemvWaves][2][16][17];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
   const int r0 = blockIdx.x * 16;
   const int row = r0 + li;
   const int nk = (n + 3) / 4;  // k-steps of 4 columns
-  const int per = (nk + 3) / 4;
+  const int per = (nk + kGemvWaves - 1) / kGemvWaves;
   const int s0 = wave * per, s1 = min(nk, s0 + per);
   const double2* mrow = M + (size_t)min(row, n - 1) * ld;
   v4d cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
@@ -181,8 +185,12 @@ __global__ __launch_bounds__(256) void gemv_kernel(const double2* __restrict__ M
     for (int q = 0; q < 4; ++q) {
       const int r = lk + 4 * q;
       if (r0 + r < n && li < nrhs) {
-        const double vr = red[0][0][r][li] + red[1][0][r][li] + red[2][0][r][li] + red[3][0][r][li];
-        const double vi = red[0][1][r][li] + red[1][1][r][li] + red[2][1][r][li] + red[3][1][r][li];
+        double vr = red[0][0][r][li], vi = red[0][1][r][li];
+#pragma unroll
+        for (int w = 1; w < kGemvWaves; ++w) {
+          vr += red[w][0][r][li];
+          vi += red[w][1][r][li];
+        }
         double2* y = Y + (size_t)(r0 + r) * nrhs + li;
         const double2 old = (beta == 0.0) ? make_double2(0.0, 0.0) : *y;
         *y = make_double2(beta * old.x + alpha * vr, beta * old.y + alpha * vi);
@@ -464,7 +472,7 @@ void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int f
 
 void launch_matvec(const double2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
                    double beta, hipStream_t st) {
-  gemv_kernel<<<(n + 15) / 16, 256, 0, st>>>(M, ld, n, X, nrhs, Y, alpha, beta);
+  gemv_kernel<<<(n + 15) / 16, 64 * kGemvWaves, 0, st>>>(M, ld, n, X, nrhs, Y, alpha, beta);
 }
 
 void launch_rhs(const double2* V, int n, int nsym, int nrhs, double bx, double2* B, hipStream_t st) {
