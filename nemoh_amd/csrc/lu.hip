@@ -39,7 +39,7 @@ constexpr int kTile = kNB * kNB;  // elements of one 64x64 block
 // 64x64x64 complex tile product on MFMA.  Four waves each own a 32x32
 // quadrant = 2x2 MFMA 16x16 tiles; the complex product is four real MFMAs.
 // K streams through LDS in chunks of 16, double buffered (the global loads of
-// chunk c+1 are in flight while chunk c feeds the MFMAs), in separate real /
+// chunk c+2 are in flight while chunk c feeds the MFMAs), in separate real /
 // imaginary planes: A row-major with a row stride of 20 elements, B k-major
 // with 72, so that the MFMA operand reads (lane (li, lk): row li, k lk of
 // A; k lk, column li of B) touch each LDS bank at most twice and the
@@ -127,37 +127,41 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
   // staging map: A chunk = 64 rows x 16 k, B chunk = 16 k x 64 cols; each
-  // thread moves 4 elements of each
-  C2<T> pa[4], pb[4];
-  auto fetch = [&](int chunk) {
+  // thread moves 4 elements of each.  Chunk c is loaded into register set
+  // c & 1 two chunks before it is needed (the loads of chunk c + 2 go out
+  // while chunk c feeds the MFMAs and are stashed in LDS at the end of chunk
+  // c + 1), so each load has two chunks' MFMA time to land.
+  C2<T> pa[2][4], pb[2][4];
+  auto fetch = [&](int chunk, int set) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u;
       const int ar_ = e >> 4, ak = chunk * 16 + (e & 15);
       const int bk = chunk * 16 + (e >> 6), bc = e & 63;
       if (kFull) {  // unguarded loads: no select waits on their data
-        pa[u] = A.p[(size_t)ar_ * A.ld + ak];
-        pb[u] = B.p[(size_t)bk * B.ld + bc];
+        pa[set][u] = A.p[(size_t)ar_ * A.ld + ak];
+        pb[set][u] = B.p[(size_t)bk * B.ld + bc];
       } else {
-        pa[u] = (ar_ < A.nr && ak < A.nc) ? A.p[(size_t)ar_ * A.ld + ak] : C2<T>{0, 0};
-        pb[u] = (bk < B.nr && bc < B.nc) ? B.p[(size_t)bk * B.ld + bc] : C2<T>{0, 0};
+        pa[set][u] = (ar_ < A.nr && ak < A.nc) ? A.p[(size_t)ar_ * A.ld + ak] : C2<T>{0, 0};
+        pb[set][u] = (bk < B.nr && bc < B.nc) ? B.p[(size_t)bk * B.ld + bc] : C2<T>{0, 0};
       }
     }
   };
-  auto stash = [&](int buf) {
+  auto stash = [&](int buf, int set) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u;
       const int ar_ = e >> 4, ak = e & 15;
       const int bk = e >> 6, bc = e & 63;
-      S.ar[buf][ar_][ak] = pa[u].re;
-      S.ai[buf][ar_][ak] = pa[u].im;
-      S.br[buf][bk][bc] = pb[u].re;
-      S.bi[buf][bk][bc] = pb[u].im;
+      S.ar[buf][ar_][ak] = pa[set][u].re;
+      S.ai[buf][ar_][ak] = pa[set][u].im;
+      S.br[buf][bk][bc] = pb[set][u].re;
+      S.bi[buf][bk][bc] = pb[set][u].im;
     }
   };
-  fetch(0);
-  stash(0);
+  fetch(0, 0);
+  fetch(1, 1);
+  stash(0, 0);  // waits for chunk 0's loads only (chunk 1's were issued after)
   __syncthreads();
   Acc64<T> cv;
   // Single precision forms the complex product with three real products
@@ -180,7 +184,7 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
 #pragma unroll
   for (int chunk = 0; chunk < 4; ++chunk) {
     const int buf = chunk & 1;
-    if (chunk < 3) fetch(chunk + 1);
+    if (chunk < 2) fetch(chunk + 2, chunk & 1);  // set chunk & 1 was stashed already
     // in flight under the last chunk's MFMAs (issued with chunk 2 instead,
     // the 12k-panel factorisation ran 4 % slower)
     if (chunk == 3 && C) acc_load(cv, C, ldc, cnr, cnc);
@@ -235,7 +239,7 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
       }
     }
     if (chunk < 3) {
-      stash(buf ^ 1);
+      stash(buf ^ 1, (chunk + 1) & 1);
       __syncthreads();
     }
   }
