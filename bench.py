@@ -126,6 +126,35 @@ def run_precision(core, prec, omegas, nvel, nds, steps, warmup, nrhs=9, host_inp
     return dt, forces, phases
 
 
+def run_outputs(core, omega=0.9, ntheta=181, ngrid=50, reps=3):
+    """The Solver's optional outputs after one mixed solve of the barge at
+    omega (finite depth): the Kochin function at ntheta angles and the
+    potential on an ngrid x ngrid free-surface grid (the sizes the reference's
+    Postprocessing case asks for), each call timed end to end (sources
+    resident, results copied back) and reported as the median of reps."""
+    from nemoh_amd import core as ncore
+    core.set_precision(ncore.MIXED)
+    core.solve_waves(omega, BETAS)
+    n = core.nfa
+    th = np.linspace(0.0, 2.0 * math.pi, ntheta)
+    xs, ys = np.linspace(-150.3, 149.7, ngrid), np.linspace(-75.3, 74.7, ngrid)
+    pts = np.stack([g.ravel() for g in np.meshgrid(xs, ys)] + [np.zeros(ngrid * ngrid)], axis=1)
+    tk, tp = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        h = core.kochin(th)
+        tk.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        phi = core.potential(pts)
+        tp.append(time.perf_counter() - t0)
+    assert np.all(np.isfinite(h)) and np.all(np.isfinite(phi))
+    mk, mp = float(np.median(tk)), float(np.median(tp))
+    return {"workload": f"barge 4000 panels at w = {omega} rad/s (finite depth), {h.shape[1]} problems: Kochin at "
+                        f"{ntheta} angles, potential at {ngrid}x{ngrid} free-surface points",
+            "kochin_ms": round(1e3 * mk, 3), "kochin_angle_panels_per_s": round(ntheta * n / mk, 1),
+            "potential_ms": round(1e3 * mp, 3), "potential_point_panels_per_s": round(pts.shape[0] * n / mp, 1)}
+
+
 def run_concurrent(m, omegas, nvel, nds, prec, steps, warmup, workers, nrhs=9):
     """The same frequency solves with `workers` frequencies in flight, each
     on its own solver handle and HIP stream (the native request loop): the
@@ -314,6 +343,8 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-outputs", dest="outputs", action="store_false",
+                    help="skip the Kochin / free-surface potential timing")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--workers", type=int, default=4, help="frequencies in flight for the throughput legs")
     ap.add_argument("--c3-steps", type=int, default=3, help="frequencies of the 12k-panel case (0: skip)")
@@ -341,6 +372,7 @@ def main():
     # the drop-in's per-call data flow: NVEL and NDS handed over from host memory
     th, fh, _ = run_precision(c, ncore.MIXED, omegas, nvel, nds, a.steps, a.warmup, host_inputs=True)
     host_same = max(float((np.abs(fh[k] - fmx[k]).max(0) / np.abs(fmx[k]).max(0)).max()) for k in fmx if k in fh)
+    outputs = run_outputs(c) if a.outputs else None
     c.close()
     ss_mixed = a.steps / tmx
     ss_fp64 = fp64_steps / t64
@@ -401,6 +433,8 @@ def main():
         "lu_factor_tflops": round(lu_flops / (fac_ms * 1e-3) / 1e12, 3),
         "assembly_gpairs_per_s": round(n * n / (float(np.median(phmx["assembly"])) * 1e-3) / 1e9, 3),
     }
+    if outputs:
+        out["outputs"] = outputs
     if a.c3_steps > 0:
         out["c3"] = run_c3(a.c3_steps)
     if a.c4_bodies > 0:
