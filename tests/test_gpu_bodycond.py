@@ -96,6 +96,27 @@ def test_solve_waves_matches_file_driven_solve(case, prec):
     assert err < (1e-10 if prec == core.FP64 else 1e-6) and err_fk < 1e-12
 
 
+def test_outputs_after_solve_waves(case):
+    """The Kochin function and the field potential read the sources of the
+    last solve: after nemoh_core_solve_waves (problems generated on the
+    device) they equal those after the file-driven solve of the same
+    problems."""
+    name, c, bc, nds, fk = case
+    c.set_precision(core.FP64)
+    w, cols, betas = _by_frequency(bc)[0]
+    v = np.asarray(c.geo["xg"]), np.asarray(c.geo["yg"])
+    r = 3.0 * max(np.abs(v[0]).max(), np.abs(v[1]).max()) + 1.0
+    t = np.linspace(0.0, 2.0 * np.pi, 16, endpoint=False)
+    pts = np.concatenate([np.stack([r * np.cos(t), r * np.sin(t), np.full(16, z)], axis=1) for z in (0.0, -1.0)])
+    th = np.linspace(0.0, 2.0 * np.pi, 13)
+    c.solve_waves(w, betas)
+    h1, p1 = c.kochin(th), c.potential(pts)
+    c.solve_frequency(2 * np.pi / w, bc.nvel[:, cols], nds=nds, want_pressure=False)
+    h2, p2 = c.kochin(th), c.potential(pts)
+    assert h1.shape == h2.shape == (th.size, cols.size) and p1.shape == p2.shape == (pts.shape[0], cols.size)
+    assert _normwise(h1, h2) < 1e-10 and _normwise(p1, p2) < 1e-10
+
+
 def test_set_dofs_rejects_bad_cases(case):
     name, c, bc, nds, fk = case
     bad = bcm.Mode(3, (1, 0, 0))
