@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # (16 m^2 + 1024 m), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
 # this workload, each scaled by the counter's reading for a 256 MiB copy
 # (profiles/lu_update_barge4000_hbm_traffic.md)
-LU_UPDATE_TRAFFIC_RATIO = 1.117
+LU_UPDATE_TRAFFIC_RATIO = 1.076
 
 
 def workload():

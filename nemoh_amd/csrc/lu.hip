@@ -570,9 +570,11 @@ union UpdLds {
 // dealt to the eight XCDs in turn (workgroup w to XCD w mod 8), each XCD
 // behind its own L2.  Each XCD gets one contiguous run of a tile order that
 // walks bands of kBand tile rows column by column: going down a column reuses
-// one U column panel kBand times, and the band's kBand L row panels (1 MB:
-// 32 KB each in single precision, 64 KB in double) stay in the L2 from one
-// column to the next.  An XCD's run (m^2 / 8 tiles) then reads about one
+// one U column panel kBand times, and the band's kBand L row panels (16 of
+// them: 0.5 MB in single precision, 1 MB in double) stay in the L2 from one
+// column to the next.  Bands of 16, 32 and 64 rows run the single-precision
+// update within 0.5 % of each other at 4000 panels; 64 is 2 % slower at
+// 12,048.  An XCD's run (m^2 / 8 tiles) then reads about one
 // band's worth of L panels and a fraction m / (8 kBand) of the U panels (when
 // below one) instead of all of both, as the row-major order does on every
 // XCD.  Workgroup 0 gets tile (0, 0).  The
@@ -599,7 +601,7 @@ __global__ __launch_bounds__(256) void lu_update_kernel(C2<T>* __restrict__ A, i
   const int s0 = (kb + 1) * kNB, k0 = kb * kNB;
   const int m = (n - s0 + kNB - 1) / kNB;
   int ti, tj;
-  update_tile<sizeof(T) == 4 ? 32 : 16>(blockIdx.x, m, ti, tj);
+  update_tile<16>(blockIdx.x, m, ti, tj);
   const int r0 = s0 + ti * kNB, c0 = s0 + tj * kNB;
   const int nr = min(kNB, n - r0), nc = min(kNB, n - c0);
   C2<T>* blk = A + (size_t)r0 * ld + c0;
