@@ -340,7 +340,7 @@ def run_c4(nbodies, workers, seed=256):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # four passes over the 40 frequencies: with 4 in flight, the start and
+    # four passes over the 40 frequencies: with 8 in flight, the start and
     # the drain of the pipeline are a small share of the timed job
     ap.add_argument("--steps", type=int, default=160)
     ap.add_argument("--warmup", type=int, default=2)
@@ -348,7 +348,9 @@ def main():
     ap.add_argument("--no-outputs", dest="outputs", action="store_false",
                     help="skip the Kochin / free-surface potential timing")
     ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--workers", type=int, default=4, help="frequencies in flight for the throughput legs")
+    # 8 in flight: 316-320 freq-solves/s mixed against 294-304 with 4 and
+    # 298-302 with 6 (160-step jobs on the 4000-panel barge)
+    ap.add_argument("--workers", type=int, default=8, help="frequencies in flight for the throughput legs")
     ap.add_argument("--c3-steps", type=int, default=3, help="frequencies of the 12k-panel case (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=32, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rates", default="5", help="comma-separated arrival rates (req/s)")
