@@ -4,8 +4,10 @@ Workload: rectangular barge 100 x 20 x 10 m (draft 10 m), no symmetry,
 4,000 quad panels (bottom 90 x 20, sides 10 high), finite depth 50 m,
 6 radiation DOF + 3 headings (0/45/90 deg) = 9 right-hand sides,
 40 frequencies uniform in 0.2-2.0 rad/s (2.0 rad/s takes the infinite-depth
-path, k h >= 20).  One step = one frequency solve: influence-matrix assembly,
-factorisation, all 9 solves, phi = S sigma and the force integration.
+path, k h >= 20).  One step = one pass over the 40 frequencies, each a
+frequency solve: influence-matrix assembly, factorisation, all 9 solves,
+phi = S sigma and the force integration.  `value` = 40 x steps / timed
+seconds (freq-solves/s); ms_per_step x steps is the timed region.
 
 `value` is the whole-job rate of the mixed-precision path (complex64 LU on
 MFMA + FP64 refinement) with several frequencies in flight on their own
@@ -155,14 +157,14 @@ def run_outputs(core, omega=0.9, ntheta=181, ngrid=50, reps=3):
             "potential_ms": round(1e3 * mp, 3), "potential_point_panels_per_s": round(pts.shape[0] * n / mp, 1)}
 
 
-def run_concurrent(m, omegas, nvel, nds, prec, steps, warmup, workers, nrhs=9):
-    """The same frequency solves with `workers` frequencies in flight, each
-    on its own solver handle and HIP stream (the native request loop): the
-    latency-bound phases of one frequency (the factorisation's dependent
-    block steps, the triangular-solve chains) leave most compute units idle,
-    and other frequencies fill them.  Results are bit for bit those of the
-    single-stream path.  Returns (seconds for `steps` frequencies, forces by
-    frequency index)."""
+def run_concurrent(m, omegas, prec, passes, warmup, workers):
+    """`passes` passes over every frequency with `workers` frequencies in
+    flight, each on its own solver handle and HIP stream (the native request
+    loop): the latency-bound phases of one frequency (the factorisation's
+    dependent block steps, the triangular-solve chains) leave most compute
+    units idle, and other frequencies fill them.  Results are bit for bit
+    those of the single-stream path.  Returns (seconds for the timed passes,
+    forces by frequency index of the last pass)."""
     from nemoh_amd import bodycond as bcm
     from nemoh_amd import serving
     nw = omegas.size
@@ -171,13 +173,14 @@ def run_concurrent(m, omegas, nvel, nds, prec, steps, warmup, workers, nrhs=9):
         return serving.WaveRequest(m, 50.0, omegas[idx], BETAS, bcm.six_dof())
 
     with serving.Server(workers=workers) as srv:
-        # warm-up: at least one frequency per worker binds every handle
-        srv.wait(srv.submit(request(np.arange(max(warmup, workers)) % nw), prec))
-        idx = np.arange(steps) % nw
+        # warm-up passes (at least one frequency per worker binds every handle)
+        srv.wait(srv.submit(request(np.arange(max(warmup * nw, workers)) % nw), prec))
+        idx = np.arange(passes * nw) % nw
         t0 = time.perf_counter()
         f, _ = srv.wait(srv.submit(request(idx), prec))
         dt = time.perf_counter() - t0
-    return dt, {int(k): f[i] for i, k in enumerate(idx)}
+    last = idx.size - nw
+    return dt, {int(idx[last + i]): f[last + i] for i in range(nw)}
 
 
 REF_HARNESS = ROOT / "oracle" / "_ref" / "harness"
@@ -340,20 +343,22 @@ def run_c4(nbodies, workers, seed=256):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # four passes over the 40 frequencies: with 8 in flight, the start and
-    # the drain of the pipeline are a small share of the timed job
-    ap.add_argument("--steps", type=int, default=160)
-    ap.add_argument("--warmup", type=int, default=2)
+    # steps = passes over the 40 frequencies: with 8 in flight, four passes
+    # (20 waves) make the start and the drain of the pipeline a small share
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-outputs", dest="outputs", action="store_false",
                     help="skip the Kochin / free-surface potential timing")
     ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--single-passes", type=int, default=1,
+                    help="passes over the 40 frequencies of the one-at-a-time legs")
     # 8 in flight: 316-320 freq-solves/s mixed against 294-304 with 4 and
     # 298-302 with 6 (160-step jobs on the 4000-panel barge)
     ap.add_argument("--workers", type=int, default=8, help="frequencies in flight for the throughput legs")
     ap.add_argument("--c3-steps", type=int, default=3, help="frequencies of the 12k-panel case (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=32, help="requests of the serving run (0: skip)")
-    ap.add_argument("--serve-rates", default="5", help="comma-separated arrival rates (req/s)")
+    ap.add_argument("--serve-rates", default="5,10,20,50,100,200", help="comma-separated arrival rates (req/s)")
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--c4-bodies", type=int, default=256, help="bodies of the batched run (0: skip)")
     ap.add_argument("--c4-workers", type=int, default=8)
@@ -370,21 +375,24 @@ def main():
     c = ncore.Core(m.vertices, m.panels, 0, 50.0, 1025.0, 9.81)
     c.set_dofs(bcm.six_dof())
     n = m.npanels
+    nw = omegas.size
     fp64_steps = a.fp64_steps or a.steps
-    t64, f64, ph64 = run_precision(c, ncore.FP64, omegas, nvel, nds, fp64_steps, a.warmup)
-    tmx, fmx, phmx = run_precision(c, ncore.MIXED, omegas, nvel, nds, a.steps, a.warmup)
+    # one frequency at a time: phases, per-kernel roofline, host-input rate
+    ns = a.single_passes * nw
+    t64, f64, ph64 = run_precision(c, ncore.FP64, omegas, nvel, nds, ns, 2)
+    tmx, fmx, phmx = run_precision(c, ncore.MIXED, omegas, nvel, nds, ns, 2)
     # the drop-in's per-call data flow: NVEL and NDS handed over from host memory
-    th, fh, _ = run_precision(c, ncore.MIXED, omegas, nvel, nds, a.steps, a.warmup, host_inputs=True)
+    th, fh, _ = run_precision(c, ncore.MIXED, omegas, nvel, nds, ns, 2, host_inputs=True)
     host_same = max(float((np.abs(fh[k] - fmx[k]).max(0) / np.abs(fmx[k]).max(0)).max()) for k in fmx if k in fh)
     outputs = run_outputs(c) if a.outputs else None
     c.close()
-    ss_mixed = a.steps / tmx
-    ss_fp64 = fp64_steps / t64
-    # whole-job throughput: the same frequencies, a.workers in flight
-    cmx, gmx = run_concurrent(m, omegas, nvel, nds, ncore.MIXED, a.steps, a.warmup, a.workers)
-    c64, g64 = run_concurrent(m, omegas, nvel, nds, ncore.FP64, fp64_steps, a.warmup, a.workers)
-    rate_mixed = a.steps / cmx
-    rate_fp64 = fp64_steps / c64
+    ss_mixed = ns / tmx
+    ss_fp64 = ns / t64
+    # the timed job: a.steps passes over the 40 frequencies, a.workers in flight
+    cmx, gmx = run_concurrent(m, omegas, ncore.MIXED, a.steps, a.warmup, a.workers)
+    c64, g64 = run_concurrent(m, omegas, ncore.FP64, fp64_steps, a.warmup, a.workers)
+    rate_mixed = a.steps * nw / cmx
+    rate_fp64 = fp64_steps * nw / c64
     same = all(np.array_equal(gmx[k], fmx[k]) for k in fmx if k in gmx) and \
         all(np.array_equal(g64[k], f64[k]) for k in f64 if k in g64)
     common = sorted(set(g64) & set(gmx))
@@ -411,13 +419,14 @@ def main():
         "config": {"workload": "barge 100x20x10 m, 4000 panels, no symmetry, depth 50 m, 6 DOF + 3 headings "
                                "(9 RHS), 40 frequencies 0.2-2.0 rad/s", "panels": n, "rhs": 9,
                    "frequencies": int(omegas.size)},
+        "step": f"one pass over the {nw} frequencies ({nw} freq-solves)",
         "fp64_value": round(rate_fp64, 3), "fp64_ms_per_step": round(1e3 * c64 / fp64_steps, 3),
         "speedup_mixed_vs_fp64": round(rate_mixed / rate_fp64, 3),
         "max_rel_err_mixed_vs_fp64": err,
         "frequencies_in_flight": a.workers,
-        "single_stream": {"mixed": round(ss_mixed, 3), "fp64": round(ss_fp64, 3),
+        "single_stream": {"passes": a.single_passes, "mixed": round(ss_mixed, 3), "fp64": round(ss_fp64, 3),
                           "identical_to_concurrent": bool(same),
-                          "mixed_host_inputs": round(a.steps / th, 3),
+                          "mixed_host_inputs": round(ns / th, 3),
                           "host_vs_device_inputs_max_rel_diff": host_same},
         "inputs": "problems generated in HBM by nemoh_core_solve_waves (no per-frequency PCIe transfer); "
                   "mixed_host_inputs = the same solves with NVEL/NDS handed over from host memory",

@@ -82,7 +82,12 @@ struct nemoh_core {
   double2* d_force = nullptr;
   double* d_red = nullptr;  // reduction scratch
   int* d_info = nullptr;     // [0] first bad pivot row, [1] solve wait timeout
-  int* d_flags = nullptr;    // block flags of the single-launch triangular solves
+  RefineState* d_rs = nullptr;  // mixed refinement state of each symmetry block
+  // flags of the single-launch triangular solves (2 ceil(n/64)), then the
+  // two triangles' ticket counters; chain_calls = chained solves since the
+  // last binding (each advances both counters by ceil(n/64))
+  int* d_flags = nullptr;
+  unsigned chain_calls = 0;
   int epoch = 0;
   int cu_count = 0;
   bool stepwise_solve = false;
@@ -272,94 +277,150 @@ int prepare_frequency(nemoh_core* c, double w) {
   return NEMOH_OK;
 }
 
-double device_absmax(nemoh_core* c, const double2* a, size_t count, int* err) {
-  const int nblk = 64;
-  launch_absmax(a, count, c->d_red, nblk, c->st);
-  double h[64];
-  if (hipMemcpyAsync(h, c->d_red, sizeof h, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
-      hipStreamSynchronize(c->st) != hipSuccess) {
-    *err = NEMOH_ERR_HIP;
-    return 0;
-  }
-  double m = 0;
-  for (int k = 0; k < nblk; ++k) m = std::fmax(m, h[k]);
-  return m;
-}
-
-// A x = b in place in B (W scratch): one launch per triangle when every row
-// block can have its own CU, else one launch per block step.
+// A x = b in place in B (W scratch): one flag-chained launch per triangle,
+// or one launch per block step (forced by NEMOH_OPT_STEPWISE_SOLVE, or after
+// a chained solve's bounded wait timed out).  skip: a refinement state whose
+// `done` turns the launches into no-ops.
 template <typename T>
-void lu_solve(nemoh_core* c, const C2<T>* A, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs) {
+void lu_solve(nemoh_core* c, const C2<T>* A, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool stepwise,
+              const RefineState* skip = nullptr) {
   const int n = c->nfa;
   const int nblk = (n + kNB - 1) / kNB;
-  const bool chained = nblk <= c->cu_count && !c->stepwise_solve;
-  if (chained) ++c->epoch;
+  const bool chained = !stepwise;
+  unsigned base = 0;
+  if (chained) {
+    if (++c->epoch == 0) ++c->epoch;  // flags start at 0: an epoch is never 0
+    base = c->chain_calls++ * (unsigned)nblk;
+  }
   // about a second of polling before a wait gives up
-  launch_lu_solve<T>(A, n, n, P, B, W, nrhs, chained, c->d_flags, c->epoch, c->d_info + 1, 1 << 22, c->st);
+  launch_lu_solve<T>(A, n, n, P, B, W, nrhs, chained, c->d_flags, c->epoch,
+                     reinterpret_cast<unsigned*>(c->d_flags + 2 * (size_t)nblk), base, c->d_info + 1, 1 << 22, c->st,
+                     skip);
 }
 
-// Solve block b for the right-hand sides in c->B (n x nrhs, row-major):
-// solution into c->Z[b].
-int solve_block(nemoh_core* c, int b, int nrhs) {
+// Mixed path, one refinement sweep of symmetry block b, every kernel a no-op
+// once the block's state is done: x (+)= rmax A32^-1 (r / rmax) in complex64,
+// then the FP64 residual r = b - D x and its convergence test.
+void mixed_sweep(nemoh_core* c, int b, int nrhs, bool first, bool stepwise) {
+  const int n = c->nfa;
+  const size_t cnt = (size_t)n * nrhs;
+  RefineState* rs = c->d_rs + b;
+  launch_f64_to_f32(c->R, c->R32, cnt, rs, c->st);
+  lu_solve(c, reinterpret_cast<const C2<float>*>(c->D32[b]), reinterpret_cast<const C2<float>*>(c->Pinv32[b]),
+           reinterpret_cast<C2<float>*>(c->R32), reinterpret_cast<C2<float>*>(c->W32), nrhs, stepwise, rs);
+  launch_f32_to_f64(c->R32, c->Z[b], cnt, rs, first ? 0 : 1, c->st);
+  launch_copy(c->B, c->R, cnt, rs, c->st);
+  launch_matvec(c->D[b], n, n, c->Z[b], nrhs, c->R, -1.0, 1.0, c->st, rs);
+  launch_refine_residual_max(c->R, cnt, c->d_red, rs, c->tol, 0, c->st);
+}
+
+// Sweeps queued per block before the host looks at convergence: two is what
+// the 1e-8 backward error takes on every workload measured (the host adds
+// sweeps one at a time when a block needs more).
+constexpr int kQueuedSweeps = 2;
+
+// Solves symmetry block b for the right-hand sides in c->B (n x nrhs,
+// row-major) into c->Z[b] and forms phi = S sigma into c->F[b].
+void solve_block(nemoh_core* c, int b, int nrhs, bool stepwise) {
   const int n = c->nfa;
   const size_t cnt = (size_t)n * nrhs;
   if (c->prec == NEMOH_FP64) {
-    HIPCHK(c, hipMemcpyAsync(c->Z[b], c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    launch_copy(c->B, c->Z[b], cnt, nullptr, c->st);
     lu_solve(c, reinterpret_cast<const C2<double>*>(c->D[b]), reinterpret_cast<const C2<double>*>(c->Pinv[b]),
-             reinterpret_cast<C2<double>*>(c->Z[b]), reinterpret_cast<C2<double>*>(c->W), nrhs);
-    HIPCHK(c, hipGetLastError());
-    return NEMOH_OK;
+             reinterpret_cast<C2<double>*>(c->Z[b]), reinterpret_cast<C2<double>*>(c->W), nrhs, stepwise);
+  } else {
+    RefineState* rs = c->d_rs + b;
+    launch_refine_reset(rs, c->st);
+    launch_copy(c->B, c->R, cnt, nullptr, c->st);
+    launch_refine_residual_max(c->B, cnt, c->d_red, rs, c->tol, 1, c->st);
+    for (int k = 0; k < kQueuedSweeps && k < c->max_iter; ++k) mixed_sweep(c, b, nrhs, k == 0, stepwise);
   }
-  // mixed: complex64 LU solve, FP64 residual r = b - D x, refine
-  HIPCHK(c, hipMemcpyAsync(c->R, c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
-  int e = 0;
-  const double bmax = device_absmax(c, c->B, cnt, &e);
-  if (e) return fail(c, e, "reduction failed");
-  if (bmax == 0.0) {  // zero right-hand sides (e.g. one symmetry block of a symmetric mode): x = 0
-    HIPCHK(c, hipMemsetAsync(c->Z[b], 0, cnt * sizeof(double2), c->st));
-    return NEMOH_OK;
+  launch_matvec(c->S[b], n, n, c->Z[b], nrhs, c->F[b], 1.0, 0.0, c->st);
+}
+
+// The pivot report first, then a non-finite or unconverged refinement.
+int refine_outcome(nemoh_core* c, const RefineState* h, int pivot_row) {
+  if (pivot_row) {
+    c->w_prev = -1.0;
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "PIVOT INFERIEUR A 1e-20 (row %d)", pivot_row);
+    return fail(c, NEMOH_ERR_PIVOT, buf);
   }
-  double rmax = bmax;
-  int it = 0;
-  for (; it < c->max_iter; ++it) {
-    if (rmax == 0.0) break;
-    launch_f64_to_f32(c->R, c->R32, cnt, 1.0 / rmax, c->st);
-    lu_solve(c, reinterpret_cast<const C2<float>*>(c->D32[b]), reinterpret_cast<const C2<float>*>(c->Pinv32[b]),
-             reinterpret_cast<C2<float>*>(c->R32), reinterpret_cast<C2<float>*>(c->W32), nrhs);
-    launch_f32_to_f64(c->R32, c->Z[b], cnt, rmax, it > 0, c->st);
-    HIPCHK(c, hipMemcpyAsync(c->R, c->B, cnt * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
-    launch_matvec(c->D[b], n, n, c->Z[b], nrhs, c->R, -1.0, 1.0,
-                          c->st);
-    HIPCHK(c, hipGetLastError());
-    rmax = device_absmax(c, c->R, cnt, &e);
-    if (e) return fail(c, e, "reduction failed");
-    if (rmax <= c->tol * bmax) {
-      ++it;
-      break;
+  if (c->prec != NEMOH_MIXED) return NEMOH_OK;
+  for (int b = 0; b < c->nb; ++b) {
+    c->stats.refine_iterations = std::max(c->stats.refine_iterations, h[b].iters);
+    if (h[b].bmax > 0) c->stats.refine_residual = std::fmax(c->stats.refine_residual, h[b].rmax / h[b].bmax);
+  }
+  for (int b = 0; b < c->nb; ++b) {
+    char buf[160];
+    if (h[b].bad) {
+      std::snprintf(buf, sizeof buf, "refinement met a non-finite %s (max = %g)",
+                    h[b].iters == 0 && h[b].bmax == 0 ? "right-hand side" : "residual", h[b].rmax);
+      return fail(c, NEMOH_ERR_CONVERGENCE, buf);
     }
-  }
-  c->stats.refine_iterations = std::max(c->stats.refine_iterations, it);
-  c->stats.refine_residual = std::fmax(c->stats.refine_residual, bmax > 0 ? rmax / bmax : 0.0);
-  if (rmax > c->tol * bmax && bmax > 0) {
-    char buf[128];
-    std::snprintf(buf, sizeof buf, "refinement stopped at max|r|/max|b| = %.3e after %d iterations", rmax / bmax, it);
-    return fail(c, NEMOH_ERR_CONVERGENCE, buf);
+    if (!h[b].done) {
+      std::snprintf(buf, sizeof buf, "refinement stopped at max|r|/max|b| = %.3e after %d iterations",
+                    h[b].bmax > 0 ? h[b].rmax / h[b].bmax : 0.0, h[b].iters);
+      return fail(c, NEMOH_ERR_CONVERGENCE, buf);
+    }
   }
   return NEMOH_OK;
 }
 
 // Everything after the frequency set-up: c->V holds (ntot x nrhs) row-major
-// normal velocities on the device.
+// normal velocities on the device.  The solves of every symmetry block, phi
+// and the combination are queued without a host wait; one wait at the end
+// reads the pivot report, the chained solves' timeout flag and the mixed
+// refinement states.  A block that needs more sweeps gets them one at a time;
+// a timed-out chained solve (which the ticket order makes unexpected) is
+// redone with one launch per block step.
 int run_pipeline(nemoh_core* c, double w, int nrhs) {
   if (int r = prepare_frequency(c, w)) return r;
   const int n = c->nfa;
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
   c->stats.refine_iterations = 0;
   c->stats.refine_residual = 0;
-  for (int b = 0; b < c->nb; ++b) {
-    launch_rhs(c->V, n, c->nsym, nrhs, b == 0 ? 1.0 : -1.0, c->B, c->st);
-    if (int r = solve_block(c, b, nrhs)) return r;
-    launch_matvec(c->S[b], n, n, c->Z[b], nrhs, c->F[b], 1.0, 0.0, c->st);
+  bool stepwise = c->stepwise_solve;
+  const size_t cnt = (size_t)n * nrhs;
+  for (int attempt = 0;; ++attempt) {
+    for (int b = 0; b < c->nb; ++b) {
+      launch_rhs(c->V, n, c->nsym, nrhs, b == 0 ? 1.0 : -1.0, c->B, c->st);
+      solve_block(c, b, nrhs, stepwise);
+    }
+    HIPCHK(c, hipGetLastError());
+    RefineState h[2] = {};
+    int info[2] = {0, 0};
+    for (;;) {
+      HIPCHK(c, hipMemcpyAsync(info, c->d_info, sizeof info, hipMemcpyDeviceToHost, c->st));
+      if (c->prec == NEMOH_MIXED)
+        HIPCHK(c, hipMemcpyAsync(h, c->d_rs, c->nb * sizeof(RefineState), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipStreamSynchronize(c->st));
+      if (info[0] || info[1] || c->prec != NEMOH_MIXED) break;
+      int more = -1;
+      for (int b = 0; b < c->nb; ++b)
+        if (!h[b].done && !h[b].bad && h[b].iters < c->max_iter) more = b;
+      if (more < 0) break;
+      // one more sweep of that block: its right-hand side and residual again
+      // (the buffers were reused by the blocks after it), then phi
+      launch_rhs(c->V, n, c->nsym, nrhs, more == 0 ? 1.0 : -1.0, c->B, c->st);
+      launch_copy(c->B, c->R, cnt, nullptr, c->st);
+      launch_matvec(c->D[more], n, n, c->Z[more], nrhs, c->R, -1.0, 1.0, c->st);
+      mixed_sweep(c, more, nrhs, false, stepwise);
+      launch_matvec(c->S[more], n, n, c->Z[more], nrhs, c->F[more], 1.0, 0.0, c->st);
+      HIPCHK(c, hipGetLastError());
+    }
+    if (info[1] && !info[0] && attempt == 0 && !stepwise) {
+      const int zero = 0;
+      HIPCHK(c, hipMemcpyAsync(c->d_info + 1, &zero, sizeof zero, hipMemcpyHostToDevice, c->st));
+      stepwise = true;
+      continue;
+    }
+    if (info[1]) {
+      c->w_prev = -1.0;
+      return fail(c, NEMOH_ERR_HIP, "triangular solve: a bounded wait for a solved block timed out");
+    }
+    if (int r = refine_outcome(c, h, info[0])) return r;
+    break;
   }
   launch_combine(c->Z[0], c->Z[1], c->F[0], c->F[1], n, c->nsym, nrhs, c->rho * w, c->sig, c->pot, c->pres, c->st);
   HIPCHK(c, hipGetLastError());
@@ -543,7 +604,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   int r = 0;
   if ((r = dalloc(c, &c->d_geo, 8 * (size_t)nfa)) || (r = dalloc(c, &c->d_vtx, 12 * (size_t)nfa)) ||
       (r = dalloc(c, &c->d_surf, surf.size())) ||
-      (r = dalloc(c, &c->d_flags, 2 * (size_t)nblk)) || (r = dalloc(c, &c->d_pts, 2 * (size_t)np)) ||
+      (r = dalloc(c, &c->d_flags, 2 * (size_t)nblk + 2)) || (r = dalloc(c, &c->d_pts, 2 * (size_t)np)) ||
       (r = dalloc(c, &c->d_ext, nred)))
     return r;
   for (int k = 0; k < 8; ++k)
@@ -554,8 +615,10 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   HIPCHK(c, hipMemcpyAsync(c->d_pts, x, np * sizeof(double), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->d_pts + np, y, np * sizeof(double), hipMemcpyHostToDevice, c->st));
   // flags of the chained triangular solves: cleared over the whole buffer
-  // (solves compare them with an epoch that only grows)
+  // (solves compare them with an epoch that is never 0), ticket counters
+  // with them
   HIPCHK(c, hipMemsetAsync(c->d_flags, 0, c->caps[(const void*)&c->d_flags] * sizeof(int), c->st));
+  c->chain_calls = 0;
   // GM = max over vertex pairs of the horizontal distance (and to the y
   // images when nsym = 1), the argument of the theta rule
   launch_extent(c->d_pts, c->d_pts + np, np, nsym, c->d_ext, c->st);
@@ -669,7 +732,8 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
       hipMalloc((void**)&c->d_xz, kNZ * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_par, sizeof(FreqParams)) != hipSuccess ||
       hipMalloc((void**)&c->d_red, 64 * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&c->d_info, 2 * sizeof(int)) != hipSuccess)
+      hipMalloc((void**)&c->d_info, 2 * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&c->d_rs, 2 * sizeof(RefineState)) != hipSuccess)
     return bail(NEMOH_ERR_HIP);
   {
     int dev = 0;
@@ -706,6 +770,7 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->d_par);
   release(c->d_red);
   release(c->d_info);
+  release(c->d_rs);
   release(c->d_flags);
   release(c->d_nds);
   release(c->d_force);

@@ -25,6 +25,15 @@ struct MatOut {
   float2* dm32;
 };
 
+// Iterative refinement of one symmetry block in device memory: bmax = max|b|,
+// rmax = max|r| of the current residual, iters = sweeps done, done = 1 once
+// rmax <= tol bmax (or on a non-finite residual, bad = 1).  Kernels given a
+// state return at once when it is done.
+struct RefineState {
+  double bmax, rmax;
+  int done, iters, bad, pad;
+};
+
 void launch_crek(const double* xr, const double* xz, double* tab, hipStream_t st);
 // fd: the regime of *P (finite depth 1, infinite 0), known to the host.
 // surf[0..nsurf) (device): the panels with zg >= ZER, whose rows and columns
@@ -45,23 +54,35 @@ void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, hip
 // Solves A x = b for nrhs <= kMaxRhs right-hand sides (B: n x nrhs
 // row-major) with the factors of launch_lu; on return B holds the solution,
 // W is scratch of the same size.  chained: one launch per triangle whose
-// workgroups pass solved blocks through flags (requires every row block to
-// be resident at once; flags: 2*ceil(n/64) ints, never reset, epoch must
-// increase per call; status set nonzero if a bounded wait timed out); else
-// one launch per block step.
+// workgroups take row blocks in ticket order and pass solved blocks through
+// flags (flags: 2*ceil(n/64) ints, never reset, epoch must change per call
+// and never be 0; tickets: 2 counters, both equal to base at the call, each
+// advanced by ceil(n/64); status set nonzero if a bounded wait timed out);
+// else one launch per block step.  Skipped when skip (optional) is done.
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
-                     int* flags, int epoch, int* status, int spin_limit, hipStream_t st);
-// Y (n x nrhs) = beta Y + alpha M X, FP64, nrhs <= kMaxRhs, all row-major.
+                     int* flags, int epoch, unsigned* tickets, unsigned base, int* status, int spin_limit,
+                     hipStream_t st, const RefineState* skip = nullptr);
+// Y (n x nrhs) = beta Y + alpha M X, FP64, nrhs <= kMaxRhs, all row-major;
+// skipped when skip (optional) is done.
 void launch_matvec(const double2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
-                   double beta, hipStream_t st);
+                   double beta, hipStream_t st, const RefineState* skip = nullptr);
 void launch_rhs(const double2* V, int n, int nsym, int nrhs, double bx, double2* B, hipStream_t st);
 void launch_combine(const double2* Z1, const double2* Z2, const double2* F1, const double2* F2, int n, int nsym,
                     int nrhs, double rho_w, double2* sig, double2* pot, double2* pres, hipStream_t st);
 void launch_force(const double2* pres, const double* nds, int nint, int ntot, int nrhs, double2* F, hipStream_t st);
-void launch_f64_to_f32(const double2* a, float2* b, size_t count, double scale, hipStream_t st);
-void launch_f32_to_f64(const float2* a, double2* b, size_t count, double scale, int accumulate, hipStream_t st);
-void launch_absmax(const double2* a, size_t count, double* out, int nblocks, hipStream_t st);
+// b = a / rs->rmax (single precision); b (+)= rs->rmax a; b = a.  All skipped
+// once rs is done (copy: when skip is given and done).
+void launch_f64_to_f32(const double2* a, float2* b, size_t count, const RefineState* rs, hipStream_t st);
+void launch_f32_to_f64(const float2* a, double2* b, size_t count, const RefineState* rs, int accumulate,
+                       hipStream_t st);
+void launch_copy(const double2* a, double2* b, size_t count, const RefineState* skip, hipStream_t st);
+// max|r| (NaN kept) into rs: first = the right-hand side before any sweep
+// (sets bmax and rmax), else the residual after a sweep (iters + 1, done
+// when rmax <= tol bmax); part: 64 doubles of scratch.
+void launch_refine_residual_max(const double2* r, size_t count, double* part, RefineState* rs, double tol, int first,
+                                hipStream_t st);
+void launch_refine_reset(RefineState* rs, hipStream_t st);
 // Degree of freedom of the preProcessor's radiation / force cases: body
 // (1-based), kind 1 = translation along dir, 2 = rotation about the axis
 // through `axis` along dir (same layout as nemoh_dof of the C ABI).

@@ -146,7 +146,9 @@ __global__ void pairs_kernel(Geometry G, Tables T, const FreqParams* __restrict_
 constexpr int kGemvWaves = 16;
 __global__ __launch_bounds__(64 * kGemvWaves) void gemv_kernel(const double2* __restrict__ M, int ld, int n,
                                                                const double2* __restrict__ X, int nrhs,
-                                                               double2* __restrict__ Y, double alpha, double beta) {
+                                                               double2* __restrict__ Y, double alpha, double beta,
+                                                               const RefineState* __restrict__ rs) {
+  if (rs && rs->done) return;
   __shared__ double red[kGemvWaves][2][16][17];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
   const int r0 = blockIdx.x * 16;
@@ -276,32 +278,85 @@ __global__ __launch_bounds__(256) void force_kernel(const double2* __restrict__ 
   if (threadIdx.x == 0) F[(size_t)k * nrhs + q] = make_double2(sr[0], si[0]);
 }
 
-// Precision conversions for the mixed path.
-__global__ void f64_to_f32_kernel(const double2* __restrict__ a, float2* __restrict__ b, size_t count, double scale) {
+// Mixed-path refinement on the device.  The sweep's kernels read the
+// block's RefineState: the residual scale rmax, and `done`, after which they
+// return at once, so the host can queue sweeps without waiting for each
+// convergence test.
+//
+// max that keeps NaN (fmax would drop it, and an all-NaN residual would
+// then look converged)
+__device__ __forceinline__ double nan_max(double m, double v) { return (v > m || v != v) ? v : m; }
+
+// R32 = R / rmax (R32 = 0 when rmax = 0: an all-zero right-hand side)
+__global__ void f64_to_f32_kernel(const double2* __restrict__ a, float2* __restrict__ b, size_t count,
+                                  const RefineState* __restrict__ rs) {
+  if (rs->done) return;
+  const double scale = rs->rmax > 0.0 ? 1.0 / rs->rmax : 0.0;
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (e < count) b[e] = make_float2((float)(a[e].x * scale), (float)(a[e].y * scale));
 }
+// Z (+)= rmax * X32
 __global__ void axpy_f32_to_f64_kernel(const float2* __restrict__ a, double2* __restrict__ b, size_t count,
-                                       double scale, int accumulate) {
+                                       const RefineState* __restrict__ rs, int accumulate) {
+  if (rs->done) return;
+  const double scale = rs->rmax;
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (e < count) {
     const double2 v = make_double2((double)a[e].x * scale, (double)a[e].y * scale);
     b[e] = accumulate ? make_double2(b[e].x + v.x, b[e].y + v.y) : v;
   }
 }
-// max |a| over count complex entries into out[blockIdx.x] (per-block maxima)
-__global__ __launch_bounds__(256) void absmax_kernel(const double2* __restrict__ a, size_t count, double* __restrict__ out) {
+__global__ void copy_c128_kernel(const double2* __restrict__ a, double2* __restrict__ b, size_t count,
+                                 const RefineState* __restrict__ rs) {
+  if (rs && rs->done) return;
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e < count) b[e] = a[e];
+}
+// max |a| over count complex entries into out[blockIdx.x] (per-block maxima,
+// NaN kept)
+__global__ __launch_bounds__(256) void absmax_kernel(const double2* __restrict__ a, size_t count,
+                                                     double* __restrict__ out, const RefineState* __restrict__ rs) {
+  if (rs && rs->done) return;
   __shared__ double s[256];
   double m = 0.0;
   for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < count; e += (size_t)gridDim.x * 256)
-    m = fmax(m, fmax(fabs(a[e].x), fabs(a[e].y)));
+    m = nan_max(nan_max(m, fabs(a[e].x)), fabs(a[e].y));
   s[threadIdx.x] = m;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
-    if (threadIdx.x < k) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + k]);
+    if (threadIdx.x < k) s[threadIdx.x] = nan_max(s[threadIdx.x], s[threadIdx.x + k]);
     __syncthreads();
   }
   if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+// Convergence test after a sweep (first: the right-hand side's max before
+// any sweep).  One wave reduces the partial maxima of absmax_kernel.
+__global__ void refine_check_kernel(const double* __restrict__ part, int nparts, RefineState* __restrict__ rs,
+                                    double tol, int first) {
+  if (rs->done) return;
+  double m = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += 64) m = nan_max(m, part[k]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
+  if (threadIdx.x != 0) return;
+  if (!(m <= 1.79e308)) {  // NaN or infinite: stop, the host reports it
+    rs->bad = 1;
+    rs->rmax = m;
+    rs->done = 1;
+    return;
+  }
+  rs->rmax = m;
+  if (first) {
+    rs->bmax = m;
+    rs->iters = 0;
+    return;
+  }
+  rs->iters += 1;
+  if (m <= tol * rs->bmax) rs->done = 1;
+}
+__global__ void refine_reset_kernel(RefineState* __restrict__ rs) {
+  rs->bmax = rs->rmax = 0.0;
+  rs->done = rs->iters = rs->bad = 0;
 }
 
 // Largest squared horizontal vertex distance (GM^2 of
@@ -471,8 +526,8 @@ void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int f
 }
 
 void launch_matvec(const double2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
-                   double beta, hipStream_t st) {
-  gemv_kernel<<<(n + 15) / 16, 64 * kGemvWaves, 0, st>>>(M, ld, n, X, nrhs, Y, alpha, beta);
+                   double beta, hipStream_t st, const RefineState* skip) {
+  gemv_kernel<<<(n + 15) / 16, 64 * kGemvWaves, 0, st>>>(M, ld, n, X, nrhs, Y, alpha, beta, skip);
 }
 
 void launch_rhs(const double2* V, int n, int nsym, int nrhs, double bx, double2* B, hipStream_t st) {
@@ -488,15 +543,23 @@ void launch_force(const double2* pres, const double* nds, int nint, int ntot, in
   force_kernel<<<nint * nrhs, 256, 0, st>>>(pres, nds, ntot, nrhs, F);
 }
 
-void launch_f64_to_f32(const double2* a, float2* b, size_t count, double scale, hipStream_t st) {
-  f64_to_f32_kernel<<<(unsigned)((count + 255) / 256), 256, 0, st>>>(a, b, count, scale);
+void launch_f64_to_f32(const double2* a, float2* b, size_t count, const RefineState* rs, hipStream_t st) {
+  f64_to_f32_kernel<<<(unsigned)((count + 255) / 256), 256, 0, st>>>(a, b, count, rs);
 }
-void launch_f32_to_f64(const float2* a, double2* b, size_t count, double scale, int accumulate, hipStream_t st) {
-  axpy_f32_to_f64_kernel<<<(unsigned)((count + 255) / 256), 256, 0, st>>>(a, b, count, scale, accumulate);
+void launch_f32_to_f64(const float2* a, double2* b, size_t count, const RefineState* rs, int accumulate,
+                       hipStream_t st) {
+  axpy_f32_to_f64_kernel<<<(unsigned)((count + 255) / 256), 256, 0, st>>>(a, b, count, rs, accumulate);
 }
-void launch_absmax(const double2* a, size_t count, double* out, int nblocks, hipStream_t st) {
-  absmax_kernel<<<nblocks, 256, 0, st>>>(a, count, out);
+void launch_copy(const double2* a, double2* b, size_t count, const RefineState* skip, hipStream_t st) {
+  copy_c128_kernel<<<(unsigned)((count + 255) / 256), 256, 0, st>>>(a, b, count, skip);
 }
+void launch_refine_residual_max(const double2* r, size_t count, double* part, RefineState* rs, double tol, int first,
+                                hipStream_t st) {
+  constexpr int kParts = 64;
+  absmax_kernel<<<kParts, 256, 0, st>>>(r, count, part, first ? nullptr : rs);
+  refine_check_kernel<<<1, 64, 0, st>>>(part, kParts, rs, tol, first);
+}
+void launch_refine_reset(RefineState* rs, hipStream_t st) { refine_reset_kernel<<<1, 1, 0, st>>>(rs); }
 
 void launch_kochin(const Geometry& G, const double2* sig, int nsym, int nrhs, double k, double depth, double xeff,
                    double yeff, const double* theta, int ntheta, double2* H, hipStream_t st) {

@@ -749,27 +749,47 @@ __device__ __forceinline__ void hop_acc_store(const typename MfmaTraits<T>::acc_
   }
 }
 
-// Single-launch triangular solve.  Workgroup w owns row block b (forward:
-// b = w, backward: b = nblk-1-w, so blocks without dependencies come first)
-// and subtracts M_bj x_j for every solved block j in dependency order,
-// waiting for block j's flag, then publishes x_b.  The host launches it only
-// when every row block can be resident at once.  Every wait is bounded: after
-// spin_limit polls the workgroup records a timeout in *status and stops
-// waiting, so the kernel always ends (the host then reports an error).
-// R: right-hand sides (forward) or the forward solution y (backward); X gets
-// the solution; flags: nblk ints of this triangle, set to epoch when solved.
+// Single-launch triangular solve.  Each workgroup takes the next ticket w of
+// the launch from a counter (atomic add) and owns row block b (forward:
+// b = w, backward: b = nblk-1-w, so blocks without dependencies come first);
+// it subtracts M_bj x_j for every solved block j in dependency order, waiting
+// for block j's flag, then publishes x_b.  A workgroup only ever waits for
+// blocks whose tickets were taken before its own, i.e. by workgroups that
+// are already running, so the chain makes progress however few of its
+// workgroups are resident (other streams' kernels and other chains may hold
+// the rest of the GPU): the lowest unsolved block never waits.  Every wait
+// is still bounded: after spin_limit polls the workgroup records a timeout
+// in *status and stops waiting, so the kernel always ends.  R: right-hand
+// sides (forward) or the forward solution y (backward); X gets the
+// solution; flags: nblk ints of this triangle, set to epoch when solved;
+// ticket: the triangle's counter, which holds base when the launch starts
+// (every launch takes exactly nblk tickets).
 template <typename T, bool kLower>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict__ A, int ld, int n,
                                                          const C2<T>* __restrict__ P, const C2<T>* __restrict__ R,
                                                          C2<T>* __restrict__ X, int nrhs, int* __restrict__ flags,
-                                                         int epoch, int* __restrict__ status, int spin_limit) {
+                                                         int epoch, unsigned* __restrict__ ticket, unsigned base,
+                                                         int* __restrict__ status, int spin_limit,
+                                                         const RefineState* __restrict__ skip) {
   typedef typename MfmaTraits<T>::acc_t acc_t;
-  __shared__ int timed_out;
+  __shared__ int timed_out, my_ticket;
   const int tid = threadIdx.x;
   const int nblk = (n + kNB - 1) / kNB;
-  const int b = kLower ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+  if (tid == 0) {
+    timed_out = 0;
+    my_ticket = (int)(atomicAdd(ticket, 1u) - base);  // unsigned: exact across wrap-around
+  }
+  __syncthreads();
+  // a converged refinement: the ticket is taken (every launch takes nblk) and
+  // nothing else is done; every workgroup reads the same state
+  if (skip && skip->done) return;
+  const int w = my_ticket;
+  if (w < 0 || w >= nblk) {  // a counter out of step with the launches: never wait on it
+    if (tid == 0) atomicMax(status, 1);
+    return;
+  }
+  const int b = kLower ? w : nblk - 1 - w;
   const int k0 = b * kNB, nb = min(kNB, n - k0);
-  if (tid == 0) timed_out = 0;
   acc_t cr, ci;
   HopTile<T> h;
   T xr[16], xi[16];
@@ -817,8 +837,10 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict
 // R_i -= M_is R_s in place (R_s is final).
 template <typename T, bool kLower>
 __global__ __launch_bounds__(256) void trsv_step_kernel(const C2<T>* __restrict__ A, int ld, int n,
-                                                        C2<T>* __restrict__ R, int nrhs, int s) {
+                                                        C2<T>* __restrict__ R, int nrhs, int s,
+                                                        const RefineState* __restrict__ skip) {
   typedef typename MfmaTraits<T>::acc_t acc_t;
+  if (skip && skip->done) return;
   const int i = kLower ? s + 1 + (int)blockIdx.x : (int)blockIdx.x;
   const int i0 = i * kNB, ni = min(kNB, n - i0);
   const int s0 = s * kNB, ns = min(kNB, n - s0);
@@ -836,8 +858,9 @@ __global__ __launch_bounds__(256) void trsv_step_kernel(const C2<T>* __restrict_
 template <typename T>
 __global__ __launch_bounds__(256) void block_apply_kernel(const C2<T>* __restrict__ P, int n,
                                                           const C2<T>* __restrict__ Y, C2<T>* __restrict__ X,
-                                                          int nrhs) {
+                                                          int nrhs, const RefineState* __restrict__ skip) {
   typedef typename MfmaTraits<T>::acc_t acc_t;
+  if (skip && skip->done) return;
   const int b = blockIdx.x, k0 = b * kNB, nb = min(kNB, n - k0);
   acc_t cr, ci;
 #pragma unroll
@@ -848,6 +871,14 @@ __global__ __launch_bounds__(256) void block_apply_kernel(const C2<T>* __restric
   hop_load_x<T, false>(xr, xi, Y + (size_t)k0 * nrhs, nb, nrhs);
   hop_mma<T>(cr, ci, h, xr, xi);
   hop_acc_store<T, false>(cr, ci, X + (size_t)k0 * nrhs, nb, nrhs);
+}
+
+template <typename T>
+__global__ void copy_c2_kernel(const C2<T>* __restrict__ a, C2<T>* __restrict__ b, size_t count,
+                               const RefineState* __restrict__ skip) {
+  if (skip && skip->done) return;
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e < count) b[e] = a[e];
 }
 
 // MFMA layout check (tests): D = A(16x4) B(4x16) with the lane mapping the
@@ -887,23 +918,28 @@ template void launch_lu<float>(C2<float>*, int, int, C2<float>*, int*, hipStream
 
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
-                     int* flags, int epoch, int* status, int spin_limit, hipStream_t st) {
+                     int* flags, int epoch, unsigned* tickets, unsigned base, int* status, int spin_limit,
+                     hipStream_t st, const RefineState* skip) {
   const int nblk = (n + kNB - 1) / kNB;
   if (chained) {
-    trsv_chain_kernel<T, true><<<nblk, 256, 0, st>>>(A, ld, n, P, B, W, nrhs, flags, epoch, status, spin_limit);
-    trsv_chain_kernel<T, false>
-        <<<nblk, 256, 0, st>>>(A, ld, n, P, W, B, nrhs, flags + nblk, epoch, status, spin_limit);
+    trsv_chain_kernel<T, true>
+        <<<nblk, 256, 0, st>>>(A, ld, n, P, B, W, nrhs, flags, epoch, tickets, base, status, spin_limit, skip);
+    trsv_chain_kernel<T, false><<<nblk, 256, 0, st>>>(A, ld, n, P, W, B, nrhs, flags + nblk, epoch, tickets + 1,
+                                                      base, status, spin_limit, skip);
     return;
   }
-  for (int s = 0; s + 1 < nblk; ++s) trsv_step_kernel<T, true><<<nblk - 1 - s, 256, 0, st>>>(A, ld, n, B, nrhs, s);
-  block_apply_kernel<T><<<nblk, 256, 0, st>>>(P, n, B, W, nrhs);
-  for (int s = nblk - 1; s >= 1; --s) trsv_step_kernel<T, false><<<s, 256, 0, st>>>(A, ld, n, W, nrhs, s);
-  (void)hipMemcpyAsync(B, W, (size_t)n * nrhs * sizeof(C2<T>), hipMemcpyDeviceToDevice, st);
+  for (int s = 0; s + 1 < nblk; ++s)
+    trsv_step_kernel<T, true><<<nblk - 1 - s, 256, 0, st>>>(A, ld, n, B, nrhs, s, skip);
+  block_apply_kernel<T><<<nblk, 256, 0, st>>>(P, n, B, W, nrhs, skip);
+  for (int s = nblk - 1; s >= 1; --s) trsv_step_kernel<T, false><<<s, 256, 0, st>>>(A, ld, n, W, nrhs, s, skip);
+  const size_t cnt = (size_t)n * nrhs;
+  copy_c2_kernel<T><<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(W, B, cnt, skip);
 }
 template void launch_lu_solve<double>(const C2<double>*, int, int, const C2<double>*, C2<double>*, C2<double>*, int,
-                                      bool, int*, int, int*, int, hipStream_t);
+                                      bool, int*, int, unsigned*, unsigned, int*, int, hipStream_t,
+                                      const RefineState*);
 template void launch_lu_solve<float>(const C2<float>*, int, int, const C2<float>*, C2<float>*, C2<float>*, int, bool,
-                                     int*, int, int*, int, hipStream_t);
+                                     int*, int, unsigned*, unsigned, int*, int, hipStream_t, const RefineState*);
 
 void launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t st) {
   mfma_probe_kernel<double><<<1, 64, 0, st>>>(a, b, d);

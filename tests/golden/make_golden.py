@@ -121,6 +121,57 @@ def synthetic_cases(tmp: Path) -> None:
     collect(work, "barge_fd", 1025.0, 9.81, 50.0)
 
 
+def barge4000_case(tmp: Path) -> None:
+    """The full-size finite-depth barge (100 x 20 x 10 m, 4000 panels, no
+    symmetry, h = 50 m, 6 DOF + headings 0/45/90) at three frequencies of its
+    0.2-2.0 rad/s band: 0.2 and 1.1 take the finite-depth path, 2.0 the
+    infinite-depth one (k h >= 20) with the NQ the finite-depth calls left.
+    The reference's wall times are written beside the fixture."""
+    import time
+    barge = nmesh.box(100.0, 20.0, 10.0, 90, 20, 10)
+    assert barge.npanels == 4000
+    work = tmp / "barge4000"
+    work.mkdir()
+    barge.write_dat(work / "barge.dat")
+    body = nio.BodySpec("barge.dat", barge.npoints, barge.npanels, nio.six_dof(), nio.six_dof())
+    nio.write_nemoh_cal(work / "Nemoh.cal", 1025.0, 9.81, 50.0, [body], 3, 0.2, 2.0, 3, 0.0, 90.0)
+    t0 = time.perf_counter()
+    run_case(work, "barge4000")
+    wall = time.perf_counter() - t0
+    collect(work, "barge4000", 1025.0, 9.81, 50.0)
+    (OUT / "case_barge4000.time").write_text(
+        f"reference preProcessor + Solver (flang -O2 FP64), one process, 3 frequencies x 9 problems, "
+        f"4000 panels: {wall:.1f} s wall\n")
+    print(f"barge4000: {wall:.1f} s")
+
+
+def c4_bodies_cases(tmp: Path) -> None:
+    """One body of each shape of the batched workload (cylinder, box, prolate
+    spheroid), drawn by nemoh_amd.serving.random_body_parts from the first
+    seeds that give that shape with 300-600 panels: 6 DOF + heading 0, the
+    8 frequencies 0.2-2.0 rad/s, infinite depth."""
+    from nemoh_amd import serving
+    found = {}
+    seed = 0
+    while len(found) < 3:
+        rng = np.random.default_rng(seed)
+        kind = int(np.random.default_rng(seed).integers(0, 3))
+        m, _, omegas, _ = serving.random_body_parts(rng, n_range=(300, 600))
+        if kind not in found and 300 <= m.npanels <= 600:
+            found[kind] = (seed, m)
+        seed += 1
+    for kind, name in ((0, "c4cyl"), (1, "c4box"), (2, "c4sph")):
+        seed, m = found[kind]
+        work = tmp / name
+        work.mkdir()
+        m.write_dat(work / "body.dat")
+        body = nio.BodySpec("body.dat", m.npoints, m.npanels, nio.six_dof(), nio.six_dof())
+        nio.write_nemoh_cal(work / "Nemoh.cal", 1025.0, 9.81, 0.0, [body], 8, 0.2, 2.0, 1, 0.0, 0.0)
+        run_case(work, name)
+        collect(work, name, 1025.0, 9.81, 0.0)
+        print(f"{name}: seed {seed}, {m.npanels} panels")
+
+
 def harness(cmds: str) -> list:
     r = subprocess.run([str(REF_BIN / "harness")], input=cmds, capture_output=True, text=True)
     if r.returncode != 0:
@@ -487,6 +538,10 @@ def main() -> None:
         if a.only in ("", "cases"):
             verification_cases(Path(a.ref), tmp)
             synthetic_cases(tmp)
+        if a.only in ("", "c4"):
+            c4_bodies_cases(tmp)
+        if a.only in ("", "barge4000"):
+            barge4000_case(tmp)
 
 
 if __name__ == "__main__":

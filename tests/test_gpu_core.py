@@ -19,7 +19,11 @@ pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-10
 MIXED_TOL = 1e-6
-CASES = ["Cylinder", "NonSymmetrical", "2Bodies", "Postprocessing", "cyl512", "barge_fd"]
+# the four Verification cases, the 512-panel cylinder, a small finite-depth
+# barge, and one body of each shape of the batched workload (cylinder, box,
+# prolate spheroid; 398-600 panels, 8 frequencies, infinite depth)
+CASES = ["Cylinder", "NonSymmetrical", "2Bodies", "Postprocessing", "cyl512", "barge_fd", "c4cyl", "c4box",
+         "c4sph"]
 
 
 def _tables():
