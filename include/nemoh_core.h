@@ -133,6 +133,17 @@ int nemoh_core_solve_waves(nemoh_core *ctx, double omega, int nbeta, const doubl
 int nemoh_core_kochin(nemoh_core *ctx, double xeff, double yeff, int ntheta, const double *theta,
                       double *hkochin);
 
+/* No counterpart in the reference: nemoh_core_kochin for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results;
+ * it calls no hipMalloc/hipFree: the angles and the values go through a
+ * fixed 2.1 MB device array of the library, one per GPU, so ntheta is at
+ * most 4096 (NEMOH_ERR_ARG beyond).  Calls on one GPU share that array:
+ * calls that overlap in time on one GPU (two threads, two handles) race and
+ * must be serialised by the caller. */
+int nemoh_core_kochin_noalloc(nemoh_core *ctx, double xeff, double yeff, int ntheta, const double *theta,
+                              double *hkochin);
+
 /* Potential of the sources of every problem of the last solve on this
  * handle at npoints field points xyz[3 p + {0,1,2}] (x, y, z):
  * phi(npoints, nrhs) complex.  Replaces COMPUTE_POTENTIAL_DOMAIN

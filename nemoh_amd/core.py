@@ -22,7 +22,7 @@ EXPORTS = (
     "nemoh_core_last_error", "nemoh_core_destroy", "nemoh_core_prepare_mesh", "nemoh_core_build_tables",
     "nemoh_core_wavenumber", "nemoh_core_theta_rule", "nemoh_core_prony", "nemoh_core_pairs",
     "nemoh_core_mfma_probe", "nemoh_core_get_extent", "nemoh_core_set_dofs", "nemoh_core_body_conditions",
-    "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_core_potential", "nemoh_server_create", "nemoh_server_submit",
+    "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_core_kochin_noalloc", "nemoh_core_potential", "nemoh_server_create", "nemoh_server_submit",
     "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_last_error", "nemoh_server_destroy",
 )
 
@@ -106,6 +106,7 @@ def load():
     L.nemoh_core_body_conditions.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
     L.nemoh_core_solve_waves.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp]
     L.nemoh_core_kochin.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp]
+    L.nemoh_core_kochin_noalloc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp]
     L.nemoh_core_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.nemoh_core_incident_wavenumber.argtypes = [C.c_double, C.c_double, C.c_double]
     L.nemoh_core_incident_wavenumber.restype = C.c_double
@@ -289,6 +290,16 @@ class Core:
         out = np.zeros((th.size, n), np.complex128, order="F")
         self._check(load().nemoh_core_kochin(self.h, float(xeff), float(yeff), th.size, _p(th),
                                              out.ctypes.data_as(_dp)))
+        return out
+
+    def kochin_noalloc(self, thetas, xeff: float = 0.0, yeff: float = 0.0, nrhs: int = None) -> np.ndarray:
+        """As kochin, through the entry point that allocates no device memory
+        (at most 4096 angles)."""
+        th = _f64(np.atleast_1d(thetas))
+        n = nrhs if nrhs is not None else self._last_nrhs
+        out = np.zeros((th.size, n), np.complex128, order="F")
+        self._check(load().nemoh_core_kochin_noalloc(self.h, float(xeff), float(yeff), th.size, _p(th),
+                                                     out.ctypes.data_as(_dp)))
         return out
 
     def potential(self, points, nrhs: int = None) -> np.ndarray:

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -17,6 +18,18 @@
 #include "kernels.h"
 
 using namespace nemoh;
+
+// Device memory of nemoh_core_kochin_noalloc, which allocates nothing: the
+// angles (kNoallocMaxAngles doubles), the Kochin values in the kernel's
+// (angle x problem) order and the same transposed to the caller's order
+// (kNoallocMaxAngles x kMaxRhs complex each): 2.1 MB, held by every GPU the
+// library runs on until the process ends, shared by every handle on that
+// GPU.  Host code reaches it only through the address hipGetSymbolAddress
+// gives for the current device (noalloc_scratch below), never by its name.
+constexpr int kNoallocMaxAngles = 4096;
+constexpr size_t kNoallocScratchBytes =
+    kNoallocMaxAngles * sizeof(double) + 2 * (size_t)kNoallocMaxAngles * kMaxRhs * sizeof(double2);
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
 
 struct nemoh_core {
   int np = 0, nfa = 0, nsym = 0, nb = 1;
@@ -96,11 +109,31 @@ struct nemoh_core {
   nemoh_stats stats{};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<hipEvent_t> upd_ev;  // around each trailing-update launch (timing on)
+  unsigned char* d_scratch = nullptr;  // this device's `scratch` (nemoh_core_kochin_noalloc)
   int upd_blocks = 0;              // symmetry blocks whose updates upd_ev covers
   std::string err;
 };
 
 namespace {
+
+// The device address of `scratch` on device dev, looked up once per device
+// with that device current (each GPU holds its own copy of the array).
+hipError_t noalloc_scratch(int dev, unsigned char** out) {
+  static std::mutex mu;
+  static std::unordered_map<int, unsigned char*> addr;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = addr.find(dev);
+  if (it != addr.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return e != hipSuccess ? e : hipErrorInvalidSymbol;
+  addr[dev] = static_cast<unsigned char*>(p);
+  *out = static_cast<unsigned char*>(p);
+  return hipSuccess;
+}
 
 int fail(nemoh_core* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -740,6 +773,7 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return bail(NEMOH_ERR_HIP);
     c->cu_count = prop.multiProcessorCount;
+    if (noalloc_scratch(dev, &c->d_scratch) != hipSuccess) return bail(NEMOH_ERR_HIP);
   }
   if (hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_rden, rden.data(), rden.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -962,6 +996,29 @@ int nemoh_core_kochin(nemoh_core* c, double xeff, double yeff, int ntheta, const
       hkochin[2 * (t + (size_t)ntheta * q)] = h[2 * ((size_t)t * nrhs + q)];
       hkochin[2 * (t + (size_t)ntheta * q) + 1] = h[2 * ((size_t)t * nrhs + q) + 1];
     }
+  return NEMOH_OK;
+}
+
+int nemoh_core_kochin_noalloc(nemoh_core* c, double xeff, double yeff, int ntheta, const double* theta,
+                              double* hkochin) {
+  if (!c) return NEMOH_ERR_ARG;
+  if (ntheta <= 0 || !theta || !hkochin) return fail(c, NEMOH_ERR_ARG, "bad Kochin arguments");
+  if (ntheta > kNoallocMaxAngles)
+    return fail(c, NEMOH_ERR_ARG,
+                "nemoh_core_kochin_noalloc takes at most " + std::to_string(kNoallocMaxAngles) + " angles");
+  if (c->last_nrhs <= 0) return fail(c, NEMOH_ERR_ARG, "no solve on this mesh yet");
+  if (!c->d_scratch) return fail(c, NEMOH_ERR_HIP, "no device scratch");
+  const int nrhs = c->last_nrhs;
+  double* d_theta = reinterpret_cast<double*>(c->d_scratch);
+  double2* d_h = reinterpret_cast<double2*>(d_theta + kNoallocMaxAngles);
+  double2* d_ht = d_h + (size_t)kNoallocMaxAngles * kMaxRhs;
+  HIPCHK(c, hipMemcpyAsync(d_theta, theta, ntheta * sizeof(double), hipMemcpyHostToDevice, c->st));
+  launch_kochin(c->G, c->sig, c->nsym, nrhs, c->h_par.ak0, c->depth, xeff, yeff, d_theta, ntheta, d_h, c->st);
+  // (angle x problem) -> the caller's column-major (angle, problem)
+  launch_transpose_c128(d_h, ntheta, nrhs, d_ht, c->st);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(hkochin, d_ht, 2 * (size_t)ntheta * nrhs * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
   return NEMOH_OK;
 }
 

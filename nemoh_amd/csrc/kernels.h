@@ -114,6 +114,8 @@ void launch_kochin(const Geometry& G, const double2* sig, int nsym, int nrhs, do
 void launch_potential(const Geometry& G, const Tables& T, const FreqParams* P, int fd, double dhsl, double a,
                       const double* pts, int npts, const double2* sig, int nrhs, double2* phi, int* bad,
                       hipStream_t st);
+// out (cols x rows, row-major) = transpose of in (rows x cols, row-major)
+void launch_transpose_c128(const double2* in, int rows, int cols, double2* out, hipStream_t st);
 // out[ntile*(ntile+1)/2] (ntile = ceil(np/256)): per-workgroup maxima of the
 // squared horizontal vertex distances (GM^2)
 void launch_extent(const double* x, const double* y, int np, int nsym, double* out, hipStream_t st);

@@ -566,6 +566,17 @@ void launch_kochin(const Geometry& G, const double2* sig, int nsym, int nrhs, do
   kochin_kernel<<<ntheta, 256, 0, st>>>(G, sig, theta, KochinArgs{k, depth, xeff, yeff, nsym, nrhs}, H);
 }
 
+// out[q * rows + r] = in[r * cols + q]
+__global__ void transpose_c128_kernel(const double2* __restrict__ in, int rows, int cols, double2* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * cols) return;
+  const int r = e / cols, q = e - r * cols;
+  out[(size_t)q * rows + r] = in[e];
+}
+void launch_transpose_c128(const double2* in, int rows, int cols, double2* out, hipStream_t st) {
+  transpose_c128_kernel<<<(rows * cols + 255) / 256, 256, 0, st>>>(in, rows, cols, out);
+}
+
 void launch_extent(const double* x, const double* y, int np, int nsym, double* out, hipStream_t st) {
   const int ntile = (np + 255) / 256;
   extent_kernel<<<(unsigned)((size_t)ntile * (ntile + 1) / 2), 256, 0, st>>>(x, y, np, nsym, out);
