@@ -46,6 +46,12 @@ def write_synthetic_case(work, name):
         body = nio.BodySpec("cyl.dat", m.npoints, m.npanels, nio.six_dof(), nio.six_dof())
         nio.write_nemoh_cal(work / "Nemoh.cal", 1025.0, 9.81, 0.0, [body], 20, 0.1, 2.0, 1, 0.0, 0.0)
         depth = 0.0
+    elif name == "barge4000":
+        m = nmesh.box(100.0, 20.0, 10.0, 90, 20, 10)
+        m.write_dat(work / "barge.dat")
+        body = nio.BodySpec("barge.dat", m.npoints, m.npanels, nio.six_dof(), nio.six_dof())
+        nio.write_nemoh_cal(work / "Nemoh.cal", 1025.0, 9.81, 50.0, [body], 3, 0.2, 2.0, 3, 0.0, 90.0)
+        depth = 50.0
     else:
         m = nmesh.box(100.0, 20.0, 10.0, 18, 4, 2)
         m.write_dat(work / "barge.dat")

@@ -9,6 +9,9 @@ oracle/Makefile.ref into oracle/_ref/).  It
   0.1-2.0 rad/s, half mesh) and on a small finite-depth barge, and stores
   the Solver's inputs (mesh/L12.dat geometry, Normalvelocities.dat,
   Integration.dat) with its output (results/Forces.dat) as case_*.npz;
+  the same for one body of each shape of the batched workload (c4cyl,
+  c4box, c4sph) and for the full-size 4000-panel barge at three
+  frequencies (barge4000, whose wall time goes to profiles/);
 * runs oracle/ref_harness.f90 (linked against the reference modules) to
   dump GG samples, the CREK tables, CINT rules, X0, LISC fits and every
   pair of VAV*+VNS* on two small meshes, stored as fn_*.npz / pairs_*.npz.
@@ -126,7 +129,7 @@ def barge4000_case(tmp: Path) -> None:
     symmetry, h = 50 m, 6 DOF + headings 0/45/90) at three frequencies of its
     0.2-2.0 rad/s band: 0.2 and 1.1 take the finite-depth path, 2.0 the
     infinite-depth one (k h >= 20) with the NQ the finite-depth calls left.
-    The reference's wall times are written beside the fixture."""
+    The reference's wall time is written to profiles/."""
     import time
     barge = nmesh.box(100.0, 20.0, 10.0, 90, 20, 10)
     assert barge.npanels == 4000
@@ -139,9 +142,11 @@ def barge4000_case(tmp: Path) -> None:
     run_case(work, "barge4000")
     wall = time.perf_counter() - t0
     collect(work, "barge4000", 1025.0, 9.81, 50.0)
-    (OUT / "case_barge4000.time").write_text(
-        f"reference preProcessor + Solver (flang -O2 FP64), one process, 3 frequencies x 9 problems, "
-        f"4000 panels: {wall:.1f} s wall\n")
+    (ROOT / "profiles" / "cpu_reference_barge4000.txt").write_text(
+        f"reference preProcessor + Solver, flang -O2 FP64 build of the unmodified sources (oracle/Makefile.ref), "
+        f"one process on one core: barge 100x20x10 m, 4000 panels, h = 50 m, 3 frequencies (0.2, 1.1, 2.0 rad/s) "
+        f"x 9 problems: {wall:.1f} s wall, {wall / 3:.1f} s per frequency\n"
+        f"command: python tests/golden/make_golden.py --only barge4000\n")
     print(f"barge4000: {wall:.1f} s")
 
 

@@ -38,7 +38,7 @@ def test_binding_module_compiles_and_names_exports(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["cyl512", "barge_fd"])
+@pytest.mark.parametrize("name", ["cyl512", "barge_fd", "barge4000"])
 @pytest.mark.parametrize("precision", ["FP64", "MIXED"])
 def test_reference_solver_through_core(tmp_path, name, precision):
     exe, pre = REF / "solver_core", REF / "preproc"

@@ -20,10 +20,13 @@ pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-10
 MIXED_TOL = 1e-6
 # the four Verification cases, the 512-panel cylinder, a small finite-depth
-# barge, and one body of each shape of the batched workload (cylinder, box,
-# prolate spheroid; 398-600 panels, 8 frequencies, infinite depth)
+# barge, one body of each shape of the batched workload (cylinder, box,
+# prolate spheroid; 398-600 panels, 8 frequencies, infinite depth) and the
+# full-size 4000-panel barge at h = 50 m (w = 0.2 and 1.1 finite depth, 2.0
+# infinite depth with the NQ the finite-depth frequencies left; 9 problems
+# each)
 CASES = ["Cylinder", "NonSymmetrical", "2Bodies", "Postprocessing", "cyl512", "barge_fd", "c4cyl", "c4box",
-         "c4sph"]
+         "c4sph", "barge4000"]
 
 
 def _tables():
