@@ -176,7 +176,15 @@ int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter
  * block step (the path taken when the row blocks outnumber the compute
  * units), 0 = automatic (single flag-chained launch per triangle when the
  * row blocks fit). */
-enum { NEMOH_OPT_STEPWISE_SOLVE = 1 };
+/* NEMOH_OPT_FP32_ASSEMBLY (mixed precision only): 1 = influence matrices
+ * with FP32 far-field Rankine terms, FP32 tabulated wave terms and
+ * exponential images, and complex64 storage of D and S (FP64 kept for the
+ * selectors, table indices, near-panel Rankine integrals, the residual's and
+ * phi's sums and the forces); 0 (default) = FP64 assembly and FP64 D, S.
+ * Its error against FP64 reaches 2e-6..9e-6 on the 4000-panel barge and
+ * 7e-4 in shallow water (h = 20 m), beyond the 1e-6 the mixed path keeps by
+ * default. */
+enum { NEMOH_OPT_STEPWISE_SOLVE = 1, NEMOH_OPT_FP32_ASSEMBLY = 2 };
 int nemoh_core_set_option(nemoh_core *ctx, int option, int value);
 
 /* Timings (ms) of the last solve's phases and refinement statistics. */

@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libnemoh_core.so"
 OK, ERR_ARG, ERR_HIP, ERR_PIVOT, ERR_OMEGA, ERR_FIT, ERR_PANEL, ERR_CONVERGENCE, ERR_WAVENUMBER = range(9)
 FP64, MIXED = 0, 1
 OPT_STEPWISE_SOLVE = 1
+OPT_FP32_ASSEMBLY = 2
 
 EXPORTS = (
     "nemoh_core_create", "nemoh_core_solve", "nemoh_core_solve_multi", "nemoh_core_solve_frequency",

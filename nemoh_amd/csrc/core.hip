@@ -68,6 +68,7 @@ struct nemoh_core {
   double* d_xr = nullptr;
   double* d_xz = nullptr;
   double* d_tab = nullptr;
+  float4* d_tab32 = nullptr;  // the tables in single precision (mixed assembly)
   double* d_rden = nullptr;  // reciprocal Lagrange denominators (XR, XZ stencils)
   Geometry G{};
   Tables T{};
@@ -76,7 +77,13 @@ struct nemoh_core {
   // influence matrices of the current frequency (per symmetry block)
   double2* D[2] = {nullptr, nullptr};
   double2* S[2] = {nullptr, nullptr};
+  // mixed path: complex64 D factored in place (rows of ld32 = nfa rounded up
+  // to a multiple of 4 elements); with NEMOH_OPT_FP32_ASSEMBLY the residual's
+  // D and S are complex64 too (D32r, S32), else FP64 (D, S)
   float2* D32[2] = {nullptr, nullptr};
+  float2* D32r[2] = {nullptr, nullptr};
+  float2* S32[2] = {nullptr, nullptr};
+  int ld32 = 0;
   double2* Pinv[2] = {nullptr, nullptr};  // inverses of the pivot blocks (64x64 each)
   float2* Pinv32[2] = {nullptr, nullptr};
   double w_prev = -1.0;
@@ -104,6 +111,7 @@ struct nemoh_core {
   int epoch = 0;
   int cu_count = 0;
   bool stepwise_solve = false;
+  bool fp32_assembly = false;  // NEMOH_OPT_FP32_ASSEMBLY (mixed path only)
   // statistics
   bool timing = false;
   nemoh_stats stats{};
@@ -161,20 +169,43 @@ int dalloc(nemoh_core* c, T** p, size_t count) {
   return NEMOH_OK;
 }
 
+// The mixed path with NEMOH_OPT_FP32_ASSEMBLY: complex64 D and S.
+bool c64_matrices(const nemoh_core* c) { return c->prec == NEMOH_MIXED && c->fp32_assembly; }
+
 int ensure_matrices(nemoh_core* c) {
-  const size_t n2 = (size_t)c->nfa * c->nfa;
+  const size_t ninv = (size_t)((c->nfa + kNB - 1) / kNB) * kNB * kNB;
   for (int b = 0; b < c->nb; ++b) {
-    if (int r = dalloc(c, &c->D[b], n2)) return r;
-    if (int r = dalloc(c, &c->S[b], n2)) return r;
-    const size_t ninv = (size_t)((c->nfa + kNB - 1) / kNB) * kNB * kNB;
+    if (c64_matrices(c)) {
+      const size_t n2 = (size_t)c->nfa * c->ld32;
+      if (int r = dalloc(c, &c->D32r[b], n2)) return r;
+      if (int r = dalloc(c, &c->S32[b], n2)) return r;
+    } else {
+      const size_t n2 = (size_t)c->nfa * c->nfa;
+      if (int r = dalloc(c, &c->D[b], n2)) return r;
+      if (int r = dalloc(c, &c->S[b], n2)) return r;
+    }
     if (c->prec == NEMOH_MIXED) {
-      if (int r = dalloc(c, &c->D32[b], n2)) return r;
+      if (int r = dalloc(c, &c->D32[b], (size_t)c->nfa * c->ld32)) return r;
       if (int r = dalloc(c, &c->Pinv32[b], ninv)) return r;
     } else {
       if (int r = dalloc(c, &c->Pinv[b], ninv)) return r;
     }
   }
   return NEMOH_OK;
+}
+
+// r = b - D z (FP64 sums) and f = S z with the matrices of the current mode
+void residual(nemoh_core* c, int b, int nrhs, const RefineState* skip) {
+  if (c64_matrices(c))
+    launch_matvec32(c->D32r[b], c->ld32, c->nfa, c->Z[b], nrhs, c->R, -1.0, 1.0, c->st, skip);
+  else
+    launch_matvec(c->D[b], c->nfa, c->nfa, c->Z[b], nrhs, c->R, -1.0, 1.0, c->st, skip);
+}
+void potential_of_block(nemoh_core* c, int b, int nrhs) {
+  if (c64_matrices(c))
+    launch_matvec32(c->S32[b], c->ld32, c->nfa, c->Z[b], nrhs, c->F[b], 1.0, 0.0, c->st);
+  else
+    launch_matvec(c->S[b], c->nfa, c->nfa, c->Z[b], nrhs, c->F[b], 1.0, 0.0, c->st);
 }
 
 // cap = right-hand sides the buffers hold for the current mesh (reset to 0
@@ -265,19 +296,26 @@ int check_info(nemoh_core* c) {
 
 // Assembly + factorisation for frequency w (skipped while w is unchanged).
 int prepare_frequency(nemoh_core* c, double w) {
-  if (w == c->w_prev && c->factored_prec == c->prec) {
+  const int mode = c->prec + (c64_matrices(c) ? 2 : 0);  // what the resident matrices hold
+  if (w == c->w_prev && c->factored_prec == mode) {
     c->stats.cache_hit = 1;
     return NEMOH_OK;
   }
   c->stats.cache_hit = 0;
   if (int r = ensure_matrices(c)) return r;
-  MatOut out{c->nfa, c->D[0], c->S[0], c->D[1], c->S[1], nullptr, nullptr};
-  if (c->prec == NEMOH_MIXED) {
-    out.dp32 = c->D32[0];
-    out.dm32 = c->D32[1];
-  }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->st));
-  launch_assemble(c->G, c->T, c->d_par, c->h_par.fd, c->nsym, out, c->d_surf, c->nsurf, c->st);
+  if (c64_matrices(c)) {
+    const MatOut32 out{c->ld32, c->D32r[0], c->D32[0], c->S32[0], c->D32r[1], c->D32[1], c->S32[1]};
+    launch_assemble_mixed(c->G, c->T, c->d_tab32, c->d_par, c->h_par.fd, c->nsym, out, c->d_surf, c->nsurf, c->st);
+  } else {
+    MatOut out{c->nfa, c->D[0], c->S[0], c->D[1], c->S[1], nullptr, nullptr};
+    if (c->prec == NEMOH_MIXED) {  // the copy of D that the complex64 LU factors in place
+      out.dp32 = c->D32[0];
+      out.dm32 = c->D32[1];
+      out.ld32 = c->ld32;
+    }
+    launch_assemble(c->G, c->T, c->d_par, c->h_par.fd, c->nsym, out, c->d_surf, c->nsurf, c->st);
+  }
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
   HIPCHK(c, hipMemsetAsync(c->d_info, 0, 2 * sizeof(int), c->st));
@@ -295,7 +333,7 @@ int prepare_frequency(nemoh_core* c, double w) {
   for (int b = 0; b < c->nb; ++b) {
     hipEvent_t* ue = c->timing ? c->upd_ev.data() + 2 * (size_t)nblk * b : nullptr;
     if (c->prec == NEMOH_MIXED)
-      launch_lu<float>(reinterpret_cast<C2<float>*>(c->D32[b]), c->nfa, c->nfa,
+      launch_lu<float>(reinterpret_cast<C2<float>*>(c->D32[b]), c->ld32, c->nfa,
                        reinterpret_cast<C2<float>*>(c->Pinv32[b]), c->d_info, c->st, ue);
     else
       launch_lu<double>(reinterpret_cast<C2<double>*>(c->D[b]), c->nfa, c->nfa,
@@ -306,7 +344,7 @@ int prepare_frequency(nemoh_core* c, double w) {
   // the pivot flag is read back with the outputs (check_info), so the host
   // never waits between factorisation and solves
   c->w_prev = w;
-  c->factored_prec = c->prec;
+  c->factored_prec = mode;
   return NEMOH_OK;
 }
 
@@ -315,7 +353,7 @@ int prepare_frequency(nemoh_core* c, double w) {
 // a chained solve's bounded wait timed out).  skip: a refinement state whose
 // `done` turns the launches into no-ops.
 template <typename T>
-void lu_solve(nemoh_core* c, const C2<T>* A, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool stepwise,
+void lu_solve(nemoh_core* c, const C2<T>* A, int ld, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool stepwise,
               const RefineState* skip = nullptr) {
   const int n = c->nfa;
   const int nblk = (n + kNB - 1) / kNB;
@@ -326,7 +364,7 @@ void lu_solve(nemoh_core* c, const C2<T>* A, const C2<T>* P, C2<T>* B, C2<T>* W,
     base = c->chain_calls++ * (unsigned)nblk;
   }
   // about a second of polling before a wait gives up
-  launch_lu_solve<T>(A, n, n, P, B, W, nrhs, chained, c->d_flags, c->epoch,
+  launch_lu_solve<T>(A, ld, n, P, B, W, nrhs, chained, c->d_flags, c->epoch,
                      reinterpret_cast<unsigned*>(c->d_flags + 2 * (size_t)nblk), base, c->d_info + 1, 1 << 22, c->st,
                      skip);
 }
@@ -339,11 +377,12 @@ void mixed_sweep(nemoh_core* c, int b, int nrhs, bool first, bool stepwise) {
   const size_t cnt = (size_t)n * nrhs;
   RefineState* rs = c->d_rs + b;
   launch_f64_to_f32(c->R, c->R32, cnt, rs, c->st);
-  lu_solve(c, reinterpret_cast<const C2<float>*>(c->D32[b]), reinterpret_cast<const C2<float>*>(c->Pinv32[b]),
-           reinterpret_cast<C2<float>*>(c->R32), reinterpret_cast<C2<float>*>(c->W32), nrhs, stepwise, rs);
+  lu_solve(c, reinterpret_cast<const C2<float>*>(c->D32[b]), c->ld32,
+           reinterpret_cast<const C2<float>*>(c->Pinv32[b]), reinterpret_cast<C2<float>*>(c->R32),
+           reinterpret_cast<C2<float>*>(c->W32), nrhs, stepwise, rs);
   launch_f32_to_f64(c->R32, c->Z[b], cnt, rs, first ? 0 : 1, c->st);
   launch_copy(c->B, c->R, cnt, rs, c->st);
-  launch_matvec(c->D[b], n, n, c->Z[b], nrhs, c->R, -1.0, 1.0, c->st, rs);
+  residual(c, b, nrhs, rs);
   launch_refine_residual_max(c->R, cnt, c->d_red, rs, c->tol, 0, c->st);
 }
 
@@ -359,7 +398,7 @@ void solve_block(nemoh_core* c, int b, int nrhs, bool stepwise) {
   const size_t cnt = (size_t)n * nrhs;
   if (c->prec == NEMOH_FP64) {
     launch_copy(c->B, c->Z[b], cnt, nullptr, c->st);
-    lu_solve(c, reinterpret_cast<const C2<double>*>(c->D[b]), reinterpret_cast<const C2<double>*>(c->Pinv[b]),
+    lu_solve(c, reinterpret_cast<const C2<double>*>(c->D[b]), n, reinterpret_cast<const C2<double>*>(c->Pinv[b]),
              reinterpret_cast<C2<double>*>(c->Z[b]), reinterpret_cast<C2<double>*>(c->W), nrhs, stepwise);
   } else {
     RefineState* rs = c->d_rs + b;
@@ -368,7 +407,7 @@ void solve_block(nemoh_core* c, int b, int nrhs, bool stepwise) {
     launch_refine_residual_max(c->B, cnt, c->d_red, rs, c->tol, 1, c->st);
     for (int k = 0; k < kQueuedSweeps && k < c->max_iter; ++k) mixed_sweep(c, b, nrhs, k == 0, stepwise);
   }
-  launch_matvec(c->S[b], n, n, c->Z[b], nrhs, c->F[b], 1.0, 0.0, c->st);
+  potential_of_block(c, b, nrhs);
 }
 
 // The pivot report first, then a non-finite or unconverged refinement.
@@ -437,9 +476,9 @@ int run_pipeline(nemoh_core* c, double w, int nrhs) {
       // (the buffers were reused by the blocks after it), then phi
       launch_rhs(c->V, n, c->nsym, nrhs, more == 0 ? 1.0 : -1.0, c->B, c->st);
       launch_copy(c->B, c->R, cnt, nullptr, c->st);
-      launch_matvec(c->D[more], n, n, c->Z[more], nrhs, c->R, -1.0, 1.0, c->st);
+      residual(c, more, nrhs, nullptr);
       mixed_sweep(c, more, nrhs, false, stepwise);
-      launch_matvec(c->S[more], n, n, c->Z[more], nrhs, c->F[more], 1.0, 0.0, c->st);
+      potential_of_block(c, more, nrhs);
       HIPCHK(c, hipGetLastError());
     }
     if (info[1] && !info[0] && attempt == 0 && !stepwise) {
@@ -538,7 +577,7 @@ static int finish_call(nemoh_core* c) {
       c->stats.ms_lu_update = ms;
       c->stats.lu_update_launches = cnt;
       c->stats.lu_update_flops = flops;
-      c->stats.lu_precision = c->factored_prec;
+      c->stats.lu_precision = c->factored_prec & 1;
     }
     float s = 0, t = 0;
     HIPCHK(c, hipEventElapsedTime(&s, c->ev[2], c->ev[3]));
@@ -666,6 +705,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   c->gm = std::sqrt(gm2);
   c->np = np;
   c->nfa = nfa;
+  c->ld32 = (nfa + 3) / 4 * 4;
   c->nsym = nsym;
   c->nb = 1 << nsym;
   c->zer = zer;
@@ -759,7 +799,9 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
   };
   stencils(xr, kNR, rden.data());
   stencils(xz, kNZ, rden.data() + 3 * (kNR - 2));
+  std::vector<float> tab32(tab.begin(), tab.end());
   if (hipMalloc((void**)&c->d_tab, tab.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->d_tab32, tab32.size() * sizeof(float)) != hipSuccess ||
       hipMalloc((void**)&c->d_rden, rden.size() * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_xr, kNR * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_xz, kNZ * sizeof(double)) != hipSuccess ||
@@ -776,6 +818,7 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
     if (noalloc_scratch(dev, &c->d_scratch) != hipSuccess) return bail(NEMOH_ERR_HIP);
   }
   if (hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_tab32, tab32.data(), tab32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_rden, rden.data(), rden.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_xr, xr, kNR * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_xz, xz, kNZ * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -800,6 +843,7 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->d_xr);
   release(c->d_xz);
   release(c->d_tab);
+  release(c->d_tab32);
   release(c->d_rden);
   release(c->d_par);
   release(c->d_red);
@@ -825,6 +869,8 @@ void nemoh_core_destroy(nemoh_core* c) {
     release(c->D[b]);
     release(c->S[b]);
     release(c->D32[b]);
+    release(c->D32r[b]);
+    release(c->S32[b]);
     release(c->Pinv[b]);
     release(c->Pinv32[b]);
     release(c->Z[b]);
@@ -860,6 +906,10 @@ int nemoh_core_set_option(nemoh_core* c, int option, int value) {
   if (!c) return NEMOH_ERR_ARG;
   if (option == NEMOH_OPT_STEPWISE_SOLVE) {
     c->stepwise_solve = value != 0;
+    return NEMOH_OK;
+  }
+  if (option == NEMOH_OPT_FP32_ASSEMBLY) {
+    c->fp32_assembly = value != 0;
     return NEMOH_OK;
   }
   return fail(c, NEMOH_ERR_ARG, "unknown option");

@@ -175,24 +175,15 @@ struct RankineImage {
   double a, vx, vy, vz;
 };
 
-__device__ __forceinline__ RankineImage rankine_image(const Geometry& G, int j, double xo, double yo,
-                                                      double zo, double my, double mz, double dh,
-                                                      double mh, double mjj, bool self) {
-  const double xoj = G.xg[j], yoj = G.yg[j] * my, zoj = G.zg[j] * mz - dh * mh;
+// Near field of one image (RO <= 7 TDIS): the exact flat-quadrilateral
+// integral over the four edges.  dx, dy, dz: field point minus the image's
+// collocation point.
+__device__ __forceinline__ RankineImage rankine_near(const Geometry& G, int j, double xo, double yo, double zo,
+                                                     double my, double mz, double dh, double mh, double mjj,
+                                                     bool self, double dx, double dy, double dz) {
   const double a3j = G.xn[j], a6j = G.yn[j] * my, a9j = G.zn[j] * mz;
-  const double dx = xo - xoj, dy = yo - yoj, dz = zo - zoj;
-  const double ro2 = dx * dx + dy * dy + dz * dz;
   const double tdis = G.tdis[j];
   RankineImage r;
-  if (ro2 > 49.0 * tdis * tdis) {  // RO > 7 TDIS
-    const double ri = rsqrt_pos(ro2);
-    r.a = G.aire[j] * ri;
-    const double asro = r.a * ri * ri;
-    r.vx = -dx * asro;
-    r.vy = -dy * asro;
-    r.vz = -dz * asro;
-    return r;
-  }
   double tx[5], ty[5], tz[5], rr[5], drx[5], dry[5], drz[5];
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
@@ -268,6 +259,26 @@ __device__ __forceinline__ RankineImage rankine_image(const Geometry& G, int j, 
     r.vz = vzs * mjj;
   }
   return r;
+}
+
+__device__ __forceinline__ RankineImage rankine_image(const Geometry& G, int j, double xo, double yo,
+                                                      double zo, double my, double mz, double dh,
+                                                      double mh, double mjj, bool self) {
+  const double xoj = G.xg[j], yoj = G.yg[j] * my, zoj = G.zg[j] * mz - dh * mh;
+  const double dx = xo - xoj, dy = yo - yoj, dz = zo - zoj;
+  const double ro2 = dx * dx + dy * dy + dz * dz;
+  const double tdis = G.tdis[j];
+  if (ro2 > 49.0 * tdis * tdis) {  // RO > 7 TDIS
+    RankineImage r;
+    const double ri = rsqrt_pos(ro2);
+    r.a = G.aire[j] * ri;
+    const double asro = r.a * ri * ri;
+    r.vx = -dx * asro;
+    r.vy = -dy * asro;
+    r.vz = -dz * asro;
+    return r;
+  }
+  return rankine_near(G, j, xo, yo, zo, my, mz, dh, mh, mjj, self, dx, dy, dz);
 }
 
 // Sum of the Rankine images combined into the +/- symmetry parts

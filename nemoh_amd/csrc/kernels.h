@@ -23,6 +23,7 @@ struct MatOut {
   double2* sm;
   float2* dp32;
   float2* dm32;
+  int ld32;  // leading dimension of dp32 / dm32
 };
 
 // Iterative refinement of one symmetry block in device memory: bmax = max|b|,
@@ -34,7 +35,26 @@ struct RefineState {
   int done, iters, bad, pad;
 };
 
+// Influence matrices of the mixed path (complex64, row-major, leading
+// dimension ld): D for the FP64 residual, a copy of D that the LU factors in
+// place, and S; m: the antisymmetric block (nsym = 1 only).
+struct MatOut32 {
+  int ld;
+  float2* dp;
+  float2* dp_fac;
+  float2* sp;
+  float2* dm;
+  float2* dm_fac;
+  float2* sm;
+};
+
 void launch_crek(const double* xr, const double* xz, double* tab, hipStream_t st);
+// Mixed-precision assembly (green_mixed.h): FP64 selectors, indices and
+// near-panel Rankine integrals, FP32 far-field, wave and exponential-image
+// terms on the FP32 tables tab32; pairs with a point at the free surface in
+// FP64, stored in single precision.
+void launch_assemble_mixed(const Geometry& G, const Tables& T, const float4* tab32, const FreqParams* P, int fd,
+                           int nsym, const MatOut32& out, const int* surf, int nsurf, hipStream_t st);
 // fd: the regime of *P (finite depth 1, infinite 0), known to the host.
 // surf[0..nsurf) (device): the panels with zg >= ZER, whose rows and columns
 // take the free-surface branch of the wave part.
@@ -67,6 +87,9 @@ void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2
 // skipped when skip (optional) is done.
 void launch_matvec(const double2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
                    double beta, hipStream_t st, const RefineState* skip = nullptr);
+// The same with M complex64 (ld a multiple of 4), X, Y and the sums in FP64.
+void launch_matvec32(const float2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
+                     double beta, hipStream_t st, const RefineState* skip = nullptr);
 void launch_rhs(const double2* V, int n, int nsym, int nrhs, double bx, double2* B, hipStream_t st);
 void launch_combine(const double2* Z1, const double2* Z2, const double2* F1, const double2* F2, int n, int nsym,
                     int nrhs, double rho_w, double2* sig, double2* pot, double2* pres, hipStream_t st);

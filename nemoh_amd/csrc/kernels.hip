@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "green.h"
+#include "green_mixed.h"
 #include "kernels.h"
 #include "linalg.h"
 
@@ -69,18 +70,18 @@ __global__ void crek_kernel(const double* __restrict__ xr, const double* __restr
 template <bool kSym, bool kF32>
 __device__ __forceinline__ void store_pair(const Geometry& G, const MatOut& out, int i, int j, const double o[16]) {
   const double xn = G.xn[i], yn = G.yn[i], zn = G.zn[i];
-  const size_t e = (size_t)i * out.ld + j;
+  const size_t e = (size_t)i * out.ld + j, e32 = (size_t)i * out.ld32 + j;
   const double dpr = o[4] * xn + o[5] * yn + o[6] * zn;
   const double dpi = o[10] * xn + o[11] * yn + o[12] * zn;
   out.dp[e] = make_double2(dpr, dpi);
   out.sp[e] = make_double2(o[0], o[2]);
-  if (kF32) out.dp32[e] = make_float2((float)dpr, (float)dpi);
+  if (kF32) out.dp32[e32] = make_float2((float)dpr, (float)dpi);
   if (kSym) {
     const double dmr = o[7] * xn + o[8] * yn + o[9] * zn;
     const double dmi = o[13] * xn + o[14] * yn + o[15] * zn;
     out.dm[e] = make_double2(dmr, dmi);
     out.sm[e] = make_double2(o[1], o[3]);
-    if (kF32) out.dm32[e] = make_float2((float)dmr, (float)dmi);
+    if (kF32) out.dm32[e32] = make_float2((float)dmr, (float)dmi);
   }
 }
 
@@ -118,6 +119,63 @@ __global__ __launch_bounds__(256) void assemble_surface_kernel(Geometry G, Table
   double o[16];
   pair16<kFd, true>(G, T, P, i, j, o);
   store_pair<kSym, kF32>(G, out, i, j, o);
+}
+
+// Mixed path: D = n_i . grad G and S = G of the pair in single precision,
+// D twice (the residual's copy and the one the LU factors in place).
+template <bool kSym, typename R>
+__device__ __forceinline__ void store_pair_m(const Geometry& G, const MatOut32& out, int i, int j, const R o[16]) {
+  const float xn = (float)G.xn[i], yn = (float)G.yn[i], zn = (float)G.zn[i];
+  const size_t e = (size_t)i * out.ld + j;
+  const float2 dp = make_float2((float)o[4] * xn + (float)o[5] * yn + (float)o[6] * zn,
+                                (float)o[10] * xn + (float)o[11] * yn + (float)o[12] * zn);
+  out.dp[e] = dp;
+  out.dp_fac[e] = dp;
+  out.sp[e] = make_float2((float)o[0], (float)o[2]);
+  if (kSym) {
+    const float2 dm = make_float2((float)o[7] * xn + (float)o[8] * yn + (float)o[9] * zn,
+                                  (float)o[13] * xn + (float)o[14] * yn + (float)o[15] * zn);
+    out.dm[e] = dm;
+    out.dm_fac[e] = dm;
+    out.sm[e] = make_float2((float)o[1], (float)o[3]);
+  }
+}
+
+// As assemble_kernel, in mixed precision (green_mixed.h); pairs with a point
+// at the free surface are left to assemble_surface_mixed_kernel.
+// Three waves per SIMD: 150 registers, no spills (four spill 20 registers of
+// the inlined near-panel integral and ran within 5 % of three).
+constexpr int kMixedWaves = 3;
+template <bool kFd, bool kSym>
+__global__ __launch_bounds__(256, kMixedWaves) void assemble_mixed_kernel(Geometry G, Tables T, Tables32 T32,
+                                                                const FreqParams* __restrict__ Pp, MatOut32 out) {
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= G.n) return;
+  float o[16];
+  pair16_m<kFd>(G, T, T32, *Pp, i, j, o);
+  store_pair_m<kSym>(G, out, i, j, o);
+}
+
+// The free-surface pairs of the mixed path, in FP64 (as
+// assemble_surface_kernel), stored in single precision.
+template <bool kFd, bool kSym>
+__global__ __launch_bounds__(256) void assemble_surface_mixed_kernel(Geometry G, Tables T,
+                                                                     const FreqParams* __restrict__ Pp, MatOut32 out,
+                                                                     const int* __restrict__ surf) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= G.n) return;
+  const FreqParams& P = *Pp;
+  const int s = surf[blockIdx.y];
+  int i = s, j = k;
+  if (blockIdx.z == 1) {
+    i = k;
+    j = s;
+    if (!(G.zg[i] < P.zer)) return;  // a listed row: done by blockIdx.z = 0
+  }
+  double o[16];
+  pair16<kFd, true>(G, T, P, i, j, o);
+  store_pair_m<kSym>(G, out, i, j, o);
 }
 
 // Raw 16-value dump of every pair (tests only; small meshes).
@@ -190,6 +248,81 @@ __global__ __launch_bounds__(64 * kGemvWaves) void gemv_kernel(const double2* __
         double vr = red[0][0][r][li], vi = red[0][1][r][li];
 #pragma unroll
         for (int w = 1; w < kGemvWaves; ++w) {
+          vr += red[w][0][r][li];
+          vi += red[w][1][r][li];
+        }
+        double2* y = Y + (size_t)(r0 + r) * nrhs + li;
+        const double2 old = (beta == 0.0) ? make_double2(0.0, 0.0) : *y;
+        *y = make_double2(beta * old.x + alpha * vr, beta * old.y + alpha * vi);
+      }
+    }
+  }
+}
+
+// Y = beta Y + alpha M X with M complex64 (the mixed path's D and S) and X,
+// Y and the accumulation in FP64.  As gemv_kernel, one workgroup per 16 rows
+// and 16 waves splitting the columns, but each lane streams 8 consecutive
+// complex64 of its row (64 bytes, 256 bytes per row and wave) per 32-column
+// batch: the MFMA k-step t of a batch takes columns 8 lk + t of the four
+// lane groups lk, and the x operand is loaded in the same permuted order, so
+// the product is unchanged.  ld: a multiple of 4 (16-byte aligned rows).
+constexpr int kGemv32Waves = 16;
+__global__ __launch_bounds__(64 * kGemv32Waves) void gemv32_kernel(const float2* __restrict__ M, int ld, int n,
+                                                                   const double2* __restrict__ X, int nrhs,
+                                                                   double2* __restrict__ Y, double alpha,
+                                                                   double beta, const RefineState* __restrict__ rs) {
+  if (rs && rs->done) return;
+  __shared__ double red[kGemv32Waves][2][16][17];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  const int r0 = blockIdx.x * 16;
+  const int row = r0 + li;
+  const int nb = (n + 31) / 32;  // 32-column batches
+  const int per = (nb + kGemv32Waves - 1) / kGemv32Waves;
+  const int b0 = wave * per, b1 = min(nb, b0 + per);
+  const float2* mrow = M + (size_t)min(row, n - 1) * ld;
+  v4d cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  for (int bb = b0; bb < b1; ++bb) {
+    const int c0 = bb * 32 + 8 * lk;
+    float2 a[8];
+    if (row < n && c0 + 8 <= n) {
+      const float4* q = reinterpret_cast<const float4*>(mrow + c0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float4 v = q[u];
+        a[2 * u] = make_float2(v.x, v.y);
+        a[2 * u + 1] = make_float2(v.z, v.w);
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) a[t] = (row < n && c0 + t < n) ? mrow[c0 + t] : make_float2(0.f, 0.f);
+    }
+    double2 x[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      x[t] = (c0 + t < n && li < nrhs) ? X[(size_t)(c0 + t) * nrhs + li] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const double ar = a[t].x, ai = a[t].y;
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, x[t].x, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, x[t].y, ci, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, x[t].y, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, x[t].x, ci, 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    red[wave][0][lk + 4 * q][li] = cr[q];
+    red[wave][1][lk + 4 * q][li] = ci[q];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = lk + 4 * q;
+      if (r0 + r < n && li < nrhs) {
+        double vr = red[0][0][r][li], vi = red[0][1][r][li];
+#pragma unroll
+        for (int w = 1; w < kGemv32Waves; ++w) {
           vr += red[w][0][r][li];
           vi += red[w][1][r][li];
         }
@@ -519,6 +652,26 @@ void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, in
   else launch_assemble_regime<false>(G, T, P, nsym, out, surf, nsurf, st);
 }
 
+template <bool kFd, bool kSym>
+static void launch_assemble_mixed_variant(const Geometry& G, const Tables& T, const float4* tab32,
+                                          const FreqParams* P, const MatOut32& out, const int* surf, int nsurf,
+                                          hipStream_t st) {
+  const int nx = (G.n + 255) / 256;
+  assemble_mixed_kernel<kFd, kSym><<<dim3(nx, G.n), 256, 0, st>>>(G, T, Tables32{tab32}, P, out);
+  if (nsurf > 0) assemble_surface_mixed_kernel<kFd, kSym><<<dim3(nx, nsurf, 2), 256, 0, st>>>(G, T, P, out, surf);
+}
+
+void launch_assemble_mixed(const Geometry& G, const Tables& T, const float4* tab32, const FreqParams* P, int fd,
+                           int nsym, const MatOut32& out, const int* surf, int nsurf, hipStream_t st) {
+  if (fd) {
+    if (nsym) launch_assemble_mixed_variant<true, true>(G, T, tab32, P, out, surf, nsurf, st);
+    else launch_assemble_mixed_variant<true, false>(G, T, tab32, P, out, surf, nsurf, st);
+  } else {
+    if (nsym) launch_assemble_mixed_variant<false, true>(G, T, tab32, P, out, surf, nsurf, st);
+    else launch_assemble_mixed_variant<false, false>(G, T, tab32, P, out, surf, nsurf, st);
+  }
+}
+
 void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int fd, double* out, hipStream_t st) {
   dim3 grid((G.n + 63) / 64, G.n);
   if (fd) pairs_kernel<true><<<grid, 64, 0, st>>>(G, T, P, out);
@@ -528,6 +681,11 @@ void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int f
 void launch_matvec(const double2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
                    double beta, hipStream_t st, const RefineState* skip) {
   gemv_kernel<<<(n + 15) / 16, 64 * kGemvWaves, 0, st>>>(M, ld, n, X, nrhs, Y, alpha, beta, skip);
+}
+
+void launch_matvec32(const float2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
+                     double beta, hipStream_t st, const RefineState* skip) {
+  gemv32_kernel<<<(n + 15) / 16, 64 * kGemv32Waves, 0, st>>>(M, ld, n, X, nrhs, Y, alpha, beta, skip);
 }
 
 void launch_rhs(const double2* V, int n, int nsym, int nrhs, double bx, double2* B, hipStream_t st) {

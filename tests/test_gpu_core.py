@@ -249,3 +249,21 @@ def test_semisub_12k_infinite_depth_mixed_vs_fp64():
     rad = kind == -1.0
     damping = -out[core.FP64][:, rad].real
     assert np.all(np.diag(damping) >= -1e-6 * np.abs(damping).max())
+
+
+# NEMOH_OPT_FP32_ASSEMBLY (off by default): FP32 far-field Rankine and wave
+# terms, complex64 D and S.  Its measured error against the reference Solver,
+# normwise per problem: up to 6e-7 on these deep-water cases, 2.4e-6 on the
+# 4000-panel barge at h = 50 m (7e-4 on 2Bodies at h = 20 m, which is why it
+# is not the default); asserted with a factor of 4 of margin.
+@pytest.mark.parametrize("name,tol", [("Cylinder", 2.5e-6), ("NonSymmetrical", 2.5e-6), ("c4box", 2.5e-6),
+                                      ("c4sph", 2.5e-6), ("barge4000", 1e-5)])
+def test_fp32_assembly_option(name, tol):
+    d, m, ref = _case(name)
+    s = Solver(m, float(d["depth"]), float(d["rho"]), float(d["g"]), precision=core.MIXED, tables=_tables())
+    s.core.set_option(core.OPT_FP32_ASSEMBLY, 1)
+    f = s.solve_problems(d["omega"], d["nvel"], d["nds"])
+    s.close()
+    err = _normwise(f, ref)
+    print(f"{name}: FP32 assembly, mixed: max normwise rel err {err:.3e}")
+    assert err < tol
