@@ -184,7 +184,12 @@ int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter
  * Its error against FP64 reaches 2e-6..9e-6 on the 4000-panel barge and
  * 7e-4 in shallow water (h = 20 m), beyond the 1e-6 the mixed path keeps by
  * default. */
-enum { NEMOH_OPT_STEPWISE_SOLVE = 1, NEMOH_OPT_FP32_ASSEMBLY = 2 };
+/* NEMOH_OPT_SOLVE_SPIN_LIMIT: polls a chained solve's workgroup makes
+ * while waiting for an earlier row block before it gives up (default 4194304,
+ * about a second); when any gives up, the solves of that call are redone
+ * with one launch per block step (nemoh_stats.solve_fallbacks counts it).
+ * 0 makes every wait give up at once, which forces that path (tests). */
+enum { NEMOH_OPT_STEPWISE_SOLVE = 1, NEMOH_OPT_FP32_ASSEMBLY = 2, NEMOH_OPT_SOLVE_SPIN_LIMIT = 3 };
 int nemoh_core_set_option(nemoh_core *ctx, int option, int value);
 
 /* Timings (ms) of the last solve's phases and refinement statistics. */
@@ -200,6 +205,7 @@ typedef struct {
   int lu_update_launches;
   double lu_update_flops;
   int lu_precision; /* NEMOH_FP64 or NEMOH_MIXED (complex64 factors) */
+  int solve_fallbacks; /* chained solves redone stepwise after a bounded wait gave up (this handle) */
 } nemoh_stats;
 int nemoh_core_get_stats(const nemoh_core *ctx, nemoh_stats *out);
 int nemoh_core_enable_timing(nemoh_core *ctx, int on);
@@ -248,9 +254,14 @@ int nemoh_core_get_extent(const nemoh_core *ctx, double *gm);
  * forces(nint, nrhs, nfreq) = -NDS . pressure at each frequency, as
  * nemoh_core_solve_frequency of a session that solves the request's
  * frequencies in order.  Work is admitted one (request, frequency) at a
- * time, first come first served, while the in-flight frequencies need at
- * most budget_bytes (N^2 complex64 = 8 N^2 bytes each, N = NFA*2**nsym; with
- * nothing in flight the next one is always admitted).  Each of the
+ * time, first come first served, while the device memory of the workers'
+ * handles stays within budget_bytes: a busy worker counts the larger of what
+ * its handle holds and what its frequency needs (FP64 influence matrices,
+ * the complex64 factor copy in mixed precision, factors, right-hand sides,
+ * geometry: about 32 N^2 bytes per symmetry block in FP64 and 40 N^2 in
+ * mixed, N = NFA), an idle worker what its handle still holds; idle
+ * workers' buffers are freed when they alone stand in the way, and with
+ * nothing running the next frequency is always admitted.  Each of the
  * `workers` threads owns a solver handle (own HIP stream, resident matrices
  * and factors), so admitted work overlaps on the device; a result does not
  * depend on what else was in flight.  xr..apd2z are the wave tables as for
@@ -284,6 +295,10 @@ int nemoh_server_submit_waves(nemoh_server *srv, int np, int nfa, int nsym, cons
                               double yeff, long long *id);
 /* fk is written for wave requests only; forces, fk and timing_ms may be NULL. */
 int nemoh_server_wait(nemoh_server *srv, long long id, double *forces, double *fk, double *timing_ms);
+/* Device bytes the workers' handles hold now, and the largest total
+ * charged against the budget while other frequencies were running (both
+ * may be NULL). */
+int nemoh_server_memory(nemoh_server *srv, double *held, double *peak);
 const char *nemoh_server_last_error(nemoh_server *srv);
 void nemoh_server_destroy(nemoh_server *srv);
 

@@ -16,6 +16,7 @@ OK, ERR_ARG, ERR_HIP, ERR_PIVOT, ERR_OMEGA, ERR_FIT, ERR_PANEL, ERR_CONVERGENCE,
 FP64, MIXED = 0, 1
 OPT_STEPWISE_SOLVE = 1
 OPT_FP32_ASSEMBLY = 2
+OPT_SOLVE_SPIN_LIMIT = 3
 
 EXPORTS = (
     "nemoh_core_create", "nemoh_core_solve", "nemoh_core_solve_multi", "nemoh_core_solve_frequency",
@@ -24,7 +25,8 @@ EXPORTS = (
     "nemoh_core_wavenumber", "nemoh_core_theta_rule", "nemoh_core_prony", "nemoh_core_pairs",
     "nemoh_core_mfma_probe", "nemoh_core_get_extent", "nemoh_core_set_dofs", "nemoh_core_body_conditions",
     "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_core_kochin_noalloc", "nemoh_core_potential", "nemoh_server_create", "nemoh_server_submit",
-    "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_last_error", "nemoh_server_destroy",
+    "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_memory", "nemoh_server_last_error",
+    "nemoh_server_destroy",
 )
 
 
@@ -61,7 +63,8 @@ class Stats(C.Structure):
     _fields_ = [("ms_assembly", C.c_double), ("ms_factor", C.c_double), ("ms_solve", C.c_double),
                 ("ms_total", C.c_double), ("refine_iterations", C.c_int), ("refine_residual", C.c_double),
                 ("cache_hit", C.c_int), ("ms_lu_update", C.c_double), ("lu_update_launches", C.c_int),
-                ("lu_update_flops", C.c_double), ("lu_precision", C.c_int)]
+                ("lu_update_flops", C.c_double), ("lu_precision", C.c_int),
+                ("solve_fallbacks", C.c_int)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -111,6 +114,7 @@ def load():
     L.nemoh_core_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.nemoh_core_incident_wavenumber.argtypes = [C.c_double, C.c_double, C.c_double]
     L.nemoh_core_incident_wavenumber.restype = C.c_double
+    L.nemoh_server_memory.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.nemoh_server_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_double] + [_dp] * 6
     L.nemoh_server_submit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [_dp] * 3 + [_ip] * 4 + \
         [C.c_double] * 3 + [C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.POINTER(C.c_longlong)]

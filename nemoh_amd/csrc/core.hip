@@ -40,6 +40,7 @@ struct nemoh_core {
   // handle field that holds it: buffers only grow, so re-binding the handle
   // to a mesh no larger than before allocates (and frees) nothing
   std::unordered_map<const void*, size_t> caps;
+  std::unordered_map<const void*, size_t> cap_bytes;  // the same in bytes
   // preProcessor side (nemoh_core_set_dofs): bodies, degrees of freedom,
   // its own panel geometry, force weights, incident-wave buffers
   bool dofs_set = false;
@@ -112,6 +113,7 @@ struct nemoh_core {
   int cu_count = 0;
   bool stepwise_solve = false;
   bool fp32_assembly = false;  // NEMOH_OPT_FP32_ASSEMBLY (mixed path only)
+  int spin_limit = 1 << 22;    // polls before a chained solve's wait gives up (about a second)
   // statistics
   bool timing = false;
   nemoh_stats stats{};
@@ -160,13 +162,23 @@ int fail(nemoh_core* c, int code, const std::string& msg) {
 template <typename T>
 int dalloc(nemoh_core* c, T** p, size_t count) {
   size_t& have = c->caps[(const void*)p];
+  size_t& bytes = c->cap_bytes[(const void*)p];
   if (*p && have >= count) return NEMOH_OK;
   if (*p) (void)hipFree(*p);
   *p = nullptr;
-  have = 0;
+  have = bytes = 0;
   HIPCHK(c, hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
   have = count;
+  bytes = std::max<size_t>(count, 1) * sizeof(T);
   return NEMOH_OK;
+}
+
+template <typename T>
+void dfree(nemoh_core* c, T** p) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  c->caps[(const void*)p] = 0;
+  c->cap_bytes[(const void*)p] = 0;
 }
 
 // The mixed path with NEMOH_OPT_FP32_ASSEMBLY: complex64 D and S.
@@ -363,10 +375,9 @@ void lu_solve(nemoh_core* c, const C2<T>* A, int ld, const C2<T>* P, C2<T>* B, C
     if (++c->epoch == 0) ++c->epoch;  // flags start at 0: an epoch is never 0
     base = c->chain_calls++ * (unsigned)nblk;
   }
-  // about a second of polling before a wait gives up
   launch_lu_solve<T>(A, ld, n, P, B, W, nrhs, chained, c->d_flags, c->epoch,
-                     reinterpret_cast<unsigned*>(c->d_flags + 2 * (size_t)nblk), base, c->d_info + 1, 1 << 22, c->st,
-                     skip);
+                     reinterpret_cast<unsigned*>(c->d_flags + 2 * (size_t)nblk), base, c->d_info + 1, c->spin_limit,
+                     c->st, skip);
 }
 
 // Mixed path, one refinement sweep of symmetry block b, every kernel a no-op
@@ -485,6 +496,7 @@ int run_pipeline(nemoh_core* c, double w, int nrhs) {
       const int zero = 0;
       HIPCHK(c, hipMemcpyAsync(c->d_info + 1, &zero, sizeof zero, hipMemcpyHostToDevice, c->st));
       stepwise = true;
+      ++c->stats.solve_fallbacks;
       continue;
     }
     if (info[1]) {
@@ -747,6 +759,49 @@ int core_nq_sequence(nemoh_core* c, int nfreq, const double* periods, int* nq_be
 
 void core_set_nq_state(nemoh_core* c, int nq) { c->nq_state = nq; }
 
+size_t core_held_bytes(const nemoh_core* c) {
+  size_t b = 0;
+  for (const auto& kv : c->cap_bytes) b += kv.second;
+  return b;
+}
+
+size_t core_footprint_bytes(int np, int nfa, int nsym, int precision, int nrhs) {
+  const size_t n = nfa, nb = (size_t)1 << nsym, nt = n * nb, k = nrhs;
+  const size_t nblk = (n + kNB - 1) / kNB, ninv = nblk * kNB * kNB, ld32 = (n + 3) / 4 * 4;
+  size_t b = nb * (2 * n * n * sizeof(double2));  // D, S
+  if (precision == NEMOH_MIXED) b += nb * (n * ld32 * sizeof(float2) + ninv * sizeof(float2));
+  else b += nb * ninv * sizeof(double2);
+  b += nt * k * sizeof(double2) + 3 * n * k * sizeof(double2) + 3 * 2 * n * k * sizeof(double2) +
+       2 * n * k * sizeof(float2) + nb * 2 * n * k * sizeof(double2);  // right-hand-side workspace
+  b += 20 * n * sizeof(double) + (2 * nblk + 2) * sizeof(int) + 2 * (size_t)np * sizeof(double);  // geometry
+  b += 7 * n * sizeof(double) + nt * 16 * sizeof(double) + nt * kMaxRhs * sizeof(double2);  // body conditions
+  return b;
+}
+
+int core_release_matrices(nemoh_core* c) {
+  if (!c) return NEMOH_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (int b = 0; b < 2; ++b) {
+    dfree(c, &c->D[b]);
+    dfree(c, &c->S[b]);
+    dfree(c, &c->D32[b]);
+    dfree(c, &c->D32r[b]);
+    dfree(c, &c->S32[b]);
+    dfree(c, &c->Pinv[b]);
+    dfree(c, &c->Pinv32[b]);
+    dfree(c, &c->Z[b]);
+    dfree(c, &c->F[b]);
+  }
+  for (double2** q : {&c->V, &c->B, &c->W, &c->R, &c->sig, &c->pot, &c->pres}) dfree(c, q);
+  dfree(c, &c->R32);
+  dfree(c, &c->W32);
+  c->cap = 0;
+  c->w_prev = -1.0;  // the factors are gone
+  c->factored_prec = -1;
+  c->last_nrhs = 0;  // and the sources
+  return NEMOH_OK;
+}
+
 int core_device_count() {
   int n = 0;
   return hipGetDeviceCount(&n) == hipSuccess ? n : 0;
@@ -910,6 +965,11 @@ int nemoh_core_set_option(nemoh_core* c, int option, int value) {
   }
   if (option == NEMOH_OPT_FP32_ASSEMBLY) {
     c->fp32_assembly = value != 0;
+    return NEMOH_OK;
+  }
+  if (option == NEMOH_OPT_SOLVE_SPIN_LIMIT) {
+    if (value < 0) return fail(c, NEMOH_ERR_ARG, "spin limit must be >= 0");
+    c->spin_limit = value;
     return NEMOH_OK;
   }
   return fail(c, NEMOH_ERR_ARG, "unknown option");

@@ -1,6 +1,7 @@
 // Library-internal entry points of the solver core, used by the request
 // loop (server.cpp); not part of the C ABI.
 #pragma once
+#include <cstddef>
 
 struct nemoh_core;
 
@@ -24,5 +25,18 @@ int core_device_count();
 
 // Sets the module NQ carried into the next standalone frequency set-up.
 void core_set_nq_state(nemoh_core* c, int nq);
+
+// Device bytes the handle holds now (every buffer it allocated).
+size_t core_held_bytes(const nemoh_core* c);
+
+// Device bytes a handle needs to solve one frequency of a mesh of nfa
+// panels (np vertices, symmetry nsym) in `precision` with nrhs problems:
+// influence matrices, factors, right-hand-side workspace, geometry and body
+// conditions.
+size_t core_footprint_bytes(int np, int nfa, int nsym, int precision, int nrhs);
+
+// Frees the handle's matrices, factors and right-hand-side workspace (they
+// are allocated again by the next solve); waits for its stream first.
+int core_release_matrices(nemoh_core* c);
 
 }  // namespace nemoh

@@ -808,6 +808,10 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict
     const int j0 = j * kNB, nj = min(kNB, n - j0);
     // the coupling block does not depend on x_j: load it before waiting
     hop_load_tile(h, A + (size_t)k0 * ld + j0, ld, nb, nj, T(-1));
+    if (tid == 0 && !timed_out && spin_limit == 0) {  // every wait gives up at once (forced fallback)
+      timed_out = 1;
+      atomicMax(status, 1);
+    }
     if (tid == 0 && !timed_out) {
       int spins = 0;
       while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != epoch) {

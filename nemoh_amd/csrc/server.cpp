@@ -4,14 +4,20 @@
 // A request is a mesh, its frequencies and the normal velocities of every
 // problem at each frequency; the reply is the forces of every problem.  The
 // unit of work is one (request, frequency).  Work is handed out first come,
-// first served, while the device-memory budget allows: N^2 complex64 =
-// 8 N^2 bytes per in-flight frequency (with nothing in flight the head of
-// the queue is always admitted, so an oversized request still runs).  Each
-// worker thread owns one solver handle, and so one HIP stream and one set of
-// resident matrices and factors; a handle is re-bound to the mesh of
-// whichever request its worker takes next, reusing its device buffers.  So
-// the frequencies of one request, and of requests that arrive while others
-// run, overlap on the GPU as soon as a worker and budget are free.
+// first served, while the device-memory budget allows.  The budget bounds
+// the device memory that the workers' handles hold: a busy worker is
+// charged the larger of what its handle holds and what its frequency needs
+// (FP64 D and S, the complex64 factor copy in mixed precision, the pivot
+// blocks, right-hand sides and geometry: about 32 N^2 bytes per symmetry
+// block in FP64, 40 N^2 in mixed), an idle worker what its handle still
+// holds.  When only idle workers' buffers stand in the way, they are freed;
+// with nothing running the head of the queue is always admitted, so an
+// oversized request still runs.  Each worker thread owns one solver handle,
+// and so one HIP stream and one set of resident matrices and factors; a
+// handle is re-bound to the mesh of whichever request its worker takes
+// next, reusing its device buffers.  So the frequencies of one request, and
+// of requests that arrive while others run, overlap on the GPU as soon as a
+// worker and budget are free.
 //
 // A frequency's result does not depend on the schedule: every kernel is
 // deterministic, handles share nothing but the read-only wave tables, and
@@ -58,7 +64,7 @@ struct Request {
   std::vector<nemoh_dof> rad, force;
   double xeff = 0, yeff = 0;
   std::vector<double> fk;  // (nint, nbeta) per frequency
-  double budget = 0;  // bytes charged per in-flight frequency
+  double footprint = 0;  // device bytes a handle needs for one of its frequencies
   // set-up shared by the frequencies, made once by the first worker
   std::mutex pm;
   std::condition_variable pcv;
@@ -93,6 +99,11 @@ struct Worker {
   nemoh_core* h = nullptr;
   long long bound = -1;     // request the handle's mesh belongs to
   long long dofs_for = -1;  // request whose degrees of freedom the handle holds
+  // guarded by the server mutex:
+  bool busy = false;
+  double held = 0;     // device bytes the handle holds (updated after each item)
+  double charged = 0;  // bytes counted against the budget (held, or more while busy)
+  bool release = false;  // free the handle's matrices before the next item
 };
 
 }  // namespace
@@ -103,7 +114,9 @@ struct nemoh_server {
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Request>> queue;  // requests with frequencies not yet handed out
   std::unordered_map<long long, std::shared_ptr<Request>> table;
-  double budget = 0, in_flight = 0;
+  std::vector<Worker> workers;
+  double budget = 0;
+  double peak = 0;  // largest total charge admitted with other work running
   int running = 0;
   bool stop = false;
   long long next_id = 0;
@@ -206,35 +219,66 @@ void run_item(nemoh_server* s, Worker& w, Request& r, int f) {
   }
 }
 
-void worker_loop(nemoh_server* s) {
-  Worker w;
+// Under the server mutex: may worker `self` take the head of the queue?
+// Frees idle workers' matrices when that is what stands in the way.  On
+// success the worker is charged and marked busy.
+bool try_admit(nemoh_server* s, int self, bool freed = false) {
+  Worker& w = s->workers[self];
+  const double need = s->queue.front()->footprint;
+  double others = 0, idle_held = 0;
+  for (size_t v = 0; v < s->workers.size(); ++v) {
+    if ((int)v == self) continue;
+    others += s->workers[v].charged;
+    if (!s->workers[v].busy) idle_held += s->workers[v].charged;
+  }
+  bool shrink = false;
+  bool ok = others + std::max(w.held, need) <= s->budget;
+  if (!ok && others + need <= s->budget) ok = shrink = true;  // after freeing this worker's own buffers
+  if (!ok && !freed && idle_held > 0 && others - idle_held + need <= s->budget) {
+    // the idle workers' buffers are what stands in the way: free them (an
+    // idle worker never touches its handle without holding this mutex)
+    for (size_t v = 0; v < s->workers.size(); ++v) {
+      Worker& o = s->workers[v];
+      if ((int)v == self || o.busy || o.held == 0) continue;
+      if (o.h) (void)nemoh::core_release_matrices(o.h);
+      o.held = o.charged = o.h ? (double)nemoh::core_held_bytes(o.h) : 0.0;
+    }
+    return try_admit(s, self, true);
+  }
+  int running = 0;
+  for (const Worker& o : s->workers) running += o.busy;
+  if (!ok && running == 0) ok = shrink = true;  // alone on the device: always admitted
+  if (!ok) return false;
+  w.release = shrink && w.held > need;
+  w.charged = w.release ? need : std::max(w.held, need);
+  w.busy = true;
+  if (running > 0) s->peak = std::max(s->peak, others + w.charged);
+  return true;
+}
+
+void worker_loop(nemoh_server* s, int self) {
+  Worker& w = s->workers[self];
   for (;;) {
     std::shared_ptr<Request> r;
     int f = 0;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_work.wait(lk, [&] {
-        if (s->stop) return true;
-        if (s->queue.empty()) return false;
-        return s->in_flight + s->queue.front()->budget <= s->budget || s->running == 0;
-      });
-      if (s->queue.empty()) {
-        if (s->stop) break;
-        continue;
-      }
+      // queued work is finished before a stop takes effect
+      s->cv_work.wait(lk, [&] { return (!s->queue.empty() && try_admit(s, self)) || (s->stop && s->queue.empty()); });
+      if (!w.busy) break;  // stopped with the queue empty
       r = s->queue.front();
       f = r->next++;
       if (r->next == r->nfreq) s->queue.pop_front();
-      s->in_flight += r->budget;
-      ++s->running;
       if (r->t_start < 0) r->t_start = now_ms();
     }
+    if (w.release && w.h) (void)nemoh::core_release_matrices(w.h);
     run_item(s, w, *r, f);
     bool last;
     {
       std::lock_guard<std::mutex> lk(s->mu);
-      s->in_flight -= r->budget;
-      --s->running;
+      w.busy = false;
+      w.release = false;
+      w.held = w.charged = w.h ? (double)nemoh::core_held_bytes(w.h) : 0.0;
       last = ++r->finished == r->nfreq;
       if (last) {
         r->t_done = now_ms();
@@ -244,7 +288,6 @@ void worker_loop(nemoh_server* s) {
     if (last) s->cv_done.notify_all();
     s->cv_work.notify_all();
   }
-  if (w.h) nemoh_core_destroy(w.h);
 }
 
 }  // namespace
@@ -276,8 +319,6 @@ std::shared_ptr<Request> new_request(int np, int nfa, int nsym, const double* x,
   r->g = g;
   r->precision = precision;
   r->nfreq = nfreq;
-  const double ntot = (double)((size_t)nfa << nsym);
-  r->budget = 8.0 * ntot * ntot;
   return r;
 }
 
@@ -315,7 +356,8 @@ int nemoh_server_create(nemoh_server** out, int workers, double budget_bytes, co
   s->apd1z.assign(apd1z, apd1z + nt);
   s->apd2x.assign(apd2x, apd2x + nt);
   s->apd2z.assign(apd2z, apd2z + nt);
-  for (int k = 0; k < workers; ++k) s->threads.emplace_back(worker_loop, s);
+  s->workers.resize(workers);
+  for (int k = 0; k < workers; ++k) s->threads.emplace_back(worker_loop, s, k);
   *out = s;
   return NEMOH_OK;
 }
@@ -335,6 +377,7 @@ int nemoh_server_submit(nemoh_server* s, int np, int nfa, int nsym, const double
   r->nvel.assign(nvel, nvel + 2 * ntot * nrhs * nfreq);
   r->nds.assign(nds, nds + (size_t)nint * ntot);
   r->forces.assign(2 * (size_t)nint * nrhs * nfreq, 0.0);
+  r->footprint = (double)nemoh::core_footprint_bytes(np, nfa, nsym, precision, nrhs);
   return enqueue(s, r, id);
 }
 
@@ -364,6 +407,7 @@ int nemoh_server_submit_waves(nemoh_server* s, int np, int nfa, int nsym, const 
   r->nint = nforce;
   r->forces.assign(2 * (size_t)nforce * r->nrhs * nfreq, 0.0);
   r->fk.assign(2 * (size_t)nforce * nbeta * nfreq, 0.0);
+  r->footprint = (double)nemoh::core_footprint_bytes(np, nfa, nsym, precision, r->nrhs);
   return enqueue(s, r, id);
 }
 
@@ -397,6 +441,16 @@ int nemoh_server_wait(nemoh_server* s, long long id, double* forces, double* fk,
   return NEMOH_OK;
 }
 
+int nemoh_server_memory(nemoh_server* s, double* held, double* peak) {
+  if (!s) return NEMOH_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  double h = 0;
+  for (const Worker& w : s->workers) h += w.held;
+  if (held) *held = h;
+  if (peak) *peak = s->peak;
+  return NEMOH_OK;
+}
+
 const char* nemoh_server_last_error(nemoh_server* s) {
   if (!s) return "null handle";
   std::lock_guard<std::mutex> lk(s->mu);
@@ -411,6 +465,8 @@ void nemoh_server_destroy(nemoh_server* s) {
   }
   s->cv_work.notify_all();
   for (auto& t : s->threads) t.join();
+  for (Worker& w : s->workers)
+    if (w.h) nemoh_core_destroy(w.h);
   delete s;
 }
 

@@ -2,9 +2,9 @@
 
 ``Server`` wraps the nemoh_server_* C ABI (the loop itself is native:
 worker threads, each owning a solver handle and HIP stream, take one
-(request, frequency) at a time first come first served while the
-in-flight frequencies fit the device-memory budget of N^2 complex64
-each).  ``two_cylinder_request`` draws the serving workload: two vertical
+(request, frequency) at a time first come first served while the device
+memory that the workers' handles hold fits the budget).
+``two_cylinder_request`` draws the serving workload: two vertical
 cylinders of one random radius r ~ U(2, 15) m at a spacing U(3, 10) r,
 N_total log-uniform in [200, 16000] panels, 12 degrees of freedom (six
 per body), 1-4 headings and 1-64 frequencies, infinite depth.
@@ -153,6 +153,15 @@ class Server:
         if with_fk:
             return out.transpose(0, 2, 1), fk[:, :nbeta].transpose(0, 2, 1), timing
         return out.transpose(0, 2, 1), timing
+
+    def memory(self):
+        """(bytes the workers' handles hold now, largest total charged against
+        the budget while other frequencies ran)."""
+        held, peak = C.c_double(), C.c_double()
+        r = core.load().nemoh_server_memory(self.h, C.byref(held), C.byref(peak))
+        if r:
+            raise core.NemohError(r, "nemoh_server_memory failed")
+        return held.value, peak.value
 
     def close(self):
         if getattr(self, "h", None):

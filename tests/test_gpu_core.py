@@ -267,3 +267,43 @@ def test_fp32_assembly_option(name, tol):
     err = _normwise(f, ref)
     print(f"{name}: FP32 assembly, mixed: max normwise rel err {err:.3e}")
     assert err < tol
+
+
+@pytest.mark.parametrize("prec", [core.FP64, core.MIXED])
+def test_solve_wait_timeout_falls_back_to_stepwise(prec):
+    """A chained triangular solve whose bounded waits give up (forced with a
+    spin limit of 0: every wait gives up at once, and every kernel still
+    ends normally) is redone one launch per block step within the same call:
+    the forces still match the reference and the handle counts the
+    fallback."""
+    d, m, ref = _case("barge_fd")
+    s = Solver(m, float(d["depth"]), float(d["rho"]), float(d["g"]), precision=prec, tables=_tables())
+    s.core.set_option(core.OPT_SOLVE_SPIN_LIMIT, 0)
+    f = s.solve_problems(d["omega"], d["nvel"], d["nds"])
+    st = s.core.stats()
+    s.close()
+    err = _normwise(f, ref)
+    print(f"spin limit 0, precision {prec}: {st.solve_fallbacks} fallbacks, max normwise rel err {err:.3e}")
+    assert st.solve_fallbacks >= 1
+    assert err < (FP64_TOL if prec == core.FP64 else MIXED_TOL)
+
+
+def test_nonfinite_rhs_is_reported_on_the_mixed_path():
+    """A NaN right-hand side: the mixed refinement reports it (its residual
+    maximum keeps NaN instead of reading it as converged), and the handle
+    solves the next frequency normally; the FP64 path returns NaN forces."""
+    d, m, ref = _case("barge_fd")
+    nv = d["nvel"][:, :9].copy()
+    nv[3, 2] = np.nan
+    c = core.Core(d["vertices"], d["panels"], 0, float(d["depth"]), float(d["rho"]), float(d["g"]), tables=_tables())
+    c.set_precision(core.MIXED)
+    period = 2 * np.pi / d["omega"][0]
+    with pytest.raises(core.NemohError) as e:
+        c.solve_frequency(period, nv, nds=d["nds"])
+    assert e.value.code == core.ERR_CONVERGENCE and "non-finite" in str(e.value)
+    _, f = c.solve_frequency(period, d["nvel"][:, :9], nds=d["nds"], want_pressure=False)
+    assert _normwise(f, ref[:, :9]) < MIXED_TOL
+    c.set_precision(core.FP64)
+    _, f = c.solve_frequency(period, nv, nds=d["nds"], want_pressure=False)
+    assert np.isnan(f).any()
+    c.close()

@@ -189,3 +189,29 @@ def test_wave_requests_match_array_requests():
         pinc = np.stack([bcm.diffraction(m, w, b, 0.0, 9.81, 1025.0)[1] for b in betas], axis=1)
         ref = -(nds @ pinc)
         assert np.abs(fk[i] - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+
+@pytest.mark.gpu
+def test_budget_bounds_device_memory():
+    """A budget that holds about two of the mix's largest frequencies at once
+    (40 N^2 bytes each in mixed precision): every request completes with the
+    forces of an unconstrained run, bit for bit; the total charged while
+    several frequencies ran never exceeded the budget; and the idle handles
+    hold no more than it at the end."""
+    rng = np.random.default_rng(11)
+    reqs = [serving.two_cylinder_request(rng, n_range=(1200, 2600), max_freq=3) for _ in range(6)]
+    nmax = max(r.npanels for r in reqs)
+    budget = 2.2 * 40.0 * nmax ** 2
+    tables = _tables()
+    out = {}
+    for b in (64.0 * 2 ** 30, budget):
+        with serving.Server(workers=4, budget_bytes=b, tables=tables) as srv:
+            ids = [srv.submit(r, core.MIXED) for r in reqs]
+            out[b] = [srv.wait(i)[0] for i in ids]
+            held, peak = srv.memory()
+        print(f"budget {b / 2**30:.2f} GiB: peak charged {peak / 2**30:.2f} GiB, held at the end {held / 2**30:.2f} GiB")
+        if b == budget:
+            assert peak <= budget and held <= budget
+    for a, c in zip(out[budget], out[64.0 * 2 ** 30]):
+        assert np.array_equal(a, c)
