@@ -104,11 +104,12 @@ struct nemoh_core {
   double* d_red = nullptr;  // reduction scratch
   int* d_info = nullptr;     // [0] first bad pivot row, [1] solve wait timeout
   RefineState* d_rs = nullptr;  // mixed refinement state of each symmetry block
-  // flags of the single-launch triangular solves (2 ceil(n/64)), then the
-  // two triangles' ticket counters; chain_calls = chained solves since the
-  // last binding (each advances both counters by ceil(n/64))
+  // the two triangles' ticket counters of the single-launch triangular
+  // solves, then their flags (forward, backward: ceil(n/64) each, shared by
+  // both precisions); ticket_base = what the counters hold (each chained
+  // solve advances both by its number of workgroups)
   int* d_flags = nullptr;
-  unsigned chain_calls = 0;
+  unsigned ticket_base = 0;
   int epoch = 0;
   int cu_count = 0;
   bool stepwise_solve = false;
@@ -368,16 +369,16 @@ template <typename T>
 void lu_solve(nemoh_core* c, const C2<T>* A, int ld, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool stepwise,
               const RefineState* skip = nullptr) {
   const int n = c->nfa;
-  const int nblk = (n + kNB - 1) / kNB;
+  const int nsb = (n + kNB - 1) / kNB;  // workgroups of each chained launch
   const bool chained = !stepwise;
   unsigned base = 0;
   if (chained) {
     if (++c->epoch == 0) ++c->epoch;  // flags start at 0: an epoch is never 0
-    base = c->chain_calls++ * (unsigned)nblk;
+    base = c->ticket_base;
+    c->ticket_base += (unsigned)nsb;
   }
-  launch_lu_solve<T>(A, ld, n, P, B, W, nrhs, chained, c->d_flags, c->epoch,
-                     reinterpret_cast<unsigned*>(c->d_flags + 2 * (size_t)nblk), base, c->d_info + 1, c->spin_limit,
-                     c->st, skip);
+  launch_lu_solve<T>(A, ld, n, P, B, W, nrhs, chained, c->d_flags + 2, c->epoch, reinterpret_cast<unsigned*>(c->d_flags),
+                     base, c->d_info + 1, c->spin_limit, c->st, skip);
 }
 
 // Mixed path, one refinement sweep of symmetry block b, every kernel a no-op
@@ -702,7 +703,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   // (solves compare them with an epoch that is never 0), ticket counters
   // with them
   HIPCHK(c, hipMemsetAsync(c->d_flags, 0, c->caps[(const void*)&c->d_flags] * sizeof(int), c->st));
-  c->chain_calls = 0;
+  c->ticket_base = 0;
   // GM = max over vertex pairs of the horizontal distance (and to the y
   // images when nsym = 1), the argument of the theta rule
   launch_extent(c->d_pts, c->d_pts + np, np, nsym, c->d_ext, c->st);

@@ -74,11 +74,12 @@ void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, hip
 // Solves A x = b for nrhs <= kMaxRhs right-hand sides (B: n x nrhs
 // row-major) with the factors of launch_lu; on return B holds the solution,
 // W is scratch of the same size.  chained: one launch per triangle whose
-// workgroups take row blocks in ticket order and pass solved blocks through
-// flags (flags: 2*ceil(n/64) ints, never reset, epoch must change per call
-// and never be 0; tickets: 2 counters, both equal to base at the call, each
-// advanced by ceil(n/64); status set nonzero if a bounded wait timed out);
-// else one launch per block step.  Skipped when skip (optional) is done.
+// workgroups take 64-row blocks in ticket order and pass solved blocks
+// through flags (flags: 2 ceil(n/64) ints, never reset, epoch must change per
+// call and never be 0; tickets: 2 counters, both equal to base at the call,
+// each advanced by ceil(n/64); status set nonzero if a
+// bounded wait timed out); else one launch per block step.  Skipped when
+// skip (optional) is done.
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
                      int* flags, int epoch, unsigned* tickets, unsigned base, int* status, int spin_limit,

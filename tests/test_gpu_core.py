@@ -307,3 +307,22 @@ def test_nonfinite_rhs_is_reported_on_the_mixed_path():
     _, f = c.solve_frequency(period, nv, nds=d["nds"], want_pressure=False)
     assert np.isnan(f).any()
     c.close()
+
+
+def test_alternating_precisions_on_one_handle():
+    """FP64 and mixed solves alternating on one handle (the request loop's
+    workers switch precision between items): the chained solves of the two
+    precisions share the handle's ticket counters and flags, and every solve
+    still matches the reference."""
+    d, m, ref = _case("cyl512")
+    c = core.Core(d["vertices"], d["panels"], int(d["isym"]), float(d["depth"]), float(d["rho"]), float(d["g"]),
+                  tables=_tables())
+    om = d["omega"]
+    for it in range(6):
+        prec = core.FP64 if it % 2 == 0 else core.MIXED
+        c.set_precision(prec)
+        j = 7 * (3 * it % 20)  # a different frequency each time (7 problems each)
+        _, f = c.solve_frequency(2 * np.pi / om[j], d["nvel"][:, j:j + 7], nds=d["nds"], want_pressure=False)
+        err = _normwise(f, ref[:, j:j + 7])
+        assert err < (FP64_TOL if prec == core.FP64 else MIXED_TOL), (it, prec, err)
+    c.close()
