@@ -90,14 +90,17 @@ def run_c3(steps):
 BETAS = np.deg2rad([0.0, 45.0, 90.0])
 
 
-def run_precision(core, prec, omegas, nvel, nds, steps, warmup, nrhs=9, host_inputs=False, betas=BETAS):
+def run_precision(core, prec, omegas, nvel, nds, steps, warmup, nrhs=9, host_inputs=False, betas=BETAS,
+                  timing=True):
     """One frequency at a time on one handle.  By default the 9 problems of
     each frequency are generated on the device (nemoh_core_solve_waves: no
     per-frequency input crosses PCIe); host_inputs=True hands the host
     arrays over instead (nemoh_core_solve_frequency: NVEL and NDS uploaded,
     the drop-in's per-call data flow)."""
     core.set_precision(prec)
-    core.enable_timing(True)
+    # per-phase events launch every kernel one by one; without them each
+    # frequency replays the handle's HIP graph
+    core.enable_timing(timing)
     forces = {}
     phases = {"assembly": [], "factor": [], "solve": [], "update": [], "update_flops": [], "it": []}
 
@@ -383,6 +386,9 @@ def main():
     tmx, fmx, phmx = run_precision(c, ncore.MIXED, omegas, nvel, nds, ns, 2)
     # the drop-in's per-call data flow: NVEL and NDS handed over from host memory
     th, fh, _ = run_precision(c, ncore.MIXED, omegas, nvel, nds, ns, 2, host_inputs=True)
+    # the same frequencies without per-phase timing: HIP-graph replays
+    tg64, _, _ = run_precision(c, ncore.FP64, omegas, nvel, nds, ns, 2, timing=False)
+    tgmx, _, _ = run_precision(c, ncore.MIXED, omegas, nvel, nds, ns, 2, timing=False)
     host_same = max(float((np.abs(fh[k] - fmx[k]).max(0) / np.abs(fmx[k]).max(0)).max()) for k in fmx if k in fh)
     outputs = run_outputs(c) if a.outputs else None
     c.close()
@@ -425,6 +431,7 @@ def main():
         "max_rel_err_mixed_vs_fp64": err,
         "frequencies_in_flight": a.workers,
         "single_stream": {"passes": a.single_passes, "mixed": round(ss_mixed, 3), "fp64": round(ss_fp64, 3),
+                          "mixed_graphs": round(ns / tgmx, 3), "fp64_graphs": round(ns / tg64, 3),
                           "identical_to_concurrent": bool(same),
                           "mixed_host_inputs": round(ns / th, 3),
                           "host_vs_device_inputs_max_rel_diff": host_same},

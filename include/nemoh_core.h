@@ -189,7 +189,13 @@ int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter
  * about a second); when any gives up, the solves of that call are redone
  * with one launch per block step (nemoh_stats.solve_fallbacks counts it).
  * 0 makes every wait give up at once, which forces that path (tests). */
-enum { NEMOH_OPT_STEPWISE_SOLVE = 1, NEMOH_OPT_FP32_ASSEMBLY = 2, NEMOH_OPT_SOLVE_SPIN_LIMIT = 3 };
+/* NEMOH_OPT_GRAPHS: 1 (default) = each frequency's assembly,
+ * factorisation, solves and phi replay from a HIP graph the handle captures
+ * on first use (one per precision, regime, problem count and solve form;
+ * dropped when the handle's buffers or mesh change); 0 = launched one by one.
+ * Timing (nemoh_core_enable_timing) always launches one by one. */
+enum { NEMOH_OPT_STEPWISE_SOLVE = 1, NEMOH_OPT_FP32_ASSEMBLY = 2, NEMOH_OPT_SOLVE_SPIN_LIMIT = 3,
+       NEMOH_OPT_GRAPHS = 4 };
 int nemoh_core_set_option(nemoh_core *ctx, int option, int value);
 
 /* Timings (ms) of the last solve's phases and refinement statistics. */
@@ -210,6 +216,8 @@ typedef struct {
 int nemoh_core_get_stats(const nemoh_core *ctx, nemoh_stats *out);
 int nemoh_core_enable_timing(nemoh_core *ctx, int on);
 
+/* The handle's last error message; with ctx = NULL, the calling thread's
+ * last nemoh_core_create failure. */
 const char *nemoh_core_last_error(const nemoh_core *ctx);
 void nemoh_core_destroy(nemoh_core *ctx);
 

@@ -17,6 +17,7 @@ FP64, MIXED = 0, 1
 OPT_STEPWISE_SOLVE = 1
 OPT_FP32_ASSEMBLY = 2
 OPT_SOLVE_SPIN_LIMIT = 3
+OPT_GRAPHS = 4
 
 EXPORTS = (
     "nemoh_core_create", "nemoh_core_solve", "nemoh_core_solve_multi", "nemoh_core_solve_frequency",

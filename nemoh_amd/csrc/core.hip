@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -104,17 +105,25 @@ struct nemoh_core {
   double* d_red = nullptr;  // reduction scratch
   int* d_info = nullptr;     // [0] first bad pivot row, [1] solve wait timeout
   RefineState* d_rs = nullptr;  // mixed refinement state of each symmetry block
-  // the two triangles' ticket counters of the single-launch triangular
-  // solves, then their flags (forward, backward: ceil(n/64) each, shared by
-  // both precisions); ticket_base = what the counters hold (each chained
-  // solve advances both by its number of workgroups)
+  // the chained triangular solves' control block (ChainCtl: tickets and
+  // epoch, kept on the device), then their flags (forward, backward:
+  // ceil(n/64) each, shared by both precisions)
   int* d_flags = nullptr;
-  unsigned ticket_base = 0;
-  int epoch = 0;
   int cu_count = 0;
   bool stepwise_solve = false;
   bool fp32_assembly = false;  // NEMOH_OPT_FP32_ASSEMBLY (mixed path only)
   int spin_limit = 1 << 22;    // polls before a chained solve's wait gives up (about a second)
+  // HIP graphs of the per-frequency pipeline (NEMOH_OPT_GRAPHS), keyed by
+  // what shapes the launch sequence; layout_gen moves on whenever a device
+  // buffer is replaced or the handle is bound to another mesh, which
+  // invalidates every graph (their kernel arguments hold the old pointers)
+  bool use_graphs = true;
+  struct GraphEntry {
+    int refactor, mode, fd, nrhs, stepwise;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  unsigned layout_gen = 0, graphs_gen = 0;
   // statistics
   bool timing = false;
   nemoh_stats stats{};
@@ -126,6 +135,20 @@ struct nemoh_core {
 };
 
 namespace {
+
+// Graph capture and device-memory management never overlap in this process:
+// a capture holds this lock from its beginning to its end, and so does every
+// allocation, free and synchronous set-up call of the library, which HIP may
+// refuse (or which may invalidate the capture) while another thread's stream
+// is capturing.  Captures happen once per handle and launch shape, and take
+// about a millisecond.
+std::recursive_mutex& device_lock() {
+  static std::recursive_mutex m;
+  return m;
+}
+
+// The last creation error of this thread (nemoh_core_last_error(NULL)).
+thread_local std::string create_error;
 
 // The device address of `scratch` on device dev, looked up once per device
 // with that device current (each GPU holds its own copy of the array).
@@ -162,6 +185,7 @@ int fail(nemoh_core* c, int code, const std::string& msg) {
 // a larger one).
 template <typename T>
 int dalloc(nemoh_core* c, T** p, size_t count) {
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
   size_t& have = c->caps[(const void*)p];
   size_t& bytes = c->cap_bytes[(const void*)p];
   if (*p && have >= count) return NEMOH_OK;
@@ -171,12 +195,15 @@ int dalloc(nemoh_core* c, T** p, size_t count) {
   HIPCHK(c, hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
   have = count;
   bytes = std::max<size_t>(count, 1) * sizeof(T);
+  ++c->layout_gen;
   return NEMOH_OK;
 }
 
 template <typename T>
 void dfree(nemoh_core* c, T** p) {
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
   if (*p) (void)hipFree(*p);
+  if (*p) ++c->layout_gen;
   *p = nullptr;
   c->caps[(const void*)p] = 0;
   c->cap_bytes[(const void*)p] = 0;
@@ -307,15 +334,12 @@ int check_info(nemoh_core* c) {
   return NEMOH_OK;
 }
 
-// Assembly + factorisation for frequency w (skipped while w is unchanged).
-int prepare_frequency(nemoh_core* c, double w) {
-  const int mode = c->prec + (c64_matrices(c) ? 2 : 0);  // what the resident matrices hold
-  if (w == c->w_prev && c->factored_prec == mode) {
-    c->stats.cache_hit = 1;
-    return NEMOH_OK;
-  }
-  c->stats.cache_hit = 0;
-  if (int r = ensure_matrices(c)) return r;
+// What the resident matrices hold: precision, and complex64 D and S.
+int matrix_mode(const nemoh_core* c) { return c->prec + (c64_matrices(c) ? 2 : 0); }
+
+// Queues assembly + factorisation of the current frequency (matrices
+// allocated; no host wait and, without timing, no event: capturable).
+int queue_factor(nemoh_core* c) {
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->st));
   if (c64_matrices(c)) {
     const MatOut32 out{c->ld32, c->D32r[0], c->D32[0], c->S32[0], c->D32r[1], c->D32[1], c->S32[1]};
@@ -331,7 +355,7 @@ int prepare_frequency(nemoh_core* c, double w) {
   }
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
-  HIPCHK(c, hipMemsetAsync(c->d_info, 0, 2 * sizeof(int), c->st));
+  launch_info_reset(c->d_info, c->st);
   const int nblk = (c->nfa + kNB - 1) / kNB;
   c->upd_blocks = 0;
   if (c->timing) {
@@ -354,10 +378,8 @@ int prepare_frequency(nemoh_core* c, double w) {
   }
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
-  // the pivot flag is read back with the outputs (check_info), so the host
-  // never waits between factorisation and solves
-  c->w_prev = w;
-  c->factored_prec = mode;
+  // the pivot flag is read back after the solves, so the host never waits
+  // between factorisation and solves
   return NEMOH_OK;
 }
 
@@ -369,16 +391,9 @@ template <typename T>
 void lu_solve(nemoh_core* c, const C2<T>* A, int ld, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool stepwise,
               const RefineState* skip = nullptr) {
   const int n = c->nfa;
-  const int nsb = (n + kNB - 1) / kNB;  // workgroups of each chained launch
-  const bool chained = !stepwise;
-  unsigned base = 0;
-  if (chained) {
-    if (++c->epoch == 0) ++c->epoch;  // flags start at 0: an epoch is never 0
-    base = c->ticket_base;
-    c->ticket_base += (unsigned)nsb;
-  }
-  launch_lu_solve<T>(A, ld, n, P, B, W, nrhs, chained, c->d_flags + 2, c->epoch, reinterpret_cast<unsigned*>(c->d_flags),
-                     base, c->d_info + 1, c->spin_limit, c->st, skip);
+  static_assert(sizeof(ChainCtl) == 4 * sizeof(int), "ChainCtl takes the first four ints of d_flags");
+  launch_lu_solve<T>(A, ld, n, P, B, W, nrhs, !stepwise, reinterpret_cast<ChainCtl*>(c->d_flags), c->d_flags + 4,
+                     c->d_info + 1, c->spin_limit, c->st, skip);
 }
 
 // Mixed path, one refinement sweep of symmetry block b, every kernel a no-op
@@ -451,6 +466,63 @@ int refine_outcome(nemoh_core* c, const RefineState* h, int pivot_row) {
   return NEMOH_OK;
 }
 
+// Queues the right-hand sides and solves of every symmetry block, with phi
+// (no host wait: capturable).
+int queue_solves(nemoh_core* c, int nrhs, bool stepwise) {
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
+  for (int b = 0; b < c->nb; ++b) {
+    launch_rhs(c->V, c->nfa, c->nsym, nrhs, b == 0 ? 1.0 : -1.0, c->B, c->st);
+    solve_block(c, b, nrhs, stepwise);
+  }
+  HIPCHK(c, hipGetLastError());
+  return NEMOH_OK;
+}
+
+void drop_graphs(nemoh_core* c) {
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+  c->graphs_gen = c->layout_gen;
+}
+
+// Assembly + factorisation (when refactor) and the solves of one frequency:
+// eagerly with timing on or graphs off, else by replaying the handle's HIP
+// graph of that launch sequence, captured on its first use.  The frequency's
+// parameters (d_par) and right-hand sides (V) are in device memory, which
+// the graph reads, and every kernel argument is fixed for a given binding,
+// precision, regime, problem count and solve form.
+int launch_frequency(nemoh_core* c, bool refactor, int nrhs, bool stepwise) {
+  if (!c->use_graphs || c->timing) {
+    if (refactor)
+      if (int r = queue_factor(c)) return r;
+    return queue_solves(c, nrhs, stepwise);
+  }
+  if (c->graphs_gen != c->layout_gen) drop_graphs(c);
+  const int key[5] = {refactor ? 1 : 0, matrix_mode(c), c->h_par.fd, nrhs, stepwise ? 1 : 0};
+  hipGraphExec_t exec = nullptr;
+  for (const auto& g : c->graphs)
+    if (g.refactor == key[0] && g.mode == key[1] && g.fd == key[2] && g.nrhs == key[3] && g.stepwise == key[4])
+      exec = g.exec;
+  if (!exec) {
+    std::lock_guard<std::recursive_mutex> lk(device_lock());
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    int r = refactor ? queue_factor(c) : NEMOH_OK;
+    if (!r) r = queue_solves(c, nrhs, stepwise);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->st, &graph);
+    if (r) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    if (e != hipSuccess) return fail(c, NEMOH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(c, NEMOH_ERR_HIP, std::string("graph instantiation: ") + hipGetErrorString(ei));
+    c->graphs.push_back({key[0], key[1], key[2], key[3], key[4], exec});
+  }
+  HIPCHK(c, hipGraphLaunch(exec, c->st));
+  return NEMOH_OK;
+}
+
 // Everything after the frequency set-up: c->V holds (ntot x nrhs) row-major
 // normal velocities on the device.  The solves of every symmetry block, phi
 // and the combination are queued without a host wait; one wait at the end
@@ -459,19 +531,22 @@ int refine_outcome(nemoh_core* c, const RefineState* h, int pivot_row) {
 // a timed-out chained solve (which the ticket order makes unexpected) is
 // redone with one launch per block step.
 int run_pipeline(nemoh_core* c, double w, int nrhs) {
-  if (int r = prepare_frequency(c, w)) return r;
+  const int mode = matrix_mode(c);
+  const bool refactor = !(w == c->w_prev && c->factored_prec == mode);  // the factors cached per frequency
+  c->stats.cache_hit = refactor ? 0 : 1;
+  if (refactor)
+    if (int r = ensure_matrices(c)) return r;
   const int n = c->nfa;
-  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
   c->stats.refine_iterations = 0;
   c->stats.refine_residual = 0;
   bool stepwise = c->stepwise_solve;
   const size_t cnt = (size_t)n * nrhs;
   for (int attempt = 0;; ++attempt) {
-    for (int b = 0; b < c->nb; ++b) {
-      launch_rhs(c->V, n, c->nsym, nrhs, b == 0 ? 1.0 : -1.0, c->B, c->st);
-      solve_block(c, b, nrhs, stepwise);
+    if (int r = launch_frequency(c, attempt == 0 && refactor, nrhs, stepwise)) return r;
+    if (attempt == 0 && refactor) {
+      c->w_prev = w;
+      c->factored_prec = mode;
     }
-    HIPCHK(c, hipGetLastError());
     RefineState h[2] = {};
     int info[2] = {0, 0};
     for (;;) {
@@ -662,6 +737,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
                    const int* m1, const int* m2, const int* m3, const int* m4, const double* const geo[8], double zer,
                    double g, double rho, double depth) {
   if (!c) return NEMOH_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
   if (!mesh_args_ok(np, nfa, nsym, x, y, z, m1, m2, m3, m4, geo, depth))
     return fail(c, NEMOH_ERR_ARG, "bad mesh arguments");
   std::vector<double> vtx(12 * (size_t)nfa);
@@ -689,7 +765,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   int r = 0;
   if ((r = dalloc(c, &c->d_geo, 8 * (size_t)nfa)) || (r = dalloc(c, &c->d_vtx, 12 * (size_t)nfa)) ||
       (r = dalloc(c, &c->d_surf, surf.size())) ||
-      (r = dalloc(c, &c->d_flags, 2 * (size_t)nblk + 2)) || (r = dalloc(c, &c->d_pts, 2 * (size_t)np)) ||
+      (r = dalloc(c, &c->d_flags, 2 * (size_t)nblk + 4)) || (r = dalloc(c, &c->d_pts, 2 * (size_t)np)) ||
       (r = dalloc(c, &c->d_ext, nred)))
     return r;
   for (int k = 0; k < 8; ++k)
@@ -699,11 +775,9 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
     HIPCHK(c, hipMemcpyAsync(c->d_surf, surf.data(), surf.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->d_pts, x, np * sizeof(double), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->d_pts + np, y, np * sizeof(double), hipMemcpyHostToDevice, c->st));
-  // flags of the chained triangular solves: cleared over the whole buffer
-  // (solves compare them with an epoch that is never 0), ticket counters
-  // with them
+  // the chained triangular solves' control block and flags, cleared over the
+  // whole buffer (solves compare the flags with an epoch that is never 0)
   HIPCHK(c, hipMemsetAsync(c->d_flags, 0, c->caps[(const void*)&c->d_flags] * sizeof(int), c->st));
-  c->ticket_base = 0;
   // GM = max over vertex pairs of the horizontal distance (and to the y
   // images when nsym = 1), the argument of the theta rule
   launch_extent(c->d_pts, c->d_pts + np, np, nsym, c->d_ext, c->st);
@@ -719,6 +793,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   c->np = np;
   c->nfa = nfa;
   c->ld32 = (nfa + 3) / 4 * 4;
+  ++c->layout_gen;  // kernel arguments of the graphs (geometry, sizes) change
   c->nsym = nsym;
   c->nb = 1 << nsym;
   c->zer = zer;
@@ -823,10 +898,19 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
   if (!xr || !xz || !apd1x || !apd1z || !apd2x || !apd2z) return NEMOH_ERR_ARG;
   const double* geo[8] = {xg, yg, zg, xn, yn, zn, aire, tdis};
   if (!mesh_args_ok(np, nfa, nsym, x, y, z, m1, m2, m3, m4, geo, depth)) return NEMOH_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
+  create_error.clear();
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NEMOH_ERR_HIP;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    create_error = "no HIP device";
+    return NEMOH_ERR_HIP;
+  }
   nemoh_core* c = new nemoh_core();
   auto bail = [&](int code) {
+    if (create_error.empty()) {
+      const hipError_t e = hipGetLastError();
+      create_error = c->err.empty() ? std::string("handle creation: ") + hipGetErrorString(e) : c->err;
+    }
     nemoh_core_destroy(c);
     return code;
   };
@@ -873,14 +957,20 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
     c->cu_count = prop.multiProcessorCount;
     if (noalloc_scratch(dev, &c->d_scratch) != hipSuccess) return bail(NEMOH_ERR_HIP);
   }
-  if (hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_tab32, tab32.data(), tab32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_rden, rden.data(), rden.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_xr, xr, kNR * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_xz, xz, kNZ * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(c->d_info, 0, 2 * sizeof(int)) != hipSuccess)
+  // copies on the handle's own stream (no legacy-stream synchronisation)
+  if (hipMemcpyAsync(c->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->st) != hipSuccess ||
+      hipMemcpyAsync(c->d_tab32, tab32.data(), tab32.size() * sizeof(float), hipMemcpyHostToDevice, c->st) !=
+          hipSuccess ||
+      hipMemcpyAsync(c->d_rden, rden.data(), rden.size() * sizeof(double), hipMemcpyHostToDevice, c->st) !=
+          hipSuccess ||
+      hipMemcpyAsync(c->d_xr, xr, kNR * sizeof(double), hipMemcpyHostToDevice, c->st) != hipSuccess ||
+      hipMemcpyAsync(c->d_xz, xz, kNZ * sizeof(double), hipMemcpyHostToDevice, c->st) != hipSuccess ||
+      hipMemsetAsync(c->d_info, 0, 2 * sizeof(int), c->st) != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess)
     return bail(NEMOH_ERR_HIP);
   c->T = Tables{c->d_xr, c->d_xz, c->d_tab, c->d_rden, c->d_rden + 3 * (kNR - 2)};
+  // NEMOH_GRAPHS=0 in the environment: launch one by one (as
+  // NEMOH_OPT_GRAPHS = 0), e.g. under a profiler that traces kernels
+  if (const char* e = std::getenv("NEMOH_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
   if (int r = core_bind_mesh(c, np, nfa, nsym, x, y, z, m1, m2, m3, m4, geo, zer, g, rho, depth)) return bail(r);
   *out = c;
   return NEMOH_OK;
@@ -888,6 +978,7 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
 
 void nemoh_core_destroy(nemoh_core* c) {
   if (!c) return;
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
   if (c->st) (void)hipStreamSynchronize(c->st);
   auto release = [](auto*& p) {
     if (p) (void)hipFree(p);
@@ -944,11 +1035,15 @@ void nemoh_core_destroy(nemoh_core* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->upd_ev) (void)hipEventDestroy(e);
+  drop_graphs(c);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
 
-const char* nemoh_core_last_error(const nemoh_core* c) { return c ? c->err.c_str() : "null handle"; }
+const char* nemoh_core_last_error(const nemoh_core* c) {
+  if (c) return c->err.c_str();
+  return create_error.empty() ? "null handle" : create_error.c_str();
+}
 
 int nemoh_core_set_precision(nemoh_core* c, int mode, double tol, int max_iter) {
   if (!c || (mode != NEMOH_FP64 && mode != NEMOH_MIXED)) return NEMOH_ERR_ARG;
@@ -966,6 +1061,10 @@ int nemoh_core_set_option(nemoh_core* c, int option, int value) {
   }
   if (option == NEMOH_OPT_FP32_ASSEMBLY) {
     c->fp32_assembly = value != 0;
+    return NEMOH_OK;
+  }
+  if (option == NEMOH_OPT_GRAPHS) {
+    c->use_graphs = value != 0;
     return NEMOH_OK;
   }
   if (option == NEMOH_OPT_SOLVE_SPIN_LIMIT) {
@@ -1267,6 +1366,7 @@ int nemoh_core_pairs(nemoh_core* c, double period, double amh, int nexp, const d
   if (int r = build_params(c, period, amh, nexp, ambda, ar, nq, qq, cq, &fd, &w)) return r;
   const size_t cnt = (size_t)c->nfa * c->nfa * 16;
   double* d = nullptr;
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
   HIPCHK(c, hipMalloc((void**)&d, cnt * sizeof(double)));
   launch_pairs(c->G, c->T, c->d_par, c->h_par.fd, d, c->st);
   hipError_t e = hipGetLastError();
@@ -1305,6 +1405,7 @@ int nemoh_core_build_tables(double* xr, double* xz, double* apd1x, double* apd1z
   for (int i = 2; i <= kNR; ++i)
     xr[i - 1] = (i < 40) ? std::fmin(std::pow(10.0, (i - 1.0) / 5 - 6), 4.0 / 3.0 + std::fabs((i - 32.0) / 3.0))
                          : 4.0 / 3.0 + std::fabs((i - 32.0) / 3.0);
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
   double *dxr = nullptr, *dxz = nullptr, *dtab = nullptr;
   std::vector<double> tab(4 * (size_t)kNR * kNZ);
   hipError_t e = hipMalloc((void**)&dxr, kNR * sizeof(double));
@@ -1344,6 +1445,7 @@ int nemoh_core_prony(double akh, double amh, int* nexp, double* ambda, double* a
 
 int nemoh_core_mfma_probe(int fp32, double* out) {
   if (!out) return NEMOH_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
   double ha[64], hb[64];
   for (int i = 0; i < 16; ++i)
     for (int k = 0; k < 4; ++k) ha[i * 4 + k] = (i + 1) * 0.5 + k * 3.0;

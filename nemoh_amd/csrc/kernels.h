@@ -71,19 +71,29 @@ void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int f
 // at its end.
 template <typename T>
 void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, hipEvent_t* upd_ev = nullptr);
+// Control block of the chained triangular solves of one handle: the two
+// triangles' ticket counters and the current solve's epoch (both set on the
+// device at the start of every chained solve).
+struct ChainCtl {
+  unsigned ticket[2];
+  int epoch, pad;
+};
 // Solves A x = b for nrhs <= kMaxRhs right-hand sides (B: n x nrhs
 // row-major) with the factors of launch_lu; on return B holds the solution,
 // W is scratch of the same size.  chained: one launch per triangle whose
 // workgroups take 64-row blocks in ticket order and pass solved blocks
-// through flags (flags: 2 ceil(n/64) ints, never reset, epoch must change per
-// call and never be 0; tickets: 2 counters, both equal to base at the call,
-// each advanced by ceil(n/64); status set nonzero if a
-// bounded wait timed out); else one launch per block step.  Skipped when
-// skip (optional) is done.
+// through flags (flags: 2 ceil(n/64) ints, cleared to 0 when the handle is
+// bound and never reset; a kernel at the start of each solve resets ctl's
+// tickets and moves its epoch on; status set nonzero if a bounded wait timed
+// out), with no per-call argument, so the launches replay from a HIP graph;
+// else one launch per block step.  Skipped when skip (optional) is done.
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
-                     int* flags, int epoch, unsigned* tickets, unsigned base, int* status, int spin_limit,
-                     hipStream_t st, const RefineState* skip = nullptr);
+                     ChainCtl* ctl, int* flags, int* status, int spin_limit, hipStream_t st,
+                     const RefineState* skip = nullptr);
+// info[0] = info[1] = 0 (a kernel rather than a memset, so it replays from a
+// HIP graph)
+void launch_info_reset(int* info, hipStream_t st);
 // Y (n x nrhs) = beta Y + alpha M X, FP64, nrhs <= kMaxRhs, all row-major;
 // skipped when skip (optional) is done.
 void launch_matvec(const double2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,

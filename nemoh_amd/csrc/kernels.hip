@@ -718,6 +718,11 @@ void launch_refine_residual_max(const double2* r, size_t count, double* part, Re
   refine_check_kernel<<<1, 64, 0, st>>>(part, kParts, rs, tol, first);
 }
 void launch_refine_reset(RefineState* rs, hipStream_t st) { refine_reset_kernel<<<1, 1, 0, st>>>(rs); }
+__global__ void info_reset_kernel(int* __restrict__ info) {
+  info[0] = 0;
+  info[1] = 0;
+}
+void launch_info_reset(int* info, hipStream_t st) { info_reset_kernel<<<1, 1, 0, st>>>(info); }
 
 void launch_kochin(const Geometry& G, const double2* sig, int nsym, int nrhs, double k, double depth, double xeff,
                    double yeff, const double* theta, int ntheta, double2* H, hipStream_t st) {

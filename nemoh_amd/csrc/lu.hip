@@ -762,9 +762,10 @@ __device__ __forceinline__ void hop_acc_store(const typename MfmaTraits<T>::acc_
 // is bounded: after spin_limit polls (at once when it is 0) the workgroup
 // records a timeout in *status and stops waiting, so the kernel always ends.
 // R: right-hand sides (forward) or the forward solution y (backward); X gets
-// the solution; flags: nblk ints of this triangle, set to epoch when solved;
-// ticket: the triangle's counter, which holds base when the launch starts
-// (every launch takes exactly nblk tickets).
+// the solution; flags: nblk ints of this triangle, set to the solve's epoch
+// when solved; ctl: the tickets (ctl->ticket[tri] is 0 when the launch
+// starts) and the epoch, both set by chain_begin_kernel before the solve, so
+// that the launches take no per-call argument and replay from a HIP graph.
 //
 // The critical path is one hand-over per block (store x_b, its flag, the
 // next workgroup's poll and load) plus the product with the block just
@@ -776,19 +777,20 @@ __device__ __forceinline__ void hop_acc_store(const typename MfmaTraits<T>::acc_
 template <typename T, bool kLower>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict__ A, int ld, int n,
                                                          const C2<T>* __restrict__ P, const C2<T>* __restrict__ R,
-                                                         C2<T>* __restrict__ X, int nrhs, int* __restrict__ flags,
-                                                         int epoch, unsigned* __restrict__ ticket, unsigned base,
-                                                         int* __restrict__ status, int spin_limit,
-                                                         const RefineState* __restrict__ skip) {
+                                                         C2<T>* __restrict__ X, int nrhs, ChainCtl* __restrict__ ctl,
+                                                         int tri, int* __restrict__ flags, int* __restrict__ status,
+                                                         int spin_limit, const RefineState* __restrict__ skip) {
   typedef typename MfmaTraits<T>::acc_t acc_t;
-  __shared__ int timed_out, my_ticket;
+  __shared__ int timed_out, my_ticket, epoch_s;
   const int tid = threadIdx.x;
   const int nblk = (n + kNB - 1) / kNB;
   if (tid == 0) {
     timed_out = 0;
-    my_ticket = (int)(atomicAdd(ticket, 1u) - base);  // unsigned: exact across wrap-around
+    my_ticket = (int)atomicAdd(&ctl->ticket[tri], 1u);
+    epoch_s = ctl->epoch;
   }
   __syncthreads();
+  const int epoch = epoch_s;
   // a converged refinement: the ticket is taken (every launch takes nblk) and
   // nothing else is done; every workgroup reads the same state
   if (skip && skip->done) return;
@@ -886,6 +888,15 @@ __global__ __launch_bounds__(256) void block_apply_kernel(const C2<T>* __restric
   hop_acc_store<T, false>(cr, ci, X + (size_t)k0 * nrhs, nb, nrhs);
 }
 
+// Start of a chained solve: both triangles' tickets back to 0 and a new
+// epoch (never 0: flags are cleared to 0 when a handle is bound).
+__global__ void chain_begin_kernel(ChainCtl* __restrict__ ctl) {
+  ctl->ticket[0] = 0u;
+  ctl->ticket[1] = 0u;
+  const int e = ctl->epoch + 1;
+  ctl->epoch = e == 0 ? 1 : e;
+}
+
 template <typename T>
 __global__ void copy_c2_kernel(const C2<T>* __restrict__ a, C2<T>* __restrict__ b, size_t count,
                                const RefineState* __restrict__ skip) {
@@ -932,14 +943,14 @@ template void launch_lu<float>(C2<float>*, int, int, C2<float>*, int*, hipStream
 
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
-                     int* flags, int epoch, unsigned* tickets, unsigned base, int* status, int spin_limit,
-                     hipStream_t st, const RefineState* skip) {
+                     ChainCtl* ctl, int* flags, int* status, int spin_limit, hipStream_t st, const RefineState* skip) {
   const int nblk = (n + kNB - 1) / kNB;
   if (chained) {
+    chain_begin_kernel<<<1, 1, 0, st>>>(ctl);
     trsv_chain_kernel<T, true>
-        <<<nblk, 256, 0, st>>>(A, ld, n, P, B, W, nrhs, flags, epoch, tickets, base, status, spin_limit, skip);
-    trsv_chain_kernel<T, false><<<nblk, 256, 0, st>>>(A, ld, n, P, W, B, nrhs, flags + nblk, epoch, tickets + 1,
-                                                      base, status, spin_limit, skip);
+        <<<nblk, 256, 0, st>>>(A, ld, n, P, B, W, nrhs, ctl, 0, flags, status, spin_limit, skip);
+    trsv_chain_kernel<T, false>
+        <<<nblk, 256, 0, st>>>(A, ld, n, P, W, B, nrhs, ctl, 1, flags + nblk, status, spin_limit, skip);
     return;
   }
   for (int s = 0; s + 1 < nblk; ++s)
@@ -950,10 +961,9 @@ void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2
   copy_c2_kernel<T><<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(W, B, cnt, skip);
 }
 template void launch_lu_solve<double>(const C2<double>*, int, int, const C2<double>*, C2<double>*, C2<double>*, int,
-                                      bool, int*, int, unsigned*, unsigned, int*, int, hipStream_t,
-                                      const RefineState*);
+                                      bool, ChainCtl*, int*, int*, int, hipStream_t, const RefineState*);
 template void launch_lu_solve<float>(const C2<float>*, int, int, const C2<float>*, C2<float>*, C2<float>*, int, bool,
-                                     int*, int, unsigned*, unsigned, int*, int, hipStream_t, const RefineState*);
+                                     ChainCtl*, int*, int*, int, hipStream_t, const RefineState*);
 
 void launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t st) {
   mfma_probe_kernel<double><<<1, 64, 0, st>>>(a, b, d);

@@ -143,7 +143,7 @@ int bind(nemoh_server* s, Worker& w, Request& r) {
                           s->apd2x.data(), s->apd2z.data());
     if (e) {
       w.h = nullptr;
-      r.set_error(e, "could not create a solver handle");
+      r.set_error(e, std::string("could not create a solver handle: ") + nemoh_core_last_error(nullptr));
       return e;
     }
   } else {

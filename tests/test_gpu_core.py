@@ -326,3 +326,34 @@ def test_alternating_precisions_on_one_handle():
         err = _normwise(f, ref[:, j:j + 7])
         assert err < (FP64_TOL if prec == core.FP64 else MIXED_TOL), (it, prec, err)
     c.close()
+
+
+@pytest.mark.parametrize("prec", [core.FP64, core.MIXED])
+def test_graph_replays_equal_eager_launches(prec):
+    """The per-frequency pipeline replayed from HIP graphs (the default)
+    gives bit for bit the forces of the same launches made one by one: over
+    several frequencies in both regimes (finite depth, then the
+    infinite-depth frequency at the same depth), each graph replayed several
+    times with eager solves and host synchronisations in between, and a
+    factor-cache hit (the solve-only graph)."""
+    d, m, ref = _case("barge_fd")
+    om = np.unique(d["omega"])
+    eager = core.Core(d["vertices"], d["panels"], 0, float(d["depth"]), float(d["rho"]), float(d["g"]),
+                      tables=_tables())
+    eager.set_option(core.OPT_GRAPHS, 0)
+    graphs = core.Core(d["vertices"], d["panels"], 0, float(d["depth"]), float(d["rho"]), float(d["g"]),
+                       tables=_tables())
+    for c in (eager, graphs):
+        c.set_precision(prec)
+    for rep in range(3):
+        for k, w in enumerate(om):
+            cols = np.where(d["omega"] == w)[0]
+            nv = d["nvel"][:, cols]
+            _, fe = eager.solve_frequency(2 * np.pi / w, nv, nds=d["nds"], want_pressure=False)
+            _, fg = graphs.solve_frequency(2 * np.pi / w, nv, nds=d["nds"], want_pressure=False)
+            assert np.array_equal(fe, fg), (rep, k)
+            _, fh = graphs.solve_frequency(2 * np.pi / w, nv[:, ::-1].copy(), nds=d["nds"], want_pressure=False)
+            assert graphs.stats().cache_hit == 1 and np.array_equal(fh, fe[:, ::-1])
+            assert _normwise(fg, ref[:, cols]) < (FP64_TOL if prec == core.FP64 else MIXED_TOL)
+    eager.close()
+    graphs.close()
