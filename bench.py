@@ -35,11 +35,13 @@ sys.path.insert(0, str(ROOT))
 METRIC = "freq-solves/s at 4k panels, mixed vs FP64 (<=1e-6 rel err); serving p99 ms"
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense TFLOP/s (FP64 / FP32 matrix)
 HBM_PEAK_GBS = 8000.0
-# HBM bytes of lu_update_kernel<float> per launch over its algorithmic bytes
-# (16 m^2 + 1024 m), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
-# this workload, each scaled by the counter's reading for a 256 MiB copy
-# (profiles/lu_update_barge4000_hbm_traffic.md)
-LU_UPDATE_TRAFFIC_RATIO = 1.076
+# HBM bytes of lu_update128_kernel<float> per launch over its algorithmic
+# bytes (16 m^2 + 2048 m for an m x m trailing block and two 64-wide panels),
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload, each
+# scaled by the counter's reading for a 256 MiB copy: 58.36 MB fetched +
+# 41.00 MB written against 85.93 MB algorithmic on average over 6386 launches
+# (profiles/lu_update128_barge4000_hbm_traffic.md)
+LU_UPDATE_TRAFFIC_RATIO = 1.156
 
 
 def workload():
@@ -404,15 +406,20 @@ def main():
     common = sorted(set(g64) & set(gmx))
     err = max(float((np.abs(gmx[k] - g64[k]).max(0) / np.abs(g64[k]).max(0)).max()) for k in common)
     # roofline of the dominant MFMA kernel, the LU trailing update
-    # (lu_update_kernel<float> on the mixed path): algorithmic flops of its
-    # launches (8 * 64 * m^2 each, m = trailing order) over their summed
+    # (lu_update128_kernel<float> on the mixed path): algorithmic flops of its
+    # launches (8 * 128 * m^2 each, m = trailing order) over their summed
     # durations, timed with HIP events on the solver stream
     upd_ms = float(np.sum(phmx["update"]))
     upd_flops = float(np.sum(phmx["update_flops"]))
     upd_launches = int(np.sum(phmx["update_launches"]))
     achieved = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
-    trail = n - 64 * np.arange(1, (n + 63) // 64)  # trailing orders of the update launches
-    upd_bytes = float(np.mean(16.0 * trail ** 2 + 1024.0 * trail))
+    # trailing orders of the rank-128 update launches (block steps in pairs):
+    # each reads and writes its m x m complex64 trailing matrix and reads the
+    # m x 128 L and 128 x m A panels
+    nblk = (n + 63) // 64
+    trail = n - 64 * (np.arange(0, nblk, 2) + 2)
+    trail = trail[trail > 0]
+    upd_bytes = float(np.mean(16.0 * trail ** 2 + 2048.0 * trail))
     # LU + solve flops over the factor + solve phases (complex64 LU 8/3 N^3,
     # 2 triangular solves 8 N^2 NRHS per refinement sweep)
     lu_flops = 8.0 / 3.0 * n ** 3
@@ -441,7 +448,8 @@ def main():
             "fp64": {k: round(float(np.median(v)), 3) for k, v in ph64.items() if k in ("assembly", "factor", "solve")},
             "mixed": {k: round(float(np.median(v)), 3) for k, v in phmx.items() if k in ("assembly", "factor", "solve")},
             "mixed_refine_iterations_max": int(max(phmx["it"]) if phmx["it"] else 0)},
-        "roofline": {"bound": "mfma", "kernel": "lu_update_kernel<float> (LU trailing update, complex64 MFMA)",
+        "roofline": {"bound": "mfma", "kernel": "lu_update128_kernel<float> (rank-128 LU trailing update, complex64 "
+                                                 "MFMA)",
                      "measured_on": "single-stream leg (one frequency in flight)",
                      "achieved": round(achieved, 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["fp32"], 4),

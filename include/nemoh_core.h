@@ -194,8 +194,16 @@ int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter
  * on first use (one per precision, regime, problem count and solve form;
  * dropped when the handle's buffers or mesh change); 0 = launched one by one.
  * Timing (nemoh_core_enable_timing) always launches one by one. */
+/* NEMOH_OPT_LU_RANK: 128 (default) = block steps in pairs with rank-128
+ * trailing updates; 64 = one rank-64 trailing update per block step.
+ * NEMOH_OPT_LU_LOOKAHEAD (rank 128 only): 1 = each pair's trailing update
+ * split into the next pair's rows and columns and the bulk, which runs on a
+ * second stream of the handle beside the next pair's panels; 0 (default) =
+ * one launch per pair.  Not validated: the one run with it on (graphs
+ * captured over the two streams by 8 request-loop threads) ended in a host
+ * segmentation fault whose cause is not established. */
 enum { NEMOH_OPT_STEPWISE_SOLVE = 1, NEMOH_OPT_FP32_ASSEMBLY = 2, NEMOH_OPT_SOLVE_SPIN_LIMIT = 3,
-       NEMOH_OPT_GRAPHS = 4 };
+       NEMOH_OPT_GRAPHS = 4, NEMOH_OPT_LU_RANK = 5, NEMOH_OPT_LU_LOOKAHEAD = 6 };
 int nemoh_core_set_option(nemoh_core *ctx, int option, int value);
 
 /* Timings (ms) of the last solve's phases and refinement statistics. */
@@ -206,7 +214,8 @@ typedef struct {
   int cache_hit;
   /* trailing-update kernel of the last factorisation (timing on): summed
    * launch durations, launch count and algorithmic real flops
-   * (8 * 64 * m^2 per launch, m = order of the trailing matrix) */
+   * (8 * r * m^2 per launch, r = the update's rank, m = order of the
+   * trailing matrix) */
   double ms_lu_update;
   int lu_update_launches;
   double lu_update_flops;

@@ -37,6 +37,12 @@ struct nemoh_core {
   double zer = 0, g = 9.81, rho = 1025, depth = 0;
   double gm = 0;  // max horizontal vertex distance (theta-rule argument)
   hipStream_t st = nullptr;
+  hipStream_t st2 = nullptr;  // the factorisation's look-ahead stream (bulk trailing updates)
+  std::vector<hipEvent_t> lu_dep;  // st <-> st2 dependencies of one factorisation (2 per block pair)
+  // NEMOH_OPT_LU_LOOKAHEAD, off: the one run with it on (its two-stream
+  // launches captured into graphs by 8 request-loop threads, 256 bodies)
+  // ended in a host segmentation fault whose cause is not established
+  bool lookahead = false;
   // element capacity of every device buffer, keyed by the address of the
   // handle field that holds it: buffers only grow, so re-binding the handle
   // to a mesh no larger than before allocates (and frees) nothing
@@ -129,8 +135,10 @@ struct nemoh_core {
   nemoh_stats stats{};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<hipEvent_t> upd_ev;  // around each trailing-update launch (timing on)
+  std::vector<double> upd_flops;   // their algorithmic flops
+  int upd_launches = 0;            // how many upd_ev pairs the last factorisation stamped
+  int lu_rank = 128;               // NEMOH_OPT_LU_RANK: rank of the trailing updates (64 or 128)
   unsigned char* d_scratch = nullptr;  // this device's `scratch` (nemoh_core_kochin_noalloc)
-  int upd_blocks = 0;              // symmetry blocks whose updates upd_ev covers
   std::string err;
 };
 
@@ -214,6 +222,12 @@ bool c64_matrices(const nemoh_core* c) { return c->prec == NEMOH_MIXED && c->fp3
 
 int ensure_matrices(nemoh_core* c) {
   const size_t ninv = (size_t)((c->nfa + kNB - 1) / kNB) * kNB * kNB;
+  const size_t ndep = 2 * (size_t)((c->nfa + 2 * kNB - 1) / (2 * kNB));  // look-ahead events: 2 per block pair
+  while (c->lu_dep.size() < ndep) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->lu_dep.push_back(e);
+  }
   for (int b = 0; b < c->nb; ++b) {
     if (c64_matrices(c)) {
       const size_t n2 = (size_t)c->nfa * c->ld32;
@@ -357,7 +371,7 @@ int queue_factor(nemoh_core* c) {
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
   launch_info_reset(c->d_info, c->st);
   const int nblk = (c->nfa + kNB - 1) / kNB;
-  c->upd_blocks = 0;
+  c->upd_launches = 0;
   if (c->timing) {
     const size_t need = 2 * (size_t)nblk * c->nb;
     while (c->upd_ev.size() < need) {
@@ -365,16 +379,22 @@ int queue_factor(nemoh_core* c) {
       HIPCHK(c, hipEventCreate(&e));
       c->upd_ev.push_back(e);
     }
-    c->upd_blocks = c->nb;
+    c->upd_flops.resize(need);
   }
+  // dependency events of the look-ahead form (2 per block pair, created by
+  // ensure_matrices, outside any graph capture)
+  hipStream_t st2 = c->lookahead && c->lu_dep.size() >= 2 * (size_t)((nblk + 1) / 2) ? c->st2 : nullptr;
   for (int b = 0; b < c->nb; ++b) {
-    hipEvent_t* ue = c->timing ? c->upd_ev.data() + 2 * (size_t)nblk * b : nullptr;
+    hipEvent_t* ue = c->timing ? c->upd_ev.data() + 2 * (size_t)c->upd_launches : nullptr;
+    double* uf = c->timing ? c->upd_flops.data() + c->upd_launches : nullptr;
     if (c->prec == NEMOH_MIXED)
-      launch_lu<float>(reinterpret_cast<C2<float>*>(c->D32[b]), c->ld32, c->nfa,
-                       reinterpret_cast<C2<float>*>(c->Pinv32[b]), c->d_info, c->st, ue);
+      c->upd_launches += launch_lu<float>(reinterpret_cast<C2<float>*>(c->D32[b]), c->ld32, c->nfa,
+                                          reinterpret_cast<C2<float>*>(c->Pinv32[b]), c->d_info, c->st, c->lu_rank,
+                                          st2, c->lu_dep.data(), ue, uf);
     else
-      launch_lu<double>(reinterpret_cast<C2<double>*>(c->D[b]), c->nfa, c->nfa,
-                        reinterpret_cast<C2<double>*>(c->Pinv[b]), c->d_info, c->st, ue);
+      c->upd_launches += launch_lu<double>(reinterpret_cast<C2<double>*>(c->D[b]), c->nfa, c->nfa,
+                                           reinterpret_cast<C2<double>*>(c->Pinv[b]), c->d_info, c->st, c->lu_rank,
+                                           st2, c->lu_dep.data(), ue, uf);
   }
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
@@ -649,21 +669,15 @@ static int finish_call(nemoh_core* c) {
       c->stats.ms_assembly = a;
       c->stats.ms_factor = f;
       // trailing-update kernels: summed durations and algorithmic flops
-      const int n = c->nfa, nblk = (n + kNB - 1) / kNB;
       double ms = 0, flops = 0;
-      int cnt = 0;
-      for (int b = 0; b < c->upd_blocks; ++b)
-        for (int kb = 0; kb + 1 < nblk; ++kb) {
-          float e = 0;
-          const size_t base = 2 * ((size_t)nblk * b + kb);
-          HIPCHK(c, hipEventElapsedTime(&e, c->upd_ev[base], c->upd_ev[base + 1]));
-          const double m = n - (double)(kb + 1) * kNB;
-          ms += e;
-          flops += 8.0 * kNB * m * m;
-          ++cnt;
-        }
+      for (int t = 0; t < c->upd_launches; ++t) {
+        float e = 0;
+        HIPCHK(c, hipEventElapsedTime(&e, c->upd_ev[2 * (size_t)t], c->upd_ev[2 * (size_t)t + 1]));
+        ms += e;
+        flops += c->upd_flops[t];
+      }
       c->stats.ms_lu_update = ms;
-      c->stats.lu_update_launches = cnt;
+      c->stats.lu_update_launches = c->upd_launches;
       c->stats.lu_update_flops = flops;
       c->stats.lu_precision = c->factored_prec & 1;
     }
@@ -806,7 +820,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   c->dofs_set = false;  // body conditions belong to the previous mesh
   c->last_nrhs = 0;
   c->stats = nemoh_stats{};
-  c->upd_blocks = 0;
+  c->upd_launches = 0;
   const double* d = c->d_geo;
   c->G = Geometry{nfa, d, d + nfa, d + 2 * (size_t)nfa, d + 3 * (size_t)nfa, d + 4 * (size_t)nfa,
                   d + 5 * (size_t)nfa, d + 6 * (size_t)nfa, d + 7 * (size_t)nfa,
@@ -914,7 +928,9 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
     nemoh_core_destroy(c);
     return code;
   };
-  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return bail(NEMOH_ERR_HIP);
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess)
+    return bail(NEMOH_ERR_HIP);
   for (auto& e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return bail(NEMOH_ERR_HIP);
   std::vector<double> tab(4 * (size_t)kNR * kNZ);
@@ -1035,7 +1051,10 @@ void nemoh_core_destroy(nemoh_core* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->upd_ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->lu_dep) (void)hipEventDestroy(e);
   drop_graphs(c);
+  if (c->st2) (void)hipStreamSynchronize(c->st2);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -1061,6 +1080,22 @@ int nemoh_core_set_option(nemoh_core* c, int option, int value) {
   }
   if (option == NEMOH_OPT_FP32_ASSEMBLY) {
     c->fp32_assembly = value != 0;
+    return NEMOH_OK;
+  }
+  if (option == NEMOH_OPT_LU_RANK) {
+    if (value != 64 && value != 128) return fail(c, NEMOH_ERR_ARG, "LU update rank must be 64 or 128");
+    if (value != c->lu_rank) {
+      c->lu_rank = value;
+      c->w_prev = -1.0;   // the factors on the device came from the other form
+      ++c->layout_gen;    // and so did the graphs' launch sequences
+    }
+    return NEMOH_OK;
+  }
+  if (option == NEMOH_OPT_LU_LOOKAHEAD) {
+    if (c->lookahead != (value != 0)) {
+      c->lookahead = value != 0;
+      ++c->layout_gen;  // the graphs' launch sequences change
+    }
     return NEMOH_OK;
   }
   if (option == NEMOH_OPT_GRAPHS) {

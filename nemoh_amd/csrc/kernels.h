@@ -65,12 +65,20 @@ void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int f
 // afterwards the blocks below the diagonal hold L (unit block lower), the
 // blocks above hold U (unit block upper) and P (ceil(n/64) blocks of 64x64,
 // row-major) the inverses of the pivot blocks.  *info receives the first row
-// whose pivot has modulus <= 1e-20 (1-based), if any.  upd_ev (optional,
-// 2*ceil(n/64) events): the trailing-update launch of block step kb is made
-// with hipExtLaunchKernel, which stamps [2 kb] at its start and [2 kb + 1]
-// at its end.
+// whose pivot has modulus <= 1e-20 (1-based), if any.  rank 64: one rank-64
+// trailing update per block step; rank 128: block steps in pairs, a small
+// cross update of the pair's second block row and column, then one rank-128
+// update of the rest of the trailing matrix; with st2 (look-ahead) that update
+// is split between the next pair's block rows and columns on st and the
+// bulk on st2, which overlaps the next pair's panels (dep: 2 events per
+// pair, ceil(n/128) pairs; st waits for st2 before returning).  upd_ev
+// (optional, 2 per block step): each trailing-update launch (the rank-128
+// ones in the paired form) is made with hipExtLaunchKernel, which stamps
+// [2 t] at its start and [2 t + 1] at its end, and upd_flops[t] gets its
+// algorithmic flops; returns the number t of such launches.
 template <typename T>
-void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, hipEvent_t* upd_ev = nullptr);
+int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int rank = 128, hipStream_t st2 = nullptr,
+              hipEvent_t* dep = nullptr, hipEvent_t* upd_ev = nullptr, double* upd_flops = nullptr);
 // Control block of the chained triangular solves of one handle: the two
 // triangles' ticket counters and the current solve's epoch (both set on the
 // device at the start of every chained solve).

@@ -120,7 +120,7 @@ __device__ __forceinline__ void acc_store(const Acc64<T>& acc, C2<T>* __restrict
 // C given, the tile C (nr x nc at C, leading dimension ldc) is loaded while
 // the last chunk runs and added at the end: acc = C + (+-A) B, so the
 // tile's first MFMA waits only for the first operand chunk.
-template <typename T, bool kNegA, bool kFull>
+template <typename T, bool kNegA, bool kFull, int kChunks = 4>
 __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B, GemmLds<T>& S,
                                             const C2<T>* __restrict__ C, int ldc, int cnr, int cnc) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -182,12 +182,12 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
         for (int q = 0; q < 4; ++q) k1[a][b][q] = T(0);
   }
 #pragma unroll
-  for (int chunk = 0; chunk < 4; ++chunk) {
+  for (int chunk = 0; chunk < kChunks; ++chunk) {
     const int buf = chunk & 1;
-    if (chunk < 2) fetch(chunk + 2, chunk & 1);  // set chunk & 1 was stashed already
-    // in flight under the last chunk's MFMAs (issued with chunk 2 instead,
-    // the 12k-panel factorisation ran 4 % slower)
-    if (chunk == 3 && C) acc_load(cv, C, ldc, cnr, cnc);
+    if (chunk + 2 < kChunks) fetch(chunk + 2, chunk & 1);  // set chunk & 1 was stashed already
+    // in flight under the last chunk's MFMAs (issued with the chunk before
+    // instead, the 12k-panel factorisation ran 4 % slower)
+    if (chunk == kChunks - 1 && C) acc_load(cv, C, ldc, cnr, cnc);
 #pragma unroll
     for (int k = 0; k < 16; k += 4) {
       T xr[2], xi[2], yr[2], yi[2];
@@ -238,7 +238,7 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
           }
       }
     }
-    if (chunk < 3) {
+    if (chunk < kChunks - 1) {
       stash(buf ^ 1, (chunk + 1) & 1);
       __syncthreads();
     }
@@ -275,6 +275,17 @@ __device__ __forceinline__ void gemm64(Acc64<T>& acc, View<T> A, View<T> B, Gemm
     gemm64_impl<T, kNegA, true>(acc, A, B, S, C, ldc, cnr, cnc);
   else
     gemm64_impl<T, kNegA, false>(acc, A, B, S, C, ldc, cnr, cnc);
+}
+
+// The same with K = 128 (A: 64 x 128, B: 128 x 64): the rank-128 trailing
+// update of a pair of block steps.
+template <typename T, bool kNegA>
+__device__ __forceinline__ void gemm128(Acc64<T>& acc, View<T> A, View<T> B, GemmLds<T>& S, const C2<T>* __restrict__ C,
+                                        int ldc, int cnr, int cnc) {
+  if (A.nr == kNB && A.nc == 2 * kNB && B.nr == 2 * kNB && B.nc == kNB)
+    gemm64_impl<T, kNegA, true, 8>(acc, A, B, S, C, ldc, cnr, cnc);
+  else
+    gemm64_impl<T, kNegA, false, 8>(acc, A, B, S, C, ldc, cnr, cnc);
 }
 
 // ---------------------------------------------------------------------------
@@ -591,6 +602,30 @@ __device__ __forceinline__ void update_tile(int bid, int m, int& ti, int& tj) {
   ti = g * kBand + (w - tj * rows);
 }
 
+// Look-ahead of the updates: workgroup 0 inverts the diagonal block it has
+// just updated (acc, nr x nr at row r0) into P.
+template <typename T>
+__device__ __forceinline__ void invert_tile(UpdLds<T>& S, const Acc64<T>& acc, int nr, int r0, C2<T>* __restrict__ P,
+                                            int* __restrict__ info) {
+  __syncthreads();  // every wave is done with the GEMM staging area
+  {
+    const int lk = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int r, c;
+          acc_coord(a, b, MfmaTraits<T>::row(lk, q), r, c);
+          S.inv.m[r][c] = (r < nr && c < nr) ? C2<T>{acc.re[a][b][q], acc.im[a][b][q]} : C2<T>{T(r == c ? 1 : 0), 0};
+        }
+  }
+  __syncthreads();
+  lds_invert(S.inv, nr, r0, info);
+  store_inverse(S.inv, P, nr);
+}
+
 // Trailing update of step kb: A_ij -= L_i,kb A_kb,j for i, j > kb, one 64x64
 // tile per workgroup.  Workgroup 0 owns tile (kb+1, kb+1), the next diagonal
 // block, and inverts it after its update (P_{kb+1}).
@@ -611,24 +646,124 @@ __global__ __launch_bounds__(256) void lu_update_kernel(C2<T>* __restrict__ A, i
                   S.g, blk, ld, nr, nc);
   acc_store(acc, blk, ld, nr, nc);
   if (blockIdx.x != 0) return;
-  // look-ahead: invert the freshly updated diagonal block
-  __syncthreads();  // every wave is done with the GEMM staging area
-  {
-    const int lk = (threadIdx.x & 63) >> 4;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          int r, c;
-          acc_coord(a, b, MfmaTraits<T>::row(lk, q), r, c);
-          S.inv.m[r][c] = (r < nr && c < nr) ? C2<T>{acc.re[a][b][q], acc.im[a][b][q]} : C2<T>{T(r == c ? 1 : 0), 0};
-        }
+  invert_tile(S, acc, nr, r0, P + (size_t)(kb + 1) * kTile, info);
+}
+
+// Paired block steps (k, k+1), for the rank-128 form of the factorisation:
+//
+//   panel     L_ik = A_ik P_k (i > k), and the U rows of the previous pair
+//   cross     A_{k+1,j} -= L_{k+1,k} A_kj (j > k), A_{i,k+1} -= L_ik A_{k,k+1}
+//             (i > k+1): block row and column k+1 take step k; workgroup 0
+//             updates (k+1, k+1) and inverts it (P_{k+1})
+//   panel     L_{i,k+1} = A_{i,k+1} P_{k+1} (i > k+1)
+//   update128 A_ij -= [L_ik L_{i,k+1}] [A_kj; A_{k+1,j}] (i, j > k+1), K = 128:
+//             both steps' updates of the trailing matrix in one pass over it;
+//             workgroup 0 inverts (k+2, k+2) (P_{k+2})
+//
+// Rows k and k+1 keep A_kj and the cross-updated A_{k+1,j} until update128
+// has read them; the next pair's first panel turns them into U.  The
+// elimination is the same as two rank-64 steps (the two products of each
+// trailing element are summed in one accumulation instead of two), with half
+// the passes over the trailing matrix and twice the work per tile load.
+
+// Panel launch of the paired form: workgroups [0, nL) form L_{lc+1+w, lc} =
+// A P_lc; the next nU0 form U_{ur0, ur0+1+w} = P_ur0 A, the next nU1
+// U_{ur1, ur1+1+w} = P_ur1 A (in place).
+struct PanelJob {
+  int lc, nL, ur0, nU0, ur1, nU1;
+};
+template <typename T>
+__global__ __launch_bounds__(256) void lu_panel2_kernel(C2<T>* __restrict__ A, int ld, int n, PanelJob J,
+                                                        const C2<T>* __restrict__ P) {
+  __shared__ GemmLds<T> S;
+  Acc64<T> acc;
+  acc_zero(acc);
+  int w = blockIdx.x;
+  if (w < J.nL) {
+    const int r0 = (J.lc + 1 + w) * kNB, c0 = J.lc * kNB;
+    const int nr = min(kNB, n - r0), nk = min(kNB, n - c0);
+    C2<T>* blk = A + (size_t)r0 * ld + c0;
+    gemm64<T, false>(acc, View<T>{blk, ld, nr, nk}, View<T>{P + (size_t)J.lc * kTile, kNB, kNB, kNB}, S);
+    acc_store(acc, blk, ld, nr, nk);
+    return;
   }
-  __syncthreads();
-  lds_invert(S.inv, nr, r0, info);
-  store_inverse(S.inv, P + (size_t)(kb + 1) * kTile, nr);
+  w -= J.nL;
+  int ur = J.ur0;
+  if (w >= J.nU0) {
+    w -= J.nU0;
+    ur = J.ur1;
+  }
+  const int r0 = ur * kNB, c0 = (ur + 1 + w) * kNB;
+  const int nk = min(kNB, n - r0), nc = min(kNB, n - c0);
+  C2<T>* blk = A + (size_t)r0 * ld + c0;
+  gemm64<T, false>(acc, View<T>{P + (size_t)ur * kTile, kNB, kNB, kNB}, View<T>{blk, ld, nk, nc}, S);
+  acc_store(acc, blk, ld, nk, nc);
+}
+
+// Cross update of the pair (k, k+1): workgroup w < nr takes tile (k+1,
+// k+1+w) of block row k+1 (nr = blocks from k+1 on), the others tile (k+2+w',
+// k+1) of block column k+1; workgroup 0 (the diagonal tile) then inverts it.
+template <typename T>
+__global__ __launch_bounds__(256) void lu_cross_kernel(C2<T>* __restrict__ A, int ld, int n, int k,
+                                                       C2<T>* __restrict__ P, int* __restrict__ info) {
+  __shared__ UpdLds<T> S;
+  const int nblk = (n + kNB - 1) / kNB, s = k + 1, nrow = nblk - s;
+  const int w = blockIdx.x;
+  const int bi = w < nrow ? s : s + 1 + (w - nrow), bj = w < nrow ? s + w : s;
+  const int r0 = bi * kNB, c0 = bj * kNB, k0 = k * kNB;
+  const int nr = min(kNB, n - r0), nc = min(kNB, n - c0);
+  C2<T>* blk = A + (size_t)r0 * ld + c0;
+  Acc64<T> acc;
+  acc_zero(acc);
+  gemm64<T, true>(acc, View<T>{A + (size_t)r0 * ld + k0, ld, nr, kNB}, View<T>{A + (size_t)k0 * ld + c0, ld, kNB, nc},
+                  S.g, blk, ld, nr, nc);
+  acc_store(acc, blk, ld, nr, nc);
+  if (w != 0) return;
+  invert_tile(S, acc, nr, r0, P + (size_t)s * kTile, info);
+}
+
+// Rank-128 trailing update of the pair (k, k+1): A_ij -= L_{i,k:k+1}
+// A_{k:k+1,j} for i, j > k+1 (m trailing blocks from s = k+2), one 64x64 tile
+// per workgroup.  region 0: every trailing tile, in the XCD-banded order of
+// update_tile; region 1 (look-ahead): only the next pair's block rows s, s+1
+// (workgroups [0, 2m)) and block columns s, s+1 below them ([2m, 4m-4));
+// region 2: the rest (i, j >= s+2), banded.  Workgroup 0 of regions 0 and 1
+// owns (s, s) and inverts it (P_s).
+template <typename T>
+__global__ __launch_bounds__(256) void lu_update128_kernel(C2<T>* __restrict__ A, int ld, int n, int k,
+                                                           C2<T>* __restrict__ P, int* __restrict__ info,
+                                                           int region) {
+  __shared__ UpdLds<T> S;
+  const int s0 = (k + 2) * kNB, k0 = k * kNB;
+  const int m = (n - s0 + kNB - 1) / kNB;
+  int ti, tj;
+  const int w = blockIdx.x;
+  if (region == 1) {
+    if (w < 2 * m) {
+      ti = w / m;
+      tj = w - ti * m;
+    } else {
+      const int t = w - 2 * m;
+      ti = 2 + (t >> 1);
+      tj = t & 1;
+    }
+  } else if (region == 2) {
+    update_tile<16>(w, m - 2, ti, tj);
+    ti += 2;
+    tj += 2;
+  } else {
+    update_tile<16>(w, m, ti, tj);
+  }
+  const int r0 = s0 + ti * kNB, c0 = s0 + tj * kNB;
+  const int nr = min(kNB, n - r0), nc = min(kNB, n - c0);
+  C2<T>* blk = A + (size_t)r0 * ld + c0;
+  Acc64<T> acc;
+  acc_zero(acc);
+  gemm128<T, true>(acc, View<T>{A + (size_t)r0 * ld + k0, ld, nr, 2 * kNB},
+                   View<T>{A + (size_t)k0 * ld + c0, ld, 2 * kNB, nc}, S.g, blk, ld, nr, nc);
+  acc_store(acc, blk, ld, nr, nc);
+  if (w != 0 || region == 2) return;
+  invert_tile(S, acc, nr, r0, P + (size_t)(k + 2) * kTile, info);
 }
 
 // ---------------------------------------------------------------------------
@@ -922,24 +1057,88 @@ __global__ void mfma_probe_kernel(const T* __restrict__ a, const T* __restrict__
 
 
 template <typename T>
-void launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, hipEvent_t* upd_ev) {
+int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int rank, hipStream_t st2,
+              hipEvent_t* dep, hipEvent_t* upd_ev, double* upd_flops) {
   const int nblk = (n + kNB - 1) / kNB;
-  block_inverse_kernel<T><<<1, 256, 0, st>>>(A, ld, n, 0, P, info);
-  for (int kb = 0; kb < nblk; ++kb) {
-    const int ncol = nblk - kb - 1;
-    const int nrow = kb >= 1 ? nblk - kb : 0;
-    if (ncol + nrow > 0) lu_panel_kernel<T><<<ncol + nrow, 256, 0, st>>>(A, ld, n, kb, P, ncol);
-    if (ncol > 0) {
-      if (upd_ev)  // events stamped by the runtime at the kernel's start and end
-        hipExtLaunchKernelGGL(lu_update_kernel<T>, dim3(ncol * ncol), dim3(256), 0, st, upd_ev[2 * kb],
-                              upd_ev[2 * kb + 1], 0, A, ld, n, kb, P, info);
-      else
-        lu_update_kernel<T><<<ncol * ncol, 256, 0, st>>>(A, ld, n, kb, P, info);
+  int timed = 0;
+  // trailing-update launch: events around it when timing (stamped by the
+  // runtime at its start and end), and its algorithmic flops
+  auto update = [&](auto kernel, hipStream_t s, int grid, int kk, double flops, auto... extra) {
+    if (upd_ev) {
+      hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(256), 0, s, upd_ev[2 * timed], upd_ev[2 * timed + 1], 0, A, ld, n,
+                            kk, P, info, extra...);
+      upd_flops[timed++] = flops;
+    } else {
+      kernel<<<grid, 256, 0, s>>>(A, ld, n, kk, P, info, extra...);
     }
+  };
+  block_inverse_kernel<T><<<1, 256, 0, st>>>(A, ld, n, 0, P, info);
+  if (rank == 64) {
+    for (int kb = 0; kb < nblk; ++kb) {
+      const int ncol = nblk - kb - 1;
+      const int nrow = kb >= 1 ? nblk - kb : 0;
+      if (ncol + nrow > 0) lu_panel_kernel<T><<<ncol + nrow, 256, 0, st>>>(A, ld, n, kb, P, ncol);
+      if (ncol > 0) {
+        const double m = n - (kb + 1.0) * kNB;
+        update(lu_update_kernel<T>, st, ncol * ncol, kb, 8.0 * kNB * m * m);
+      }
+    }
+    return timed;
   }
+  // Paired steps, rank-128 trailing updates.  With st2 (look-ahead), the
+  // update of each pair is split: the next pair's block rows and columns
+  // (and its pivot inversion) on st, the bulk on st2, which then runs beside
+  // the next pair's panels and cross update; the bulk of pair k must be done
+  // before the look-ahead update of pair k+2 (it updates the same tiles
+  // first) and before rows k, k+1 are turned into U (it reads them), which
+  // the panel launch of pair k+4 does.  dep: one event per pair for st -> st2
+  // (its panels are done) and one for st2 -> st (its bulk is done).
+  const bool ahead = st2 != nullptr;
+  PanelJob pend[2] = {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}};  // U rows waiting for a bulk update to end
+  int pair = 0;
+  int bulk_done = -1;  // event of the latest bulk update (-1: none yet)
+  for (int k = 0; k < nblk; k += 2, ++pair) {
+    PanelJob J = pend[0];
+    J.lc = k;
+    J.nL = nblk - k - 1;
+    if (J.nL + J.nU0 + J.nU1 > 0) lu_panel2_kernel<T><<<J.nL + J.nU0 + J.nU1, 256, 0, st>>>(A, ld, n, J, P);
+    pend[0] = pend[1];
+    pend[1] = PanelJob{0, 0, 0, 0, 0, 0};
+    if (k + 1 >= nblk) break;  // odd count: the last block has no pair
+    const int nrow = nblk - (k + 1), m = nblk - (k + 2);
+    lu_cross_kernel<T><<<nrow + m, 256, 0, st>>>(A, ld, n, k, P, info);
+    if (m > 0) {
+      const PanelJob L1 = {k + 1, m, 0, 0, 0, 0};
+      lu_panel2_kernel<T><<<m, 256, 0, st>>>(A, ld, n, L1, P);
+      const double mr = n - (k + 2.0) * kNB;
+      if (ahead && m > 2) {
+        const double mb = mr - 2.0 * kNB;
+        (void)hipEventRecord(dep[2 * pair], st);  // this pair's panels are done
+        (void)hipStreamWaitEvent(st2, dep[2 * pair], 0);
+        update(lu_update128_kernel<T>, st2, (m - 2) * (m - 2), k, 8.0 * 2 * kNB * mb * mb, 2);
+        if (bulk_done >= 0) (void)hipStreamWaitEvent(st, dep[bulk_done], 0);  // the previous bulk is done
+        update(lu_update128_kernel<T>, st, 4 * m - 4, k, 8.0 * 2 * kNB * (mr * mr - mb * mb), 1);
+        bulk_done = 2 * pair + 1;
+        (void)hipEventRecord(dep[bulk_done], st2);  // this pair's bulk is done
+      } else {
+        if (bulk_done >= 0) (void)hipStreamWaitEvent(st, dep[bulk_done], 0);
+        update(lu_update128_kernel<T>, st, m * m, k, 8.0 * 2 * kNB * mr * mr, 0);
+        bulk_done = -1;  // on st itself: nothing to wait for
+      }
+    }
+    // rows k and k+1 become U once the bulk update that reads them is done:
+    // two pairs on without look-ahead ordering them, one on with it
+    pend[ahead ? 1 : 0] = PanelJob{0, 0, k, nblk - k - 1, k + 1, nblk - k - 2};
+  }
+  if (bulk_done >= 0) (void)hipStreamWaitEvent(st, dep[bulk_done], 0);  // the last bulk update
+  for (const PanelJob& q : pend)
+    if (q.nU0 + q.nU1 > 0) lu_panel2_kernel<T><<<q.nU0 + q.nU1, 256, 0, st>>>(A, ld, n, q, P);
+  return timed;
 }
-template void launch_lu<double>(C2<double>*, int, int, C2<double>*, int*, hipStream_t, hipEvent_t*);
-template void launch_lu<float>(C2<float>*, int, int, C2<float>*, int*, hipStream_t, hipEvent_t*);
+template int launch_lu<double>(C2<double>*, int, int, C2<double>*, int*, hipStream_t, int, hipStream_t, hipEvent_t*,
+                               hipEvent_t*, double*);
+template int launch_lu<float>(C2<float>*, int, int, C2<float>*, int*, hipStream_t, int, hipStream_t, hipEvent_t*,
+                              hipEvent_t*, double*);
 
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
