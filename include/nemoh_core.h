@@ -174,8 +174,8 @@ int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter
 /* Execution options (no counterpart in the reference).
  * NEMOH_OPT_STEPWISE_SOLVE: 1 = triangular solves as one launch per 64-row
  * block step (the path taken when the row blocks outnumber the compute
- * units), 0 = automatic (single flag-chained launch per triangle when the
- * row blocks fit). */
+ * units), 0 = automatic (a single chained launch per triangle, whose
+ * workgroups poll the solved blocks they need, when the row blocks fit). */
 /* NEMOH_OPT_FP32_ASSEMBLY (mixed precision only): 1 = influence matrices
  * with FP32 far-field Rankine terms, FP32 tabulated wave terms and
  * exponential images, and complex64 storage of D and S (FP64 kept for the
@@ -184,9 +184,9 @@ int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter
  * Its error against FP64 reaches 2e-6..9e-6 on the 4000-panel barge and
  * 7e-4 in shallow water (h = 20 m), beyond the 1e-6 the mixed path keeps by
  * default. */
-/* NEMOH_OPT_SOLVE_SPIN_LIMIT: polls a chained solve's workgroup makes
- * while waiting for an earlier row block before it gives up (default 4194304,
- * about a second); when any gives up, the solves of that call are redone
+/* NEMOH_OPT_SOLVE_SPIN_LIMIT: polls a chained solve's wave makes while
+ * waiting for an earlier row block before it gives up (default 1048576,
+ * one to two seconds); when any gives up, the solves of that call are redone
  * with one launch per block step (nemoh_stats.solve_fallbacks counts it).
  * 0 makes every wait give up at once, which forces that path (tests). */
 /* NEMOH_OPT_GRAPHS: 1 (default) = each frequency's assembly,

@@ -111,14 +111,13 @@ struct nemoh_core {
   double* d_red = nullptr;  // reduction scratch
   int* d_info = nullptr;     // [0] first bad pivot row, [1] solve wait timeout
   RefineState* d_rs = nullptr;  // mixed refinement state of each symmetry block
-  // the chained triangular solves' control block (ChainCtl: tickets and
-  // epoch, kept on the device), then their flags (forward, backward:
-  // ceil(n/64) each, shared by both precisions)
-  int* d_flags = nullptr;
+  // the chained triangular solves' control block (ChainCtl: the tickets,
+  // kept on the device)
+  int* d_chain = nullptr;
   int cu_count = 0;
   bool stepwise_solve = false;
   bool fp32_assembly = false;  // NEMOH_OPT_FP32_ASSEMBLY (mixed path only)
-  int spin_limit = 1 << 22;    // polls before a chained solve's wait gives up (about a second)
+  int spin_limit = 1 << 20;    // polls before a chained solve's wait gives up (one to two seconds)
   // HIP graphs of the per-frequency pipeline (NEMOH_OPT_GRAPHS), keyed by
   // what shapes the launch sequence; layout_gen moves on whenever a device
   // buffer is replaced or the handle is bound to another mesh, which
@@ -411,8 +410,8 @@ template <typename T>
 void lu_solve(nemoh_core* c, const C2<T>* A, int ld, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool stepwise,
               const RefineState* skip = nullptr) {
   const int n = c->nfa;
-  static_assert(sizeof(ChainCtl) == 4 * sizeof(int), "ChainCtl takes the first four ints of d_flags");
-  launch_lu_solve<T>(A, ld, n, P, B, W, nrhs, !stepwise, reinterpret_cast<ChainCtl*>(c->d_flags), c->d_flags + 4,
+  static_assert(sizeof(ChainCtl) == 4 * sizeof(int), "ChainCtl takes the four ints of d_chain");
+  launch_lu_solve<T>(A, ld, n, P, B, W, nrhs, !stepwise, reinterpret_cast<ChainCtl*>(c->d_chain),
                      c->d_info + 1, c->spin_limit, c->st, skip);
 }
 
@@ -768,7 +767,6 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->w_prev = -1.0;
   c->factored_prec = -1;
-  const int nblk = (nfa + kNB - 1) / kNB;
   const int ntile = (np + 255) / 256;
   const size_t nred = (size_t)ntile * (ntile + 1) / 2;
   // panels whose pairs take the free-surface branch of the wave part
@@ -779,7 +777,7 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
   int r = 0;
   if ((r = dalloc(c, &c->d_geo, 8 * (size_t)nfa)) || (r = dalloc(c, &c->d_vtx, 12 * (size_t)nfa)) ||
       (r = dalloc(c, &c->d_surf, surf.size())) ||
-      (r = dalloc(c, &c->d_flags, 2 * (size_t)nblk + 4)) || (r = dalloc(c, &c->d_pts, 2 * (size_t)np)) ||
+      (r = dalloc(c, &c->d_chain, 4)) || (r = dalloc(c, &c->d_pts, 2 * (size_t)np)) ||
       (r = dalloc(c, &c->d_ext, nred)))
     return r;
   for (int k = 0; k < 8; ++k)
@@ -789,9 +787,9 @@ int core_bind_mesh(nemoh_core* c, int np, int nfa, int nsym, const double* x, co
     HIPCHK(c, hipMemcpyAsync(c->d_surf, surf.data(), surf.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->d_pts, x, np * sizeof(double), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->d_pts + np, y, np * sizeof(double), hipMemcpyHostToDevice, c->st));
-  // the chained triangular solves' control block and flags, cleared over the
-  // whole buffer (solves compare the flags with an epoch that is never 0)
-  HIPCHK(c, hipMemsetAsync(c->d_flags, 0, c->caps[(const void*)&c->d_flags] * sizeof(int), c->st));
+  // the chained triangular solves' control block (every solve resets its
+  // tickets; cleared here so that no launch ever reads an unset counter)
+  HIPCHK(c, hipMemsetAsync(c->d_chain, 0, 4 * sizeof(int), c->st));
   // GM = max over vertex pairs of the horizontal distance (and to the y
   // images when nsym = 1), the argument of the theta rule
   launch_extent(c->d_pts, c->d_pts + np, np, nsym, c->d_ext, c->st);
@@ -863,7 +861,7 @@ size_t core_footprint_bytes(int np, int nfa, int nsym, int precision, int nrhs) 
   else b += nb * ninv * sizeof(double2);
   b += nt * k * sizeof(double2) + 3 * n * k * sizeof(double2) + 3 * 2 * n * k * sizeof(double2) +
        2 * n * k * sizeof(float2) + nb * 2 * n * k * sizeof(double2);  // right-hand-side workspace
-  b += 20 * n * sizeof(double) + (2 * nblk + 2) * sizeof(int) + 2 * (size_t)np * sizeof(double);  // geometry
+  b += 20 * n * sizeof(double) + 4 * sizeof(int) + 2 * (size_t)np * sizeof(double);  // geometry
   b += 7 * n * sizeof(double) + nt * 16 * sizeof(double) + nt * kMaxRhs * sizeof(double2);  // body conditions
   return b;
 }
@@ -1012,7 +1010,7 @@ void nemoh_core_destroy(nemoh_core* c) {
   release(c->d_red);
   release(c->d_info);
   release(c->d_rs);
-  release(c->d_flags);
+  release(c->d_chain);
   release(c->d_nds);
   release(c->d_force);
   release(c->d_pts);

@@ -80,25 +80,24 @@ template <typename T>
 int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int rank = 128, hipStream_t st2 = nullptr,
               hipEvent_t* dep = nullptr, hipEvent_t* upd_ev = nullptr, double* upd_flops = nullptr);
 // Control block of the chained triangular solves of one handle: the two
-// triangles' ticket counters and the current solve's epoch (both set on the
-// device at the start of every chained solve).
+// triangles' ticket counters (reset on the device at the start of every
+// chained solve).
 struct ChainCtl {
   unsigned ticket[2];
-  int epoch, pad;
+  int pad[2];
 };
 // Solves A x = b for nrhs <= kMaxRhs right-hand sides (B: n x nrhs
 // row-major) with the factors of launch_lu; on return B holds the solution,
 // W is scratch of the same size.  chained: one launch per triangle whose
-// workgroups take 64-row blocks in ticket order and pass solved blocks
-// through flags (flags: 2 ceil(n/64) ints, cleared to 0 when the handle is
-// bound and never reset; a kernel at the start of each solve resets ctl's
-// tickets and moves its epoch on; status set nonzero if a bounded wait timed
-// out), with no per-call argument, so the launches replay from a HIP graph;
-// else one launch per block step.  Skipped when skip (optional) is done.
+// workgroups take 64-row blocks in ticket order and poll the solved blocks
+// they need in W (forward) and B (backward), which hold an unsolved marker
+// until written (a kernel at the start of each solve resets ctl's tickets and
+// marks W; status set nonzero if a bounded wait timed out), with no per-call
+// argument, so the launches replay from a HIP graph; else one launch per
+// block step.  Skipped when skip (optional) is done.
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
-                     ChainCtl* ctl, int* flags, int* status, int spin_limit, hipStream_t st,
-                     const RefineState* skip = nullptr);
+                     ChainCtl* ctl, int* status, int spin_limit, hipStream_t st, const RefineState* skip = nullptr);
 // info[0] = info[1] = 0 (a kernel rather than a memset, so it replays from a
 // HIP graph)
 void launch_info_reset(int* info, hipStream_t st);

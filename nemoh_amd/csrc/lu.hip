@@ -778,36 +778,33 @@ struct HopTile {
   T pr[16], pi[16], npi[16];
 };
 
+// The loads below take a clamped address (element 0, always inside the
+// block, for an element outside nr x nc) and select zero afterwards, so that
+// they carry no branch: a load under a divergent branch is waited for inside
+// it, and the 16 loads of a lane would go out one at a time.
 template <typename T>
 __device__ __forceinline__ void hop_load_tile(HopTile<T>& h, const C2<T>* __restrict__ M, int ld, int nr, int nc,
                                               T sign) {
   const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;  // wave within its 64-row block
   const int r = 16 * wave + (lane & 15), lk = lane >> 4;
+  C2<T> v[16];
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
     const int c = 4 * s + lk;
-    const C2<T> v = (r < nr && c < nc) ? M[(size_t)r * ld + c] : C2<T>{0, 0};
-    h.pr[s] = sign * v.re;
-    h.pi[s] = sign * v.im;
-    h.npi[s] = -sign * v.im;
+    v[s] = M[(r < nr && c < nc) ? (size_t)r * ld + c : 0];
+  }
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const bool in = r < nr && 4 * s + lk < nc;
+    h.pr[s] = in ? sign * v[s].re : T(0);
+    h.pi[s] = in ? sign * v[s].im : T(0);
+    h.npi[s] = in ? -sign * v[s].im : T(0);
   }
 }
 
-// Loads and stores of solved blocks that other workgroups of the same launch
-// read: system-scope relaxed atomics (coherent across the XCDs' L2 caches
-// without a cache-wide write-back or invalidation).
-__device__ __forceinline__ C2<double> ld_coherent(const C2<double>* p) {
-  const double* q = reinterpret_cast<const double*>(p);
-  return {__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
-          __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)};
-}
-__device__ __forceinline__ C2<float> ld_coherent(const C2<float>* p) {
-  const unsigned long long u =
-      __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  C2<float> v;
-  __builtin_memcpy(&v, &u, sizeof v);
-  return v;
-}
+// Stores of solved blocks that other workgroups of the same launch read:
+// system-scope relaxed atomics (coherent across the XCDs' L2 caches without a
+// cache-wide write-back or invalidation), one per 64-bit word.
 __device__ __forceinline__ void st_coherent(C2<double>* p, C2<double> v) {
   double* q = reinterpret_cast<double*>(p);
   __hip_atomic_store(q, v.re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -819,17 +816,120 @@ __device__ __forceinline__ void st_coherent(C2<float>* p, C2<float> v) {
   __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// An element of a chained solve's x that is not solved yet holds a marker: a
+// signalling-NaN bit pattern in every 64-bit word (both halves of a complex64
+// word).  Arithmetic never yields a signalling NaN, so a solved element never
+// equals it, and since each word is stored and loaded whole (64-bit atomics)
+// a reader sees either the marker or the final value.  Readers therefore poll
+// the data itself: no flag, no wait for the stores to complete before one.
+constexpr unsigned long long kUnsolved32 = 0x7FA5A5A57FA5A5A5ull;
+constexpr unsigned long long kUnsolved64 = 0x7FF5A5A5A5A5A5A5ull;
+
+__device__ __forceinline__ void st_unsolved(C2<float>* p) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), kUnsolved32, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_unsolved(C2<double>* p) {
+  unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+  __hip_atomic_store(q, kUnsolved64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(q + 1, kUnsolved64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+// whole-word loads of elements that other workgroups store (see st_coherent)
+__device__ __forceinline__ C2<float> ld_coherent(const C2<float>* p) {
+  const unsigned long long u =
+      __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  C2<float> v;
+  __builtin_memcpy(&v, &u, sizeof v);
+  return v;
+}
+__device__ __forceinline__ C2<double> ld_coherent(const C2<double>* p) {
+  const double* q = reinterpret_cast<const double*>(p);
+  return {__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+          __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)};
+}
+// false while v is (partly) the marker
+__device__ __forceinline__ bool is_solved(C2<float> v) {
+  unsigned long long u;
+  __builtin_memcpy(&u, &v, sizeof u);
+  return u != kUnsolved32;
+}
+__device__ __forceinline__ bool is_solved(C2<double> v) {
+  unsigned long long a, b;
+  __builtin_memcpy(&a, &v.re, sizeof a);
+  __builtin_memcpy(&b, &v.im, sizeof b);
+  return a != kUnsolved64 && b != kUnsolved64;
+}
+
 // x operand of one block step: lane (li, lk) holds x[4 s + lk][li], s < 16.
-template <typename T, bool kCoherent>
+template <typename T>
 __device__ __forceinline__ void hop_load_x(T xr[16], T xi[16], const C2<T>* __restrict__ X, int nj, int nrhs) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  C2<T> v[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int r = 4 * s + lk;
+    v[s] = X[(r < nj && li < nrhs) ? (size_t)r * nrhs + li : 0];
+  }
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const bool in = 4 * s + lk < nj && li < nrhs;
+    xr[s] = in ? v[s].re : T(0);
+    xi[s] = in ? v[s].im : T(0);
+  }
+}
+
+// The chained solve's operands are loaded ahead of their use in two steps:
+// hop_issue_* sends the loads (clamped addresses, no branch), and the
+// operand is formed after the loads land.  Tile: lane (li, lk) of wave w
+// holds M[16 w + li][4 s + lk]; elements outside nr x nc hold M[0][0], which
+// only ever meets a zero x element (columns) or fills accumulator rows that
+// are not stored (rows).
+template <typename T>
+__device__ __forceinline__ void hop_issue_tile(C2<T> t[16], const C2<T>* __restrict__ M, int ld, int nr, int nc) {
+  const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;
+  const int r = 16 * wave + (lane & 15), lk = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int c = 4 * s + lk;
+    t[s] = M[(r < nr && c < nc) ? (size_t)r * ld + c : 0];
+  }
+}
+// x_j as other workgroups store it (lane (li, lk): x[4 s + lk][li])
+template <typename T>
+__device__ __forceinline__ void hop_issue_x(C2<T> v[16], const C2<T>* __restrict__ X, int nj, int nrhs) {
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
     const int r = 4 * s + lk;
-    C2<T> v = {0, 0};
-    if (r < nj && li < nrhs) v = kCoherent ? ld_coherent(X + (size_t)r * nrhs + li) : X[(size_t)r * nrhs + li];
-    xr[s] = v.re;
-    xi[s] = v.im;
+    v[s] = ld_coherent(X + ((r < nj && li < nrhs) ? (size_t)r * nrhs + li : 0));
+  }
+}
+// x operand from hop_issue_x's values (zero outside nj x nrhs), and nxi =
+// -xi; true when every element this lane holds is solved
+template <typename T>
+__device__ __forceinline__ bool hop_x_operand(T xr[16], T xi[16], T nxi[16], const C2<T> v[16], int nj, int nrhs) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  bool ok = true;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const bool in = 4 * s + lk < nj && li < nrhs;
+    ok = ok && (!in || is_solved(v[s]));
+    xr[s] = in ? v[s].re : T(0);
+    xi[s] = in ? v[s].im : T(0);
+    nxi[s] = -xi[s];
+  }
+  return ok;
+}
+// (cr, ci) += M x with M as hop_issue_tile loaded it
+template <typename T>
+__device__ __forceinline__ void hop_mma_raw(typename MfmaTraits<T>::acc_t& cr, typename MfmaTraits<T>::acc_t& ci,
+                                            const C2<T> m[16], const T xr[16], const T xi[16], const T nxi[16]) {
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    cr = MfmaTraits<T>::mma(m[s].re, xr[s], cr);
+    ci = MfmaTraits<T>::mma(m[s].re, xi[s], ci);
+    cr = MfmaTraits<T>::mma(m[s].im, nxi[s], cr);
+    ci = MfmaTraits<T>::mma(m[s].im, xr[s], ci);
   }
 }
 
@@ -856,13 +956,17 @@ template <typename T>
 __device__ __forceinline__ void hop_acc_load(typename MfmaTraits<T>::acc_t& cr, typename MfmaTraits<T>::acc_t& ci,
                                              const C2<T>* __restrict__ R, int nb, int nrhs) {
   const int li = threadIdx.x & 15;
+  C2<T> v[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int r = hop_row<T>(q);
-    C2<T> v = {0, 0};
-    if (r < nb && li < nrhs) v = R[(size_t)r * nrhs + li];
-    cr[q] = v.re;
-    ci[q] = v.im;
+    v[q] = R[(r < nb && li < nrhs) ? (size_t)r * nrhs + li : 0];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const bool in = hop_row<T>(q) < nb && li < nrhs;
+    cr[q] = in ? v[q].re : T(0);
+    ci[q] = in ? v[q].im : T(0);
   }
 }
 
@@ -884,51 +988,60 @@ __device__ __forceinline__ void hop_acc_store(const typename MfmaTraits<T>::acc_
   }
 }
 
+// the elements of a 64-row block that hop_acc_store writes, set to the marker
+template <typename T>
+__device__ __forceinline__ void hop_mark_unsolved(C2<T>* __restrict__ X, int nb, int nrhs) {
+  const int li = threadIdx.x & 15;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = hop_row<T>(q);
+    if (r < nb && li < nrhs) st_unsolved(X + (size_t)r * nrhs + li);
+  }
+}
+
 // Single-launch triangular solve.  Each workgroup takes the next ticket w of
 // the launch from a counter (atomic add) and owns row block b (forward:
 // b = w, backward: b = nblk-1-w, so blocks without dependencies come first);
 // for every solved block j, in dependency order, it loads the coupling block
-// (which does not depend on x_j), waits for block j's flag, loads x_j and
-// subtracts M_bj x_j, then publishes x_b.  A workgroup only ever waits for
-// blocks whose tickets were taken before its own, i.e. by workgroups that
-// are already running, so the chain makes progress however few of its
-// workgroups are resident (other streams' kernels and other chains may hold
-// the rest of the GPU): the lowest unsolved block never waits.  Every wait
-// is bounded: after spin_limit polls (at once when it is 0) the workgroup
-// records a timeout in *status and stops waiting, so the kernel always ends.
+// (which does not depend on x_j), polls x_j until no element holds the
+// unsolved marker and subtracts M_bj x_j, then stores x_b.  Each wave polls
+// and proceeds on its own.  A workgroup only ever waits for blocks whose
+// tickets were taken before its own, i.e. by workgroups that are already
+// running, so the chain makes progress however few of its workgroups are
+// resident (other streams' kernels and other chains may hold the rest of the
+// GPU): the lowest unsolved block never waits.  Every wait is bounded: after
+// spin_limit polls (at once when it is 0) the wave records a timeout in
+// *status and stops waiting, so the kernel always ends (its result is then
+// discarded by the caller).
 // R: right-hand sides (forward) or the forward solution y (backward); X gets
-// the solution; flags: nblk ints of this triangle, set to the solve's epoch
-// when solved; ctl: the tickets (ctl->ticket[tri] is 0 when the launch
-// starts) and the epoch, both set by chain_begin_kernel before the solve, so
-// that the launches take no per-call argument and replay from a HIP graph.
+// the solution and holds the marker in every element when the launch starts.
+// The forward launch also marks its R (the backward launch's X) block by
+// block once it has read it.  ctl->ticket[tri] is 0 when the launch starts
+// (set, with X's marker, by chain_begin_kernel for the forward triangle), so
+// the launches take no per-call argument and replay from a HIP graph.
 //
-// The critical path is one hand-over per block (store x_b, its flag, the
-// next workgroup's poll and load) plus the product with the block just
-// solved.  Superblocks of two blocks per workgroup (128 rows, 8 waves, the
-// inner coupling through LDS) halve the hand-overs but make the product on
-// the critical path four times larger, 128 x 128 against 64 x 64, i.e. 3.4
+// The critical path is one hand-over per block (x_b's stores reaching memory
+// and the next workgroup's poll seeing them) plus the product with the block
+// just solved.  Superblocks of two blocks per workgroup (128 rows, 8 waves,
+// the inner coupling through LDS) halve the hand-overs but make the product
+// on the critical path four times larger, 128 x 128 against 64 x 64, i.e. 3.4
 // against 0.85 us of one CU's FP32 MFMA peak: measured 480 us per triangle
-// at 4000 panels against 307 us for this form.
+// at 4000 panels against 307 us for the 64-row form (both with flags).
 template <typename T, bool kLower>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict__ A, int ld, int n,
-                                                         const C2<T>* __restrict__ P, const C2<T>* __restrict__ R,
+                                                         const C2<T>* __restrict__ P, C2<T>* __restrict__ R,
                                                          C2<T>* __restrict__ X, int nrhs, ChainCtl* __restrict__ ctl,
-                                                         int tri, int* __restrict__ flags, int* __restrict__ status,
-                                                         int spin_limit, const RefineState* __restrict__ skip) {
+                                                         int tri, int* __restrict__ status, int spin_limit,
+                                                         const RefineState* __restrict__ skip) {
   typedef typename MfmaTraits<T>::acc_t acc_t;
-  __shared__ int timed_out, my_ticket, epoch_s;
-  const int tid = threadIdx.x;
+  __shared__ int my_ticket;
+  const int tid = threadIdx.x, lane = tid & 63;
   const int nblk = (n + kNB - 1) / kNB;
-  if (tid == 0) {
-    timed_out = 0;
-    my_ticket = (int)atomicAdd(&ctl->ticket[tri], 1u);
-    epoch_s = ctl->epoch;
-  }
-  __syncthreads();
-  const int epoch = epoch_s;
-  // a converged refinement: the ticket is taken (every launch takes nblk) and
-  // nothing else is done; every workgroup reads the same state
+  // a converged refinement: nothing is done (every workgroup reads the same
+  // state, and no ticket is taken)
   if (skip && skip->done) return;
+  if (tid == 0) my_ticket = (int)atomicAdd(&ctl->ticket[tri], 1u);
+  __syncthreads();
   const int w = my_ticket;
   if (w < 0 || w >= nblk) {  // a counter out of step with the launches: never wait on it
     if (tid == 0) atomicMax(status, 1);
@@ -936,50 +1049,86 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict
   }
   const int b = kLower ? w : nblk - 1 - w;
   const int k0 = b * kNB, nb = min(kNB, n - k0);
+  // (cr, ci) = -(the block's running value): -r_b (forward) or -P_b y_b
+  // (backward), then + M_bj x_j for every dependency j
   acc_t cr, ci;
-  HopTile<T> h;
-  T xr[16], xi[16];
+  T xr[16], xi[16], nxi[16];
   if (kLower) {
     hop_acc_load<T>(cr, ci, R + (size_t)k0 * nrhs, nb, nrhs);
-  } else {  // P_b y_b
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cr[q] = -cr[q];
+      ci[q] = -ci[q];
+    }
+  } else {  // P_b (-y_b)
 #pragma unroll
     for (int q = 0; q < 4; ++q) cr[q] = ci[q] = T(0);
-    hop_load_tile(h, P + (size_t)b * kTile, kNB, kNB, kNB, T(1));
-    hop_load_x<T, false>(xr, xi, R + (size_t)k0 * nrhs, nb, nrhs);
-    hop_mma<T>(cr, ci, h, xr, xi);
+    C2<T> m[16];
+    hop_issue_tile(m, P + (size_t)b * kTile, kNB, kNB, kNB);
+    hop_load_x<T>(xr, xi, R + (size_t)k0 * nrhs, nb, nrhs);
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      nxi[s] = xi[s];
+      xr[s] = -xr[s];
+      xi[s] = -xi[s];
+    }
+    hop_mma_raw<T>(cr, ci, m, xr, xi, nxi);
   }
+  // wave-uniform: a wave that gave up waiting (at once when spin_limit is 0,
+  // which forces the stepwise fallback) does no further waits
+  bool gave_up = spin_limit == 0;
+  if (gave_up && lane == 0) atomicMax(status, 1);
   const int ndep = kLower ? b : nblk - 1 - b;
-  for (int d = 0; d < ndep; ++d) {
+  auto dep_rows = [&](int d, int& j0, int& nj) {
     const int j = kLower ? d : nblk - 1 - d;
-    const int j0 = j * kNB, nj = min(kNB, n - j0);
-    // the coupling block does not depend on x_j: load it before waiting
-    hop_load_tile(h, A + (size_t)k0 * ld + j0, ld, nb, nj, T(-1));
-    if (tid == 0 && !timed_out && spin_limit == 0) {  // every wait gives up at once (forced fallback)
-      timed_out = 1;
-      atomicMax(status, 1);
-    }
-    if (tid == 0 && !timed_out) {
-      int spins = 0;
-      while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != epoch) {
-        if (++spins > spin_limit) {
-          timed_out = 1;
-          atomicMax(status, 1);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
+    j0 = j * kNB;
+    nj = min(kNB, n - j0);
+  };
+  // the coupling block and x of dependency d + 1 are loaded while the
+  // product of dependency d runs
+  C2<T> t[16], xv[16];
+  if (ndep > 0) {
+    int j0, nj;
+    dep_rows(0, j0, nj);
+    hop_issue_tile(t, A + (size_t)k0 * ld + j0, ld, nb, nj);
+    hop_issue_x(xv, X + (size_t)j0 * nrhs, nj, nrhs);
+  }
+  for (int d = 0; d < ndep; ++d) {
+    int j0, nj;
+    dep_rows(d, j0, nj);
+    bool ok = hop_x_operand(xr, xi, nxi, xv, nj, nrhs);
+    int spins = 0;
+    while (!__all(ok) && !gave_up) {
+      if (++spins > spin_limit) {
+        gave_up = true;
+        if (lane == 0) atomicMax(status, 1);
+        break;
       }
+      __builtin_amdgcn_s_sleep(1);
+      hop_issue_x(xv, X + (size_t)j0 * nrhs, nj, nrhs);
+      ok = hop_x_operand(xr, xi, nxi, xv, nj, nrhs);
     }
-    __syncthreads();
-    hop_load_x<T, true>(xr, xi, X + (size_t)j0 * nrhs, nj, nrhs);
-    hop_mma<T>(cr, ci, h, xr, xi);
+    C2<T> m[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) m[s] = t[s];
+    if (d + 1 < ndep) {
+      int j1, n1;
+      dep_rows(d + 1, j1, n1);
+      hop_issue_tile(t, A + (size_t)k0 * ld + j1, ld, nb, n1);
+      hop_issue_x(xv, X + (size_t)j1 * nrhs, n1, nrhs);
+    }
+    hop_mma_raw<T>(cr, ci, m, xr, xi, nxi);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    cr[q] = -cr[q];
+    ci[q] = -ci[q];
   }
   hop_acc_store<T, true>(cr, ci, X + (size_t)k0 * nrhs, nb, nrhs);
-  // every store of x_b is performed before the flag is raised
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0);
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __syncthreads();
-  if (tid == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  // R_b was read into the accumulators above (every MFMA and the store of x_b
+  // waited for those loads): mark it for the backward launch, which solves
+  // into it
+  if (kLower) hop_mark_unsolved<T>(R + (size_t)k0 * nrhs, nb, nrhs);
 }
 
 // Stepwise fallback (one launch per block step s, for matrices with more row
@@ -999,7 +1148,7 @@ __global__ __launch_bounds__(256) void trsv_step_kernel(const C2<T>* __restrict_
   T xr[16], xi[16];
   hop_acc_load<T>(cr, ci, R + (size_t)i0 * nrhs, ni, nrhs);
   hop_load_tile(h, A + (size_t)i0 * ld + s0, ld, ni, ns, T(-1));
-  hop_load_x<T, false>(xr, xi, R + (size_t)s0 * nrhs, ns, nrhs);
+  hop_load_x<T>(xr, xi, R + (size_t)s0 * nrhs, ns, nrhs);
   hop_mma<T>(cr, ci, h, xr, xi);
   hop_acc_store<T, false>(cr, ci, R + (size_t)i0 * nrhs, ni, nrhs);
 }
@@ -1018,18 +1167,24 @@ __global__ __launch_bounds__(256) void block_apply_kernel(const C2<T>* __restric
   HopTile<T> h;
   T xr[16], xi[16];
   hop_load_tile(h, P + (size_t)b * kTile, kNB, kNB, kNB, T(1));
-  hop_load_x<T, false>(xr, xi, Y + (size_t)k0 * nrhs, nb, nrhs);
+  hop_load_x<T>(xr, xi, Y + (size_t)k0 * nrhs, nb, nrhs);
   hop_mma<T>(cr, ci, h, xr, xi);
   hop_acc_store<T, false>(cr, ci, X + (size_t)k0 * nrhs, nb, nrhs);
 }
 
-// Start of a chained solve: both triangles' tickets back to 0 and a new
-// epoch (never 0: flags are cleared to 0 when a handle is bound).
-__global__ void chain_begin_kernel(ChainCtl* __restrict__ ctl) {
-  ctl->ticket[0] = 0u;
-  ctl->ticket[1] = 0u;
-  const int e = ctl->epoch + 1;
-  ctl->epoch = e == 0 ? 1 : e;
+// Start of a chained solve: both triangles' tickets back to 0, and every
+// element of the forward launch's X (count elements) set to the unsolved
+// marker.
+template <typename T>
+__global__ void chain_begin_kernel(ChainCtl* __restrict__ ctl, C2<T>* __restrict__ X, size_t count,
+                                   const RefineState* __restrict__ skip) {
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e == 0) {
+    ctl->ticket[0] = 0u;
+    ctl->ticket[1] = 0u;
+  }
+  if (skip && skip->done) return;
+  if (e < count) st_unsolved(X + e);
 }
 
 template <typename T>
@@ -1142,27 +1297,25 @@ template int launch_lu<float>(C2<float>*, int, int, C2<float>*, int*, hipStream_
 
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
-                     ChainCtl* ctl, int* flags, int* status, int spin_limit, hipStream_t st, const RefineState* skip) {
+                     ChainCtl* ctl, int* status, int spin_limit, hipStream_t st, const RefineState* skip) {
   const int nblk = (n + kNB - 1) / kNB;
+  const size_t cnt = (size_t)n * nrhs;
   if (chained) {
-    chain_begin_kernel<<<1, 1, 0, st>>>(ctl);
-    trsv_chain_kernel<T, true>
-        <<<nblk, 256, 0, st>>>(A, ld, n, P, B, W, nrhs, ctl, 0, flags, status, spin_limit, skip);
-    trsv_chain_kernel<T, false>
-        <<<nblk, 256, 0, st>>>(A, ld, n, P, W, B, nrhs, ctl, 1, flags + nblk, status, spin_limit, skip);
+    chain_begin_kernel<T><<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(ctl, W, cnt, skip);
+    trsv_chain_kernel<T, true><<<nblk, 256, 0, st>>>(A, ld, n, P, B, W, nrhs, ctl, 0, status, spin_limit, skip);
+    trsv_chain_kernel<T, false><<<nblk, 256, 0, st>>>(A, ld, n, P, W, B, nrhs, ctl, 1, status, spin_limit, skip);
     return;
   }
   for (int s = 0; s + 1 < nblk; ++s)
     trsv_step_kernel<T, true><<<nblk - 1 - s, 256, 0, st>>>(A, ld, n, B, nrhs, s, skip);
   block_apply_kernel<T><<<nblk, 256, 0, st>>>(P, n, B, W, nrhs, skip);
   for (int s = nblk - 1; s >= 1; --s) trsv_step_kernel<T, false><<<s, 256, 0, st>>>(A, ld, n, W, nrhs, s, skip);
-  const size_t cnt = (size_t)n * nrhs;
   copy_c2_kernel<T><<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(W, B, cnt, skip);
 }
 template void launch_lu_solve<double>(const C2<double>*, int, int, const C2<double>*, C2<double>*, C2<double>*, int,
-                                      bool, ChainCtl*, int*, int*, int, hipStream_t, const RefineState*);
+                                      bool, ChainCtl*, int*, int, hipStream_t, const RefineState*);
 template void launch_lu_solve<float>(const C2<float>*, int, int, const C2<float>*, C2<float>*, C2<float>*, int, bool,
-                                     ChainCtl*, int*, int*, int, hipStream_t, const RefineState*);
+                                     ChainCtl*, int*, int, hipStream_t, const RefineState*);
 
 void launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t st) {
   mfma_probe_kernel<double><<<1, 64, 0, st>>>(a, b, d);

@@ -312,8 +312,8 @@ def test_nonfinite_rhs_is_reported_on_the_mixed_path():
 def test_alternating_precisions_on_one_handle():
     """FP64 and mixed solves alternating on one handle (the request loop's
     workers switch precision between items): the chained solves of the two
-    precisions share the handle's ticket counters and flags, and every solve
-    still matches the reference."""
+    precisions share the handle's ticket counters, and every solve still
+    matches the reference."""
     d, m, ref = _case("cyl512")
     c = core.Core(d["vertices"], d["panels"], int(d["isym"]), float(d["depth"]), float(d["rho"]), float(d["g"]),
                   tables=_tables())
