@@ -191,44 +191,64 @@ def run_concurrent(m, omegas, prec, passes, warmup, workers):
 REF_HARNESS = ROOT / "oracle" / "_ref" / "harness"
 
 
-def cpu_baseline_reference(m, omegas, nrhs=9, procs=8, stride=4, m_gauss=800):
+def cpu_baseline_reference(m, omegas, nrhs=9, procs=8, stride=4, m_small=800, m_large=1600):
     """The reference Solver's own routines (FP64 build of the unmodified
     sources, oracle/_ref) on the host cores: `procs` single-threaded processes,
     one per sampled frequency (the reference is single-threaded and
     frequencies are independent).  Each times VAVFD+VNSFD (or the INFD pair)
     for every `stride`-th field row against all panels, and GAUSSZ on an
-    m_gauss x 2 m_gauss system; one frequency is then
-    N^2 (1 + NRHS) pairs (assembly + phi per problem, SOLVE_BEM_FD_DIRECT.f90:
-    103-183) + GAUSSZ scaled by (N / m_gauss)^3."""
+    m_large x 2 m_large system; then the same processes time GAUSSZ on
+    m_small x 2 m_small.  One frequency is N^2 (1 + NRHS) pairs (assembly +
+    phi per problem, SOLVE_BEM_FD_DIRECT.f90:103-183) + GAUSSZ at N,
+    extrapolated from m_large with the power of n that the two sizes show
+    (GAUSSZ leaves the caches as n grows, so its time grows faster than n^3:
+    the n^3 scaling of an in-cache size underestimates it)."""
     import subprocess
+    import tempfile
     n = m.npanels
     head = f"mesh\n{m.npoints} {m.npanels} {m.isym} {50.0:.17e}\n"
     head += "".join(f"{x:.17e} {y:.17e} {z:.17e}\n" for x, y, z in m.vertices)
     head += "".join(f"{a} {b} {c} {d}\n" for a, b, c, d in m.panels + 1)
     rows = (n + stride - 1) // stride
     idx = np.linspace(0, omegas.size - 1, procs).round().astype(int)
+
+    def batch(nrows, mg):
+        # each process writes to a file of its own: the harness echoes more
+        # than a pipe holds, and a full pipe would make the processes wait
+        # for each other
+        outs = [tempfile.TemporaryFile(mode="w+") for _ in idx]
+        ps = [subprocess.Popen([str(REF_HARNESS)], stdin=subprocess.PIPE, stdout=f, text=True) for f in outs]
+        for p, k in zip(ps, idx):
+            p.stdin.write(head + f"bench\n{2 * math.pi / omegas[k]:.17e} {nrows} {stride} {mg}\nend\n")
+            p.stdin.close()
+        res = []
+        for p, f in zip(ps, outs):
+            if p.wait() != 0:
+                raise RuntimeError("reference harness failed")
+            f.seek(0)
+            last = f.read().strip().split("\n")[-1].split()
+            f.close()
+            res.append((float(last[2]), float(last[4])))
+        return res
+
     t0 = time.perf_counter()
-    ps = [subprocess.Popen([str(REF_HARNESS)], stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
-          for _ in idx]
-    for p, k in zip(ps, idx):
-        p.stdin.write(head + f"bench\n{2 * math.pi / omegas[k]:.17e} {rows} {stride} {m_gauss}\nend\n")
-        p.stdin.close()
-    t_freq = []
-    for p in ps:
-        last = p.stdout.read().strip().split("\n")[-1].split()
-        if p.wait() != 0:
-            raise RuntimeError("reference harness failed")
-        t_pairs, t_gauss = float(last[2]), float(last[4])
-        t_freq.append(t_pairs / (rows * n) * n * n * (1 + nrhs) + t_gauss * (n / m_gauss) ** 3)
+    big = batch(rows, m_large)
+    small = batch(1, m_small)
     wall = time.perf_counter() - t0
+    g_large = float(np.mean([g for _, g in big]))
+    g_small = float(np.mean([g for _, g in small]))
+    expo = math.log(g_large / g_small) / math.log(m_large / m_small)
+    t_freq = [tp / (rows * n) * n * n * (1 + nrhs) + tg * (n / m_large) ** expo for tp, tg in big]
     mean_t = float(np.mean(t_freq))
     return {"value": procs / mean_t, "unit": "freq-solves/s", "cores": procs, "kind": "reference",
             "sample": f"reference Solver routines (flang -O2 FP64 build of the unmodified sources), {procs} "
                       f"single-threaded processes at w = {', '.join(f'{omegas[k]:.3f}' for k in idx)} rad/s: "
                       f"each times {rows} field rows x {n} panels of VAVFD+VNSFD pairs and GAUSSZ on "
-                      f"{m_gauss}x{2 * m_gauss}, scaled to one frequency as N^2(1+{nrhs}) pairs + "
-                      f"(N/{m_gauss})^3 GAUSSZ; mean {mean_t:.1f} s per frequency per core "
-                      f"(sample wall {wall:.1f} s)"}
+                      f"{m_large}x{2 * m_large} ({g_large:.1f} s) and {m_small}x{2 * m_small} ({g_small:.2f} s); "
+                      f"one frequency = N^2(1+{nrhs}) pairs + GAUSSZ extrapolated as n^{expo:.2f} from "
+                      f"{m_large}; mean {mean_t:.1f} s per frequency per core (sample wall {wall:.1f} s; the "
+                      f"complete reference Solver took 302 s per frequency on one core of another host, "
+                      f"profiles/cpu_reference_barge4000.txt)"}
 
 
 def cpu_baseline(m, omegas, budget_s=20.0):
@@ -266,10 +286,12 @@ def run_serving(rates, nreq, workers, seed=2024):
     """Serving: Poisson arrivals (fixed seed) of two-cylinder requests
     (r ~ U(2,15) m, spacing U(3,10) r, N log-uniform in [200, 16000], 12 DOF
     + 1-4 headings, 1-64 frequencies, infinite depth), admitted by the
-    native request loop under a budget of N^2 complex64 per in-flight
-    frequency.  Latency = submission to completion, stamped by the loop.
-    The first rate runs on the mixed and the plain FP64 paths, the others
-    on the mixed path only."""
+    native request loop under its default budget of 32 GiB of device memory
+    (each frequency is admitted with its handle's resident bytes charged:
+    FP64 D and S plus the complex64 copy and factors on the mixed path).
+    Latency = submission to completion, stamped by the loop.  The first rate
+    runs on the mixed and the plain FP64 paths, the others on the mixed path
+    only; the loop saturates where req_per_s stops following the rate."""
     from nemoh_amd import core as ncore
     from nemoh_amd import serving
     rng = np.random.default_rng(seed)
@@ -290,23 +312,27 @@ def run_serving(rates, nreq, workers, seed=2024):
                 ids.append(srv.submit(r, prec))
             tim = [srv.wait(i)[1] for i in ids]
             wall = time.perf_counter() - t0
+            _, peak = srv.memory()
         lat = np.array([t["latency"] for t in tim])
         svc = np.array([t["service"] for t in tim])
         return {"p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
                 "mean_service_ms": round(float(svc.mean()), 1), "wall_s": round(wall, 2),
-                "req_per_s": round(nreq / wall, 3), "freq_solves_per_s": round(nfreq / wall, 2)}
+                "req_per_s": round(nreq / wall, 3), "freq_solves_per_s": round(nfreq / wall, 2),
+                "peak_charged_gib": round(peak / 2 ** 30, 2)}
 
     out = {"workload": f"{nreq} requests, Poisson arrivals (seed {seed}): two cylinders r ~ U(2,15) m, spacing "
                        "U(3,10) r, N log-uniform 200-16000 panels, 12 DOF + 1-4 headings, 1-64 frequencies, "
                        "infinite depth",
            "panels": [r.npanels for r in reqs], "frequencies": nfreq, "workers": workers,
-           "budget": "N^2 complex64 per in-flight frequency, 32 GiB", "rates": {}}
+           "budget": "32 GiB of device memory: each in-flight frequency charged its handle's resident bytes "
+                     "(about 40 N^2 B mixed, 32 N^2 B FP64, per symmetry block)", "rates": {}}
     for k, rate in enumerate(rates):
         res = {"mixed": stream(rate, ncore.MIXED)}
         if k == 0:
             res["fp64"] = stream(rate, ncore.FP64)
         out["rates"][str(rate)] = res
         print(f"serving at {rate} req/s: {res}", file=sys.stderr, flush=True)
+    out["saturation_req_per_s"] = max(v["mixed"]["req_per_s"] for v in out["rates"].values())
     return out
 
 
@@ -392,6 +418,13 @@ def main():
     tg64, _, _ = run_precision(c, ncore.FP64, omegas, nvel, nds, ns, 2, timing=False)
     tgmx, _, _ = run_precision(c, ncore.MIXED, omegas, nvel, nds, ns, 2, timing=False)
     host_same = max(float((np.abs(fh[k] - fmx[k]).max(0) / np.abs(fmx[k]).max(0)).max()) for k in fmx if k in fh)
+    # the mixed path with the optional FP32 assembly (FP32 far field, wave
+    # terms and exponential images, complex64 D and S): its speed and its
+    # error against the FP64 path on every force
+    c.set_option(ncore.OPT_FP32_ASSEMBLY, 1)
+    t32, f32, ph32 = run_precision(c, ncore.MIXED, omegas, nvel, nds, ns, 2)
+    c.set_option(ncore.OPT_FP32_ASSEMBLY, 0)
+    err32 = max(float((np.abs(f32[k] - f64[k]).max(0) / np.abs(f64[k]).max(0)).max()) for k in f64 if k in f32)
     outputs = run_outputs(c) if a.outputs else None
     c.close()
     ss_mixed = ns / tmx
@@ -436,17 +469,21 @@ def main():
         "fp64_value": round(rate_fp64, 3), "fp64_ms_per_step": round(1e3 * c64 / fp64_steps, 3),
         "speedup_mixed_vs_fp64": round(rate_mixed / rate_fp64, 3),
         "max_rel_err_mixed_vs_fp64": err,
+        "max_rel_err_mixed_fp32_assembly_vs_fp64": err32,
         "frequencies_in_flight": a.workers,
         "single_stream": {"passes": a.single_passes, "mixed": round(ss_mixed, 3), "fp64": round(ss_fp64, 3),
                           "mixed_graphs": round(ns / tgmx, 3), "fp64_graphs": round(ns / tg64, 3),
                           "identical_to_concurrent": bool(same),
                           "mixed_host_inputs": round(ns / th, 3),
+                          "mixed_fp32_assembly": round(ns / t32, 3),
                           "host_vs_device_inputs_max_rel_diff": host_same},
         "inputs": "problems generated in HBM by nemoh_core_solve_waves (no per-frequency PCIe transfer); "
                   "mixed_host_inputs = the same solves with NVEL/NDS handed over from host memory",
         "phase_ms_median": {
             "fp64": {k: round(float(np.median(v)), 3) for k, v in ph64.items() if k in ("assembly", "factor", "solve")},
             "mixed": {k: round(float(np.median(v)), 3) for k, v in phmx.items() if k in ("assembly", "factor", "solve")},
+            "mixed_fp32_assembly": {k: round(float(np.median(v)), 3) for k, v in ph32.items()
+                                    if k in ("assembly", "factor", "solve")},
             "mixed_refine_iterations_max": int(max(phmx["it"]) if phmx["it"] else 0)},
         "roofline": {"bound": "mfma", "kernel": "lu_update128_kernel<float> (rank-128 LU trailing update, complex64 "
                                                  "MFMA)",

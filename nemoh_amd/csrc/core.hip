@@ -985,6 +985,13 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
   // NEMOH_GRAPHS=0 in the environment: launch one by one (as
   // NEMOH_OPT_GRAPHS = 0), e.g. under a profiler that traces kernels
   if (const char* e = std::getenv("NEMOH_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
+  // NEMOH_LU_RANK=64 or 128: the factorisation form of every handle (as
+  // NEMOH_OPT_LU_RANK), for handles a caller does not create itself (the
+  // request loop's workers)
+  if (const char* e = std::getenv("NEMOH_LU_RANK")) {
+    const int r = std::atoi(e);
+    if (r == 64 || r == 128) c->lu_rank = r;
+  }
   if (int r = core_bind_mesh(c, np, nfa, nsym, x, y, z, m1, m2, m3, m4, geo, zer, g, rho, depth)) return bail(r);
   *out = c;
   return NEMOH_OK;
