@@ -190,10 +190,12 @@ int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter
  * with one launch per block step (nemoh_stats.solve_fallbacks counts it).
  * 0 makes every wait give up at once, which forces that path (tests). */
 /* NEMOH_OPT_GRAPHS: 1 (default) = each frequency's assembly,
- * factorisation, solves and phi replay from a HIP graph the handle captures
- * on first use (one per precision, regime, problem count and solve form;
- * dropped when the handle's buffers or mesh change); 0 = launched one by one.
- * Timing (nemoh_core_enable_timing) always launches one by one. */
+ * factorisation, solves and phi replay from a HIP graph that the handle
+ * captures on the third use of a launch shape (precision, regime, problem
+ * count and solve form; the first two run one by one, so that shapes used
+ * once or twice, e.g. a batch of distinct bodies, pay no capture); graphs
+ * are dropped when the handle's buffers or mesh change; 0 = always launched
+ * one by one.  Timing (nemoh_core_enable_timing) always launches one by one. */
 /* NEMOH_OPT_LU_RANK: 128 (default) = block steps in pairs with rank-128
  * trailing updates; 64 = one rank-64 trailing update per block step.
  * NEMOH_OPT_LU_LOOKAHEAD (rank 128 only): 1 = each pair's trailing update
@@ -221,6 +223,7 @@ typedef struct {
   double lu_update_flops;
   int lu_precision; /* NEMOH_FP64 or NEMOH_MIXED (complex64 factors) */
   int solve_fallbacks; /* chained solves redone stepwise after a bounded wait gave up (this handle) */
+  int graph_replays;   /* frequencies replayed from a HIP graph (this handle) */
 } nemoh_stats;
 int nemoh_core_get_stats(const nemoh_core *ctx, nemoh_stats *out);
 int nemoh_core_enable_timing(nemoh_core *ctx, int on);

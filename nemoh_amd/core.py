@@ -67,7 +67,7 @@ class Stats(C.Structure):
                 ("ms_total", C.c_double), ("refine_iterations", C.c_int), ("refine_residual", C.c_double),
                 ("cache_hit", C.c_int), ("ms_lu_update", C.c_double), ("lu_update_launches", C.c_int),
                 ("lu_update_flops", C.c_double), ("lu_precision", C.c_int),
-                ("solve_fallbacks", C.c_int)]
+                ("solve_fallbacks", C.c_int), ("graph_replays", C.c_int)]
 
 
 _dp = C.POINTER(C.c_double)

@@ -125,7 +125,8 @@ struct nemoh_core {
   bool use_graphs = true;
   struct GraphEntry {
     int refactor, mode, fd, nrhs, stepwise;
-    hipGraphExec_t exec;
+    int uses;               // launches of this shape since the graphs were last dropped
+    hipGraphExec_t exec;    // captured on the kGraphAfter-th use
   };
   std::vector<GraphEntry> graphs;
   unsigned layout_gen = 0, graphs_gen = 0;
@@ -497,8 +498,15 @@ int queue_solves(nemoh_core* c, int nrhs, bool stepwise) {
   return NEMOH_OK;
 }
 
+// Uses of a launch shape before the handle captures it in a HIP graph: a
+// capture and instantiation cost about as much as several small-body
+// frequencies, so shapes met once or twice (a batch of distinct bodies on a
+// request-loop worker) run eagerly.
+constexpr int kGraphAfter = 3;
+
 void drop_graphs(nemoh_core* c) {
-  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  for (auto& g : c->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   c->graphs.clear();
   c->graphs_gen = c->layout_gen;
 }
@@ -510,18 +518,23 @@ void drop_graphs(nemoh_core* c) {
 // the graph reads, and every kernel argument is fixed for a given binding,
 // precision, regime, problem count and solve form.
 int launch_frequency(nemoh_core* c, bool refactor, int nrhs, bool stepwise) {
-  if (!c->use_graphs || c->timing) {
+  auto eager = [&]() -> int {
     if (refactor)
       if (int r = queue_factor(c)) return r;
     return queue_solves(c, nrhs, stepwise);
-  }
+  };
+  if (!c->use_graphs || c->timing) return eager();
   if (c->graphs_gen != c->layout_gen) drop_graphs(c);
   const int key[5] = {refactor ? 1 : 0, matrix_mode(c), c->h_par.fd, nrhs, stepwise ? 1 : 0};
-  hipGraphExec_t exec = nullptr;
-  for (const auto& g : c->graphs)
-    if (g.refactor == key[0] && g.mode == key[1] && g.fd == key[2] && g.nrhs == key[3] && g.stepwise == key[4])
-      exec = g.exec;
-  if (!exec) {
+  nemoh_core::GraphEntry* g = nullptr;
+  for (auto& e : c->graphs)
+    if (e.refactor == key[0] && e.mode == key[1] && e.fd == key[2] && e.nrhs == key[3] && e.stepwise == key[4]) g = &e;
+  if (!g) {
+    c->graphs.push_back({key[0], key[1], key[2], key[3], key[4], 0, nullptr});
+    g = &c->graphs.back();
+  }
+  if (!g->exec && ++g->uses < kGraphAfter) return eager();
+  if (!g->exec) {
     std::lock_guard<std::recursive_mutex> lk(device_lock());
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     int r = refactor ? queue_factor(c) : NEMOH_OK;
@@ -533,12 +546,14 @@ int launch_frequency(nemoh_core* c, bool refactor, int nrhs, bool stepwise) {
       return r;
     }
     if (e != hipSuccess) return fail(c, NEMOH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
     const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess) return fail(c, NEMOH_ERR_HIP, std::string("graph instantiation: ") + hipGetErrorString(ei));
-    c->graphs.push_back({key[0], key[1], key[2], key[3], key[4], exec});
+    g->exec = exec;
   }
-  HIPCHK(c, hipGraphLaunch(exec, c->st));
+  HIPCHK(c, hipGraphLaunch(g->exec, c->st));
+  ++c->stats.graph_replays;
   return NEMOH_OK;
 }
 

@@ -330,12 +330,13 @@ def test_alternating_precisions_on_one_handle():
 
 @pytest.mark.parametrize("prec", [core.FP64, core.MIXED])
 def test_graph_replays_equal_eager_launches(prec):
-    """The per-frequency pipeline replayed from HIP graphs (the default)
-    gives bit for bit the forces of the same launches made one by one: over
-    several frequencies in both regimes (finite depth, then the
-    infinite-depth frequency at the same depth), each graph replayed several
-    times with eager solves and host synchronisations in between, and a
-    factor-cache hit (the solve-only graph)."""
+    """The per-frequency pipeline replayed from HIP graphs (the default; a
+    handle captures a launch shape on its third use) gives bit for bit the
+    forces of the same launches made one by one: over several frequencies in
+    both regimes (finite depth, then the infinite-depth frequency at the same
+    depth), each graph replayed several times with eager solves and host
+    synchronisations in between, and a factor-cache hit (the solve-only
+    graph)."""
     d, m, ref = _case("barge_fd")
     om = np.unique(d["omega"])
     eager = core.Core(d["vertices"], d["panels"], 0, float(d["depth"]), float(d["rho"]), float(d["g"]),
@@ -355,6 +356,10 @@ def test_graph_replays_equal_eager_launches(prec):
             _, fh = graphs.solve_frequency(2 * np.pi / w, nv[:, ::-1].copy(), nds=d["nds"], want_pressure=False)
             assert graphs.stats().cache_hit == 1 and np.array_equal(fh, fe[:, ::-1])
             assert _normwise(fg, ref[:, cols]) < (FP64_TOL if prec == core.FP64 else MIXED_TOL)
+    assert eager.stats().graph_replays == 0
+    # 3 passes x (factorise + cache hit) per frequency: the finite-depth
+    # shapes are met 3 x (frequencies - 1) times each, so most of them replay
+    assert graphs.stats().graph_replays >= 2 * (om.size - 1), graphs.stats().graph_replays
     eager.close()
     graphs.close()
 
