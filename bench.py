@@ -23,9 +23,17 @@ from __future__ import annotations
 import argparse
 import json
 import math
+import os
 import sys
 import time
 from pathlib import Path
+
+# Hardware queues of this process (read by the HIP runtime when it starts,
+# before any device call): the throughput legs run 8 frequencies on 8
+# streams, and with the runtime's default of 4 queues two streams share each
+# queue and their kernels run one after the other (8 queues: 4000-panel
+# barge 338 against 310 freq-solves/s, 256-body batch 917 against 875)
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 import numpy as np
 
@@ -471,6 +479,7 @@ def main():
         "max_rel_err_mixed_vs_fp64": err,
         "max_rel_err_mixed_fp32_assembly_vs_fp64": err32,
         "frequencies_in_flight": a.workers,
+        "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
         "single_stream": {"passes": a.single_passes, "mixed": round(ss_mixed, 3), "fp64": round(ss_fp64, 3),
                           "mixed_graphs": round(ns / tgmx, 3), "fp64_graphs": round(ns / tg64, 3),
                           "identical_to_concurrent": bool(same),

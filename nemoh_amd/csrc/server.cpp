@@ -3,8 +3,12 @@
 //
 // A request is a mesh, its frequencies and the normal velocities of every
 // problem at each frequency; the reply is the forces of every problem.  The
-// unit of work is one (request, frequency).  Work is handed out first come,
-// first served, while the device-memory budget allows.  The budget bounds
+// unit of work is one (request, frequency).  Work is handed out while the
+// device-memory budget allows: a worker continues the request its last item
+// came from, else takes the oldest request no other worker is on, else the
+// oldest (so distinct requests spread over the workers, each handle keeps
+// its mesh across a request's frequencies, and a lone request is shared by
+// every worker).  The budget bounds
 // the device memory that the workers' handles hold: a busy worker is
 // charged the larger of what its handle holds and what its frequency needs
 // (FP64 D and S, the complex64 factor copy in mixed precision, the pivot
@@ -104,6 +108,8 @@ struct Worker {
   double held = 0;     // device bytes the handle holds (updated after each item)
   double charged = 0;  // bytes counted against the budget (held, or more while busy)
   bool release = false;  // free the handle's matrices before the next item
+  long long claim = -1;  // request of this worker's current or last item
+  size_t pick = 0;       // queue position of the request admitted for its next item
 };
 
 }  // namespace
@@ -219,12 +225,33 @@ void run_item(nemoh_server* s, Worker& w, Request& r, int f) {
   }
 }
 
-// Under the server mutex: may worker `self` take the head of the queue?
-// Frees idle workers' matrices when that is what stands in the way.  On
-// success the worker is charged and marked busy.
+// The queued request a worker takes its next frequency from (server mutex
+// held): the one its last item came from while it has frequencies left, so
+// the handle keeps its mesh, factors-to-be and graphs; else the first that no
+// other worker's last item came from, so that distinct requests (a batch of
+// bodies) spread over the workers; else the oldest, so that a lone request's
+// frequencies are shared by every worker.
+size_t pick_request(const nemoh_server* s, int self) {
+  const long long mine = s->workers[self].claim;
+  for (size_t k = 0; k < s->queue.size(); ++k)
+    if (s->queue[k]->id == mine) return k;
+  for (size_t k = 0; k < s->queue.size(); ++k) {
+    bool taken = false;
+    for (size_t v = 0; v < s->workers.size() && !taken; ++v)
+      taken = (int)v != self && s->workers[v].claim == s->queue[k]->id;
+    if (!taken) return k;
+  }
+  return 0;
+}
+
+// Under the server mutex: may worker `self` take its next frequency (from
+// the request pick_request chooses, recorded in w.pick)?  Frees idle
+// workers' matrices when that is what stands in the way.  On success the
+// worker is charged and marked busy.
 bool try_admit(nemoh_server* s, int self, bool freed = false) {
   Worker& w = s->workers[self];
-  const double need = s->queue.front()->footprint;
+  w.pick = pick_request(s, self);
+  const double need = s->queue[w.pick]->footprint;
   double others = 0, idle_held = 0;
   for (size_t v = 0; v < s->workers.size(); ++v) {
     if ((int)v == self) continue;
@@ -266,9 +293,10 @@ void worker_loop(nemoh_server* s, int self) {
       // queued work is finished before a stop takes effect
       s->cv_work.wait(lk, [&] { return (!s->queue.empty() && try_admit(s, self)) || (s->stop && s->queue.empty()); });
       if (!w.busy) break;  // stopped with the queue empty
-      r = s->queue.front();
+      r = s->queue[w.pick];
       f = r->next++;
-      if (r->next == r->nfreq) s->queue.pop_front();
+      w.claim = r->id;
+      if (r->next == r->nfreq) s->queue.erase(s->queue.begin() + (std::ptrdiff_t)w.pick);
       if (r->t_start < 0) r->t_start = now_ms();
     }
     if (w.release && w.h) (void)nemoh::core_release_matrices(w.h);
