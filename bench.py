@@ -29,11 +29,13 @@ import time
 from pathlib import Path
 
 # Hardware queues of this process (read by the HIP runtime when it starts,
-# before any device call): the throughput legs run 8 frequencies on 8
-# streams, and with the runtime's default of 4 queues two streams share each
-# queue and their kernels run one after the other (8 queues: 4000-panel
-# barge 338 against 310 freq-solves/s, 256-body batch 917 against 875)
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# before any device call), raised to 8 when the environment asks for fewer:
+# the throughput legs run 8 frequencies on 8 streams, and with 4 queues (the
+# runtime's default) two streams share each queue and their kernels run one
+# after the other (8 queues: 4000-panel barge 338-342 against 310-311
+# freq-solves/s, 256-body batch 917-930 against 875)
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 import numpy as np
 
