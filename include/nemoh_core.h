@@ -204,8 +204,13 @@ int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter
  * one launch per pair.  Not validated: the one run with it on (graphs
  * captured over the two streams by 8 request-loop threads) ended in a host
  * segmentation fault whose cause is not established. */
+/* NEMOH_OPT_LU_FUSED_PANEL (rank 128 only): 1 = the panel of each pair's
+ * second block column runs inside the pair's cross launch, whose workgroups
+ * wait (bounded) for the pivot block that launch inverts; 0 (default) = a
+ * launch of its own.  Both take the same time (the launch boundary is not
+ * what bounds a pair's steps); a failed wait is reported as NEMOH_ERR_HIP. */
 enum { NEMOH_OPT_STEPWISE_SOLVE = 1, NEMOH_OPT_FP32_ASSEMBLY = 2, NEMOH_OPT_SOLVE_SPIN_LIMIT = 3,
-       NEMOH_OPT_GRAPHS = 4, NEMOH_OPT_LU_RANK = 5, NEMOH_OPT_LU_LOOKAHEAD = 6 };
+       NEMOH_OPT_GRAPHS = 4, NEMOH_OPT_LU_RANK = 5, NEMOH_OPT_LU_LOOKAHEAD = 6, NEMOH_OPT_LU_FUSED_PANEL = 7 };
 int nemoh_core_set_option(nemoh_core *ctx, int option, int value);
 
 /* Timings (ms) of the last solve's phases and refinement statistics. */

@@ -76,9 +76,16 @@ void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int f
 // ones in the paired form) is made with hipExtLaunchKernel, which stamps
 // [2 t] at its start and [2 t + 1] at its end, and upd_flops[t] gets its
 // algorithmic flops; returns the number t of such launches.
+// luctl (optional, 2 lu_pairs(n) ints of device memory): with it, the
+// paired form folds each pair's second panel into its cross launch, whose
+// workgroups wait (at most spin_limit polls; info[1] |= 2 when one gives up)
+// for the pivot block inverted inside the same launch.
 template <typename T>
 int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int rank = 128, hipStream_t st2 = nullptr,
-              hipEvent_t* dep = nullptr, hipEvent_t* upd_ev = nullptr, double* upd_flops = nullptr);
+              hipEvent_t* dep = nullptr, hipEvent_t* upd_ev = nullptr, double* upd_flops = nullptr,
+              int* luctl = nullptr, int spin_limit = 1 << 20);
+// block pairs of an order-n factorisation, with one to spare
+inline int lu_pairs(int n) { return ((n + kNB - 1) / kNB + 1) / 2 + 1; }
 // Control block of the chained triangular solves of one handle: the two
 // triangles' ticket counters (reset on the device at the start of every
 // chained solve).

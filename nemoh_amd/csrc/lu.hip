@@ -64,6 +64,34 @@ struct View {
   int ld, nr, nc;
 };
 
+// Stores and loads of data that other workgroups of the same launch read or
+// wrote: system-scope relaxed atomics, one per 64-bit word, which are
+// coherent across the XCDs' L2 caches without a cache-wide write-back or
+// invalidation.
+__device__ __forceinline__ void st_coherent(C2<double>* p, C2<double> v) {
+  double* q = reinterpret_cast<double*>(p);
+  __hip_atomic_store(q, v.re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(q + 1, v.im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_coherent(C2<float>* p, C2<float> v) {
+  unsigned long long u;
+  __builtin_memcpy(&u, &v, sizeof u);
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ C2<float> ld_coherent(const C2<float>* p) {
+  const unsigned long long u =
+      __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  C2<float> v;
+  __builtin_memcpy(&v, &u, sizeof v);
+  return v;
+}
+__device__ __forceinline__ C2<double> ld_coherent(const C2<double>* p) {
+  const double* q = reinterpret_cast<const double*>(p);
+  return {__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+          __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)};
+}
+
 template <typename T>
 __device__ __forceinline__ void acc_zero(Acc64<T>& acc) {
 #pragma unroll
@@ -120,7 +148,7 @@ __device__ __forceinline__ void acc_store(const Acc64<T>& acc, C2<T>* __restrict
 // C given, the tile C (nr x nc at C, leading dimension ldc) is loaded while
 // the last chunk runs and added at the end: acc = C + (+-A) B, so the
 // tile's first MFMA waits only for the first operand chunk.
-template <typename T, bool kNegA, bool kFull, int kChunks = 4>
+template <typename T, bool kNegA, bool kFull, int kChunks = 4, bool kCoherentB = false>
 __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B, GemmLds<T>& S,
                                             const C2<T>* __restrict__ C, int ldc, int cnr, int cnc) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -138,12 +166,15 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
       const int e = tid + 256 * u;
       const int ar_ = e >> 4, ak = chunk * 16 + (e & 15);
       const int bk = chunk * 16 + (e >> 6), bc = e & 63;
+      // kCoherentB: B was written during this launch by another workgroup
       if (kFull) {  // unguarded loads: no select waits on their data
         pa[set][u] = A.p[(size_t)ar_ * A.ld + ak];
-        pb[set][u] = B.p[(size_t)bk * B.ld + bc];
+        pb[set][u] = kCoherentB ? ld_coherent(B.p + (size_t)bk * B.ld + bc) : B.p[(size_t)bk * B.ld + bc];
       } else {
         pa[set][u] = (ar_ < A.nr && ak < A.nc) ? A.p[(size_t)ar_ * A.ld + ak] : C2<T>{0, 0};
-        pb[set][u] = (bk < B.nr && bc < B.nc) ? B.p[(size_t)bk * B.ld + bc] : C2<T>{0, 0};
+        pb[set][u] = (bk < B.nr && bc < B.nc)
+                         ? (kCoherentB ? ld_coherent(B.p + (size_t)bk * B.ld + bc) : B.p[(size_t)bk * B.ld + bc])
+                         : C2<T>{0, 0};
       }
     }
   };
@@ -267,14 +298,14 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
   }
 }
 
-template <typename T, bool kNegA>
+template <typename T, bool kNegA, bool kCoherentB = false>
 __device__ __forceinline__ void gemm64(Acc64<T>& acc, View<T> A, View<T> B, GemmLds<T>& S,
                                        const C2<T>* __restrict__ C = nullptr, int ldc = 0, int cnr = 0,
                                        int cnc = 0) {
   if (A.nr == kNB && A.nc == kNB && B.nr == kNB && B.nc == kNB)  // uniform over the workgroup
-    gemm64_impl<T, kNegA, true>(acc, A, B, S, C, ldc, cnr, cnc);
+    gemm64_impl<T, kNegA, true, 4, kCoherentB>(acc, A, B, S, C, ldc, cnr, cnc);
   else
-    gemm64_impl<T, kNegA, false>(acc, A, B, S, C, ldc, cnr, cnc);
+    gemm64_impl<T, kNegA, false, 4, kCoherentB>(acc, A, B, S, C, ldc, cnr, cnc);
 }
 
 // The same with K = 128 (A: 64 x 128, B: 128 x 64): the rank-128 trailing
@@ -522,12 +553,17 @@ __device__ __forceinline__ void lds_invert(InvLds<T>& L, int nb, int row0, int* 
   }
 }
 
-// P (64x64 row-major) <- L.m, zero outside the nb x nb block.
-template <typename T>
+// P (64x64 row-major) <- L.m, zero outside the nb x nb block (kCoherent:
+// with system-scope stores, for readers in the same launch).
+template <typename T, bool kCoherent = false>
 __device__ __forceinline__ void store_inverse(const InvLds<T>& L, C2<T>* __restrict__ P, int nb) {
   for (int e = threadIdx.x; e < kTile; e += 256) {
     const int r = e >> 6, c = e & 63;
-    P[e] = (r < nb && c < nb) ? L.m[r][c] : C2<T>{0, 0};
+    const C2<T> v = (r < nb && c < nb) ? L.m[r][c] : C2<T>{0, 0};
+    if (kCoherent)
+      st_coherent(P + e, v);
+    else
+      P[e] = v;
   }
 }
 
@@ -604,7 +640,7 @@ __device__ __forceinline__ void update_tile(int bid, int m, int& ti, int& tj) {
 
 // Look-ahead of the updates: workgroup 0 inverts the diagonal block it has
 // just updated (acc, nr x nr at row r0) into P.
-template <typename T>
+template <typename T, bool kCoherent = false>
 __device__ __forceinline__ void invert_tile(UpdLds<T>& S, const Acc64<T>& acc, int nr, int r0, C2<T>* __restrict__ P,
                                             int* __restrict__ info) {
   __syncthreads();  // every wave is done with the GEMM staging area
@@ -623,7 +659,7 @@ __device__ __forceinline__ void invert_tile(UpdLds<T>& S, const Acc64<T>& acc, i
   }
   __syncthreads();
   lds_invert(S.inv, nr, r0, info);
-  store_inverse(S.inv, P, nr);
+  store_inverse<T, kCoherent>(S.inv, P, nr);
 }
 
 // Trailing update of step kb: A_ij -= L_i,kb A_kb,j for i, j > kb, one 64x64
@@ -722,6 +758,79 @@ __global__ __launch_bounds__(256) void lu_cross_kernel(C2<T>* __restrict__ A, in
   invert_tile(S, acc, nr, r0, P + (size_t)s * kTile, info);
 }
 
+// The same cross update with the panel of block column k+1 folded in (one
+// launch instead of two).  Workgroups take tickets from ctl[0] (0 when the
+// launch starts): ticket 0 takes the diagonal tile (k+1, k+1) and inverts
+// it, publishing P_{k+1} through ctl[1] (set to 1); tickets [1, nrow) the
+// rest of block row k+1; the others the tiles (i, k+1) of block column k+1,
+// which, once their cross update is stored, wait for P_{k+1} and turn the
+// tile into L_{i,k+1} = A_{i,k+1} P_{k+1} in place.  A workgroup only waits
+// for ticket 0, which was taken by a running workgroup, so the wait ends
+// however the launch's workgroups are placed; it is bounded all the same
+// (spin_limit polls, then info[1] |= 2 and the tile is left unfinished; the
+// caller reports the factorisation as failed).
+template <typename T>
+__global__ __launch_bounds__(256) void lu_cross_panel_kernel(C2<T>* __restrict__ A, int ld, int n, int k,
+                                                             C2<T>* __restrict__ P, int* __restrict__ info,
+                                                             int* __restrict__ ctl, int spin_limit) {
+  __shared__ UpdLds<T> S;
+  __shared__ int ticket_s, ready_s;
+  const int tid = threadIdx.x;
+  const int nblk = (n + kNB - 1) / kNB, s = k + 1, nrow = nblk - s, m = nblk - s - 1;
+  if (tid == 0) ticket_s = (int)atomicAdd(reinterpret_cast<unsigned*>(ctl), 1u);
+  __syncthreads();
+  const int w = ticket_s;
+  if (w < 0 || w >= nrow + m) {  // a counter out of step with the launch: never wait on it
+    if (tid == 0) atomicOr(info + 1, 2);
+    return;
+  }
+  const int bi = w < nrow ? s : s + 1 + (w - nrow), bj = w < nrow ? s + w : s;
+  const int r0 = bi * kNB, c0 = bj * kNB, k0 = k * kNB;
+  const int nr = min(kNB, n - r0), nc = min(kNB, n - c0);
+  C2<T>* blk = A + (size_t)r0 * ld + c0;
+  Acc64<T> acc;
+  acc_zero(acc);
+  gemm64<T, true>(acc, View<T>{A + (size_t)r0 * ld + k0, ld, nr, kNB}, View<T>{A + (size_t)k0 * ld + c0, ld, kNB, nc},
+                  S.g, blk, ld, nr, nc);
+  acc_store(acc, blk, ld, nr, nc);
+  if (w == 0) {
+    // P_{k+1} with system-scope stores, every one performed before the flag
+    invert_tile<T, true>(S, acc, nr, r0, P + (size_t)s * kTile, info);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return;
+  }
+  if (w < nrow) return;
+  if (tid == 0) {
+    int spins = 0, ok = 1;
+    while (__hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == 0) {
+      if (++spins > spin_limit) {
+        ok = 0;
+        atomicOr(info + 1, 2);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    ready_s = ok;
+  }
+  __syncthreads();
+  if (!ready_s) return;
+  // P_{k+1} read with system-scope loads (no cache-wide invalidation); the
+  // tile itself was stored by this workgroup
+  acc_zero(acc);
+  gemm64<T, false, true>(acc, View<T>{blk, ld, nr, nc}, View<T>{P + (size_t)s * kTile, kNB, kNB, kNB}, S.g);
+  acc_store(acc, blk, ld, nr, nc);
+}
+
+// Control words of the fused cross launches (2 per block pair: a ticket
+// counter and the P_{k+1} flag) set to 0 at the start of a factorisation.
+__global__ void lu_ctl_reset_kernel(int* __restrict__ ctl, int count) {
+  for (int e = threadIdx.x; e < count; e += blockDim.x) ctl[e] = 0;
+}
+
 // Rank-128 trailing update of the pair (k, k+1): A_ij -= L_{i,k:k+1}
 // A_{k:k+1,j} for i, j > k+1 (m trailing blocks from s = k+2), one 64x64 tile
 // per workgroup.  region 0: every trailing tile, in the XCD-banded order of
@@ -802,20 +911,6 @@ __device__ __forceinline__ void hop_load_tile(HopTile<T>& h, const C2<T>* __rest
   }
 }
 
-// Stores of solved blocks that other workgroups of the same launch read:
-// system-scope relaxed atomics (coherent across the XCDs' L2 caches without a
-// cache-wide write-back or invalidation), one per 64-bit word.
-__device__ __forceinline__ void st_coherent(C2<double>* p, C2<double> v) {
-  double* q = reinterpret_cast<double*>(p);
-  __hip_atomic_store(q, v.re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(q + 1, v.im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-__device__ __forceinline__ void st_coherent(C2<float>* p, C2<float> v) {
-  unsigned long long u;
-  __builtin_memcpy(&u, &v, sizeof u);
-  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
 // An element of a chained solve's x that is not solved yet holds a marker: a
 // signalling-NaN bit pattern in every 64-bit word (both halves of a complex64
 // word).  Arithmetic never yields a signalling NaN, so a solved element never
@@ -833,19 +928,6 @@ __device__ __forceinline__ void st_unsolved(C2<double>* p) {
   unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
   __hip_atomic_store(q, kUnsolved64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   __hip_atomic_store(q + 1, kUnsolved64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-// whole-word loads of elements that other workgroups store (see st_coherent)
-__device__ __forceinline__ C2<float> ld_coherent(const C2<float>* p) {
-  const unsigned long long u =
-      __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  C2<float> v;
-  __builtin_memcpy(&v, &u, sizeof v);
-  return v;
-}
-__device__ __forceinline__ C2<double> ld_coherent(const C2<double>* p) {
-  const double* q = reinterpret_cast<const double*>(p);
-  return {__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
-          __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)};
 }
 // false while v is (partly) the marker
 __device__ __forceinline__ bool is_solved(C2<float> v) {
@@ -1213,7 +1295,7 @@ __global__ void mfma_probe_kernel(const T* __restrict__ a, const T* __restrict__
 
 template <typename T>
 int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int rank, hipStream_t st2,
-              hipEvent_t* dep, hipEvent_t* upd_ev, double* upd_flops) {
+              hipEvent_t* dep, hipEvent_t* upd_ev, double* upd_flops, int* luctl, int spin_limit) {
   const int nblk = (n + kNB - 1) / kNB;
   int timed = 0;
   // trailing-update launch: events around it when timing (stamped by the
@@ -1249,6 +1331,9 @@ int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int 
   // the panel launch of pair k+4 does.  dep: one event per pair for st -> st2
   // (its panels are done) and one for st2 -> st (its bulk is done).
   const bool ahead = st2 != nullptr;
+  // the panel of each pair's second block column inside its cross launch
+  const bool fused = luctl != nullptr && !ahead;
+  if (fused) lu_ctl_reset_kernel<<<1, 256, 0, st>>>(luctl, 2 * lu_pairs(n));
   PanelJob pend[2] = {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}};  // U rows waiting for a bulk update to end
   int pair = 0;
   int bulk_done = -1;  // event of the latest bulk update (-1: none yet)
@@ -1261,10 +1346,16 @@ int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int 
     pend[1] = PanelJob{0, 0, 0, 0, 0, 0};
     if (k + 1 >= nblk) break;  // odd count: the last block has no pair
     const int nrow = nblk - (k + 1), m = nblk - (k + 2);
-    lu_cross_kernel<T><<<nrow + m, 256, 0, st>>>(A, ld, n, k, P, info);
+    if (fused) {
+      lu_cross_panel_kernel<T><<<nrow + m, 256, 0, st>>>(A, ld, n, k, P, info, luctl + 2 * pair, spin_limit);
+    } else {
+      lu_cross_kernel<T><<<nrow + m, 256, 0, st>>>(A, ld, n, k, P, info);
+      if (m > 0) {
+        const PanelJob L1 = {k + 1, m, 0, 0, 0, 0};
+        lu_panel2_kernel<T><<<m, 256, 0, st>>>(A, ld, n, L1, P);
+      }
+    }
     if (m > 0) {
-      const PanelJob L1 = {k + 1, m, 0, 0, 0, 0};
-      lu_panel2_kernel<T><<<m, 256, 0, st>>>(A, ld, n, L1, P);
       const double mr = n - (k + 2.0) * kNB;
       if (ahead && m > 2) {
         const double mb = mr - 2.0 * kNB;
@@ -1291,9 +1382,9 @@ int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int 
   return timed;
 }
 template int launch_lu<double>(C2<double>*, int, int, C2<double>*, int*, hipStream_t, int, hipStream_t, hipEvent_t*,
-                               hipEvent_t*, double*);
+                               hipEvent_t*, double*, int*, int);
 template int launch_lu<float>(C2<float>*, int, int, C2<float>*, int*, hipStream_t, int, hipStream_t, hipEvent_t*,
-                              hipEvent_t*, double*);
+                              hipEvent_t*, double*, int*, int);
 
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,

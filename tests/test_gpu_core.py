@@ -366,11 +366,12 @@ def test_graph_replays_equal_eager_launches(prec):
 
 @pytest.mark.parametrize("name", ["cyl512", "barge_fd", "c4cyl", "barge4000"])
 @pytest.mark.parametrize("prec", [core.FP64, core.MIXED])
-@pytest.mark.parametrize("form", ["rank64", "rank128", "rank128-lookahead"])
+@pytest.mark.parametrize("form", ["rank64", "rank128", "rank128-fused-panel", "rank128-lookahead"])
 def test_lu_forms_match_reference(name, prec, form):
     """Every form of the factorisation against the reference forces: one
     rank-64 trailing update per block step; block steps in pairs with
-    rank-128 updates (the default); and the look-ahead form, whose pair
+    rank-128 updates (the default), also with each pair's second panel
+    inside its cross launch; and the look-ahead form, whose pair
     updates split the next pair's rows and columns from the bulk, which runs
     on a second stream beside the next pair's panels (blocks: 8 (cyl512, with
     both symmetry blocks), 3, 10 and 63: even and odd counts, the last pairs
@@ -382,6 +383,7 @@ def test_lu_forms_match_reference(name, prec, form):
     s = Solver(m, float(d["depth"]), float(d["rho"]), float(d["g"]), precision=prec, tables=_tables())
     s.core.set_option(core.OPT_LU_RANK, 64 if form == "rank64" else 128)
     s.core.set_option(core.OPT_LU_LOOKAHEAD, 1 if form == "rank128-lookahead" else 0)
+    s.core.set_option(core.OPT_LU_FUSED_PANEL, 1 if form == "rank128-fused-panel" else 0)
     f = s.solve_problems(d["omega"], d["nvel"], d["nds"])
     s.close()
     err = _normwise(f, ref)
