@@ -209,8 +209,15 @@ int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter
  * wait (bounded) for the pivot block that launch inverts; 0 (default) = a
  * launch of its own.  Both take the same time (the launch boundary is not
  * what bounds a pair's steps); a failed wait is reported as NEMOH_ERR_HIP. */
+/* NEMOH_OPT_PAIRED_ASSEMBLY (FP64 assembly): 1 (default) = the wave part of
+ * the Green function, which depends on a panel pair through its horizontal
+ * distance and image depths only, is evaluated once per unordered pair
+ * {i, j} for both elements D_ij and D_ji (the Rankine part, whose near-field
+ * switch is the source panel's, once per element); 0 = every element on its
+ * own.  Both agree to rounding. */
 enum { NEMOH_OPT_STEPWISE_SOLVE = 1, NEMOH_OPT_FP32_ASSEMBLY = 2, NEMOH_OPT_SOLVE_SPIN_LIMIT = 3,
-       NEMOH_OPT_GRAPHS = 4, NEMOH_OPT_LU_RANK = 5, NEMOH_OPT_LU_LOOKAHEAD = 6, NEMOH_OPT_LU_FUSED_PANEL = 7 };
+       NEMOH_OPT_GRAPHS = 4, NEMOH_OPT_LU_RANK = 5, NEMOH_OPT_LU_LOOKAHEAD = 6, NEMOH_OPT_LU_FUSED_PANEL = 7,
+       NEMOH_OPT_PAIRED_ASSEMBLY = 8 };
 int nemoh_core_set_option(nemoh_core *ctx, int option, int value);
 
 /* Timings (ms) of the last solve's phases and refinement statistics. */

@@ -21,6 +21,7 @@ OPT_GRAPHS = 4
 OPT_LU_RANK = 5
 OPT_LU_LOOKAHEAD = 6
 OPT_LU_FUSED_PANEL = 7
+OPT_PAIRED_ASSEMBLY = 8
 
 EXPORTS = (
     "nemoh_core_create", "nemoh_core_solve", "nemoh_core_solve_multi", "nemoh_core_solve_frequency",

@@ -143,6 +143,9 @@ struct nemoh_core {
   // measured equal to the separate launch, 3.36 ms per 4000-panel
   // factorisation either way, without the in-launch wait)
   bool lu_fused = false;
+  // NEMOH_OPT_PAIRED_ASSEMBLY: the wave part of each unordered panel pair
+  // once for both matrix elements
+  bool paired_assembly = true;
   unsigned char* d_scratch = nullptr;  // this device's `scratch` (nemoh_core_kochin_noalloc)
   std::string err;
 };
@@ -378,7 +381,8 @@ int queue_factor(nemoh_core* c) {
       out.dm32 = c->D32[1];
       out.ld32 = c->ld32;
     }
-    launch_assemble(c->G, c->T, c->d_par, c->h_par.fd, c->nsym, out, c->d_surf, c->nsurf, c->st);
+    launch_assemble(c->G, c->T, c->d_par, c->h_par.fd, c->nsym, out, c->d_surf, c->nsurf, c->paired_assembly,
+                    c->st);
   }
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->st));
@@ -1144,6 +1148,14 @@ int nemoh_core_set_option(nemoh_core* c, int option, int value) {
     if (c->lookahead != (value != 0)) {
       c->lookahead = value != 0;
       ++c->layout_gen;  // the graphs' launch sequences change
+    }
+    return NEMOH_OK;
+  }
+  if (option == NEMOH_OPT_PAIRED_ASSEMBLY) {
+    if (c->paired_assembly != (value != 0)) {
+      c->paired_assembly = value != 0;
+      c->w_prev = -1.0;  // the matrices of the other form differ in rounding
+      ++c->layout_gen;   // and so do the graphs' launch sequences
     }
     return NEMOH_OK;
   }

@@ -785,6 +785,166 @@ __device__ WavePart wave_fd_image(const Geometry& G, const Tables& T, const Freq
   return w;
 }
 
+// ---------------------------------------------------------------------------
+// The wave part of an unordered pair {i, j} with both points below the free
+// surface, once for both orders.  It depends on the pair through the
+// horizontal distance and the images' depths only:
+//   - infinite depth: one image at z_i + z_j;
+//   - finite depth: z-images at z_i + z_j, z_j - z_i - 2h, z_i - z_j - 2h,
+//     -z_i - z_j - 4h (the middle two trade places when i and j do) and the
+//     exponential images at z_i - z_j - 2h + hl, -z_i - z_j - hl,
+//     -z_i - z_j - 4h + hl, z_i - z_j + 2h - hl (the first and last trade
+//     places, negated).
+// So the potentials and the radial gradient coefficients are the same for
+// (i, j) and (j, i), and the vertical gradient splits into a part that is
+// the same and a part that changes sign.  Values are per unit source area:
+// the order's source panel scales them (VNSINFD.f90:455-503 and VNSFD.f90:
+// 548-706 write the same terms per ordered pair).
+struct WaveSym {
+  double f1, f2;    // potential terms 1 and 2
+  double h1, h2;    // horizontal gradient k = hk * (field - source offset)
+  double z1s, z1a;  // vertical gradient 1 = z1s + z1a for (i, j), z1s - z1a for (j, i)
+  double z2s, z2a;
+};
+
+// Infinite depth (VNSINFD.f90:269-384, the submerged branch).  ddx, ddy:
+// horizontal offset of field point i from the y-image of source j.
+__device__ __forceinline__ WaveSym pair_wave_infd(const Tables& T, const FreqParams& P, double ddx, double ddy,
+                                                  double zgi, double zgj) {
+  WaveSym w = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double ak0 = P.ak0;
+  const double eps = 0.0001;
+  const double rrr = sqrt(ddx * ddx + ddy * ddy);
+  const double akr = ak0 * rrr;
+  const double zzz = zgi + zgj;
+  const double akz = ak0 * zzz;
+  const double dd2 = rrr * rrr + zzz * zzz;
+  double psurr = 0.0;
+  if (dd2 > eps * eps) {  // DD > EPS; psurr = pi / (k dd)^3
+    const double ri = rsqrt_pos(dd2);
+    psurr = M_PI / (ak0 * ak0 * ak0) * (ri * ri * ri);
+  }
+  double vr1 = 0, vr2 = 0;
+  if (akz <= -1.5E-6) {  // (the reference leaves values unset above)
+    if (akz > -16.0) {
+      double pd1x, pd2x;
+      if (akr < 99.7) {
+        const RWeights rw = r_weights(T, akr);
+        double t4[4];
+        interp4(T, rw, akz, kj_of(akz), t4);
+        pd1x = t4[0];
+        w.f1 = t4[1];
+        pd2x = t4[2];
+        w.f2 = t4[3];
+      } else {
+        const double epz = exp(akz);
+        const double akp4 = akr - 0.25 * M_PI;
+        const double sq = sqrt(2.0 * M_PI / akr);
+        double sik, csk;
+        sincos(akp4, &sik, &csk);
+        w.f1 = psurr * akz - M_PI * epz * sq * sik;
+        w.f2 = epz * sq * csk;
+        const double dsk = 0.5 / akr;
+        pd1x = -psurr * akr + M_PI * epz * sq * (csk - dsk * sik);
+        pd2x = epz * sq * (sik + dsk * csk);
+      }
+      w.z1s = w.f1 - psurr * akz;
+      w.z2s = w.f2;
+      vr1 = -pd1x - psurr * akr;
+      vr2 = -pd2x;
+    } else {
+      w.f1 = psurr * akz;
+    }
+  }
+  if (rrr > eps) {
+    const double rinv = 1.0 / rrr;
+    w.h1 = vr1 * rinv;
+    w.h2 = vr2 * rinv;
+  }
+  return w;
+}
+
+// Finite depth (VNSFD.f90:284-706, the submerged branch), per unit area.
+__device__ __forceinline__ WaveSym pair_wave_fd(const Tables& T, const FreqParams& P, double xpg, double ypg,
+                                                double zgi, double zgj) {
+  WaveSym w = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double am0 = P.ak0, h = P.depth;
+  const double eps = 0.0001;
+  const double rrr = sqrt(xpg * xpg + ypg * ypg);
+  const double akr = am0 * rrr;
+  // quantities shared by the four images (from the shallowest, at z_i + z_j)
+  RWeights rw = {2, 0, 0, 0};
+  double psk = 0, sck = 0, scds = 0, ssds = 0;
+  const double zz1 = zgi + zgj;
+  if (am0 * zz1 > -16.0) {
+    if (akr < 99.7) {
+      rw = r_weights(T, akr);
+    } else {
+      const double sq = sqrt(2.0 * M_PI / akr);
+      double sik, csk;
+      sincos(akr - 0.25 * M_PI, &sik, &csk);
+      psk = M_PI * sq * sik;
+      sck = sq * csk;
+      const double dsk = 0.5 / akr;
+      scds = M_PI * sq * (csk - dsk * sik);
+      ssds = sq * (sik + dsk * csk);
+    }
+  }
+  const double pi_k = M_PI / am0, pi_k3 = pi_k / (am0 * am0);
+  // images 0 and 3 keep their places under i <-> j, 1 and 2 trade them
+  const FdImage m0 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zz1, psk, sck, scds, ssds);
+  const FdImage m3 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, -zgj - zgi - 4.0 * h, psk, sck, scds, ssds);
+  double s1z = m0.pd1z + m3.pd1z, s2z = m0.pd2z + m3.pd2z, s1x = m0.pd1x + m3.pd1x, svr = m0.vr2 + m3.vr2;
+  double spsr = m0.psr + m3.psr;
+  const double a1 = m0.pd1z - m3.pd1z, a2 = m0.vz2 - m3.vz2;
+  const FdImage m1 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zgj - zgi - 2.0 * h, psk, sck, scds, ssds);
+  const FdImage m2 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zgi - zgj - 2.0 * h, psk, sck, scds, ssds);
+  s1z += m1.pd1z + m2.pd1z;
+  s2z += m1.pd2z + m2.pd2z;
+  s1x += m1.pd1x + m2.pd1x;
+  svr += m1.vr2 + m2.vr2;
+  spsr += m1.psr + m2.psr;
+  const double b1 = m1.pd1z - m2.pd1z, b2 = m1.vz2 - m2.vz2;
+  // exponential images as one-point sources: fs, the radial weight sum, and
+  // the vertical gradient of the images that keep their places (v12) and of
+  // those that trade them negated (v03)
+  const double act = -0.5 / (4.0 * M_PI);
+  const double r2 = rrr * rrr;
+  double fs = 0.0, asum = 0.0, v12 = 0.0, v03 = 0.0;
+  for (int ke = 0; ke < P.nexp1; ++ke) {
+    const double aqt = act * P.ar[ke];
+    const double hl = h * P.ambda[ke];
+    const double zp[4] = {zgi - 2.0 * h + hl - zgj, -zgi - hl - zgj, -zgi - 4.0 * h + hl - zgj,
+                          zgi + 2.0 * h - hl - zgj};
+    double a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double rr = r2 + zp[k] * zp[k];
+      const double ri = (rr > eps * eps) ? rsqrt_pos(rr) : 0.0;  // RO > 1e-4, else no term
+      const double f = aqt * ri;
+      a[k] = f * (ri * ri);
+      fs += f;
+      asum += a[k];
+    }
+    v12 = fma(zp[1], a[1], fma(zp[2], a[2], v12));
+    v03 = fma(-zp[0], a[0], fma(-zp[3], a[3], v03));
+  }
+  const double c3 = am0 * P.coe3, c4 = am0 * P.coe4;
+  w.f1 = P.coe3 * (s1z - spsr) + fs;
+  w.f2 = P.coe4 * s2z;
+  w.h1 = -asum;
+  if (rrr > eps) {
+    const double rinv = 1.0 / rrr;
+    w.h1 -= c3 * s1x * rinv;
+    w.h2 = c4 * svr * rinv;
+  }
+  w.z1s = c3 * a1 + v12;
+  w.z1a = v03 - c3 * b1;
+  w.z2s = c4 * a2;
+  w.z2a = -c4 * b2;
+  return w;
+}
+
 // All 16 outputs of one pair (order SP1 SM1 SP2 SM2, VS{X,Y,Z}P1,
 // VS{X,Y,Z}M1, VS{X,Y,Z}P2, VS{X,Y,Z}M2).
 // kFd: finite depth (P.fd == 1) or infinite depth, a compile-time choice so

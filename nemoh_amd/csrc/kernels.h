@@ -57,9 +57,11 @@ void launch_assemble_mixed(const Geometry& G, const Tables& T, const float4* tab
                            int nsym, const MatOut32& out, const int* surf, int nsurf, hipStream_t st);
 // fd: the regime of *P (finite depth 1, infinite 0), known to the host.
 // surf[0..nsurf) (device): the panels with zg >= ZER, whose rows and columns
-// take the free-surface branch of the wave part.
+// take the free-surface branch of the wave part.  paired: the wave part of
+// each unordered pair once for both elements (assemble_pair_kernel), else
+// once per element (assemble_kernel).
 void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, int fd, int nsym, const MatOut& out,
-                     const int* surf, int nsurf, hipStream_t st);
+                     const int* surf, int nsurf, bool paired, hipStream_t st);
 void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int fd, double* out, hipStream_t st);
 // Block LU without pivoting in place in A (n x n, leading dimension ld):
 // afterwards the blocks below the diagonal hold L (unit block lower), the

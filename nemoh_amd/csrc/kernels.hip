@@ -99,6 +99,146 @@ __global__ __launch_bounds__(256, 3) void assemble_kernel(Geometry G, Tables T, 
   store_pair<kSym, kF32>(G, out, i, j, o);
 }
 
+// ---------------------------------------------------------------------------
+// Paired assembly: the wave part of each unordered pair {i, j} once
+// (green.h, WaveSym), the Rankine part of each order (its near/far switch is
+// the source panel's 7 TDIS), both elements written.
+//
+// Each wavefront takes an 8 x 8 block (ti, tj), ti <= tj, of the 8-panel
+// blocks of the matrix: lane l holds i = 8 ti + l / 8, j = 8 tj + l % 8.
+// Element (i, j) and element (j, i) are then both written as eight 128-byte
+// row segments per store.  Waves take the blocks of the upper triangle in
+// row-major order; in a diagonal block (ti == tj) only the lanes with i <= j
+// work (i == j: the self term, one element).
+//
+// Each order's element is accumulated already projected on the field
+// point's normal (PairAcc: S+-, D+- = n . grad S+-): the Rankine part of the
+// order (rankine(), VAVINFD.f90:187-217), then the wave part of each y-image
+// (the combination VNSINFD.f90:455-503 / VNSFD.f90:830-864) scaled by the
+// order's source area.  Symmetric meshes (two y-images) hold more values
+// live and run at two waves per SIMD, without spills; the others at three.
+struct PairAcc {  // one order's element: S+ S- D+ D- (re, im)
+  double spr, spi, smr, smi, dpr, dpi, dmr, dmi;
+};
+
+// Rankine part of order (f, s) into its element (real parts).
+template <bool kFd, bool kSym>
+__device__ __forceinline__ void pair_rankine(const Geometry& G, const FreqParams& P, int f, int s, double nx,
+                                             double ny, double nz, PairAcc& a) {
+  double rk[8];
+  if (kFd) rankine(G, f, s, kSym ? 1 : 0, P.zer, 2, 2.0 * P.depth, rk);
+  else rankine(G, f, s, kSym ? 1 : 0, P.zer, 1, 0.0, rk);
+  a.spr = rk[0];
+  a.dpr = rk[2] * nx + rk[3] * ny + rk[4] * nz;
+  a.spi = a.dpi = 0.0;
+  if (kSym) {
+    a.smr = rk[1];
+    a.dmr = rk[5] * nx + rk[6] * ny + rk[7] * nz;
+    a.smi = a.dmi = 0.0;
+  }
+}
+
+// Wave part of one y-image into an order's element: sd = +1 for (i, j), -1
+// for (j, i); ey: the y offset of this image for this order; mb = +1 for the
+// direct y-image, -1 for the mirrored one (its sign in the - block).
+template <bool kSym>
+__device__ __forceinline__ void pair_wave_add(const WaveSym& w, double sd, double ex, double ey, double nx,
+                                              double ny, double nz, double pot1, double pot2, double grad1,
+                                              double grad2, double mb, PairAcc& a) {
+  const double s1 = pot1 * w.f1, s2 = pot2 * w.f2;
+  const double d1 = grad1 * (w.h1 * (ex * nx + ey * ny) + (w.z1s + sd * w.z1a) * nz);
+  const double d2 = grad2 * (w.h2 * (ex * nx + ey * ny) + (w.z2s + sd * w.z2a) * nz);
+  a.spr += s1;
+  a.spi += s2;
+  a.dpr += d1;
+  a.dpi += d2;
+  if (kSym) {
+    a.smr += mb * s1;
+    a.smi += mb * s2;
+    a.dmr += mb * d1;
+    a.dmi += mb * d2;
+  }
+}
+
+template <bool kSym, bool kF32>
+__device__ __forceinline__ void store_acc(const MatOut& out, int f, int s, const PairAcc& a) {
+  const size_t e = (size_t)f * out.ld + s, e32 = (size_t)f * out.ld32 + s;
+  out.dp[e] = make_double2(a.dpr, a.dpi);
+  out.sp[e] = make_double2(a.spr, a.spi);
+  if (kF32) out.dp32[e32] = make_float2((float)a.dpr, (float)a.dpi);
+  if (kSym) {
+    out.dm[e] = make_double2(a.dmr, a.dmi);
+    out.sm[e] = make_double2(a.smr, a.smi);
+    if (kF32) out.dm32[e32] = make_float2((float)a.dmr, (float)a.dmi);
+  }
+}
+
+// Block (ti, tj) of wave number g in the row-major upper triangle of an
+// nt x nt block grid (row t starts at t nt - t (t - 1) / 2).
+__device__ __forceinline__ void tri_block(int g, int nt, int& ti, int& tj) {
+  const double b = 2.0 * nt + 1.0;
+  int t = (int)((b - sqrt(b * b - 8.0 * g)) * 0.5);
+  t = max(0, min(t, nt - 1));
+  while (t > 0 && (long long)t * nt - (long long)t * (t - 1) / 2 > g) --t;
+  while (t + 1 < nt && (long long)(t + 1) * nt - (long long)(t + 1) * t / 2 <= g) ++t;
+  ti = t;
+  tj = t + (g - (int)((long long)t * nt - (long long)t * (t - 1) / 2));
+}
+
+template <bool kFd, bool kSym, bool kF32>
+__global__ __launch_bounds__(256, kSym ? 2 : 3) void assemble_pair_kernel(Geometry G, Tables T, const FreqParams* __restrict__ Pp,
+                                                               MatOut out, int nt, int nwaves) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= nwaves) return;
+  int ti, tj;
+  tri_block(g, nt, ti, tj);
+  const int lane = threadIdx.x & 63;
+  const int i = 8 * ti + (lane >> 3), j = 8 * tj + (lane & 7);
+  if (i >= G.n || j >= G.n || (ti == tj && i > j)) return;
+  const FreqParams& P = *Pp;
+  const bool two = i != j;  // the self term is one element
+  PairAcc a0, a1;
+  pair_rankine<kFd, kSym>(G, P, i, j, G.xn[i], G.yn[i], G.zn[i], a0);
+  if (two) pair_rankine<kFd, kSym>(G, P, j, i, G.xn[j], G.yn[j], G.zn[j], a1);
+  const double xi = G.xg[i], yi = G.yg[i], zi = G.zg[i];
+  const double xj = G.xg[j], yj = G.yg[j], zj = G.zg[j];
+  // a pair with a point at the free surface keeps a zero wave part here; it
+  // is overwritten by assemble_surface_kernel
+  if (zi < P.zer && zj < P.zer) {
+    const double dx = xi - xj, dy1 = yi - yj, dy2 = yi + yj;
+    const double ai = G.aire[i], aj = G.aire[j];
+    double p1[2], p2[2], g1[2], g2[2];  // scaling by the order's source area: [0] (i, j), [1] (j, i)
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      const double aire = d ? ai : aj;
+      if (kFd) {
+        p1[d] = p2[d] = g1[d] = g2[d] = aire;
+      } else {
+        const double akair = P.ak0 * aire;
+        p1[d] = -akair / (2.0 * M_PI * M_PI);
+        p2[d] = -akair / (2.0 * M_PI);
+        g1[d] = p1[d] * P.ak0;
+        g2[d] = p2[d] * P.ak0;
+      }
+    }
+    const double nxi = G.xn[i], nyi = G.yn[i], nzi = G.zn[i];
+    const double nxj = G.xn[j], nyj = G.yn[j], nzj = G.zn[j];
+    {
+      const WaveSym w = kFd ? pair_wave_fd(T, P, dx, dy1, zi, zj) : pair_wave_infd(T, P, dx, dy1, zi, zj);
+      pair_wave_add<kSym>(w, 1.0, dx, dy1, nxi, nyi, nzi, p1[0], p2[0], g1[0], g2[0], 1.0, a0);
+      pair_wave_add<kSym>(w, -1.0, -dx, -dy1, nxj, nyj, nzj, p1[1], p2[1], g1[1], g2[1], 1.0, a1);
+    }
+    if (kSym) {
+      const WaveSym w = kFd ? pair_wave_fd(T, P, dx, dy2, zi, zj) : pair_wave_infd(T, P, dx, dy2, zi, zj);
+      pair_wave_add<kSym>(w, 1.0, dx, dy2, nxi, nyi, nzi, p1[0], p2[0], g1[0], g2[0], -1.0, a0);
+      pair_wave_add<kSym>(w, -1.0, -dx, dy2, nxj, nyj, nzj, p1[1], p2[1], g1[1], g2[1], -1.0, a1);
+    }
+  }
+  store_acc<kSym, kF32>(out, i, j, a0);
+  if (two) store_acc<kSym, kF32>(out, j, i, a1);
+}
+
+
 // The pairs with a point at the free surface: blockIdx.z = 0 takes the rows
 // of the listed panels (every j), blockIdx.z = 1 their columns (every i that
 // is not itself listed, so that no element is written twice).
@@ -627,29 +767,34 @@ void launch_crek(const double* xr, const double* xz, double* tab, hipStream_t st
 
 template <bool kFd, bool kSym, bool kF32>
 static void launch_assemble_variant(const Geometry& G, const Tables& T, const FreqParams* P, const MatOut& out,
-                                    const int* surf, int nsurf, hipStream_t st) {
+                                    const int* surf, int nsurf, bool paired, hipStream_t st) {
   const int nx = (G.n + 255) / 256;
-  assemble_kernel<kFd, kSym, kF32><<<dim3(nx, G.n), 256, 0, st>>>(G, T, P, out);
+  if (paired) {
+    const int nt = (G.n + 7) / 8, nwaves = nt * (nt + 1) / 2;
+    assemble_pair_kernel<kFd, kSym, kF32><<<(nwaves + 3) / 4, 256, 0, st>>>(G, T, P, out, nt, nwaves);
+  } else {
+    assemble_kernel<kFd, kSym, kF32><<<dim3(nx, G.n), 256, 0, st>>>(G, T, P, out);
+  }
   if (nsurf > 0) assemble_surface_kernel<kFd, kSym, kF32><<<dim3(nx, nsurf, 2), 256, 0, st>>>(G, T, P, out, surf);
 }
 
 template <bool kFd>
 static void launch_assemble_regime(const Geometry& G, const Tables& T, const FreqParams* P, int nsym,
-                                   const MatOut& out, const int* surf, int nsurf, hipStream_t st) {
+                                   const MatOut& out, const int* surf, int nsurf, bool paired, hipStream_t st) {
   const bool f32 = out.dp32 != nullptr;
   if (nsym) {
-    if (f32) launch_assemble_variant<kFd, true, true>(G, T, P, out, surf, nsurf, st);
-    else launch_assemble_variant<kFd, true, false>(G, T, P, out, surf, nsurf, st);
+    if (f32) launch_assemble_variant<kFd, true, true>(G, T, P, out, surf, nsurf, paired, st);
+    else launch_assemble_variant<kFd, true, false>(G, T, P, out, surf, nsurf, paired, st);
   } else {
-    if (f32) launch_assemble_variant<kFd, false, true>(G, T, P, out, surf, nsurf, st);
-    else launch_assemble_variant<kFd, false, false>(G, T, P, out, surf, nsurf, st);
+    if (f32) launch_assemble_variant<kFd, false, true>(G, T, P, out, surf, nsurf, paired, st);
+    else launch_assemble_variant<kFd, false, false>(G, T, P, out, surf, nsurf, paired, st);
   }
 }
 
 void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, int fd, int nsym, const MatOut& out,
-                     const int* surf, int nsurf, hipStream_t st) {
-  if (fd) launch_assemble_regime<true>(G, T, P, nsym, out, surf, nsurf, st);
-  else launch_assemble_regime<false>(G, T, P, nsym, out, surf, nsurf, st);
+                     const int* surf, int nsurf, bool paired, hipStream_t st) {
+  if (fd) launch_assemble_regime<true>(G, T, P, nsym, out, surf, nsurf, paired, st);
+  else launch_assemble_regime<false>(G, T, P, nsym, out, surf, nsurf, paired, st);
 }
 
 template <bool kFd, bool kSym>

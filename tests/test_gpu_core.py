@@ -163,9 +163,10 @@ def _lidcyl(isym):
     return np.vstack([v, vm]), np.vstack([p, pm]), None, float(d["depth_fd"])
 
 
+@pytest.mark.parametrize("paired", [1, 0])
 @pytest.mark.parametrize("isym", [1, 0])
 @pytest.mark.parametrize("prec", [core.FP64, core.MIXED])
-def test_free_surface_panels_match_oracle(isym, prec):
+def test_free_surface_panels_match_oracle(isym, prec, paired):
     """Panels at the free surface (the lid at z = 0), whose pairs take the
     other branch of the wave part and are assembled by a launch of their own:
     a finite-depth frequency, then an infinite-depth one (k h >= 20 at the
@@ -176,6 +177,7 @@ def test_free_surface_panels_match_oracle(isym, prec):
     v, p, geo, depth = _lidcyl(isym)
     c = core.Core(v, p, isym, depth, 1025.0, 9.81, geometry=geo, tables=_tables())
     c.set_precision(prec)
+    c.set_option(core.OPT_PAIRED_ASSEMBLY, paired)
     oc = po.Case(v, p, isym, depth=depth, rho=1025.0, g=9.81, tables=_tables(), geometry=geo)
     assert np.sum(oc.geo["zg"] >= oc.geo["zer"]) >= 6
     s = oc.session()
@@ -192,6 +194,31 @@ def test_free_surface_panels_match_oracle(isym, prec):
         for a, b in zip(got, (zigb, zigs, zpb, zps)):
             assert np.abs(a - b).max() <= tol * scale, (period, np.abs(a - b).max() / scale)
     c.close()
+
+
+@pytest.mark.parametrize("name", ["Cylinder", "cyl512", "2Bodies", "barge_fd", "c4sph", "barge4000"])
+def test_paired_assembly_matches_reference_and_unpaired(name):
+    """The paired assembly (the wave part once per unordered panel pair, the
+    default) and the one-element-at-a-time assembly, both against the
+    reference forces, and against each other (<= 1e-11: the paired form
+    sums the images' symmetric and antisymmetric parts separately, and the
+    solve amplifies that regrouping's rounding, up to 5e-12 on 2Bodies).  Cases: infinite depth with y-symmetry (Cylinder,
+    cyl512), finite depth with y-symmetry (2Bodies), finite depth without
+    (barge_fd, and the 4000-panel barge, whose last frequency is infinite
+    depth), infinite depth without (c4sph)."""
+    d, m, ref = _case(name)
+    got = []
+    for paired in (1, 0):
+        s = Solver(m, float(d["depth"]), float(d["rho"]), float(d["g"]), precision=core.FP64, tables=_tables())
+        s.core.set_option(core.OPT_PAIRED_ASSEMBLY, paired)
+        got.append(s.solve_problems(d["omega"], d["nvel"], d["nds"]))
+        s.close()
+        err = _normwise(got[-1], ref)
+        print(f"{name}: paired {paired}: FP64 max normwise rel err {err:.3e}")
+        assert err < FP64_TOL
+    diff = _normwise(got[0], got[1])
+    print(f"{name}: paired vs unpaired {diff:.3e}")
+    assert diff < 1e-11
 
 
 def test_omega_below_threshold_is_an_error():
