@@ -310,6 +310,8 @@ int build_params(nemoh_core* c, double period, double amh, int nexp, const doubl
   P.depth = c->depth;
   if (!fd) {
     P.ak0 = w * w / c->g;
+    P.pik = M_PI / P.ak0;
+    P.pik3 = M_PI / (P.ak0 * P.ak0 * P.ak0);
     if (nq < 0 || nq > 101 || (nq > 0 && (!qq || !cq))) return fail(c, NEMOH_ERR_ARG, "nq out of range");
     P.nq = nq;
     for (int k = 0; k < nq; ++k) {
@@ -324,6 +326,8 @@ int build_params(nemoh_core* c, double period, double amh, int nexp, const doubl
     const double am0 = amh / h;
     const double adpi2 = -a / (8. * pi * pi), adpi = -a / (8 * pi);
     P.ak0 = am0;
+    P.pik = M_PI / am0;
+    P.pik3 = P.pik / (am0 * am0);
     P.coe1 = adpi2 / am0;
     P.coe2 = adpi / am0;
     P.coe3 = adpi2;

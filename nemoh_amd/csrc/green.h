@@ -33,6 +33,7 @@ struct FreqParams {
   double depth;    // h (FD)
   double zer;      // ZER = -1e-5 * max extent
   double coe1, coe2, coe3, coe4;  // FD scaling constants (VNSFD:264-269)
+  double pik, pik3;               // pi / k and pi / k^3 (k = ak0), for the images' 1/r terms
   double ambda[32], ar[32];       // FD: exponents / amplitudes, [nexp] = (0, 2)
   double qq[101], cq[101];        // theta rule
 };
@@ -100,6 +101,16 @@ __device__ __forceinline__ double rsqrt_pos(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   const double e = fma(-x * y, y, 1.0);
   return fma(0.5 * y, e, y);
+}
+
+// 1/x for x > 0: the hardware estimate (v_rcp_f64) refined by two Newton
+// steps y <- y + y (1 - x y), which leave the rounding of the last step.
+__device__ __forceinline__ double recip_pos(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
 }
 
 // Complex exponential-integral approximation GG(z) ~ e^z E1(z) +- i pi e^z,
@@ -197,10 +208,12 @@ __device__ __forceinline__ RankineImage rankine_near(const Geometry& G, int j, d
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
     const double ex = xo - tx[l], ey = yo - ty[l], ez = zo - tz[l];
-    rr[l] = sqrt(ex * ex + ey * ey + ez * ez);
-    drx[l] = ex / rr[l];
-    dry[l] = ey / rr[l];
-    drz[l] = ez / rr[l];
+    const double e2 = ex * ex + ey * ey + ez * ez;
+    const double ri = rsqrt_pos(e2);
+    rr[l] = e2 * ri;
+    drx[l] = ex * ri;
+    dry[l] = ey * ri;
+    drz[l] = ez * ri;
   }
   rr[4] = rr[0];
   drx[4] = drx[0];
@@ -213,9 +226,11 @@ __device__ __forceinline__ RankineImage rankine_near(const Geometry& G, int j, d
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
     const double ex = tx[l + 1] - tx[l], ey = ty[l + 1] - ty[l], ez = tz[l + 1] - tz[l];
-    const double dk = sqrt(ex * ex + ey * ey + ez * ez);
+    const double dk2 = ex * ex + ey * ey + ez * ez;
+    const double idk = rsqrt_pos(dk2);
+    const double dk = dk2 * idk;  // (NaN for a degenerate edge, which the test below skips)
     if (!(dk >= 1.E-3 * tdis)) continue;
-    const double pj = ex / dk, qj = ey / dk, rj = ez / dk;
+    const double pj = ex * idk, qj = ey * idk, rj = ez * idk;
     const double gyx = a6j * rj - a9j * qj;
     const double gyy = a9j * pj - a3j * rj;
     const double gyz = a3j * qj - a6j * pj;
@@ -238,10 +253,11 @@ __device__ __forceinline__ RankineImage rankine_near(const Geometry& G, int j, d
     const double dntz = dr * anlz + a9j * ds;
     const double denl = anl * dnl;
     const double dent = ant * ant + dnt * dnt;
-    const double atx = (gyx * ddk * dnt - dntx * ant) / dent;
-    const double aty = (gyy * ddk * dnt - dnty * ant) / dent;
-    const double atz = (gyz * ddk * dnt - dntz * ant) / dent;
-    const double dlogg = (dnl - anl) / denl;
+    const double rdent = recip_pos(dent);
+    const double atx = (gyx * ddk * dnt - dntx * ant) * rdent;
+    const double aty = (gyy * ddk * dnt - dnty * ant) * rdent;
+    const double atz = (gyz * ddk * dnt - dntz * ant) * rdent;
+    const double dlogg = (dnl - anl) * recip_pos(denl);
     vxs = vxs + gyx * alden + gy * anlx * dlogg - 2.0 * agz * atx - dat * a3j;
     vys = vys + gyy * alden + gy * anly * dlogg - 2.0 * agz * aty - dat * a6j;
     vzs = vzs + gyz * alden + gy * anlz * dlogg - 2.0 * agz * atz - dat * a9j;
@@ -292,23 +308,23 @@ __device__ __forceinline__ void rankine(const Geometry& G, int i, int j, int nsy
   // images: 1 direct, 2 z-mirror (mjj -1), 3 y+z mirror (+1), 4 y-mirror (-1)
   const RankineImage r1 = rankine_image(G, j, xo, yo, zo, 1.0, 1.0, dh, 0.0, 1.0, i == j);
   const RankineImage r2 = rankine_image(G, j, xo, yo, zo, 1.0, -1.0, dh, 1.0, -1.0, false);
-  const double qpi = 4.0 * M_PI;
+  const double mq = -1.0 / (4.0 * M_PI);  // the images sum times -1/(4 pi)
   if (nsym == 1) {
     const RankineImage r3 = rankine_image(G, j, xo, yo, zo, -1.0, -1.0, dh, 1.0, 1.0, false);
     const RankineImage r4 = rankine_image(G, j, xo, yo, zo, -1.0, 1.0, dh, 0.0, -1.0, false);
-    out[0] = -(r1.a - mk * (r2.a + r3.a) + r4.a) / qpi;
-    out[1] = -(r1.a - mk * (r2.a - r3.a) - r4.a) / qpi;
-    out[2] = -(r1.vx - mk * (r2.vx + r3.vx) + r4.vx) / qpi;
-    out[3] = -(r1.vy - mk * (r2.vy + r3.vy) + r4.vy) / qpi;
-    out[4] = -(r1.vz - mk * (r2.vz + r3.vz) + r4.vz) / qpi;
-    out[5] = -(r1.vx - mk * (r2.vx - r3.vx) - r4.vx) / qpi;
-    out[6] = -(r1.vy - mk * (r2.vy - r3.vy) - r4.vy) / qpi;
-    out[7] = -(r1.vz - mk * (r2.vz - r3.vz) - r4.vz) / qpi;
+    out[0] = (r1.a - mk * (r2.a + r3.a) + r4.a) * mq;
+    out[1] = (r1.a - mk * (r2.a - r3.a) - r4.a) * mq;
+    out[2] = (r1.vx - mk * (r2.vx + r3.vx) + r4.vx) * mq;
+    out[3] = (r1.vy - mk * (r2.vy + r3.vy) + r4.vy) * mq;
+    out[4] = (r1.vz - mk * (r2.vz + r3.vz) + r4.vz) * mq;
+    out[5] = (r1.vx - mk * (r2.vx - r3.vx) - r4.vx) * mq;
+    out[6] = (r1.vy - mk * (r2.vy - r3.vy) - r4.vy) * mq;
+    out[7] = (r1.vz - mk * (r2.vz - r3.vz) - r4.vz) * mq;
   } else {
-    out[0] = out[1] = -(r1.a - mk * r2.a) / qpi;
-    out[2] = out[5] = -(r1.vx - mk * r2.vx) / qpi;
-    out[3] = out[6] = -(r1.vy - mk * r2.vy) / qpi;
-    out[4] = out[7] = -(r1.vz - mk * r2.vz) / qpi;
+    out[0] = out[1] = (r1.a - mk * r2.a) * mq;
+    out[2] = out[5] = (r1.vx - mk * r2.vx) * mq;
+    out[3] = out[6] = (r1.vy - mk * r2.vy) * mq;
+    out[4] = out[7] = (r1.vz - mk * r2.vz) * mq;
   }
 }
 
@@ -412,7 +428,7 @@ __device__ WavePart wave_infd_image(const Geometry& G, const Tables& T, const Fr
     double psurr = 0.0;
     if (dd2 > eps * eps) {  // DD > EPS; psurr = pi / (k dd)^3
       const double ri = rsqrt_pos(dd2);
-      psurr = M_PI / (ak0 * ak0 * ak0) * (ri * ri * ri);
+      psurr = P.pik3 * (ri * ri * ri);
     }
     double pd1z = 0, pd2z = 0, vz1 = 0, vz2 = 0, vr1 = 0, vr2 = 0;
     if (akz <= -1.5E-6) {  // (the reference leaves values unset above)
@@ -699,7 +715,7 @@ __device__ WavePart wave_fd_image(const Geometry& G, const Tables& T, const Freq
         ssds = sq * (sik + dsk * csk);
       }
     }
-    const double pi_k = M_PI / am0, pi_k3 = pi_k / (am0 * am0);
+    const double pi_k = P.pik, pi_k3 = P.pik3;
     // the four images' terms are summed as each image is formed (left to
     // right, as the sums are written in VNSFD.f90:548-600), so that only the
     // running sums stay live, not four images' worth of values
@@ -822,7 +838,7 @@ __device__ __forceinline__ WaveSym pair_wave_infd(const Tables& T, const FreqPar
   double psurr = 0.0;
   if (dd2 > eps * eps) {  // DD > EPS; psurr = pi / (k dd)^3
     const double ri = rsqrt_pos(dd2);
-    psurr = M_PI / (ak0 * ak0 * ak0) * (ri * ri * ri);
+    psurr = P.pik3 * (ri * ri * ri);
   }
   double vr1 = 0, vr2 = 0;
   if (akz <= -1.5E-6) {  // (the reference leaves values unset above)
@@ -857,7 +873,7 @@ __device__ __forceinline__ WaveSym pair_wave_infd(const Tables& T, const FreqPar
     }
   }
   if (rrr > eps) {
-    const double rinv = 1.0 / rrr;
+    const double rinv = rsqrt_pos(ddx * ddx + ddy * ddy);
     w.h1 = vr1 * rinv;
     w.h2 = vr2 * rinv;
   }
@@ -890,7 +906,7 @@ __device__ __forceinline__ WaveSym pair_wave_fd(const Tables& T, const FreqParam
       ssds = sq * (sik + dsk * csk);
     }
   }
-  const double pi_k = M_PI / am0, pi_k3 = pi_k / (am0 * am0);
+  const double pi_k = P.pik, pi_k3 = P.pik3;
   // images 0 and 3 keep their places under i <-> j, 1 and 2 trade them
   const FdImage m0 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, zz1, psk, sck, scds, ssds);
   const FdImage m3 = fd_image(T, rw, am0, pi_k, pi_k3, rrr, akr, -zgj - zgi - 4.0 * h, psk, sck, scds, ssds);
@@ -910,12 +926,13 @@ __device__ __forceinline__ WaveSym pair_wave_fd(const Tables& T, const FreqParam
   // those that trade them negated (v03)
   const double act = -0.5 / (4.0 * M_PI);
   const double r2 = rrr * rrr;
+  const double dz = zgi - zgj, sz = -zgi - zgj;
   double fs = 0.0, asum = 0.0, v12 = 0.0, v03 = 0.0;
   for (int ke = 0; ke < P.nexp1; ++ke) {
     const double aqt = act * P.ar[ke];
     const double hl = h * P.ambda[ke];
-    const double zp[4] = {zgi - 2.0 * h + hl - zgj, -zgi - hl - zgj, -zgi - 4.0 * h + hl - zgj,
-                          zgi + 2.0 * h - hl - zgj};
+    const double c = hl - 2.0 * h;  // the same for every lane
+    const double zp[4] = {dz + c, sz - hl, sz + (c - 2.0 * h), dz - c};
     double a[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -934,7 +951,7 @@ __device__ __forceinline__ WaveSym pair_wave_fd(const Tables& T, const FreqParam
   w.f2 = P.coe4 * s2z;
   w.h1 = -asum;
   if (rrr > eps) {
-    const double rinv = 1.0 / rrr;
+    const double rinv = rsqrt_pos(xpg * xpg + ypg * ypg);
     w.h1 -= c3 * s1x * rinv;
     w.h2 = c4 * svr * rinv;
   }

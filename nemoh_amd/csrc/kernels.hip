@@ -215,8 +215,8 @@ __global__ __launch_bounds__(256, kSym ? 2 : 3) void assemble_pair_kernel(Geomet
         p1[d] = p2[d] = g1[d] = g2[d] = aire;
       } else {
         const double akair = P.ak0 * aire;
-        p1[d] = -akair / (2.0 * M_PI * M_PI);
-        p2[d] = -akair / (2.0 * M_PI);
+        p1[d] = akair * (-1.0 / (2.0 * M_PI * M_PI));
+        p2[d] = akair * (-1.0 / (2.0 * M_PI));
         g1[d] = p1[d] * P.ak0;
         g2[d] = p2[d] * P.ak0;
       }
