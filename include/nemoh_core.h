@@ -264,6 +264,32 @@ double nemoh_core_incident_wavenumber(double w, double depth, double g);
 int nemoh_core_theta_rule(double akk, double *qq, double *cq);
 int nemoh_core_prony(double akh, double amh, int *nexp, double *ambda, double *ar);
 
+/* Batches (no counterpart in the reference, whose Solver runs one body per
+ * process: Solver/NEMOH.f90:129-143 loops over the problems of one mesh).
+ * Many independent bodies, each an ordinary handle (its mesh, tables and
+ * degrees of freedom set), solved together one frequency at a time: the
+ * assembly of every body is one launch, every step of the block LU one
+ * launch over all matrices that have it (max_i ceil(nfa_i / 128) block
+ * pairs of launches instead of the sum), each triangular solve one launch,
+ * with the light per-body kernels on the same stream.  Every body's results
+ * equal those of nemoh_core_solve_waves on its handle alone.  The handles
+ * stay the caller's, must outlive the batch and are not to be used by other
+ * threads during a batch call.  All bodies share nsym, depth and g, solve
+ * the same number of problems (headings + radiation dofs <= max_rhs) and
+ * take the batch's precision (FP32 assembly is not batched). */
+typedef struct nemoh_batch nemoh_batch;
+int nemoh_batch_create(nemoh_batch **batch, int nbody, nemoh_core *const *bodies, int precision, int max_rhs);
+/* One frequency for every body: body i's forces (nforce_i x (nbeta +
+ * nrad_i), column-major complex) at forces + 2 * offset_i, offset_i = the
+ * sum of nforce_j (nbeta + nrad_j) over j < i; fk likewise (nforce_i x
+ * nbeta); either may be NULL.  A body whose refinement needs more sweeps, or
+ * whose chained solve gave up waiting, is solved again on its own handle
+ * from the factors already on the device. */
+int nemoh_batch_solve_waves(nemoh_batch *batch, double omega, int nbeta, const double *beta, double *forces,
+                            double *fk);
+const char *nemoh_batch_last_error(const nemoh_batch *batch);
+void nemoh_batch_destroy(nemoh_batch *batch);
+
 /* Diagnostics used by the tests: the 16 Green-function values of every
  * (ISP, IFP) pair (the VAV*+VNS* outputs SP1 SM1 SP2 SM2 VSXP1 VSYP1 VSZP1
  * VSXM1 VSYM1 VSZM1 VSXP2 VSYP2 VSZP2 VSXM2 VSYM2 VSZM2, COM_VAR.f90:85-90)

@@ -31,7 +31,8 @@ EXPORTS = (
     "nemoh_core_mfma_probe", "nemoh_core_get_extent", "nemoh_core_set_dofs", "nemoh_core_body_conditions",
     "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_core_kochin_noalloc", "nemoh_core_potential", "nemoh_server_create", "nemoh_server_submit",
     "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_memory", "nemoh_server_last_error",
-    "nemoh_server_destroy",
+    "nemoh_server_destroy", "nemoh_batch_create", "nemoh_batch_solve_waves", "nemoh_batch_last_error",
+    "nemoh_batch_destroy",
 )
 
 
@@ -131,6 +132,12 @@ def load():
     L.nemoh_server_last_error.restype = C.c_char_p
     L.nemoh_server_destroy.argtypes = [C.c_void_p]
     L.nemoh_server_destroy.restype = None
+    L.nemoh_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p), C.c_int, C.c_int]
+    L.nemoh_batch_solve_waves.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp]
+    L.nemoh_batch_last_error.argtypes = [C.c_void_p]
+    L.nemoh_batch_last_error.restype = C.c_char_p
+    L.nemoh_batch_destroy.argtypes = [C.c_void_p]
+    L.nemoh_batch_destroy.restype = None
     _lib = L
     return L
 
@@ -403,3 +410,50 @@ class Core:
         self._check(load().nemoh_core_pairs(self.h, float(period), amh, nexp, _p(ambda), _p(ar), nq, _p(qq), _p(cq),
                                             _p(out)))
         return out.reshape(self.nfa, self.nfa, 16)
+
+
+class Batch:
+    """Many bodies solved together (nemoh_batch_*): each heavy step of a
+    frequency is one launch over every body.  ``cores``: Core handles with
+    their degrees of freedom set; they must stay open while the batch
+    exists."""
+
+    def __init__(self, cores, precision: int = FP64, max_rhs: int = 16):
+        L = load()
+        self.cores = list(cores)
+        arr = (C.c_void_p * len(self.cores))(*[c.h for c in self.cores])
+        h = C.c_void_p()
+        r = L.nemoh_batch_create(C.byref(h), len(self.cores), arr, int(precision), int(max_rhs))
+        if r:
+            raise NemohError(r, L.nemoh_batch_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().nemoh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def solve_waves(self, omega: float, betas):
+        """Per body: forces (nforce, nbeta + nrad) and Froude-Krylov forces
+        (nforce, nbeta), as Core.solve_waves gives them."""
+        b = _f64(np.atleast_1d(betas)) if len(betas) else np.zeros(0)
+        nb = b.size
+        shapes = [(c._nforce, nb + c._nrad) for c in self.cores]
+        total = sum(a * q for a, q in shapes)
+        ktotal = sum(c._nforce * nb for c in self.cores)
+        f = np.zeros(max(total, 1), np.complex128)
+        k = np.zeros(max(ktotal, 1), np.complex128)
+        r = load().nemoh_batch_solve_waves(self.h, float(omega), nb, _p(b) if nb else None, f.ctypes.data_as(_dp),
+                                           k.ctypes.data_as(_dp))
+        if r:
+            raise NemohError(r, load().nemoh_batch_last_error(self.h).decode())
+        out, fo, ko = [], 0, 0
+        for c, (a, q) in zip(self.cores, shapes):
+            out.append((f[fo:fo + a * q].reshape((a, q), order="F"), k[ko:ko + a * nb].reshape((a, nb), order="F")))
+            fo += a * q
+            ko += a * nb
+            c._last_nrhs = q
+        return out

@@ -1593,3 +1593,329 @@ int nemoh_core_mfma_probe(int fp32, double* out) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many independent bodies (one handle each) solved together, each
+// heavy step one launch over all of them -- the paired assembly, every step
+// of the block LU (max_i ceil(n_i / 128) block pairs of launches instead of
+// the sum), the chained triangular solves -- with the light per-body kernels
+// (body conditions, right-hand sides, refinement, phi, forces) queued on the
+// same stream.  No counterpart in the reference, which solves one body per
+// run; every body gets the arithmetic of its own handle's path, so its
+// results equal a solve on its handle alone, bit for bit.
+struct nemoh_batch {
+  std::vector<nemoh_core*> bodies;
+  int prec = NEMOH_FP64, nsym = 0, nb = 1, max_rhs = 0;
+  hipStream_t st = nullptr;
+  // job tables, built for the bodies' current buffers (layout_gen of each
+  // body when built) and for the right-hand-side count they were built for
+  std::vector<unsigned> gens;
+  int built_rhs = -1;
+  AsmJob* d_asm = nullptr;
+  int asm_count = 0, asm_grid = 0;
+  LuGroup<float> lu32;
+  LuGroup<double> lu64;
+  TrsvGroup<float> tr32[2];
+  TrsvGroup<double> tr64[2];
+  std::string err;
+};
+
+namespace {
+
+int batch_fail(nemoh_batch* b, int code, const std::string& msg) {
+  b->err = msg;
+  return code;
+}
+
+void batch_free_tables(nemoh_batch* b) {
+  if (b->d_asm) (void)hipFree(b->d_asm);
+  b->d_asm = nullptr;
+  for (auto* p : {&b->lu32.d_jobs}) if (*p) (void)hipFree(*p), *p = nullptr;
+  for (auto* p : {&b->lu64.d_jobs}) if (*p) (void)hipFree(*p), *p = nullptr;
+  for (int k = 0; k < 2; ++k) {
+    for (auto* p : {&b->tr32[k].d_begin, &b->tr32[k].d_tri}) if (*p) (void)hipFree(*p), *p = nullptr;
+    for (auto* p : {&b->tr64[k].d_begin, &b->tr64[k].d_tri}) if (*p) (void)hipFree(*p), *p = nullptr;
+  }
+  b->gens.clear();
+  b->built_rhs = -1;
+}
+
+// The influence-matrix outputs of a handle's assembly (as queue_factor).
+MatOut mat_out(const nemoh_core* c) {
+  MatOut out{c->nfa, c->D[0], c->S[0], c->D[1], c->S[1], nullptr, nullptr, 0};
+  if (c->prec == NEMOH_MIXED) {
+    out.dp32 = c->D32[0];
+    out.dm32 = c->D32[1];
+    out.ld32 = c->ld32;
+  }
+  return out;
+}
+
+// Job tables of the batch for its bodies' current buffers and nrhs problems
+// each (device lock held: allocations).
+int batch_build(nemoh_batch* b, const std::vector<int>& nrhs) {
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
+  batch_free_tables(b);
+  std::vector<AsmJob> asmj;
+  std::vector<LuJob<float>> m32;
+  std::vector<LuJob<double>> m64;
+  std::vector<TrsvJob<float>> t32[2];
+  std::vector<TrsvJob<double>> t64[2];
+  int wg = 0;
+  for (size_t i = 0; i < b->bodies.size(); ++i) {
+    nemoh_core* c = b->bodies[i];
+    const int nt = (c->nfa + 7) / 8, nw = nt * (nt + 1) / 2;
+    AsmJob a{c->G, c->T, c->d_par, mat_out(c), nt, nw, wg, wg + (nw + 3) / 4};
+    wg = a.wg1;
+    asmj.push_back(a);
+    ChainCtl* ctl = reinterpret_cast<ChainCtl*>(c->d_chain);
+    for (int k = 0; k < c->nb; ++k) {
+      if (b->prec == NEMOH_MIXED) {
+        auto* A = reinterpret_cast<C2<float>*>(c->D32[k]);
+        auto* P = reinterpret_cast<C2<float>*>(c->Pinv32[k]);
+        m32.push_back(LuJob<float>{A, P, c->d_info, c->ld32, c->nfa, 0, 0, 0, PanelJob{0, 0, 0, 0, 0, 0}});
+        t32[k].push_back(TrsvJob<float>{A, P, reinterpret_cast<C2<float>*>(c->R32),
+                                        reinterpret_cast<C2<float>*>(c->W32), ctl, c->d_info + 1, c->d_rs + k,
+                                        c->ld32, c->nfa, nrhs[i], 0, 0});
+      } else {
+        auto* A = reinterpret_cast<C2<double>*>(c->D[k]);
+        auto* P = reinterpret_cast<C2<double>*>(c->Pinv[k]);
+        m64.push_back(LuJob<double>{A, P, c->d_info, c->nfa, c->nfa, 0, 0, 0, PanelJob{0, 0, 0, 0, 0, 0}});
+        t64[k].push_back(TrsvJob<double>{A, P, reinterpret_cast<C2<double>*>(c->Z[k]),
+                                         reinterpret_cast<C2<double>*>(c->W), ctl, c->d_info + 1, nullptr, c->nfa,
+                                         c->nfa, nrhs[i], 0, 0});
+      }
+    }
+  }
+  hipError_t e = hipMalloc((void**)&b->d_asm, std::max<size_t>(1, asmj.size()) * sizeof(AsmJob));
+  if (e == hipSuccess) e = hipMemcpy(b->d_asm, asmj.data(), asmj.size() * sizeof(AsmJob), hipMemcpyHostToDevice);
+  b->asm_count = (int)asmj.size();
+  b->asm_grid = wg;
+  if (e == hipSuccess) e = b->prec == NEMOH_MIXED ? build_lu_group(m32, &b->lu32) : build_lu_group(m64, &b->lu64);
+  for (int k = 0; k < b->nb && e == hipSuccess; ++k)
+    e = b->prec == NEMOH_MIXED ? build_trsv_group(t32[k], &b->tr32[k]) : build_trsv_group(t64[k], &b->tr64[k]);
+  if (e != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, std::string("batch tables: ") + hipGetErrorString(e));
+  for (nemoh_core* c : b->bodies) b->gens.push_back(c->layout_gen);
+  b->built_rhs = nrhs.empty() ? 0 : nrhs[0];
+  return NEMOH_OK;
+}
+
+bool batch_stale(const nemoh_batch* b, const std::vector<int>& nrhs) {
+  if (b->gens.size() != b->bodies.size() || nrhs.empty() || b->built_rhs != nrhs[0]) return true;
+  for (size_t i = 0; i < b->bodies.size(); ++i)
+    if (b->gens[i] != b->bodies[i]->layout_gen) return true;
+  return false;
+}
+
+// Points every body's handle at the batch stream for the duration of a call.
+struct StreamSwap {
+  nemoh_batch* b;
+  std::vector<hipStream_t> saved;
+  explicit StreamSwap(nemoh_batch* bb) : b(bb) {
+    for (nemoh_core* c : b->bodies) {
+      saved.push_back(c->st);
+      c->st = b->st;
+    }
+  }
+  ~StreamSwap() {
+    for (size_t i = 0; i < b->bodies.size(); ++i) b->bodies[i]->st = saved[i];
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int nemoh_batch_create(nemoh_batch** out, int nbody, nemoh_core* const* bodies, int precision, int max_rhs) {
+  if (!out) return NEMOH_ERR_ARG;
+  *out = nullptr;
+  if (nbody <= 0 || !bodies || (precision != NEMOH_FP64 && precision != NEMOH_MIXED) || max_rhs < 1 ||
+      max_rhs > kMaxRhs)
+    return NEMOH_ERR_ARG;
+  auto* b = new nemoh_batch();
+  b->prec = precision;
+  b->max_rhs = max_rhs;
+  auto bail = [&](int code, const std::string& msg) {
+    create_error = msg;
+    nemoh_batch_destroy(b);
+    return code;
+  };
+  for (int i = 0; i < nbody; ++i) {
+    nemoh_core* c = bodies[i];
+    if (!c) return bail(NEMOH_ERR_ARG, "null body handle");
+    if (!c->dofs_set) return bail(NEMOH_ERR_ARG, "body " + std::to_string(i) + ": nemoh_core_set_dofs not called");
+    if (c->fp32_assembly) return bail(NEMOH_ERR_ARG, "body " + std::to_string(i) + ": FP32 assembly is not batched");
+    if (i > 0 && (c->nsym != bodies[0]->nsym || c->depth != bodies[0]->depth || c->g != bodies[0]->g))
+      return bail(NEMOH_ERR_ARG, "bodies of a batch share symmetry, depth and g");
+    for (int j = 0; j < i; ++j)
+      if (bodies[j] == c) return bail(NEMOH_ERR_ARG, "a handle appears twice in the batch");
+    b->bodies.push_back(c);
+  }
+  b->nsym = bodies[0]->nsym;
+  b->nb = bodies[0]->nb;
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess)
+    return bail(NEMOH_ERR_HIP, "batch stream creation failed");
+  for (nemoh_core* c : b->bodies) {
+    c->prec = precision;
+    if (ensure_matrices(c) || ensure_rhs(c, max_rhs)) return bail(NEMOH_ERR_HIP, c->err);
+  }
+  *out = b;
+  return NEMOH_OK;
+}
+
+// One frequency for every body of the batch: forces of body i (nforce_i x
+// (nbeta + nrad_i), column-major complex) at forces + offset_i, with offset_i
+// the sum of the earlier bodies' sizes; fk likewise (nforce_i x nbeta).
+int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const double* beta, double* forces, double* fk) {
+  if (!b) return NEMOH_ERR_ARG;
+  if (!(std::fabs(omega) >= 1.E-4)) return batch_fail(b, NEMOH_ERR_OMEGA, "ABS(WR) < 1.E-4");
+  StreamSwap swap(b);
+  const double pi = 4.0 * std::atan(1.0);
+  const double period = 2.0 * pi / omega;
+  const size_t nbody = b->bodies.size();
+  std::vector<int> nrhs(nbody);
+  int fd = -1;
+  double w = 0;
+  // frequency set-up and body conditions of every body
+  for (size_t i = 0; i < nbody; ++i) {
+    nemoh_core* c = b->bodies[i];
+    if (c->prec != b->prec) return batch_fail(b, NEMOH_ERR_ARG, "a body's precision changed outside the batch");
+    nrhs[i] = nbeta + c->nrad;
+    if (nrhs[i] > b->max_rhs)
+      return batch_fail(b, NEMOH_ERR_ARG, "body " + std::to_string(i) + ": more problems than the batch was made for");
+    nemoh_freq f;
+    bool bfd;
+    if (int r = nemoh_core_freq_setup(c, period, &f)) return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
+    if (int r = build_params(c, period, f.amh, f.nexp, f.ambda, f.ar, f.nq, f.qq, f.cq, &bfd, &w))
+      return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
+    if (int r = generate_waves(c, omega, nbeta, beta)) return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
+    if (fd >= 0 && fd != (bfd ? 1 : 0)) return batch_fail(b, NEMOH_ERR_ARG, "bodies in different depth regimes");
+    fd = bfd ? 1 : 0;
+    c->stats.cache_hit = 0;
+    c->stats.refine_iterations = 0;
+    c->stats.refine_residual = 0;
+  }
+  for (size_t i = 1; i < nbody; ++i)
+    if (nrhs[i] != nrhs[0]) return batch_fail(b, NEMOH_ERR_ARG, "bodies of a batch solve the same number of problems");
+  if (batch_stale(b, nrhs))
+    if (int r = batch_build(b, nrhs)) return r;
+  const bool mixed = b->prec == NEMOH_MIXED;
+  // assembly of every body, one launch (free-surface panels after it, per body)
+  for (nemoh_core* c : b->bodies) launch_info_reset(c->d_info, b->st);
+  launch_assemble_group(b->d_asm, b->asm_count, b->asm_grid, fd, b->nsym, mixed, b->st);
+  for (nemoh_core* c : b->bodies)
+    launch_assemble_surface(c->G, c->T, c->d_par, fd, c->nsym, mat_out(c), c->d_surf, c->nsurf, b->st);
+  // factorisation of every matrix, one launch per step
+  if (mixed) launch_lu_group(b->lu32, b->st);
+  else launch_lu_group(b->lu64, b->st);
+  const int mode = mixed ? NEMOH_MIXED : NEMOH_FP64;
+  for (nemoh_core* c : b->bodies) {  // the factors on the device belong to this frequency
+    c->w_prev = w;
+    c->factored_prec = mode;
+  }
+  // solves, symmetry block by symmetry block (the handles' right-hand-side
+  // buffers serve both blocks)
+  for (int k = 0; k < b->nb; ++k) {
+    for (size_t i = 0; i < nbody; ++i) {
+      nemoh_core* c = b->bodies[i];
+      launch_rhs(c->V, c->nfa, c->nsym, nrhs[i], k == 0 ? 1.0 : -1.0, c->B, b->st);
+    }
+    if (!mixed) {
+      for (size_t i = 0; i < nbody; ++i) {
+        nemoh_core* c = b->bodies[i];
+        launch_copy(c->B, c->Z[k], (size_t)c->nfa * nrhs[i], nullptr, b->st);
+      }
+      launch_trsv_group(b->tr64[k], b->bodies[0]->spin_limit, b->st);
+    } else {
+      for (size_t i = 0; i < nbody; ++i) {
+        nemoh_core* c = b->bodies[i];
+        const size_t cnt = (size_t)c->nfa * nrhs[i];
+        RefineState* rs = c->d_rs + k;
+        launch_refine_reset(rs, b->st);
+        launch_copy(c->B, c->R, cnt, nullptr, b->st);
+        launch_refine_residual_max(c->B, cnt, c->d_red, rs, c->tol, 1, b->st);
+      }
+      int sweeps = kQueuedSweeps;  // as solve_block queues them (bodies needing more are redone below)
+      for (nemoh_core* c : b->bodies) sweeps = std::min(sweeps, c->max_iter);
+      for (int sweep = 0; sweep < sweeps; ++sweep) {
+        for (size_t i = 0; i < nbody; ++i) {
+          nemoh_core* c = b->bodies[i];
+          launch_f64_to_f32(c->R, c->R32, (size_t)c->nfa * nrhs[i], c->d_rs + k, b->st);
+        }
+        launch_trsv_group(b->tr32[k], b->bodies[0]->spin_limit, b->st);
+        for (size_t i = 0; i < nbody; ++i) {
+          nemoh_core* c = b->bodies[i];
+          const size_t cnt = (size_t)c->nfa * nrhs[i];
+          RefineState* rs = c->d_rs + k;
+          launch_f32_to_f64(c->R32, c->Z[k], cnt, rs, sweep == 0 ? 0 : 1, b->st);
+          launch_copy(c->B, c->R, cnt, rs, b->st);
+          residual(c, k, nrhs[i], rs);
+          launch_refine_residual_max(c->R, cnt, c->d_red, rs, c->tol, 0, b->st);
+        }
+      }
+    }
+    for (size_t i = 0; i < nbody; ++i) potential_of_block(b->bodies[i], k, nrhs[i]);
+  }
+  if (hipGetLastError() != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "batch launch failed");
+  // one look at every body's pivot report, wait status and refinement
+  std::vector<int> info(2 * nbody, 0);
+  std::vector<RefineState> rs(2 * nbody);
+  for (size_t i = 0; i < nbody; ++i) {
+    nemoh_core* c = b->bodies[i];
+    (void)hipMemcpyAsync(&info[2 * i], c->d_info, 2 * sizeof(int), hipMemcpyDeviceToHost, b->st);
+    if (mixed) (void)hipMemcpyAsync(&rs[2 * i], c->d_rs, c->nb * sizeof(RefineState), hipMemcpyDeviceToHost, b->st);
+  }
+  if (hipStreamSynchronize(b->st) != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "batch synchronisation failed");
+  size_t fo = 0, ko = 0;
+  for (size_t i = 0; i < nbody; ++i) {
+    nemoh_core* c = b->bodies[i];
+    bool clean = info[2 * i] == 0 && info[2 * i + 1] == 0;
+    if (mixed)
+      for (int k = 0; k < c->nb; ++k) clean = clean && rs[2 * i + k].done && !rs[2 * i + k].bad;
+    if (info[2 * i]) {
+      char buf[96];
+      std::snprintf(buf, sizeof buf, "body %zu: PIVOT INFERIEUR A 1e-20 (row %d)", i, info[2 * i]);
+      c->w_prev = -1.0;
+      return batch_fail(b, NEMOH_ERR_PIVOT, buf);
+    }
+    if (clean) {
+      if (mixed) (void)refine_outcome(c, &rs[2 * i], 0);  // refinement statistics
+      launch_combine(c->Z[0], c->Z[1], c->F[0], c->F[1], c->nfa, c->nsym, nrhs[i], c->rho * w, c->sig, c->pot,
+                     c->pres, b->st);
+      c->last_nrhs = nrhs[i];
+    } else {
+      // a wait that gave up, or a refinement that needs more sweeps: this
+      // body's solves again on its own handle, from its factors (cache hit)
+      if (int r = run_pipeline(c, w, nrhs[i])) return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
+    }
+    if (forces && c->nforce > 0) {
+      if (int r = download_forces(c, c->pres, c->d_fnds, c->nforce, nrhs[i], forces + 2 * fo))
+        return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
+      fo += (size_t)c->nforce * nrhs[i];
+    }
+    if (fk && c->nforce > 0 && nbeta > 0) {
+      if (int r = download_forces(c, c->d_pinc, c->d_fnds, c->nforce, nbeta, fk + 2 * ko))
+        return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
+      ko += (size_t)c->nforce * nbeta;
+    }
+  }
+  if (hipStreamSynchronize(b->st) != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "batch synchronisation failed");
+  return NEMOH_OK;
+}
+
+const char* nemoh_batch_last_error(const nemoh_batch* b) {
+  if (b) return b->err.c_str();
+  return create_error.empty() ? "null batch" : create_error.c_str();
+}
+
+void nemoh_batch_destroy(nemoh_batch* b) {
+  if (!b) return;
+  std::lock_guard<std::recursive_mutex> lk(device_lock());
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  batch_free_tables(b);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+}  // extern "C"

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <algorithm>
 #include <cstddef>
 
@@ -63,6 +65,26 @@ void launch_assemble_mixed(const Geometry& G, const Tables& T, const float4* tab
 void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, int fd, int nsym, const MatOut& out,
                      const int* surf, int nsurf, bool paired, hipStream_t st);
 void launch_pairs(const Geometry& G, const Tables& T, const FreqParams* P, int fd, double* out, hipStream_t st);
+// One body of a grouped paired assembly: its geometry, frequency
+// parameters and matrices, its nt x nt grid of 8-panel blocks (nwaves =
+// nt (nt + 1) / 2 pair blocks, four per workgroup) on workgroups [wg0, wg1)
+// of the launch.
+struct AsmJob {
+  Geometry G;
+  Tables T;
+  const FreqParams* P;
+  MatOut out;
+  int nt, nwaves, wg0, wg1;
+};
+// The paired assembly of every body of jobs[0, count) in one launch of grid
+// workgroups (the bulk pairs; free-surface panels are the caller's, through
+// launch_assemble's surface pass).  All bodies share the regime (fd) and
+// symmetry (nsym); f32: every job writes the complex64 copy of D.
+void launch_assemble_group(const AsmJob* jobs, int count, int grid, int fd, int nsym, bool f32, hipStream_t st);
+// The free-surface pass of launch_assemble alone (rows and columns of the
+// panels surf[0, nsurf) with zg >= ZER), after a grouped assembly.
+void launch_assemble_surface(const Geometry& G, const Tables& T, const FreqParams* P, int fd, int nsym,
+                             const MatOut& out, const int* surf, int nsurf, hipStream_t st);
 // Block LU without pivoting in place in A (n x n, leading dimension ld):
 // afterwards the blocks below the diagonal hold L (unit block lower), the
 // blocks above hold U (unit block upper) and P (ceil(n/64) blocks of 64x64,
@@ -88,6 +110,21 @@ int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int 
               int* luctl = nullptr, int spin_limit = 1 << 20);
 // block pairs of an order-n factorisation, with one to spare
 inline int lu_pairs(int n) { return ((n + kNB - 1) / kNB + 1) / 2 + 1; }
+// Panel launch of the paired form: L panel of block column lc (nL blocks
+// below the diagonal) and the U rows ur0 (nU0 blocks) and ur1 (nU1).
+struct PanelJob {
+  int lc, nL, ur0, nU0, ur1, nU1;
+};
+// One matrix's share of a grouped LU launch (launch_lu_group): its
+// workgroups are [wg0, wg1) of the launch, wg0 a multiple of 8.
+template <typename T>
+struct LuJob {
+  C2<T>* A;
+  C2<T>* P;
+  int* info;
+  int ld, n, k, wg0, wg1;
+  PanelJob J;
+};
 // Control block of the chained triangular solves of one handle: the two
 // triangles' ticket counters (reset on the device at the start of every
 // chained solve).
@@ -95,6 +132,55 @@ struct ChainCtl {
   unsigned ticket[2];
   int pad[2];
 };
+// One system's share of a grouped chained solve (launch_trsv_group):
+// factors A (leading dimension ld) and P, right-hand sides / solution B,
+// scratch W, its control block and status word, and its refinement state;
+// workgroups [wg0, wg1) of the launch.
+template <typename T>
+struct TrsvJob {
+  const C2<T>* A;
+  const C2<T>* P;
+  C2<T>* B;
+  C2<T>* W;
+  ChainCtl* ctl;
+  int* status;
+  const RefineState* skip;
+  int ld, n, nrhs, wg0, wg1;
+};
+// Grouped factorisation of many independent matrices: the launch sequence
+// of launch_lu's paired form (rank 128, one stream), each launch doing that
+// step for every matrix that has it (one workgroup range per matrix), so a
+// batch of bodies of uneven sizes takes max_i ceil(n_i / 128) block pairs of
+// launches instead of the sum.  Each matrix ends with the same factors, bit
+// for bit, as launch_lu's.
+template <typename T>
+struct LuGroup {
+  struct Launch {
+    int kind;   // 0: first pivot inversion, 1: panel, 2: cross update, 3: rank-128 update
+    int first;  // its jobs: d_jobs[first, first + count)
+    int count;
+    int grid;
+  };
+  std::vector<Launch> launches;
+  LuJob<T>* d_jobs = nullptr;  // device memory (hipFree by the owner)
+};
+// mats: A, P, info, ld, n of each matrix (the other fields are set here).
+template <typename T>
+hipError_t build_lu_group(const std::vector<LuJob<T>>& mats, LuGroup<T>* g);
+template <typename T>
+void launch_lu_group(const LuGroup<T>& g, hipStream_t st);
+// Grouped chained solves (one launch per triangle for every system): the
+// systems' TrsvJob fields except wg0 / wg1, which are set here.
+template <typename T>
+struct TrsvGroup {
+  TrsvJob<T>* d_begin = nullptr;  // chain_begin's workgroup ranges
+  TrsvJob<T>* d_tri = nullptr;    // the triangles' (one workgroup per row block)
+  int count = 0, grid_begin = 0, grid_tri = 0;
+};
+template <typename T>
+hipError_t build_trsv_group(const std::vector<TrsvJob<T>>& sys, TrsvGroup<T>* g);
+template <typename T>
+void launch_trsv_group(const TrsvGroup<T>& g, int spin_limit, hipStream_t st);
 // Solves A x = b for nrhs <= kMaxRhs right-hand sides (B: n x nrhs
 // row-major) with the factors of launch_lu; on return B holds the solution,
 // W is scratch of the same size.  chained: one launch per triangle whose

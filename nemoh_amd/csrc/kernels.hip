@@ -186,10 +186,8 @@ __device__ __forceinline__ void tri_block(int g, int nt, int& ti, int& tj) {
 }
 
 template <bool kFd, bool kSym, bool kF32>
-__global__ __launch_bounds__(256, kSym ? 2 : 3) void assemble_pair_kernel(Geometry G, Tables T, const FreqParams* __restrict__ Pp,
-                                                               MatOut out, int nt, int nwaves) {
-  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (g >= nwaves) return;
+__device__ __forceinline__ void assemble_pair_body(const Geometry& G, const Tables& T, const FreqParams* __restrict__ Pp,
+                                                   const MatOut& out, int nt, int g) {
   int ti, tj;
   tri_block(g, nt, ti, tj);
   const int lane = threadIdx.x & 63;
@@ -236,6 +234,32 @@ __global__ __launch_bounds__(256, kSym ? 2 : 3) void assemble_pair_kernel(Geomet
   }
   store_acc<kSym, kF32>(out, i, j, a0);
   if (two) store_acc<kSym, kF32>(out, j, i, a1);
+}
+
+template <bool kFd, bool kSym, bool kF32>
+__global__ __launch_bounds__(256, kSym ? 2 : 3) void assemble_pair_kernel(Geometry G, Tables T,
+                                                                          const FreqParams* __restrict__ Pp,
+                                                                          MatOut out, int nt, int nwaves) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= nwaves) return;
+  assemble_pair_body<kFd, kSym, kF32>(G, T, Pp, out, nt, g);
+}
+
+// Grouped paired assembly: the matrices of many bodies in one launch
+// (AsmJob j: workgroups [wg0, wg1), four waves of 8 x 8 pair blocks each).
+template <bool kFd, bool kSym, bool kF32>
+__global__ __launch_bounds__(256, kSym ? 2 : 3) void assemble_group_kernel(const AsmJob* __restrict__ jobs, int count) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].wg0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const AsmJob& J = jobs[lo];
+  const int g = (b - J.wg0) * 4 + (threadIdx.x >> 6);
+  if (b >= J.wg1 || g >= J.nwaves) return;
+  assemble_pair_body<kFd, kSym, kF32>(J.G, J.T, J.P, J.out, J.nt, g);
 }
 
 
@@ -795,6 +819,50 @@ void launch_assemble(const Geometry& G, const Tables& T, const FreqParams* P, in
                      const int* surf, int nsurf, bool paired, hipStream_t st) {
   if (fd) launch_assemble_regime<true>(G, T, P, nsym, out, surf, nsurf, paired, st);
   else launch_assemble_regime<false>(G, T, P, nsym, out, surf, nsurf, paired, st);
+}
+
+template <bool kFd, bool kSym, bool kF32>
+static void launch_surface_variant(const Geometry& G, const Tables& T, const FreqParams* P, const MatOut& out,
+                                   const int* surf, int nsurf, hipStream_t st) {
+  const int nx = (G.n + 255) / 256;
+  assemble_surface_kernel<kFd, kSym, kF32><<<dim3(nx, nsurf, 2), 256, 0, st>>>(G, T, P, out, surf);
+}
+
+void launch_assemble_surface(const Geometry& G, const Tables& T, const FreqParams* P, int fd, int nsym,
+                             const MatOut& out, const int* surf, int nsurf, hipStream_t st) {
+  if (nsurf <= 0) return;
+  const bool f32 = out.dp32 != nullptr;
+  if (fd) {
+    if (nsym) f32 ? launch_surface_variant<true, true, true>(G, T, P, out, surf, nsurf, st)
+              : launch_surface_variant<true, true, false>(G, T, P, out, surf, nsurf, st);
+    else f32 ? launch_surface_variant<true, false, true>(G, T, P, out, surf, nsurf, st)
+             : launch_surface_variant<true, false, false>(G, T, P, out, surf, nsurf, st);
+  } else {
+    if (nsym) f32 ? launch_surface_variant<false, true, true>(G, T, P, out, surf, nsurf, st)
+              : launch_surface_variant<false, true, false>(G, T, P, out, surf, nsurf, st);
+    else f32 ? launch_surface_variant<false, false, true>(G, T, P, out, surf, nsurf, st)
+             : launch_surface_variant<false, false, false>(G, T, P, out, surf, nsurf, st);
+  }
+}
+
+template <bool kFd, bool kSym, bool kF32>
+static void launch_group_variant(const AsmJob* jobs, int count, int grid, hipStream_t st) {
+  assemble_group_kernel<kFd, kSym, kF32><<<grid, 256, 0, st>>>(jobs, count);
+}
+
+void launch_assemble_group(const AsmJob* jobs, int count, int grid, int fd, int nsym, bool f32, hipStream_t st) {
+  if (count <= 0) return;
+  if (fd) {
+    if (nsym) f32 ? launch_group_variant<true, true, true>(jobs, count, grid, st)
+              : launch_group_variant<true, true, false>(jobs, count, grid, st);
+    else f32 ? launch_group_variant<true, false, true>(jobs, count, grid, st)
+             : launch_group_variant<true, false, false>(jobs, count, grid, st);
+  } else {
+    if (nsym) f32 ? launch_group_variant<false, true, true>(jobs, count, grid, st)
+              : launch_group_variant<false, true, false>(jobs, count, grid, st);
+    else f32 ? launch_group_variant<false, false, true>(jobs, count, grid, st)
+             : launch_group_variant<false, false, false>(jobs, count, grid, st);
+  }
 }
 
 template <bool kFd, bool kSym>

@@ -26,6 +26,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <utility>
 
 #include "kernels.h"
@@ -569,9 +571,8 @@ __device__ __forceinline__ void store_inverse(const InvLds<T>& L, C2<T>* __restr
 
 // P_kb for one diagonal block read from A (the first block of a factorisation).
 template <typename T>
-__global__ __launch_bounds__(256) void block_inverse_kernel(const C2<T>* __restrict__ A, int ld, int n, int kb,
-                                                            C2<T>* __restrict__ P, int* __restrict__ info) {
-  __shared__ InvLds<T> L;
+__device__ __forceinline__ void block_inverse_body(InvLds<T>& L, const C2<T>* __restrict__ A, int ld, int n, int kb,
+                                                   C2<T>* __restrict__ P, int* __restrict__ info) {
   const int k0 = kb * kNB, nb = min(kNB, n - k0);
   for (int e = threadIdx.x; e < kTile; e += 256) {
     const int r = e >> 6, c = e & 63;
@@ -580,6 +581,12 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(const C2<T>* __restr
   __syncthreads();
   lds_invert(L, nb, k0, info);
   store_inverse(L, P + (size_t)kb * kTile, nb);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void block_inverse_kernel(const C2<T>* __restrict__ A, int ld, int n, int kb,
+                                                            C2<T>* __restrict__ P, int* __restrict__ info) {
+  __shared__ InvLds<T> L;
+  block_inverse_body(L, A, ld, n, kb, P, info);
 }
 
 // Panel step kb: workgroups [0, ncol) form L_ib = A_ib P_kb for the row
@@ -702,19 +709,14 @@ __global__ __launch_bounds__(256) void lu_update_kernel(C2<T>* __restrict__ A, i
 // trailing element are summed in one accumulation instead of two), with half
 // the passes over the trailing matrix and twice the work per tile load.
 
-// Panel launch of the paired form: workgroups [0, nL) form L_{lc+1+w, lc} =
-// A P_lc; the next nU0 form U_{ur0, ur0+1+w} = P_ur0 A, the next nU1
-// U_{ur1, ur1+1+w} = P_ur1 A (in place).
-struct PanelJob {
-  int lc, nL, ur0, nU0, ur1, nU1;
-};
+// Panel launch of the paired form (PanelJob, kernels.h): workgroups [0, nL)
+// form L_{lc+1+w, lc} = A P_lc; the next nU0 form U_{ur0, ur0+1+w} = P_ur0
+// A, the next nU1 U_{ur1, ur1+1+w} = P_ur1 A (in place).
 template <typename T>
-__global__ __launch_bounds__(256) void lu_panel2_kernel(C2<T>* __restrict__ A, int ld, int n, PanelJob J,
-                                                        const C2<T>* __restrict__ P) {
-  __shared__ GemmLds<T> S;
+__device__ __forceinline__ void lu_panel2_body(GemmLds<T>& S, C2<T>* __restrict__ A, int ld, int n, const PanelJob& J,
+                                               const C2<T>* __restrict__ P, int w) {
   Acc64<T> acc;
   acc_zero(acc);
-  int w = blockIdx.x;
   if (w < J.nL) {
     const int r0 = (J.lc + 1 + w) * kNB, c0 = J.lc * kNB;
     const int nr = min(kNB, n - r0), nk = min(kNB, n - c0);
@@ -735,16 +737,20 @@ __global__ __launch_bounds__(256) void lu_panel2_kernel(C2<T>* __restrict__ A, i
   gemm64<T, false>(acc, View<T>{P + (size_t)ur * kTile, kNB, kNB, kNB}, View<T>{blk, ld, nk, nc}, S);
   acc_store(acc, blk, ld, nk, nc);
 }
+template <typename T>
+__global__ __launch_bounds__(256) void lu_panel2_kernel(C2<T>* __restrict__ A, int ld, int n, PanelJob J,
+                                                        const C2<T>* __restrict__ P) {
+  __shared__ GemmLds<T> S;
+  lu_panel2_body(S, A, ld, n, J, P, (int)blockIdx.x);
+}
 
 // Cross update of the pair (k, k+1): workgroup w < nr takes tile (k+1,
 // k+1+w) of block row k+1 (nr = blocks from k+1 on), the others tile (k+2+w',
 // k+1) of block column k+1; workgroup 0 (the diagonal tile) then inverts it.
 template <typename T>
-__global__ __launch_bounds__(256) void lu_cross_kernel(C2<T>* __restrict__ A, int ld, int n, int k,
-                                                       C2<T>* __restrict__ P, int* __restrict__ info) {
-  __shared__ UpdLds<T> S;
+__device__ __forceinline__ void lu_cross_body(UpdLds<T>& S, C2<T>* __restrict__ A, int ld, int n, int k,
+                                              C2<T>* __restrict__ P, int* __restrict__ info, int w) {
   const int nblk = (n + kNB - 1) / kNB, s = k + 1, nrow = nblk - s;
-  const int w = blockIdx.x;
   const int bi = w < nrow ? s : s + 1 + (w - nrow), bj = w < nrow ? s + w : s;
   const int r0 = bi * kNB, c0 = bj * kNB, k0 = k * kNB;
   const int nr = min(kNB, n - r0), nc = min(kNB, n - c0);
@@ -756,6 +762,12 @@ __global__ __launch_bounds__(256) void lu_cross_kernel(C2<T>* __restrict__ A, in
   acc_store(acc, blk, ld, nr, nc);
   if (w != 0) return;
   invert_tile(S, acc, nr, r0, P + (size_t)s * kTile, info);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void lu_cross_kernel(C2<T>* __restrict__ A, int ld, int n, int k,
+                                                       C2<T>* __restrict__ P, int* __restrict__ info) {
+  __shared__ UpdLds<T> S;
+  lu_cross_body(S, A, ld, n, k, P, info, (int)blockIdx.x);
 }
 
 // The same cross update with the panel of block column k+1 folded in (one
@@ -839,14 +851,11 @@ __global__ void lu_ctl_reset_kernel(int* __restrict__ ctl, int count) {
 // region 2: the rest (i, j >= s+2), banded.  Workgroup 0 of regions 0 and 1
 // owns (s, s) and inverts it (P_s).
 template <typename T>
-__global__ __launch_bounds__(256) void lu_update128_kernel(C2<T>* __restrict__ A, int ld, int n, int k,
-                                                           C2<T>* __restrict__ P, int* __restrict__ info,
-                                                           int region) {
-  __shared__ UpdLds<T> S;
+__device__ __forceinline__ void lu_update128_body(UpdLds<T>& S, C2<T>* __restrict__ A, int ld, int n, int k,
+                                                  C2<T>* __restrict__ P, int* __restrict__ info, int region, int w) {
   const int s0 = (k + 2) * kNB, k0 = k * kNB;
   const int m = (n - s0 + kNB - 1) / kNB;
   int ti, tj;
-  const int w = blockIdx.x;
   if (region == 1) {
     if (w < 2 * m) {
       ti = w / m;
@@ -873,6 +882,72 @@ __global__ __launch_bounds__(256) void lu_update128_kernel(C2<T>* __restrict__ A
   acc_store(acc, blk, ld, nr, nc);
   if (w != 0 || region == 2) return;
   invert_tile(S, acc, nr, r0, P + (size_t)(k + 2) * kTile, info);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void lu_update128_kernel(C2<T>* __restrict__ A, int ld, int n, int k,
+                                                           C2<T>* __restrict__ P, int* __restrict__ info,
+                                                           int region) {
+  __shared__ UpdLds<T> S;
+  lu_update128_body(S, A, ld, n, k, P, info, region, (int)blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches: one launch does the same step for the matrices of many
+// independent bodies (LuJob j: workgroups [wg0, wg1) of the launch, wg0 a
+// multiple of 8 so that a job's workgroups meet the XCDs in the order the
+// single-matrix launch deals them, and its tile map stays XCD-banded).  Each
+// workgroup finds its job by binary search over wg0 and runs the body of the
+// single-matrix kernel with its index within the job, so every matrix gets
+// the arithmetic of its own launch sequence, bit for bit.
+// the job of this workgroup (-1: padding up to the next multiple of 8) and
+// the workgroup's index within it
+template <typename J>
+__device__ __forceinline__ int group_job(const J* __restrict__ jobs, int count, int& w) {
+  const int g = blockIdx.x;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].wg0 <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  w = g - jobs[lo].wg0;
+  return (g < jobs[lo].wg1) ? lo : -1;
+}
+template <typename T>
+__global__ __launch_bounds__(256) void group_inverse_kernel(const LuJob<T>* __restrict__ jobs, int count) {
+  __shared__ InvLds<T> L;
+  int w;
+  const int j = group_job(jobs, count, w);
+  if (j < 0) return;
+  const LuJob<T> J = jobs[j];
+  block_inverse_body(L, J.A, J.ld, J.n, J.k, J.P, J.info);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void group_panel2_kernel(const LuJob<T>* __restrict__ jobs, int count) {
+  __shared__ GemmLds<T> S;
+  int w;
+  const int j = group_job(jobs, count, w);
+  if (j < 0) return;
+  const LuJob<T> J = jobs[j];
+  lu_panel2_body(S, J.A, J.ld, J.n, J.J, J.P, w);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void group_cross_kernel(const LuJob<T>* __restrict__ jobs, int count) {
+  __shared__ UpdLds<T> S;
+  int w;
+  const int j = group_job(jobs, count, w);
+  if (j < 0) return;
+  const LuJob<T> J = jobs[j];
+  lu_cross_body(S, J.A, J.ld, J.n, J.k, J.P, J.info, w);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void group_update128_kernel(const LuJob<T>* __restrict__ jobs, int count) {
+  __shared__ UpdLds<T> S;
+  int w;
+  const int j = group_job(jobs, count, w);
+  if (j < 0) return;
+  const LuJob<T> J = jobs[j];
+  lu_update128_body(S, J.A, J.ld, J.n, J.k, J.P, J.info, 0, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -1110,13 +1185,12 @@ __device__ __forceinline__ void hop_mark_unsolved(C2<T>* __restrict__ X, int nb,
 // against 0.85 us of one CU's FP32 MFMA peak: measured 480 us per triangle
 // at 4000 panels against 307 us for the 64-row form (both with flags).
 template <typename T, bool kLower>
-__global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict__ A, int ld, int n,
-                                                         const C2<T>* __restrict__ P, C2<T>* __restrict__ R,
-                                                         C2<T>* __restrict__ X, int nrhs, ChainCtl* __restrict__ ctl,
-                                                         int tri, int* __restrict__ status, int spin_limit,
-                                                         const RefineState* __restrict__ skip) {
+__device__ __forceinline__ void trsv_chain_body(int& my_ticket, const C2<T>* __restrict__ A, int ld, int n,
+                                                const C2<T>* __restrict__ P, C2<T>* __restrict__ R,
+                                                C2<T>* __restrict__ X, int nrhs, ChainCtl* __restrict__ ctl, int tri,
+                                                int* __restrict__ status, int spin_limit,
+                                                const RefineState* __restrict__ skip) {
   typedef typename MfmaTraits<T>::acc_t acc_t;
-  __shared__ int my_ticket;
   const int tid = threadIdx.x, lane = tid & 63;
   const int nblk = (n + kNB - 1) / kNB;
   // a converged refinement: nothing is done (every workgroup reads the same
@@ -1212,6 +1286,37 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict
   // into it
   if (kLower) hop_mark_unsolved<T>(R + (size_t)k0 * nrhs, nb, nrhs);
 }
+template <typename T, bool kLower>
+__global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict__ A, int ld, int n,
+                                                         const C2<T>* __restrict__ P, C2<T>* __restrict__ R,
+                                                         C2<T>* __restrict__ X, int nrhs, ChainCtl* __restrict__ ctl,
+                                                         int tri, int* __restrict__ status, int spin_limit,
+                                                         const RefineState* __restrict__ skip) {
+  __shared__ int my_ticket;
+  trsv_chain_body<T, kLower>(my_ticket, A, ld, n, P, R, X, nrhs, ctl, tri, status, spin_limit, skip);
+}
+
+// Grouped chained solves: one launch per triangle for many systems (TrsvJob
+// j: workgroups [wg0, wg1), one per row block).  A workgroup finds its
+// system and takes its ticket from that system's counter, so it waits only
+// for row blocks of its own system whose tickets were taken earlier, by
+// running workgroups: the argument of the single-system launch holds per
+// system.  chain_begin's grouped form gives each system ceil(n nrhs / 256)
+// workgroups.
+template <typename T, bool kLower>
+__global__ __launch_bounds__(256) void group_trsv_kernel(const TrsvJob<T>* __restrict__ jobs, int count,
+                                                         int spin_limit) {
+  __shared__ int my_ticket;
+  int w;
+  const int j = group_job(jobs, count, w);
+  if (j < 0) return;
+  const TrsvJob<T> J = jobs[j];
+  if (kLower)
+    trsv_chain_body<T, true>(my_ticket, J.A, J.ld, J.n, J.P, J.B, J.W, J.nrhs, J.ctl, 0, J.status, spin_limit, J.skip);
+  else
+    trsv_chain_body<T, false>(my_ticket, J.A, J.ld, J.n, J.P, J.W, J.B, J.nrhs, J.ctl, 1, J.status, spin_limit,
+                              J.skip);
+}
 
 // Stepwise fallback (one launch per block step s, for matrices with more row
 // blocks than can be resident): every remaining row block i does
@@ -1267,6 +1372,21 @@ __global__ void chain_begin_kernel(ChainCtl* __restrict__ ctl, C2<T>* __restrict
   }
   if (skip && skip->done) return;
   if (e < count) st_unsolved(X + e);
+}
+
+template <typename T>
+__global__ void group_chain_begin_kernel(const TrsvJob<T>* __restrict__ jobs, int count) {
+  int w;
+  const int j = group_job(jobs, count, w);
+  if (j < 0) return;
+  const TrsvJob<T> J = jobs[j];
+  const size_t e = (size_t)w * blockDim.x + threadIdx.x;
+  if (e == 0) {
+    J.ctl->ticket[0] = 0u;
+    J.ctl->ticket[1] = 0u;
+  }
+  if (J.skip && J.skip->done) return;
+  if (e < (size_t)J.n * J.nrhs) st_unsolved(J.W + e);
 }
 
 template <typename T>
@@ -1407,6 +1527,137 @@ template void launch_lu_solve<double>(const C2<double>*, int, int, const C2<doub
                                       bool, ChainCtl*, int*, int, hipStream_t, const RefineState*);
 template void launch_lu_solve<float>(const C2<float>*, int, int, const C2<float>*, C2<float>*, C2<float>*, int, bool,
                                      ChainCtl*, int*, int, hipStream_t, const RefineState*);
+
+// ---------------------------------------------------------------------------
+// Grouped launches (host side)
+
+namespace {
+// appends a launch of the given jobs, each on a workgroup range starting at a
+// multiple of 8
+template <typename T>
+void add_launch(LuGroup<T>* g, std::vector<LuJob<T>>& all, int kind, std::vector<LuJob<T>>& jobs) {
+  if (jobs.empty()) return;
+  int wg = 0;
+  for (auto& j : jobs) {
+    j.wg0 = wg;
+    j.wg1 = wg + j.wg1;  // (wg1 holds the job's workgroup count until here)
+    wg = (j.wg1 + 7) & ~7;
+  }
+  g->launches.push_back({kind, (int)all.size(), (int)jobs.size(), jobs.back().wg1});
+  all.insert(all.end(), jobs.begin(), jobs.end());
+  jobs.clear();
+}
+}  // namespace
+
+template <typename T>
+hipError_t build_lu_group(const std::vector<LuJob<T>>& mats, LuGroup<T>* g) {
+  g->launches.clear();
+  std::vector<LuJob<T>> all, jobs;
+  const int nm = (int)mats.size();
+  std::vector<int> nblk(nm);
+  std::vector<PanelJob> pend(nm, PanelJob{0, 0, 0, 0, 0, 0});
+  int maxblk = 0;
+  for (int i = 0; i < nm; ++i) {
+    nblk[i] = (mats[i].n + kNB - 1) / kNB;
+    maxblk = std::max(maxblk, nblk[i]);
+  }
+  auto job = [&](int i, int k, PanelJob J, int grid) {
+    LuJob<T> j = mats[i];
+    j.k = k;
+    j.J = J;
+    j.wg1 = grid;
+    jobs.push_back(j);
+  };
+  for (int i = 0; i < nm; ++i) job(i, 0, PanelJob{0, 0, 0, 0, 0, 0}, 1);
+  add_launch(g, all, 0, jobs);
+  // launch_lu's paired loop, step k for every matrix that has it
+  for (int k = 0; k < maxblk; k += 2) {
+    for (int i = 0; i < nm; ++i) {
+      if (k >= nblk[i]) continue;
+      PanelJob J = pend[i];
+      J.lc = k;
+      J.nL = nblk[i] - k - 1;
+      pend[i] = PanelJob{0, 0, 0, 0, 0, 0};
+      if (J.nL + J.nU0 + J.nU1 > 0) job(i, k, J, J.nL + J.nU0 + J.nU1);
+    }
+    add_launch(g, all, 1, jobs);
+    for (int i = 0; i < nm; ++i)
+      if (k + 1 < nblk[i]) job(i, k, PanelJob{0, 0, 0, 0, 0, 0}, (nblk[i] - (k + 1)) + (nblk[i] - (k + 2)));
+    add_launch(g, all, 2, jobs);
+    for (int i = 0; i < nm; ++i)
+      if (k + 2 < nblk[i]) job(i, k, PanelJob{k + 1, nblk[i] - (k + 2), 0, 0, 0, 0}, nblk[i] - (k + 2));
+    add_launch(g, all, 1, jobs);
+    for (int i = 0; i < nm; ++i)
+      if (k + 2 < nblk[i]) job(i, k, PanelJob{0, 0, 0, 0, 0, 0}, (nblk[i] - (k + 2)) * (nblk[i] - (k + 2)));
+    add_launch(g, all, 3, jobs);
+    for (int i = 0; i < nm; ++i)
+      if (k + 1 < nblk[i]) pend[i] = PanelJob{0, 0, k, nblk[i] - k - 1, k + 1, nblk[i] - k - 2};
+  }
+  for (int i = 0; i < nm; ++i)  // the U rows of each matrix's last pair
+    if (pend[i].nU0 + pend[i].nU1 > 0) job(i, 0, pend[i], pend[i].nU0 + pend[i].nU1);
+  add_launch(g, all, 1, jobs);
+  if (g->d_jobs) (void)hipFree(g->d_jobs);
+  g->d_jobs = nullptr;
+  hipError_t e = hipMalloc((void**)&g->d_jobs, std::max<size_t>(1, all.size()) * sizeof(LuJob<T>));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(g->d_jobs, all.data(), all.size() * sizeof(LuJob<T>), hipMemcpyHostToDevice);
+}
+
+template <typename T>
+void launch_lu_group(const LuGroup<T>& g, hipStream_t st) {
+  for (const auto& L : g.launches) {
+    const LuJob<T>* j = g.d_jobs + L.first;
+    switch (L.kind) {
+      case 0: group_inverse_kernel<T><<<L.grid, 256, 0, st>>>(j, L.count); break;
+      case 1: group_panel2_kernel<T><<<L.grid, 256, 0, st>>>(j, L.count); break;
+      case 2: group_cross_kernel<T><<<L.grid, 256, 0, st>>>(j, L.count); break;
+      default: group_update128_kernel<T><<<L.grid, 256, 0, st>>>(j, L.count); break;
+    }
+  }
+}
+
+template <typename T>
+hipError_t build_trsv_group(const std::vector<TrsvJob<T>>& sys, TrsvGroup<T>* g) {
+  std::vector<TrsvJob<T>> begin(sys), tri(sys);
+  int wb = 0, wt = 0;
+  for (size_t i = 0; i < sys.size(); ++i) {
+    begin[i].wg0 = wb;
+    begin[i].wg1 = wb + (int)(((size_t)sys[i].n * sys[i].nrhs + 255) / 256);
+    wb = begin[i].wg1;
+    tri[i].wg0 = wt;
+    tri[i].wg1 = wt + (sys[i].n + kNB - 1) / kNB;
+    wt = tri[i].wg1;
+  }
+  for (TrsvJob<T>** p : {&g->d_begin, &g->d_tri})
+    if (*p) (void)hipFree(*p);
+  g->d_begin = g->d_tri = nullptr;
+  g->count = (int)sys.size();
+  g->grid_begin = wb;
+  g->grid_tri = wt;
+  const size_t bytes = std::max<size_t>(1, sys.size()) * sizeof(TrsvJob<T>);
+  hipError_t e = hipMalloc((void**)&g->d_begin, bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&g->d_tri, bytes);
+  if (e == hipSuccess) e = hipMemcpy(g->d_begin, begin.data(), sys.size() * sizeof(TrsvJob<T>), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->d_tri, tri.data(), sys.size() * sizeof(TrsvJob<T>), hipMemcpyHostToDevice);
+  return e;
+}
+
+template <typename T>
+void launch_trsv_group(const TrsvGroup<T>& g, int spin_limit, hipStream_t st) {
+  if (g.count == 0) return;
+  group_chain_begin_kernel<T><<<g.grid_begin, 256, 0, st>>>(g.d_begin, g.count);
+  group_trsv_kernel<T, true><<<g.grid_tri, 256, 0, st>>>(g.d_tri, g.count, spin_limit);
+  group_trsv_kernel<T, false><<<g.grid_tri, 256, 0, st>>>(g.d_tri, g.count, spin_limit);
+}
+
+template hipError_t build_lu_group<float>(const std::vector<LuJob<float>>&, LuGroup<float>*);
+template hipError_t build_lu_group<double>(const std::vector<LuJob<double>>&, LuGroup<double>*);
+template void launch_lu_group<float>(const LuGroup<float>&, hipStream_t);
+template void launch_lu_group<double>(const LuGroup<double>&, hipStream_t);
+template hipError_t build_trsv_group<float>(const std::vector<TrsvJob<float>>&, TrsvGroup<float>*);
+template hipError_t build_trsv_group<double>(const std::vector<TrsvJob<double>>&, TrsvGroup<double>*);
+template void launch_trsv_group<float>(const TrsvGroup<float>&, int, hipStream_t);
+template void launch_trsv_group<double>(const TrsvGroup<double>&, int, hipStream_t);
 
 void launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t st) {
   mfma_probe_kernel<double><<<1, 64, 0, st>>>(a, b, d);

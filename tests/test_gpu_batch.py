@@ -1,0 +1,106 @@
+"""Batches: many bodies solved together, each heavy step one launch over all
+of them (nemoh_batch_*).  Every body's forces and Froude-Krylov forces must
+equal, bit for bit, those of nemoh_core_solve_waves on a handle of its own:
+the grouped launches give each matrix the arithmetic of its own launch
+sequence.  The single-handle path itself is pinned to the reference Solver
+and the oracle (test_gpu_core.py, test_gpu_c4.py).
+
+Cases: bodies of the batched workload's generator with uneven panel counts
+(odd and even numbers of 64-row blocks, one body alone in the last block
+pairs) in infinite depth; boxes in finite depth (the finite-depth assembly
+and the exponential images); half meshes with y-symmetry (two symmetry
+blocks per body); both precisions; and the batch's fallback for a chained
+solve that gives up waiting (spin limit 0), which redoes a body on its own
+handle.
+"""
+import numpy as np
+import pytest
+
+from nemoh_amd import bodycond as bcm
+from nemoh_amd import core
+from nemoh_amd import mesh as nmesh
+from nemoh_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+def _handle(m, depth, tables, prec, spin=None):
+    c = core.Core(m.vertices, m.panels, m.isym, depth, 1025.0, 9.81, tables=tables)
+    c.set_dofs(bcm.six_dof())
+    c.set_precision(prec)
+    if spin is not None:
+        c.set_option(core.OPT_SOLVE_SPIN_LIMIT, spin)
+    return c
+
+
+def _check(meshes, depth, omegas, betas, prec, spin=None):
+    tables = core.build_tables()
+    alone = []
+    for m in meshes:
+        c = _handle(m, depth, tables, prec, spin)
+        alone.append([c.solve_waves(w, betas) for w in omegas])
+        c.close()
+    cores = [_handle(m, depth, tables, prec, spin) for m in meshes]
+    b = core.Batch(cores, prec, max_rhs=len(betas) + 6)
+    for fi, w in enumerate(omegas):
+        got = b.solve_waves(w, betas)
+        for k, ((f, fk), (fa, fka)) in enumerate(zip(got, [a[fi] for a in alone])):
+            assert np.all(np.isfinite(f)) and f.shape == fa.shape
+            assert np.array_equal(f, fa), (k, meshes[k].npanels, w, float(np.abs(f - fa).max() / np.abs(fa).max()))
+            assert np.array_equal(fk, fka), (k, w)
+    b.close()
+    for c in cores:
+        c.close()
+
+
+def _c4_meshes(n, nrange, seed):
+    rng = np.random.default_rng(seed)
+    return [serving.random_body_parts(rng, nrange)[0] for _ in range(n)]
+
+
+@pytest.mark.parametrize("prec", [core.MIXED, core.FP64])
+def test_batch_equals_single_bodies_infinite_depth(prec):
+    ms = _c4_meshes(10, (150, 1400), 11)
+    blocks = sorted({(m.npanels + 63) // 64 for m in ms})
+    print("panels", [m.npanels for m in ms], "blocks", blocks, flush=True)
+    assert any(b % 2 for b in blocks) and any(b % 2 == 0 for b in blocks)
+    _check(ms, 0.0, [0.4, 1.3], np.zeros(1), prec)
+
+
+@pytest.mark.parametrize("prec", [core.MIXED, core.FP64])
+def test_batch_equals_single_bodies_finite_depth(prec):
+    ms = [nmesh.box(30.0, 10.0, 5.0, 15, 5, 3), nmesh.box(20.0, 8.0, 4.0, 20, 8, 4), nmesh.box(12.0, 12.0, 6.0, 6, 6, 3)]
+    _check(ms, 30.0, [0.5, 1.1], np.deg2rad([0.0, 45.0]), prec)
+
+
+@pytest.mark.parametrize("prec", [core.MIXED, core.FP64])
+def test_batch_equals_single_bodies_symmetric(prec):
+    ms = [nmesh.cylinder(5.0, 4.0, 16, 4, 3, half=True), nmesh.cylinder(8.0, 3.0, 24, 6, 5, half=True)]
+    assert all(m.isym == 1 for m in ms)
+    _check(ms, 0.0, [0.6, 1.5], np.zeros(1), prec)
+
+
+def test_batch_chained_solve_fallback():
+    """Spin limit 0: every chained wait gives up at once, and each body is
+    solved again on its own handle (stepwise), as a handle alone does."""
+    ms = _c4_meshes(3, (150, 700), 5)
+    _check(ms, 0.0, [0.9], np.zeros(1), core.MIXED, spin=0)
+
+
+def test_batch_argument_errors():
+    tables = core.build_tables()
+    a = _handle(nmesh.cylinder(5.0, 4.0, 16, 4, 3, half=True), 0.0, tables, core.FP64)
+    b = _handle(nmesh.box(10.0, 5.0, 3.0, 5, 3, 2), 0.0, tables, core.FP64)
+    with pytest.raises(core.NemohError) as e:
+        core.Batch([a, b], core.FP64)  # symmetric and unsymmetric bodies
+    assert e.value.code == core.ERR_ARG
+    with pytest.raises(core.NemohError):
+        core.Batch([b, b], core.FP64)  # the same handle twice
+    bt = core.Batch([b], core.FP64, max_rhs=7)
+    with pytest.raises(core.NemohError):
+        bt.solve_waves(0.8, np.zeros(3))  # 3 headings + 6 dofs > max_rhs
+    f = bt.solve_waves(0.8, np.zeros(1))
+    assert np.all(np.isfinite(f[0][0]))
+    bt.close()
+    a.close()
+    b.close()
