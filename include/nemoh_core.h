@@ -215,9 +215,17 @@ int nemoh_core_set_precision(nemoh_core *ctx, int mode, double tol, int max_iter
  * {i, j} for both elements D_ij and D_ji (the Rankine part, whose near-field
  * switch is the source panel's, once per element); 0 = every element on its
  * own.  Both agree to rounding. */
+/* NEMOH_OPT_LU_BAND: 16 (default), 32 or 64 = height, in 64-row tiles, of
+ * the bands in which the trailing update deals its tiles to the XCDs (each
+ * XCD walks one contiguous run of band-high strips column by column); the
+ * factors do not depend on it.
+ * NEMOH_OPT_LU_TUNED: 1 = the update rank and band come from the
+ * autotuner's table (nemoh_core_autotune) for the handle's precision and
+ * number of 64-row blocks when it has an entry; 0 (default) = the handle's
+ * own NEMOH_OPT_LU_RANK and NEMOH_OPT_LU_BAND. */
 enum { NEMOH_OPT_STEPWISE_SOLVE = 1, NEMOH_OPT_FP32_ASSEMBLY = 2, NEMOH_OPT_SOLVE_SPIN_LIMIT = 3,
        NEMOH_OPT_GRAPHS = 4, NEMOH_OPT_LU_RANK = 5, NEMOH_OPT_LU_LOOKAHEAD = 6, NEMOH_OPT_LU_FUSED_PANEL = 7,
-       NEMOH_OPT_PAIRED_ASSEMBLY = 8 };
+       NEMOH_OPT_PAIRED_ASSEMBLY = 8, NEMOH_OPT_LU_BAND = 9, NEMOH_OPT_LU_TUNED = 10 };
 int nemoh_core_set_option(nemoh_core *ctx, int option, int value);
 
 /* Timings (ms) of the last solve's phases and refinement statistics. */
@@ -236,8 +244,28 @@ typedef struct {
   int lu_precision; /* NEMOH_FP64 or NEMOH_MIXED (complex64 factors) */
   int solve_fallbacks; /* chained solves redone stepwise after a bounded wait gave up (this handle) */
   int graph_replays;   /* frequencies replayed from a HIP graph (this handle) */
+  int lu_rank, lu_band; /* factorisation form of the last factorisation (0 before the first) */
 } nemoh_stats;
 int nemoh_core_get_stats(const nemoh_core *ctx, nemoh_stats *out);
+
+/* Autotuner (no counterpart in the reference): times the factorisation of
+ * an order-nfa test matrix (well conditioned, pivots near 4) in the handle's
+ * precision under each of the NEMOH_TUNE_CANDIDATES forms -- update rank 64
+ * or 128 x tile band 16, 32 or 64 -- taking the fastest of reps runs of each,
+ * records the fastest form in a process-wide table for that precision and
+ * number of 64-row blocks, and turns NEMOH_OPT_LU_TUNED on for this handle.
+ * entries (optional, NEMOH_TUNE_CANDIDATES of them) get each form's time,
+ * chosen (optional) the index of the one recorded.  The handle's matrices
+ * are overwritten (the next solve factorises again).  The two ranks
+ * eliminate in a different order, so factors (and results, within
+ * rounding) depend on the choice; a process whose table is fixed gives the
+ * same results for the same inputs. */
+#define NEMOH_TUNE_CANDIDATES 6
+typedef struct {
+  int rank, band;
+  double ms; /* fastest factorisation time of the form */
+} nemoh_tune_entry;
+int nemoh_core_autotune(nemoh_core *ctx, int reps, nemoh_tune_entry *entries, int *chosen);
 int nemoh_core_enable_timing(nemoh_core *ctx, int on);
 
 /* The handle's last error message; with ctx = NULL, the calling thread's

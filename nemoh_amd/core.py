@@ -22,6 +22,9 @@ OPT_LU_RANK = 5
 OPT_LU_LOOKAHEAD = 6
 OPT_LU_FUSED_PANEL = 7
 OPT_PAIRED_ASSEMBLY = 8
+OPT_LU_BAND = 9
+OPT_LU_TUNED = 10
+TUNE_CANDIDATES = 6
 
 EXPORTS = (
     "nemoh_core_create", "nemoh_core_solve", "nemoh_core_solve_multi", "nemoh_core_solve_frequency",
@@ -32,7 +35,7 @@ EXPORTS = (
     "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_core_kochin_noalloc", "nemoh_core_potential", "nemoh_server_create", "nemoh_server_submit",
     "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_memory", "nemoh_server_last_error",
     "nemoh_server_destroy", "nemoh_batch_create", "nemoh_batch_solve_waves", "nemoh_batch_last_error",
-    "nemoh_batch_destroy",
+    "nemoh_batch_destroy", "nemoh_core_autotune",
 )
 
 
@@ -70,7 +73,12 @@ class Stats(C.Structure):
                 ("ms_total", C.c_double), ("refine_iterations", C.c_int), ("refine_residual", C.c_double),
                 ("cache_hit", C.c_int), ("ms_lu_update", C.c_double), ("lu_update_launches", C.c_int),
                 ("lu_update_flops", C.c_double), ("lu_precision", C.c_int),
-                ("solve_fallbacks", C.c_int), ("graph_replays", C.c_int)]
+                ("solve_fallbacks", C.c_int), ("graph_replays", C.c_int), ("lu_rank", C.c_int),
+                ("lu_band", C.c_int)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("rank", C.c_int), ("band", C.c_int), ("ms", C.c_double)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -136,6 +144,7 @@ def load():
     L.nemoh_batch_solve_waves.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp]
     L.nemoh_batch_last_error.argtypes = [C.c_void_p]
     L.nemoh_batch_last_error.restype = C.c_char_p
+    L.nemoh_core_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(TuneEntry), C.POINTER(C.c_int)]
     L.nemoh_batch_destroy.argtypes = [C.c_void_p]
     L.nemoh_batch_destroy.restype = None
     _lib = L
@@ -258,6 +267,16 @@ class Core:
     def _check(self, r):
         if r:
             raise NemohError(r, load().nemoh_core_last_error(self.h).decode())
+
+    def autotune(self, reps: int = 3):
+        """Times the factorisation forms (update rank x tile band) at this
+        handle's size and precision; returns ([(rank, band, ms)], chosen
+        index).  The choice goes to the process's table and this handle uses
+        it from now on."""
+        e = (TuneEntry * TUNE_CANDIDATES)()
+        k = C.c_int()
+        self._check(load().nemoh_core_autotune(self.h, int(reps), e, C.byref(k)))
+        return [(x.rank, x.band, x.ms) for x in e], k.value
 
     def set_precision(self, mode: int, tol: float = 0.0, max_iter: int = 0):
         self._check(load().nemoh_core_set_precision(self.h, int(mode), float(tol), int(max_iter)))

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -125,7 +126,7 @@ struct nemoh_core {
   // invalidates every graph (their kernel arguments hold the old pointers)
   bool use_graphs = true;
   struct GraphEntry {
-    int refactor, mode, fd, nrhs, stepwise;
+    int refactor, mode, fd, nrhs, stepwise, form;
     int uses;               // launches of this shape since the graphs were last dropped
     hipGraphExec_t exec;    // captured on the kGraphAfter-th use
   };
@@ -139,6 +140,11 @@ struct nemoh_core {
   std::vector<double> upd_flops;   // their algorithmic flops
   int upd_launches = 0;            // how many upd_ev pairs the last factorisation stamped
   int lu_rank = 128;               // NEMOH_OPT_LU_RANK: rank of the trailing updates (64 or 128)
+  int lu_band = 16;                // NEMOH_OPT_LU_BAND: tile bands of the trailing update (16, 32, 64)
+  // NEMOH_OPT_LU_TUNED: the factorisation form comes from the autotuner's
+  // table for this precision and size class when it has an entry
+  bool lu_tuned = false;
+  int lu_rank_used = 0, lu_band_used = 0;  // form of the last factorisation
   // NEMOH_OPT_LU_FUSED_PANEL: second panel inside the cross launch (off:
   // measured equal to the separate launch, 3.36 ms per 4000-panel
   // factorisation either way, without the in-launch wait)
@@ -161,6 +167,21 @@ namespace {
 std::recursive_mutex& device_lock() {
   static std::recursive_mutex m;
   return m;
+}
+
+// The autotuner's choices (nemoh_core_autotune): factorisation form (update
+// rank, tile-band height) by precision and number of 64-row blocks, for the
+// handles of this process that opt in (NEMOH_OPT_LU_TUNED).
+struct TunedForm {
+  int rank, band;
+};
+std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+std::map<std::pair<int, int>, TunedForm>& tune_table() {
+  static std::map<std::pair<int, int>, TunedForm> t;
+  return t;
 }
 
 // The last creation error of this thread (nemoh_core_last_error(NULL)).
@@ -371,6 +392,18 @@ int check_info(nemoh_core* c) {
 // What the resident matrices hold: precision, and complex64 D and S.
 int matrix_mode(const nemoh_core* c) { return c->prec + (c64_matrices(c) ? 2 : 0); }
 
+// The factorisation form the handle uses now: the tuner's entry for its
+// precision and size class when it opted in and one exists, else its own.
+TunedForm lu_form(const nemoh_core* c) {
+  TunedForm f{c->lu_rank, c->lu_band};
+  if (c->lu_tuned) {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    auto it = tune_table().find({c->prec, (c->nfa + kNB - 1) / kNB});
+    if (it != tune_table().end()) f = it->second;
+  }
+  return f;
+}
+
 // Queues assembly + factorisation of the current frequency (matrices
 // allocated; no host wait and, without timing, no event: capturable).
 int queue_factor(nemoh_core* c) {
@@ -405,6 +438,9 @@ int queue_factor(nemoh_core* c) {
   // dependency events of the look-ahead form (2 per block pair, created by
   // ensure_matrices, outside any graph capture)
   hipStream_t st2 = c->lookahead && c->lu_dep.size() >= 2 * (size_t)((nblk + 1) / 2) ? c->st2 : nullptr;
+  const TunedForm form = lu_form(c);
+  c->lu_rank_used = form.rank;
+  c->lu_band_used = form.band;
   for (int b = 0; b < c->nb; ++b) {
     hipEvent_t* ue = c->timing ? c->upd_ev.data() + 2 * (size_t)c->upd_launches : nullptr;
     double* uf = c->timing ? c->upd_flops.data() + c->upd_launches : nullptr;
@@ -414,12 +450,12 @@ int queue_factor(nemoh_core* c) {
     int* luctl = c->lu_fused ? c->d_chain + 4 : nullptr;
     if (c->prec == NEMOH_MIXED)
       c->upd_launches += launch_lu<float>(reinterpret_cast<C2<float>*>(c->D32[b]), c->ld32, c->nfa,
-                                          reinterpret_cast<C2<float>*>(c->Pinv32[b]), c->d_info, c->st, c->lu_rank,
-                                          st2, c->lu_dep.data(), ue, uf, luctl, kLuSpinLimit);
+                                          reinterpret_cast<C2<float>*>(c->Pinv32[b]), c->d_info, c->st, form.rank,
+                                          st2, c->lu_dep.data(), ue, uf, luctl, kLuSpinLimit, form.band);
     else
       c->upd_launches += launch_lu<double>(reinterpret_cast<C2<double>*>(c->D[b]), c->nfa, c->nfa,
-                                           reinterpret_cast<C2<double>*>(c->Pinv[b]), c->d_info, c->st, c->lu_rank,
-                                           st2, c->lu_dep.data(), ue, uf, luctl, kLuSpinLimit);
+                                           reinterpret_cast<C2<double>*>(c->Pinv[b]), c->d_info, c->st, form.rank,
+                                           st2, c->lu_dep.data(), ue, uf, luctl, kLuSpinLimit, form.band);
   }
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->st));
@@ -550,12 +586,16 @@ int launch_frequency(nemoh_core* c, bool refactor, int nrhs, bool stepwise) {
   };
   if (!c->use_graphs || c->timing) return eager();
   if (c->graphs_gen != c->layout_gen) drop_graphs(c);
-  const int key[5] = {refactor ? 1 : 0, matrix_mode(c), c->h_par.fd, nrhs, stepwise ? 1 : 0};
+  const TunedForm form = lu_form(c);
+  const int key[6] = {refactor ? 1 : 0, matrix_mode(c), c->h_par.fd, nrhs, stepwise ? 1 : 0,
+                      form.rank * 1000 + form.band};
   nemoh_core::GraphEntry* g = nullptr;
   for (auto& e : c->graphs)
-    if (e.refactor == key[0] && e.mode == key[1] && e.fd == key[2] && e.nrhs == key[3] && e.stepwise == key[4]) g = &e;
+    if (e.refactor == key[0] && e.mode == key[1] && e.fd == key[2] && e.nrhs == key[3] && e.stepwise == key[4] &&
+        e.form == key[5])
+      g = &e;
   if (!g) {
-    c->graphs.push_back({key[0], key[1], key[2], key[3], key[4], 0, nullptr});
+    c->graphs.push_back({key[0], key[1], key[2], key[3], key[4], key[5], 0, nullptr});
     g = &c->graphs.back();
   }
   if (!g->exec && ++g->uses < kGraphAfter) return eager();
@@ -1036,6 +1076,9 @@ int nemoh_core_create(nemoh_core** out, int np, int nfa, int nsym, const double*
     const int r = std::atoi(e);
     if (r == 64 || r == 128) c->lu_rank = r;
   }
+  // NEMOH_LU_TUNED=1: the handle takes its factorisation form from the
+  // autotuner's table (as NEMOH_OPT_LU_TUNED), likewise
+  if (const char* e = std::getenv("NEMOH_LU_TUNED")) c->lu_tuned = std::atoi(e) != 0;
   if (int r = core_bind_mesh(c, np, nfa, nsym, x, y, z, m1, m2, m3, m4, geo, zer, g, rho, depth)) return bail(r);
   *out = c;
   return NEMOH_OK;
@@ -1155,6 +1198,18 @@ int nemoh_core_set_option(nemoh_core* c, int option, int value) {
     }
     return NEMOH_OK;
   }
+  if (option == NEMOH_OPT_LU_BAND) {
+    if (value != 16 && value != 32 && value != 64) return fail(c, NEMOH_ERR_ARG, "LU tile band must be 16, 32 or 64");
+    c->lu_band = value;  // the factors do not depend on it
+    return NEMOH_OK;
+  }
+  if (option == NEMOH_OPT_LU_TUNED) {
+    if (c->lu_tuned != (value != 0)) {
+      c->lu_tuned = value != 0;
+      c->w_prev = -1.0;  // the factors on the device may be of the other form
+    }
+    return NEMOH_OK;
+  }
   if (option == NEMOH_OPT_PAIRED_ASSEMBLY) {
     if (c->paired_assembly != (value != 0)) {
       c->paired_assembly = value != 0;
@@ -1175,6 +1230,55 @@ int nemoh_core_set_option(nemoh_core* c, int option, int value) {
   return fail(c, NEMOH_ERR_ARG, "unknown option");
 }
 
+int nemoh_core_autotune(nemoh_core* c, int reps, nemoh_tune_entry* entries, int* chosen) {
+  if (!c || reps < 1 || reps > 100) return NEMOH_ERR_ARG;
+  if (int r = ensure_matrices(c)) return r;
+  const bool mixed = c->prec == NEMOH_MIXED;
+  const int n = c->nfa;
+  static const TunedForm cand[NEMOH_TUNE_CANDIDATES] = {{64, 16}, {64, 32}, {64, 64}, {128, 16}, {128, 32}, {128, 64}};
+  double best = 0;
+  int ib = -1;
+  for (int k = 0; k < NEMOH_TUNE_CANDIDATES; ++k) {
+    float mn = 0;
+    for (int r = 0; r < reps; ++r) {
+      if (mixed) {
+        auto* A = reinterpret_cast<C2<float>*>(c->D32[0]);
+        launch_fill_test_matrix(A, c->ld32, n, c->st);
+        HIPCHK(c, hipEventRecord(c->ev[0], c->st));
+        launch_lu<float>(A, c->ld32, n, reinterpret_cast<C2<float>*>(c->Pinv32[0]), c->d_info, c->st, cand[k].rank,
+                         nullptr, nullptr, nullptr, nullptr, nullptr, 1 << 20, cand[k].band);
+      } else {
+        auto* A = reinterpret_cast<C2<double>*>(c->D[0]);
+        launch_fill_test_matrix(A, n, n, c->st);
+        HIPCHK(c, hipEventRecord(c->ev[0], c->st));
+        launch_lu<double>(A, n, n, reinterpret_cast<C2<double>*>(c->Pinv[0]), c->d_info, c->st, cand[k].rank,
+                          nullptr, nullptr, nullptr, nullptr, nullptr, 1 << 20, cand[k].band);
+      }
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipEventRecord(c->ev[1], c->st));
+      HIPCHK(c, hipEventSynchronize(c->ev[1]));
+      float ms = 0;
+      HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+      if (r == 0 || ms < mn) mn = ms;
+    }
+    if (entries) entries[k] = nemoh_tune_entry{cand[k].rank, cand[k].band, (double)mn};
+    if (ib < 0 || mn < best) {
+      best = mn;
+      ib = k;
+    }
+  }
+  launch_info_reset(c->d_info, c->st);
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    tune_table()[{c->prec, (n + kNB - 1) / kNB}] = cand[ib];
+  }
+  c->lu_tuned = true;
+  c->w_prev = -1.0;  // the matrices now hold test data
+  if (chosen) *chosen = ib;
+  return NEMOH_OK;
+}
+
 int nemoh_core_enable_timing(nemoh_core* c, int on) {
   if (!c) return NEMOH_ERR_ARG;
   c->timing = on != 0;
@@ -1184,6 +1288,8 @@ int nemoh_core_enable_timing(nemoh_core* c, int on) {
 int nemoh_core_get_stats(const nemoh_core* c, nemoh_stats* out) {
   if (!c || !out) return NEMOH_ERR_ARG;
   *out = c->stats;
+  out->lu_rank = c->lu_rank_used;
+  out->lu_band = c->lu_band_used;
   return NEMOH_OK;
 }
 
@@ -1673,14 +1779,16 @@ int batch_build(nemoh_batch* b, const std::vector<int>& nrhs) {
       if (b->prec == NEMOH_MIXED) {
         auto* A = reinterpret_cast<C2<float>*>(c->D32[k]);
         auto* P = reinterpret_cast<C2<float>*>(c->Pinv32[k]);
-        m32.push_back(LuJob<float>{A, P, c->d_info, c->ld32, c->nfa, 0, 0, 0, PanelJob{0, 0, 0, 0, 0, 0}});
+        m32.push_back(LuJob<float>{A, P, c->d_info, c->ld32, c->nfa, 0, 0, 0, PanelJob{0, 0, 0, 0, 0, 0},
+                                   lu_form(c).band});
         t32[k].push_back(TrsvJob<float>{A, P, reinterpret_cast<C2<float>*>(c->R32),
                                         reinterpret_cast<C2<float>*>(c->W32), ctl, c->d_info + 1, c->d_rs + k,
                                         c->ld32, c->nfa, nrhs[i], 0, 0});
       } else {
         auto* A = reinterpret_cast<C2<double>*>(c->D[k]);
         auto* P = reinterpret_cast<C2<double>*>(c->Pinv[k]);
-        m64.push_back(LuJob<double>{A, P, c->d_info, c->nfa, c->nfa, 0, 0, 0, PanelJob{0, 0, 0, 0, 0, 0}});
+        m64.push_back(LuJob<double>{A, P, c->d_info, c->nfa, c->nfa, 0, 0, 0, PanelJob{0, 0, 0, 0, 0, 0},
+                                    lu_form(c).band});
         t64[k].push_back(TrsvJob<double>{A, P, reinterpret_cast<C2<double>*>(c->Z[k]),
                                          reinterpret_cast<C2<double>*>(c->W), ctl, c->d_info + 1, nullptr, c->nfa,
                                          c->nfa, nrhs[i], 0, 0});
@@ -1781,6 +1889,8 @@ int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const doubl
   for (size_t i = 0; i < nbody; ++i) {
     nemoh_core* c = b->bodies[i];
     if (c->prec != b->prec) return batch_fail(b, NEMOH_ERR_ARG, "a body's precision changed outside the batch");
+    if (lu_form(c).rank != 128)
+      return batch_fail(b, NEMOH_ERR_ARG, "body " + std::to_string(i) + ": batches factor in the rank-128 form");
     nrhs[i] = nbeta + c->nrad;
     if (nrhs[i] > b->max_rhs)
       return batch_fail(b, NEMOH_ERR_ARG, "body " + std::to_string(i) + ": more problems than the batch was made for");

@@ -104,10 +104,16 @@ void launch_assemble_surface(const Geometry& G, const Tables& T, const FreqParam
 // paired form folds each pair's second panel into its cross launch, whose
 // workgroups wait (at most spin_limit polls; info[1] |= 2 when one gives up)
 // for the pivot block inverted inside the same launch.
+// band: height, in 64-row tiles, of the bands in which the trailing update
+// deals its tiles to the XCDs (update_tile; the factors do not depend on it).
 template <typename T>
 int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int rank = 128, hipStream_t st2 = nullptr,
               hipEvent_t* dep = nullptr, hipEvent_t* upd_ev = nullptr, double* upd_flops = nullptr,
-              int* luctl = nullptr, int spin_limit = 1 << 20);
+              int* luctl = nullptr, int spin_limit = 1 << 20, int band = 16);
+// Writes a well-conditioned n x n test matrix (pivots near 4) into A, for
+// timing factorisations.
+template <typename T>
+void launch_fill_test_matrix(C2<T>* A, int ld, int n, hipStream_t st);
 // block pairs of an order-n factorisation, with one to spare
 inline int lu_pairs(int n) { return ((n + kNB - 1) / kNB + 1) / 2 + 1; }
 // Panel launch of the paired form: L panel of block column lc (nL blocks
@@ -124,6 +130,7 @@ struct LuJob {
   int* info;
   int ld, n, k, wg0, wg1;
   PanelJob J;
+  int band;  // the update's tile bands (0: the default, 16)
 };
 // Control block of the chained triangular solves of one handle: the two
 // triangles' ticket counters (reset on the device at the start of every

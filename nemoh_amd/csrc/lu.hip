@@ -633,16 +633,15 @@ union UpdLds {
 // below one) instead of all of both, as the row-major order does on every
 // XCD.  Workgroup 0 gets tile (0, 0).  The
 // map is a bijection for every m.
-template <int kBand>
-__device__ __forceinline__ void update_tile(int bid, int m, int& ti, int& tj) {
+__device__ __forceinline__ void update_tile(int bid, int m, int band, int& ti, int& tj) {
   constexpr int kXcds = 8;
   const int total = m * m, per = total / kXcds, rem = total % kXcds;
   const int x = bid % kXcds, local = bid / kXcds;
   const int idx = (x < rem) ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
-  const int g = idx / (kBand * m), w = idx - g * kBand * m;
-  const int rows = min(kBand, m - g * kBand);
+  const int g = idx / (band * m), w = idx - g * band * m;
+  const int rows = min(band, m - g * band);
   tj = w / rows;
-  ti = g * kBand + (w - tj * rows);
+  ti = g * band + (w - tj * rows);
 }
 
 // Look-ahead of the updates: workgroup 0 inverts the diagonal block it has
@@ -674,12 +673,12 @@ __device__ __forceinline__ void invert_tile(UpdLds<T>& S, const Acc64<T>& acc, i
 // block, and inverts it after its update (P_{kb+1}).
 template <typename T>
 __global__ __launch_bounds__(256) void lu_update_kernel(C2<T>* __restrict__ A, int ld, int n, int kb,
-                                                        C2<T>* __restrict__ P, int* __restrict__ info) {
+                                                        C2<T>* __restrict__ P, int* __restrict__ info, int band) {
   __shared__ UpdLds<T> S;
   const int s0 = (kb + 1) * kNB, k0 = kb * kNB;
   const int m = (n - s0 + kNB - 1) / kNB;
   int ti, tj;
-  update_tile<16>(blockIdx.x, m, ti, tj);
+  update_tile(blockIdx.x, m, band, ti, tj);
   const int r0 = s0 + ti * kNB, c0 = s0 + tj * kNB;
   const int nr = min(kNB, n - r0), nc = min(kNB, n - c0);
   C2<T>* blk = A + (size_t)r0 * ld + c0;
@@ -852,7 +851,8 @@ __global__ void lu_ctl_reset_kernel(int* __restrict__ ctl, int count) {
 // owns (s, s) and inverts it (P_s).
 template <typename T>
 __device__ __forceinline__ void lu_update128_body(UpdLds<T>& S, C2<T>* __restrict__ A, int ld, int n, int k,
-                                                  C2<T>* __restrict__ P, int* __restrict__ info, int region, int w) {
+                                                  C2<T>* __restrict__ P, int* __restrict__ info, int region, int band,
+                                                  int w) {
   const int s0 = (k + 2) * kNB, k0 = k * kNB;
   const int m = (n - s0 + kNB - 1) / kNB;
   int ti, tj;
@@ -866,11 +866,11 @@ __device__ __forceinline__ void lu_update128_body(UpdLds<T>& S, C2<T>* __restric
       tj = t & 1;
     }
   } else if (region == 2) {
-    update_tile<16>(w, m - 2, ti, tj);
+    update_tile(w, m - 2, band, ti, tj);
     ti += 2;
     tj += 2;
   } else {
-    update_tile<16>(w, m, ti, tj);
+    update_tile(w, m, band, ti, tj);
   }
   const int r0 = s0 + ti * kNB, c0 = s0 + tj * kNB;
   const int nr = min(kNB, n - r0), nc = min(kNB, n - c0);
@@ -886,9 +886,9 @@ __device__ __forceinline__ void lu_update128_body(UpdLds<T>& S, C2<T>* __restric
 template <typename T>
 __global__ __launch_bounds__(256) void lu_update128_kernel(C2<T>* __restrict__ A, int ld, int n, int k,
                                                            C2<T>* __restrict__ P, int* __restrict__ info,
-                                                           int region) {
+                                                           int region, int band) {
   __shared__ UpdLds<T> S;
-  lu_update128_body(S, A, ld, n, k, P, info, region, (int)blockIdx.x);
+  lu_update128_body(S, A, ld, n, k, P, info, region, band, (int)blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -947,7 +947,7 @@ __global__ __launch_bounds__(256) void group_update128_kernel(const LuJob<T>* __
   const int j = group_job(jobs, count, w);
   if (j < 0) return;
   const LuJob<T> J = jobs[j];
-  lu_update128_body(S, J.A, J.ld, J.n, J.k, J.P, J.info, 0, w);
+  lu_update128_body(S, J.A, J.ld, J.n, J.k, J.P, J.info, 0, J.band, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -1389,6 +1389,24 @@ __global__ void group_chain_begin_kernel(const TrsvJob<T>* __restrict__ jobs, in
   if (e < (size_t)J.n * J.nrhs) st_unsolved(J.W + e);
 }
 
+// A well-conditioned order-n test matrix for timing factorisations: 4 + 0i
+// on the diagonal, a hashed value in [-1/n, 1/n] (each part) elsewhere, so
+// that every pivot stays near 4.
+template <typename T>
+__global__ void fill_test_matrix_kernel(C2<T>* __restrict__ A, int ld, int n) {
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  unsigned h = (unsigned)e * 2654435761u;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  const T s = T(1) / T(n);
+  const T re = (i == j) ? T(4) : s * (T(h & 0xFFFF) / T(32767.5) - T(1));
+  const T im = (i == j) ? T(0) : s * (T(h >> 16) / T(32767.5) - T(1));
+  A[(size_t)i * ld + j] = C2<T>{re, im};
+}
+
 template <typename T>
 __global__ void copy_c2_kernel(const C2<T>* __restrict__ a, C2<T>* __restrict__ b, size_t count,
                                const RefineState* __restrict__ skip) {
@@ -1415,7 +1433,7 @@ __global__ void mfma_probe_kernel(const T* __restrict__ a, const T* __restrict__
 
 template <typename T>
 int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int rank, hipStream_t st2,
-              hipEvent_t* dep, hipEvent_t* upd_ev, double* upd_flops, int* luctl, int spin_limit) {
+              hipEvent_t* dep, hipEvent_t* upd_ev, double* upd_flops, int* luctl, int spin_limit, int band) {
   const int nblk = (n + kNB - 1) / kNB;
   int timed = 0;
   // trailing-update launch: events around it when timing (stamped by the
@@ -1437,7 +1455,7 @@ int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int 
       if (ncol + nrow > 0) lu_panel_kernel<T><<<ncol + nrow, 256, 0, st>>>(A, ld, n, kb, P, ncol);
       if (ncol > 0) {
         const double m = n - (kb + 1.0) * kNB;
-        update(lu_update_kernel<T>, st, ncol * ncol, kb, 8.0 * kNB * m * m);
+        update(lu_update_kernel<T>, st, ncol * ncol, kb, 8.0 * kNB * m * m, band);
       }
     }
     return timed;
@@ -1481,14 +1499,14 @@ int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int 
         const double mb = mr - 2.0 * kNB;
         (void)hipEventRecord(dep[2 * pair], st);  // this pair's panels are done
         (void)hipStreamWaitEvent(st2, dep[2 * pair], 0);
-        update(lu_update128_kernel<T>, st2, (m - 2) * (m - 2), k, 8.0 * 2 * kNB * mb * mb, 2);
+        update(lu_update128_kernel<T>, st2, (m - 2) * (m - 2), k, 8.0 * 2 * kNB * mb * mb, 2, band);
         if (bulk_done >= 0) (void)hipStreamWaitEvent(st, dep[bulk_done], 0);  // the previous bulk is done
-        update(lu_update128_kernel<T>, st, 4 * m - 4, k, 8.0 * 2 * kNB * (mr * mr - mb * mb), 1);
+        update(lu_update128_kernel<T>, st, 4 * m - 4, k, 8.0 * 2 * kNB * (mr * mr - mb * mb), 1, band);
         bulk_done = 2 * pair + 1;
         (void)hipEventRecord(dep[bulk_done], st2);  // this pair's bulk is done
       } else {
         if (bulk_done >= 0) (void)hipStreamWaitEvent(st, dep[bulk_done], 0);
-        update(lu_update128_kernel<T>, st, m * m, k, 8.0 * 2 * kNB * mr * mr, 0);
+        update(lu_update128_kernel<T>, st, m * m, k, 8.0 * 2 * kNB * mr * mr, 0, band);
         bulk_done = -1;  // on st itself: nothing to wait for
       }
     }
@@ -1502,9 +1520,9 @@ int launch_lu(C2<T>* A, int ld, int n, C2<T>* P, int* info, hipStream_t st, int 
   return timed;
 }
 template int launch_lu<double>(C2<double>*, int, int, C2<double>*, int*, hipStream_t, int, hipStream_t, hipEvent_t*,
-                               hipEvent_t*, double*, int*, int);
+                               hipEvent_t*, double*, int*, int, int);
 template int launch_lu<float>(C2<float>*, int, int, C2<float>*, int*, hipStream_t, int, hipStream_t, hipEvent_t*,
-                              hipEvent_t*, double*, int*, int);
+                              hipEvent_t*, double*, int*, int, int);
 
 template <typename T>
 void launch_lu_solve(const C2<T>* A, int ld, int n, const C2<T>* P, C2<T>* B, C2<T>* W, int nrhs, bool chained,
@@ -1563,6 +1581,7 @@ hipError_t build_lu_group(const std::vector<LuJob<T>>& mats, LuGroup<T>* g) {
   }
   auto job = [&](int i, int k, PanelJob J, int grid) {
     LuJob<T> j = mats[i];
+    if (j.band <= 0) j.band = 16;
     j.k = k;
     j.J = J;
     j.wg1 = grid;
@@ -1658,6 +1677,14 @@ template hipError_t build_trsv_group<float>(const std::vector<TrsvJob<float>>&, 
 template hipError_t build_trsv_group<double>(const std::vector<TrsvJob<double>>&, TrsvGroup<double>*);
 template void launch_trsv_group<float>(const TrsvGroup<float>&, int, hipStream_t);
 template void launch_trsv_group<double>(const TrsvGroup<double>&, int, hipStream_t);
+
+template <typename T>
+void launch_fill_test_matrix(C2<T>* A, int ld, int n, hipStream_t st) {
+  const size_t cnt = (size_t)n * n;
+  fill_test_matrix_kernel<T><<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(A, ld, n);
+}
+template void launch_fill_test_matrix<float>(C2<float>*, int, int, hipStream_t);
+template void launch_fill_test_matrix<double>(C2<double>*, int, int, hipStream_t);
 
 void launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t st) {
   mfma_probe_kernel<double><<<1, 64, 0, st>>>(a, b, d);
