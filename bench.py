@@ -466,6 +466,15 @@ def main():
     c.set_option(ncore.OPT_FP32_ASSEMBLY, 0)
     err32 = max(float((np.abs(f32[k] - f64[k]).max(0) / np.abs(f64[k]).max(0)).max()) for k in f64 if k in f32)
     outputs = run_outputs(c) if a.outputs else None
+    # the autotuner's timings of the factorisation forms at this size (one
+    # factorisation at a time; the throughput legs below keep the default
+    # form, rank 128 with 16-tile bands: they do not opt in to the table)
+    tuning = {"objective": "one factorisation at a time, fastest of 3 runs per form; the in-flight legs keep the "
+                           "default form (rank 128, band 16)"}
+    for name, prec in (("mixed", ncore.MIXED), ("fp64", ncore.FP64)):
+        c.set_precision(prec)
+        forms, k = c.autotune(reps=3)
+        tuning[name] = {"forms_ms": [[r, b, round(ms, 3)] for r, b, ms in forms], "chosen": list(forms[k][:2])}
     c.close()
     ss_mixed = ns / tmx
     ss_fp64 = ns / t64
@@ -536,6 +545,7 @@ def main():
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated, x algorithmic",
                      "launches": upd_launches, "avg_launch_us": round(1e3 * upd_ms / max(upd_launches, 1), 3),
                      "flops_per_launch_avg": upd_flops / max(upd_launches, 1)},
+        "autotune": tuning,
         "lu_factor_tflops": round(lu_flops / (fac_ms * 1e-3) / 1e12, 3),
         "assembly_gpairs_per_s": round(n * n / (float(np.median(phmx["assembly"])) * 1e-3) / 1e9, 3),
     }

@@ -12,7 +12,7 @@ from nemoh_amd import core
 
 def test_library_exports_every_declared_symbol():
     header = (ROOT / "include" / "nemoh_core.h").read_text()
-    declared = set(re.findall(r"\b(nemoh_(?:core|server)_\w+)\s*\(", header))
+    declared = set(re.findall(r"\b(nemoh_(?:core|server|batch)_\w+)\s*\(", header))
     L = core.load()
     missing = [s for s in sorted(declared) if not hasattr(L, s)]
     assert not missing, missing
