@@ -795,7 +795,7 @@ static int finish_outputs(nemoh_core* c, int nrhs, double* pressure, int nint, c
 
 // Body conditions of one frequency into c->V (ntot x (nbeta + nrad),
 // headings first) and the incident pressures into c->d_pinc.
-static int generate_waves(nemoh_core* c, double omega, int nbeta, const double* beta) {
+static int generate_waves(nemoh_core* c, double omega, int nbeta, const double* beta, bool wait = true) {
   if (!c->dofs_set) return fail(c, NEMOH_ERR_ARG, "nemoh_core_set_dofs has not been called for this mesh");
   const int nrhs = nbeta + c->nrad;
   if (nbeta < 0 || nrhs < 1 || nrhs > kMaxRhs || (nbeta > 0 && !beta))
@@ -814,7 +814,8 @@ static int generate_waves(nemoh_core* c, double omega, int nbeta, const double* 
                    c->st);
   HIPCHK(c, hipGetLastError());
   // the headings are read from host memory by the copy above before return
-  HIPCHK(c, hipStreamSynchronize(c->st));
+  // (a batch waits once for all its bodies instead)
+  if (wait) HIPCHK(c, hipStreamSynchronize(c->st));
   return NEMOH_OK;
 }
 
@@ -1702,13 +1703,14 @@ int nemoh_core_mfma_probe(int fp32, double* out) {
 
 // ---------------------------------------------------------------------------
 // Batches: many independent bodies (one handle each) solved together, each
-// heavy step one launch over all of them -- the paired assembly, every step
-// of the block LU (max_i ceil(n_i / 128) block pairs of launches instead of
-// the sum), the chained triangular solves -- with the light per-body kernels
-// (body conditions, right-hand sides, refinement, phi, forces) queued on the
-// same stream.  No counterpart in the reference, which solves one body per
-// run; every body gets the arithmetic of its own handle's path, so its
-// results equal a solve on its handle alone, bit for bit.
+// step one launch over all of them -- the paired assembly, every step of the
+// block LU (max_i ceil(n_i / 128) block pairs of launches instead of the
+// sum), the chained triangular solves, and the right-hand sides, refinement
+// steps, phi, combination and forces -- on one stream; only the frequency
+// set-up and the body conditions stay per body.  No counterpart in the
+// reference, which solves one body per run; every body gets the arithmetic
+// of its own handle's path, so its results equal a solve on its handle
+// alone, bit for bit.
 struct nemoh_batch {
   std::vector<nemoh_core*> bodies;
   int prec = NEMOH_FP64, nsym = 0, nb = 1, max_rhs = 0;
@@ -1723,6 +1725,15 @@ struct nemoh_batch {
   LuGroup<double> lu64;
   TrsvGroup<float> tr32[2];
   TrsvGroup<double> tr64[2];
+  // the per-body steps: one job table per symmetry block, the workgroups of
+  // each kind of grouped launch, the forces of every body in one buffer
+  // (body i at force_off[i]: nforce x nrhs, then nforce x nbeta) and the
+  // status words gathered after the solves
+  BodyJob* d_body[2] = {nullptr, nullptr};
+  int body_grid[4] = {0, 0, 0, 0};
+  double2* d_force = nullptr;
+  std::vector<size_t> force_off;
+  BatchStatus* d_status = nullptr;
   std::string err;
 };
 
@@ -1736,6 +1747,11 @@ int batch_fail(nemoh_batch* b, int code, const std::string& msg) {
 void batch_free_tables(nemoh_batch* b) {
   if (b->d_asm) (void)hipFree(b->d_asm);
   b->d_asm = nullptr;
+  for (auto* p : {&b->d_body[0], &b->d_body[1]}) if (*p) (void)hipFree(*p), *p = nullptr;
+  if (b->d_force) (void)hipFree(b->d_force);
+  b->d_force = nullptr;
+  if (b->d_status) (void)hipFree(b->d_status);
+  b->d_status = nullptr;
   for (auto* p : {&b->lu32.d_jobs}) if (*p) (void)hipFree(*p), *p = nullptr;
   for (auto* p : {&b->lu64.d_jobs}) if (*p) (void)hipFree(*p), *p = nullptr;
   for (int k = 0; k < 2; ++k) {
@@ -1767,9 +1783,52 @@ int batch_build(nemoh_batch* b, const std::vector<int>& nrhs) {
   std::vector<LuJob<double>> m64;
   std::vector<TrsvJob<float>> t32[2];
   std::vector<TrsvJob<double>> t64[2];
+  std::vector<BodyJob> bj[2];
   int wg = 0;
+  int bwg[4] = {0, 0, 0, 0};
+  size_t foff = 0;
+  b->force_off.assign(1, 0);
   for (size_t i = 0; i < b->bodies.size(); ++i) {
     nemoh_core* c = b->bodies[i];
+    const int nbeta = nrhs[i] - c->nrad;
+    const int nwg[4] = {(c->nfa * nrhs[i] + 255) / 256, (c->nfa + 15) / 16, c->nforce * nrhs[i], c->nforce * nbeta};
+    for (int k = 0; k < c->nb; ++k) {
+      BodyJob j{};
+      j.V = c->V;
+      j.B = c->B;
+      j.R = c->R;
+      j.Z = c->Z[k];
+      j.F = c->F[k];
+      j.Z2 = c->Z[1];
+      j.F2 = c->F[1];
+      j.R32 = c->R32;
+      j.rs = c->d_rs + k;
+      j.red = c->d_red;
+      j.D = c->D[k];
+      j.S = c->S[k];
+      j.ld = c->nfa;
+      j.n = c->nfa;
+      j.nsym = c->nsym;
+      j.nrhs = nrhs[i];
+      j.nbeta = nbeta;
+      j.nforce = c->nforce;
+      j.tol = c->tol;
+      j.rho = c->rho;
+      j.sig = c->sig;
+      j.pot = c->pot;
+      j.pres = c->pres;
+      j.pinc = c->d_pinc;
+      j.nds = c->d_fnds;
+      j.info = c->d_info;
+      for (int q = 0; q < 4; ++q) {
+        j.wg[q] = bwg[q];
+        j.nwg[q] = nwg[q];
+      }
+      bj[k].push_back(j);
+    }
+    for (int q = 0; q < 4; ++q) bwg[q] += nwg[q];
+    foff += (size_t)c->nforce * (nrhs[i] + nbeta);
+    b->force_off.push_back(foff);
     const int nt = (c->nfa + 7) / 8, nw = nt * (nt + 1) / 2;
     AsmJob a{c->G, c->T, c->d_par, mat_out(c), nt, nw, wg, wg + (nw + 3) / 4};
     wg = a.wg1;
@@ -1795,7 +1854,18 @@ int batch_build(nemoh_batch* b, const std::vector<int>& nrhs) {
       }
     }
   }
-  hipError_t e = hipMalloc((void**)&b->d_asm, std::max<size_t>(1, asmj.size()) * sizeof(AsmJob));
+  // each body's forces at its offset of the batch's force buffer
+  std::vector<BodyJob>* jobs_k = bj;
+  hipError_t e = hipMalloc((void**)&b->d_force, std::max<size_t>(1, foff) * sizeof(double2));
+  for (int k = 0; k < b->nb && e == hipSuccess; ++k) {
+    for (size_t i = 0; i < jobs_k[k].size(); ++i) jobs_k[k][i].force = b->d_force + b->force_off[i];
+    e = hipMalloc((void**)&b->d_body[k], std::max<size_t>(1, jobs_k[k].size()) * sizeof(BodyJob));
+    if (e == hipSuccess)
+      e = hipMemcpy(b->d_body[k], jobs_k[k].data(), jobs_k[k].size() * sizeof(BodyJob), hipMemcpyHostToDevice);
+  }
+  for (int q = 0; q < 4; ++q) b->body_grid[q] = bwg[q];
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_status, std::max<size_t>(1, b->bodies.size()) * sizeof(BatchStatus));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_asm, std::max<size_t>(1, asmj.size()) * sizeof(AsmJob));
   if (e == hipSuccess) e = hipMemcpy(b->d_asm, asmj.data(), asmj.size() * sizeof(AsmJob), hipMemcpyHostToDevice);
   b->asm_count = (int)asmj.size();
   b->asm_grid = wg;
@@ -1899,23 +1969,29 @@ int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const doubl
     if (int r = nemoh_core_freq_setup(c, period, &f)) return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
     if (int r = build_params(c, period, f.amh, f.nexp, f.ambda, f.ar, f.nq, f.qq, f.cq, &bfd, &w))
       return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
-    if (int r = generate_waves(c, omega, nbeta, beta)) return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
+    if (int r = generate_waves(c, omega, nbeta, beta, false))
+      return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
     if (fd >= 0 && fd != (bfd ? 1 : 0)) return batch_fail(b, NEMOH_ERR_ARG, "bodies in different depth regimes");
     fd = bfd ? 1 : 0;
     c->stats.cache_hit = 0;
     c->stats.refine_iterations = 0;
     c->stats.refine_residual = 0;
   }
+  // the headings' copies above read host memory: done before this call returns
+  if (hipStreamSynchronize(b->st) != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "batch synchronisation failed");
   for (size_t i = 1; i < nbody; ++i)
     if (nrhs[i] != nrhs[0]) return batch_fail(b, NEMOH_ERR_ARG, "bodies of a batch solve the same number of problems");
   if (batch_stale(b, nrhs))
     if (int r = batch_build(b, nrhs)) return r;
   const bool mixed = b->prec == NEMOH_MIXED;
+  const int nbj = (int)nbody;
+  const int* G = b->body_grid;
   // assembly of every body, one launch (free-surface panels after it, per body)
-  for (nemoh_core* c : b->bodies) launch_info_reset(c->d_info, b->st);
+  launch_group_info_reset(b->d_body[0], nbj, b->st);
   launch_assemble_group(b->d_asm, b->asm_count, b->asm_grid, fd, b->nsym, mixed, b->st);
   for (nemoh_core* c : b->bodies)
-    launch_assemble_surface(c->G, c->T, c->d_par, fd, c->nsym, mat_out(c), c->d_surf, c->nsurf, b->st);
+    if (c->nsurf > 0)
+      launch_assemble_surface(c->G, c->T, c->d_par, fd, c->nsym, mat_out(c), c->d_surf, c->nsurf, b->st);
   // factorisation of every matrix, one launch per step
   if (mixed) launch_lu_group(b->lu32, b->st);
   else launch_lu_group(b->lu64, b->st);
@@ -1925,92 +2001,83 @@ int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const doubl
     c->factored_prec = mode;
   }
   // solves, symmetry block by symmetry block (the handles' right-hand-side
-  // buffers serve both blocks)
+  // buffers serve both blocks), each step one launch over every body
+  int sweeps = kQueuedSweeps;  // as solve_block queues them (bodies needing more are redone below)
+  for (nemoh_core* c : b->bodies) sweeps = std::min(sweeps, c->max_iter);
   for (int k = 0; k < b->nb; ++k) {
-    for (size_t i = 0; i < nbody; ++i) {
-      nemoh_core* c = b->bodies[i];
-      launch_rhs(c->V, c->nfa, c->nsym, nrhs[i], k == 0 ? 1.0 : -1.0, c->B, b->st);
-    }
+    const BodyJob* J = b->d_body[k];
+    launch_group_elementwise(J, nbj, G[0], kGroupRhs, k == 0 ? 1.0 : -1.0, b->st);
     if (!mixed) {
-      for (size_t i = 0; i < nbody; ++i) {
-        nemoh_core* c = b->bodies[i];
-        launch_copy(c->B, c->Z[k], (size_t)c->nfa * nrhs[i], nullptr, b->st);
-      }
+      launch_group_elementwise(J, nbj, G[0], kGroupCopyBZ, 0.0, b->st);
       launch_trsv_group(b->tr64[k], b->bodies[0]->spin_limit, b->st);
     } else {
-      for (size_t i = 0; i < nbody; ++i) {
-        nemoh_core* c = b->bodies[i];
-        const size_t cnt = (size_t)c->nfa * nrhs[i];
-        RefineState* rs = c->d_rs + k;
-        launch_refine_reset(rs, b->st);
-        launch_copy(c->B, c->R, cnt, nullptr, b->st);
-        launch_refine_residual_max(c->B, cnt, c->d_red, rs, c->tol, 1, b->st);
-      }
-      int sweeps = kQueuedSweeps;  // as solve_block queues them (bodies needing more are redone below)
-      for (nemoh_core* c : b->bodies) sweeps = std::min(sweeps, c->max_iter);
+      launch_group_refine_reset(J, nbj, b->st);
+      launch_group_elementwise(J, nbj, G[0], kGroupCopyBR, 0.0, b->st);
+      launch_group_residual_max(J, nbj, 1, b->st);
       for (int sweep = 0; sweep < sweeps; ++sweep) {
-        for (size_t i = 0; i < nbody; ++i) {
-          nemoh_core* c = b->bodies[i];
-          launch_f64_to_f32(c->R, c->R32, (size_t)c->nfa * nrhs[i], c->d_rs + k, b->st);
-        }
+        launch_group_elementwise(J, nbj, G[0], kGroupToF32, 0.0, b->st);
         launch_trsv_group(b->tr32[k], b->bodies[0]->spin_limit, b->st);
-        for (size_t i = 0; i < nbody; ++i) {
-          nemoh_core* c = b->bodies[i];
-          const size_t cnt = (size_t)c->nfa * nrhs[i];
-          RefineState* rs = c->d_rs + k;
-          launch_f32_to_f64(c->R32, c->Z[k], cnt, rs, sweep == 0 ? 0 : 1, b->st);
-          launch_copy(c->B, c->R, cnt, rs, b->st);
-          residual(c, k, nrhs[i], rs);
-          launch_refine_residual_max(c->R, cnt, c->d_red, rs, c->tol, 0, b->st);
-        }
+        launch_group_elementwise(J, nbj, G[0], sweep == 0 ? kGroupToF64Set : kGroupToF64Add, 0.0, b->st);
+        launch_group_elementwise(J, nbj, G[0], kGroupCopyBRSkip, 0.0, b->st);
+        launch_group_gemv(J, nbj, G[1], 0, b->st);
+        launch_group_residual_max(J, nbj, 0, b->st);
       }
     }
-    for (size_t i = 0; i < nbody; ++i) potential_of_block(b->bodies[i], k, nrhs[i]);
+    launch_group_gemv(J, nbj, G[1], 1, b->st);
   }
-  if (hipGetLastError() != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "batch launch failed");
   // one look at every body's pivot report, wait status and refinement
-  std::vector<int> info(2 * nbody, 0);
-  std::vector<RefineState> rs(2 * nbody);
+  launch_group_status(b->d_body[0], nbj, b->nb, b->d_status, b->st);
+  std::vector<BatchStatus> status(nbody);
+  (void)hipMemcpyAsync(status.data(), b->d_status, nbody * sizeof(BatchStatus), hipMemcpyDeviceToHost, b->st);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->st) != hipSuccess)
+    return batch_fail(b, NEMOH_ERR_HIP, "batch launch failed");
+  // sigma, phi and pressure of every body; a body whose wait gave up, or
+  // whose refinement needs more sweeps, is solved again on its own handle
+  // from its factors (a cache hit), which overwrites its share
+  launch_group_combine(b->d_body[0], nbj, G[0], w, b->st);
   for (size_t i = 0; i < nbody; ++i) {
     nemoh_core* c = b->bodies[i];
-    (void)hipMemcpyAsync(&info[2 * i], c->d_info, 2 * sizeof(int), hipMemcpyDeviceToHost, b->st);
-    if (mixed) (void)hipMemcpyAsync(&rs[2 * i], c->d_rs, c->nb * sizeof(RefineState), hipMemcpyDeviceToHost, b->st);
-  }
-  if (hipStreamSynchronize(b->st) != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "batch synchronisation failed");
-  size_t fo = 0, ko = 0;
-  for (size_t i = 0; i < nbody; ++i) {
-    nemoh_core* c = b->bodies[i];
-    bool clean = info[2 * i] == 0 && info[2 * i + 1] == 0;
-    if (mixed)
-      for (int k = 0; k < c->nb; ++k) clean = clean && rs[2 * i + k].done && !rs[2 * i + k].bad;
-    if (info[2 * i]) {
+    const BatchStatus& h = status[i];
+    if (h.info[0]) {
       char buf[96];
-      std::snprintf(buf, sizeof buf, "body %zu: PIVOT INFERIEUR A 1e-20 (row %d)", i, info[2 * i]);
+      std::snprintf(buf, sizeof buf, "body %zu: PIVOT INFERIEUR A 1e-20 (row %d)", i, h.info[0]);
       c->w_prev = -1.0;
       return batch_fail(b, NEMOH_ERR_PIVOT, buf);
     }
+    bool clean = h.info[1] == 0;
+    if (mixed)
+      for (int k = 0; k < c->nb; ++k) clean = clean && h.rs[k].done && !h.rs[k].bad;
     if (clean) {
-      if (mixed) (void)refine_outcome(c, &rs[2 * i], 0);  // refinement statistics
-      launch_combine(c->Z[0], c->Z[1], c->F[0], c->F[1], c->nfa, c->nsym, nrhs[i], c->rho * w, c->sig, c->pot,
-                     c->pres, b->st);
+      if (mixed) (void)refine_outcome(c, h.rs, 0);  // refinement statistics
       c->last_nrhs = nrhs[i];
-    } else {
-      // a wait that gave up, or a refinement that needs more sweeps: this
-      // body's solves again on its own handle, from its factors (cache hit)
-      if (int r = run_pipeline(c, w, nrhs[i])) return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
-    }
-    if (forces && c->nforce > 0) {
-      if (int r = download_forces(c, c->pres, c->d_fnds, c->nforce, nrhs[i], forces + 2 * fo))
-        return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
-      fo += (size_t)c->nforce * nrhs[i];
-    }
-    if (fk && c->nforce > 0 && nbeta > 0) {
-      if (int r = download_forces(c, c->d_pinc, c->d_fnds, c->nforce, nbeta, fk + 2 * ko))
-        return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
-      ko += (size_t)c->nforce * nbeta;
+    } else if (int r = run_pipeline(c, w, nrhs[i])) {
+      return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
     }
   }
-  if (hipStreamSynchronize(b->st) != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "batch synchronisation failed");
+  // forces and Froude-Krylov forces of every body, one launch each, one copy
+  if (forces) launch_group_force(b->d_body[0], nbj, G[2], 0, b->st);
+  if (fk && nbeta > 0) launch_group_force(b->d_body[0], nbj, G[3], 1, b->st);
+  std::vector<double> hf(2 * std::max<size_t>(1, b->force_off.back()));
+  (void)hipMemcpyAsync(hf.data(), b->d_force, b->force_off.back() * sizeof(double2), hipMemcpyDeviceToHost, b->st);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->st) != hipSuccess)
+    return batch_fail(b, NEMOH_ERR_HIP, "batch force download failed");
+  // device (nforce x cols) row-major -> the caller's column-major, per body
+  size_t fo = 0, ko = 0;
+  for (size_t i = 0; i < nbody; ++i) {
+    const int nint = b->bodies[i]->nforce;
+    auto unpack = [&](const double* src, double* dst, int cols) {
+      for (int k = 0; k < nint; ++k)
+        for (int q = 0; q < cols; ++q) {
+          dst[2 * (k + (size_t)nint * q)] = src[2 * ((size_t)k * cols + q)];
+          dst[2 * (k + (size_t)nint * q) + 1] = src[2 * ((size_t)k * cols + q) + 1];
+        }
+    };
+    const double* src = hf.data() + 2 * b->force_off[i];
+    if (forces && nint > 0) unpack(src, forces + 2 * fo, nrhs[i]);
+    if (fk && nint > 0 && nbeta > 0) unpack(src + 2 * (size_t)nint * nrhs[i], fk + 2 * ko, nbeta);
+    fo += (size_t)nint * nrhs[i];
+    ko += (size_t)nint * nbeta;
+  }
   return NEMOH_OK;
 }
 

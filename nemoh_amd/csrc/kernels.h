@@ -154,6 +154,49 @@ struct TrsvJob {
   const RefineState* skip;
   int ld, n, nrhs, wg0, wg1;
 };
+// One body (and symmetry block) of a batch's grouped per-body kernels: its
+// buffers, sizes and the workgroup range it takes in each kind of grouped
+// launch (kind 0: elementwise over n x nrhs, 256 per workgroup; 1: gemv, 16
+// rows per workgroup; 2: forces, nforce x nrhs; 3: Froude-Krylov forces,
+// nforce x nbeta).  Block 0's job also carries block 1's Z and F (Z2, F2)
+// for the combination.
+struct BodyJob {
+  const double2* V;
+  double2 *B, *R, *Z, *F, *Z2, *F2;
+  float2* R32;
+  RefineState* rs;
+  double* red;  // 64 partial maxima
+  const double2 *D, *S;
+  int ld, n, nsym, nrhs, nbeta, nforce;
+  double tol, rho;
+  double2 *sig, *pot, *pres;
+  const double2* pinc;
+  const double* nds;
+  double2* force;  // nforce x nrhs, then nforce x nbeta
+  int* info;       // the body's pivot report and wait status
+  int wg[4], nwg[4];
+};
+// A body's pivot report, wait status and refinement states after a batch's
+// solves (launch_group_status).
+struct BatchStatus {
+  int info[2];
+  int pad[2];
+  RefineState rs[2];
+};
+void launch_group_info_reset(const BodyJob* jobs, int count, hipStream_t st);
+void launch_group_status(const BodyJob* jobs, int count, int nb, BatchStatus* out, hipStream_t st);
+// operations of launch_group_elementwise
+enum { kGroupRhs = 0, kGroupCopyBZ, kGroupCopyBR, kGroupCopyBRSkip, kGroupToF32, kGroupToF64Set, kGroupToF64Add };
+void launch_group_elementwise(const BodyJob* jobs, int count, int grid, int op, double bx, hipStream_t st);
+// the right-hand side's (first) or the residual's max and the convergence test
+void launch_group_residual_max(const BodyJob* jobs, int count, int first, hipStream_t st);
+void launch_group_refine_reset(const BodyJob* jobs, int count, hipStream_t st);
+// phi = 0: R -= D Z (skipped once converged); 1: F = S Z
+void launch_group_gemv(const BodyJob* jobs, int count, int grid, int phi, hipStream_t st);
+// pressure = rho i w phi with each body's rho and this frequency's w
+void launch_group_combine(const BodyJob* jobs, int count, int grid, double w, hipStream_t st);
+// fk = 0: forces into force[0, nforce nrhs); 1: Froude-Krylov forces after them
+void launch_group_force(const BodyJob* jobs, int count, int grid, int fk, hipStream_t st);
 // Grouped factorisation of many independent matrices: the launch sequence
 // of launch_lu's paired form (rank 128, one stream), each launch doing that
 // step for every matrix that has it (one workgroup range per matrix), so a

@@ -366,14 +366,12 @@ __global__ void pairs_kernel(Geometry G, Tables T, const FreqParams* __restrict_
 // flight per CU, which HBM's latency needs at full bandwidth; four waves
 // streamed at 2.9 TB/s.
 constexpr int kGemvWaves = 16;
-__global__ __launch_bounds__(64 * kGemvWaves) void gemv_kernel(const double2* __restrict__ M, int ld, int n,
-                                                               const double2* __restrict__ X, int nrhs,
-                                                               double2* __restrict__ Y, double alpha, double beta,
-                                                               const RefineState* __restrict__ rs) {
-  if (rs && rs->done) return;
-  __shared__ double red[kGemvWaves][2][16][17];
+typedef double GemvRed[kGemvWaves][2][16][17];
+__device__ __forceinline__ void gemv_body(GemvRed& red, const double2* __restrict__ M, int ld, int n,
+                                          const double2* __restrict__ X, int nrhs, double2* __restrict__ Y,
+                                          double alpha, double beta, int blk) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
-  const int r0 = blockIdx.x * 16;
+  const int r0 = blk * 16;
   const int row = r0 + li;
   const int nk = (n + 3) / 4;  // k-steps of 4 columns
   const int per = (nk + kGemvWaves - 1) / kGemvWaves;
@@ -421,6 +419,14 @@ __global__ __launch_bounds__(64 * kGemvWaves) void gemv_kernel(const double2* __
       }
     }
   }
+}
+__global__ __launch_bounds__(64 * kGemvWaves) void gemv_kernel(const double2* __restrict__ M, int ld, int n,
+                                                               const double2* __restrict__ X, int nrhs,
+                                                               double2* __restrict__ Y, double alpha, double beta,
+                                                               const RefineState* __restrict__ rs) {
+  if (rs && rs->done) return;
+  __shared__ GemvRed red;
+  gemv_body(red, M, ld, n, X, nrhs, Y, alpha, beta, (int)blockIdx.x);
 }
 
 // Y = beta Y + alpha M X with M complex64 (the mixed path's D and S) and X,
@@ -551,10 +557,10 @@ __global__ void combine_kernel(const double2* __restrict__ Z1, const double2* __
 
 // F(k, q) = -sum_c p(c, q) NDS(k, c); one workgroup per (k, q), fixed-order
 // tree reduction (deterministic, batch-invariant).
-__global__ __launch_bounds__(256) void force_kernel(const double2* __restrict__ pres, const double* __restrict__ nds,
-                                                    int ntot, int nrhs, double2* __restrict__ F) {
-  const int k = blockIdx.x / nrhs, q = blockIdx.x % nrhs;
-  __shared__ double sr[256], si[256];
+__device__ __forceinline__ void force_body(double* sr, double* si, const double2* __restrict__ pres,
+                                           const double* __restrict__ nds, int ntot, int nrhs, double2* __restrict__ F,
+                                           int blk) {
+  const int k = blk / nrhs, q = blk % nrhs;
   double ar = 0.0, ai = 0.0;
   for (int c = threadIdx.x; c < ntot; c += 256) {
     const double2 p = pres[(size_t)c * nrhs + q];
@@ -573,6 +579,11 @@ __global__ __launch_bounds__(256) void force_kernel(const double2* __restrict__ 
     __syncthreads();
   }
   if (threadIdx.x == 0) F[(size_t)k * nrhs + q] = make_double2(sr[0], si[0]);
+}
+__global__ __launch_bounds__(256) void force_kernel(const double2* __restrict__ pres, const double* __restrict__ nds,
+                                                    int ntot, int nrhs, double2* __restrict__ F) {
+  __shared__ double sr[256], si[256];
+  force_body(sr, si, pres, nds, ntot, nrhs, F, (int)blockIdx.x);
 }
 
 // Mixed-path refinement on the device.  The sweep's kernels read the
@@ -784,6 +795,202 @@ __global__ __launch_bounds__(256) void kochin_kernel(Geometry G, const double2* 
 }
 
 // ---------------------------------------------------------------------------
+// Grouped per-body kernels of a batch: the same per-element operations over
+// every body's buffers in one launch (BodyJob j: its workgroup range of each
+// kind; bodies found by binary search).  Each body gets the arithmetic of
+// its own launches, so results do not depend on the grouping.
+template <int kKind>
+__device__ __forceinline__ int body_of(const BodyJob* __restrict__ jobs, int count, int& w) {
+  const int g = blockIdx.x;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].wg[kKind] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  w = g - jobs[lo].wg[kKind];
+  return (w < jobs[lo].nwg[kKind]) ? lo : -1;
+}
+
+// elementwise steps over n x nrhs entries (kind 0)
+__global__ void group_elementwise_kernel(const BodyJob* __restrict__ jobs, int count, int op, double bx) {
+  int w;
+  const int j = body_of<0>(jobs, count, w);
+  if (j < 0) return;
+  const BodyJob& J = jobs[j];
+  const size_t e = (size_t)w * blockDim.x + threadIdx.x, cnt = (size_t)J.n * J.nrhs;
+  if (e >= cnt) return;
+  switch (op) {
+    case kGroupRhs: {  // B = the block's right-hand sides from V
+      const int i = (int)(e / J.nrhs), q = (int)(e % J.nrhs);
+      const double2 a = J.V[(size_t)i * J.nrhs + q];
+      if (J.nsym) {
+        const double2 b = J.V[(size_t)(i + J.n) * J.nrhs + q];
+        J.B[e] = make_double2((a.x + bx * b.x) * 0.5, (a.y + bx * b.y) * 0.5);
+      } else {
+        J.B[e] = a;
+      }
+      break;
+    }
+    case kGroupCopyBZ: J.Z[e] = J.B[e]; break;
+    case kGroupCopyBR: J.R[e] = J.B[e]; break;
+    case kGroupCopyBRSkip:
+      if (!J.rs->done) J.R[e] = J.B[e];
+      break;
+    case kGroupToF32:
+      if (!J.rs->done) {
+        const double scale = J.rs->rmax > 0.0 ? 1.0 / J.rs->rmax : 0.0;
+        J.R32[e] = make_float2((float)(J.R[e].x * scale), (float)(J.R[e].y * scale));
+      }
+      break;
+    case kGroupToF64Set:
+    case kGroupToF64Add:
+      if (!J.rs->done) {
+        const double scale = J.rs->rmax;
+        const double2 v = make_double2((double)J.R32[e].x * scale, (double)J.R32[e].y * scale);
+        J.Z[e] = op == kGroupToF64Add ? make_double2(J.Z[e].x + v.x, J.Z[e].y + v.y) : v;
+      }
+      break;
+    default: break;
+  }
+}
+
+// per-body maxima of |R| (or |B| for the first test) in 64 parts each, then
+// the convergence test (one wave per body), as absmax_kernel and
+// refine_check_kernel do for one body
+__global__ __launch_bounds__(256) void group_absmax_kernel(const BodyJob* __restrict__ jobs, int count, int first) {
+  const int j = blockIdx.x / 64, part = blockIdx.x % 64;
+  if (j >= count) return;
+  const BodyJob& J = jobs[j];
+  if (!first && J.rs->done) return;
+  const double2* a = first ? J.B : J.R;
+  const size_t cnt = (size_t)J.n * J.nrhs;
+  __shared__ double s[256];
+  double m = 0.0;
+  for (size_t e = part * (size_t)256 + threadIdx.x; e < cnt; e += (size_t)64 * 256)
+    m = nan_max(nan_max(m, fabs(a[e].x)), fabs(a[e].y));
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) s[threadIdx.x] = nan_max(s[threadIdx.x], s[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) J.red[part] = s[0];
+}
+__global__ void group_refine_check_kernel(const BodyJob* __restrict__ jobs, int count, int first) {
+  const int j = blockIdx.x;
+  if (j >= count) return;
+  const BodyJob& J = jobs[j];
+  RefineState* rs = J.rs;
+  if (rs->done) return;
+  double m = nan_max(0.0, J.red[threadIdx.x]);  // 64 partial maxima, one per lane
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
+  if (threadIdx.x != 0) return;
+  if (!(m <= 1.79e308)) {  // NaN or infinite: stop, the host reports it
+    rs->bad = 1;
+    rs->rmax = m;
+    rs->done = 1;
+    return;
+  }
+  rs->rmax = m;
+  if (first) {
+    rs->bmax = m;
+    rs->iters = 0;
+    return;
+  }
+  rs->iters += 1;
+  if (m <= J.tol * rs->bmax) rs->done = 1;
+}
+__global__ void group_refine_reset_kernel(const BodyJob* __restrict__ jobs, int count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  RefineState* rs = jobs[j].rs;
+  rs->bmax = rs->rmax = 0.0;
+  rs->done = rs->iters = rs->bad = 0;
+}
+
+// the residual R -= D Z (skipped once converged) or phi F = S Z (kind 1:
+// ceil(n / 16) workgroups per body)
+__global__ __launch_bounds__(64 * kGemvWaves) void group_gemv_kernel(const BodyJob* __restrict__ jobs, int count,
+                                                                     int phi) {
+  int w;
+  const int j = body_of<1>(jobs, count, w);
+  if (j < 0) return;
+  const BodyJob& J = jobs[j];
+  if (!phi && J.rs->done) return;
+  __shared__ GemvRed red;
+  if (phi) gemv_body(red, J.S, J.ld, J.n, J.Z, J.nrhs, J.F, 1.0, 0.0, w);
+  else gemv_body(red, J.D, J.ld, J.n, J.Z, J.nrhs, J.R, -1.0, 1.0, w);
+}
+
+// sigma, phi and pressure of both symmetry blocks (block 0's jobs carry
+// block 1's Z and F in Z2, F2), as combine_kernel
+__global__ void group_combine_kernel(const BodyJob* __restrict__ jobs, int count, double omega) {
+  int w;
+  const int j = body_of<0>(jobs, count, w);
+  if (j < 0) return;
+  const BodyJob& J = jobs[j];
+  const int e = w * blockDim.x + threadIdx.x;
+  if (e >= J.n * J.nrhs) return;
+  const int i = e / J.nrhs, q = e % J.nrhs, nrhs = J.nrhs, n = J.n;
+  double2 zb, zs, pb, ps;
+  if (J.nsym) {
+    const double2 a = J.Z[e], b = J.Z2[e], c = J.F[e], d = J.F2[e];
+    zb = make_double2(a.x + b.x, a.y + b.y);
+    zs = make_double2(a.x - b.x, a.y - b.y);
+    pb = make_double2(c.x + d.x, c.y + d.y);
+    ps = make_double2(c.x - d.x, c.y - d.y);
+  } else {
+    zb = J.Z[e];
+    zs = make_double2(0.0, 0.0);
+    pb = J.F[e];
+    ps = make_double2(0.0, 0.0);
+  }
+  J.sig[(size_t)i * nrhs + q] = zb;
+  J.sig[(size_t)(i + n) * nrhs + q] = zs;
+  J.pot[(size_t)i * nrhs + q] = pb;
+  J.pot[(size_t)(i + n) * nrhs + q] = ps;
+  const double rho_w = J.rho * omega;
+  J.pres[(size_t)i * nrhs + q] = make_double2(-rho_w * pb.y, rho_w * pb.x);
+  if (J.nsym) J.pres[(size_t)(i + n) * nrhs + q] = make_double2(-rho_w * ps.y, rho_w * ps.x);
+}
+
+// forces -NDS . p of every body (kind 2: nforce x nrhs workgroups) and the
+// Froude-Krylov forces from the incident pressure (kind 3: nforce x nbeta),
+// into force[0, nforce nrhs) and after it
+__global__ __launch_bounds__(256) void group_force_kernel(const BodyJob* __restrict__ jobs, int count, int fk) {
+  int w;
+  const int j = fk ? body_of<3>(jobs, count, w) : body_of<2>(jobs, count, w);
+  if (j < 0) return;
+  const BodyJob& J = jobs[j];
+  __shared__ double sr[256], si[256];
+  const int ntot = J.n * (J.nsym ? 2 : 1);
+  if (fk) force_body(sr, si, J.pinc, J.nds, ntot, J.nbeta, J.force + (size_t)J.nforce * J.nrhs, w);
+  else force_body(sr, si, J.pres, J.nds, ntot, J.nrhs, J.force, w);
+}
+
+// every body's pivot report reset (before the factorisation), and every
+// body's report and refinement states gathered into one array (after the
+// solves): out[j] = {info[0], info[1]}, then the block states
+__global__ void group_info_reset_kernel(const BodyJob* __restrict__ jobs, int count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  jobs[j].info[0] = 0;
+  jobs[j].info[1] = 0;
+}
+__global__ void group_status_kernel(const BodyJob* __restrict__ jobs, int count, int nb, BatchStatus* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  BatchStatus s;
+  s.info[0] = jobs[j].info[0];
+  s.info[1] = jobs[j].info[1];
+  s.rs[0] = jobs[j].rs[0];
+  if (nb > 1) s.rs[1] = jobs[j].rs[1];
+  out[j] = s;
+}
+
+// ---------------------------------------------------------------------------
 // host-side launchers
 void launch_crek(const double* xr, const double* xz, double* tab, hipStream_t st) {
   crek_kernel<<<(kNR * kNZ + 127) / 128, 128, 0, st>>>(xr, xz, tab);
@@ -899,6 +1106,33 @@ void launch_matvec(const double2* M, int ld, int n, const double2* X, int nrhs, 
 void launch_matvec32(const float2* M, int ld, int n, const double2* X, int nrhs, double2* Y, double alpha,
                      double beta, hipStream_t st, const RefineState* skip) {
   gemv32_kernel<<<(n + 15) / 16, 64 * kGemv32Waves, 0, st>>>(M, ld, n, X, nrhs, Y, alpha, beta, skip);
+}
+
+void launch_group_elementwise(const BodyJob* jobs, int count, int grid, int op, double bx, hipStream_t st) {
+  if (count > 0 && grid > 0) group_elementwise_kernel<<<grid, 256, 0, st>>>(jobs, count, op, bx);
+}
+void launch_group_residual_max(const BodyJob* jobs, int count, int first, hipStream_t st) {
+  if (count <= 0) return;
+  group_absmax_kernel<<<64 * count, 256, 0, st>>>(jobs, count, first);
+  group_refine_check_kernel<<<count, 64, 0, st>>>(jobs, count, first);
+}
+void launch_group_refine_reset(const BodyJob* jobs, int count, hipStream_t st) {
+  if (count > 0) group_refine_reset_kernel<<<(count + 255) / 256, 256, 0, st>>>(jobs, count);
+}
+void launch_group_gemv(const BodyJob* jobs, int count, int grid, int phi, hipStream_t st) {
+  if (count > 0 && grid > 0) group_gemv_kernel<<<grid, 64 * kGemvWaves, 0, st>>>(jobs, count, phi);
+}
+void launch_group_combine(const BodyJob* jobs, int count, int grid, double w, hipStream_t st) {
+  if (count > 0 && grid > 0) group_combine_kernel<<<grid, 256, 0, st>>>(jobs, count, w);
+}
+void launch_group_force(const BodyJob* jobs, int count, int grid, int fk, hipStream_t st) {
+  if (count > 0 && grid > 0) group_force_kernel<<<grid, 256, 0, st>>>(jobs, count, fk);
+}
+void launch_group_info_reset(const BodyJob* jobs, int count, hipStream_t st) {
+  if (count > 0) group_info_reset_kernel<<<(count + 255) / 256, 256, 0, st>>>(jobs, count);
+}
+void launch_group_status(const BodyJob* jobs, int count, int nb, BatchStatus* out, hipStream_t st) {
+  if (count > 0) group_status_kernel<<<(count + 255) / 256, 256, 0, st>>>(jobs, count, nb, out);
 }
 
 void launch_rhs(const double2* V, int n, int nsym, int nrhs, double bx, double2* B, hipStream_t st) {
