@@ -1636,7 +1636,11 @@ void launch_lu_group(const LuGroup<T>& g, hipStream_t st) {
 }
 
 template <typename T>
-hipError_t build_trsv_group(const std::vector<TrsvJob<T>>& sys, TrsvGroup<T>* g) {
+hipError_t build_trsv_group(const std::vector<TrsvJob<T>>& in, TrsvGroup<T>* g) {
+  // longest chains first: workgroups are dispatched in order, so the
+  // largest systems' chains start at once and the short ones fill in
+  std::vector<TrsvJob<T>> sys(in);
+  std::stable_sort(sys.begin(), sys.end(), [](const TrsvJob<T>& a, const TrsvJob<T>& b) { return a.n > b.n; });
   std::vector<TrsvJob<T>> begin(sys), tri(sys);
   int wb = 0, wt = 0;
   for (size_t i = 0; i < sys.size(); ++i) {
