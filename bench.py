@@ -399,9 +399,10 @@ def run_c4(nbodies, workers, seed=256):
         print(f"c4 batched {name}: {nfreq} freq-solves in {wall:.2f} s (set-up {setup:.2f} s)", file=sys.stderr,
               flush=True)
     batched["identical_to_request_loop"] = bool(same)
-    batched["how"] = ("nemoh_batch: one frequency of all bodies at a time, the assembly of every body one launch, "
-                      "each block-LU step one launch over every matrix, each triangular solve one launch; "
-                      "handles, matrices and job tables resident (set-up untimed, first frequency included in it)")
+    batched["how"] = ("nemoh_batch: one frequency of all bodies at a time, every step one launch over all bodies "
+                      "(assembly, each block-LU step, each triangular solve, right-hand sides, refinement steps, "
+                      "residual and phi gemvs, combination, forces), body conditions per body; handles, matrices "
+                      "and job tables resident (set-up untimed, first frequency included in it)")
     return {"workload": f"{nbodies} bodies (seed {seed}): cylinder/box/prolate spheroid, U(2,50) m, draft "
                         "U(0.2,0.8) of height, N log-uniform 300-6000, 7 RHS, 8 frequencies 0.2-2.0 rad/s, "
                         "infinite depth",
