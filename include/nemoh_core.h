@@ -315,6 +315,12 @@ int nemoh_batch_create(nemoh_batch **batch, int nbody, nemoh_core *const *bodies
  * from the factors already on the device. */
 int nemoh_batch_solve_waves(nemoh_batch *batch, double omega, int nbeta, const double *beta, double *forces,
                             double *fk);
+/* The same with a frequency per body (omega[i] for body i; the bodies may
+ * then fall in both depth regimes, which take one assembly launch each):
+ * e.g. the frequencies of one mesh, one handle per frequency.  Each handle
+ * keeps its own module NQ, as for nemoh_core_solve_waves on it. */
+int nemoh_batch_solve_waves_multi(nemoh_batch *batch, const double *omega, int nbeta, const double *beta,
+                                  double *forces, double *fk);
 const char *nemoh_batch_last_error(const nemoh_batch *batch);
 void nemoh_batch_destroy(nemoh_batch *batch);
 

@@ -35,7 +35,7 @@ EXPORTS = (
     "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_core_kochin_noalloc", "nemoh_core_potential", "nemoh_server_create", "nemoh_server_submit",
     "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_memory", "nemoh_server_last_error",
     "nemoh_server_destroy", "nemoh_batch_create", "nemoh_batch_solve_waves", "nemoh_batch_last_error",
-    "nemoh_batch_destroy", "nemoh_core_autotune",
+    "nemoh_batch_destroy", "nemoh_core_autotune", "nemoh_batch_solve_waves_multi",
 )
 
 
@@ -142,6 +142,7 @@ def load():
     L.nemoh_server_destroy.restype = None
     L.nemoh_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p), C.c_int, C.c_int]
     L.nemoh_batch_solve_waves.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp]
+    L.nemoh_batch_solve_waves_multi.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp]
     L.nemoh_batch_last_error.argtypes = [C.c_void_p]
     L.nemoh_batch_last_error.restype = C.c_char_p
     L.nemoh_core_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(TuneEntry), C.POINTER(C.c_int)]
@@ -455,9 +456,14 @@ class Batch:
     def __del__(self):
         self.close()
 
-    def solve_waves(self, omega: float, betas):
+    def solve_waves(self, omega, betas):
         """Per body: forces (nforce, nbeta + nrad) and Froude-Krylov forces
-        (nforce, nbeta), as Core.solve_waves gives them."""
+        (nforce, nbeta), as Core.solve_waves gives them.  omega: one
+        frequency for every body, or one per body."""
+        om = np.atleast_1d(np.asarray(omega, dtype=np.float64))
+        per_body = om.size > 1
+        if per_body and om.size != len(self.cores):
+            raise ValueError("one frequency, or one per body")
         b = _f64(np.atleast_1d(betas)) if len(betas) else np.zeros(0)
         nb = b.size
         shapes = [(c._nforce, nb + c._nrad) for c in self.cores]
@@ -465,8 +471,13 @@ class Batch:
         ktotal = sum(c._nforce * nb for c in self.cores)
         f = np.zeros(max(total, 1), np.complex128)
         k = np.zeros(max(ktotal, 1), np.complex128)
-        r = load().nemoh_batch_solve_waves(self.h, float(omega), nb, _p(b) if nb else None, f.ctypes.data_as(_dp),
-                                           k.ctypes.data_as(_dp))
+        if per_body:
+            om = _f64(om)
+            r = load().nemoh_batch_solve_waves_multi(self.h, _p(om), nb, _p(b) if nb else None,
+                                                     f.ctypes.data_as(_dp), k.ctypes.data_as(_dp))
+        else:
+            r = load().nemoh_batch_solve_waves(self.h, float(om[0]), nb, _p(b) if nb else None,
+                                               f.ctypes.data_as(_dp), k.ctypes.data_as(_dp))
         if r:
             raise NemohError(r, load().nemoh_batch_last_error(self.h).decode())
         out, fo, ko = [], 0, 0

@@ -1734,6 +1734,12 @@ struct nemoh_batch {
   double2* d_force = nullptr;
   std::vector<size_t> force_off;
   BatchStatus* d_status = nullptr;
+  // per call: each body's frequency (the combination's rho w) and, when the
+  // bodies' frequencies fall in both depth regimes, the assembly jobs of
+  // each regime (host copies of the jobs kept for that)
+  double* d_omega = nullptr;
+  std::vector<AsmJob> asm_host, asm_part;
+  AsmJob* d_asm_part = nullptr;
   std::string err;
 };
 
@@ -1752,6 +1758,10 @@ void batch_free_tables(nemoh_batch* b) {
   b->d_force = nullptr;
   if (b->d_status) (void)hipFree(b->d_status);
   b->d_status = nullptr;
+  if (b->d_omega) (void)hipFree(b->d_omega);
+  b->d_omega = nullptr;
+  if (b->d_asm_part) (void)hipFree(b->d_asm_part);
+  b->d_asm_part = nullptr;
   for (auto* p : {&b->lu32.d_jobs}) if (*p) (void)hipFree(*p), *p = nullptr;
   for (auto* p : {&b->lu64.d_jobs}) if (*p) (void)hipFree(*p), *p = nullptr;
   for (int k = 0; k < 2; ++k) {
@@ -1865,6 +1875,9 @@ int batch_build(nemoh_batch* b, const std::vector<int>& nrhs) {
   }
   for (int q = 0; q < 4; ++q) b->body_grid[q] = bwg[q];
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_status, std::max<size_t>(1, b->bodies.size()) * sizeof(BatchStatus));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_omega, std::max<size_t>(1, b->bodies.size()) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_asm_part, std::max<size_t>(1, asmj.size()) * sizeof(AsmJob));
+  b->asm_host = asmj;
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_asm, std::max<size_t>(1, asmj.size()) * sizeof(AsmJob));
   if (e == hipSuccess) e = hipMemcpy(b->d_asm, asmj.data(), asmj.size() * sizeof(AsmJob), hipMemcpyHostToDevice);
   b->asm_count = (int)asmj.size();
@@ -1942,19 +1955,19 @@ int nemoh_batch_create(nemoh_batch** out, int nbody, nemoh_core* const* bodies, 
   return NEMOH_OK;
 }
 
-// One frequency for every body of the batch: forces of body i (nforce_i x
-// (nbeta + nrad_i), column-major complex) at forces + offset_i, with offset_i
-// the sum of the earlier bodies' sizes; fk likewise (nforce_i x nbeta).
-int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const double* beta, double* forces, double* fk) {
-  if (!b) return NEMOH_ERR_ARG;
-  if (!(std::fabs(omega) >= 1.E-4)) return batch_fail(b, NEMOH_ERR_OMEGA, "ABS(WR) < 1.E-4");
+// Body i at frequency omega[i]: its forces (nforce_i x (nbeta + nrad_i),
+// column-major complex) at forces + offset_i, with offset_i the sum of the
+// earlier bodies' sizes; fk likewise (nforce_i x nbeta).  Bodies in both
+// depth regimes take one assembly launch per regime.
+static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const double* beta, double* forces,
+                       double* fk) {
+  const size_t nbody = b->bodies.size();
+  for (size_t i = 0; i < nbody; ++i)
+    if (!(std::fabs(omega[i]) >= 1.E-4)) return batch_fail(b, NEMOH_ERR_OMEGA, "ABS(WR) < 1.E-4");
   StreamSwap swap(b);
   const double pi = 4.0 * std::atan(1.0);
-  const double period = 2.0 * pi / omega;
-  const size_t nbody = b->bodies.size();
-  std::vector<int> nrhs(nbody);
-  int fd = -1;
-  double w = 0;
+  std::vector<int> nrhs(nbody), fdv(nbody);
+  std::vector<double> wv(nbody);
   // frequency set-up and body conditions of every body
   for (size_t i = 0; i < nbody; ++i) {
     nemoh_core* c = b->bodies[i];
@@ -1964,15 +1977,15 @@ int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const doubl
     nrhs[i] = nbeta + c->nrad;
     if (nrhs[i] > b->max_rhs)
       return batch_fail(b, NEMOH_ERR_ARG, "body " + std::to_string(i) + ": more problems than the batch was made for");
+    const double period = 2.0 * pi / omega[i];  // T as the Solver's main program forms it
     nemoh_freq f;
     bool bfd;
     if (int r = nemoh_core_freq_setup(c, period, &f)) return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
-    if (int r = build_params(c, period, f.amh, f.nexp, f.ambda, f.ar, f.nq, f.qq, f.cq, &bfd, &w))
+    if (int r = build_params(c, period, f.amh, f.nexp, f.ambda, f.ar, f.nq, f.qq, f.cq, &bfd, &wv[i]))
       return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
-    if (int r = generate_waves(c, omega, nbeta, beta, false))
+    if (int r = generate_waves(c, omega[i], nbeta, beta, false))
       return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
-    if (fd >= 0 && fd != (bfd ? 1 : 0)) return batch_fail(b, NEMOH_ERR_ARG, "bodies in different depth regimes");
-    fd = bfd ? 1 : 0;
+    fdv[i] = bfd ? 1 : 0;
     c->stats.cache_hit = 0;
     c->stats.refine_iterations = 0;
     c->stats.refine_residual = 0;
@@ -1986,19 +1999,44 @@ int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const doubl
   const bool mixed = b->prec == NEMOH_MIXED;
   const int nbj = (int)nbody;
   const int* G = b->body_grid;
-  // assembly of every body, one launch (free-surface panels after it, per body)
+  (void)hipMemcpyAsync(b->d_omega, wv.data(), nbody * sizeof(double), hipMemcpyHostToDevice, b->st);
+  // assembly of every body, one launch per depth regime present (free-surface
+  // panels after it, per body)
   launch_group_info_reset(b->d_body[0], nbj, b->st);
-  launch_assemble_group(b->d_asm, b->asm_count, b->asm_grid, fd, b->nsym, mixed, b->st);
-  for (nemoh_core* c : b->bodies)
+  const int nfd = (int)std::count(fdv.begin(), fdv.end(), 1);
+  if (nfd == 0 || nfd == nbj) {
+    launch_assemble_group(b->d_asm, b->asm_count, b->asm_grid, nfd ? 1 : 0, b->nsym, mixed, b->st);
+  } else {
+    // the jobs of each regime, packed with new workgroup ranges: finite
+    // depth first, then infinite depth
+    b->asm_part.clear();
+    int wg[2] = {0, 0};
+    for (int reg = 1; reg >= 0; --reg)
+      for (size_t i = 0; i < nbody; ++i)
+        if (fdv[i] == reg) {
+          AsmJob a = b->asm_host[i];
+          const int nwg = a.wg1 - a.wg0;
+          a.wg0 = wg[1 - reg];
+          a.wg1 = a.wg0 + nwg;
+          wg[1 - reg] = a.wg1;
+          b->asm_part.push_back(a);
+        }
+    (void)hipMemcpyAsync(b->d_asm_part, b->asm_part.data(), nbody * sizeof(AsmJob), hipMemcpyHostToDevice, b->st);
+    launch_assemble_group(b->d_asm_part, nfd, wg[0], 1, b->nsym, mixed, b->st);
+    launch_assemble_group(b->d_asm_part + nfd, nbj - nfd, wg[1], 0, b->nsym, mixed, b->st);
+  }
+  for (size_t i = 0; i < nbody; ++i) {
+    nemoh_core* c = b->bodies[i];
     if (c->nsurf > 0)
-      launch_assemble_surface(c->G, c->T, c->d_par, fd, c->nsym, mat_out(c), c->d_surf, c->nsurf, b->st);
+      launch_assemble_surface(c->G, c->T, c->d_par, fdv[i], c->nsym, mat_out(c), c->d_surf, c->nsurf, b->st);
+  }
   // factorisation of every matrix, one launch per step
   if (mixed) launch_lu_group(b->lu32, b->st);
   else launch_lu_group(b->lu64, b->st);
   const int mode = mixed ? NEMOH_MIXED : NEMOH_FP64;
-  for (nemoh_core* c : b->bodies) {  // the factors on the device belong to this frequency
-    c->w_prev = w;
-    c->factored_prec = mode;
+  for (size_t i = 0; i < nbody; ++i) {  // the factors on the device belong to this frequency
+    b->bodies[i]->w_prev = wv[i];
+    b->bodies[i]->factored_prec = mode;
   }
   // solves, symmetry block by symmetry block (the handles' right-hand-side
   // buffers serve both blocks), each step one launch over every body
@@ -2034,7 +2072,7 @@ int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const doubl
   // sigma, phi and pressure of every body; a body whose wait gave up, or
   // whose refinement needs more sweeps, is solved again on its own handle
   // from its factors (a cache hit), which overwrites its share
-  launch_group_combine(b->d_body[0], nbj, G[0], w, b->st);
+  launch_group_combine(b->d_body[0], nbj, G[0], b->d_omega, b->st);
   for (size_t i = 0; i < nbody; ++i) {
     nemoh_core* c = b->bodies[i];
     const BatchStatus& h = status[i];
@@ -2050,7 +2088,7 @@ int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const doubl
     if (clean) {
       if (mixed) (void)refine_outcome(c, h.rs, 0);  // refinement statistics
       c->last_nrhs = nrhs[i];
-    } else if (int r = run_pipeline(c, w, nrhs[i])) {
+    } else if (int r = run_pipeline(c, wv[i], nrhs[i])) {
       return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
     }
   }
@@ -2079,6 +2117,18 @@ int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const doubl
     ko += (size_t)nint * nbeta;
   }
   return NEMOH_OK;
+}
+
+int nemoh_batch_solve_waves(nemoh_batch* b, double omega, int nbeta, const double* beta, double* forces, double* fk) {
+  if (!b) return NEMOH_ERR_ARG;
+  const std::vector<double> w(b->bodies.size(), omega);
+  return batch_solve(b, w.data(), nbeta, beta, forces, fk);
+}
+
+int nemoh_batch_solve_waves_multi(nemoh_batch* b, const double* omega, int nbeta, const double* beta, double* forces,
+                                  double* fk) {
+  if (!b || !omega) return NEMOH_ERR_ARG;
+  return batch_solve(b, omega, nbeta, beta, forces, fk);
 }
 
 const char* nemoh_batch_last_error(const nemoh_batch* b) {

@@ -193,8 +193,9 @@ void launch_group_residual_max(const BodyJob* jobs, int count, int first, hipStr
 void launch_group_refine_reset(const BodyJob* jobs, int count, hipStream_t st);
 // phi = 0: R -= D Z (skipped once converged); 1: F = S Z
 void launch_group_gemv(const BodyJob* jobs, int count, int grid, int phi, hipStream_t st);
-// pressure = rho i w phi with each body's rho and this frequency's w
-void launch_group_combine(const BodyJob* jobs, int count, int grid, double w, hipStream_t st);
+// pressure = rho i w phi with each body's rho and frequency (omega: device
+// array, one per body)
+void launch_group_combine(const BodyJob* jobs, int count, int grid, const double* omega, hipStream_t st);
 // fk = 0: forces into force[0, nforce nrhs); 1: Froude-Krylov forces after them
 void launch_group_force(const BodyJob* jobs, int count, int grid, int fk, hipStream_t st);
 // Grouped factorisation of many independent matrices: the launch sequence

@@ -925,8 +925,9 @@ __global__ __launch_bounds__(64 * kGemvWaves) void group_gemv_kernel(const BodyJ
 }
 
 // sigma, phi and pressure of both symmetry blocks (block 0's jobs carry
-// block 1's Z and F in Z2, F2), as combine_kernel
-__global__ void group_combine_kernel(const BodyJob* __restrict__ jobs, int count, double omega) {
+// block 1's Z and F in Z2, F2), as combine_kernel; omega[j]: body j's
+// frequency
+__global__ void group_combine_kernel(const BodyJob* __restrict__ jobs, int count, const double* __restrict__ omega) {
   int w;
   const int j = body_of<0>(jobs, count, w);
   if (j < 0) return;
@@ -951,7 +952,7 @@ __global__ void group_combine_kernel(const BodyJob* __restrict__ jobs, int count
   J.sig[(size_t)(i + n) * nrhs + q] = zs;
   J.pot[(size_t)i * nrhs + q] = pb;
   J.pot[(size_t)(i + n) * nrhs + q] = ps;
-  const double rho_w = J.rho * omega;
+  const double rho_w = J.rho * omega[j];
   J.pres[(size_t)i * nrhs + q] = make_double2(-rho_w * pb.y, rho_w * pb.x);
   if (J.nsym) J.pres[(size_t)(i + n) * nrhs + q] = make_double2(-rho_w * ps.y, rho_w * ps.x);
 }
@@ -1122,8 +1123,8 @@ void launch_group_refine_reset(const BodyJob* jobs, int count, hipStream_t st) {
 void launch_group_gemv(const BodyJob* jobs, int count, int grid, int phi, hipStream_t st) {
   if (count > 0 && grid > 0) group_gemv_kernel<<<grid, 64 * kGemvWaves, 0, st>>>(jobs, count, phi);
 }
-void launch_group_combine(const BodyJob* jobs, int count, int grid, double w, hipStream_t st) {
-  if (count > 0 && grid > 0) group_combine_kernel<<<grid, 256, 0, st>>>(jobs, count, w);
+void launch_group_combine(const BodyJob* jobs, int count, int grid, const double* omega, hipStream_t st) {
+  if (count > 0 && grid > 0) group_combine_kernel<<<grid, 256, 0, st>>>(jobs, count, omega);
 }
 void launch_group_force(const BodyJob* jobs, int count, int grid, int fk, hipStream_t st) {
   if (count > 0 && grid > 0) group_force_kernel<<<grid, 256, 0, st>>>(jobs, count, fk);

@@ -9,7 +9,8 @@ Cases: bodies of the batched workload's generator with uneven panel counts
 (odd and even numbers of 64-row blocks, one body alone in the last block
 pairs) in infinite depth; boxes in finite depth (the finite-depth assembly
 and the exponential images); half meshes with y-symmetry (two symmetry
-blocks per body); both precisions; and the batch's fallback for a chained
+blocks per body); both precisions; a frequency per body (one mesh at many
+frequencies, in both depth regimes at once); and the batch's fallback for a chained
 solve that gives up waiting (spin limit 0), which redoes a body on its own
 handle.
 """
@@ -53,6 +54,29 @@ def _check(meshes, depth, omegas, betas, prec, spin=None):
         c.close()
 
 
+def _check_multi(meshes, depth, omega_rows, betas, prec):
+    """Body i at omega_rows[r][i] in call r; each lone handle makes the same
+    calls, so the module state a handle carries across calls (NQ) matches."""
+    tables = core.build_tables()
+    alone = []
+    for i, m in enumerate(meshes):
+        c = _handle(m, depth, tables, prec)
+        alone.append([c.solve_waves(row[i], betas) for row in omega_rows])
+        c.close()
+    cores = [_handle(m, depth, tables, prec) for m in meshes]
+    b = core.Batch(cores, prec, max_rhs=len(betas) + 6)
+    for r, row in enumerate(omega_rows):
+        got = b.solve_waves(np.asarray(row), betas)
+        for k, (f, fk) in enumerate(got):
+            fa, fka = alone[k][r]
+            assert np.all(np.isfinite(f)) and f.shape == fa.shape
+            assert np.array_equal(f, fa), (k, row[k], float(np.abs(f - fa).max() / np.abs(fa).max()))
+            assert np.array_equal(fk, fka), (k, row[k])
+    b.close()
+    for c in cores:
+        c.close()
+
+
 def _c4_meshes(n, nrange, seed):
     rng = np.random.default_rng(seed)
     return [serving.random_body_parts(rng, nrange)[0] for _ in range(n)]
@@ -80,6 +104,21 @@ def test_batch_equals_single_bodies_symmetric(prec):
     _check(ms, 0.0, [0.6, 1.5], np.zeros(1), prec)
 
 
+@pytest.mark.parametrize("prec", [core.MIXED, core.FP64])
+def test_batch_frequency_per_body_one_mesh(prec):
+    """The frequencies of one mesh, one handle each (nemoh_batch_solve_waves_multi):
+    at depth 30 m, 3.0 rad/s is past k h = 20 and takes the infinite-depth
+    assembly, so the first call mixes both regimes and the second (rolled) too."""
+    m = nmesh.box(30.0, 10.0, 5.0, 15, 5, 3)
+    row = [0.5, 3.0, 1.1, 0.8, 2.0]
+    _check_multi([m] * len(row), 30.0, [row, row[1:] + row[:1]], np.deg2rad([0.0, 30.0]), prec)
+
+
+def test_batch_frequency_per_body_mixed_bodies():
+    ms = _c4_meshes(4, (150, 900), 21)
+    _check_multi(ms, 0.0, [[0.4, 0.9, 1.6, 2.5], [1.2, 1.2, 0.3, 0.7]], np.zeros(1), core.MIXED)
+
+
 def test_batch_chained_solve_fallback():
     """Spin limit 0: every chained wait gives up at once, and each body is
     solved again on its own handle (stepwise), as a handle alone does."""
@@ -101,6 +140,11 @@ def test_batch_argument_errors():
         bt.solve_waves(0.8, np.zeros(3))  # 3 headings + 6 dofs > max_rhs
     f = bt.solve_waves(0.8, np.zeros(1))
     assert np.all(np.isfinite(f[0][0]))
+    with pytest.raises(ValueError):
+        bt.solve_waves([0.8, 0.9], np.zeros(1))  # two frequencies for one body
+    with pytest.raises(core.NemohError) as e:
+        bt.solve_waves([0.0], np.zeros(1))
+    assert e.value.code == core.ERR_OMEGA
     bt.close()
     a.close()
     b.close()
