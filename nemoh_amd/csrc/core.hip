@@ -1765,8 +1765,8 @@ void batch_free_tables(nemoh_batch* b) {
   for (auto* p : {&b->lu32.d_jobs}) if (*p) (void)hipFree(*p), *p = nullptr;
   for (auto* p : {&b->lu64.d_jobs}) if (*p) (void)hipFree(*p), *p = nullptr;
   for (int k = 0; k < 2; ++k) {
-    for (auto* p : {&b->tr32[k].d_begin, &b->tr32[k].d_tri}) if (*p) (void)hipFree(*p), *p = nullptr;
-    for (auto* p : {&b->tr64[k].d_begin, &b->tr64[k].d_tri}) if (*p) (void)hipFree(*p), *p = nullptr;
+    free_trsv_group(&b->tr32[k]);
+    free_trsv_group(&b->tr64[k]);
   }
   b->gens.clear();
   b->built_rhs = -1;

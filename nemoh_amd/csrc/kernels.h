@@ -225,11 +225,15 @@ void launch_lu_group(const LuGroup<T>& g, hipStream_t st);
 template <typename T>
 struct TrsvGroup {
   TrsvJob<T>* d_begin = nullptr;  // chain_begin's workgroup ranges
-  TrsvJob<T>* d_tri = nullptr;    // the triangles' (one workgroup per row block)
-  int count = 0, grid_begin = 0, grid_tri = 0;
+  TrsvJob<T>* d_tri = nullptr;    // the systems, largest first
+  int* d_cum = nullptr;           // [nlevel + 1]: tickets dealt before each chain position
+  unsigned* d_ticket = nullptr;   // the two triangles' launch-wide ticket counters
+  int count = 0, grid_begin = 0, grid_tri = 0, nlevel = 0;
 };
 template <typename T>
 hipError_t build_trsv_group(const std::vector<TrsvJob<T>>& sys, TrsvGroup<T>* g);
+template <typename T>
+void free_trsv_group(TrsvGroup<T>* g);
 template <typename T>
 void launch_trsv_group(const TrsvGroup<T>& g, int spin_limit, hipStream_t st);
 // Solves A x = b for nrhs <= kMaxRhs right-hand sides (B: n x nrhs

@@ -1156,53 +1156,17 @@ __device__ __forceinline__ void hop_mark_unsolved(C2<T>* __restrict__ X, int nb,
   }
 }
 
-// Single-launch triangular solve.  Each workgroup takes the next ticket w of
-// the launch from a counter (atomic add) and owns row block b (forward:
-// b = w, backward: b = nblk-1-w, so blocks without dependencies come first);
-// for every solved block j, in dependency order, it loads the coupling block
-// (which does not depend on x_j), polls x_j until no element holds the
-// unsolved marker and subtracts M_bj x_j, then stores x_b.  Each wave polls
-// and proceeds on its own.  A workgroup only ever waits for blocks whose
-// tickets were taken before its own, i.e. by workgroups that are already
-// running, so the chain makes progress however few of its workgroups are
-// resident (other streams' kernels and other chains may hold the rest of the
-// GPU): the lowest unsolved block never waits.  Every wait is bounded: after
-// spin_limit polls (at once when it is 0) the wave records a timeout in
-// *status and stops waiting, so the kernel always ends (its result is then
-// discarded by the caller).
-// R: right-hand sides (forward) or the forward solution y (backward); X gets
-// the solution and holds the marker in every element when the launch starts.
-// The forward launch also marks its R (the backward launch's X) block by
-// block once it has read it.  ctl->ticket[tri] is 0 when the launch starts
-// (set, with X's marker, by chain_begin_kernel for the forward triangle), so
-// the launches take no per-call argument and replay from a HIP graph.
-//
-// The critical path is one hand-over per block (x_b's stores reaching memory
-// and the next workgroup's poll seeing them) plus the product with the block
-// just solved.  Superblocks of two blocks per workgroup (128 rows, 8 waves,
-// the inner coupling through LDS) halve the hand-overs but make the product
-// on the critical path four times larger, 128 x 128 against 64 x 64, i.e. 3.4
-// against 0.85 us of one CU's FP32 MFMA peak: measured 480 us per triangle
-// at 4000 panels against 307 us for the 64-row form (both with flags).
+// The workgroup at chain position w (0 <= w < row blocks; the forward chain
+// solves row block w, the backward chain row block nblk - 1 - w) of one
+// triangle: it waits only for the positions before w.
 template <typename T, bool kLower>
-__device__ __forceinline__ void trsv_chain_body(int& my_ticket, const C2<T>* __restrict__ A, int ld, int n,
-                                                const C2<T>* __restrict__ P, C2<T>* __restrict__ R,
-                                                C2<T>* __restrict__ X, int nrhs, ChainCtl* __restrict__ ctl, int tri,
-                                                int* __restrict__ status, int spin_limit,
-                                                const RefineState* __restrict__ skip) {
+__device__ __forceinline__ void trsv_chain_run(int w, const C2<T>* __restrict__ A, int ld, int n,
+                                               const C2<T>* __restrict__ P, C2<T>* __restrict__ R,
+                                               C2<T>* __restrict__ X, int nrhs, int* __restrict__ status,
+                                               int spin_limit) {
   typedef typename MfmaTraits<T>::acc_t acc_t;
-  const int tid = threadIdx.x, lane = tid & 63;
+  const int lane = threadIdx.x & 63;
   const int nblk = (n + kNB - 1) / kNB;
-  // a converged refinement: nothing is done (every workgroup reads the same
-  // state, and no ticket is taken)
-  if (skip && skip->done) return;
-  if (tid == 0) my_ticket = (int)atomicAdd(&ctl->ticket[tri], 1u);
-  __syncthreads();
-  const int w = my_ticket;
-  if (w < 0 || w >= nblk) {  // a counter out of step with the launches: never wait on it
-    if (tid == 0) atomicMax(status, 1);
-    return;
-  }
   const int b = kLower ? w : nblk - 1 - w;
   const int k0 = b * kNB, nb = min(kNB, n - k0);
   // (cr, ci) = -(the block's running value): -r_b (forward) or -P_b y_b
@@ -1286,6 +1250,56 @@ __device__ __forceinline__ void trsv_chain_body(int& my_ticket, const C2<T>* __r
   // into it
   if (kLower) hop_mark_unsolved<T>(R + (size_t)k0 * nrhs, nb, nrhs);
 }
+
+// Single-launch triangular solve.  Each workgroup takes the next ticket w of
+// the launch from a counter (atomic add) and owns row block b (forward:
+// b = w, backward: b = nblk-1-w, so blocks without dependencies come first);
+// for every solved block j, in dependency order, it loads the coupling block
+// (which does not depend on x_j), polls x_j until no element holds the
+// unsolved marker and subtracts M_bj x_j, then stores x_b.  Each wave polls
+// and proceeds on its own.  A workgroup only ever waits for blocks whose
+// tickets were taken before its own, i.e. by workgroups that are already
+// running, so the chain makes progress however few of its workgroups are
+// resident (other streams' kernels and other chains may hold the rest of the
+// GPU): the lowest unsolved block never waits.  Every wait is bounded: after
+// spin_limit polls (at once when it is 0) the wave records a timeout in
+// *status and stops waiting, so the kernel always ends (its result is then
+// discarded by the caller).
+// R: right-hand sides (forward) or the forward solution y (backward); X gets
+// the solution and holds the marker in every element when the launch starts.
+// The forward launch also marks its R (the backward launch's X) block by
+// block once it has read it.  ctl->ticket[tri] is 0 when the launch starts
+// (set, with X's marker, by chain_begin_kernel for the forward triangle), so
+// the launches take no per-call argument and replay from a HIP graph.
+//
+// The critical path is one hand-over per block (x_b's stores reaching memory
+// and the next workgroup's poll seeing them) plus the product with the block
+// just solved.  Superblocks of two blocks per workgroup (128 rows, 8 waves,
+// the inner coupling through LDS) halve the hand-overs but make the product
+// on the critical path four times larger, 128 x 128 against 64 x 64, i.e. 3.4
+// against 0.85 us of one CU's FP32 MFMA peak: measured 480 us per triangle
+// at 4000 panels against 307 us for the 64-row form (both with flags).
+template <typename T, bool kLower>
+__device__ __forceinline__ void trsv_chain_body(int& my_ticket, const C2<T>* __restrict__ A, int ld, int n,
+                                                const C2<T>* __restrict__ P, C2<T>* __restrict__ R,
+                                                C2<T>* __restrict__ X, int nrhs, ChainCtl* __restrict__ ctl, int tri,
+                                                int* __restrict__ status, int spin_limit,
+                                                const RefineState* __restrict__ skip) {
+  const int tid = threadIdx.x;
+  const int nblk = (n + kNB - 1) / kNB;
+  // a converged refinement: nothing is done (every workgroup reads the same
+  // state, and no ticket is taken)
+  if (skip && skip->done) return;
+  if (tid == 0) my_ticket = (int)atomicAdd(&ctl->ticket[tri], 1u);
+  __syncthreads();
+  const int w = my_ticket;
+  if (w < 0 || w >= nblk) {  // a counter out of step with the launches: never wait on it
+    if (tid == 0) atomicMax(status, 1);
+    return;
+  }
+  trsv_chain_run<T, kLower>(w, A, ld, n, P, R, X, nrhs, status, spin_limit);
+}
+
 template <typename T, bool kLower>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict__ A, int ld, int n,
                                                          const C2<T>* __restrict__ P, C2<T>* __restrict__ R,
@@ -1296,26 +1310,44 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const C2<T>* __restrict
   trsv_chain_body<T, kLower>(my_ticket, A, ld, n, P, R, X, nrhs, ctl, tri, status, spin_limit, skip);
 }
 
-// Grouped chained solves: one launch per triangle for many systems (TrsvJob
-// j: workgroups [wg0, wg1), one per row block).  A workgroup finds its
-// system and takes its ticket from that system's counter, so it waits only
-// for row blocks of its own system whose tickets were taken earlier, by
-// running workgroups: the argument of the single-system launch holds per
-// system.  chain_begin's grouped form gives each system ceil(n nrhs / 256)
-// workgroups.
+// Grouped chained solves: one launch per triangle for many systems, one
+// workgroup per row block of each.  The workgroups take tickets from one
+// counter for the whole launch (ticket[tri], 0 when the launch starts), dealt
+// in chain order across the systems: level b (chain position b) of every
+// system that has one, then level b + 1, with the systems sorted by size so
+// that those having level b are jobs [0, cum[b+1] - cum[b]) and ticket t at
+// level b (cum[b] <= t < cum[b+1]) is system t - cum[b].  A workgroup waits
+// only for earlier positions of its own system, whose tickets are smaller,
+// i.e. taken by running workgroups: the argument of the single-system launch
+// holds.  Dealt system by system instead, the launch ran its chains a few
+// residents' worth at a time (40 chains of 63 blocks at 4000 panels: 2 ms a
+// triangle, ten times one chain's time).  Each position runs the single
+// system's arithmetic, so results equal the single launches' bit for bit.
 template <typename T, bool kLower>
 __global__ __launch_bounds__(256) void group_trsv_kernel(const TrsvJob<T>* __restrict__ jobs, int count,
-                                                         int spin_limit) {
+                                                         const int* __restrict__ cum, int nlevel,
+                                                         unsigned* __restrict__ ticket, int spin_limit) {
   __shared__ int my_ticket;
-  int w;
-  const int j = group_job(jobs, count, w);
-  if (j < 0) return;
-  const TrsvJob<T> J = jobs[j];
+  if (threadIdx.x == 0) my_ticket = (int)atomicAdd(ticket + (kLower ? 0 : 1), 1u);
+  __syncthreads();
+  const int t = my_ticket;
+  if (t < 0 || t >= cum[nlevel]) {  // a counter out of step with the launches: never wait on it
+    if (threadIdx.x == 0)
+      for (int j = 0; j < count; ++j) atomicMax(jobs[j].status, 1);
+    return;
+  }
+  int lo = 0, hi = nlevel - 1;  // the level: the last b with cum[b] <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cum[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const TrsvJob<T> J = jobs[t - cum[lo]];
+  if (J.skip && J.skip->done) return;  // a converged refinement: nothing to do
   if (kLower)
-    trsv_chain_body<T, true>(my_ticket, J.A, J.ld, J.n, J.P, J.B, J.W, J.nrhs, J.ctl, 0, J.status, spin_limit, J.skip);
+    trsv_chain_run<T, true>(lo, J.A, J.ld, J.n, J.P, J.B, J.W, J.nrhs, J.status, spin_limit);
   else
-    trsv_chain_body<T, false>(my_ticket, J.A, J.ld, J.n, J.P, J.W, J.B, J.nrhs, J.ctl, 1, J.status, spin_limit,
-                              J.skip);
+    trsv_chain_run<T, false>(lo, J.A, J.ld, J.n, J.P, J.W, J.B, J.nrhs, J.status, spin_limit);
 }
 
 // Stepwise fallback (one launch per block step s, for matrices with more row
@@ -1374,17 +1406,20 @@ __global__ void chain_begin_kernel(ChainCtl* __restrict__ ctl, C2<T>* __restrict
   if (e < count) st_unsolved(X + e);
 }
 
+// The grouped form: every system's forward X marked, and the launch-wide
+// ticket counters of both grouped triangles back to 0.
 template <typename T>
-__global__ void group_chain_begin_kernel(const TrsvJob<T>* __restrict__ jobs, int count) {
+__global__ void group_chain_begin_kernel(const TrsvJob<T>* __restrict__ jobs, int count,
+                                         unsigned* __restrict__ ticket) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ticket[0] = 0u;
+    ticket[1] = 0u;
+  }
   int w;
   const int j = group_job(jobs, count, w);
   if (j < 0) return;
   const TrsvJob<T> J = jobs[j];
   const size_t e = (size_t)w * blockDim.x + threadIdx.x;
-  if (e == 0) {
-    J.ctl->ticket[0] = 0u;
-    J.ctl->ticket[1] = 0u;
-  }
   if (J.skip && J.skip->done) return;
   if (e < (size_t)J.n * J.nrhs) st_unsolved(J.W + e);
 }
@@ -1636,9 +1671,19 @@ void launch_lu_group(const LuGroup<T>& g, hipStream_t st) {
 }
 
 template <typename T>
+void free_trsv_group(TrsvGroup<T>* g) {
+  for (void* p : {(void*)g->d_begin, (void*)g->d_tri, (void*)g->d_cum, (void*)g->d_ticket})
+    if (p) (void)hipFree(p);
+  g->d_begin = g->d_tri = nullptr;
+  g->d_cum = nullptr;
+  g->d_ticket = nullptr;
+  g->count = g->grid_begin = g->grid_tri = g->nlevel = 0;
+}
+
+template <typename T>
 hipError_t build_trsv_group(const std::vector<TrsvJob<T>>& in, TrsvGroup<T>* g) {
-  // longest chains first: workgroups are dispatched in order, so the
-  // largest systems' chains start at once and the short ones fill in
+  // largest systems first: the systems that have chain position b are then
+  // a prefix of the list, for every b
   std::vector<TrsvJob<T>> sys(in);
   std::stable_sort(sys.begin(), sys.end(), [](const TrsvJob<T>& a, const TrsvJob<T>& b) { return a.n > b.n; });
   std::vector<TrsvJob<T>> begin(sys), tri(sys);
@@ -1651,26 +1696,40 @@ hipError_t build_trsv_group(const std::vector<TrsvJob<T>>& in, TrsvGroup<T>* g) 
     tri[i].wg1 = wt + (sys[i].n + kNB - 1) / kNB;
     wt = tri[i].wg1;
   }
-  for (TrsvJob<T>** p : {&g->d_begin, &g->d_tri})
-    if (*p) (void)hipFree(*p);
-  g->d_begin = g->d_tri = nullptr;
+  // cum[b]: tickets dealt before level b (the systems with more than b row
+  // blocks, summed over the levels below b)
+  const int nlevel = sys.empty() ? 0 : (sys[0].n + kNB - 1) / kNB;
+  std::vector<int> cum(nlevel + 1, 0);
+  for (int b = 0; b < nlevel; ++b) {
+    int have = 0;
+    for (const TrsvJob<T>& j : sys) have += (j.n + kNB - 1) / kNB > b ? 1 : 0;
+    cum[b + 1] = cum[b] + have;
+  }
+  free_trsv_group(g);
   g->count = (int)sys.size();
   g->grid_begin = wb;
   g->grid_tri = wt;
+  g->nlevel = nlevel;
+  if (cum[nlevel] != wt) return hipErrorInvalidValue;  // every row block gets one ticket
   const size_t bytes = std::max<size_t>(1, sys.size()) * sizeof(TrsvJob<T>);
   hipError_t e = hipMalloc((void**)&g->d_begin, bytes);
   if (e == hipSuccess) e = hipMalloc((void**)&g->d_tri, bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&g->d_cum, cum.size() * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void**)&g->d_ticket, 2 * sizeof(unsigned));
   if (e == hipSuccess) e = hipMemcpy(g->d_begin, begin.data(), sys.size() * sizeof(TrsvJob<T>), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->d_tri, tri.data(), sys.size() * sizeof(TrsvJob<T>), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->d_cum, cum.data(), cum.size() * sizeof(int), hipMemcpyHostToDevice);
   return e;
 }
 
 template <typename T>
 void launch_trsv_group(const TrsvGroup<T>& g, int spin_limit, hipStream_t st) {
   if (g.count == 0) return;
-  group_chain_begin_kernel<T><<<g.grid_begin, 256, 0, st>>>(g.d_begin, g.count);
-  group_trsv_kernel<T, true><<<g.grid_tri, 256, 0, st>>>(g.d_tri, g.count, spin_limit);
-  group_trsv_kernel<T, false><<<g.grid_tri, 256, 0, st>>>(g.d_tri, g.count, spin_limit);
+  group_chain_begin_kernel<T><<<g.grid_begin, 256, 0, st>>>(g.d_begin, g.count, g.d_ticket);
+  group_trsv_kernel<T, true><<<g.grid_tri, 256, 0, st>>>(g.d_tri, g.count, g.d_cum, g.nlevel, g.d_ticket,
+                                                          spin_limit);
+  group_trsv_kernel<T, false><<<g.grid_tri, 256, 0, st>>>(g.d_tri, g.count, g.d_cum, g.nlevel, g.d_ticket,
+                                                           spin_limit);
 }
 
 template hipError_t build_lu_group<float>(const std::vector<LuJob<float>>&, LuGroup<float>*);
@@ -1681,6 +1740,8 @@ template hipError_t build_trsv_group<float>(const std::vector<TrsvJob<float>>&, 
 template hipError_t build_trsv_group<double>(const std::vector<TrsvJob<double>>&, TrsvGroup<double>*);
 template void launch_trsv_group<float>(const TrsvGroup<float>&, int, hipStream_t);
 template void launch_trsv_group<double>(const TrsvGroup<double>&, int, hipStream_t);
+template void free_trsv_group<float>(TrsvGroup<float>*);
+template void free_trsv_group<double>(TrsvGroup<double>*);
 
 template <typename T>
 void launch_fill_test_matrix(C2<T>* A, int ld, int n, hipStream_t st) {
