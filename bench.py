@@ -16,9 +16,10 @@ step, every triangular solve, the refinement's steps, the gemvs, the
 forces) one launch over all 40.  The plain FP64 path is measured the same
 way, and both also through the native request loop (several frequencies in
 flight on their own streams); the speed-up is taken against the faster
-FP64 rate.  The single-stream rates, per-phase times and the roofline of
-the dominant kernel come from a one-frequency-at-a-time leg, where each
-kernel runs alone.  Also reported: the mixed-vs-FP64 error, the 12k-panel
+FP64 rate.  `roofline` is that of value's dominant kernel, the grouped
+rank-128 LU update, timed over the value leg's timed passes.  The
+single-stream rates, per-phase times and the same kernel's roofline one
+matrix at a time come from a one-frequency-at-a-time leg.  Also reported: the mixed-vs-FP64 error, the 12k-panel
 semi-submersible, the 256-body batch, serving latency under Poisson arrivals
 and the reference Solver routines' CPU time on a bounded sample.
 """
@@ -56,6 +57,11 @@ HBM_PEAK_GBS = 8000.0
 # 41.00 MB written against 85.93 MB algorithmic on average over 6386 launches
 # (profiles/lu_update128_barge4000_hbm_traffic.md)
 LU_UPDATE_TRAFFIC_RATIO = 1.156
+# The same for group_update128_kernel<float>, the grouped update of the 40
+# matrices of a batch pass (value's dominant kernel): 2270.9 MB fetched +
+# 1640.0 MB written against 3437.4 MB algorithmic per launch on average over
+# 62 launches (profiles/group_update128_barge4000_batch40_hbm_traffic.md)
+BATCH_UPDATE_TRAFFIC_RATIO = 1.138
 
 
 def workload():
@@ -209,7 +215,9 @@ def run_batched_frequencies(m, omegas, prec, passes, warmup, ref):
     each).  Handles, matrices and job tables resident; set-up and `warmup`
     passes (at least one: it builds the job tables) untimed.  Returns
     (seconds for the timed passes, whether every force equals the
-    single-stream one bit for bit)."""
+    single-stream one bit for bit, and the grouped rank-128 updates of the
+    timed passes as (ms, algorithmic flops, algorithmic bytes, launches),
+    timed with HIP events on the batch's stream)."""
     from nemoh_amd import bodycond as bcm
     from nemoh_amd import core as ncore
     tables = ncore.build_tables()
@@ -221,15 +229,17 @@ def run_batched_frequencies(m, omegas, prec, passes, warmup, ref):
     b = ncore.Batch(cores, prec, max_rhs=9)
     for _ in range(max(1, warmup)):
         got = b.solve_waves(omegas, BETAS)
+    b.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(passes):
         got = b.solve_waves(omegas, BETAS)
     dt = time.perf_counter() - t0
+    upd = b.update_time()
     same = all(np.array_equal(got[k][0], ref[k]) for k in range(omegas.size) if k in ref)
     b.close()
     for h in cores:
         h.close()
-    return dt, same
+    return dt, same, upd
 
 
 REF_HARNESS = ROOT / "oracle" / "_ref" / "harness"
@@ -516,8 +526,8 @@ def main():
     ss_fp64 = ns / t64
     # the timed job (value): a.steps passes over the 40 frequencies, each pass
     # one batch call over 40 handles of the mesh, one frequency each
-    bmx, bsame_mx = run_batched_frequencies(m, omegas, ncore.MIXED, a.steps, a.warmup, fmx)
-    b64, bsame_64 = run_batched_frequencies(m, omegas, ncore.FP64, fp64_steps, a.warmup, f64)
+    bmx, bsame_mx, bupd = run_batched_frequencies(m, omegas, ncore.MIXED, a.steps, a.warmup, fmx)
+    b64, bsame_64, _ = run_batched_frequencies(m, omegas, ncore.FP64, fp64_steps, a.warmup, f64)
     rate_mixed = a.steps * nw / bmx
     rate_fp64_batch = fp64_steps * nw / b64
     # the same passes through the request loop, a.workers frequencies in flight
@@ -533,10 +543,16 @@ def main():
         all(np.array_equal(g64[k], f64[k]) for k in f64 if k in g64)
     common = sorted(set(g64) & set(gmx))
     err = max(float((np.abs(gmx[k] - g64[k]).max(0) / np.abs(g64[k]).max(0)).max()) for k in common)
-    # roofline of the dominant MFMA kernel, the LU trailing update
-    # (lu_update128_kernel<float> on the mixed path): algorithmic flops of its
-    # launches (8 * 128 * m^2 each, m = trailing order) over their summed
-    # durations, timed with HIP events on the solver stream
+    # roofline of value's dominant kernel, the batch's grouped rank-128 LU
+    # update (group_update128_kernel<float>): algorithmic flops of its
+    # launches in the timed passes (8 * 128 * m^2 per matrix and launch, m =
+    # trailing order) over their summed durations, HIP events on the batch's
+    # stream; bytes likewise, scaled by the measured HBM traffic ratio
+    b_ms, b_flops, b_bytes, b_launches = bupd
+    b_achieved = b_flops / (b_ms * 1e-3) / 1e12 if b_ms > 0 else 0.0
+    b_bytes_launch = b_bytes / max(b_launches, 1)
+    # the same kernel body one frequency at a time (lu_update128_kernel<float>
+    # on the single-stream leg, HIP events on the solver stream)
     upd_ms = float(np.sum(phmx["update"]))
     upd_flops = float(np.sum(phmx["update_flops"]))
     upd_launches = int(np.sum(phmx["update_launches"]))
@@ -585,16 +601,23 @@ def main():
             "mixed_fp32_assembly": {k: round(float(np.median(v)), 3) for k, v in ph32.items()
                                     if k in ("assembly", "factor", "solve")},
             "mixed_refine_iterations_max": int(max(phmx["it"]) if phmx["it"] else 0)},
-        "roofline": {"bound": "mfma", "kernel": "lu_update128_kernel<float> (rank-128 LU trailing update, complex64 "
-                                                 "MFMA)",
-                     "measured_on": "single-stream leg (one frequency in flight)",
-                     "achieved": round(achieved, 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK["fp32"], 4),
-                     "traffic": round(LU_UPDATE_TRAFFIC_RATIO * upd_bytes),
-                     "algorithmic_bytes_per_launch": round(upd_bytes),
+        "roofline": {"bound": "mfma", "kernel": "group_update128_kernel<float> (rank-128 LU trailing update of the "
+                                                 "40 matrices of a batch pass, complex64 MFMA)",
+                     "measured_on": "value's timed batch passes",
+                     "achieved": round(b_achieved, 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
+                     "frac": round(b_achieved / PEAK["fp32"], 4),
+                     "traffic": round(BATCH_UPDATE_TRAFFIC_RATIO * b_bytes_launch),
+                     "algorithmic_bytes_per_launch": round(b_bytes_launch),
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated, x algorithmic",
-                     "launches": upd_launches, "avg_launch_us": round(1e3 * upd_ms / max(upd_launches, 1), 3),
-                     "flops_per_launch_avg": upd_flops / max(upd_launches, 1)},
+                     "launches": b_launches, "avg_launch_us": round(1e3 * b_ms / max(b_launches, 1), 3),
+                     "flops_per_launch_avg": b_flops / max(b_launches, 1)},
+        "roofline_single_stream": {
+            "kernel": "lu_update128_kernel<float> (the same body, one matrix per launch)",
+            "measured_on": "single-stream leg (one frequency in flight)",
+            "achieved": round(achieved, 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK["fp32"], 4), "traffic": round(LU_UPDATE_TRAFFIC_RATIO * upd_bytes),
+            "algorithmic_bytes_per_launch": round(upd_bytes), "launches": upd_launches,
+            "avg_launch_us": round(1e3 * upd_ms / max(upd_launches, 1), 3)},
         "autotune": tuning,
         "lu_factor_tflops": round(lu_flops / (fac_ms * 1e-3) / 1e12, 3),
         "assembly_gpairs_per_s": round(n * n / (float(np.median(phmx["assembly"])) * 1e-3) / 1e9, 3),

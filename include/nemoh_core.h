@@ -321,6 +321,16 @@ int nemoh_batch_solve_waves(nemoh_batch *batch, double omega, int nbeta, const d
  * keeps its own module NQ, as for nemoh_core_solve_waves on it. */
 int nemoh_batch_solve_waves_multi(nemoh_batch *batch, const double *omega, int nbeta, const double *beta,
                                   double *forces, double *fk);
+/* Timing of a batch's grouped rank-128 LU trailing updates (a measurement
+ * hook, no counterpart in the reference).  on != 0 records events around
+ * every update launch of the calls that follow and resets the sums; then
+ * nemoh_batch_update_time gives, summed over those calls, the launches'
+ * milliseconds, their algorithmic flops (8 x 128 x m^2 per matrix and
+ * launch, m its trailing order) and bytes (2 m^2 + 256 m elements of 8
+ * bytes on the mixed path, of 16 on FP64: the trailing matrix read and
+ * written, the two 128-wide panels read), and the launch count. */
+int nemoh_batch_set_timing(nemoh_batch *batch, int on);
+int nemoh_batch_update_time(const nemoh_batch *batch, double *ms, double *flops, double *bytes, int *launches);
 const char *nemoh_batch_last_error(const nemoh_batch *batch);
 void nemoh_batch_destroy(nemoh_batch *batch);
 

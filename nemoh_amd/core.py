@@ -35,7 +35,8 @@ EXPORTS = (
     "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_core_kochin_noalloc", "nemoh_core_potential", "nemoh_server_create", "nemoh_server_submit",
     "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_memory", "nemoh_server_last_error",
     "nemoh_server_destroy", "nemoh_batch_create", "nemoh_batch_solve_waves", "nemoh_batch_last_error",
-    "nemoh_batch_destroy", "nemoh_core_autotune", "nemoh_batch_solve_waves_multi",
+    "nemoh_batch_destroy", "nemoh_core_autotune", "nemoh_batch_solve_waves_multi", "nemoh_batch_set_timing",
+    "nemoh_batch_update_time",
 )
 
 
@@ -143,6 +144,8 @@ def load():
     L.nemoh_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p), C.c_int, C.c_int]
     L.nemoh_batch_solve_waves.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp]
     L.nemoh_batch_solve_waves_multi.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp]
+    L.nemoh_batch_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.nemoh_batch_update_time.argtypes = [C.c_void_p, _dp, _dp, _dp, C.POINTER(C.c_int)]
     L.nemoh_batch_last_error.argtypes = [C.c_void_p]
     L.nemoh_batch_last_error.restype = C.c_char_p
     L.nemoh_core_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(TuneEntry), C.POINTER(C.c_int)]
@@ -455,6 +458,18 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+    def set_timing(self, on: bool):
+        """Events around the grouped rank-128 update launches of the calls
+        that follow (sums reset)."""
+        load().nemoh_batch_set_timing(self.h, int(bool(on)))
+
+    def update_time(self):
+        """(milliseconds, algorithmic flops, algorithmic bytes, launches) of
+        the timed update launches since set_timing(True)."""
+        ms, fl, by, n = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+        load().nemoh_batch_update_time(self.h, C.byref(ms), C.byref(fl), C.byref(by), C.byref(n))
+        return ms.value, fl.value, by.value, n.value
 
     def solve_waves(self, omega, betas):
         """Per body: forces (nforce, nbeta + nrad) and Froude-Krylov forces

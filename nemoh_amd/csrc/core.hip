@@ -1740,10 +1740,30 @@ struct nemoh_batch {
   double* d_omega = nullptr;
   std::vector<AsmJob> asm_host, asm_part;
   AsmJob* d_asm_part = nullptr;
+  // timing of the grouped rank-128 updates (nemoh_batch_set_timing): two
+  // events per update launch of a call, the durations summed over calls
+  bool timing = false;
+  std::vector<hipEvent_t> upd_ev;
+  double upd_ms = 0.0, upd_flops = 0.0, upd_bytes = 0.0;
+  int upd_launches = 0;
   std::string err;
 };
 
 namespace {
+
+// The rank-128 update launches of a grouped factorisation: their count, and
+// their algorithmic flops and bytes added to flops and bytes.
+template <typename T>
+size_t update_work(const LuGroup<T>& g, double& flops, double& bytes) {
+  size_t n = 0;
+  for (const auto& L : g.launches)
+    if (L.kind == 3) {
+      ++n;
+      flops += L.flops;
+      bytes += L.bytes;
+    }
+  return n;
+}
 
 int batch_fail(nemoh_batch* b, int code, const std::string& msg) {
   b->err = msg;
@@ -2031,8 +2051,18 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
       launch_assemble_surface(c->G, c->T, c->d_par, fdv[i], c->nsym, mat_out(c), c->d_surf, c->nsurf, b->st);
   }
   // factorisation of every matrix, one launch per step
-  if (mixed) launch_lu_group(b->lu32, b->st);
-  else launch_lu_group(b->lu64, b->st);
+  const hipEvent_t* ev = nullptr;
+  double ev_flops = 0.0, ev_bytes = 0.0;
+  if (b->timing) {
+    const size_t need = 2 * (mixed ? update_work(b->lu32, ev_flops, ev_bytes) : update_work(b->lu64, ev_flops, ev_bytes));
+    while (b->upd_ev.size() < need) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "event creation failed");
+      b->upd_ev.push_back(e);
+    }
+    ev = b->upd_ev.data();
+  }
+  const int nupd = mixed ? launch_lu_group(b->lu32, b->st, ev) : launch_lu_group(b->lu64, b->st, ev);
   const int mode = mixed ? NEMOH_MIXED : NEMOH_FP64;
   for (size_t i = 0; i < nbody; ++i) {  // the factors on the device belong to this frequency
     b->bodies[i]->w_prev = wv[i];
@@ -2069,6 +2099,17 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
   (void)hipMemcpyAsync(status.data(), b->d_status, nbody * sizeof(BatchStatus), hipMemcpyDeviceToHost, b->st);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->st) != hipSuccess)
     return batch_fail(b, NEMOH_ERR_HIP, "batch launch failed");
+  if (ev) {  // the update launches are complete
+    for (int u = 0; u < nupd; ++u) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev[2 * u], ev[2 * u + 1]) != hipSuccess)
+        return batch_fail(b, NEMOH_ERR_HIP, "update timing failed");
+      b->upd_ms += ms;
+    }
+    b->upd_flops += ev_flops;
+    b->upd_bytes += ev_bytes;
+    b->upd_launches += nupd;
+  }
   // sigma, phi and pressure of every body; a body whose wait gave up, or
   // whose refinement needs more sweeps, is solved again on its own handle
   // from its factors (a cache hit), which overwrites its share
@@ -2131,6 +2172,23 @@ int nemoh_batch_solve_waves_multi(nemoh_batch* b, const double* omega, int nbeta
   return batch_solve(b, omega, nbeta, beta, forces, fk);
 }
 
+int nemoh_batch_set_timing(nemoh_batch* b, int on) {
+  if (!b) return NEMOH_ERR_ARG;
+  b->timing = on != 0;
+  b->upd_ms = b->upd_flops = b->upd_bytes = 0.0;
+  b->upd_launches = 0;
+  return NEMOH_OK;
+}
+
+int nemoh_batch_update_time(const nemoh_batch* b, double* ms, double* flops, double* bytes, int* launches) {
+  if (!b) return NEMOH_ERR_ARG;
+  if (ms) *ms = b->upd_ms;
+  if (flops) *flops = b->upd_flops;
+  if (bytes) *bytes = b->upd_bytes;
+  if (launches) *launches = b->upd_launches;
+  return NEMOH_OK;
+}
+
 const char* nemoh_batch_last_error(const nemoh_batch* b) {
   if (b) return b->err.c_str();
   return create_error.empty() ? "null batch" : create_error.c_str();
@@ -2141,6 +2199,7 @@ void nemoh_batch_destroy(nemoh_batch* b) {
   std::lock_guard<std::recursive_mutex> lk(device_lock());
   if (b->st) (void)hipStreamSynchronize(b->st);
   batch_free_tables(b);
+  for (hipEvent_t e : b->upd_ev) (void)hipEventDestroy(e);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
 }

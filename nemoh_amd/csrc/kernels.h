@@ -211,6 +211,8 @@ struct LuGroup {
     int first;  // its jobs: d_jobs[first, first + count)
     int count;
     int grid;
+    double flops;  // rank-128 updates: 8 x 128 x m^2 summed over the jobs (m: trailing order)
+    double bytes;  // and (2 m^2 + 256 m) elements (C read and written, the two 128-wide panels read)
   };
   std::vector<Launch> launches;
   LuJob<T>* d_jobs = nullptr;  // device memory (hipFree by the owner)
@@ -218,8 +220,10 @@ struct LuGroup {
 // mats: A, P, info, ld, n of each matrix (the other fields are set here).
 template <typename T>
 hipError_t build_lu_group(const std::vector<LuJob<T>>& mats, LuGroup<T>* g);
+// ev (optional): two events per rank-128 update launch, recorded around it
+// in order; returns the number of update launches.
 template <typename T>
-void launch_lu_group(const LuGroup<T>& g, hipStream_t st);
+int launch_lu_group(const LuGroup<T>& g, hipStream_t st, const hipEvent_t* ev = nullptr);
 // Grouped chained solves (one launch per triangle for every system): the
 // systems' TrsvJob fields except wg0 / wg1, which are set here.
 template <typename T>

@@ -1596,7 +1596,14 @@ void add_launch(LuGroup<T>* g, std::vector<LuJob<T>>& all, int kind, std::vector
     j.wg1 = wg + j.wg1;  // (wg1 holds the job's workgroup count until here)
     wg = (j.wg1 + 7) & ~7;
   }
-  g->launches.push_back({kind, (int)all.size(), (int)jobs.size(), jobs.back().wg1});
+  double flops = 0.0, bytes = 0.0;
+  if (kind == 3)
+    for (const auto& j : jobs) {
+      const double m = (double)(j.n - (j.k + 2) * kNB);
+      flops += 8.0 * 128.0 * m * m;
+      bytes += (double)sizeof(C2<T>) * (2.0 * m * m + 256.0 * m);
+    }
+  g->launches.push_back({kind, (int)all.size(), (int)jobs.size(), jobs.back().wg1, flops, bytes});
   all.insert(all.end(), jobs.begin(), jobs.end());
   jobs.clear();
 }
@@ -1658,16 +1665,23 @@ hipError_t build_lu_group(const std::vector<LuJob<T>>& mats, LuGroup<T>* g) {
 }
 
 template <typename T>
-void launch_lu_group(const LuGroup<T>& g, hipStream_t st) {
+int launch_lu_group(const LuGroup<T>& g, hipStream_t st, const hipEvent_t* ev) {
+  int updates = 0;
   for (const auto& L : g.launches) {
     const LuJob<T>* j = g.d_jobs + L.first;
     switch (L.kind) {
       case 0: group_inverse_kernel<T><<<L.grid, 256, 0, st>>>(j, L.count); break;
       case 1: group_panel2_kernel<T><<<L.grid, 256, 0, st>>>(j, L.count); break;
       case 2: group_cross_kernel<T><<<L.grid, 256, 0, st>>>(j, L.count); break;
-      default: group_update128_kernel<T><<<L.grid, 256, 0, st>>>(j, L.count); break;
+      default:
+        if (ev) (void)hipEventRecord(ev[2 * updates], st);
+        group_update128_kernel<T><<<L.grid, 256, 0, st>>>(j, L.count);
+        if (ev) (void)hipEventRecord(ev[2 * updates + 1], st);
+        ++updates;
+        break;
     }
   }
+  return updates;
 }
 
 template <typename T>
@@ -1734,8 +1748,8 @@ void launch_trsv_group(const TrsvGroup<T>& g, int spin_limit, hipStream_t st) {
 
 template hipError_t build_lu_group<float>(const std::vector<LuJob<float>>&, LuGroup<float>*);
 template hipError_t build_lu_group<double>(const std::vector<LuJob<double>>&, LuGroup<double>*);
-template void launch_lu_group<float>(const LuGroup<float>&, hipStream_t);
-template void launch_lu_group<double>(const LuGroup<double>&, hipStream_t);
+template int launch_lu_group<float>(const LuGroup<float>&, hipStream_t, const hipEvent_t*);
+template int launch_lu_group<double>(const LuGroup<double>&, hipStream_t, const hipEvent_t*);
 template hipError_t build_trsv_group<float>(const std::vector<TrsvJob<float>>&, TrsvGroup<float>*);
 template hipError_t build_trsv_group<double>(const std::vector<TrsvJob<double>>&, TrsvGroup<double>*);
 template void launch_trsv_group<float>(const TrsvGroup<float>&, int, hipStream_t);

@@ -119,6 +119,37 @@ def test_batch_frequency_per_body_mixed_bodies():
     _check_multi(ms, 0.0, [[0.4, 0.9, 1.6, 2.5], [1.2, 1.2, 0.3, 0.7]], np.zeros(1), core.MIXED)
 
 
+@pytest.mark.parametrize("prec", [core.MIXED, core.FP64])
+def test_batch_update_timing(prec):
+    """Events around the grouped rank-128 updates: one per block pair of the
+    largest body per call, algorithmic flops and bytes summed over the
+    bodies' launches, forces unchanged by the timing."""
+    ms = [nmesh.box(30.0, 10.0, 5.0, 15, 5, 3), nmesh.box(20.0, 8.0, 4.0, 20, 8, 4)]
+    tables = core.build_tables()
+    cores = [_handle(m, 30.0, tables, prec) for m in ms]
+    b = core.Batch(cores, prec, max_rhs=7)
+    ref = b.solve_waves(0.7, np.zeros(1))
+    b.set_timing(True)
+    for _ in range(2):
+        got = b.solve_waves(0.7, np.zeros(1))
+    t_ms, flops, nbytes, launches = b.update_time()
+    b.close()
+    for c in cores:
+        c.close()
+    for (f, fk), (fr, fkr) in zip(got, ref):
+        assert np.array_equal(f, fr) and np.array_equal(fk, fkr)
+    nblk = [(m.npanels + 63) // 64 for m in ms]
+    assert launches == 2 * len(range(0, max(nblk) - 2, 2)), (launches, nblk)
+    want_f = want_b = 0.0
+    for n in (m.npanels for m in ms):
+        for k in range(0, (n + 63) // 64 - 2, 2):
+            t = n - 64 * (k + 2)
+            want_f += 8.0 * 128 * t * t
+            want_b += (8 if prec == core.MIXED else 16) * (2.0 * t * t + 256.0 * t)
+    assert flops == pytest.approx(2 * want_f) and nbytes == pytest.approx(2 * want_b)
+    assert t_ms > 0.0
+
+
 def test_batch_chained_solve_fallback():
     """Spin limit 0: every chained wait gives up at once, and each body is
     solved again on its own handle (stepwise), as a handle alone does."""
