@@ -230,13 +230,6 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
         xi[a] = S.ai[buf][wr + a * 16 + li][k + lk];
         yr[a] = S.br[buf][k + lk][wc + a * 16 + li];
         yi[a] = S.bi[buf][k + lk][wc + a * 16 + li];
-        // the sign goes on the operand read from LDS, not on the staged
-        // global values: a negation there would make the wave wait for the
-        // next chunk's loads before this chunk's MFMAs
-        if (kNegA) {
-          xr[a] = -xr[a];
-          xi[a] = -xi[a];
-        }
       }
       if (kGauss) {  // acc.re accumulates K3, acc.im K2
         T sx[2], dy[2], ey[2];
@@ -276,15 +269,35 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
       __syncthreads();
     }
   }
-  if (kGauss) {  // acc = K1 - K3, K1 + K2 (acc entered as zero)
+  // acc = AB (Gauss: K1 - K3, K1 + K2; acc entered as zero), then -AB for
+  // kNegA: the sign is applied here, once per element, not to the operands
+  // at every k-step.  Round-to-nearest is symmetric, so every product and
+  // partial sum of (-A)B is the negation of AB's and the result is bit for
+  // bit that of negating the operands.
+  if (kGauss) {
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          acc.re[a][b][q] = k1[a][b][q] - acc.re[a][b][q];
-          acc.im[a][b][q] = k1[a][b][q] + acc.im[a][b][q];
+          if (kNegA) {
+            acc.re[a][b][q] = acc.re[a][b][q] - k1[a][b][q];
+            acc.im[a][b][q] = -k1[a][b][q] - acc.im[a][b][q];
+          } else {
+            acc.re[a][b][q] = k1[a][b][q] - acc.re[a][b][q];
+            acc.im[a][b][q] = k1[a][b][q] + acc.im[a][b][q];
+          }
+        }
+  } else if (kNegA) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          acc.re[a][b][q] = -acc.re[a][b][q];
+          acc.im[a][b][q] = -acc.im[a][b][q];
         }
   }
   if (C) {
