@@ -1765,6 +1765,10 @@ size_t update_work(const LuGroup<T>& g, double& flops, double& bytes) {
   return n;
 }
 
+// Workgroups of a body's paired assembly from which it takes a launch of its
+// own in a batch (about five rounds of the GPU at three workgroups per CU).
+constexpr int kOwnAssemblyWg = 4096;
+
 int batch_fail(nemoh_batch* b, int code, const std::string& msg) {
   b->err = msg;
   return code;
@@ -2023,27 +2027,42 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
   // assembly of every body, one launch per depth regime present (free-surface
   // panels after it, per body)
   launch_group_info_reset(b->d_body[0], nbj, b->st);
+  // the assembly: a body whose pairs fill the GPU on their own takes a launch
+  // of its own (the single-body kernel keeps its job in kernel arguments; the
+  // grouped one reads it from memory and spills 4 registers: 4 % of a
+  // 40-body batch of 4000 panels); the others one grouped launch per depth
+  // regime present.  Both run the same pair arithmetic.
   const int nfd = (int)std::count(fdv.begin(), fdv.end(), 1);
-  if (nfd == 0 || nfd == nbj) {
+  bool any_big = false;
+  for (size_t i = 0; i < nbody; ++i) any_big = any_big || b->asm_host[i].wg1 - b->asm_host[i].wg0 >= kOwnAssemblyWg;
+  if (!any_big && (nfd == 0 || nfd == nbj)) {
     launch_assemble_group(b->d_asm, b->asm_count, b->asm_grid, nfd ? 1 : 0, b->nsym, mixed, b->st);
   } else {
-    // the jobs of each regime, packed with new workgroup ranges: finite
-    // depth first, then infinite depth
+    // the grouped bodies of each regime, packed with new workgroup ranges:
+    // finite depth first, then infinite depth
     b->asm_part.clear();
-    int wg[2] = {0, 0};
+    int wg[2] = {0, 0}, cnt[2] = {0, 0};
     for (int reg = 1; reg >= 0; --reg)
-      for (size_t i = 0; i < nbody; ++i)
-        if (fdv[i] == reg) {
-          AsmJob a = b->asm_host[i];
-          const int nwg = a.wg1 - a.wg0;
-          a.wg0 = wg[1 - reg];
-          a.wg1 = a.wg0 + nwg;
-          wg[1 - reg] = a.wg1;
-          b->asm_part.push_back(a);
-        }
-    (void)hipMemcpyAsync(b->d_asm_part, b->asm_part.data(), nbody * sizeof(AsmJob), hipMemcpyHostToDevice, b->st);
-    launch_assemble_group(b->d_asm_part, nfd, wg[0], 1, b->nsym, mixed, b->st);
-    launch_assemble_group(b->d_asm_part + nfd, nbj - nfd, wg[1], 0, b->nsym, mixed, b->st);
+      for (size_t i = 0; i < nbody; ++i) {
+        AsmJob a = b->asm_host[i];
+        const int nwg = a.wg1 - a.wg0;
+        if (fdv[i] != reg || nwg >= kOwnAssemblyWg) continue;
+        a.wg0 = wg[1 - reg];
+        a.wg1 = a.wg0 + nwg;
+        wg[1 - reg] = a.wg1;
+        ++cnt[1 - reg];
+        b->asm_part.push_back(a);
+      }
+    if (!b->asm_part.empty())
+      (void)hipMemcpyAsync(b->d_asm_part, b->asm_part.data(), b->asm_part.size() * sizeof(AsmJob),
+                           hipMemcpyHostToDevice, b->st);
+    launch_assemble_group(b->d_asm_part, cnt[0], wg[0], 1, b->nsym, mixed, b->st);
+    launch_assemble_group(b->d_asm_part + cnt[0], cnt[1], wg[1], 0, b->nsym, mixed, b->st);
+    for (size_t i = 0; i < nbody; ++i) {
+      nemoh_core* c = b->bodies[i];
+      if (b->asm_host[i].wg1 - b->asm_host[i].wg0 >= kOwnAssemblyWg)
+        launch_assemble(c->G, c->T, c->d_par, fdv[i], c->nsym, mat_out(c), nullptr, 0, true, b->st);
+    }
   }
   for (size_t i = 0; i < nbody; ++i) {
     nemoh_core* c = b->bodies[i];

@@ -10,7 +10,9 @@ Cases: bodies of the batched workload's generator with uneven panel counts
 pairs) in infinite depth; boxes in finite depth (the finite-depth assembly
 and the exponential images); half meshes with y-symmetry (two symmetry
 blocks per body); both precisions; a frequency per body (one mesh at many
-frequencies, in both depth regimes at once); and the batch's fallback for a chained
+frequencies, in both depth regimes at once); a body large enough to take
+the single-body assembly kernel beside grouped ones; update timing; and the
+batch's fallback for a chained
 solve that gives up waiting (spin limit 0), which redoes a body on its own
 handle.
 """
@@ -148,6 +150,18 @@ def test_batch_update_timing(prec):
             want_b += (8 if prec == core.MIXED else 16) * (2.0 * t * t + 256.0 * t)
     assert flops == pytest.approx(2 * want_f) and nbytes == pytest.approx(2 * want_b)
     assert t_ms > 0.0
+
+
+@pytest.mark.parametrize("prec", [core.MIXED, core.FP64])
+def test_batch_large_body_own_assembly(prec):
+    """A body whose assembly fills the GPU on its own (1664 panels) takes the
+    single-body kernel inside the batch, the small one the grouped launch:
+    both equal the lone handles, at one frequency and at a frequency each in
+    both depth regimes (3.0 rad/s is infinite depth at 30 m)."""
+    big, small = nmesh.box(40.0, 20.0, 10.0, 32, 16, 12), nmesh.box(12.0, 12.0, 6.0, 6, 6, 3)
+    assert big.npanels == 1664
+    _check([big, small], 30.0, [0.8], np.deg2rad([0.0, 30.0]), prec)
+    _check_multi([big, small, big], 30.0, [[0.8, 3.0, 3.0], [3.0, 0.8, 1.1]], np.zeros(1), prec)
 
 
 def test_batch_chained_solve_fallback():
