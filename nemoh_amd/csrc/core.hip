@@ -1738,6 +1738,7 @@ struct nemoh_batch {
   // bodies' frequencies fall in both depth regimes, the assembly jobs of
   // each regime (host copies of the jobs kept for that)
   double* d_omega = nullptr;
+  std::vector<double> omega_host;  // the source of d_omega's copy (kept until the next call)
   std::vector<AsmJob> asm_host, asm_part;
   AsmJob* d_asm_part = nullptr;
   // timing of the grouped rank-128 updates (nemoh_batch_set_timing): two
@@ -1768,6 +1769,14 @@ size_t update_work(const LuGroup<T>& g, double& flops, double& bytes) {
 // Workgroups of a body's paired assembly from which it takes a launch of its
 // own in a batch (about five rounds of the GPU at three workgroups per CU).
 constexpr int kOwnAssemblyWg = 4096;
+
+// Waits for the batch's stream whatever the launch status (copies into host
+// memory may be pending), then reports whether both were clean.
+bool batch_sync(nemoh_batch* b) {
+  const hipError_t launched = hipGetLastError();
+  const hipError_t synced = hipStreamSynchronize(b->st);
+  return launched == hipSuccess && synced == hipSuccess;
+}
 
 int batch_fail(nemoh_batch* b, int code, const std::string& msg) {
   b->err = msg;
@@ -2023,7 +2032,8 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
   const bool mixed = b->prec == NEMOH_MIXED;
   const int nbj = (int)nbody;
   const int* G = b->body_grid;
-  (void)hipMemcpyAsync(b->d_omega, wv.data(), nbody * sizeof(double), hipMemcpyHostToDevice, b->st);
+  b->omega_host = wv;
+  (void)hipMemcpyAsync(b->d_omega, b->omega_host.data(), nbody * sizeof(double), hipMemcpyHostToDevice, b->st);
   // assembly of every body, one launch per depth regime present (free-surface
   // panels after it, per body)
   launch_group_info_reset(b->d_body[0], nbj, b->st);
@@ -2116,8 +2126,7 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
   launch_group_status(b->d_body[0], nbj, b->nb, b->d_status, b->st);
   std::vector<BatchStatus> status(nbody);
   (void)hipMemcpyAsync(status.data(), b->d_status, nbody * sizeof(BatchStatus), hipMemcpyDeviceToHost, b->st);
-  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->st) != hipSuccess)
-    return batch_fail(b, NEMOH_ERR_HIP, "batch launch failed");
+  if (!batch_sync(b)) return batch_fail(b, NEMOH_ERR_HIP, "batch launch failed");
   if (ev) {  // the update launches are complete
     for (int u = 0; u < nupd; ++u) {
       float ms = 0.0f;
@@ -2157,8 +2166,7 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
   if (fk && nbeta > 0) launch_group_force(b->d_body[0], nbj, G[3], 1, b->st);
   std::vector<double> hf(2 * std::max<size_t>(1, b->force_off.back()));
   (void)hipMemcpyAsync(hf.data(), b->d_force, b->force_off.back() * sizeof(double2), hipMemcpyDeviceToHost, b->st);
-  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->st) != hipSuccess)
-    return batch_fail(b, NEMOH_ERR_HIP, "batch force download failed");
+  if (!batch_sync(b)) return batch_fail(b, NEMOH_ERR_HIP, "batch force download failed");
   // device (nforce x cols) row-major -> the caller's column-major, per body
   size_t fo = 0, ko = 0;
   for (size_t i = 0; i < nbody; ++i) {
