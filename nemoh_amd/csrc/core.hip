@@ -2084,10 +2084,13 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
   double ev_flops = 0.0, ev_bytes = 0.0;
   if (b->timing) {
     const size_t need = 2 * (mixed ? update_work(b->lu32, ev_flops, ev_bytes) : update_work(b->lu64, ev_flops, ev_bytes));
-    while (b->upd_ev.size() < need) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "event creation failed");
-      b->upd_ev.push_back(e);
+    if (b->upd_ev.size() < need) {  // created under the device lock, as the handles' events are
+      std::lock_guard<std::recursive_mutex> lk(device_lock());
+      while (b->upd_ev.size() < need) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "event creation failed");
+        b->upd_ev.push_back(e);
+      }
     }
     ev = b->upd_ev.data();
   }
