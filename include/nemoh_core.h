@@ -401,6 +401,13 @@ int nemoh_server_wait(nemoh_server *srv, long long id, double *forces, double *f
  * charged against the budget while other frequencies were running (both
  * may be NULL). */
 int nemoh_server_memory(nemoh_server *srv, double *held, double *peak);
+/* Most frequencies of one wave request a worker solves at once (1 to 64;
+ * 1, the default, takes one frequency at a time).  A worker then takes as
+ * many of its request's pending frequencies as the budget allows at the
+ * request's footprint each, binds that many handles of its own to the mesh
+ * and solves them as one batch (nemoh_batch_solve_waves_multi); results are
+ * those of one frequency at a time, bit for bit. */
+int nemoh_server_set_batch(nemoh_server *srv, int frequencies);
 const char *nemoh_server_last_error(nemoh_server *srv);
 void nemoh_server_destroy(nemoh_server *srv);
 

@@ -33,7 +33,8 @@ EXPORTS = (
     "nemoh_core_wavenumber", "nemoh_core_theta_rule", "nemoh_core_prony", "nemoh_core_pairs",
     "nemoh_core_mfma_probe", "nemoh_core_get_extent", "nemoh_core_set_dofs", "nemoh_core_body_conditions",
     "nemoh_core_solve_waves", "nemoh_core_incident_wavenumber", "nemoh_core_kochin", "nemoh_core_kochin_noalloc", "nemoh_core_potential", "nemoh_server_create", "nemoh_server_submit",
-    "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_memory", "nemoh_server_last_error",
+    "nemoh_server_submit_waves", "nemoh_server_wait", "nemoh_server_memory", "nemoh_server_set_batch",
+    "nemoh_server_last_error",
     "nemoh_server_destroy", "nemoh_batch_create", "nemoh_batch_solve_waves", "nemoh_batch_last_error",
     "nemoh_batch_destroy", "nemoh_core_autotune", "nemoh_batch_solve_waves_multi", "nemoh_batch_set_timing",
     "nemoh_batch_update_time",
@@ -130,6 +131,7 @@ def load():
     L.nemoh_core_incident_wavenumber.argtypes = [C.c_double, C.c_double, C.c_double]
     L.nemoh_core_incident_wavenumber.restype = C.c_double
     L.nemoh_server_memory.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.nemoh_server_set_batch.argtypes = [C.c_void_p, C.c_int]
     L.nemoh_server_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_double] + [_dp] * 6
     L.nemoh_server_submit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [_dp] * 3 + [_ip] * 4 + \
         [C.c_double] * 3 + [C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.POINTER(C.c_longlong)]

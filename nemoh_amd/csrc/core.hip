@@ -1718,6 +1718,7 @@ struct nemoh_batch {
   // job tables, built for the bodies' current buffers (layout_gen of each
   // body when built) and for the right-hand-side count they were built for
   std::vector<unsigned> gens;
+  std::vector<int> shapes;  // per body when built: nfa, nb, nforce, nrad
   int built_rhs = -1;
   AsmJob* d_asm = nullptr;
   int asm_count = 0, asm_grid = 0;
@@ -1802,6 +1803,7 @@ void batch_free_tables(nemoh_batch* b) {
     free_trsv_group(&b->tr64[k]);
   }
   b->gens.clear();
+  b->shapes.clear();
   b->built_rhs = -1;
 }
 
@@ -1919,15 +1921,26 @@ int batch_build(nemoh_batch* b, const std::vector<int>& nrhs) {
   for (int k = 0; k < b->nb && e == hipSuccess; ++k)
     e = b->prec == NEMOH_MIXED ? build_trsv_group(t32[k], &b->tr32[k]) : build_trsv_group(t64[k], &b->tr64[k]);
   if (e != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, std::string("batch tables: ") + hipGetErrorString(e));
-  for (nemoh_core* c : b->bodies) b->gens.push_back(c->layout_gen);
+  for (nemoh_core* c : b->bodies) {
+    b->gens.push_back(c->layout_gen);
+    b->shapes.insert(b->shapes.end(), {c->nfa, c->nb, c->nforce, c->nrad});
+  }
   b->built_rhs = nrhs.empty() ? 0 : nrhs[0];
   return NEMOH_OK;
 }
 
+// The job tables no longer match the bodies: a buffer moved, or the problem
+// count, the mesh size or the degrees of freedom changed.
 bool batch_stale(const nemoh_batch* b, const std::vector<int>& nrhs) {
-  if (b->gens.size() != b->bodies.size() || nrhs.empty() || b->built_rhs != nrhs[0]) return true;
-  for (size_t i = 0; i < b->bodies.size(); ++i)
-    if (b->gens[i] != b->bodies[i]->layout_gen) return true;
+  if (b->gens.size() != b->bodies.size() || b->shapes.size() != 4 * b->bodies.size() || nrhs.empty() ||
+      b->built_rhs != nrhs[0])
+    return true;
+  for (size_t i = 0; i < b->bodies.size(); ++i) {
+    const nemoh_core* c = b->bodies[i];
+    const int* sh = b->shapes.data() + 4 * i;
+    if (b->gens[i] != c->layout_gen || sh[0] != c->nfa || sh[1] != c->nb || sh[2] != c->nforce || sh[3] != c->nrad)
+      return true;
+  }
   return false;
 }
 
@@ -2005,6 +2018,12 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
   for (size_t i = 0; i < nbody; ++i) {
     nemoh_core* c = b->bodies[i];
     if (c->prec != b->prec) return batch_fail(b, NEMOH_ERR_ARG, "a body's precision changed outside the batch");
+    if (c->fp32_assembly)
+      return batch_fail(b, NEMOH_ERR_ARG, "body " + std::to_string(i) + ": FP32 assembly is not batched");
+    // the matrices of the body's current mesh: a handle may have been bound
+    // to another mesh, or had its matrices freed, since the last call (its
+    // buffers then move, and the job tables are rebuilt below)
+    if (int r = ensure_matrices(c)) return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
     if (lu_form(c).rank != 128)
       return batch_fail(b, NEMOH_ERR_ARG, "body " + std::to_string(i) + ": batches factor in the rank-128 form");
     nrhs[i] = nbeta + c->nrad;

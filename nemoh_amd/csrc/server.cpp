@@ -23,6 +23,14 @@
 // of requests that arrive while others run, overlap on the GPU as soon as a
 // worker and budget are free.
 //
+// A worker may take several frequencies of a wave request at once (up to the
+// server's batch size, as many as the budget allows at the request's
+// footprint each): it then binds as many handles of its own to the mesh and
+// solves them as one batch (nemoh_batch_solve_waves_multi), every heavy
+// step one launch over all of them, which fills the GPU where one
+// frequency of a small mesh cannot.  A batch gives each frequency the
+// arithmetic of a lone handle, so results do not change either.
+//
 // A frequency's result does not depend on the schedule: every kernel is
 // deterministic, handles share nothing but the read-only wave tables, and
 // the module NQ that the reference carries from one frequency to the next
@@ -99,18 +107,40 @@ struct Request {
   }
 };
 
-struct Worker {
+// One solver handle of a worker.
+struct Slot {
   nemoh_core* h = nullptr;
   long long bound = -1;     // request the handle's mesh belongs to
   long long dofs_for = -1;  // request whose degrees of freedom the handle holds
+};
+
+struct Worker {
+  Slot main;               // the handle of single frequencies (and the first of a batch)
+  std::vector<Slot> more;  // the batch's further handles
+  nemoh_batch* batch = nullptr;  // over the handles it was made for
+  std::vector<nemoh_core*> batch_handles;
+  int batch_prec = -1;
   // guarded by the server mutex:
   bool busy = false;
-  double held = 0;     // device bytes the handle holds (updated after each item)
+  double held = 0;     // device bytes the handles hold (updated after each item)
   double charged = 0;  // bytes counted against the budget (held, or more while busy)
-  bool release = false;  // free the handle's matrices before the next item
+  bool release = false;  // free the handles' matrices before the next item
   long long claim = -1;  // request of this worker's current or last item
   size_t pick = 0;       // queue position of the request admitted for its next item
+  int take = 1;          // frequencies admitted for the next item
 };
+
+double held_bytes(const Worker& w) {
+  double b = w.main.h ? (double)nemoh::core_held_bytes(w.main.h) : 0.0;
+  for (const Slot& sl : w.more) b += sl.h ? (double)nemoh::core_held_bytes(sl.h) : 0.0;
+  return b;
+}
+
+void release_all(Worker& w) {
+  if (w.main.h) (void)nemoh::core_release_matrices(w.main.h);
+  for (Slot& sl : w.more)
+    if (sl.h) (void)nemoh::core_release_matrices(sl.h);
+}
 
 }  // namespace
 
@@ -122,6 +152,7 @@ struct nemoh_server {
   std::unordered_map<long long, std::shared_ptr<Request>> table;
   std::vector<Worker> workers;
   double budget = 0;
+  int batch = 1;    // most frequencies of a wave request one worker solves at once
   double peak = 0;  // largest total charge admitted with other work running
   int running = 0;
   bool stop = false;
@@ -136,8 +167,8 @@ void geo_ptrs(const Request& r, const double* out[8]) {
   for (int k = 0; k < 8; ++k) out[k] = r.geo.data() + (size_t)k * r.nfa;
 }
 
-// Points the worker's handle at r's mesh (creating the handle on first use).
-int bind(nemoh_server* s, Worker& w, Request& r) {
+// Points a handle at r's mesh (creating the handle on first use).
+int bind(nemoh_server* s, Slot& w, Request& r) {
   if (w.h && w.bound == r.id) return NEMOH_OK;
   const double* geo[8];
   geo_ptrs(r, geo);
@@ -179,11 +210,11 @@ bool prepare(nemoh_server* s, Worker& w, Request& r) {
                                     r.m3.data(), r.m4.data(), g, g + n, g + 2 * n, g + 3 * n, g + 4 * n, g + 5 * n,
                                     g + 6 * n, g + 7 * n, &r.zer);
     if (e) r.set_error(e, "degenerate panel in the request mesh");
-    if (!e) e = bind(s, w, r);
+    if (!e) e = bind(s, w.main, r);
     if (!e) {
       r.nq_before.assign(r.nfreq, 0);
-      e = nemoh::core_nq_sequence(w.h, r.nfreq, r.periods.data(), r.nq_before.data());
-      if (e) r.set_error(e, nemoh_core_last_error(w.h));
+      e = nemoh::core_nq_sequence(w.main.h, r.nfreq, r.periods.data(), r.nq_before.data());
+      if (e) r.set_error(e, nemoh_core_last_error(w.main.h));
     }
     lk.lock();
     r.prep = e ? -1 : 2;
@@ -194,8 +225,9 @@ bool prepare(nemoh_server* s, Worker& w, Request& r) {
   return r.prep == 2;
 }
 
-void run_item(nemoh_server* s, Worker& w, Request& r, int f) {
-  if (!prepare(s, w, r) || r.failed()) return;
+void run_item(nemoh_server* s, Worker& ww, Request& r, int f) {
+  if (!prepare(s, ww, r) || r.failed()) return;
+  Slot& w = ww.main;
   if (bind(s, w, r)) return;
   const size_t ntot = (size_t)r.nfa << r.nsym;
   const size_t per_f = 2 * ntot * r.nrhs, per_force = 2 * (size_t)r.nint * r.nrhs;
@@ -225,6 +257,54 @@ void run_item(nemoh_server* s, Worker& w, Request& r, int f) {
   }
 }
 
+// Frequencies f0 .. f0 + k - 1 of wave request r as one batch of the
+// worker's handles, one per frequency, each with the module NQ that
+// frequency has in the request's sequence.
+void run_batch(nemoh_server* s, Worker& w, Request& r, int f0, int k) {
+  if (!prepare(s, w, r) || r.failed()) return;
+  while ((int)w.more.size() < k - 1) w.more.emplace_back();
+  std::vector<nemoh_core*> hs;
+  int e = NEMOH_OK;
+  std::string msg;
+  for (int i = 0; i < k && !e; ++i) {
+    Slot& sl = i == 0 ? w.main : w.more[i - 1];
+    if ((e = bind(s, sl, r))) return;  // bind reported it
+    e = nemoh_core_set_precision(sl.h, r.precision, 0.0, 0);
+    nemoh::core_set_nq_state(sl.h, r.nq_before[f0 + i]);
+    if (!e && sl.dofs_for != r.id) {
+      e = nemoh_core_set_dofs(sl.h, r.body.empty() ? nullptr : r.body.data(), (int)r.rad.size(), r.rad.data(),
+                              (int)r.force.size(), r.force.data(), r.xeff, r.yeff);
+      if (!e) sl.dofs_for = r.id;
+    }
+    if (e) msg = nemoh_core_last_error(sl.h);
+    hs.push_back(sl.h);
+  }
+  if (!e && (!w.batch || w.batch_handles != hs || w.batch_prec != r.precision)) {
+    if (w.batch) nemoh_batch_destroy(w.batch);
+    w.batch = nullptr;
+    w.batch_handles.clear();
+    e = nemoh_batch_create(&w.batch, k, hs.data(), r.precision, 16);
+    if (e) {
+      msg = nemoh_batch_last_error(nullptr);
+      w.batch = nullptr;
+    } else {
+      w.batch_handles = hs;
+      w.batch_prec = r.precision;
+    }
+  }
+  if (!e) {
+    const size_t per_force = 2 * (size_t)r.nint * r.nrhs, per_fk = 2 * (size_t)r.nint * r.nbeta;
+    e = nemoh_batch_solve_waves_multi(w.batch, r.omegas.data() + f0, r.nbeta, r.nbeta > 0 ? r.betas.data() : nullptr,
+                                      r.forces.data() + f0 * per_force, r.nbeta > 0 ? r.fk.data() + f0 * per_fk : nullptr);
+    if (e) msg = nemoh_batch_last_error(w.batch);
+  }
+  if (e) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "frequencies %d-%d: ", f0, f0 + k - 1);
+    r.set_error(e, buf + msg);
+  }
+}
+
 // The queued request a worker takes its next frequency from (server mutex
 // held): the one its last item came from while it has frequencies left, so
 // the handle keeps its mesh, factors-to-be and graphs; else the first that no
@@ -251,36 +331,45 @@ size_t pick_request(const nemoh_server* s, int self) {
 bool try_admit(nemoh_server* s, int self, bool freed = false) {
   Worker& w = s->workers[self];
   w.pick = pick_request(s, self);
-  const double need = s->queue[w.pick]->footprint;
+  const Request& q = *s->queue[w.pick];
   double others = 0, idle_held = 0;
   for (size_t v = 0; v < s->workers.size(); ++v) {
     if ((int)v == self) continue;
     others += s->workers[v].charged;
     if (!s->workers[v].busy) idle_held += s->workers[v].charged;
   }
-  bool shrink = false;
-  bool ok = others + std::max(w.held, need) <= s->budget;
-  if (!ok && others + need <= s->budget) ok = shrink = true;  // after freeing this worker's own buffers
-  if (!ok && !freed && idle_held > 0 && others - idle_held + need <= s->budget) {
+  int running = 0;
+  for (const Worker& o : s->workers) running += o.busy;
+  auto admit = [&](int k, double need, bool shrink) {
+    w.take = k;
+    w.release = shrink && w.held > need;
+    w.charged = w.release ? need : std::max(w.held, need);
+    w.busy = true;
+    if (running > 0) s->peak = std::max(s->peak, others + w.charged);
+    return true;
+  };
+  // the most frequencies of the request that fit, halving from the batch size
+  const int want = q.waves ? std::max(1, std::min(s->batch, q.nfreq - q.next)) : 1;
+  for (int k = want;; k = std::max(1, k / 2)) {
+    const double need = k * q.footprint;
+    if (others + std::max(w.held, need) <= s->budget) return admit(k, need, false);
+    if (others + need <= s->budget) return admit(k, need, true);  // after freeing this worker's own buffers
+    if (k == 1) break;
+  }
+  const double need = q.footprint;
+  if (!freed && idle_held > 0 && others - idle_held + need <= s->budget) {
     // the idle workers' buffers are what stands in the way: free them (an
-    // idle worker never touches its handle without holding this mutex)
+    // idle worker never touches its handles without holding this mutex)
     for (size_t v = 0; v < s->workers.size(); ++v) {
       Worker& o = s->workers[v];
       if ((int)v == self || o.busy || o.held == 0) continue;
-      if (o.h) (void)nemoh::core_release_matrices(o.h);
-      o.held = o.charged = o.h ? (double)nemoh::core_held_bytes(o.h) : 0.0;
+      release_all(o);
+      o.held = o.charged = held_bytes(o);
     }
     return try_admit(s, self, true);
   }
-  int running = 0;
-  for (const Worker& o : s->workers) running += o.busy;
-  if (!ok && running == 0) ok = shrink = true;  // alone on the device: always admitted
-  if (!ok) return false;
-  w.release = shrink && w.held > need;
-  w.charged = w.release ? need : std::max(w.held, need);
-  w.busy = true;
-  if (running > 0) s->peak = std::max(s->peak, others + w.charged);
-  return true;
+  if (running == 0) return admit(1, need, true);  // alone on the device: always admitted
+  return false;
 }
 
 void worker_loop(nemoh_server* s, int self) {
@@ -294,20 +383,23 @@ void worker_loop(nemoh_server* s, int self) {
       s->cv_work.wait(lk, [&] { return (!s->queue.empty() && try_admit(s, self)) || (s->stop && s->queue.empty()); });
       if (!w.busy) break;  // stopped with the queue empty
       r = s->queue[w.pick];
-      f = r->next++;
+      f = r->next;
+      r->next += w.take;
       w.claim = r->id;
       if (r->next == r->nfreq) s->queue.erase(s->queue.begin() + (std::ptrdiff_t)w.pick);
       if (r->t_start < 0) r->t_start = now_ms();
     }
-    if (w.release && w.h) (void)nemoh::core_release_matrices(w.h);
-    run_item(s, w, *r, f);
+    if (w.release) release_all(w);
+    if (w.take == 1) run_item(s, w, *r, f);
+    else run_batch(s, w, *r, f, w.take);
     bool last;
     {
       std::lock_guard<std::mutex> lk(s->mu);
       w.busy = false;
       w.release = false;
-      w.held = w.charged = w.h ? (double)nemoh::core_held_bytes(w.h) : 0.0;
-      last = ++r->finished == r->nfreq;
+      w.held = w.charged = held_bytes(w);
+      r->finished += w.take;
+      last = r->finished == r->nfreq;
       if (last) {
         r->t_done = now_ms();
         r->done = true;
@@ -469,6 +561,13 @@ int nemoh_server_wait(nemoh_server* s, long long id, double* forces, double* fk,
   return NEMOH_OK;
 }
 
+int nemoh_server_set_batch(nemoh_server* s, int frequencies) {
+  if (!s || frequencies < 1 || frequencies > 64) return NEMOH_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->batch = frequencies;
+  return NEMOH_OK;
+}
+
 int nemoh_server_memory(nemoh_server* s, double* held, double* peak) {
   if (!s) return NEMOH_ERR_ARG;
   std::lock_guard<std::mutex> lk(s->mu);
@@ -493,8 +592,12 @@ void nemoh_server_destroy(nemoh_server* s) {
   }
   s->cv_work.notify_all();
   for (auto& t : s->threads) t.join();
-  for (Worker& w : s->workers)
-    if (w.h) nemoh_core_destroy(w.h);
+  for (Worker& w : s->workers) {
+    if (w.batch) nemoh_batch_destroy(w.batch);
+    if (w.main.h) nemoh_core_destroy(w.main.h);
+    for (Slot& sl : w.more)
+      if (sl.h) nemoh_core_destroy(sl.h);
+  }
   delete s;
 }
 

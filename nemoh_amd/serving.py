@@ -74,7 +74,10 @@ def _f64(a):
 
 
 class Server:
-    def __init__(self, workers: int = 4, budget_bytes: float | None = None, tables=None):
+    def __init__(self, workers: int = 4, budget_bytes: float | None = None, tables=None, batch: int = 8):
+        """batch: most frequencies of one wave request a worker solves at
+        once, as one batch of handles (nemoh_server_set_batch; 1 takes one
+        frequency at a time)."""
         L = core.load()
         if tables is None:
             tables = core.build_tables()
@@ -88,6 +91,11 @@ class Server:
         if r:
             raise core.NemohError(r, "nemoh_server_create failed (is a HIP device present?)")
         self.h = h
+        r = L.nemoh_server_set_batch(h, int(batch))
+        if r:
+            L.nemoh_server_destroy(h)
+            self.h = None
+            raise core.NemohError(r, "nemoh_server_set_batch: batch out of range")
         self._shapes = {}
 
     def submit(self, req, precision: int = core.MIXED) -> int:

@@ -215,3 +215,32 @@ def test_budget_bounds_device_memory():
             assert peak <= budget and held <= budget
     for a, c in zip(out[budget], out[64.0 * 2 ** 30]):
         assert np.array_equal(a, c)
+
+
+@pytest.mark.gpu
+def test_batched_items_equal_single_frequencies():
+    """Workers that take several frequencies of a wave request at once and
+    solve them as one batch of handles (nemoh_server_set_batch) give the
+    forces and Froude-Krylov forces of one frequency at a time, bit for bit:
+    batches of up to 4, and of up to 8 under a budget that only holds a few
+    of the mix's frequencies at once (so batches shrink, or run alone)."""
+    rng = np.random.default_rng(31)
+    reqs = [serving.random_body_waves(rng, n_range=(300, 1500), nfreq=5) for _ in range(4)]
+    m, body, modes, omegas, betas = serving.two_cylinder_parts(rng, n_range=(800, 1600), max_freq=6)
+    reqs.append(serving.WaveRequest(m, 0.0, omegas, betas, modes, None, body))
+    biggest = max(r.npanels for r in reqs)
+    tables = _tables()
+    runs = []
+    for batch, budget in ((1, None), (4, None), (8, 3 * 40.0 * biggest ** 2)):
+        with serving.Server(workers=3, tables=tables, batch=batch, budget_bytes=budget) as srv:
+            ids = [srv.submit(r, core.MIXED) for r in reqs]
+            runs.append([srv.wait(i, with_fk=True)[:2] for i in ids])
+    for k, (f1, k1) in enumerate(runs[0]):
+        for other in runs[1:]:
+            f2, k2 = other[k]
+            assert np.all(np.isfinite(f1)) and np.array_equal(f1, f2) and np.array_equal(k1, k2), k
+
+
+def test_server_batch_size_is_checked():
+    """The batch-size setter rejects a null server (no device needed)."""
+    assert core.load().nemoh_server_set_batch(None, 4) == core.ERR_ARG
