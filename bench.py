@@ -220,7 +220,7 @@ def run_concurrent(m, omegas, prec, passes, warmup, workers):
     def request(idx):  # problems generated on the device, as in the single-stream leg
         return serving.WaveRequest(m, 50.0, omegas[idx], BETAS, bcm.six_dof())
 
-    with serving.Server(workers=workers, batch=1) as srv:  # one frequency per handle and stream
+    with serving.Server(workers=workers) as srv:  # one frequency per handle and stream
         # warm-up passes (at least one frequency per worker binds every handle)
         srv.wait(srv.submit(request(np.arange(max(warmup * nw, workers)) % nw), prec))
         idx = np.arange(passes * nw) % nw

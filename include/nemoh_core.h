@@ -406,7 +406,9 @@ int nemoh_server_memory(nemoh_server *srv, double *held, double *peak);
  * many of its request's pending frequencies as the budget allows at the
  * request's footprint each, binds that many handles of its own to the mesh
  * and solves them as one batch (nemoh_batch_solve_waves_multi); results are
- * those of one frequency at a time, bit for bit. */
+ * those of one frequency at a time, bit for bit.  It pays for requests with
+ * many frequencies of one small mesh; when each request brings a new mesh,
+ * re-binding and re-tabling the handles costs more than the batch gains. */
 int nemoh_server_set_batch(nemoh_server *srv, int frequencies);
 const char *nemoh_server_last_error(nemoh_server *srv);
 void nemoh_server_destroy(nemoh_server *srv);

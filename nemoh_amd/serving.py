@@ -74,10 +74,13 @@ def _f64(a):
 
 
 class Server:
-    def __init__(self, workers: int = 4, budget_bytes: float | None = None, tables=None, batch: int = 8):
+    def __init__(self, workers: int = 4, budget_bytes: float | None = None, tables=None, batch: int = 1):
         """batch: most frequencies of one wave request a worker solves at
-        once, as one batch of handles (nemoh_server_set_batch; 1 takes one
-        frequency at a time)."""
+        once, as one batch of handles (nemoh_server_set_batch; 1, the
+        default, takes one frequency at a time: on the batched-bodies and
+        serving workloads, whose requests each bring a new mesh, re-binding
+        and re-tabling several handles per request cost more than the
+        batches gained)."""
         L = core.load()
         if tables is None:
             tables = core.build_tables()

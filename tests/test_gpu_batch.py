@@ -11,8 +11,8 @@ pairs) in infinite depth; boxes in finite depth (the finite-depth assembly
 and the exponential images); half meshes with y-symmetry (two symmetry
 blocks per body); both precisions; a frequency per body (one mesh at many
 frequencies, in both depth regimes at once); a body large enough to take
-the single-body assembly kernel beside grouped ones; update timing; and the
-batch's fallback for a chained
+the single-body assembly kernel beside grouped ones; update timing; tables
+rebuilt when the degrees of freedom change; and the batch's fallback for a chained
 solve that gives up waiting (spin limit 0), which redoes a body on its own
 handle.
 """
@@ -162,6 +162,35 @@ def test_batch_large_body_own_assembly(prec):
     assert big.npanels == 1664
     _check([big, small], 30.0, [0.8], np.deg2rad([0.0, 30.0]), prec)
     _check_multi([big, small, big], 30.0, [[0.8, 3.0, 3.0], [3.0, 0.8, 1.1]], np.zeros(1), prec)
+
+
+def test_batch_rebuilds_after_dofs_change():
+    """Degrees of freedom changed between two calls of one batch, with the
+    same number of problems (6 dofs + 1 heading, then 3 dofs + 4 headings):
+    the batch rebuilds its tables and both calls equal the lone handles."""
+    ms = [nmesh.box(30.0, 10.0, 5.0, 15, 5, 3), nmesh.box(20.0, 8.0, 4.0, 20, 8, 4)]
+    tables = core.build_tables()
+    plans = [(bcm.six_dof(), np.zeros(1)), (bcm.six_dof()[:3], np.deg2rad([0.0, 30.0, 60.0, 90.0]))]
+    alone = []
+    for m in ms:
+        c = _handle(m, 30.0, tables, core.MIXED)
+        row = []
+        for dofs, betas in plans:
+            c.set_dofs(dofs)
+            row.append(c.solve_waves(0.9, betas))
+        alone.append(row)
+        c.close()
+    cores = [_handle(m, 30.0, tables, core.MIXED) for m in ms]
+    b = core.Batch(cores, core.MIXED, max_rhs=7)
+    for p, (dofs, betas) in enumerate(plans):
+        for c in cores:
+            c.set_dofs(dofs)
+        got = b.solve_waves(0.9, betas)
+        for k, (f, fk) in enumerate(got):
+            assert np.array_equal(f, alone[k][p][0]) and np.array_equal(fk, alone[k][p][1]), (p, k)
+    b.close()
+    for c in cores:
+        c.close()
 
 
 def test_batch_chained_solve_fallback():
