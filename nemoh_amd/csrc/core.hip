@@ -2053,10 +2053,9 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
   const int* G = b->body_grid;
   b->omega_host = wv;
   (void)hipMemcpyAsync(b->d_omega, b->omega_host.data(), nbody * sizeof(double), hipMemcpyHostToDevice, b->st);
-  // assembly of every body, one launch per depth regime present (free-surface
-  // panels after it, per body)
   launch_group_info_reset(b->d_body[0], nbj, b->st);
-  // the assembly: a body whose pairs fill the GPU on their own takes a launch
+  // the assembly (free-surface panels after it, per body): a body whose
+  // pairs fill the GPU on their own takes a launch
   // of its own (the single-body kernel keeps its job in kernel arguments; the
   // grouped one reads it from memory and spills 4 registers: 4 % of a
   // 40-body batch of 4000 panels); the others one grouped launch per depth

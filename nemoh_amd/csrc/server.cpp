@@ -23,13 +23,15 @@
 // of requests that arrive while others run, overlap on the GPU as soon as a
 // worker and budget are free.
 //
-// A worker may take several frequencies of a wave request at once (up to the
-// server's batch size, as many as the budget allows at the request's
-// footprint each): it then binds as many handles of its own to the mesh and
-// solves them as one batch (nemoh_batch_solve_waves_multi), every heavy
-// step one launch over all of them, which fills the GPU where one
-// frequency of a small mesh cannot.  A batch gives each frequency the
-// arithmetic of a lone handle, so results do not change either.
+// With a batch size above 1 (nemoh_server_set_batch; 1 by default) a worker
+// may take several frequencies of a wave request at once, as many as the
+// budget allows at the request's footprint each: it then binds as many
+// handles of its own to the mesh and solves them as one batch
+// (nemoh_batch_solve_waves_multi), every heavy step one launch over all of
+// them.  That pays for many frequencies of one small mesh; when each request
+// brings a new mesh, re-binding and re-tabling the handles costs more.  A
+// batch gives each frequency the arithmetic of a lone handle, so results do
+// not change either.
 //
 // A frequency's result does not depend on the schedule: every kernel is
 // deterministic, handles share nothing but the read-only wave tables, and
