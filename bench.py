@@ -265,6 +265,50 @@ def run_batched_frequencies(m, omegas, prec, passes, warmup, ref):
     return dt, same, upd
 
 
+def run_batched_streams(m, omegas, prec, passes, warmup, streams=4):
+    """The 40 frequencies as `streams` batches of 40 / streams handles, each
+    batch driven by its own host thread on its own stream, so that one
+    batch's assembly (FP64 VALU) runs beside another's factorisation (MFMA).
+    Returns the seconds of the timed passes and the forces of the last pass
+    by frequency index."""
+    import threading
+    from nemoh_amd import bodycond as bcm
+    from nemoh_amd import core as ncore
+    tables = ncore.build_tables()
+    parts = np.array_split(np.arange(omegas.size), streams)
+    batches, cores, got = [], [], [None] * streams
+    for p in parts:
+        cs = []
+        for _ in p:
+            h = ncore.Core(m.vertices, m.panels, 0, 50.0, 1025.0, 9.81, tables=tables)
+            h.set_dofs(bcm.six_dof())
+            cs.append(h)
+        cores += cs
+        batches.append(ncore.Batch(cs, prec, max_rhs=9))
+
+    def run(i, n):
+        for _ in range(n):
+            got[i] = batches[i].solve_waves(omegas[parts[i]], BETAS)
+
+    def go(n):
+        th = [threading.Thread(target=run, args=(i, n)) for i in range(streams)]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return time.perf_counter() - t0
+
+    go(max(1, warmup))
+    dt = go(passes)
+    forces = {int(k): got[i][j][0] for i, p in enumerate(parts) for j, k in enumerate(p)}
+    for b in batches:
+        b.close()
+    for h in cores:
+        h.close()
+    return dt, forces
+
+
 REF_HARNESS = ROOT / "oracle" / "_ref" / "harness"
 
 
@@ -553,15 +597,21 @@ def main():
     b64, bsame_64, _ = run_batched_frequencies(m, omegas, ncore.FP64, fp64_steps, a.warmup, f64)
     rate_mixed = a.steps * nw / bmx
     rate_fp64_batch = fp64_steps * nw / b64
+    # the same passes as 4 batches of 10 on 4 streams at once
+    smx, fsmx = run_batched_streams(m, omegas, ncore.MIXED, a.steps, a.warmup)
+    s64, fs64 = run_batched_streams(m, omegas, ncore.FP64, fp64_steps, a.warmup)
+    streams_same = all(np.array_equal(fsmx[k], fmx[k]) for k in fmx if k in fsmx) and \
+        all(np.array_equal(fs64[k], f64[k]) for k in f64 if k in fs64)
     # the same passes through the request loop, a.workers frequencies in flight
     cmx, gmx = run_concurrent(m, omegas, ncore.MIXED, a.steps, a.warmup, a.workers)
     c64, g64 = run_concurrent(m, omegas, ncore.FP64, fp64_steps, a.warmup, a.workers)
     rate_mixed_loop = a.steps * nw / cmx
     rate_fp64_loop = fp64_steps * nw / c64
-    # the comparison is the faster FP64 rate of the two
-    fp64_batch_faster = rate_fp64_batch >= rate_fp64_loop
-    rate_fp64 = max(rate_fp64_batch, rate_fp64_loop)
-    fp64_time = b64 if fp64_batch_faster else c64
+    # the comparison is the fastest FP64 rate measured
+    fp64_rates = {"batch": (rate_fp64_batch, b64), "4 batches on 4 streams": (fp64_steps * nw / s64, s64),
+                  "request loop": (rate_fp64_loop, c64)}
+    fp64_method = max(fp64_rates, key=lambda k: fp64_rates[k][0])
+    rate_fp64, fp64_time = fp64_rates[fp64_method]
     same = all(np.array_equal(gmx[k], fmx[k]) for k in fmx if k in gmx) and \
         all(np.array_equal(g64[k], f64[k]) for k in f64 if k in g64)
     common = sorted(set(g64) & set(gmx))
@@ -603,11 +653,14 @@ def main():
                 f"over {nw} handles of the mesh, one frequency each, every heavy step one launch over all of them",
         "identical_to_single_stream": bool(bsame_mx and bsame_64),
         "fp64_value": round(rate_fp64, 3), "fp64_ms_per_step": round(1e3 * fp64_time / fp64_steps, 3),
-        "fp64_method": "batch" if fp64_batch_faster else "request loop",
+        "fp64_method": fp64_method,
         "speedup_mixed_vs_fp64": round(rate_mixed / rate_fp64, 3),
         "max_rel_err_mixed_vs_fp64": err,
         "max_rel_err_mixed_fp32_assembly_vs_fp64": err32,
         "batch": {"mixed": round(rate_mixed, 3), "fp64": round(rate_fp64_batch, 3)},
+        "batch_4_streams": {"mixed": round(a.steps * nw / smx, 3), "fp64": round(fp64_steps * nw / s64, 3),
+                            "identical_to_single_stream": bool(streams_same),
+                            "how": "4 batches of 10 frequencies, each on its own host thread and stream"},
         "request_loop": {"mixed": round(rate_mixed_loop, 3), "fp64": round(rate_fp64_loop, 3),
                          "frequencies_in_flight": a.workers, "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
                          "identical_to_single_stream": bool(same)},
