@@ -403,11 +403,11 @@ def cpu_baseline(m, omegas, budget_s=20.0):
                       f"as N^2(1+9) pairs + (N/{ns})^3 GAUSSZ = {t_freq:.1f} s"}
 
 
-def run_serving(rates, nreq, workers, seed=2024):
+def run_serving(rates, nreq, workers, seed=2024, budget_gib=32.0):
     """Serving: Poisson arrivals (fixed seed) of two-cylinder requests
     (r ~ U(2,15) m, spacing U(3,10) r, N log-uniform in [200, 16000], 12 DOF
     + 1-4 headings, 1-64 frequencies, infinite depth), admitted by the
-    native request loop under its default budget of 32 GiB of device memory
+    native request loop under a budget of budget_gib of device memory
     (each frequency is admitted with its handle's resident bytes charged:
     FP64 D and S plus the complex64 copy and factors on the mixed path).
     Latency = submission to completion, stamped by the loop.  The first rate
@@ -423,7 +423,7 @@ def run_serving(rates, nreq, workers, seed=2024):
 
     def stream(rate, prec):
         arrivals = np.cumsum(gaps) / rate
-        with serving.Server(workers=workers, tables=tables) as srv:
+        with serving.Server(workers=workers, tables=tables, budget_bytes=budget_gib * 2 ** 30) as srv:
             t0 = time.perf_counter()
             ids = []
             for r, ta in zip(reqs, arrivals):
@@ -445,8 +445,8 @@ def run_serving(rates, nreq, workers, seed=2024):
                        "U(3,10) r, N log-uniform 200-16000 panels, 12 DOF + 1-4 headings, 1-64 frequencies, "
                        "infinite depth",
            "panels": [r.npanels for r in reqs], "frequencies": nfreq, "workers": workers,
-           "budget": "32 GiB of device memory: each in-flight frequency charged its handle's resident bytes "
-                     "(about 40 N^2 B mixed, 32 N^2 B FP64, per symmetry block)", "rates": {}}
+           "budget": f"{budget_gib:g} GiB of device memory: each in-flight frequency charged its handle's "
+                     "resident bytes (about 40 N^2 B mixed, 32 N^2 B FP64, per symmetry block)", "rates": {}}
     for k, rate in enumerate(rates):
         res = {"mixed": stream(rate, ncore.MIXED)}
         if k == 0:
@@ -546,6 +546,7 @@ def main():
     # 8 workers: saturation 12.6 req/s and p50 61 ms at 5 req/s, against 11.4
     # req/s and 79 ms with 4 (the 32 GiB budget then binds)
     ap.add_argument("--serve-workers", type=int, default=8)
+    ap.add_argument("--serve-budget-gib", type=float, default=32.0, help="device memory budget of the request loop")
     ap.add_argument("--c4-bodies", type=int, default=256, help="bodies of the batched run (0: skip)")
     ap.add_argument("--c4-workers", type=int, default=8)
     a = ap.parse_args()
@@ -708,7 +709,7 @@ def main():
         out["c4"] = run_c4(a.c4_bodies, a.c4_workers)
     if a.serve_requests > 0:
         out["serving"] = run_serving([float(x) for x in a.serve_rates.split(",")], a.serve_requests,
-                                     a.serve_workers)
+                                     a.serve_workers, budget_gib=a.serve_budget_gib)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = (cpu_baseline_reference(m, omegas) if REF_HARNESS.exists()
                                else cpu_baseline(m, omegas))
