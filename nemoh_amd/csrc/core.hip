@@ -1770,6 +1770,12 @@ size_t update_work(const LuGroup<T>& g, double& flops, double& bytes) {
 // Workgroups of a body's paired assembly from which it takes a launch of its
 // own in a batch (about five rounds of the GPU at three workgroups per CU).
 constexpr int kOwnAssemblyWg = 4096;
+// Workgroups of a body's paired assembly: 8 x 8 panel tiles of the lower
+// triangle with the diagonal, four per workgroup.
+int pair_assembly_wg(const nemoh_core* c) {
+  const int nt = (c->nfa + 7) / 8, nw = nt * (nt + 1) / 2;
+  return (nw + 3) / 4;
+}
 
 // Waits for the batch's stream whatever the launch status (copies into host
 // memory may be pending), then reports whether both were clean.
@@ -1780,6 +1786,8 @@ bool batch_sync(nemoh_batch* b) {
 }
 
 int batch_fail(nemoh_batch* b, int code, const std::string& msg) {
+  // work already queued may read the caller's headings: finished first
+  (void)hipStreamSynchronize(b->st);
   b->err = msg;
   return code;
 }
@@ -1875,7 +1883,7 @@ int batch_build(nemoh_batch* b, const std::vector<int>& nrhs) {
     foff += (size_t)c->nforce * (nrhs[i] + nbeta);
     b->force_off.push_back(foff);
     const int nt = (c->nfa + 7) / 8, nw = nt * (nt + 1) / 2;
-    AsmJob a{c->G, c->T, c->d_par, mat_out(c), nt, nw, wg, wg + (nw + 3) / 4};
+    AsmJob a{c->G, c->T, c->d_par, mat_out(c), nt, nw, wg, wg + pair_assembly_wg(c)};
     wg = a.wg1;
     asmj.push_back(a);
     ChainCtl* ctl = reinterpret_cast<ChainCtl*>(c->d_chain);
@@ -2013,8 +2021,15 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
   StreamSwap swap(b);
   const double pi = 4.0 * std::atan(1.0);
   std::vector<int> nrhs(nbody), fdv(nbody);
+  std::vector<char> own(nbody);
   std::vector<double> wv(nbody);
-  // frequency set-up and body conditions of every body
+  const bool mixed = b->prec == NEMOH_MIXED;
+  // frequency set-up and body conditions of every body; a body whose pairs
+  // fill the GPU on their own takes an assembly launch of its own, queued at
+  // once so that it runs while the host sets up the next bodies (the
+  // single-body kernel keeps its job in kernel arguments; the grouped one
+  // reads it from memory and spills 4 registers: 4 % of a 40-body batch of
+  // 4000 panels)
   for (size_t i = 0; i < nbody; ++i) {
     nemoh_core* c = b->bodies[i];
     if (c->prec != b->prec) return batch_fail(b, NEMOH_ERR_ARG, "a body's precision changed outside the batch");
@@ -2029,6 +2044,7 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
     nrhs[i] = nbeta + c->nrad;
     if (nrhs[i] > b->max_rhs)
       return batch_fail(b, NEMOH_ERR_ARG, "body " + std::to_string(i) + ": more problems than the batch was made for");
+    if (nrhs[i] != nrhs[0]) return batch_fail(b, NEMOH_ERR_ARG, "bodies of a batch solve the same number of problems");
     const double period = 2.0 * pi / omega[i];  // T as the Solver's main program forms it
     nemoh_freq f;
     bool bfd;
@@ -2038,31 +2054,27 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
     if (int r = generate_waves(c, omega[i], nbeta, beta, false))
       return batch_fail(b, r, "body " + std::to_string(i) + ": " + c->err);
     fdv[i] = bfd ? 1 : 0;
+    own[i] = pair_assembly_wg(c) >= kOwnAssemblyWg;
+    c->w_prev = -1.0;  // the factors on the device are overwritten from here on
+    if (own[i]) launch_assemble(c->G, c->T, c->d_par, fdv[i], c->nsym, mat_out(c), nullptr, 0, true, b->st);
     c->stats.cache_hit = 0;
     c->stats.refine_iterations = 0;
     c->stats.refine_residual = 0;
   }
-  // the headings' copies above read host memory: done before this call returns
-  if (hipStreamSynchronize(b->st) != hipSuccess) return batch_fail(b, NEMOH_ERR_HIP, "batch synchronisation failed");
-  for (size_t i = 1; i < nbody; ++i)
-    if (nrhs[i] != nrhs[0]) return batch_fail(b, NEMOH_ERR_ARG, "bodies of a batch solve the same number of problems");
+  // (the headings' copies above read host memory: every return waits for the
+  // stream)
   if (batch_stale(b, nrhs))
     if (int r = batch_build(b, nrhs)) return r;
-  const bool mixed = b->prec == NEMOH_MIXED;
   const int nbj = (int)nbody;
   const int* G = b->body_grid;
   b->omega_host = wv;
   (void)hipMemcpyAsync(b->d_omega, b->omega_host.data(), nbody * sizeof(double), hipMemcpyHostToDevice, b->st);
   launch_group_info_reset(b->d_body[0], nbj, b->st);
-  // the assembly (free-surface panels after it, per body): a body whose
-  // pairs fill the GPU on their own takes a launch
-  // of its own (the single-body kernel keeps its job in kernel arguments; the
-  // grouped one reads it from memory and spills 4 registers: 4 % of a
-  // 40-body batch of 4000 panels); the others one grouped launch per depth
-  // regime present.  Both run the same pair arithmetic.
+  // the assembly of the other bodies, one grouped launch per depth regime
+  // present (the same pair arithmetic as the single-body kernel), then the
+  // free-surface panels, per body
   const int nfd = (int)std::count(fdv.begin(), fdv.end(), 1);
-  bool any_big = false;
-  for (size_t i = 0; i < nbody; ++i) any_big = any_big || b->asm_host[i].wg1 - b->asm_host[i].wg0 >= kOwnAssemblyWg;
+  const bool any_big = std::count(own.begin(), own.end(), 1) > 0;
   if (!any_big && (nfd == 0 || nfd == nbj)) {
     launch_assemble_group(b->d_asm, b->asm_count, b->asm_grid, nfd ? 1 : 0, b->nsym, mixed, b->st);
   } else {
@@ -2074,7 +2086,7 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
       for (size_t i = 0; i < nbody; ++i) {
         AsmJob a = b->asm_host[i];
         const int nwg = a.wg1 - a.wg0;
-        if (fdv[i] != reg || nwg >= kOwnAssemblyWg) continue;
+        if (fdv[i] != reg || own[i]) continue;
         a.wg0 = wg[1 - reg];
         a.wg1 = a.wg0 + nwg;
         wg[1 - reg] = a.wg1;
@@ -2086,11 +2098,6 @@ static int batch_solve(nemoh_batch* b, const double* omega, int nbeta, const dou
                            hipMemcpyHostToDevice, b->st);
     launch_assemble_group(b->d_asm_part, cnt[0], wg[0], 1, b->nsym, mixed, b->st);
     launch_assemble_group(b->d_asm_part + cnt[0], cnt[1], wg[1], 0, b->nsym, mixed, b->st);
-    for (size_t i = 0; i < nbody; ++i) {
-      nemoh_core* c = b->bodies[i];
-      if (b->asm_host[i].wg1 - b->asm_host[i].wg0 >= kOwnAssemblyWg)
-        launch_assemble(c->G, c->T, c->d_par, fdv[i], c->nsym, mat_out(c), nullptr, 0, true, b->st);
-    }
   }
   for (size_t i = 0; i < nbody; ++i) {
     nemoh_core* c = b->bodies[i];
