@@ -12,9 +12,10 @@ and the exponential images); half meshes with y-symmetry (two symmetry
 blocks per body); both precisions; a frequency per body (one mesh at many
 frequencies, in both depth regimes at once); a body large enough to take
 the single-body assembly kernel beside grouped ones; update timing; tables
-rebuilt when the degrees of freedom change; and the batch's fallback for a chained
-solve that gives up waiting (spin limit 0), which redoes a body on its own
-handle.
+rebuilt when the degrees of freedom change; a call that fails part-way, after
+which no handle claims factors its matrices no longer hold; and the batch's
+fallback for a chained solve that gives up waiting (spin limit 0), which
+redoes a body on its own handle.
 """
 import numpy as np
 import pytest
@@ -191,6 +192,29 @@ def test_batch_rebuilds_after_dofs_change():
     b.close()
     for c in cores:
         c.close()
+
+
+def test_batch_failed_call_invalidates_cached_factors():
+    """A call that fails part-way (here: the second body solves a different
+    number of problems, found after the first body's own assembly launch was
+    queued) leaves no handle claiming factors that its matrices no longer
+    hold: the first body, solved alone again at its earlier frequency, gives
+    its earlier forces bit for bit."""
+    tables = core.build_tables()
+    big = _handle(nmesh.box(40.0, 20.0, 10.0, 32, 16, 12), 30.0, tables, core.MIXED)
+    small = _handle(nmesh.box(12.0, 12.0, 6.0, 6, 6, 3), 30.0, tables, core.MIXED)
+    small.set_dofs(bcm.six_dof()[:3])
+    betas = np.zeros(1)
+    before = big.solve_waves(0.8, betas)
+    b = core.Batch([big, small], core.MIXED, max_rhs=7)
+    with pytest.raises(core.NemohError) as e:
+        b.solve_waves(np.array([1.1, 1.1]), betas)
+    assert e.value.code == core.ERR_ARG
+    again = big.solve_waves(0.8, betas)
+    assert np.array_equal(again[0], before[0]) and np.array_equal(again[1], before[1])
+    b.close()
+    big.close()
+    small.close()
 
 
 def test_batch_chained_solve_fallback():
