@@ -1173,15 +1173,13 @@ __device__ __forceinline__ void hop_mark_unsolved(C2<T>* __restrict__ X, int nb,
 // solves row block w, the backward chain row block nblk - 1 - w) of one
 // triangle: it waits only for the positions before w.
 //
-// kPrefetchX: x of dependency d + 1 is loaded with the coupling block, while
-// the product of dependency d runs.  That pays where workgroups start behind
-// the chain's front (a launch of many chains, whose tickets outnumber the
-// resident workgroups).  In a lone chain every workgroup keeps pace with the
-// front, so the early load mostly returns the unsolved marker, and issuing
-// it (16 uncached loads per lane in each wave) delays the product: clock
-// stamps of a 4000-panel chain gave 3.7 us per dependency with it and 3.0
-// without, x then being loaded where it is polled.
-template <typename T, bool kLower, bool kPrefetchX>
+// x of a dependency is loaded where it is polled, not with the coupling
+// block one dependency early: workgroups keep pace with the chain's front,
+// so an early load mostly returns the unsolved marker, and issuing it (16
+// uncached loads per lane in each wave) delays the product.  Clock stamps of
+// a 4000-panel chain: 3.7 us per dependency with the early load, 3.0
+// without; a batch of 40 chains is slightly faster without it too.
+template <typename T, bool kLower>
 __device__ __forceinline__ void trsv_chain_run(int w, const C2<T>* __restrict__ A, int ld, int n,
                                                const C2<T>* __restrict__ P, C2<T>* __restrict__ R,
                                                C2<T>* __restrict__ X, int nrhs, int* __restrict__ status,
@@ -1226,19 +1224,18 @@ __device__ __forceinline__ void trsv_chain_run(int w, const C2<T>* __restrict__ 
     j0 = j * kNB;
     nj = min(kNB, n - j0);
   };
-  // the coupling block (and, kPrefetchX, x) of dependency d + 1 is loaded
-  // while the product of dependency d runs
+  // the coupling block of dependency d + 1 is loaded while the product of
+  // dependency d runs
   C2<T> t[16], xv[16];
   if (ndep > 0) {
     int j0, nj;
     dep_rows(0, j0, nj);
     hop_issue_tile(t, A + (size_t)k0 * ld + j0, ld, nb, nj);
-    if (kPrefetchX) hop_issue_x(xv, X + (size_t)j0 * nrhs, nj, nrhs);
   }
   for (int d = 0; d < ndep; ++d) {
     int j0, nj;
     dep_rows(d, j0, nj);
-    if (!kPrefetchX) hop_issue_x(xv, X + (size_t)j0 * nrhs, nj, nrhs);
+    hop_issue_x(xv, X + (size_t)j0 * nrhs, nj, nrhs);
     bool ok = hop_x_operand(xr, xi, nxi, xv, nj, nrhs);
     int spins = 0;
     while (!__all(ok) && !gave_up) {
@@ -1258,7 +1255,6 @@ __device__ __forceinline__ void trsv_chain_run(int w, const C2<T>* __restrict__ 
       int j1, n1;
       dep_rows(d + 1, j1, n1);
       hop_issue_tile(t, A + (size_t)k0 * ld + j1, ld, nb, n1);
-      if (kPrefetchX) hop_issue_x(xv, X + (size_t)j1 * nrhs, n1, nrhs);
     }
     hop_mma_raw<T>(cr, ci, m, xr, xi, nxi);
   }
@@ -1320,7 +1316,7 @@ __device__ __forceinline__ void trsv_chain_body(int& my_ticket, const C2<T>* __r
     if (tid == 0) atomicMax(status, 1);
     return;
   }
-  trsv_chain_run<T, kLower, false>(w, A, ld, n, P, R, X, nrhs, status, spin_limit);
+  trsv_chain_run<T, kLower>(w, A, ld, n, P, R, X, nrhs, status, spin_limit);
 }
 
 template <typename T, bool kLower>
@@ -1368,9 +1364,9 @@ __global__ __launch_bounds__(256) void group_trsv_kernel(const TrsvJob<T>* __res
   const TrsvJob<T> J = jobs[t - cum[lo]];
   if (J.skip && J.skip->done) return;  // a converged refinement: nothing to do
   if (kLower)
-    trsv_chain_run<T, true, true>(lo, J.A, J.ld, J.n, J.P, J.B, J.W, J.nrhs, J.status, spin_limit);
+    trsv_chain_run<T, true>(lo, J.A, J.ld, J.n, J.P, J.B, J.W, J.nrhs, J.status, spin_limit);
   else
-    trsv_chain_run<T, false, true>(lo, J.A, J.ld, J.n, J.P, J.W, J.B, J.nrhs, J.status, spin_limit);
+    trsv_chain_run<T, false>(lo, J.A, J.ld, J.n, J.P, J.W, J.B, J.nrhs, J.status, spin_limit);
 }
 
 // Stepwise fallback (one launch per block step s, for matrices with more row
