@@ -975,6 +975,18 @@ struct HopTile {
   T pr[16], pi[16], npi[16];
 };
 
+// Column of the block (row of x) that MFMA k-step s takes from lane group
+// lk; the product sums over the 64 columns in this order, and M and x use
+// the same map.  complex64: steps 2p and 2p + 1 take two adjacent columns,
+// so a lane loads each pair as one 16-byte word, and the four lane groups of
+// a row read 64 contiguous bytes per load (8 loads per lane for a block).
+// complex128: one element is already 16 bytes; four consecutive columns per
+// step, 64 contiguous bytes per row per load (16 loads).
+template <typename T>
+__device__ __forceinline__ int hop_k(int s, int lk) {
+  return sizeof(T) == 4 ? 8 * (s >> 1) + 2 * lk + (s & 1) : 4 * s + lk;
+}
+
 // The loads below take a clamped address (element 0, always inside the
 // block, for an element outside nr x nc) and select zero afterwards, so that
 // they carry no branch: a load under a divergent branch is waited for inside
@@ -987,12 +999,12 @@ __device__ __forceinline__ void hop_load_tile(HopTile<T>& h, const C2<T>* __rest
   C2<T> v[16];
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const int c = 4 * s + lk;
+    const int c = hop_k<T>(s, lk);
     v[s] = M[(r < nr && c < nc) ? (size_t)r * ld + c : 0];
   }
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const bool in = r < nr && 4 * s + lk < nc;
+    const bool in = r < nr && hop_k<T>(s, lk) < nc;
     h.pr[s] = in ? sign * v[s].re : T(0);
     h.pi[s] = in ? sign * v[s].im : T(0);
     h.npi[s] = in ? -sign * v[s].im : T(0);
@@ -1030,19 +1042,19 @@ __device__ __forceinline__ bool is_solved(C2<double> v) {
   return a != kUnsolved64 && b != kUnsolved64;
 }
 
-// x operand of one block step: lane (li, lk) holds x[4 s + lk][li], s < 16.
+// x operand of one block step: lane (li, lk) holds x[hop_k(s, lk)][li], s < 16.
 template <typename T>
 __device__ __forceinline__ void hop_load_x(T xr[16], T xi[16], const C2<T>* __restrict__ X, int nj, int nrhs) {
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   C2<T> v[16];
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const int r = 4 * s + lk;
+    const int r = hop_k<T>(s, lk);
     v[s] = X[(r < nj && li < nrhs) ? (size_t)r * nrhs + li : 0];
   }
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const bool in = 4 * s + lk < nj && li < nrhs;
+    const bool in = hop_k<T>(s, lk) < nj && li < nrhs;
     xr[s] = in ? v[s].re : T(0);
     xi[s] = in ? v[s].im : T(0);
   }
@@ -1051,26 +1063,34 @@ __device__ __forceinline__ void hop_load_x(T xr[16], T xi[16], const C2<T>* __re
 // The chained solve's operands are loaded ahead of their use in two steps:
 // hop_issue_* sends the loads (clamped addresses, no branch), and the
 // operand is formed after the loads land.  Tile: lane (li, lk) of wave w
-// holds M[16 w + li][4 s + lk]; elements outside nr x nc hold M[0][0], which
-// only ever meets a zero x element (columns) or fills accumulator rows that
-// are not stored (rows).
+// holds M[16 w + li][hop_k(s, lk)]; elements outside nr x nc hold M[0][0],
+// which only ever meets a zero x element (columns) or fills accumulator rows
+// that are not stored (rows).  A full complex64 block (the same for the
+// whole workgroup) takes unclamped addresses, which lets the loads of
+// adjacent columns merge (complex128 measured slower that way).
 template <typename T>
 __device__ __forceinline__ void hop_issue_tile(C2<T> t[16], const C2<T>* __restrict__ M, int ld, int nr, int nc) {
   const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;
   const int r = 16 * wave + (lane & 15), lk = lane >> 4;
+  if (sizeof(T) == 4 && nr == kNB && nc == kNB) {
+    const C2<T>* row = M + (size_t)r * ld;
 #pragma unroll
-  for (int s = 0; s < 16; ++s) {
-    const int c = 4 * s + lk;
-    t[s] = M[(r < nr && c < nc) ? (size_t)r * ld + c : 0];
+    for (int s = 0; s < 16; ++s) t[s] = row[hop_k<T>(s, lk)];
+  } else {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int c = hop_k<T>(s, lk);
+      t[s] = M[(r < nr && c < nc) ? (size_t)r * ld + c : 0];
+    }
   }
 }
-// x_j as other workgroups store it (lane (li, lk): x[4 s + lk][li])
+// x_j as other workgroups store it (lane (li, lk): x[hop_k(s, lk)][li])
 template <typename T>
 __device__ __forceinline__ void hop_issue_x(C2<T> v[16], const C2<T>* __restrict__ X, int nj, int nrhs) {
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const int r = 4 * s + lk;
+    const int r = hop_k<T>(s, lk);
     v[s] = ld_coherent(X + ((r < nj && li < nrhs) ? (size_t)r * nrhs + li : 0));
   }
 }
@@ -1082,7 +1102,7 @@ __device__ __forceinline__ bool hop_x_operand(T xr[16], T xi[16], T nxi[16], con
   bool ok = true;
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const bool in = 4 * s + lk < nj && li < nrhs;
+    const bool in = hop_k<T>(s, lk) < nj && li < nrhs;
     ok = ok && (!in || is_solved(v[s]));
     xr[s] = in ? v[s].re : T(0);
     xi[s] = in ? v[s].im : T(0);
