@@ -111,8 +111,28 @@ __device__ __forceinline__ void acc_coord(int a, int b, int row_in16, int& r, in
   c = (wave & 1) * 32 + b * 16 + (lane & 15);
 }
 
+// The block LU's operands all live in device memory.  Read and written
+// through a global (address space 1) pointer they are vector-memory
+// accesses, whose completion the compiler's wait counts track one by one;
+// through a generic pointer taken from a job table they would be flat
+// accesses, whose waits also cover LDS traffic and every access in flight,
+// so that a stash of one operand chunk would wait for the loads of the
+// chunks after it.  The cast is taken of a block's base pointer, before any
+// per-lane offset.
 template <typename T>
-__device__ __forceinline__ void acc_load(Acc64<T>& acc, const C2<T>* __restrict__ C, int ld, int nr, int nc) {
+using GlobalC2 = __attribute__((address_space(1))) C2<T>;
+template <typename T>
+__device__ __forceinline__ const GlobalC2<T>* global_view(const C2<T>* p) {
+  return (const GlobalC2<T>*)p;
+}
+template <typename T>
+__device__ __forceinline__ GlobalC2<T>* global_view(C2<T>* p) {
+  return (GlobalC2<T>*)p;
+}
+
+template <typename T>
+__device__ __forceinline__ void acc_load(Acc64<T>& acc, const C2<T>* __restrict__ Cp, int ld, int nr, int nc) {
+  const GlobalC2<T>* C = global_view(Cp);
   const int lk = (threadIdx.x & 63) >> 4;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -123,14 +143,18 @@ __device__ __forceinline__ void acc_load(Acc64<T>& acc, const C2<T>* __restrict_
         int r, c;
         acc_coord(a, b, MfmaTraits<T>::row(lk, q), r, c);
         C2<T> v = {0, 0};
-        if (r < nr && c < nc) v = C[(size_t)r * ld + c];
+        if (r < nr && c < nc) {
+          const GlobalC2<T>& g = C[(size_t)r * ld + c];
+          v = C2<T>{g.re, g.im};
+        }
         acc.re[a][b][q] = v.re;
         acc.im[a][b][q] = v.im;
       }
 }
 
 template <typename T>
-__device__ __forceinline__ void acc_store(const Acc64<T>& acc, C2<T>* __restrict__ C, int ld, int nr, int nc) {
+__device__ __forceinline__ void acc_store(const Acc64<T>& acc, C2<T>* __restrict__ Cp, int ld, int nr, int nc) {
+  GlobalC2<T>* C = global_view(Cp);
   const int lk = (threadIdx.x & 63) >> 4;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -140,7 +164,11 @@ __device__ __forceinline__ void acc_store(const Acc64<T>& acc, C2<T>* __restrict
       for (int q = 0; q < 4; ++q) {
         int r, c;
         acc_coord(a, b, MfmaTraits<T>::row(lk, q), r, c);
-        if (r < nr && c < nc) C[(size_t)r * ld + c] = C2<T>{acc.re[a][b][q], acc.im[a][b][q]};
+        if (r < nr && c < nc) {
+          GlobalC2<T>& g = C[(size_t)r * ld + c];
+          g.re = acc.re[a][b][q];
+          g.im = acc.im[a][b][q];
+        }
       }
 }
 
@@ -162,7 +190,29 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
   // while chunk c feeds the MFMAs and are stashed in LDS at the end of chunk
   // c + 1), so each load has two chunks' MFMA time to land.
   C2<T> pa[2][4], pb[2][4];
+  // complex64 full blocks: each thread moves two adjacent elements per load
+  // (pairs along k for A, along columns for B), which the loads fetch as one
+  // 16-byte word: half the load instructions.  The LDS contents, and so the
+  // products, are the same as with the element map below.
+  constexpr bool kPairs = sizeof(T) == 4 && kFull && !kCoherentB;
+  const GlobalC2<T>* ga = global_view(A.p);
+  const GlobalC2<T>* gb = global_view(B.p);
+  auto gload = [](const GlobalC2<T>& g) { return C2<T>{g.re, g.im}; };
   auto fetch = [&](int chunk, int set) {
+    if constexpr (kPairs) {
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const int e = tid + 256 * v;
+        const int ar_ = e >> 3, ak = chunk * 16 + 2 * (e & 7);
+        const int bk = chunk * 16 + (e >> 5), bc = 2 * (e & 31);
+        const size_t ia = (size_t)ar_ * A.ld + ak, ib = (size_t)bk * B.ld + bc;
+        pa[set][2 * v] = gload(ga[ia]);
+        pa[set][2 * v + 1] = gload(ga[ia + 1]);
+        pb[set][2 * v] = gload(gb[ib]);
+        pb[set][2 * v + 1] = gload(gb[ib + 1]);
+      }
+      return;
+    }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u;
@@ -170,17 +220,33 @@ __device__ __forceinline__ void gemm64_impl(Acc64<T>& acc, View<T> A, View<T> B,
       const int bk = chunk * 16 + (e >> 6), bc = e & 63;
       // kCoherentB: B was written during this launch by another workgroup
       if (kFull) {  // unguarded loads: no select waits on their data
-        pa[set][u] = A.p[(size_t)ar_ * A.ld + ak];
-        pb[set][u] = kCoherentB ? ld_coherent(B.p + (size_t)bk * B.ld + bc) : B.p[(size_t)bk * B.ld + bc];
+        pa[set][u] = gload(ga[(size_t)ar_ * A.ld + ak]);
+        pb[set][u] = kCoherentB ? ld_coherent(B.p + (size_t)bk * B.ld + bc) : gload(gb[(size_t)bk * B.ld + bc]);
       } else {
-        pa[set][u] = (ar_ < A.nr && ak < A.nc) ? A.p[(size_t)ar_ * A.ld + ak] : C2<T>{0, 0};
+        pa[set][u] = (ar_ < A.nr && ak < A.nc) ? gload(ga[(size_t)ar_ * A.ld + ak]) : C2<T>{0, 0};
         pb[set][u] = (bk < B.nr && bc < B.nc)
-                         ? (kCoherentB ? ld_coherent(B.p + (size_t)bk * B.ld + bc) : B.p[(size_t)bk * B.ld + bc])
+                         ? (kCoherentB ? ld_coherent(B.p + (size_t)bk * B.ld + bc) : gload(gb[(size_t)bk * B.ld + bc]))
                          : C2<T>{0, 0};
       }
     }
   };
   auto stash = [&](int buf, int set) {
+    if constexpr (kPairs) {
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const int e = tid + 256 * v;
+        const int ar_ = e >> 3, ak = 2 * (e & 7);
+        const int bk = e >> 5, bc = 2 * (e & 31);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          S.ar[buf][ar_][ak + h] = pa[set][2 * v + h].re;
+          S.ai[buf][ar_][ak + h] = pa[set][2 * v + h].im;
+          S.br[buf][bk][bc + h] = pb[set][2 * v + h].re;
+          S.bi[buf][bk][bc + h] = pb[set][2 * v + h].im;
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u;
