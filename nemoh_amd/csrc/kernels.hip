@@ -376,7 +376,14 @@ __device__ __forceinline__ void gemv_body(GemvRed& red, const double2* __restric
   const int nk = (n + 3) / 4;  // k-steps of 4 columns
   const int per = (nk + kGemvWaves - 1) / kGemvWaves;
   const int s0 = wave * per, s1 = min(nk, s0 + per);
-  const double2* mrow = M + (size_t)min(row, n - 1) * ld;
+  // M and X through global (address space 1) views of their base pointers:
+  // a grouped launch takes them from its job table, and through those
+  // generic pointers the loads would be flat loads, whose waits also count
+  // LDS accesses and every load in flight
+  typedef const __attribute__((address_space(1))) double2 GD2;
+  GD2* gm = (GD2*)M;
+  GD2* gx = (GD2*)X;
+  GD2* mrow = gm + (size_t)min(row, n - 1) * ld;
   v4d cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
   for (int s = s0; s < s1; s += 8) {
     double2 a[8], x[8];
@@ -384,8 +391,9 @@ __device__ __forceinline__ void gemv_body(GemvRed& red, const double2* __restric
     for (int u = 0; u < 8; ++u) {
       const int k = 4 * (s + u) + lk;
       const bool kin = (s + u < s1) && k < n;
-      a[u] = (kin && row < n) ? mrow[k] : make_double2(0.0, 0.0);
-      x[u] = (kin && li < nrhs) ? X[(size_t)k * nrhs + li] : make_double2(0.0, 0.0);
+      a[u] = (kin && row < n) ? make_double2(mrow[k].x, mrow[k].y) : make_double2(0.0, 0.0);
+      x[u] = (kin && li < nrhs) ? make_double2(gx[(size_t)k * nrhs + li].x, gx[(size_t)k * nrhs + li].y)
+                                : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
